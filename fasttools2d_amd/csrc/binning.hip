@@ -1,0 +1,280 @@
+// Binning of points by key with a deterministic, stable counting sort.
+//
+// Cell keys follow Assign (PeriodicEwald/src/mex/common/ewald_tools.cpp:7-80);
+// grid-tile keys use FindClosestNode (ewald_tools.cpp:86-114).  The
+// reference's serial counting sort visits points in input order, so each
+// bucket lists its points by increasing index.  Here the histogram and the
+// scatter run with integer atomics (the bucket *sets* are order independent)
+// and every bucket is then sorted by index, which yields exactly that order.
+#include "se2p_internal.h"
+
+#include <climits>
+
+namespace se2p {
+
+namespace {
+
+constexpr int kSortCap = 4096;  // largest bucket sorted in LDS by one workgroup
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__global__ void k_cell_keys(const double* __restrict__ p, int n, double Lx, double Ly, int nsx,
+                            int nsy, int* __restrict__ key, int* __restrict__ count)
+{
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = p[2 * i], y = p[2 * i + 1];
+    int bx = clampi((int)floor(nsx * (x / Lx + 0.5)), 0, nsx - 1);
+    int by = clampi((int)floor(nsy * (y / Ly + 0.5)), 0, nsy - 1);
+    int k = by * nsx + bx;
+    key[i] = k;
+    atomicAdd(&count[k], 1);
+}
+
+// FindClosestNode with the same operation order and no contraction, so the
+// start node and the snap decisions are bit-identical to the reference.
+__device__ __forceinline__ void closest_node(double x, double y, double Lx, double Ly, double h,
+                                             int P, int* mx, int* my, double* px, double* py)
+{
+#pragma clang fp contract(off)
+    const double snap = 1e-13;
+    double ox = x - h * floor(x / h);
+    double oy = y - h * floor(y / h);
+    double hx = Lx / 2.0, hy = Ly / 2.0;
+    int half = P / 2;
+    int sx = (int)(floor((x + hx) / h) - half);
+    int sy = (int)(floor((y + hy) / h) - half);
+    if (fabs(ox) < snap || fabs(ox - h) < snap) ox = 0;
+    if (fabs(oy) < snap || fabs(oy - h) < snap) oy = 0;
+    if (ox == 0 && remainder((x + hx) / h - half, 1.0) < 0) sx++;
+    if (oy == 0 && remainder((y + hy) / h - half, 1.0) < 0) sy++;
+    *mx = sx;
+    *my = sy;
+    *px = ox;
+    *py = oy;
+}
+
+__global__ void k_tile_keys(const double* __restrict__ p, int n, double Lx, double Ly, double h,
+                            int P, int Mx, int My, int TX, int TY, int ntx,
+                            int* __restrict__ key, int* __restrict__ count,
+                            int2* __restrict__ node, double2* __restrict__ off)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int mx, my;
+    double px, py;
+    closest_node(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+    int wx = ((mx % Mx) + Mx) % Mx;
+    int wy = ((my % My) + My) % My;
+    int k = (wy / TY) * ntx + (wx / TX);
+    key[i] = k;
+    node[i] = make_int2(wx, wy);
+    off[i] = make_double2(px, py);
+    atomicAdd(&count[k], 1);
+}
+
+__global__ void k_closest_nodes(const double* __restrict__ p, int n, double Lx, double Ly,
+                                double h, int P, int* __restrict__ mx, int* __restrict__ my,
+                                double* __restrict__ px, double* __restrict__ py)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    closest_node(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx[i], &my[i], &px[i], &py[i]);
+}
+
+__global__ void k_fill_int(int* p, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Exclusive scan of count[0..n) into start[0..n]; one workgroup of 1024 threads,
+// each owning a contiguous segment.
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ count, int n,
+                                               int* __restrict__ start)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int seg = (n + 1023) / 1024;
+    const int b = t * seg, e = min(n, b + seg);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += count[i];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        int v = (t >= d) ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;  // exclusive prefix of this segment
+    for (int i = b; i < e; ++i) {
+        start[i] = run;
+        run += count[i];
+    }
+    if (t == 1023) start[n] = part[1023];
+}
+
+__global__ void k_copy_int(const int* __restrict__ a, int* __restrict__ b, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_scatter(const int* __restrict__ key, int n, int* __restrict__ cursor,
+                          int* __restrict__ perm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int pos = atomicAdd(&cursor[key[i]], 1);
+    perm[pos] = i;
+}
+
+// One workgroup per bucket: bitonic sort of the bucket's indices in LDS.
+// Buckets larger than kSortCap are queued in biglist (biglist[0] = count).
+__global__ __launch_bounds__(256) void k_sort_buckets(int* __restrict__ perm,
+                                                      const int* __restrict__ start,
+                                                      int* __restrict__ biglist)
+{
+    __shared__ int s[kSortCap];
+    const int b = blockIdx.x;
+    const int lo = start[b], size = start[b + 1] - lo;
+    if (size <= 1) return;
+    if (size > kSortCap) {
+        if (threadIdx.x == 0) {
+            int slot = atomicAdd(&biglist[0], 1);
+            biglist[1 + slot] = b;
+        }
+        return;
+    }
+    int np = 2;
+    while (np < size) np <<= 1;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s[i] = i < size ? perm[lo + i] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                int ij = i ^ j;
+                if (ij > i) {
+                    int a = s[i], c = s[ij];
+                    bool up = (i & k) == 0;
+                    if ((a > c) == up) {
+                        s[i] = c;
+                        s[ij] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < size; i += blockDim.x) perm[lo + i] = s[i];
+}
+
+// Buckets beyond the LDS capacity: each element's final position is its rank
+// among the (distinct) indices of its bucket.  O(size^2) per bucket; only
+// reached by heavily clustered inputs.
+__global__ __launch_bounds__(256) void k_sort_big(int* __restrict__ perm,
+                                                  const int* __restrict__ start,
+                                                  const int* __restrict__ biglist,
+                                                  int* __restrict__ tmp)
+{
+    __shared__ int tile[1024];
+    const int nbig = biglist[0];
+    for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
+        const int b = biglist[1 + q];
+        const int lo = start[b], size = start[b + 1] - lo;
+        for (int i0 = 0; i0 < size; i0 += blockDim.x) {
+            int i = i0 + threadIdx.x;
+            int v = i < size ? perm[lo + i] : INT_MAX;
+            int rank = 0;
+            for (int j0 = 0; j0 < size; j0 += 1024) {
+                int cnt = min(1024, size - j0);
+                __syncthreads();
+                for (int j = threadIdx.x; j < cnt; j += blockDim.x) tile[j] = perm[lo + j0 + j];
+                __syncthreads();
+                for (int j = 0; j < cnt; ++j) rank += tile[j] < v;
+            }
+            if (i < size) tmp[lo + rank] = v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < size; i += blockDim.x) perm[lo + i] = tmp[lo + i];
+        __syncthreads();
+    }
+}
+
+inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
+
+}  // namespace
+
+int launch_fill_int(hipStream_t s, int* p, int n, int v)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, double Ly,
+                         double h, int P, int* mx, int* my, double* px, double* py)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_closest_nodes, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, Lx, Ly, h,
+                       P, mx, my, px, py);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_cell_keys(hipStream_t s, const double* p, int n, double Lx, double Ly, int nsx,
+                     int nsy, int* key, int* count)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, Lx, Ly, nsx,
+                       nsy, key, count);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly, double h,
+                     int P, int Mx, int My, int TX, int TY, int ntx, int* key, int* count,
+                     int2* node, double2* off)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_tile_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, Lx, Ly, h, P,
+                       Mx, My, TX, TY, ntx, key, count, node, off);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int sort_by_key(se2p_ctx* ctx, const int* key, int* count, int n, int nkeys, int* perm,
+                int* start, Slot tmp_slot)
+{
+    hipStream_t s = ctx->stream;
+    // count[] was filled by the key kernel; start = exclusive scan.
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, nkeys, start);
+    SE2P_HIP_CHECK(hipGetLastError());
+    if (n <= 0) return SE2P_OK;
+    // count[] is reused as the scatter cursor.
+    hipLaunchKernelGGL(k_copy_int, dim3(grid_for(nkeys, 256)), dim3(256), 0, s, start, count,
+                       nkeys);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, count, perm);
+    SE2P_HIP_CHECK(hipGetLastError());
+    // biglist: [count, bucket ids...]; tmp: n ints.
+    void* big = nullptr;
+    void* tmp = nullptr;
+    SE2P_TRY(ws_get(ctx, WS_BIGLIST, sizeof(int) * (size_t)(nkeys + 1), &big));
+    SE2P_TRY(ws_get(ctx, tmp_slot, sizeof(int) * (size_t)n, &tmp));
+    SE2P_TRY(launch_fill_int(s, (int*)big, 1, 0));
+    hipLaunchKernelGGL(k_sort_buckets, dim3(nkeys), dim3(256), 0, s, perm, start, (int*)big);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_sort_big, dim3(64), dim3(256), 0, s, perm, start, (const int*)big,
+                       (int*)tmp);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+}  // namespace se2p

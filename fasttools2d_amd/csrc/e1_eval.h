@@ -1,0 +1,57 @@
+// Device evaluator of the exponential integral E1(x), x > 0.
+//
+// The reference evaluates E1 inline with its own rational fits
+// (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_real.cpp:19-48,166-205).
+// This evaluator uses the library's own fits (e1_coeffs.h, generated by
+// gen_e1_coeffs.py from mpmath):
+//   x <  1:  E1(x) = -gamma - ln x + x A(x),  A a polynomial in x,
+//   x >= 1:  E1(x) = e^{-x} t P(t)/Q(t),      t = 1/x,
+// in accuracy tiers: kE1Full (max rel. error < 1e-18 before rounding) and
+// kE1Fast (< 1e-12, for tolerances of 1e-10 and above).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "e1_coeffs.h"
+
+namespace se2p {
+
+constexpr int kE1Full = 0;
+constexpr int kE1Fast = 1;
+
+template <int N>
+__device__ __forceinline__ double horner(const double (&c)[N], double x)
+{
+    double r = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) r = fma(r, x, c[i]);
+    return r;
+}
+
+template <int TIER>
+struct E1Coeffs;
+
+template <>
+struct E1Coeffs<kE1Full> {
+    static constexpr double near[SE2P_E1_FULL_NEAR_DEG + 1] = SE2P_E1_FULL_NEAR;
+    static constexpr double farp[SE2P_E1_FULL_FAR_DEG + 1] = SE2P_E1_FULL_FARP;
+    static constexpr double farq[SE2P_E1_FULL_FAR_DEG + 1] = SE2P_E1_FULL_FARQ;
+};
+
+template <>
+struct E1Coeffs<kE1Fast> {
+    static constexpr double near[SE2P_E1_FAST_NEAR_DEG + 1] = SE2P_E1_FAST_NEAR;
+    static constexpr double farp[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARP;
+    static constexpr double farq[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARQ;
+};
+
+// E1(x) given e = exp(-x) and t = 1/x, both already needed by the caller.
+template <int TIER>
+__device__ __forceinline__ double e1_eval(double x, double e, double t)
+{
+    using C = E1Coeffs<TIER>;
+    if (x < 1.0) return (-SE2P_EULER_GAMMA - log(x)) + x * horner(C::near, x);
+    return e * t * (horner(C::farp, t) / horner(C::farq, t));
+}
+
+}  // namespace se2p

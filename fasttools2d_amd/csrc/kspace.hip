@@ -1,0 +1,473 @@
+// k-space part of the spectral Ewald sum: FGG spreading, halo merge, Stokes
+// filters, FGG gather.
+//
+// Follows PeriodicEwald/src/mex/common/ewald_tools.cpp:120-271 (Spread,
+// Gather) and mex_stokes_{slp,dlp}_kspace.cpp (filters).  The Gaussian
+// exp(-2 xi^2 |x - x_node|^2 / eta) over the (P+1)^2 nodes that start at a
+// point's start node factors into wx[i] wy[j]; both 1D factors come from the
+// fast-Gaussian-gridding recurrence wx[i] = exp(t(px^2 + 2 w px)) e3x^i e1[i],
+// e3x = exp(-2 t h px), e1[i] = exp(t h^2 (i - P/2)^2), t = -2 xi^2/eta.
+//
+// Spreading.  Points are binned by the tile (TX x TY start nodes) their start
+// node falls in, in increasing index order.  A tile's points touch an R x R
+// node region (R = TX + P).  Its contribution to the two real grids is the
+// product  H_c = A^T diag(d_c) B  with A[k][gx] = wx_k(gx), B[k][gy] = wy_k(gy)
+// (zero outside the support), evaluated as a register-blocked FP64 outer-
+// product accumulation: 64 lanes, each owning a (R/8) x (R/8) block of nodes
+// of both grids.  Every node sums its points in index order, so the result is
+// bit-reproducible.  The R x R partial regions are then merged into the grid
+// by a node-parallel pass that adds the overlapping tiles in a fixed order.
+//
+// Two real grids travel as one complex grid Z = H_0 + i H_1, so the SLP needs
+// one forward and one inverse complex FFT; the DLP spreads (f1 n1, f2 n1) and
+// (f1 n2, f2 n2) into two complex grids.
+#include "se2p_internal.h"
+
+namespace se2p {
+
+namespace {
+
+constexpr int kMaxP = 40;
+
+__device__ __forceinline__ void point_density(int mode, int p, const double* __restrict__ f,
+                                              const double* __restrict__ nrm, double& d0,
+                                              double& d1)
+{
+    const double f1 = f[2 * p], f2 = f[2 * p + 1];
+    if (mode == 0) {  // SLP: (f1, f2)
+        d0 = f1;
+        d1 = f2;
+    } else {  // DLP: (f1 n_c, f2 n_c), c = mode - 1 (mex_stokes_dlp_kspace.cpp:79-87)
+        const double nc = nrm[2 * p + (mode - 1)];
+        d0 = f1 * nc;
+        d1 = f2 * nc;
+    }
+}
+
+template <int BLK>
+__global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__ tstart,
+                                               const int* __restrict__ perm,
+                                               const int2* __restrict__ node,
+                                               const double2* __restrict__ off,
+                                               const double* __restrict__ f,
+                                               const double* __restrict__ nrm, int P, int TX,
+                                               int TY, int ntx, double t, double w, double h,
+                                               const double* __restrict__ e1tab,
+                                               double* __restrict__ partial)
+{
+    constexpr int R = 8 * BLK;
+    constexpr int KC = 16;
+    __shared__ double A[KC][R];
+    __shared__ double B0[KC][R];
+    __shared__ double B1[KC][R];
+    __shared__ double e1s[kMaxP + 1];
+
+    const int tile = blockIdx.x;
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int lane = threadIdx.x;
+    const int bi = (lane >> 3) * BLK, bj = (lane & 7) * BLK;
+    for (int i = lane; i <= P; i += 64) e1s[i] = e1tab[i];
+
+    double acc0[BLK][BLK], acc1[BLK][BLK];
+#pragma unroll
+    for (int i = 0; i < BLK; ++i)
+#pragma unroll
+        for (int j = 0; j < BLK; ++j) acc0[i][j] = acc1[i][j] = 0;
+
+    const int k0 = tstart[tile], k1 = tstart[tile + 1];
+    for (int kb = k0; kb < k1; kb += KC) {
+        const int cnt = min(KC, k1 - kb);
+        __syncthreads();
+        for (int i = lane; i < KC * R; i += 64) {
+            (&A[0][0])[i] = 0;
+            (&B0[0][0])[i] = 0;
+            (&B1[0][0])[i] = 0;
+        }
+        __syncthreads();
+        if (lane < cnt) {
+            const int p = perm[kb + lane];
+            const int2 m = node[p];
+            const double2 o = off[p];
+            const int ox = m.x - tx * TX, oy = m.y - ty * TY;
+            double d0, d1;
+            point_density(mode, p, f, nrm, d0, d1);
+            double ex = exp(t * (o.x * o.x + 2 * w * o.x));
+            double ey = exp(t * (o.y * o.y + 2 * w * o.y));
+            const double e3x = exp(-2 * t * h * o.x);
+            const double e3y = exp(-2 * t * h * o.y);
+            for (int i = 0; i <= P; ++i) {
+                A[lane][ox + i] = ex * e1s[i];
+                const double wy = ey * e1s[i];
+                B0[lane][oy + i] = d0 * wy;
+                B1[lane][oy + i] = d1 * wy;
+                ex *= e3x;
+                ey *= e3y;
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+            double a[BLK], b0[BLK], b1[BLK];
+#pragma unroll
+            for (int i = 0; i < BLK; ++i) {
+                a[i] = A[k][bi + i];
+                b0[i] = B0[k][bj + i];
+                b1[i] = B1[k][bj + i];
+            }
+#pragma unroll
+            for (int i = 0; i < BLK; ++i)
+#pragma unroll
+                for (int j = 0; j < BLK; ++j) {
+                    acc0[i][j] = fma(a[i], b0[j], acc0[i][j]);
+                    acc1[i][j] = fma(a[i], b1[j], acc1[i][j]);
+                }
+        }
+    }
+    double* base = partial + (size_t)tile * 2 * R * R;
+#pragma unroll
+    for (int i = 0; i < BLK; ++i)
+#pragma unroll
+        for (int j = 0; j < BLK; ++j) {
+            base[(bi + i) * R + bj + j] = acc0[i][j];
+            base[R * R + (bi + i) * R + bj + j] = acc1[i][j];
+        }
+}
+
+// Node-parallel halo merge: node (ix, iy) adds, in a fixed order, every tile
+// region that covers it (regions wrap periodically, possibly more than once
+// on grids smaller than R).
+__global__ void k_merge(const double* __restrict__ partial, int Mx, int My, int TX, int TY,
+                        int ntx, int nty, int R, double2* __restrict__ Z)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)Mx * My) return;
+    const int ix = (int)(gid / My), iy = (int)(gid % My);
+    const int ndx = min(ntx, R / TX + 2), ndy = min(nty, R / TY + 2);
+    const int bx = ix / TX, by = iy / TY;
+    double s0 = 0, s1 = 0;
+    for (int dx = 0; dx < ndx; ++dx) {
+        const int tx = ((bx - dx) % ntx + ntx) % ntx;
+        for (int lx = ((ix - tx * TX) % Mx + Mx) % Mx; lx < R; lx += Mx) {
+            for (int dy = 0; dy < ndy; ++dy) {
+                const int ty = ((by - dy) % nty + nty) % nty;
+                const double* q = partial + (size_t)(ty * ntx + tx) * 2 * R * R + (size_t)lx * R;
+                for (int ly = ((iy - ty * TY) % My + My) % My; ly < R; ly += My) {
+                    s0 += q[ly];
+                    s1 += q[R * R + ly];
+                }
+            }
+        }
+    }
+    Z[gid] = make_double2(s0, s1);
+}
+
+// Wavenumber of grid index j: indices up to M/2 non-negative
+// (mex_stokes_slp_kspace.cpp:115-128,141-150).
+__device__ __forceinline__ double wavenum(int j, int M, double L)
+{
+    return 2.0 * kPi / L * (double)(j <= M / 2 ? j : j - M);
+}
+
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+
+// Real-field spectra packed as Z = A + i B: A(k) = (Z(k) + conj Z(-k))/2,
+// B(k) = (Z(k) - conj Z(-k))/(2i).
+__device__ __forceinline__ void unpack(double2 zk, double2 zm, cplx& a, cplx& b)
+{
+    a = {0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y)};
+    b = {0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x)};
+}
+
+// SLP filter at (k1, k2) applied to q = (q1, q2) (mex_stokes_slp_kspace.cpp:130-139):
+//   (K^2 q - k (k.q)) (1/K^4 + 1/(4 xi^2 K^2)) exp(-(1 - eta) K^2/(4 xi^2))
+__device__ __forceinline__ void slp_apply(double k1, double k2, double a, double eta, cplx q1,
+                                          cplx q2, cplx& g1, cplx& g2)
+{
+    const double K2 = k1 * k1 + k2 * k2;
+    const double e = (1.0 / (K2 * K2) + a / K2) * exp(-(1 - eta) * a * K2);
+    const cplx kd = cadd(cscale(q1, k1), cscale(q2, k2));
+    g1 = cscale(csub(cscale(q1, K2), cscale(kd, k1)), e);
+    g2 = cscale(csub(cscale(q2, K2), cscale(kd, k2)), e);
+}
+
+// DLP bracket at (k1, k2) (mex_stokes_dlp_kspace.cpp:164-191): with Qs = Q12 + Q21,
+//   v_i = 2 k_i Q_ii + k_j Qs + k_i (Q11 + Q22) - 2 k_i (k.Q.k)/K^2,
+// scaled by e = exp(-(1-eta) K^2/(4 xi^2)) (1 + K^2/(4 xi^2))/K^2.  The filter
+// output is i e v.
+__device__ __forceinline__ void dlp_bracket(double k1, double k2, double a, double eta,
+                                            cplx q11, cplx qs, cplx q22, cplx& v1, cplx& v2)
+{
+    const double K2 = k1 * k1 + k2 * k2;
+    const double e = exp(-(1 - eta) * a * K2) * (1 + a * K2) / K2;
+    const cplx tr = cadd(q11, q22);
+    const cplx m = cscale(cadd(cadd(cscale(q11, k1 * k1), cscale(qs, k1 * k2)), cscale(q22, k2 * k2)),
+                          1.0 / K2);
+    v1 = cscale(csub(cadd(cadd(cscale(q11, 2 * k1), cscale(qs, k2)), cscale(tr, k1)), cscale(m, 2 * k1)), e);
+    v2 = cscale(csub(cadd(cadd(cscale(q22, 2 * k2), cscale(qs, k1)), cscale(tr, k2)), cscale(m, 2 * k2)), e);
+}
+
+// Filter, k = 0 removal and the 1/(Mx My) of the inverse transform.  The
+// reference keeps only the real part of each inverse transform
+// (mex_stokes_slp_kspace.cpp:180-181), i.e. the Hermitian part of each filtered
+// spectrum; the output W = G1h + i G2h packs both Hermitian parts so a single
+// inverse FFT returns g1 + i g2.  G(-k) uses the reference's wavenumbers at
+// index -k, which differ from -k only on Nyquist lines.
+template <int KIND>
+__global__ void k_filter(const double2* __restrict__ Z0, const double2* __restrict__ Z1,
+                         double2* __restrict__ W, int Mx, int My, double Lx, double Ly,
+                         double xi, double eta, double scale)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)Mx * My) return;
+    const int ix = (int)(gid / My), iy = (int)(gid % My);
+    if (ix == 0 && iy == 0) {
+        W[0] = make_double2(0.0, 0.0);
+        return;
+    }
+    const int jx = (Mx - ix) % Mx, jy = (My - iy) % My;
+    const long gm = (long)jx * My + jy;
+    const double k1 = wavenum(ix, Mx, Lx), k2 = wavenum(iy, My, Ly);
+    const double m1 = wavenum(jx, Mx, Lx), m2 = wavenum(jy, My, Ly);
+    const double a = 0.25 / (xi * xi);
+    cplx g1, g2;
+    if (KIND == SE2P_SLP) {
+        cplx q1, q2;
+        unpack(Z0[gid], Z0[gm], q1, q2);
+        cplx p1, p2, r1, r2;
+        slp_apply(k1, k2, a, eta, q1, q2, p1, p2);
+        slp_apply(m1, m2, a, eta, q1, q2, r1, r2);
+        g1 = cscale(cadd(p1, r1), 0.5);
+        g2 = cscale(cadd(p2, r2), 0.5);
+    } else {
+        cplx q11, q21, q12, q22;
+        unpack(Z0[gid], Z0[gm], q11, q21);  // spread (f1 n1, f2 n1)
+        unpack(Z1[gid], Z1[gm], q12, q22);  // spread (f1 n2, f2 n2)
+        const cplx qs = cadd(q12, q21);
+        cplx p1, p2, r1, r2;
+        dlp_bracket(k1, k2, a, eta, q11, qs, q22, p1, p2);
+        dlp_bracket(m1, m2, a, eta, q11, qs, q22, r1, r2);
+        // Hermitian part of i e v(k): i (e v(k) - e' v(-k index)) / 2
+        const cplx d1 = cscale(csub(p1, r1), 0.5), d2 = cscale(csub(p2, r2), 0.5);
+        g1 = {-d1.im, d1.re};
+        g2 = {-d2.im, d2.re};
+    }
+    // W = g1 + i g2
+    W[gid] = make_double2((g1.re - g2.im) * scale, (g1.im + g2.re) * scale);
+}
+
+__global__ __launch_bounds__(64) void k_gather(const int* __restrict__ tstart,
+                                               const int* __restrict__ perm,
+                                               const int2* __restrict__ node,
+                                               const double2* __restrict__ off,
+                                               const double2* __restrict__ W, int Mx, int My,
+                                               int TX, int TY, int ntx, int P, int R, double t,
+                                               double w, double h,
+                                               const double* __restrict__ e1tab, double scale,
+                                               const double* add_first,
+                                               const double* __restrict__ zm, double* u)
+{
+    extern __shared__ double2 reg[];
+    __shared__ double e1s[kMaxP + 1];
+    const int tile = blockIdx.x;
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int X0 = tx * TX, Y0 = ty * TY;
+    const int k0 = tstart[tile], k1 = tstart[tile + 1];
+    if (k0 == k1) return;
+    for (int i = threadIdx.x; i <= P; i += blockDim.x) e1s[i] = e1tab[i];
+    for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
+        const int lx = idx / R, ly = idx % R;
+        reg[idx] = W[(size_t)((X0 + lx) % Mx) * My + (Y0 + ly) % My];
+    }
+    __syncthreads();
+    const double z0 = zm ? zm[0] : 0.0, z1 = zm ? zm[1] : 0.0;
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const int p = perm[k];
+        const int2 m = node[p];
+        const double2 o = off[p];
+        const int ox = m.x - X0, oy = m.y - Y0;
+        double ex = exp(t * (o.x * o.x + 2 * w * o.x));
+        const double ey0 = exp(t * (o.y * o.y + 2 * w * o.y));
+        const double e3x = exp(-2 * t * h * o.x);
+        const double e3y = exp(-2 * t * h * o.y);
+        double s0 = 0, s1 = 0;
+        for (int i = 0; i <= P; ++i) {
+            double ey = ex * ey0 * e1s[i];
+            const double2* row = reg + (ox + i) * R + oy;
+            for (int j = 0; j <= P; ++j) {
+                const double g = ey * e1s[j];
+                const double2 v = row[j];
+                s0 += g * v.x;
+                s1 += g * v.y;
+                ey *= e3y;
+            }
+            ex *= e3x;
+        }
+        double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
+        if (add_first) {
+            v0 = add_first[2 * p] + v0;
+            v1 = add_first[2 * p + 1] + v1;
+        }
+        u[2 * p] = v0;
+        u[2 * p + 1] = v1;
+    }
+}
+
+__global__ void k_e1_table(double xi, double eta, double h, int P, double* tab)
+{
+    int j = threadIdx.x;
+    if (j > P) return;
+    // ewald_tools.cpp:125-127
+    const double c = -2 * xi * xi / eta * h * h;
+    const int q = j - P / 2;
+    tab[j] = exp(c * q * q);
+}
+
+// Deterministic two-level reduction of sum_s (f.n)_s (x_s, y_s).
+__global__ __launch_bounds__(256) void k_zero_mode_part(const double* __restrict__ p,
+                                                        const double* __restrict__ f,
+                                                        const double* __restrict__ nrm, int n,
+                                                        double* __restrict__ part)
+{
+    __shared__ double sx[256], sy[256];
+    double ax = 0, ay = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double fn = f[2 * i] * nrm[2 * i] + f[2 * i + 1] * nrm[2 * i + 1];
+        ax += fn * p[2 * i];
+        ay += fn * p[2 * i + 1];
+    }
+    sx[threadIdx.x] = ax;
+    sy[threadIdx.x] = ay;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            sx[threadIdx.x] += sx[threadIdx.x + d];
+            sy[threadIdx.x] += sy[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sx[0];
+        part[2 * blockIdx.x + 1] = sy[0];
+    }
+}
+
+__global__ void k_zero_mode_final(const double* __restrict__ part, int nblk, double inv_area,
+                                  double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double sx = 0, sy = 0;
+    for (int b = 0; b < nblk; ++b) {
+        sx += part[2 * b];
+        sy += part[2 * b + 1];
+    }
+    out[0] = sx * inv_area;
+    out[1] = sy * inv_area;
+}
+
+}  // namespace
+
+int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi, double eta,
+               double w)
+{
+    if (P < 2 || (P & 1) || P > kMaxP)
+        return set_error(SE2P_ERR_UNSUPPORTED, "P = %d: need an even P in [2, %d]", P, kMaxP);
+    g->Mx = Mx;
+    g->My = My;
+    g->P = P;
+    g->Lx = Lx;
+    g->Ly = Ly;
+    g->h = Lx / Mx;
+    g->xi = xi;
+    g->eta = eta;
+    g->w = w;
+    g->R = P <= 24 ? 32 : 48;
+    g->TX = g->TY = g->R - P;
+    g->ntx = (Mx + g->TX - 1) / g->TX;
+    g->nty = (My + g->TY - 1) / g->TY;
+    return SE2P_OK;
+}
+
+int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
+{
+    hipLaunchKernelGGL(k_e1_table, dim3(1), dim3(64), 0, s, g.xi, g.eta, g.h, g.P, tab);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_spread(hipStream_t s, int mode, const KGeom& g, const int* tstart, const int* perm,
+                  const int2* node, const double2* off, const double* f, const double* nrm,
+                  const double* e1tab, double* partial)
+{
+    const int ntiles = g.ntx * g.nty;
+    const double t = -2 * g.xi * g.xi / g.eta;
+    if (g.R == 32)
+        hipLaunchKernelGGL(k_spread<4>, dim3(ntiles), dim3(64), 0, s, mode, tstart, perm, node,
+                           off, f, nrm, g.P, g.TX, g.TY, g.ntx, t, g.w, g.h, e1tab, partial);
+    else
+        hipLaunchKernelGGL(k_spread<6>, dim3(ntiles), dim3(64), 0, s, mode, tstart, perm, node,
+                           off, f, nrm, g.P, g.TX, g.TY, g.ntx, t, g.w, g.h, e1tab, partial);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_merge(hipStream_t s, int /*ncomp*/, const KGeom& g, const double* partial,
+                 double2* Z0, double2* /*Z1*/)
+{
+    const long n = (long)g.Mx * g.My;
+    hipLaunchKernelGGL(k_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, g.Mx,
+                       g.My, g.TX, g.TY, g.ntx, g.nty, g.R, Z0);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_filter(hipStream_t s, int kind, const KGeom& g, const double2* Z0,
+                  const double2* Z1, double2* W)
+{
+    const long n = (long)g.Mx * g.My;
+    const double scale = 1.0 / ((double)g.Mx * (double)g.My);
+    dim3 gr((unsigned)((n + 255) / 256)), b(256);
+    if (kind == SE2P_SLP)
+        hipLaunchKernelGGL(k_filter<SE2P_SLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
+                           g.xi, g.eta, scale);
+    else
+        hipLaunchKernelGGL(k_filter<SE2P_DLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
+                           g.xi, g.eta, scale);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_gather(hipStream_t s, const KGeom& g, const int* tstart, const int* perm,
+                  const int2* node, const double2* off, const double2* W, const double* e1tab,
+                  const double* add_first, const double* zero_mode, double* u)
+{
+    const int ntiles = g.ntx * g.nty;
+    const double t = -2 * g.xi * g.xi / g.eta;
+    // (4 xi^2/eta)^2 h^2/pi/(4 pi)  (ewald_tools.cpp:267-269)
+    double sc = 4 * g.xi * g.xi / g.eta;
+    sc = sc * sc * g.h * g.h / kPi;
+    sc = sc / (4 * kPi);
+    const size_t lds = sizeof(double2) * (size_t)g.R * g.R;
+    hipLaunchKernelGGL(k_gather, dim3(ntiles), dim3(64), lds, s, tstart, perm, node, off, W, g.Mx,
+                       g.My, g.TX, g.TY, g.ntx, g.P, g.R, t, g.w, g.h, e1tab, sc, add_first,
+                       zero_mode, u);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_dlp_zero_mode(hipStream_t s, const double* p, const double* f, const double* nrm,
+                         int n, double Lx, double Ly, double* partial, double* out)
+{
+    constexpr int nblk = 128;
+    hipLaunchKernelGGL(k_zero_mode_part, dim3(nblk), dim3(256), 0, s, p, f, nrm, n, partial);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_mode_final, dim3(1), dim3(64), 0, s, partial, nblk,
+                       1.0 / (Lx * Ly), out);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+}  // namespace se2p

@@ -1,0 +1,1008 @@
+// Host side of libse2p.so: the C ABI, contexts, workspace, stage timing, the
+// mex-equivalent entry points and the driver (parameter recipe, wrapping,
+// zero mode).  Reference paths are relative to the reference repository root.
+#include "se2p_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace se2p {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+int set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int ws_get(se2p_ctx* ctx, Slot slot, size_t bytes, void** out)
+{
+    if (bytes < 256) bytes = 256;
+    if (ctx->ws_size[slot] < bytes) {
+        if (ctx->ws_ptr[slot]) {
+            // The stream may still read the old buffer.
+            SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            SE2P_HIP_CHECK(hipFree(ctx->ws_ptr[slot]));
+            ctx->ws_ptr[slot] = nullptr;
+            ctx->ws_size[slot] = 0;
+        }
+        size_t sz = bytes + bytes / 8;  // some headroom for slowly growing inputs
+        void* p = nullptr;
+        SE2P_HIP_CHECK(hipMalloc(&p, sz));
+        ctx->ws_ptr[slot] = p;
+        ctx->ws_size[slot] = sz;
+    }
+    *out = ctx->ws_ptr[slot];
+    return SE2P_OK;
+}
+
+int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec)
+{
+    auto take = [&](hipEvent_t* e) -> int {
+        if (!ctx->event_pool.empty()) {
+            *e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+            return SE2P_OK;
+        }
+        SE2P_HIP_CHECK(hipEventCreate(e));
+        return SE2P_OK;
+    };
+    rec->name = name;
+    SE2P_TRY(take(&rec->start));
+    SE2P_TRY(take(&rec->stop));
+    SE2P_HIP_CHECK(hipEventRecord(rec->start, ctx->stream));
+    return SE2P_OK;
+}
+
+int prof_end(se2p_ctx* ctx, ProfRecord* rec)
+{
+    SE2P_HIP_CHECK(hipEventRecord(rec->stop, ctx->stream));
+    ctx->prof_pending.push_back(*rec);
+    return SE2P_OK;
+}
+
+int prof_flush(se2p_ctx* ctx)
+{
+    if (ctx->prof_pending.empty()) return SE2P_OK;
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto& r : ctx->prof_pending) {
+        float ms = 0;
+        SE2P_HIP_CHECK(hipEventElapsedTime(&ms, r.start, r.stop));
+        ProfAcc* a = nullptr;
+        for (auto& x : ctx->prof_acc)
+            if (x.name == r.name) a = &x;
+        if (!a) {
+            ctx->prof_acc.push_back(ProfAcc{r.name, 0.0, 0});
+            a = &ctx->prof_acc.back();
+        }
+        a->ms += ms;
+        a->count += 1;
+        ctx->event_pool.push_back(r.start);
+        ctx->event_pool.push_back(r.stop);
+    }
+    ctx->prof_pending.clear();
+    return SE2P_OK;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Validation
+// ---------------------------------------------------------------------------
+int check_common(se2p_ctx* ctx, int Ns, int Nt, double Lx, double Ly)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (Ns < 0 || Nt < 0) return set_error(SE2P_ERR_ARG, "negative point count");
+    if (!(Lx > 0) || !(Ly > 0) || !std::isfinite(Lx) || !std::isfinite(Ly))
+        return set_error(SE2P_ERR_ARG, "box lengths must be positive and finite");
+    return SE2P_OK;
+}
+
+int check_real(int nsx, int nsy, double xi)
+{
+    if (nsx < 1 || nsy < 1) return set_error(SE2P_ERR_ARG, "nside_x and nside_y must be >= 1");
+    if ((long)nsx * nsy > (1L << 26)) return set_error(SE2P_ERR_UNSUPPORTED, "too many cells");
+    if (!(xi > 0) || !std::isfinite(xi)) return set_error(SE2P_ERR_ARG, "xi must be positive");
+    return SE2P_OK;
+}
+
+int check_kspace(int Mx, int My, double Lx, double Ly, double xi, double eta, double w, int P)
+{
+    if (Mx < 2 || My < 2 || (Mx & 1) || (My & 1))
+        return set_error(SE2P_ERR_ARG, "Mx and My must be even and >= 2 (got %d, %d)", Mx, My);
+    if (std::fabs(Lx / Mx - Ly / My) > 1e-12 * (Lx / Mx))
+        return set_error(SE2P_ERR_ARG, "grid spacing must agree: Lx/Mx = %.17g, Ly/My = %.17g",
+                         Lx / Mx, Ly / My);
+    if (P < 2 || (P & 1)) return set_error(SE2P_ERR_ARG, "P must be even and >= 2 (got %d)", P);
+    if (!(xi > 0) || !(eta > 0) || !std::isfinite(w))
+        return set_error(SE2P_ERR_ARG, "xi and eta must be positive");
+    if (Mx > kFftMaxLen || My > kFftMaxLen)
+        return set_error(SE2P_ERR_UNSUPPORTED, "grid %d x %d beyond %d per side", Mx, My,
+                         kFftMaxLen);
+    return SE2P_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device pipelines
+// ---------------------------------------------------------------------------
+int real_space(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+               const double* d_f, const double* d_n, int Ns, int Nt, double xi, int nsx,
+               int nsy, double Lx, double Ly, double* d_u)
+{
+    if (Nt == 0) return SE2P_OK;
+    hipStream_t s = ctx->stream;
+    const int nbox = nsx * nsy;
+    void *skey, *sperm, *sstart, *scount, *tkey, *tperm, *tstart, *tcount, *ssorted, *tsorted;
+    SE2P_TRY(ws_get(ctx, WS_SRC_KEY, sizeof(int) * (size_t)Ns, &skey));
+    SE2P_TRY(ws_get(ctx, WS_SRC_PERM, sizeof(int) * (size_t)Ns, &sperm));
+    SE2P_TRY(ws_get(ctx, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &sstart));
+    SE2P_TRY(ws_get(ctx, WS_SRC_COUNT, sizeof(int) * (size_t)nbox, &scount));
+    SE2P_TRY(ws_get(ctx, WS_TAR_KEY, sizeof(int) * (size_t)Nt, &tkey));
+    SE2P_TRY(ws_get(ctx, WS_TAR_PERM, sizeof(int) * (size_t)Nt, &tperm));
+    SE2P_TRY(ws_get(ctx, WS_TAR_START, sizeof(int) * (size_t)(nbox + 1), &tstart));
+    SE2P_TRY(ws_get(ctx, WS_TAR_COUNT, sizeof(int) * (size_t)nbox, &tcount));
+    const size_t rec = kind == SE2P_SLP ? 32 : 64;
+    SE2P_TRY(ws_get(ctx, WS_SRC_SORTED, rec * (size_t)Ns, &ssorted));
+    SE2P_TRY(ws_get(ctx, WS_TAR_SORTED, sizeof(double2) * (size_t)Nt, &tsorted));
+    {
+        StageTimer st(ctx, "bin");
+        SE2P_TRY(launch_fill_int(s, (int*)scount, nbox, 0));
+        SE2P_TRY(launch_cell_keys(s, d_psrc, Ns, Lx, Ly, nsx, nsy, (int*)skey, (int*)scount));
+        SE2P_TRY(sort_by_key(ctx, (int*)skey, (int*)scount, Ns, nbox, (int*)sperm, (int*)sstart,
+                             WS_SORT_TMP));
+        SE2P_TRY(launch_fill_int(s, (int*)tcount, nbox, 0));
+        SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
+        SE2P_TRY(sort_by_key(ctx, (int*)tkey, (int*)tcount, Nt, nbox, (int*)tperm, (int*)tstart,
+                             WS_SORT_TMP));
+        SE2P_TRY(launch_pack_sources(s, kind, (int*)sperm, Ns, d_psrc, d_f, d_n,
+                                     (double*)ssorted));
+        SE2P_TRY(launch_pack_targets(s, (int*)tperm, Nt, d_ptar, (double2*)tsorted));
+    }
+    {
+        StageTimer st(ctx, "real");
+        const int mean_t = (int)((Nt + nbox - 1) / nbox);
+        SE2P_TRY(launch_real(s, kind, nsx, nsy, Lx, Ly, xi, (int*)tstart, (int*)tperm,
+                             (double2*)tsorted, (int*)sstart, (double*)ssorted, mean_t, d_u));
+    }
+    return SE2P_OK;
+}
+
+int kspace(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+           const double* d_f, const double* d_n, int Ns, int Nt, double xi, double eta, int Mx,
+           int My, double Lx, double Ly, double w, int P, const double* d_add_first,
+           const double* d_zm, double* d_u)
+{
+    if (Nt == 0) return SE2P_OK;
+    hipStream_t s = ctx->stream;
+    KGeom g;
+    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
+    FftAxis ax, ay;
+    SE2P_TRY(fft_axis_init(&ax, Mx));
+    SE2P_TRY(fft_axis_init(&ay, My));
+    double2 *twx, *twy;
+    SE2P_TRY(fft_twiddles(ctx, Mx, WS_TWIDDLE_X, &twx));
+    SE2P_TRY(fft_twiddles(ctx, My, WS_TWIDDLE_Y, &twy));
+    const int ntiles = g.ntx * g.nty;
+    const size_t G = (size_t)Mx * My;
+    void *e1tab, *skey, *sperm, *sstart, *scount, *snode, *soff, *part, *Z0, *Z1 = nullptr, *W;
+    void *tkey, *tperm, *tstart, *tcount, *tnode, *toff;
+    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
+    SE2P_TRY(ws_get(ctx, WS_SPR_KEY, sizeof(int) * (size_t)Ns, &skey));
+    SE2P_TRY(ws_get(ctx, WS_SPR_PERM, sizeof(int) * (size_t)Ns, &sperm));
+    SE2P_TRY(ws_get(ctx, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &sstart));
+    SE2P_TRY(ws_get(ctx, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &scount));
+    SE2P_TRY(ws_get(ctx, WS_SPR_NODE, sizeof(int2) * (size_t)Ns, &snode));
+    SE2P_TRY(ws_get(ctx, WS_SPR_OFF, sizeof(double2) * (size_t)Ns, &soff));
+    SE2P_TRY(ws_get(ctx, WS_GAT_KEY, sizeof(int) * (size_t)Nt, &tkey));
+    SE2P_TRY(ws_get(ctx, WS_GAT_PERM, sizeof(int) * (size_t)Nt, &tperm));
+    SE2P_TRY(ws_get(ctx, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &tstart));
+    SE2P_TRY(ws_get(ctx, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &tcount));
+    SE2P_TRY(ws_get(ctx, WS_GAT_NODE, sizeof(int2) * (size_t)Nt, &tnode));
+    SE2P_TRY(ws_get(ctx, WS_GAT_OFF, sizeof(double2) * (size_t)Nt, &toff));
+    SE2P_TRY(ws_get(ctx, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
+    SE2P_TRY(ws_get(ctx, WS_GRID_A, sizeof(double2) * G, &Z0));
+    if (kind == SE2P_DLP) SE2P_TRY(ws_get(ctx, WS_GRID_B, sizeof(double2) * G, &Z1));
+    SE2P_TRY(ws_get(ctx, WS_GRID_C, sizeof(double2) * G, &W));
+
+    {
+        StageTimer st(ctx, "bin_grid");
+        SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
+        SE2P_TRY(launch_fill_int(s, (int*)scount, ntiles, 0));
+        SE2P_TRY(launch_tile_keys(s, d_psrc, Ns, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
+                                  (int*)skey, (int*)scount, (int2*)snode, (double2*)soff));
+        SE2P_TRY(sort_by_key(ctx, (int*)skey, (int*)scount, Ns, ntiles, (int*)sperm,
+                             (int*)sstart, WS_SORT_TMP));
+        SE2P_TRY(launch_fill_int(s, (int*)tcount, ntiles, 0));
+        SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
+                                  (int*)tkey, (int*)tcount, (int2*)tnode, (double2*)toff));
+        SE2P_TRY(sort_by_key(ctx, (int*)tkey, (int*)tcount, Nt, ntiles, (int*)tperm,
+                             (int*)tstart, WS_SORT_TMP));
+    }
+    const int nspread = kind == SE2P_SLP ? 1 : 2;
+    for (int c = 0; c < nspread; ++c) {
+        double2* Zc = (double2*)(c == 0 ? Z0 : Z1);
+        {
+            StageTimer st(ctx, "spread");
+            SE2P_TRY(launch_spread(s, kind == SE2P_SLP ? 0 : 1 + c, g, (int*)sstart, (int*)sperm,
+                                   (int2*)snode, (double2*)soff, d_f, d_n, (double*)e1tab,
+                                   (double*)part));
+        }
+        {
+            StageTimer st(ctx, "merge");
+            SE2P_TRY(launch_merge(s, 2, g, (double*)part, Zc, nullptr));
+        }
+        {
+            StageTimer st(ctx, "fft");
+            SE2P_TRY(fft2d(ctx, Zc, Mx, My, ax, ay, twx, twy, 0));
+        }
+    }
+    {
+        StageTimer st(ctx, "filter");
+        SE2P_TRY(launch_filter(s, kind, g, (double2*)Z0, (double2*)Z1, (double2*)W));
+    }
+    {
+        StageTimer st(ctx, "fft");
+        SE2P_TRY(fft2d(ctx, (double2*)W, Mx, My, ax, ay, twx, twy, 1));
+    }
+    {
+        StageTimer st(ctx, "gather");
+        SE2P_TRY(launch_gather(s, g, (int*)tstart, (int*)tperm, (int2*)tnode, (double2*)toff,
+                               (double2*)W, (double*)e1tab, d_add_first, d_zm, d_u));
+    }
+    return SE2P_OK;
+}
+
+// Copies host inputs into workspace slots.
+int upload(se2p_ctx* ctx, Slot slot, const double* h, size_t n, double** d)
+{
+    void* p = nullptr;
+    SE2P_TRY(ws_get(ctx, slot, sizeof(double) * n, &p));
+    if (n) SE2P_HIP_CHECK(hipMemcpyAsync(p, h, sizeof(double) * n, hipMemcpyHostToDevice,
+                                         ctx->stream));
+    *d = (double*)p;
+    return SE2P_OK;
+}
+
+int download(se2p_ctx* ctx, double* h, const double* d, size_t n)
+{
+    if (n) SE2P_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SE2P_OK;
+}
+
+int need_ptr(const void* p, int n, const char* what)
+{
+    if (n > 0 && !p) return set_error(SE2P_ERR_ARG, "%s is NULL", what);
+    return SE2P_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Parameter recipe (StokesSLP_ewald_2p.m:84-104,147-202;
+// StokesDLP_ewald_2p.m:104-125,171-225)
+// ---------------------------------------------------------------------------
+// MATLAB rat() with its default tolerance 1e-6 |X|: continued-fraction
+// convergents of X.
+void rat(double X, int* N, int* D)
+{
+    const double tol = 1e-6 * std::fabs(X);
+    double nn = std::round(X), dd = 1, lastn = 1, lastd = 0;
+    double frac = X - nn;
+    int guard = 0;
+    while (std::fabs(X - nn / dd) >= tol && guard++ < 64) {
+        const double flip = 1 / frac;
+        const double step = std::round(flip);
+        frac = flip - step;
+        const double sn = nn, sd = dd;
+        nn = nn * step + lastn;
+        dd = dd * step + lastd;
+        lastn = sn;
+        lastd = sd;
+    }
+    *N = (int)(nn * (dd < 0 ? -1 : 1));
+    *D = (int)std::fabs(dd);
+}
+
+// Newton solve with the drivers' step rule: step limited to 0.9 x, at most 101
+// steps, stop once |f| <= tol.  which = 0: xi from rc; which = 1: k_inf from xi.
+double newton(int kind, int which, double x, double Q, double Lx, double Ly, double rc, double xi,
+              double tol, bool* ok)
+{
+    *ok = true;
+    for (int it = 0;; ++it) {
+        double fv, fp;
+        if (which == 0) {
+            const double g = std::exp(-x * x * rc * rc);
+            if (kind == SE2P_SLP) {
+                fv = std::sqrt(Q * kPi / (4 * Lx * Ly * x)) * g - tol;
+                fp = -std::sqrt(Q * kPi / (4 * Lx * Ly)) * g *
+                     (0.5 * std::pow(x, -1.5) + 2 * rc * rc * x * std::sqrt(1 / x));
+            } else {
+                const double c = std::sqrt(2 * kPi * Q / (Lx * Ly));
+                fv = g * x * rc * c - tol;
+                fp = (1 - 2 * x * x * rc * rc) * g * c * rc;
+            }
+        } else {
+            const double Lm = Lx > Ly ? Lx : Ly;
+            const double g = std::exp(-x * x / (4 * xi * xi));
+            if (kind == SE2P_SLP) {
+                const double c = 4 * Q * kPi * Lm / (std::pow(Lx, 3) * std::pow(Ly, 3));
+                fv = std::sqrt(c / x) * g - tol;
+                fp = -std::sqrt(c) * g *
+                     (0.5 * std::pow(x, -1.5) + std::sqrt(1 / x) * 2 * x / (4 * xi * xi));
+            } else {
+                // StokesDLP_ewald_2p.m:177: MATLAB evaluates .../Lx^3*Ly^3 left to right.
+                const double c = 8 * kPi * Q * Lm / std::pow(Lx, 3) * std::pow(Ly, 3);
+                fv = std::sqrt(x * c) * g - tol;
+                fp = std::sqrt(c) * g * (0.5 * std::pow(x, -0.5) - 2 * std::pow(x, 1.5) / (4 * xi * xi));
+            }
+        }
+        if (!(std::fabs(fv) > tol)) break;
+        if (it > 100) {
+            *ok = false;
+            break;
+        }
+        double d = fv / fp;
+        if (!(d < 0.9 * x)) d = 0.9 * x;
+        x -= d;
+    }
+    return x;
+}
+
+double default_nb(double npts, int P)
+{
+    // min(P*round(log2(npts)), npts/4), MATLAB round = half away from zero
+    const double a = P * std::round(std::log2(npts));
+    const double b = npts / 4;
+    return a < b ? a : b;
+}
+
+int params_ref(int kind, int Ns, int Nt, double Q, double Lx, double Ly, const se2p_options* o,
+               se2p_params* out)
+{
+    se2p_options d;
+    se2p_options_default(&d);
+    if (!o) o = &d;
+    const int P = o->P;
+    if (P < 2 || (P & 1)) return set_error(SE2P_ERR_ARG, "P must be even and >= 2");
+    if (!(o->tol > 0)) return set_error(SE2P_ERR_ARG, "tol must be positive");
+    const double npts0 = (double)Ns + Nt;
+    if (npts0 < 1) return set_error(SE2P_ERR_ARG, "no points");
+    const double Nb = o->Nb > 0 ? o->Nb : default_nb(npts0, P);
+    // The DLP driver recomputes npts from the 2 x N arrays (StokesDLP_ewald_2p.m:107).
+    const double npts = kind == SE2P_SLP ? npts0 : (double)(Ns > 2 ? Ns : 2) + (Nt > 2 ? Nt : 2);
+    int A, B;
+    rat(Lx / Ly, &A, &B);
+    if (A <= 0 || B <= 0) return set_error(SE2P_ERR_ARG, "rat(Lx/Ly) failed");
+    const double a = std::ceil(std::sqrt(npts / (Nb * A * B)));
+    const double m = 0.95 * std::sqrt(kPi * P);
+    const int nsx = (int)(a * A), nsy = (int)(a * B);
+    const double rc = Lx / nsx;
+    bool ok1, ok2;
+    const double xi = newton(kind, 0, kind == SE2P_SLP ? 4 / rc : 1 / rc, Q, Lx, Ly, rc, 0,
+                             o->tol, &ok1);
+    // Both drivers pass Lx for Ly here (StokesSLP_ewald_2p.m:96).
+    const double kinf = newton(kind, 1, std::round(5 * xi), Q, Lx, Lx, rc, xi, o->tol, &ok2);
+    const int kx = (int)std::round(Lx * kinf / (2 * kPi));
+    const int M0 = 2 * kx < 10000 ? 2 * kx : 10000;
+    out->P = P;
+    out->nside_x = nsx;
+    out->nside_y = nsy;
+    out->rc = rc;
+    out->xi = xi;
+    out->kinf = kx;
+    out->Mx = A * M0;
+    out->My = B * M0;
+    out->w = P * Lx / out->Mx / 2;
+    out->eta = std::pow(2 * xi * out->w / m, 2);
+    out->converged = ok1 && ok2;
+    if (o->verbose)
+        fprintf(stderr,
+                "se2p %s: xi %.6g rc %.6g kinf %d Mx %d My %d nside %d x %d w %.6g eta %.6g\n",
+                kind == SE2P_SLP ? "SLP" : "DLP", xi, rc, kx, out->Mx, out->My, nsx, nsy, out->w,
+                out->eta);
+    return SE2P_OK;
+}
+
+// mod(x + L/2, L) - L/2 with MATLAB's mod (StokesSLP_ewald_2p.m:75-78).
+double wrap1(double x, double L)
+{
+#pragma clang fp contract(off)
+    double v = x + L / 2;
+    v = v - std::floor(v / L) * L;
+    return v - L / 2;
+}
+
+int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc,
+                 const double* xtar, const double* ytar, const double* n1, const double* n2,
+                 const double* f1, const double* f2, int Ns, int Nt, double Lx, double Ly,
+                 const se2p_options* opt, double* u1, double* u2, double* ur_out,
+                 double* uk_out, double* xi_out)
+{
+    SE2P_TRY(check_common(ctx, Ns, Nt, Lx, Ly));
+    SE2P_TRY(need_ptr(xsrc, Ns, "xsrc"));
+    SE2P_TRY(need_ptr(ysrc, Ns, "ysrc"));
+    SE2P_TRY(need_ptr(f1, Ns, "f1"));
+    SE2P_TRY(need_ptr(f2, Ns, "f2"));
+    SE2P_TRY(need_ptr(xtar, Nt, "xtar"));
+    SE2P_TRY(need_ptr(ytar, Nt, "ytar"));
+    SE2P_TRY(need_ptr(u1, Nt, "u1"));
+    SE2P_TRY(need_ptr(u2, Nt, "u2"));
+    if (kind == SE2P_DLP) {
+        SE2P_TRY(need_ptr(n1, Ns, "n1"));
+        SE2P_TRY(need_ptr(n2, Ns, "n2"));
+    }
+    std::vector<double> ps(2 * (size_t)Ns), pt(2 * (size_t)Nt), f(2 * (size_t)Ns), nn;
+    double Q = 1;
+    for (int i = 0; i < Ns; ++i) {
+        ps[2 * i] = wrap1(xsrc[i], Lx);
+        ps[2 * i + 1] = wrap1(ysrc[i], Ly);
+        f[2 * i] = f1[i];
+        f[2 * i + 1] = f2[i];
+    }
+    // Q = sum(sum(f.^2)) + 1, MATLAB sums each row then the two row sums.
+    double q1 = 0, q2 = 0;
+    for (int i = 0; i < Ns; ++i) {
+        q1 += f1[i] * f1[i];
+        q2 += f2[i] * f2[i];
+    }
+    Q = (q1 + q2) + 1;
+    for (int i = 0; i < Nt; ++i) {
+        pt[2 * i] = wrap1(xtar[i], Lx);
+        pt[2 * i + 1] = wrap1(ytar[i], Ly);
+    }
+    if (kind == SE2P_DLP) {
+        nn.resize(2 * (size_t)Ns);
+        for (int i = 0; i < Ns; ++i) {
+            nn[2 * i] = n1[i];
+            nn[2 * i + 1] = n2[i];
+        }
+    }
+    se2p_params prm;
+    SE2P_TRY(params_ref(kind, Ns, Nt, Q, Lx, Ly, opt, &prm));
+    SE2P_TRY(check_real(prm.nside_x, prm.nside_y, prm.xi));
+    SE2P_TRY(check_kspace(prm.Mx, prm.My, Lx, Ly, prm.xi, prm.eta, prm.w, prm.P));
+    if (xi_out) *xi_out = prm.xi;
+    if (Nt == 0) return SE2P_OK;
+
+    double *dps, *dpt, *df, *dn = nullptr, *dur, *duk, *dzm = nullptr;
+    SE2P_TRY(upload(ctx, WS_PTS_A, ps.data(), ps.size(), &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, pt.data(), pt.size(), &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f.data(), f.size(), &df));
+    if (kind == SE2P_DLP) SE2P_TRY(upload(ctx, WS_DENS_B, nn.data(), nn.size(), &dn));
+    void* out = nullptr;
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 4 * (size_t)Nt + 64, &out));
+    dur = (double*)out;
+    duk = dur + 2 * (size_t)Nt;
+    if (kind == SE2P_DLP) {
+        void* part = nullptr;
+        SE2P_TRY(ws_get(ctx, WS_REDUCE, sizeof(double) * 512, &part));
+        dzm = duk + 2 * (size_t)Nt;
+        SE2P_TRY(launch_dlp_zero_mode(ctx->stream, dps, df, dn, Ns, Lx, Ly, (double*)part, dzm));
+    }
+    SE2P_TRY(real_space(ctx, kind, dps, dpt, df, dn, Ns, Nt, prm.xi, prm.nside_x, prm.nside_y,
+                        Lx, Ly, dur));
+    SE2P_TRY(kspace(ctx, kind, dps, dpt, df, dn, Ns, Nt, prm.xi, prm.eta, prm.Mx, prm.My, Lx, Ly,
+                    prm.w, prm.P, nullptr, dzm, duk));
+    std::vector<double> h(4 * (size_t)Nt);
+    SE2P_TRY(download(ctx, h.data(), dur, 4 * (size_t)Nt));
+    for (int i = 0; i < Nt; ++i) {
+        const double r1 = h[2 * i], r2 = h[2 * i + 1];
+        const double k1 = h[2 * (size_t)Nt + 2 * i], k2 = h[2 * (size_t)Nt + 2 * i + 1];
+        u1[i] = r1 + k1;
+        u2[i] = r2 + k2;
+        if (ur_out) {
+            ur_out[2 * i] = r1;
+            ur_out[2 * i + 1] = r2;
+        }
+        if (uk_out) {
+            uk_out[2 * i] = k1;
+            uk_out[2 * i + 1] = k2;
+        }
+    }
+    return SE2P_OK;
+}
+
+}  // namespace
+}  // namespace se2p
+
+using namespace se2p;
+
+extern "C" {
+
+int se2p_version(void) { return 1; }
+
+const char* se2p_last_error(void) { return se2p::g_err; }
+
+int se2p_device_count(int* count)
+{
+    if (!count) return set_error(SE2P_ERR_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return SE2P_OK;
+}
+
+int se2p_ctx_create(int device, se2p_ctx** out)
+{
+    if (!out) return set_error(SE2P_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return set_error(SE2P_ERR_NODEV, "no HIP device available");
+    if (device < 0 || device >= n)
+        return set_error(SE2P_ERR_ARG, "device %d outside [0, %d)", device, n);
+    SE2P_HIP_CHECK(hipSetDevice(device));
+    se2p_ctx* c = new (std::nothrow) se2p_ctx();
+    if (!c) return set_error(SE2P_ERR_NOMEM, "context allocation failed");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return set_error(SE2P_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return SE2P_OK;
+}
+
+int se2p_ctx_destroy(se2p_ctx* ctx)
+{
+    if (!ctx) return SE2P_OK;
+    // Teardown is best effort: every resource is released even if one call fails.
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < WS_NUM_SLOTS; ++i)
+        if (ctx->ws_ptr[i]) (void)hipFree(ctx->ws_ptr[i]);
+    for (void* p : ctx->user_allocs) (void)hipFree(p);
+    for (auto& r : ctx->prof_pending) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SE2P_OK;
+}
+
+int se2p_slp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                  int Nsrc, int Ntar, double xi, int nside_x, int nside_y, double Lx, double Ly,
+                  double* ur)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, xi));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(ur, Ntar, "ur"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    double *dps, *dpt, *df;
+    void* out;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
+    SE2P_TRY(real_space(ctx, SE2P_SLP, dps, dpt, df, nullptr, Nsrc, Ntar, xi, nside_x, nside_y,
+                        Lx, Ly, (double*)out));
+    return download(ctx, ur, (double*)out, 2 * (size_t)Ntar);
+}
+
+int se2p_dlp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                  const double* n, int Nsrc, int Ntar, double xi, int nside_x, int nside_y,
+                  double Lx, double Ly, double* ur)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, xi));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(ur, Ntar, "ur"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    double *dps, *dpt, *df, *dn;
+    void* out;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
+    SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Nsrc, &dn));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
+    SE2P_TRY(real_space(ctx, SE2P_DLP, dps, dpt, df, dn, Nsrc, Ntar, xi, nside_x, nside_y, Lx,
+                        Ly, (double*)out));
+    return download(ctx, ur, (double*)out, 2 * (size_t)Ntar);
+}
+
+int se2p_slp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                    double eta, const double* f, int Nsrc, int Ntar, int Mx, int My, double Lx,
+                    double Ly, double w, int P, double* uk)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(uk, Ntar, "uk"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    double *dps, *dpt, *df;
+    void* out;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
+    SE2P_TRY(kspace(ctx, SE2P_SLP, dps, dpt, df, nullptr, Nsrc, Ntar, xi, eta, Mx, My, Lx, Ly, w,
+                    P, nullptr, nullptr, (double*)out));
+    return download(ctx, uk, (double*)out, 2 * (size_t)Ntar);
+}
+
+int se2p_dlp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                    double eta, const double* f, const double* n, int Nsrc, int Ntar, int Mx,
+                    int My, double Lx, double Ly, double w, int P, double* uk)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(uk, Ntar, "uk"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    double *dps, *dpt, *df, *dn;
+    void* out;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
+    SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Nsrc, &dn));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
+    SE2P_TRY(kspace(ctx, SE2P_DLP, dps, dpt, df, dn, Nsrc, Ntar, xi, eta, Mx, My, Lx, Ly, w, P,
+                    nullptr, nullptr, (double*)out));
+    return download(ctx, uk, (double*)out, 2 * (size_t)Ntar);
+}
+
+void se2p_options_default(se2p_options* opt)
+{
+    if (!opt) return;
+    opt->P = 24;
+    opt->Nb = 0;
+    opt->tol = 1e-16;
+    opt->verbose = 0;
+}
+
+int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly,
+                    const se2p_options* opt, se2p_params* out)
+{
+    if (!out) return set_error(SE2P_ERR_ARG, "null output");
+    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    if (!(Lx > 0) || !(Ly > 0)) return set_error(SE2P_ERR_ARG, "box lengths must be positive");
+    return params_ref(kind, Nsrc, Ntar, Q, Lx, Ly, opt, out);
+}
+
+int se2p_slp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
+                   const double* ytar, const double* f1, const double* f2, int Nsrc, int Ntar,
+                   double Lx, double Ly, const se2p_options* opt, double* u1, double* u2,
+                   double* ur, double* uk, double* xi)
+{
+    if (ctx) SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return ewald_driver(ctx, SE2P_SLP, xsrc, ysrc, xtar, ytar, nullptr, nullptr, f1, f2, Nsrc,
+                        Ntar, Lx, Ly, opt, u1, u2, ur, uk, xi);
+}
+
+int se2p_dlp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
+                   const double* ytar, const double* n1, const double* n2, const double* f1,
+                   const double* f2, int Nsrc, int Ntar, double Lx, double Ly,
+                   const se2p_options* opt, double* u1, double* u2, double* ur, double* uk,
+                   double* xi)
+{
+    if (ctx) SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return ewald_driver(ctx, SE2P_DLP, xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Nsrc, Ntar, Lx,
+                        Ly, opt, u1, u2, ur, uk, xi);
+}
+
+int se2p_dev_alloc(se2p_ctx* ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return set_error(SE2P_ERR_ARG, "null argument");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    void* p = nullptr;
+    SE2P_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+    ctx->user_allocs.push_back(p);
+    *dptr = p;
+    return SE2P_OK;
+}
+
+int se2p_dev_free(se2p_ctx* ctx, void* dptr)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    for (size_t i = 0; i < ctx->user_allocs.size(); ++i)
+        if (ctx->user_allocs[i] == dptr) {
+            SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            SE2P_HIP_CHECK(hipFree(dptr));
+            ctx->user_allocs.erase(ctx->user_allocs.begin() + i);
+            return SE2P_OK;
+        }
+    return set_error(SE2P_ERR_ARG, "pointer was not allocated by se2p_dev_alloc");
+}
+
+int se2p_memcpy_h2d(se2p_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    SE2P_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SE2P_OK;
+}
+
+int se2p_memcpy_d2h(se2p_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    SE2P_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SE2P_OK;
+}
+
+int se2p_ctx_sync(se2p_ctx* ctx)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SE2P_OK;
+}
+
+int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+                     const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
+                     double Ly, const se2p_params* prm, double* d_u)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    if (!prm) return set_error(SE2P_ERR_ARG, "null parameters");
+    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    SE2P_TRY(check_real(prm->nside_x, prm->nside_y, prm->xi));
+    SE2P_TRY(check_kspace(prm->Mx, prm->My, Lx, Ly, prm->xi, prm->eta, prm->w, prm->P));
+    if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    double* dzm = nullptr;
+    if (kind == SE2P_DLP) {
+        void *part = nullptr, *zm = nullptr;
+        SE2P_TRY(ws_get(ctx, WS_REDUCE, sizeof(double) * 512, &part));
+        SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2, &zm));
+        dzm = (double*)zm;
+        SE2P_TRY(launch_dlp_zero_mode(ctx->stream, d_psrc, d_f, d_n, Nsrc, Lx, Ly, (double*)part,
+                                      dzm));
+    }
+    SE2P_TRY(real_space(ctx, kind, d_psrc, d_ptar, d_f, d_n, Nsrc, Ntar, prm->xi, prm->nside_x,
+                        prm->nside_y, Lx, Ly, d_u));
+    SE2P_TRY(kspace(ctx, kind, d_psrc, d_ptar, d_f, d_n, Nsrc, Ntar, prm->xi, prm->eta, prm->Mx,
+                    prm->My, Lx, Ly, prm->w, prm->P, d_u, dzm, d_u));
+    return SE2P_OK;
+}
+
+int se2p_prof_enable(se2p_ctx* ctx, int on)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    ctx->prof = on != 0;
+    return SE2P_OK;
+}
+
+int se2p_prof_reset(se2p_ctx* ctx)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    SE2P_TRY(prof_flush(ctx));
+    ctx->prof_acc.clear();
+    return SE2P_OK;
+}
+
+int se2p_prof_get(se2p_ctx* ctx, const char* stage, double* total_ms, int* count)
+{
+    if (!ctx || !stage || !total_ms || !count) return set_error(SE2P_ERR_ARG, "null argument");
+    SE2P_TRY(prof_flush(ctx));
+    *total_ms = 0;
+    *count = 0;
+    for (auto& a : ctx->prof_acc)
+        if (a.name == stage) {
+            *total_ms = a.ms;
+            *count = a.count;
+        }
+    return SE2P_OK;
+}
+
+int se2p_assign(se2p_ctx* ctx, const double* p, int n, double Lx, double Ly, int nside_x,
+                int nside_y, int* particle_offsets, int* box_offsets, int* n_in_box)
+{
+    SE2P_TRY(check_common(ctx, n, 0, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, 1.0));
+    SE2P_TRY(need_ptr(p, n, "p"));
+    SE2P_TRY(need_ptr(particle_offsets, n, "particle_offsets"));
+    if (!box_offsets || !n_in_box) return set_error(SE2P_ERR_ARG, "null box arrays");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    const int nbox = nside_x * nside_y;
+    double* dp;
+    void *key, *perm, *start, *count;
+    SE2P_TRY(upload(ctx, WS_PTS_A, p, 2 * (size_t)n, &dp));
+    SE2P_TRY(ws_get(ctx, WS_SRC_KEY, sizeof(int) * (size_t)n, &key));
+    SE2P_TRY(ws_get(ctx, WS_SRC_PERM, sizeof(int) * (size_t)n, &perm));
+    SE2P_TRY(ws_get(ctx, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &start));
+    SE2P_TRY(ws_get(ctx, WS_SRC_COUNT, sizeof(int) * (size_t)nbox, &count));
+    SE2P_TRY(launch_fill_int(ctx->stream, (int*)count, nbox, 0));
+    SE2P_TRY(launch_cell_keys(ctx->stream, dp, n, Lx, Ly, nside_x, nside_y, (int*)key,
+                              (int*)count));
+    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, n, nbox, (int*)perm, (int*)start,
+                         WS_SORT_TMP));
+    if (n) SE2P_HIP_CHECK(hipMemcpyAsync(particle_offsets, perm, sizeof(int) * (size_t)n,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+    SE2P_HIP_CHECK(hipMemcpyAsync(box_offsets, start, sizeof(int) * (size_t)(nbox + 1),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < nbox; ++b) n_in_box[b] = box_offsets[b + 1] - box_offsets[b];
+    return SE2P_OK;
+}
+
+int se2p_find_closest_node(se2p_ctx* ctx, const double* p, int n, double Lx, double Ly,
+                           double h, int P, int* mx, int* my, double* px, double* py)
+{
+    SE2P_TRY(check_common(ctx, n, 0, Lx, Ly));
+    if (!(h > 0)) return set_error(SE2P_ERR_ARG, "h must be positive");
+    SE2P_TRY(need_ptr(p, n, "p"));
+    if (n > 0 && (!mx || !my || !px || !py)) return set_error(SE2P_ERR_ARG, "null output");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (n == 0) return SE2P_OK;
+    double* dp;
+    void* out;
+    SE2P_TRY(upload(ctx, WS_PTS_A, p, 2 * (size_t)n, &dp));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, (2 * sizeof(int) + 2 * sizeof(double)) * (size_t)n, &out));
+    double* dpx = (double*)out;
+    double* dpy = dpx + n;
+    int* dmx = (int*)(dpy + n);
+    int* dmy = dmx + n;
+    SE2P_TRY(launch_closest_nodes(ctx->stream, dp, n, Lx, Ly, h, P, dmx, dmy, dpx, dpy));
+    hipStream_t s = ctx->stream;
+    SE2P_HIP_CHECK(hipMemcpyAsync(mx, dmx, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    SE2P_HIP_CHECK(hipMemcpyAsync(my, dmy, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    SE2P_HIP_CHECK(hipMemcpyAsync(px, dpx, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+    SE2P_HIP_CHECK(hipMemcpyAsync(py, dpy, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+    SE2P_HIP_CHECK(hipStreamSynchronize(s));
+    return SE2P_OK;
+}
+
+int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, double Lx,
+                double Ly, double xi, double w, double eta, int P, int Mx, int My, double* H1,
+                double* H2)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, 0, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    if (!H1 || !H2) return set_error(SE2P_ERR_ARG, "null grid");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    KGeom g;
+    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
+    const int ntiles = g.ntx * g.nty;
+    const size_t G = (size_t)Mx * My;
+    double *dps, *df;
+    void *e1tab, *key, *perm, *start, *count, *node, *off, *part, *Z;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
+    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
+    SE2P_TRY(ws_get(ctx, WS_SPR_KEY, sizeof(int) * (size_t)Nsrc, &key));
+    SE2P_TRY(ws_get(ctx, WS_SPR_PERM, sizeof(int) * (size_t)Nsrc, &perm));
+    SE2P_TRY(ws_get(ctx, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &start));
+    SE2P_TRY(ws_get(ctx, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &count));
+    SE2P_TRY(ws_get(ctx, WS_SPR_NODE, sizeof(int2) * (size_t)Nsrc, &node));
+    SE2P_TRY(ws_get(ctx, WS_SPR_OFF, sizeof(double2) * (size_t)Nsrc, &off));
+    SE2P_TRY(ws_get(ctx, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
+    SE2P_TRY(ws_get(ctx, WS_GRID_A, sizeof(double2) * G, &Z));
+    SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
+    SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
+    SE2P_TRY(launch_tile_keys(s, dps, Nsrc, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
+                              (int*)count, (int2*)node, (double2*)off));
+    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, Nsrc, ntiles, (int*)perm, (int*)start,
+                         WS_SORT_TMP));
+    SE2P_TRY(launch_spread(s, 0, g, (int*)start, (int*)perm, (int2*)node, (double2*)off, df,
+                           nullptr, (double*)e1tab, (double*)part));
+    SE2P_TRY(launch_merge(s, 2, g, (double*)part, (double2*)Z, nullptr));
+    std::vector<double> h(2 * G);
+    SE2P_TRY(download(ctx, h.data(), (double*)Z, 2 * G));
+    for (size_t i = 0; i < G; ++i) {
+        H1[i] = h[2 * i];
+        H2[i] = h[2 * i + 1];
+    }
+    return SE2P_OK;
+}
+
+int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, double Lx,
+                double Ly, double xi, double w, double eta, int P, int Mx, int My, double* out)
+{
+    SE2P_TRY(check_common(ctx, 0, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(out, Ntar, "out"));
+    if (!H) return set_error(SE2P_ERR_ARG, "null grid");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    hipStream_t s = ctx->stream;
+    KGeom g;
+    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
+    const int ntiles = g.ntx * g.nty;
+    const size_t G = (size_t)Mx * My;
+    std::vector<double> wz(2 * G, 0.0);
+    for (size_t i = 0; i < G; ++i) wz[2 * i] = H[i];
+    double *dpt, *dW;
+    void *e1tab, *key, *perm, *start, *count, *node, *off, *u;
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
+    SE2P_TRY(upload(ctx, WS_GRID_C, wz.data(), 2 * G, &dW));
+    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
+    SE2P_TRY(ws_get(ctx, WS_GAT_KEY, sizeof(int) * (size_t)Ntar, &key));
+    SE2P_TRY(ws_get(ctx, WS_GAT_PERM, sizeof(int) * (size_t)Ntar, &perm));
+    SE2P_TRY(ws_get(ctx, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &start));
+    SE2P_TRY(ws_get(ctx, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &count));
+    SE2P_TRY(ws_get(ctx, WS_GAT_NODE, sizeof(int2) * (size_t)Ntar, &node));
+    SE2P_TRY(ws_get(ctx, WS_GAT_OFF, sizeof(double2) * (size_t)Ntar, &off));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &u));
+    SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
+    SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
+    SE2P_TRY(launch_tile_keys(s, dpt, Ntar, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
+                              (int*)count, (int2*)node, (double2*)off));
+    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, Ntar, ntiles, (int*)perm, (int*)start,
+                         WS_SORT_TMP));
+    SE2P_TRY(launch_gather(s, g, (int*)start, (int*)perm, (int2*)node, (double2*)off,
+                           (double2*)dW, (double*)e1tab, nullptr, nullptr, (double*)u));
+    std::vector<double> h(2 * (size_t)Ntar);
+    SE2P_TRY(download(ctx, h.data(), (double*)u, 2 * (size_t)Ntar));
+    for (int i = 0; i < Ntar; ++i) out[i] = h[2 * i];
+    return SE2P_OK;
+}
+
+int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (Mx < 1 || My < 1 || !re || !im) return set_error(SE2P_ERR_ARG, "bad grid");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    FftAxis ax, ay;
+    SE2P_TRY(fft_axis_init(&ax, Mx));
+    SE2P_TRY(fft_axis_init(&ay, My));
+    double2 *twx, *twy;
+    SE2P_TRY(fft_twiddles(ctx, Mx, WS_TWIDDLE_X, &twx));
+    SE2P_TRY(fft_twiddles(ctx, My, WS_TWIDDLE_Y, &twy));
+    const size_t G = (size_t)Mx * My;
+    std::vector<double> z(2 * G);
+    for (size_t i = 0; i < G; ++i) {
+        z[2 * i] = re[i];
+        z[2 * i + 1] = im[i];
+    }
+    double* dz;
+    SE2P_TRY(upload(ctx, WS_GRID_A, z.data(), 2 * G, &dz));
+    SE2P_TRY(fft2d(ctx, (double2*)dz, Mx, My, ax, ay, twx, twy, inverse ? 1 : 0));
+    SE2P_TRY(download(ctx, z.data(), dz, 2 * G));
+    const double sc = inverse ? 1.0 / ((double)Mx * (double)My) : 1.0;
+    for (size_t i = 0; i < G; ++i) {
+        re[i] = z[2 * i] * sc;
+        im[i] = z[2 * i + 1] * sc;
+    }
+    return SE2P_OK;
+}
+
+int se2p_e1(se2p_ctx* ctx, const double* x, int n, int tier, double* out)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (n < 0) return set_error(SE2P_ERR_ARG, "negative count");
+    if (tier != 0 && tier != 1) return set_error(SE2P_ERR_ARG, "tier must be 0 or 1");
+    SE2P_TRY(need_ptr(x, n, "x"));
+    SE2P_TRY(need_ptr(out, n, "out"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (n == 0) return SE2P_OK;
+    double* dx;
+    void* dout;
+    SE2P_TRY(upload(ctx, WS_PTS_A, x, (size_t)n, &dx));
+    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * (size_t)n, &dout));
+    SE2P_TRY(launch_e1(ctx->stream, dx, n, tier, (double*)dout));
+    return download(ctx, out, (double*)dout, (size_t)n);
+}
+
+}  // extern "C"

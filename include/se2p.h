@@ -1,0 +1,193 @@
+/*
+ * se2p -- spectral Ewald for doubly-periodic 2D Stokes sums on AMD Instinct MI355X.
+ *
+ * C ABI of libse2p.so.  Plain pointers and sizes only.  All functions return
+ * SE2P_OK (0) or a negative status; se2p_last_error() gives the text of the
+ * last failure on the calling thread.  No function aborts the process.
+ *
+ * Layouts follow the reference mex interface:
+ *   points, densities f and normals n are 2 x N column-major double arrays,
+ *   i.e. interleaved [x0, y0, x1, y1, ...];
+ *   velocities are returned in the same 2 x N layout.
+ * Host-pointer entry points are synchronous (like a mex call): inputs are only
+ * read, outputs are fully overwritten, and nothing is retained after return.
+ *
+ * Reference paths are relative to the reference repository root
+ * (PeriodicEwald/src/...).
+ */
+#ifndef SE2P_H
+#define SE2P_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SE2P_OK 0
+#define SE2P_ERR_ARG (-1)         /* bad argument, shape or violated precondition */
+#define SE2P_ERR_HIP (-2)         /* HIP runtime failure */
+#define SE2P_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define SE2P_ERR_UNSUPPORTED (-4) /* valid input outside what this build handles */
+#define SE2P_ERR_NODEV (-5)       /* no usable GPU */
+
+#define SE2P_SLP 0 /* Stokeslet, single-layer potential */
+#define SE2P_DLP 1 /* stresslet, double-layer potential */
+
+/* Arithmetic of a plan (only FP64 in this build). */
+#define SE2P_FP64 0
+
+typedef struct se2p_ctx se2p_ctx;
+typedef struct se2p_plan se2p_plan;
+
+/* ------------------------------------------------------------------------ */
+/* Context: one GPU, one HIP stream, a grow-only device workspace.           */
+/* One context per host thread.                                              */
+/* ------------------------------------------------------------------------ */
+int se2p_ctx_create(int device, se2p_ctx** out);
+int se2p_ctx_destroy(se2p_ctx* ctx);
+const char* se2p_last_error(void);
+int se2p_version(void);
+/* Number of visible GPUs (0 when none). */
+int se2p_device_count(int* count);
+
+/* ------------------------------------------------------------------------ */
+/* The four mex kernels, same argument order, sizes made explicit.          */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces mex_stokes_slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_real.cpp:58-83).
+ * ur: 2 x Ntar.  Points must lie in [-Lx/2, Lx/2) x [-Ly/2, Ly/2). */
+int se2p_slp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                  int Nsrc, int Ntar, double xi, int nside_x, int nside_y, double Lx,
+                  double Ly, double* ur);
+
+/* Replaces mex_stokes_slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P)
+ * (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_kspace.cpp:7-45).
+ * Requires Lx/Mx == Ly/My, Mx and My even, P even, P + 2 <= min(Mx, My). */
+int se2p_slp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                    double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                    double Lx, double Ly, double w, int P, double* uk);
+
+/* Replaces mex_stokes_dlp_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_real.cpp:15-52). */
+int se2p_dlp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                  const double* n, int Nsrc, int Ntar, double xi, int nside_x, int nside_y,
+                  double Lx, double Ly, double* ur);
+
+/* Replaces mex_stokes_dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P)
+ * (PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_kspace.cpp:10-56).  Like the
+ * mex, the zero mode is not included. */
+int se2p_dlp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                    double eta, const double* f, const double* n, int Nsrc, int Ntar, int Mx,
+                    int My, double Lx, double Ly, double w, int P, double* uk);
+
+/* ------------------------------------------------------------------------ */
+/* Drivers: StokesSLP_ewald_2p / StokesDLP_ewald_2p restated.                */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int P;      /* Gaussian support nodes per direction minus one, even (default 24) */
+    double Nb;  /* mean points per cell; <= 0 selects min(P round(log2 npts), npts/4) */
+    double tol; /* error tolerance of the analytic estimates (default 1e-16) */
+    int verbose;
+} se2p_options;
+
+typedef struct {
+    double xi, rc, w, eta;
+    int nside_x, nside_y, Mx, My, P, kinf;
+    int converged; /* 0 when a Newton solve hit its iteration cap */
+} se2p_params;
+
+void se2p_options_default(se2p_options* opt);
+
+/* Parameter recipe of StokesSLP_ewald_2p.m:27-37,84-104,147-202 (kind SE2P_SLP)
+ * or StokesDLP_ewald_2p.m:28-38,104-125,171-225 (kind SE2P_DLP).  Q is
+ * sum(f.^2) + 1 over both components of the density. */
+int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly,
+                    const se2p_options* opt, se2p_params* out);
+
+/* StokesSLP_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, ...) ->
+ * [u1, u2, ur, uk, xi] (StokesSLP_ewald_2p.m:1-138).  Points are wrapped into
+ * the box first.  ur, uk (2 x Ntar) and xi may be NULL. */
+int se2p_slp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                   const double* xtar, const double* ytar, const double* f1,
+                   const double* f2, int Nsrc, int Ntar, double Lx, double Ly,
+                   const se2p_options* opt, double* u1, double* u2, double* ur, double* uk,
+                   double* xi);
+
+/* StokesDLP_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, ...)
+ * (StokesDLP_ewald_2p.m:1-163), zero mode Σ(f.n)x_s/(Lx Ly) added to uk. */
+int se2p_dlp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                   const double* xtar, const double* ytar, const double* n1,
+                   const double* n2, const double* f1, const double* f2, int Nsrc, int Ntar,
+                   double Lx, double Ly, const se2p_options* opt, double* u1, double* u2,
+                   double* ur, double* uk, double* xi);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident evaluation.                                               */
+/* ------------------------------------------------------------------------ */
+/* Device buffers owned by the caller, allocated through the context. */
+int se2p_dev_alloc(se2p_ctx* ctx, size_t bytes, void** dptr);
+int se2p_dev_free(se2p_ctx* ctx, void* dptr);
+int se2p_memcpy_h2d(se2p_ctx* ctx, void* dst, const void* src, size_t bytes);
+int se2p_memcpy_d2h(se2p_ctx* ctx, void* dst, const void* src, size_t bytes);
+int se2p_ctx_sync(se2p_ctx* ctx);
+
+/* One complete evaluation on device pointers with explicit parameters: binning,
+ * real space and k space recomputed on every call, as the reference does.
+ * d_n must be given for SE2P_DLP (NULL for SLP).  d_u (2 x Ntar) receives
+ * ur + uk (+ the DLP zero mode).  Stream-ordered on the context's stream; call
+ * se2p_ctx_sync() before reading d_u. */
+int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+                     const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
+                     double Ly, const se2p_params* prm, double* d_u);
+
+/* ------------------------------------------------------------------------ */
+/* Stage-level entry points (host buffers, synchronous): the helpers of      */
+/* PeriodicEwald/src/mex/common/ewald_tools.h, each replaced by the GPU path  */
+/* the mex kernels above use internally.                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces Assign (ewald_tools.cpp:7-80) for one point set: stable counting
+ * sort into nside_x x nside_y cells.  particle_offsets: n; box_offsets:
+ * nside_x*nside_y + 1; n_in_box: nside_x*nside_y.  Bit-identical. */
+int se2p_assign(se2p_ctx* ctx, const double* p, int n, double Lx, double Ly, int nside_x,
+                int nside_y, int* particle_offsets, int* box_offsets, int* n_in_box);
+
+/* Replaces FindClosestNode (ewald_tools.cpp:86-114) for n points; mx, my are
+ * the unwrapped start nodes as in the reference.  Bit-identical. */
+int se2p_find_closest_node(se2p_ctx* ctx, const double* p, int n, double Lx, double Ly,
+                           double h, int P, int* mx, int* my, double* px, double* py);
+
+/* Replaces Spread (ewald_tools.cpp:120-199): H1, H2 are My x Mx column-major. */
+int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, double Lx,
+                double Ly, double xi, double w, double eta, int P, int Mx, int My, double* H1,
+                double* H2);
+
+/* Replaces Gather (ewald_tools.cpp:206-271) for one real grid H (My x Mx);
+ * out[k] (k < Ntar) receives the gathered, scaled value. */
+int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, double Lx,
+                double Ly, double xi, double w, double eta, int P, int Mx, int My, double* out);
+
+/* Replaces MATLAB fft2/ifft2 on a My x Mx column-major complex grid held as
+ * separate real and imaginary arrays (mex_stokes_slp_kspace.cpp:80-81,176-177);
+ * in place; the inverse scales by 1/(Mx My) like ifft2. */
+int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse);
+
+/* The real-space E1 evaluator (mex_stokes_slp_real.cpp:166-205) on n values;
+ * tier 0 = full FP64, 1 = reduced (relative error below 1e-12). */
+int se2p_e1(se2p_ctx* ctx, const double* x, int n, int tier, double* out);
+
+/* ------------------------------------------------------------------------ */
+/* Per-stage timing with HIP events on the context's stream.                 */
+/* ------------------------------------------------------------------------ */
+int se2p_prof_enable(se2p_ctx* ctx, int on);
+int se2p_prof_reset(se2p_ctx* ctx);
+/* Sum of event-measured milliseconds and launch count for a stage name
+ * ("bin", "real", "spread", "merge", "fft", "filter", "gather", ...). */
+int se2p_prof_get(se2p_ctx* ctx, const char* stage, double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

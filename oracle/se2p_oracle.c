@@ -1,0 +1,907 @@
+/*
+ * CPU oracle for the doubly-periodic 2D Stokes spectral Ewald path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker (and the timed CPU
+ * baseline of the benchmark).  The product library never links or calls it.
+ *
+ * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).  Results do
+ * not depend on the OpenMP thread count: every parallel loop writes disjoint
+ * outputs and every floating-point reduction runs in a fixed order.
+ *
+ * Reference paths below are relative to the reference repository root.
+ */
+#include "se2p_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846
+#define ORC_PIL 3.141592653589793238462643383279502884L
+#define ORC_EULER 0.577215664901532860606512090082402431L
+
+/* ------------------------------------------------------------------------ */
+/* a1: Assign, PeriodicEwald/src/mex/common/ewald_tools.cpp:7-80            */
+/* ------------------------------------------------------------------------ */
+static int orc_cell_of(double x, double y, double Lx, double Ly, int nsx, int nsy)
+{
+    /* ewald_tools.cpp:28-34: floor(nside*(x/L+0.5)), clamped to [0, nside-1]. */
+    int bx = (int)floor(nsx * (x / Lx + 0.5));
+    int by = (int)floor(nsy * (y / Ly + 0.5));
+    if (bx < 0) bx = 0;
+    if (bx >= nsx) bx = nsx - 1;
+    if (by < 0) by = 0;
+    if (by >= nsy) by = nsy - 1;
+    return by * nsx + bx;
+}
+
+void orc_assign(const double* p, int n, double Lx, double Ly, int nside_x, int nside_y,
+                int* particle_offsets, int* box_offsets, int* n_in_box)
+{
+    const int nbox = nside_x * nside_y;
+    int* key = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int* cursor = (int*)malloc(sizeof(int) * (nbox + 1));
+    for (int b = 0; b < nbox; ++b) n_in_box[b] = 0;
+    for (int i = 0; i < n; ++i) {
+        key[i] = orc_cell_of(p[2 * i], p[2 * i + 1], Lx, Ly, nside_x, nside_y);
+        n_in_box[key[i]]++;
+    }
+    /* Exclusive prefix (ewald_tools.cpp:52-68), then a stable scatter in input
+     * order (ewald_tools.cpp:71-74). */
+    box_offsets[0] = 0;
+    for (int b = 0; b < nbox; ++b) box_offsets[b + 1] = box_offsets[b] + n_in_box[b];
+    for (int b = 0; b <= nbox; ++b) cursor[b] = box_offsets[b];
+    for (int i = 0; i < n; ++i) particle_offsets[cursor[key[i]]++] = i;
+    free(key);
+    free(cursor);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a7: FindClosestNode, ewald_tools.cpp:86-114                              */
+/* ------------------------------------------------------------------------ */
+void orc_find_closest_node(double x, double y, double Lx, double Ly, double h, int P,
+                           int* mx, int* my, double* px, double* py)
+{
+    const double snap = 1e-13;
+    double ox = x - h * floor(x / h);
+    double oy = y - h * floor(y / h);
+    double hx = Lx / 2.0, hy = Ly / 2.0;
+    int sx = (int)(floor((x + hx) / h) - P / 2);
+    int sy = (int)(floor((y + hy) / h) - P / 2);
+    /* A point within 1e-13 of a node is moved onto it; when that node is the
+     * upper one the start index advances (ewald_tools.cpp:103-113). */
+    if (fabs(ox) < snap || fabs(ox - h) < snap) ox = 0;
+    if (fabs(oy) < snap || fabs(oy - h) < snap) oy = 0;
+    if (ox == 0 && remainder((x + hx) / h - P / 2, 1) < 0) sx++;
+    if (oy == 0 && remainder((y + hy) / h - P / 2, 1) < 0) sy++;
+    *mx = sx;
+    *my = sy;
+    *px = ox;
+    *py = oy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4: E1(x).  The reference evaluates E1 with rational fits inline          */
+/* (mex_stokes_slp_real.cpp:19-48,166-205).  The oracle uses the textbook    */
+/* power series (x <= 1.5) and the even continued fraction (x > 1.5) in long */
+/* double, which is independent of any fitted coefficients.                  */
+/* ------------------------------------------------------------------------ */
+double orc_e1(double xd)
+{
+    long double x = xd;
+    if (!(x > 0)) return INFINITY;
+    if (x <= 1.5L) {
+        /* E1(x) = -gamma - ln x - sum_{k>=1} (-x)^k / (k k!) */
+        long double t = 1.0L, s = 0.0L;
+        for (int k = 1; k < 400; ++k) {
+            t *= -x / k;
+            long double a = t / k;
+            s += a;
+            if (fabsl(a) <= 1e-22L * fabsl(s)) break;
+        }
+        return (double)(-ORC_EULER - logl(x) - s);
+    }
+    /* E1(x) = e^{-x} / (x+1 - 1^2/(x+3 - 2^2/(x+5 - ...))), modified Lentz. */
+    const long double tiny = 1e-4000L;
+    long double b = x + 1.0L;
+    long double c = 1.0L / tiny;
+    long double d = 1.0L / b;
+    long double hcf = d;
+    for (int i = 1; i < 100000; ++i) {
+        long double a = -(long double)i * (long double)i;
+        b += 2.0L;
+        d = 1.0L / (a * d + b);
+        c = b + a / c;
+        long double del = c * d;
+        hcf *= del;
+        if (fabsl(del - 1.0L) < 1e-21L) break;
+    }
+    return (double)(hcf * expl(-x));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Real-space pair kernels                                                   */
+/* ------------------------------------------------------------------------ */
+/* SLP pair (times 4 pi), r = target - source: (1/2)E1(xi^2 r^2) f + e^{-xi^2 r^2}
+ * ((r.f) r / r^2 - f), written as the cancellation-free
+ *   u1 += E/2 f1 + (e/r^2) dy (dx f2 - dy f1),  u2 += E/2 f2 - (e/r^2) dx (dx f2 - dy f1)
+ * (mex_stokes_slp_real.cpp:213-225; stokes_slp_real_ds.m:61-67). */
+static inline void slp_pair(double dx, double dy, double r2, double xi2, double f1, double f2,
+                            double* a1, double* a2)
+{
+    double x = xi2 * r2;
+    double e = exp(-x);
+    double E = orc_e1(x);
+    double c = e / r2;
+    double cr = dx * f2 - dy * f1;
+    *a1 += 0.5 * E * f1 + c * dy * cr;
+    *a2 += 0.5 * E * f2 - c * dx * cr;
+}
+
+/* DLP pair (times 4 pi), r = target - source (mex_stokes_dlp_real.cpp:117-138;
+ * stokes_dlp_real_ds.m:208-220):
+ *   e^{-xi^2 r^2} [ -4 (1 + xi^2 r^2)/r^4 r_i (r.f)(r.n) + 2 xi^2 (f_i (r.n) + n_i (r.f) + r_i (f.n)) ] */
+static inline void dlp_pair(double r1, double r2v, double xi2, double f1, double f2,
+                            double n1, double n2, double* a1, double* a2)
+{
+    double rr = r1 * r1 + r2v * r2v;
+    double e = exp(-xi2 * rr);
+    double rf = r1 * f1 + r2v * f2;
+    double rn = r1 * n1 + r2v * n2;
+    double fn = f1 * n1 + f2 * n2;
+    double b = -4.0 * (1.0 + xi2 * rr) / (rr * rr) * rf * rn;
+    double c = 2.0 * xi2;
+    *a1 += e * (b * r1 + c * (f1 * rn + n1 * rf + fn * r1));
+    *a2 += e * (b * r2v + c * (f2 * rn + n2 * rf + fn * r2v));
+}
+
+/* Neighbour order of the 3x3 stencil (mex_stokes_slp_real.cpp:61-62). */
+static const int nb_dx[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+static const int nb_dy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+typedef struct {
+    int* po_s; int* bo_s; int* nb_s;
+    int* po_t; int* bo_t; int* nb_t;
+} cells_t;
+
+static void cells_build(cells_t* c, const double* psrc, int Ns, const double* ptar, int Nt,
+                        double Lx, double Ly, int nsx, int nsy)
+{
+    int nbox = nsx * nsy;
+    c->po_s = (int*)malloc(sizeof(int) * (Ns > 0 ? Ns : 1));
+    c->bo_s = (int*)malloc(sizeof(int) * (nbox + 1));
+    c->nb_s = (int*)malloc(sizeof(int) * nbox);
+    c->po_t = (int*)malloc(sizeof(int) * (Nt > 0 ? Nt : 1));
+    c->bo_t = (int*)malloc(sizeof(int) * (nbox + 1));
+    c->nb_t = (int*)malloc(sizeof(int) * nbox);
+    orc_assign(psrc, Ns, Lx, Ly, nsx, nsy, c->po_s, c->bo_s, c->nb_s);
+    orc_assign(ptar, Nt, Lx, Ly, nsx, nsy, c->po_t, c->bo_t, c->nb_t);
+}
+
+static void cells_free(cells_t* c)
+{
+    free(c->po_s); free(c->bo_s); free(c->nb_s);
+    free(c->po_t); free(c->bo_t); free(c->nb_t);
+}
+
+/* a3/a5: mex_stokes_slp_real.cpp:58-357.  For each target cell: every source of
+ * the cell itself with no cutoff, coincident points (r^2 < 1e-15) taking the self
+ * term -(1 + gamma/2 + ln xi) f (:124,161-163); then the 8 neighbour cells in the
+ * fixed stencil order with cutoff r^2 < Lx Ly/(nside_x nside_y) (:127,233-332),
+ * each neighbour's sum kept apart and added at the end (:255,323,327).  Output
+ * scaled by 1/(4 pi) in input order (:340-346). */
+void orc_slp_real(const double* psrc, const double* ptar, const double* f, int Ns, int Nt,
+                  double xi, int nside_x, int nside_y, double Lx, double Ly, double* u)
+{
+    cells_t c;
+    cells_build(&c, psrc, Ns, ptar, Nt, Lx, Ly, nside_x, nside_y);
+    const int nbox = nside_x * nside_y;
+    const double xi2 = xi * xi;
+    const double self = -(1.0 + (double)(ORC_EULER / 2) + log(xi));
+    const double cut2 = Lx * Ly / nside_x / nside_y;
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int bx = b % nside_x, by = b / nside_x;
+        for (int it = c.bo_t[b]; it < c.bo_t[b + 1]; ++it) {
+            int t = c.po_t[it];
+            double xt = ptar[2 * t], yt = ptar[2 * t + 1];
+            double a1 = 0, a2 = 0;
+            for (int is = c.bo_s[b]; is < c.bo_s[b + 1]; ++is) {
+                int s = c.po_s[is];
+                double dx = xt - psrc[2 * s], dy = yt - psrc[2 * s + 1];
+                double r2 = dx * dx + dy * dy;
+                if (fabs(r2) < 1e-15) {
+                    a1 += self * f[2 * s];
+                    a2 += self * f[2 * s + 1];
+                } else {
+                    slp_pair(dx, dy, r2, xi2, f[2 * s], f[2 * s + 1], &a1, &a2);
+                }
+            }
+            for (int j = 0; j < 8; ++j) {
+                int qx = bx + nb_dx[j], qy = by + nb_dy[j];
+                int wx = (qx + nside_x) % nside_x, wy = (qy + nside_y) % nside_y;
+                int sb = wy * nside_x + wx;
+                double ox = (Lx * (qx - wx)) / nside_x;
+                double oy = (Ly * (qy - wy)) / nside_y;
+                double t1 = 0, t2 = 0;
+                for (int is = c.bo_s[sb]; is < c.bo_s[sb + 1]; ++is) {
+                    int s = c.po_s[is];
+                    double dx = xt - (psrc[2 * s] + ox), dy = yt - (psrc[2 * s + 1] + oy);
+                    double r2 = dx * dx + dy * dy;
+                    if (r2 < cut2) slp_pair(dx, dy, r2, xi2, f[2 * s], f[2 * s + 1], &t1, &t2);
+                }
+                a1 += t1;
+                a2 += t2;
+            }
+            u[2 * t] = a1 / (4 * ORC_PI);
+            u[2 * t + 1] = a2 / (4 * ORC_PI);
+        }
+    }
+    cells_free(&c);
+}
+
+/* a6: mex_stokes_dlp_real.cpp:15-211.  Same cells and stencil; coincident
+ * points (r^2 == 0) are skipped (:122); output scaled by 1/(4 pi) (:197-200). */
+void orc_dlp_real(const double* psrc, const double* ptar, const double* f, const double* n,
+                  int Ns, int Nt, double xi, int nside_x, int nside_y, double Lx, double Ly,
+                  double* u)
+{
+    cells_t c;
+    cells_build(&c, psrc, Ns, ptar, Nt, Lx, Ly, nside_x, nside_y);
+    const int nbox = nside_x * nside_y;
+    const double xi2 = xi * xi;
+    const double cut2 = Lx * Ly / nside_x / nside_y;
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int bx = b % nside_x, by = b / nside_x;
+        for (int it = c.bo_t[b]; it < c.bo_t[b + 1]; ++it) {
+            int t = c.po_t[it];
+            double xt = ptar[2 * t], yt = ptar[2 * t + 1];
+            double a1 = 0, a2 = 0;
+            for (int is = c.bo_s[b]; is < c.bo_s[b + 1]; ++is) {
+                int s = c.po_s[is];
+                double r1 = xt - psrc[2 * s], r2 = yt - psrc[2 * s + 1];
+                if (r1 * r1 + r2 * r2 == 0) continue;
+                dlp_pair(r1, r2, xi2, f[2 * s], f[2 * s + 1], n[2 * s], n[2 * s + 1], &a1, &a2);
+            }
+            for (int j = 0; j < 8; ++j) {
+                int qx = bx + nb_dx[j], qy = by + nb_dy[j];
+                int wx = (qx + nside_x) % nside_x, wy = (qy + nside_y) % nside_y;
+                int sb = wy * nside_x + wx;
+                double ox = (Lx * (qx - wx)) / nside_x;
+                double oy = (Ly * (qy - wy)) / nside_y;
+                for (int is = c.bo_s[sb]; is < c.bo_s[sb + 1]; ++is) {
+                    int s = c.po_s[is];
+                    double r1 = xt - (psrc[2 * s] + ox), r2 = yt - (psrc[2 * s + 1] + oy);
+                    if (r1 * r1 + r2 * r2 < cut2)
+                        dlp_pair(r1, r2, xi2, f[2 * s], f[2 * s + 1], n[2 * s], n[2 * s + 1],
+                                 &a1, &a2);
+                }
+            }
+            u[2 * t] = a1 / (4 * ORC_PI);
+            u[2 * t + 1] = a2 / (4 * ORC_PI);
+        }
+    }
+    cells_free(&c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a8: Spread, ewald_tools.cpp:120-199                                       */
+/* ------------------------------------------------------------------------ */
+/* The Gaussian exp(-2 xi^2 |x - x_node|^2 / eta) on the (P+1)^2 nodes starting
+ * at the node returned by find_closest_node is a product wx[i] wy[j].  Like the
+ * reference, the 1D factors come from the fast-Gaussian-gridding recurrence
+ *   wx[i] = exp(t (px^2 + P h px)) e3x^i e1[i],  e3x = exp(-2 t h px),
+ *   e1[i] = exp(t h^2 (i - P/2)^2),  t = -2 xi^2 / eta
+ * (ewald_tools.cpp:125-127,155-190). */
+static void fgg_weights(double px, double py, double h, double t, double w, int P,
+                        const double* e1, double* wx, double* wy)
+{
+    double ex = exp(t * (px * px + 2 * w * px));
+    double ey = exp(t * (py * py + 2 * w * py));
+    double e3x = exp(-2 * t * h * px);
+    double e3y = exp(-2 * t * h * py);
+    for (int i = 0; i <= P; ++i) {
+        wx[i] = ex * e1[i];
+        wy[i] = ey * e1[i];
+        ex *= e3x;
+        ey *= e3y;
+    }
+}
+
+static void fgg_table(double xi, double eta, double h, int P, double* e1)
+{
+    double t = -2 * xi * xi / eta * h * h;
+    for (int j = -P / 2; j <= P / 2; ++j) e1[j + P / 2] = exp(t * j * j);
+}
+
+void orc_spread(const double* psrc, const double* f, int Ns, double Lx, double Ly,
+                double xi, double w, double eta, int P, int Mx, int My,
+                double* H1, double* H2)
+{
+    const double h = Lx / Mx;
+    const double t = -2 * xi * xi / eta;
+    double* e1 = (double*)malloc(sizeof(double) * (P + 1));
+    fgg_table(xi, eta, h, P, e1);
+    memset(H1, 0, sizeof(double) * (size_t)Mx * My);
+    memset(H2, 0, sizeof(double) * (size_t)Mx * My);
+
+    /* Every thread owns a contiguous range of grid columns and visits all
+     * sources in input order, so each node accumulates in the same order as a
+     * serial spread whatever the thread count (the reference serialises
+     * columns with locks instead, ewald_tools.cpp:133-138,167,188). */
+#pragma omp parallel
+    {
+        int nth = 1, tid = 0;
+#ifdef _OPENMP
+        nth = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        int c0 = (int)((long)Mx * tid / nth), c1 = (int)((long)Mx * (tid + 1) / nth);
+        double wx[64], wy[64];
+        for (int k = 0; k < Ns; ++k) {
+            int mx, my;
+            double px, py;
+            orc_find_closest_node(psrc[2 * k], psrc[2 * k + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+            /* Does the support [mx, mx+P] (mod Mx) meet [c0, c1)? */
+            int hit = 0;
+            for (int i = 0; i <= P && !hit; ++i) {
+                int cx = ((i + mx) % Mx + Mx) % Mx;
+                hit = (cx >= c0 && cx < c1);
+            }
+            if (!hit) continue;
+            fgg_weights(px, py, h, t, w, P, e1, wx, wy);
+            for (int i = 0; i <= P; ++i) {
+                int cx = ((i + mx) % Mx + Mx) % Mx;
+                if (cx < c0 || cx >= c1) continue;
+                double a = wx[i];
+                for (int j = 0; j <= P; ++j) {
+                    int cy = ((j + my) % My + My) % My;
+                    double g = a * wy[j];
+                    H1[(size_t)cx * My + cy] += g * f[2 * k];
+                    H2[(size_t)cx * My + cy] += g * f[2 * k + 1];
+                }
+            }
+        }
+    }
+    free(e1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a12: Gather, ewald_tools.cpp:206-271                                      */
+/* ------------------------------------------------------------------------ */
+void orc_gather(const double* H, int ncomp, int comp, const double* ptar, int Nt,
+                double Lx, double Ly, double xi, double w, double eta, int P, int Mx,
+                int My, double* out)
+{
+    const double h = Lx / Mx;
+    const double t = -2 * xi * xi / eta;
+    double* e1 = (double*)malloc(sizeof(double) * (P + 1));
+    fgg_table(xi, eta, h, P, e1);
+    /* (4 xi^2/eta)^2 h^2 / pi / (4 pi) (ewald_tools.cpp:267-269). */
+    double s = 4 * xi * xi / eta;
+    s = s * s * h * h / ORC_PI / (4 * ORC_PI);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < Nt; ++k) {
+        int mx, my;
+        double px, py, wx[64], wy[64];
+        orc_find_closest_node(ptar[2 * k], ptar[2 * k + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+        fgg_weights(px, py, h, t, w, P, e1, wx, wy);
+        double acc = 0;
+        for (int i = 0; i <= P; ++i) {
+            int cx = ((i + mx) % Mx + Mx) % Mx;
+            double a = wx[i];
+            for (int j = 0; j <= P; ++j) {
+                int cy = ((j + my) % My + My) % My;
+                acc += a * wy[j] * H[(size_t)cx * My + cy];
+            }
+        }
+        out[ncomp * k + comp] = acc * s;
+    }
+    free(e1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a9: 2D DFT.  The reference calls MATLAB's fft2/ifft2 through mexCallMATLAB */
+/* (mex_stokes_slp_kspace.cpp:80-81,176-177).  The oracle has its own        */
+/* recursive mixed-radix DFT for any length: split off the smallest prime    */
+/* factor p, transform the p decimated subsequences, combine with O(p) work  */
+/* per output.  Twiddles are rounded from long double.                       */
+/* ------------------------------------------------------------------------ */
+static int smallest_factor(int n)
+{
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* x: n complex values at stride s (interleaved re,im); y: output, contiguous;
+ * tw: table of exp(sign 2 pi i k / N) for k < N, with N / n = step. */
+static void dft_rec(const double* x, int n, int s, double* y, const double* tw, int step,
+                    double* scratch)
+{
+    if (n == 1) {
+        y[0] = x[0];
+        y[1] = x[1];
+        return;
+    }
+    int p = smallest_factor(n);
+    int m = n / p;
+    /* Sub-transforms of x[r + p j], j < m, land in y[r*m .. r*m+m). */
+    for (int r = 0; r < p; ++r)
+        dft_rec(x + 2 * (size_t)r * s, m, s * p, y + 2 * (size_t)r * m, tw, step * p,
+                scratch + 2 * (size_t)n);
+    /* X[k + m q] = sum_r W_n^{r (k + m q)} Y_r[k] */
+    const int N = n * step;
+    for (int k = 0; k < m; ++k) {
+        for (int q = 0; q < p; ++q) {
+            int kk = k + m * q;
+            double re = 0, im = 0;
+            for (int r = 0; r < p; ++r) {
+                long e = ((long)r * kk) % n;
+                const double* w = tw + 2 * (size_t)((e * step) % N);
+                double yr = y[2 * ((size_t)r * m + k)], yi = y[2 * ((size_t)r * m + k) + 1];
+                re += yr * w[0] - yi * w[1];
+                im += yr * w[1] + yi * w[0];
+            }
+            scratch[2 * kk] = re;
+            scratch[2 * kk + 1] = im;
+        }
+    }
+    memcpy(y, scratch, sizeof(double) * 2 * (size_t)n);
+}
+
+static double* twiddles(int N, int sign)
+{
+    double* tw = (double*)malloc(sizeof(double) * 2 * (size_t)N);
+    for (int k = 0; k < N; ++k) {
+        long double a = 2.0L * ORC_PIL * (long double)k / (long double)N;
+        tw[2 * k] = (double)cosl(a);
+        tw[2 * k + 1] = (double)(sign * sinl(a));
+    }
+    return tw;
+}
+
+static void dft_lines(double* re, double* im, int len, int count, long stride_elem,
+                      long line_step, int inverse)
+{
+    double* tw = twiddles(len, inverse ? 1 : -1);
+    const double sc = inverse ? 1.0 / len : 1.0;
+#pragma omp parallel
+    {
+        double* in = (double*)malloc(sizeof(double) * 2 * (size_t)len);
+        double* out = (double*)malloc(sizeof(double) * 2 * (size_t)len);
+        /* recursion scratch: 2n + 2n/p + ... < 4n complex */
+        double* scr = (double*)malloc(sizeof(double) * 8 * (size_t)len + 16);
+#pragma omp for schedule(static)
+        for (int c = 0; c < count; ++c) {
+            for (int i = 0; i < len; ++i) {
+                size_t o = (size_t)(c * line_step + i * stride_elem);
+                in[2 * i] = re[o];
+                in[2 * i + 1] = im[o];
+            }
+            dft_rec(in, len, 1, out, tw, 1, scr);
+            for (int i = 0; i < len; ++i) {
+                size_t o = (size_t)(c * line_step + i * stride_elem);
+                re[o] = out[2 * i] * sc;
+                im[o] = out[2 * i + 1] * sc;
+            }
+        }
+        free(in);
+        free(out);
+        free(scr);
+    }
+    free(tw);
+}
+
+void orc_fft2(double* re, double* im, int My, int Mx, int inverse)
+{
+    /* columns: contiguous, length My; rows: stride My, length Mx */
+    dft_lines(re, im, My, Mx, 1, My, inverse);
+    dft_lines(re, im, Mx, My, My, 1, inverse);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a10/a11: k-space filters                                                  */
+/* ------------------------------------------------------------------------ */
+/* Wavenumber of grid index j on an M-point axis of length L: indices up to
+ * M/2 are non-negative (mex_stokes_slp_kspace.cpp:115-128,141-150). */
+static inline double wavenumber(int j, int M, double L)
+{
+    return 2.0 * ORC_PI / L * (j <= M / 2 ? j : j - M);
+}
+
+/* SLP: H <- (K^2 q - k (k.q)) (1/K^4 + 1/(4 xi^2 K^2)) exp(-(1-eta) K^2/(4 xi^2)),
+ * k = 0 set to zero (mex_stokes_slp_kspace.cpp:110-169). */
+void orc_slp_filter(double* h1re, double* h1im, double* h2re, double* h2im, int Mx, int My,
+                    double Lx, double Ly, double xi, double eta)
+{
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < Mx; ++ix) {
+        double k1 = wavenumber(ix, Mx, Lx);
+        for (int iy = 0; iy < My; ++iy) {
+            size_t o = (size_t)ix * My + iy;
+            if (ix == 0 && iy == 0) {
+                h1re[o] = h1im[o] = h2re[o] = h2im[o] = 0;
+                continue;
+            }
+            double k2 = wavenumber(iy, My, Ly);
+            double K2 = k1 * k1 + k2 * k2;
+            double g = (1.0 / (K2 * K2) + a / K2) * exp(-(1 - eta) * a * K2);
+            double dr = k1 * h1re[o] + k2 * h2re[o];
+            double di = k1 * h1im[o] + k2 * h2im[o];
+            double q1r = h1re[o], q1i = h1im[o], q2r = h2re[o], q2i = h2im[o];
+            h1re[o] = (K2 * q1r - k1 * dr) * g;
+            h1im[o] = (K2 * q1i - k1 * di) * g;
+            h2re[o] = (K2 * q2r - k2 * dr) * g;
+            h2im[o] = (K2 * q2i - k2 * di) * g;
+        }
+    }
+}
+
+/* DLP: with Q the 2x2 tensor of spread f_i n_j and e = exp(-(1-eta)K^2/(4xi^2))
+ * (1 + K^2/(4 xi^2))/K^2, the two outputs are
+ *   G_i = i [ 2 k_i Q_ii + k_j (Q_12 + Q_21) + k_i (Q_11 + Q_22) - 2 k_i (k.Q.k)/K^2 ] e,
+ * j != i, k = 0 set to zero (mex_stokes_dlp_kspace.cpp:151-232).  Inputs follow
+ * the reference's spreading: h1 = f1 n1, h2 = f2 n1, h3 = f1 n2, h4 = f2 n2
+ * (:79-92); only Q_12 + Q_21 = h2 + h3 enters. */
+void orc_dlp_filter(double* h1re, double* h1im, double* h2re, double* h2im,
+                    const double* h3re, const double* h3im, const double* h4re,
+                    const double* h4im, int Mx, int My, double Lx, double Ly, double xi,
+                    double eta)
+{
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < Mx; ++ix) {
+        double k1 = wavenumber(ix, Mx, Lx);
+        for (int iy = 0; iy < My; ++iy) {
+            size_t o = (size_t)ix * My + iy;
+            if (ix == 0 && iy == 0) {
+                h1re[o] = h1im[o] = h2re[o] = h2im[o] = 0;
+                continue;
+            }
+            double k2 = wavenumber(iy, My, Ly);
+            double K2 = k1 * k1 + k2 * k2;
+            double e = exp(-(1 - eta) * a * K2) * (1 + a * K2) / K2;
+            double q11[2] = {h1re[o], h1im[o]}, q22[2] = {h4re[o], h4im[o]};
+            double qs[2] = {h2re[o] + h3re[o], h2im[o] + h3im[o]};
+            double v[2][2];
+            for (int c = 0; c < 2; ++c) {
+                double tr = q11[c] + q22[c];
+                double kqk = k1 * k1 * q11[c] + k1 * k2 * qs[c] + k2 * k2 * q22[c];
+                double m = kqk / K2;
+                v[0][c] = 2 * k1 * q11[c] + k2 * qs[c] + k1 * tr - 2 * k1 * m;
+                v[1][c] = 2 * k2 * q22[c] + k1 * qs[c] + k2 * tr - 2 * k2 * m;
+            }
+            /* multiply by i e */
+            h1re[o] = -v[0][1] * e;
+            h1im[o] = v[0][0] * e;
+            h2re[o] = -v[1][1] * e;
+            h2im[o] = v[1][0] * e;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole k-space parts                                                       */
+/* ------------------------------------------------------------------------ */
+/* mex_stokes_slp_kspace.cpp:7-215: spread (f1, f2), forward DFT of each real
+ * grid, filter, inverse DFT, real parts gathered into components 1 and 2. */
+void orc_slp_kspace(const double* psrc, const double* ptar, double xi, double eta,
+                    const double* f, int Ns, int Nt, int Mx, int My, double Lx, double Ly,
+                    double w, int P, double* uk)
+{
+    size_t G = (size_t)Mx * My;
+    double* b = (double*)calloc(4 * G, sizeof(double));
+    double *h1r = b, *h1i = b + G, *h2r = b + 2 * G, *h2i = b + 3 * G;
+    orc_spread(psrc, f, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h1r, h2r);
+    orc_fft2(h1r, h1i, My, Mx, 0);
+    orc_fft2(h2r, h2i, My, Mx, 0);
+    orc_slp_filter(h1r, h1i, h2r, h2i, Mx, My, Lx, Ly, xi, eta);
+    orc_fft2(h1r, h1i, My, Mx, 1);
+    orc_fft2(h2r, h2i, My, Mx, 1);
+    orc_gather(h1r, 2, 0, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, uk);
+    orc_gather(h2r, 2, 1, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, uk);
+    free(b);
+}
+
+/* mex_stokes_dlp_kspace.cpp:10-282: spread v1 = (f1 n1, f2 n1) and
+ * v2 = (f1 n2, f2 n2), four forward DFTs, filter, two inverse DFTs, gather.
+ * The zero mode is not added here (the driver adds it, StokesDLP_ewald_2p.m:149-151). */
+void orc_dlp_kspace(const double* psrc, const double* ptar, double xi, double eta,
+                    const double* f, const double* n, int Ns, int Nt, int Mx, int My,
+                    double Lx, double Ly, double w, int P, double* uk)
+{
+    size_t G = (size_t)Mx * My;
+    double* b = (double*)calloc(8 * G, sizeof(double));
+    double* v1 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+    double* v2 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+    for (int i = 0; i < Ns; ++i) {
+        v1[2 * i] = f[2 * i] * n[2 * i];
+        v1[2 * i + 1] = f[2 * i + 1] * n[2 * i];
+        v2[2 * i] = f[2 * i] * n[2 * i + 1];
+        v2[2 * i + 1] = f[2 * i + 1] * n[2 * i + 1];
+    }
+    double* h[8];
+    for (int j = 0; j < 8; ++j) h[j] = b + j * G;
+    orc_spread(psrc, v1, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h[0], h[2]);
+    orc_spread(psrc, v2, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h[4], h[6]);
+    for (int j = 0; j < 4; ++j) orc_fft2(h[2 * j], h[2 * j + 1], My, Mx, 0);
+    orc_dlp_filter(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], Mx, My, Lx, Ly, xi, eta);
+    orc_fft2(h[0], h[1], My, Mx, 1);
+    orc_fft2(h[2], h[3], My, Mx, 1);
+    orc_gather(h[0], 2, 0, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, uk);
+    orc_gather(h[2], 2, 1, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, uk);
+    free(v1);
+    free(v2);
+    free(b);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Direct sums (tests/direct_sums/matlab_sums/stokes_*_ds.m files)           */
+/* ------------------------------------------------------------------------ */
+/* stokes_slp_real_ds.m:21-69: all 3x3 images, no cutoff, self term when r < 1e-13. */
+void orc_slp_real_ds(const double* psrc, const double* ptar, const double* f, int Ns,
+                     int Nt, double Lx, double Ly, double xi, double* u)
+{
+    const double xi2 = xi * xi;
+    const double self = -((double)(ORC_EULER / 2) + log(xi) + 1.0);
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double a1 = 0, a2 = 0;
+        for (int s = 0; s < Ns; ++s) {
+            for (int jx = -1; jx <= 1; ++jx) {
+                for (int jy = -1; jy <= 1; ++jy) {
+                    double dx = ptar[2 * m] - (psrc[2 * s] + jx * Lx);
+                    double dy = ptar[2 * m + 1] - (psrc[2 * s + 1] + jy * Ly);
+                    double r2 = dx * dx + dy * dy;
+                    if (sqrt(r2) < 1e-13) {
+                        a1 += self * f[2 * s];
+                        a2 += self * f[2 * s + 1];
+                    } else {
+                        slp_pair(dx, dy, r2, xi2, f[2 * s], f[2 * s + 1], &a1, &a2);
+                    }
+                }
+            }
+        }
+        u[2 * m] = a1 / (4 * ORC_PI);
+        u[2 * m + 1] = a2 / (4 * ORC_PI);
+    }
+}
+
+/* stokes_dlp_real_ds.m:168-221: all 3x3 images, coincident images skipped. */
+void orc_dlp_real_ds(const double* psrc, const double* ptar, const double* f,
+                     const double* n, int Ns, int Nt, double Lx, double Ly, double xi,
+                     double* u)
+{
+    const double xi2 = xi * xi;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double a1 = 0, a2 = 0;
+        for (int s = 0; s < Ns; ++s) {
+            for (int jx = -1; jx <= 1; ++jx) {
+                for (int jy = -1; jy <= 1; ++jy) {
+                    double r1 = ptar[2 * m] - (psrc[2 * s] + jx * Lx);
+                    double r2 = ptar[2 * m + 1] - (psrc[2 * s + 1] + jy * Ly);
+                    if (sqrt(r1 * r1 + r2 * r2) < 1e-13) continue;
+                    dlp_pair(r1, r2, xi2, f[2 * s], f[2 * s + 1], n[2 * s], n[2 * s + 1], &a1, &a2);
+                }
+            }
+        }
+        u[2 * m] = a1 / (4 * ORC_PI);
+        u[2 * m + 1] = a2 / (4 * ORC_PI);
+    }
+}
+
+/* Fourier sums over modes (-kinf, kinf]^2 \ {0}, written through structure
+ * factors S(k) = sum_s v_s e^{-i k.x_s} so the cost is O(modes (Ns + Nt))
+ * instead of O(modes Ns Nt); the sum is the same as stokes_slp_kspace_ds.m:90-134
+ * (the SLP filter is even in k, so e^{-ik.r} and e^{ik.r} agree) and
+ * stokes_dlp_kspace_ds.m:244-306 (e^{+ik.(x_t - x_s)}). */
+static void kspace_ds(int kind, const double* psrc, const double* ptar, const double* f,
+                      const double* n, int Ns, int Nt, double Lx, double Ly, double xi,
+                      int kinf, double* u)
+{
+    const int K = 2 * kinf;
+    const size_t nm = (size_t)K * K;
+    double* S = (double*)calloc(nm * 8, sizeof(double)); /* up to 4 complex per mode */
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)nm; ++q) {
+        int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+        double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+        double* s = S + 8 * q;
+        for (int i = 0; i < Ns; ++i) {
+            double ph = -(k1 * psrc[2 * i] + k2 * psrc[2 * i + 1]);
+            double c = cos(ph), sn = sin(ph);
+            if (kind == 0) {
+                s[0] += f[2 * i] * c; s[1] += f[2 * i] * sn;
+                s[2] += f[2 * i + 1] * c; s[3] += f[2 * i + 1] * sn;
+            } else {
+                /* DLP per-source vector: f (k.n) + n (k.f) + k (f.n) - 2 k (k.n)(k.f)/K^2 */
+                double K2 = k1 * k1 + k2 * k2;
+                if (K2 == 0) continue;
+                double kn = k1 * n[2 * i] + k2 * n[2 * i + 1];
+                double kf = k1 * f[2 * i] + k2 * f[2 * i + 1];
+                double fn = f[2 * i] * n[2 * i] + f[2 * i + 1] * n[2 * i + 1];
+                double v1 = f[2 * i] * kn + n[2 * i] * kf + k1 * fn - 2 * k1 * kn * kf / K2;
+                double v2 = f[2 * i + 1] * kn + n[2 * i + 1] * kf + k2 * fn - 2 * k2 * kn * kf / K2;
+                s[0] += v1 * c; s[1] += v1 * sn;
+                s[2] += v2 * c; s[3] += v2 * sn;
+            }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double a1 = 0, a2 = 0;
+        for (size_t q = 0; q < nm; ++q) {
+            int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+            if (j1 == 0 && j2 == 0) continue;
+            double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+            double K2 = k1 * k1 + k2 * k2;
+            double g = (1 + a * K2) * exp(-a * K2) / K2;
+            double ph = k1 * ptar[2 * m] + k2 * ptar[2 * m + 1];
+            double c = cos(ph), sn = sin(ph);
+            const double* s = S + 8 * q;
+            if (kind == 0) {
+                /* (f - k (k.f)/K^2) g, real part of times e^{ik.x_t} */
+                double kr = (k1 * s[0] + k2 * s[2]) / K2, ki = (k1 * s[1] + k2 * s[3]) / K2;
+                double v1r = s[0] - k1 * kr, v1i = s[1] - k1 * ki;
+                double v2r = s[2] - k2 * kr, v2i = s[3] - k2 * ki;
+                a1 += g * (v1r * c - v1i * sn);
+                a2 += g * (v2r * c - v2i * sn);
+            } else {
+                /* i v g e^{ik.x_t}: real part is -(v_r sin + v_i cos) g */
+                a1 += -g * (s[0] * sn + s[1] * c);
+                a2 += -g * (s[2] * sn + s[3] * c);
+            }
+        }
+        u[2 * m] = a1 / (Lx * Ly);
+        u[2 * m + 1] = a2 / (Lx * Ly);
+    }
+    free(S);
+}
+
+void orc_slp_kspace_ds(const double* psrc, const double* ptar, const double* f, int Ns,
+                       int Nt, double Lx, double Ly, double xi, int kinf, double* u)
+{
+    kspace_ds(0, psrc, ptar, f, NULL, Ns, Nt, Lx, Ly, xi, kinf, u);
+}
+
+void orc_dlp_kspace_ds(const double* psrc, const double* ptar, const double* f,
+                       const double* n, int Ns, int Nt, double Lx, double Ly, double xi,
+                       int kinf, double* u)
+{
+    kspace_ds(1, psrc, ptar, f, n, Ns, Nt, Lx, Ly, xi, kinf, u);
+    double z[2];
+    orc_dlp_zero_mode(psrc, f, n, Ns, Lx, Ly, z);
+    for (int m = 0; m < Nt; ++m) {
+        u[2 * m] += z[0];
+        u[2 * m + 1] += z[1];
+    }
+}
+
+void orc_dlp_zero_mode(const double* psrc, const double* f, const double* n, int Ns,
+                       double Lx, double Ly, double out[2])
+{
+    double sx = 0, sy = 0;
+    for (int i = 0; i < Ns; ++i) {
+        double fn = f[2 * i] * n[2 * i] + f[2 * i + 1] * n[2 * i + 1];
+        sx += fn * psrc[2 * i];
+        sy += fn * psrc[2 * i + 1];
+    }
+    out[0] = sx / (Lx * Ly);
+    out[1] = sy / (Lx * Ly);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a14: parameter recipe                                                     */
+/* ------------------------------------------------------------------------ */
+/* MATLAB rat(): continued-fraction expansion of X until |X - N/D| < tol with
+ * tol = 1e-6 |X| (MATLAB's documented default). */
+void orc_rat(double X, int* N, int* D)
+{
+    double tol = 1e-6 * fabs(X);
+    double y = X;
+    double lastn = 1, lastd = 0;
+    double nn = round(y), dd = 1;
+    double frac = y - nn;
+    while (fabs(y - nn / dd) >= tol) {
+        double flip = 1 / frac;
+        double step = round(flip);
+        frac = flip - step;
+        double sn = nn, sd = dd;
+        nn = nn * step + lastn;
+        dd = dd * step + lastd;
+        lastn = sn;
+        lastd = sd;
+    }
+    *N = (int)(nn * (dd < 0 ? -1 : 1));
+    *D = (int)fabs(dd);
+}
+
+/* Newton iterations of StokesSLP_ewald_2p.m:147-202 and StokesDLP_ewald_2p.m:171-225:
+ * step limited to 0.9 x, at most 101 steps, stop when |f| <= tol. */
+static double newton(int kind, int which, double x, double Q, double Lx, double Ly, double rc,
+                     double xi, double tol, int* ok)
+{
+    *ok = 1;
+    for (int it = 0;; ++it) {
+        double fv, fp;
+        if (which == 0) { /* xi from rc */
+            if (kind == 0) {
+                fv = sqrt(Q * ORC_PI / (4 * Lx * Ly * x)) * exp(-x * x * rc * rc) - tol;
+                fp = -sqrt(Q * ORC_PI / (4 * Lx * Ly)) * exp(-x * x * rc * rc) *
+                     (0.5 * pow(x, -1.5) + 2 * rc * rc * x * sqrt(1 / x));
+            } else {
+                fv = exp(-x * x * rc * rc) * x * rc * sqrt(2 * ORC_PI * Q / (Lx * Ly)) - tol;
+                fp = (1 - 2 * x * x * rc * rc) * exp(-x * x * rc * rc) *
+                     sqrt(2 * ORC_PI * Q / (Lx * Ly)) * rc;
+            }
+        } else { /* k_inf from xi */
+            double Lm = Lx > Ly ? Lx : Ly;
+            double g = exp(-x * x / (4 * xi * xi));
+            if (kind == 0) {
+                double c = 4 * Q * ORC_PI * Lm / (pow(Lx, 3) * pow(Ly, 3));
+                fv = sqrt(c / x) * g - tol;
+                fp = -sqrt(c) * g * (0.5 * pow(x, -1.5) + sqrt(1 / x) * 2 * x / (4 * xi * xi));
+            } else {
+                /* StokesDLP_ewald_2p.m:177-179, with MATLAB's left-to-right precedence
+                 * (.../Lx^3*Ly^3 multiplies by Ly^3). */
+                double c = 8 * ORC_PI * Q * Lm / pow(Lx, 3) * pow(Ly, 3);
+                fv = sqrt(x * c) * g - tol;
+                fp = sqrt(c) * g * (0.5 * pow(x, -0.5) - 2 * pow(x, 1.5) / (4 * xi * xi));
+            }
+        }
+        if (!(fabs(fv) > tol)) break;
+        if (it > 100) {
+            *ok = 0;
+            break;
+        }
+        double d = fv / fp;
+        if (!(d < 0.9 * x)) d = 0.9 * x;
+        x = x - d;
+    }
+    return x;
+}
+
+int orc_params(int kind, int npts, double Q, double Lx, double Ly, int P, double Nb,
+               double tol, double* out, int* iout)
+{
+    int A, B, ok1, ok2;
+    orc_rat(Lx / Ly, &A, &B);
+    if (A <= 0 || B <= 0) return -1;
+    double a = ceil(sqrt(npts / (Nb * A * B)));
+    double m = 0.95 * sqrt(ORC_PI * P);
+    int nsx = (int)(a * A), nsy = (int)(a * B);
+    double rc = Lx / nsx;
+    double xi = newton(kind, 0, kind == 0 ? 4 / rc : 1 / rc, Q, Lx, Ly, rc, 0, tol, &ok1);
+    /* Both drivers pass Lx for Ly here (StokesSLP_ewald_2p.m:96). */
+    double k0 = round(5 * xi);
+    double kinf = newton(kind, 1, k0, Q, Lx, Lx, rc, xi, tol, &ok2);
+    int kx = (int)round(Lx * kinf / (2 * ORC_PI));
+    int M0 = 2 * kx < 10000 ? 2 * kx : 10000;
+    int Mx = A * M0, My = B * M0;
+    double w = P * Lx / Mx / 2;
+    double eta = pow(2 * xi * w / m, 2);
+    out[0] = xi;
+    out[1] = rc;
+    out[2] = w;
+    out[3] = eta;
+    iout[0] = nsx;
+    iout[1] = nsy;
+    iout[2] = Mx;
+    iout[3] = My;
+    iout[4] = kx;
+    iout[5] = A;
+    iout[6] = B;
+    return (ok1 && ok2) ? 0 : 1;
+}
