@@ -96,7 +96,7 @@ def test_spread_matches_oracle(gpu_ctx, oracle, N, M, P, L):
     assert _rel(H1.T, R1) < RTOL and _rel(H2.T, R2) < RTOL
 
 
-@pytest.mark.parametrize("N,Mx,My,P,Lx,Ly", [(700, 64, 64, 24, 1.0, 1.0), (900, 90, 45, 16, 2.0, 1.0),
+@pytest.mark.parametrize("N,Mx,My,P,Lx,Ly", [(700, 64, 64, 24, 1.0, 1.0), (900, 92, 46, 16, 2.0, 1.0),
                                               (300, 26, 26, 24, 1.0, 1.0)])
 def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
     xi = 10.0
