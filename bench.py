@@ -157,14 +157,30 @@ def cpu_baseline(p, f, prm, frac=0.05):
             "stage_s": {"real_sample": t_real, "grid": t_grid, "gather_sample": t_gather}}
 
 
+def stage_profile(ctx, fn, steps):
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    for _ in range(steps):
+        fn()
+    ctx.sync()
+    stages = {}
+    for st in ("bin", "bin_grid", "pack", "real", "spread", "merge", "fft", "filter", "gather"):
+        ms, cnt = ctx.prof_get(st)
+        if cnt:
+            stages[st] = ms / steps
+    ctx.prof_enable(False)
+    return stages
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
@@ -172,6 +188,8 @@ def main():
 
     import fasttools2d_amd as ft
     from fasttools2d_amd import ewald_params
+    from fasttools2d_amd.plan import DeviceArray, Plan
+    from fasttools2d_amd.tune import autotune, rel_err
 
     N = args.n
     log(f"bench: building workload N={N}")
@@ -187,59 +205,77 @@ def main():
     ms_plain = time_steps(ctx, lambda: dev.run(ps), args.steps, args.warmup) * 1e3
     log(f"bench: plain {ms_plain:.3f} ms/eval")
     u_plain = dev.result()
+    stages_plain = stage_profile(ctx, lambda: dev.run(ps), args.steps)
+    log(f"bench: plain stage ms {json.dumps(stages_plain)}")
 
-    # stage timing with events over a second timed pass
-    ctx.prof_enable(True)
-    ctx.prof_reset()
-    for _ in range(args.steps):
-        dev.run(ps)
-    ctx.sync()
-    stages = {}
-    for st in ("bin", "real", "bin_grid", "spread", "merge", "fft", "filter", "gather"):
-        ms, cnt = ctx.prof_get(st)
-        stages[st] = ms / max(1, args.steps)
-    ctx.prof_enable(False)
-    log(f"bench: stage ms per eval {json.dumps(stages)}")
+    # accurate reference: tol 1e-13 with two different Ewald splits
+    ref_prm = ewald_params("slp", N, N, Q, 1.0, 1.0, P=24, tol=1e-13)
+    ref_prm2 = ewald_params("slp", N, N, Q, 1.0, 1.0, P=24, tol=1e-13,
+                            Nb=4 * min(24 * round(math.log2(2 * N)), N / 2))
+    refs = []
+    for rp in (ref_prm, ref_prm2):
+        pl = Plan(ctx, "slp", p, p, 1.0, 1.0, rp)
+        refs.append(pl.apply(f))
+        pl.close()
+    u_ref = refs[0]
+    split_agree = rel_err(refs[1], refs[0])
+    err_plain = rel_err(u_plain, u_ref)
+    log(f"bench: reference splits agree to {split_agree:.2e}; plain rel err {err_plain:.2e}")
 
-    # accuracy: against an independent evaluation at tol 1e-13 (other xi, rc, M)
-    prm_t = ewald_params("slp", N, N, Q, 1.0, 1.0, P=24, tol=1e-13)
-    if prm_t["Mx"] <= 4096:
-        dev.run(params_struct(prm_t))
-        ctx.sync()
-        u_ref = dev.result()
-        err = float(np.linalg.norm(u_plain - u_ref) / np.linalg.norm(u_ref))
+    # tuned path: measured parameters, plan reuse, graph replay
+    if args.no_tune:
+        tprm = dict(prm)
+        report = {}
     else:
-        err = None
-    log(f"bench: rel-L2 vs tol-1e-13 evaluation: {err}")
+        tprm, report = autotune(ctx, "slp", p, p, f, 1.0, 1.0, args.tol, u_ref=u_ref, log=log)
+    log(f"bench: tuned parameters {tprm} report {json.dumps(report)}")
+    plan = Plan(ctx, "slp", p, p, 1.0, 1.0, tprm)
+    df = DeviceArray.from_host(ctx, f)
+    du = DeviceArray(ctx, (N, 2))
+    ms_tuned = time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr, graph=True), args.steps,
+                          args.warmup) * 1e3
+    u_tuned = du.download()
+    err_tuned = rel_err(u_tuned, u_ref)
+    stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps)
+    log(f"bench: tuned {ms_tuned:.3f} ms/eval err {err_tuned:.2e} stages {json.dumps(stages)}")
 
-    cand, evald = real_space_work(p, prm)
-    wm = work_model(prm, N, cand, evald)
-    real_ms = stages["real"]
-    achieved = wm["real"] / (real_ms * 1e-3) / 1e12
+    cand, evald = real_space_work(p, tprm)
+    wm = work_model(tprm, N, cand, evald)
+    dom = max(("real", "spread", "gather", "fft"), key=lambda k: stages.get(k, 0))
+    dom_ms = stages[dom]
+    achieved = wm[dom] / (dom_ms * 1e-3) / 1e12
+    cand_p, evald_p = real_space_work(p, prm)
+    wm_plain = work_model(prm, N, cand_p, evald_p)
+    plain_real_tf = wm_plain["real"] / (stages_plain["real"] * 1e-3) / 1e12
     out = {
         "metric": "periodic Stokes evals/s, N=1M, rel err<=1e-8: tuned-batched vs plain GPU",
-        "value": 1e3 / ms_plain,
+        "value": 1e3 / ms_tuned,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_plain,
+        "ms_per_step": ms_tuned,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: uniform points in the unit box, f ~ N(0,1), seed 1",
         "config": {"workload": "SLP spectral Ewald, N=1e6 sources=targets, unit box, tol 1e-8",
-                   "path": "plain FP64, reference parameter recipe",
-                   "params": {k: prm[k] for k in ("xi", "rc", "nside_x", "Mx", "P", "eta", "w")}},
-        "plain_evals_per_s": 1e3 / ms_plain,
-        "rel_err_vs_tight": err,
+                   "path": "tuned: autotuned split, plan reuse, hipGraph replay, FP64",
+                   "params": {k: tprm[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier")}},
+        "rel_err": err_tuned,
+        "plain": {"evals_per_s": 1e3 / ms_plain, "ms": ms_plain, "rel_err": err_plain,
+                  "params": {k: prm[k] for k in ("xi", "nside_x", "Mx", "P")},
+                  "stage_ms": stages_plain, "real_tflops": plain_real_tf},
+        "tuned_vs_plain": ms_plain / ms_tuned,
+        "reference_split_agreement": split_agree,
         "stage_ms": stages,
         "work_model_flops": wm,
-        "roofline": {"bound": "valu", "kernel": "real-space (k_slp_real)",
-                     "achieved": achieved, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved,
+                     "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
-                     "per_launch_flops": wm["real"], "avg_launch_ms": real_ms},
+                     "per_launch_flops": wm[dom], "avg_launch_ms": dom_ms},
+        "tune_report": report,
     }
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")

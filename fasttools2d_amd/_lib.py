@@ -28,7 +28,8 @@ EXPORTS = (
     "se2p_dlp_ewald", "se2p_dev_alloc", "se2p_dev_free", "se2p_memcpy_h2d",
     "se2p_memcpy_d2h", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
     "se2p_prof_reset", "se2p_prof_get", "se2p_assign", "se2p_find_closest_node",
-    "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1",
+    "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1", "se2p_plan_create",
+    "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy",
 )
 
 
@@ -47,7 +48,7 @@ class Params(ctypes.Structure):
     _fields_ = [("xi", ctypes.c_double), ("rc", ctypes.c_double), ("w", ctypes.c_double),
                 ("eta", ctypes.c_double), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
-                ("kinf", ctypes.c_int), ("converged", ctypes.c_int)]
+                ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -99,6 +100,11 @@ def lib():
             "se2p_gather": [P, P, P, I, D, D, D, D, D, I, I, I, P],
             "se2p_fft2": [P, P, P, I, I, I],
             "se2p_e1": [P, P, I, I, P],
+            "se2p_plan_create": [P, I, P, P, P, I, I, D, D, ctypes.POINTER(Params),
+                                 ctypes.POINTER(P)],
+            "se2p_plan_apply": [P, P, P],
+            "se2p_plan_apply_graph": [P, P, P],
+            "se2p_plan_destroy": [P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -206,6 +206,57 @@ __global__ __launch_bounds__(256) void k_sort_big(int* __restrict__ perm,
     }
 }
 
+// Reorders each cell's indices along a Morton curve over a 16 x 16 sub-grid of
+// the cell (ties keep index order), so consecutive runs of 8 points are
+// spatially compact.  Cells larger than kSortCap keep index order.
+__global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
+                                                      const int* __restrict__ start,
+                                                      const double* __restrict__ p, double Lx,
+                                                      double Ly, int nsx, int nsy)
+{
+    __shared__ unsigned key[kSortCap];
+    __shared__ int old[kSortCap];
+    const int b = blockIdx.x;
+    const int lo = start[b], size = start[b + 1] - lo;
+    if (size <= 1 || size > kSortCap) return;
+    const int bx = b % nsx, by = b / nsx;
+    int np = 2;
+    while (np < size) np <<= 1;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        if (i < size) {
+            const int o = perm[lo + i];
+            old[i] = o;
+            double fx = (p[2 * o] / Lx + 0.5) * nsx - bx;
+            double fy = (p[2 * o + 1] / Ly + 0.5) * nsy - by;
+            int qx = min(15, max(0, (int)(fx * 16))), qy = min(15, max(0, (int)(fy * 16)));
+            unsigned code = 0;
+            for (int k = 0; k < 4; ++k)
+                code |= (((qx >> k) & 1u) << (2 * k)) | (((qy >> k) & 1u) << (2 * k + 1));
+            key[i] = (code << 12) | (unsigned)i;
+        } else {
+            key[i] = 0xFFFFFFFFu;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                int ij = i ^ j;
+                if (ij > i) {
+                    unsigned a = key[i], c = key[ij];
+                    bool up = (i & k) == 0;
+                    if ((a > c) == up) {
+                        key[i] = c;
+                        key[ij] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < size; i += blockDim.x) perm[lo + i] = old[key[i] & 0xFFFu];
+}
+
 inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace
@@ -249,8 +300,8 @@ int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly
     return SE2P_OK;
 }
 
-int sort_by_key(se2p_ctx* ctx, const int* key, int* count, int n, int nkeys, int* perm,
-                int* start, Slot tmp_slot)
+int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n, int nkeys,
+                int* perm, int* start, Slot tmp_slot)
 {
     hipStream_t s = ctx->stream;
     // count[] was filled by the key kernel; start = exclusive scan.
@@ -266,13 +317,22 @@ int sort_by_key(se2p_ctx* ctx, const int* key, int* count, int n, int nkeys, int
     // biglist: [count, bucket ids...]; tmp: n ints.
     void* big = nullptr;
     void* tmp = nullptr;
-    SE2P_TRY(ws_get(ctx, WS_BIGLIST, sizeof(int) * (size_t)(nkeys + 1), &big));
-    SE2P_TRY(ws_get(ctx, tmp_slot, sizeof(int) * (size_t)n, &tmp));
+    SE2P_TRY(ws_get(ctx, ws, WS_BIGLIST, sizeof(int) * (size_t)(nkeys + 1), &big));
+    SE2P_TRY(ws_get(ctx, ws, tmp_slot, sizeof(int) * (size_t)n, &tmp));
     SE2P_TRY(launch_fill_int(s, (int*)big, 1, 0));
     hipLaunchKernelGGL(k_sort_buckets, dim3(nkeys), dim3(256), 0, s, perm, start, (int*)big);
     SE2P_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sort_big, dim3(64), dim3(256), 0, s, perm, start, (const int*)big,
                        (int*)tmp);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
+                        double Lx, double Ly, int nsx, int nsy)
+{
+    hipLaunchKernelGGL(k_morton_cells, dim3(nbox), dim3(256), 0, s, perm, start, p, Lx, Ly, nsx,
+                       nsy);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

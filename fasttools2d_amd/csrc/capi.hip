@@ -1,6 +1,7 @@
-// Host side of libse2p.so: the C ABI, contexts, workspace, stage timing, the
-// mex-equivalent entry points and the driver (parameter recipe, wrapping,
-// zero mode).  Reference paths are relative to the reference repository root.
+// Host side of libse2p.so: the C ABI, contexts, workspaces, stage timing, the
+// mex-equivalent entry points, plans and the driver (parameter recipe,
+// wrapping, zero mode).  Reference paths are relative to the reference
+// repository root.
 #include "se2p_internal.h"
 
 #include <cmath>
@@ -8,6 +9,16 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+
+struct se2p_plan {
+    se2p_ctx* ctx = nullptr;
+    se2p::Workspace ws;
+    se2p::Geom G;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const double* graph_f = nullptr;
+    double* graph_u = nullptr;
+};
 
 namespace se2p {
 
@@ -24,25 +35,36 @@ int set_error(int code, const char* fmt, ...)
     return code;
 }
 
-int ws_get(se2p_ctx* ctx, Slot slot, size_t bytes, void** out)
+int ws_get(se2p_ctx* ctx, Workspace* ws, Slot slot, size_t bytes, void** out)
 {
     if (bytes < 256) bytes = 256;
-    if (ctx->ws_size[slot] < bytes) {
-        if (ctx->ws_ptr[slot]) {
+    if (ws->size[slot] < bytes) {
+        if (ctx->capturing)
+            return set_error(SE2P_ERR_ARG, "allocation requested while a graph is captured");
+        if (ws->ptr[slot]) {
             // The stream may still read the old buffer.
             SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-            SE2P_HIP_CHECK(hipFree(ctx->ws_ptr[slot]));
-            ctx->ws_ptr[slot] = nullptr;
-            ctx->ws_size[slot] = 0;
+            SE2P_HIP_CHECK(hipFree(ws->ptr[slot]));
+            ws->ptr[slot] = nullptr;
+            ws->size[slot] = 0;
         }
-        size_t sz = bytes + bytes / 8;  // some headroom for slowly growing inputs
+        size_t sz = bytes + bytes / 8;  // headroom for slowly growing inputs
         void* p = nullptr;
         SE2P_HIP_CHECK(hipMalloc(&p, sz));
-        ctx->ws_ptr[slot] = p;
-        ctx->ws_size[slot] = sz;
+        ws->ptr[slot] = p;
+        ws->size[slot] = sz;
     }
-    *out = ctx->ws_ptr[slot];
+    *out = ws->ptr[slot];
     return SE2P_OK;
+}
+
+void ws_release(Workspace* ws)
+{
+    for (int i = 0; i < WS_NUM_SLOTS; ++i) {
+        if (ws->ptr[i]) (void)hipFree(ws->ptr[i]);
+        ws->ptr[i] = nullptr;
+        ws->size[i] = 0;
+    }
 }
 
 int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec)
@@ -131,151 +153,33 @@ int check_kspace(int Mx, int My, double Lx, double Ly, double xi, double eta, do
     return SE2P_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Device pipelines
-// ---------------------------------------------------------------------------
-int real_space(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
-               const double* d_f, const double* d_n, int Ns, int Nt, double xi, int nsx,
-               int nsy, double Lx, double Ly, double* d_u)
+int check_params(const se2p_params* prm, double Lx, double Ly)
 {
-    if (Nt == 0) return SE2P_OK;
-    hipStream_t s = ctx->stream;
-    const int nbox = nsx * nsy;
-    void *skey, *sperm, *sstart, *scount, *tkey, *tperm, *tstart, *tcount, *ssorted, *tsorted;
-    SE2P_TRY(ws_get(ctx, WS_SRC_KEY, sizeof(int) * (size_t)Ns, &skey));
-    SE2P_TRY(ws_get(ctx, WS_SRC_PERM, sizeof(int) * (size_t)Ns, &sperm));
-    SE2P_TRY(ws_get(ctx, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &sstart));
-    SE2P_TRY(ws_get(ctx, WS_SRC_COUNT, sizeof(int) * (size_t)nbox, &scount));
-    SE2P_TRY(ws_get(ctx, WS_TAR_KEY, sizeof(int) * (size_t)Nt, &tkey));
-    SE2P_TRY(ws_get(ctx, WS_TAR_PERM, sizeof(int) * (size_t)Nt, &tperm));
-    SE2P_TRY(ws_get(ctx, WS_TAR_START, sizeof(int) * (size_t)(nbox + 1), &tstart));
-    SE2P_TRY(ws_get(ctx, WS_TAR_COUNT, sizeof(int) * (size_t)nbox, &tcount));
-    const size_t rec = kind == SE2P_SLP ? 32 : 64;
-    SE2P_TRY(ws_get(ctx, WS_SRC_SORTED, rec * (size_t)Ns, &ssorted));
-    SE2P_TRY(ws_get(ctx, WS_TAR_SORTED, sizeof(double2) * (size_t)Nt, &tsorted));
-    {
-        StageTimer st(ctx, "bin");
-        SE2P_TRY(launch_fill_int(s, (int*)scount, nbox, 0));
-        SE2P_TRY(launch_cell_keys(s, d_psrc, Ns, Lx, Ly, nsx, nsy, (int*)skey, (int*)scount));
-        SE2P_TRY(sort_by_key(ctx, (int*)skey, (int*)scount, Ns, nbox, (int*)sperm, (int*)sstart,
-                             WS_SORT_TMP));
-        SE2P_TRY(launch_fill_int(s, (int*)tcount, nbox, 0));
-        SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
-        SE2P_TRY(sort_by_key(ctx, (int*)tkey, (int*)tcount, Nt, nbox, (int*)tperm, (int*)tstart,
-                             WS_SORT_TMP));
-        SE2P_TRY(launch_pack_sources(s, kind, (int*)sperm, Ns, d_psrc, d_f, d_n,
-                                     (double*)ssorted));
-        SE2P_TRY(launch_pack_targets(s, (int*)tperm, Nt, d_ptar, (double2*)tsorted));
-    }
-    {
-        StageTimer st(ctx, "real");
-        const int mean_t = (int)((Nt + nbox - 1) / nbox);
-        SE2P_TRY(launch_real(s, kind, nsx, nsy, Lx, Ly, xi, (int*)tstart, (int*)tperm,
-                             (double2*)tsorted, (int*)sstart, (double*)ssorted, mean_t, d_u));
-    }
+    if (!prm) return set_error(SE2P_ERR_ARG, "null parameters");
+    SE2P_TRY(check_real(prm->nside_x, prm->nside_y, prm->xi));
+    SE2P_TRY(check_kspace(prm->Mx, prm->My, Lx, Ly, prm->xi, prm->eta, prm->w, prm->P));
+    if (prm->e1_tier != 0 && prm->e1_tier != 1)
+        return set_error(SE2P_ERR_ARG, "e1_tier must be 0 or 1");
     return SE2P_OK;
 }
 
-int kspace(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
-           const double* d_f, const double* d_n, int Ns, int Nt, double xi, double eta, int Mx,
-           int My, double Lx, double Ly, double w, int P, const double* d_add_first,
-           const double* d_zm, double* d_u)
-{
-    if (Nt == 0) return SE2P_OK;
-    hipStream_t s = ctx->stream;
-    KGeom g;
-    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
-    FftAxis ax, ay;
-    SE2P_TRY(fft_axis_init(&ax, Mx));
-    SE2P_TRY(fft_axis_init(&ay, My));
-    double2 *twx, *twy;
-    SE2P_TRY(fft_twiddles(ctx, Mx, WS_TWIDDLE_X, &twx));
-    SE2P_TRY(fft_twiddles(ctx, My, WS_TWIDDLE_Y, &twy));
-    const int ntiles = g.ntx * g.nty;
-    const size_t G = (size_t)Mx * My;
-    void *e1tab, *skey, *sperm, *sstart, *scount, *snode, *soff, *part, *Z0, *Z1 = nullptr, *W;
-    void *tkey, *tperm, *tstart, *tcount, *tnode, *toff;
-    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
-    SE2P_TRY(ws_get(ctx, WS_SPR_KEY, sizeof(int) * (size_t)Ns, &skey));
-    SE2P_TRY(ws_get(ctx, WS_SPR_PERM, sizeof(int) * (size_t)Ns, &sperm));
-    SE2P_TRY(ws_get(ctx, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &sstart));
-    SE2P_TRY(ws_get(ctx, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &scount));
-    SE2P_TRY(ws_get(ctx, WS_SPR_NODE, sizeof(int2) * (size_t)Ns, &snode));
-    SE2P_TRY(ws_get(ctx, WS_SPR_OFF, sizeof(double2) * (size_t)Ns, &soff));
-    SE2P_TRY(ws_get(ctx, WS_GAT_KEY, sizeof(int) * (size_t)Nt, &tkey));
-    SE2P_TRY(ws_get(ctx, WS_GAT_PERM, sizeof(int) * (size_t)Nt, &tperm));
-    SE2P_TRY(ws_get(ctx, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &tstart));
-    SE2P_TRY(ws_get(ctx, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &tcount));
-    SE2P_TRY(ws_get(ctx, WS_GAT_NODE, sizeof(int2) * (size_t)Nt, &tnode));
-    SE2P_TRY(ws_get(ctx, WS_GAT_OFF, sizeof(double2) * (size_t)Nt, &toff));
-    SE2P_TRY(ws_get(ctx, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
-    SE2P_TRY(ws_get(ctx, WS_GRID_A, sizeof(double2) * G, &Z0));
-    if (kind == SE2P_DLP) SE2P_TRY(ws_get(ctx, WS_GRID_B, sizeof(double2) * G, &Z1));
-    SE2P_TRY(ws_get(ctx, WS_GRID_C, sizeof(double2) * G, &W));
-
-    {
-        StageTimer st(ctx, "bin_grid");
-        SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
-        SE2P_TRY(launch_fill_int(s, (int*)scount, ntiles, 0));
-        SE2P_TRY(launch_tile_keys(s, d_psrc, Ns, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
-                                  (int*)skey, (int*)scount, (int2*)snode, (double2*)soff));
-        SE2P_TRY(sort_by_key(ctx, (int*)skey, (int*)scount, Ns, ntiles, (int*)sperm,
-                             (int*)sstart, WS_SORT_TMP));
-        SE2P_TRY(launch_fill_int(s, (int*)tcount, ntiles, 0));
-        SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
-                                  (int*)tkey, (int*)tcount, (int2*)tnode, (double2*)toff));
-        SE2P_TRY(sort_by_key(ctx, (int*)tkey, (int*)tcount, Nt, ntiles, (int*)tperm,
-                             (int*)tstart, WS_SORT_TMP));
-    }
-    const int nspread = kind == SE2P_SLP ? 1 : 2;
-    for (int c = 0; c < nspread; ++c) {
-        double2* Zc = (double2*)(c == 0 ? Z0 : Z1);
-        {
-            StageTimer st(ctx, "spread");
-            SE2P_TRY(launch_spread(s, kind == SE2P_SLP ? 0 : 1 + c, g, (int*)sstart, (int*)sperm,
-                                   (int2*)snode, (double2*)soff, d_f, d_n, (double*)e1tab,
-                                   (double*)part));
-        }
-        {
-            StageTimer st(ctx, "merge");
-            SE2P_TRY(launch_merge(s, 2, g, (double*)part, Zc, nullptr));
-        }
-        {
-            StageTimer st(ctx, "fft");
-            SE2P_TRY(fft2d(ctx, Zc, Mx, My, ax, ay, twx, twy, 0));
-        }
-    }
-    {
-        StageTimer st(ctx, "filter");
-        SE2P_TRY(launch_filter(s, kind, g, (double2*)Z0, (double2*)Z1, (double2*)W));
-    }
-    {
-        StageTimer st(ctx, "fft");
-        SE2P_TRY(fft2d(ctx, (double2*)W, Mx, My, ax, ay, twx, twy, 1));
-    }
-    {
-        StageTimer st(ctx, "gather");
-        SE2P_TRY(launch_gather(s, g, (int*)tstart, (int*)tperm, (int2*)tnode, (double2*)toff,
-                               (double2*)W, (double*)e1tab, d_add_first, d_zm, d_u));
-    }
-    return SE2P_OK;
-}
-
-// Copies host inputs into workspace slots.
+// Copies host inputs into a workspace slot of the context.
 int upload(se2p_ctx* ctx, Slot slot, const double* h, size_t n, double** d)
 {
     void* p = nullptr;
-    SE2P_TRY(ws_get(ctx, slot, sizeof(double) * n, &p));
-    if (n) SE2P_HIP_CHECK(hipMemcpyAsync(p, h, sizeof(double) * n, hipMemcpyHostToDevice,
-                                         ctx->stream));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, slot, sizeof(double) * n, &p));
+    if (n)
+        SE2P_HIP_CHECK(hipMemcpyAsync(p, h, sizeof(double) * n, hipMemcpyHostToDevice,
+                                      ctx->stream));
     *d = (double*)p;
     return SE2P_OK;
 }
 
 int download(se2p_ctx* ctx, double* h, const double* d, size_t n)
 {
-    if (n) SE2P_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                         ctx->stream));
+    if (n)
+        SE2P_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                      ctx->stream));
     SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SE2P_OK;
 }
@@ -286,9 +190,28 @@ int need_ptr(const void* p, int n, const char* what)
     return SE2P_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Parameter recipe (StokesSLP_ewald_2p.m:84-104,147-202;
-// StokesDLP_ewald_2p.m:104-125,171-225)
+// Parameters carrying only what one half of a mex call needs.
+se2p_params real_only(double xi, int nsx, int nsy)
+{
+    se2p_params p{};
+    p.xi = xi;
+    p.nside_x = nsx;
+    p.nside_y = nsy;
+    return p;
+}
+
+se2p_params k_only(double xi, double eta, int Mx, int My, double w, int P)
+{
+    se2p_params p{};
+    p.xi = xi;
+    p.eta = eta;
+    p.Mx = Mx;
+    p.My = My;
+    p.w = w;
+    p.P = P;
+    return p;
+}
+
 // ---------------------------------------------------------------------------
 // MATLAB rat() with its default tolerance 1e-6 |X|: continued-fraction
 // convergents of X.
@@ -405,6 +328,7 @@ int params_ref(int kind, int Ns, int Nt, double Q, double Lx, double Ly, const s
     out->w = P * Lx / out->Mx / 2;
     out->eta = std::pow(2 * xi * out->w / m, 2);
     out->converged = ok1 && ok2;
+    out->e1_tier = 0;  // the reference evaluates E1 to full FP64 accuracy
     if (o->verbose)
         fprintf(stderr,
                 "se2p %s: xi %.6g rc %.6g kinf %d Mx %d My %d nside %d x %d w %.6g eta %.6g\n",
@@ -442,20 +366,19 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
         SE2P_TRY(need_ptr(n2, Ns, "n2"));
     }
     std::vector<double> ps(2 * (size_t)Ns), pt(2 * (size_t)Nt), f(2 * (size_t)Ns), nn;
-    double Q = 1;
     for (int i = 0; i < Ns; ++i) {
         ps[2 * i] = wrap1(xsrc[i], Lx);
         ps[2 * i + 1] = wrap1(ysrc[i], Ly);
         f[2 * i] = f1[i];
         f[2 * i + 1] = f2[i];
     }
-    // Q = sum(sum(f.^2)) + 1, MATLAB sums each row then the two row sums.
+    // Q = sum(sum(f.^2)) + 1: MATLAB sums each row, then the two row sums.
     double q1 = 0, q2 = 0;
     for (int i = 0; i < Ns; ++i) {
         q1 += f1[i] * f1[i];
         q2 += f2[i] * f2[i];
     }
-    Q = (q1 + q2) + 1;
+    const double Q = (q1 + q2) + 1;
     for (int i = 0; i < Nt; ++i) {
         pt[2 * i] = wrap1(xtar[i], Lx);
         pt[2 * i + 1] = wrap1(ytar[i], Ly);
@@ -469,30 +392,23 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     }
     se2p_params prm;
     SE2P_TRY(params_ref(kind, Ns, Nt, Q, Lx, Ly, opt, &prm));
-    SE2P_TRY(check_real(prm.nside_x, prm.nside_y, prm.xi));
-    SE2P_TRY(check_kspace(prm.Mx, prm.My, Lx, Ly, prm.xi, prm.eta, prm.w, prm.P));
+    SE2P_TRY(check_params(&prm, Lx, Ly));
     if (xi_out) *xi_out = prm.xi;
     if (Nt == 0) return SE2P_OK;
 
-    double *dps, *dpt, *df, *dn = nullptr, *dur, *duk, *dzm = nullptr;
+    double *dps, *dpt, *df, *dn = nullptr;
     SE2P_TRY(upload(ctx, WS_PTS_A, ps.data(), ps.size(), &dps));
     SE2P_TRY(upload(ctx, WS_PTS_B, pt.data(), pt.size(), &dpt));
     SE2P_TRY(upload(ctx, WS_DENS_A, f.data(), f.size(), &df));
     if (kind == SE2P_DLP) SE2P_TRY(upload(ctx, WS_DENS_B, nn.data(), nn.size(), &dn));
     void* out = nullptr;
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 4 * (size_t)Nt + 64, &out));
-    dur = (double*)out;
-    duk = dur + 2 * (size_t)Nt;
-    if (kind == SE2P_DLP) {
-        void* part = nullptr;
-        SE2P_TRY(ws_get(ctx, WS_REDUCE, sizeof(double) * 512, &part));
-        dzm = duk + 2 * (size_t)Nt;
-        SE2P_TRY(launch_dlp_zero_mode(ctx->stream, dps, df, dn, Ns, Lx, Ly, (double*)part, dzm));
-    }
-    SE2P_TRY(real_space(ctx, kind, dps, dpt, df, dn, Ns, Nt, prm.xi, prm.nside_x, prm.nside_y,
-                        Lx, Ly, dur));
-    SE2P_TRY(kspace(ctx, kind, dps, dpt, df, dn, Ns, Nt, prm.xi, prm.eta, prm.Mx, prm.My, Lx, Ly,
-                    prm.w, prm.P, nullptr, dzm, duk));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 4 * (size_t)Nt, &out));
+    double* dur = (double*)out;
+    double* duk = dur + 2 * (size_t)Nt;
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, true, true));
+    SE2P_TRY(geom_apply_real(ctx, G, df, dur));
+    SE2P_TRY(geom_apply_k(ctx, G, df, nullptr, kind == SE2P_DLP ? G.zm : nullptr, duk));
     std::vector<double> h(4 * (size_t)Nt);
     SE2P_TRY(download(ctx, h.data(), dur, 4 * (size_t)Nt));
     for (int i = 0; i < Nt; ++i) {
@@ -512,6 +428,29 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     return SE2P_OK;
 }
 
+// One mex call: upload, build the half that is needed, apply, download.
+int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const double* ptar,
+             const double* f, const double* n, int Ns, int Nt, double Lx, double Ly,
+             const se2p_params& prm, double* out)
+{
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Nt == 0) return SE2P_OK;
+    double *dps, *dpt, *df, *dn = nullptr;
+    void* du;
+    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Ns, &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Nt, &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Ns, &df));
+    if (kind == SE2P_DLP) SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Ns, &dn));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &du));
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, real, !real));
+    if (real)
+        SE2P_TRY(geom_apply_real(ctx, G, df, (double*)du));
+    else
+        SE2P_TRY(geom_apply_k(ctx, G, df, nullptr, nullptr, (double*)du));
+    return download(ctx, out, (double*)du, 2 * (size_t)Nt);
+}
+
 }  // namespace
 }  // namespace se2p
 
@@ -519,7 +458,7 @@ using namespace se2p;
 
 extern "C" {
 
-int se2p_version(void) { return 1; }
+int se2p_version(void) { return 2; }
 
 const char* se2p_last_error(void) { return se2p::g_err; }
 
@@ -560,8 +499,7 @@ int se2p_ctx_destroy(se2p_ctx* ctx)
     // Teardown is best effort: every resource is released even if one call fails.
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < WS_NUM_SLOTS; ++i)
-        if (ctx->ws_ptr[i]) (void)hipFree(ctx->ws_ptr[i]);
+    ws_release(&ctx->ws);
     for (void* p : ctx->user_allocs) (void)hipFree(p);
     for (auto& r : ctx->prof_pending) {
         (void)hipEventDestroy(r.start);
@@ -583,17 +521,8 @@ int se2p_slp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const d
     SE2P_TRY(need_ptr(f, Nsrc, "f"));
     SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
     SE2P_TRY(need_ptr(ur, Ntar, "ur"));
-    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    if (Ntar == 0) return SE2P_OK;
-    double *dps, *dpt, *df;
-    void* out;
-    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
-    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
-    SE2P_TRY(real_space(ctx, SE2P_SLP, dps, dpt, df, nullptr, Nsrc, Ntar, xi, nside_x, nside_y,
-                        Lx, Ly, (double*)out));
-    return download(ctx, ur, (double*)out, 2 * (size_t)Ntar);
+    return mex_call(ctx, SE2P_SLP, true, psrc, ptar, f, nullptr, Nsrc, Ntar, Lx, Ly,
+                    real_only(xi, nside_x, nside_y), ur);
 }
 
 int se2p_dlp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
@@ -607,18 +536,8 @@ int se2p_dlp_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const d
     SE2P_TRY(need_ptr(n, Nsrc, "n"));
     SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
     SE2P_TRY(need_ptr(ur, Ntar, "ur"));
-    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    if (Ntar == 0) return SE2P_OK;
-    double *dps, *dpt, *df, *dn;
-    void* out;
-    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
-    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Nsrc, &dn));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
-    SE2P_TRY(real_space(ctx, SE2P_DLP, dps, dpt, df, dn, Nsrc, Ntar, xi, nside_x, nside_y, Lx,
-                        Ly, (double*)out));
-    return download(ctx, ur, (double*)out, 2 * (size_t)Ntar);
+    return mex_call(ctx, SE2P_DLP, true, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    real_only(xi, nside_x, nside_y), ur);
 }
 
 int se2p_slp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
@@ -631,17 +550,8 @@ int se2p_slp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, doubl
     SE2P_TRY(need_ptr(f, Nsrc, "f"));
     SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
     SE2P_TRY(need_ptr(uk, Ntar, "uk"));
-    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    if (Ntar == 0) return SE2P_OK;
-    double *dps, *dpt, *df;
-    void* out;
-    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
-    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
-    SE2P_TRY(kspace(ctx, SE2P_SLP, dps, dpt, df, nullptr, Nsrc, Ntar, xi, eta, Mx, My, Lx, Ly, w,
-                    P, nullptr, nullptr, (double*)out));
-    return download(ctx, uk, (double*)out, 2 * (size_t)Ntar);
+    return mex_call(ctx, SE2P_SLP, false, psrc, ptar, f, nullptr, Nsrc, Ntar, Lx, Ly,
+                    k_only(xi, eta, Mx, My, w, P), uk);
 }
 
 int se2p_dlp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
@@ -655,18 +565,8 @@ int se2p_dlp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, doubl
     SE2P_TRY(need_ptr(n, Nsrc, "n"));
     SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
     SE2P_TRY(need_ptr(uk, Ntar, "uk"));
-    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    if (Ntar == 0) return SE2P_OK;
-    double *dps, *dpt, *df, *dn;
-    void* out;
-    SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
-    SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Nsrc, &dn));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &out));
-    SE2P_TRY(kspace(ctx, SE2P_DLP, dps, dpt, df, dn, Nsrc, Ntar, xi, eta, Mx, My, Lx, Ly, w, P,
-                    nullptr, nullptr, (double*)out));
-    return download(ctx, uk, (double*)out, 2 * (size_t)Ntar);
+    return mex_call(ctx, SE2P_DLP, false, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    k_only(xi, eta, Mx, My, w, P), uk);
 }
 
 void se2p_options_default(se2p_options* opt)
@@ -760,26 +660,101 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
                      double Ly, const se2p_params* prm, double* d_u)
 {
     SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
-    if (!prm) return set_error(SE2P_ERR_ARG, "null parameters");
     if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
-    SE2P_TRY(check_real(prm->nside_x, prm->nside_y, prm->xi));
-    SE2P_TRY(check_kspace(prm->Mx, prm->My, Lx, Ly, prm->xi, prm->eta, prm->w, prm->P));
+    SE2P_TRY(check_params(prm, Lx, Ly));
     if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
     if (Ntar == 0) return SE2P_OK;
-    double* dzm = nullptr;
-    if (kind == SE2P_DLP) {
-        void *part = nullptr, *zm = nullptr;
-        SE2P_TRY(ws_get(ctx, WS_REDUCE, sizeof(double) * 512, &part));
-        SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2, &zm));
-        dzm = (double*)zm;
-        SE2P_TRY(launch_dlp_zero_mode(ctx->stream, d_psrc, d_f, d_n, Nsrc, Lx, Ly, (double*)part,
-                                      dzm));
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, d_psrc, d_ptar, d_n, Nsrc, Ntar, Lx, Ly, *prm,
+                        true, true));
+    return geom_apply(ctx, G, d_f, d_u);
+}
+
+int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+                     const double* d_n, int Nsrc, int Ntar, double Lx, double Ly,
+                     const se2p_params* prm, se2p_plan** out)
+{
+    if (!out) return set_error(SE2P_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    SE2P_TRY(check_params(prm, Lx, Ly));
+    SE2P_TRY(need_ptr(d_psrc, Nsrc, "d_psrc"));
+    SE2P_TRY(need_ptr(d_ptar, Ntar, "d_ptar"));
+    if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    se2p_plan* pl = new (std::nothrow) se2p_plan();
+    if (!pl) return set_error(SE2P_ERR_NOMEM, "plan allocation failed");
+    pl->ctx = ctx;
+    int rc = geom_build(ctx, &pl->ws, &pl->G, kind, d_psrc, d_ptar, d_n, Nsrc, Ntar, Lx, Ly,
+                        *prm, true, true);
+    if (rc == SE2P_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = set_error(SE2P_ERR_HIP, "plan build: %s", hipGetErrorString(e));
     }
-    SE2P_TRY(real_space(ctx, kind, d_psrc, d_ptar, d_f, d_n, Nsrc, Ntar, prm->xi, prm->nside_x,
-                        prm->nside_y, Lx, Ly, d_u));
-    SE2P_TRY(kspace(ctx, kind, d_psrc, d_ptar, d_f, d_n, Nsrc, Ntar, prm->xi, prm->eta, prm->Mx,
-                    prm->My, Lx, Ly, prm->w, prm->P, d_u, dzm, d_u));
+    if (rc != SE2P_OK) {
+        ws_release(&pl->ws);
+        delete pl;
+        return rc;
+    }
+    *out = pl;
+    return SE2P_OK;
+}
+
+int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u)
+{
+    if (!plan) return set_error(SE2P_ERR_ARG, "null plan");
+    SE2P_TRY(need_ptr(d_f, plan->G.Ns, "d_f"));
+    SE2P_TRY(need_ptr(d_u, plan->G.Nt, "d_u"));
+    return geom_apply(plan->ctx, plan->G, d_f, d_u);
+}
+
+int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u)
+{
+    if (!plan) return set_error(SE2P_ERR_ARG, "null plan");
+    SE2P_TRY(need_ptr(d_f, plan->G.Ns, "d_f"));
+    SE2P_TRY(need_ptr(d_u, plan->G.Nt, "d_u"));
+    se2p_ctx* ctx = plan->ctx;
+    if (!plan->exec || plan->graph_f != d_f || plan->graph_u != d_u) {
+        if (plan->exec) {
+            SE2P_HIP_CHECK(hipGraphExecDestroy(plan->exec));
+            plan->exec = nullptr;
+        }
+        if (plan->graph) {
+            SE2P_HIP_CHECK(hipGraphDestroy(plan->graph));
+            plan->graph = nullptr;
+        }
+        SE2P_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        int rc = geom_apply(ctx, plan->G, d_f, d_u);
+        ctx->capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != SE2P_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return set_error(SE2P_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        plan->graph = g;
+        SE2P_HIP_CHECK(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+        plan->graph_f = d_f;
+        plan->graph_u = d_u;
+    }
+    SE2P_HIP_CHECK(hipGraphLaunch(plan->exec, ctx->stream));
+    return SE2P_OK;
+}
+
+int se2p_plan_destroy(se2p_plan* plan)
+{
+    if (!plan) return SE2P_OK;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    ws_release(&plan->ws);
+    delete plan;
     return SE2P_OK;
 }
 
@@ -825,14 +800,14 @@ int se2p_assign(se2p_ctx* ctx, const double* p, int n, double Lx, double Ly, int
     double* dp;
     void *key, *perm, *start, *count;
     SE2P_TRY(upload(ctx, WS_PTS_A, p, 2 * (size_t)n, &dp));
-    SE2P_TRY(ws_get(ctx, WS_SRC_KEY, sizeof(int) * (size_t)n, &key));
-    SE2P_TRY(ws_get(ctx, WS_SRC_PERM, sizeof(int) * (size_t)n, &perm));
-    SE2P_TRY(ws_get(ctx, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &start));
-    SE2P_TRY(ws_get(ctx, WS_SRC_COUNT, sizeof(int) * (size_t)nbox, &count));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SRC_KEY, sizeof(int) * (size_t)n, &key));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SRC_PERM, sizeof(int) * (size_t)n, &perm));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &start));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SRC_COUNT, sizeof(int) * (size_t)nbox, &count));
     SE2P_TRY(launch_fill_int(ctx->stream, (int*)count, nbox, 0));
     SE2P_TRY(launch_cell_keys(ctx->stream, dp, n, Lx, Ly, nside_x, nside_y, (int*)key,
                               (int*)count));
-    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, n, nbox, (int*)perm, (int*)start,
+    SE2P_TRY(sort_by_key(ctx, &ctx->ws, (int*)key, (int*)count, n, nbox, (int*)perm, (int*)start,
                          WS_SORT_TMP));
     if (n) SE2P_HIP_CHECK(hipMemcpyAsync(particle_offsets, perm, sizeof(int) * (size_t)n,
                                          hipMemcpyDeviceToHost, ctx->stream));
@@ -855,7 +830,7 @@ int se2p_find_closest_node(se2p_ctx* ctx, const double* p, int n, double Lx, dou
     double* dp;
     void* out;
     SE2P_TRY(upload(ctx, WS_PTS_A, p, 2 * (size_t)n, &dp));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, (2 * sizeof(int) + 2 * sizeof(double)) * (size_t)n, &out));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, (2 * sizeof(int) + 2 * sizeof(double)) * (size_t)n, &out));
     double* dpx = (double*)out;
     double* dpy = dpx + n;
     int* dmx = (int*)(dpy + n);
@@ -889,20 +864,20 @@ int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, do
     void *e1tab, *key, *perm, *start, *count, *node, *off, *part, *Z;
     SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
-    SE2P_TRY(ws_get(ctx, WS_SPR_KEY, sizeof(int) * (size_t)Nsrc, &key));
-    SE2P_TRY(ws_get(ctx, WS_SPR_PERM, sizeof(int) * (size_t)Nsrc, &perm));
-    SE2P_TRY(ws_get(ctx, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &start));
-    SE2P_TRY(ws_get(ctx, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &count));
-    SE2P_TRY(ws_get(ctx, WS_SPR_NODE, sizeof(int2) * (size_t)Nsrc, &node));
-    SE2P_TRY(ws_get(ctx, WS_SPR_OFF, sizeof(double2) * (size_t)Nsrc, &off));
-    SE2P_TRY(ws_get(ctx, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
-    SE2P_TRY(ws_get(ctx, WS_GRID_A, sizeof(double2) * G, &Z));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_E1TAB, sizeof(double) * 64, &e1tab));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_KEY, sizeof(int) * (size_t)Nsrc, &key));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_PERM, sizeof(int) * (size_t)Nsrc, &perm));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &start));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &count));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_NODE, sizeof(int2) * (size_t)Nsrc, &node));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_OFF, sizeof(double2) * (size_t)Nsrc, &off));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_A, sizeof(double2) * G, &Z));
     SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
     SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
     SE2P_TRY(launch_tile_keys(s, dps, Nsrc, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
                               (int*)count, (int2*)node, (double2*)off));
-    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, Nsrc, ntiles, (int*)perm, (int*)start,
+    SE2P_TRY(sort_by_key(ctx, &ctx->ws, (int*)key, (int*)count, Nsrc, ntiles, (int*)perm, (int*)start,
                          WS_SORT_TMP));
     SE2P_TRY(launch_spread(s, 0, g, (int*)start, (int*)perm, (int2*)node, (double2*)off, df,
                            nullptr, (double*)e1tab, (double*)part));
@@ -937,19 +912,19 @@ int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, do
     void *e1tab, *key, *perm, *start, *count, *node, *off, *u;
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
     SE2P_TRY(upload(ctx, WS_GRID_C, wz.data(), 2 * G, &dW));
-    SE2P_TRY(ws_get(ctx, WS_E1TAB, sizeof(double) * 64, &e1tab));
-    SE2P_TRY(ws_get(ctx, WS_GAT_KEY, sizeof(int) * (size_t)Ntar, &key));
-    SE2P_TRY(ws_get(ctx, WS_GAT_PERM, sizeof(int) * (size_t)Ntar, &perm));
-    SE2P_TRY(ws_get(ctx, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &start));
-    SE2P_TRY(ws_get(ctx, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &count));
-    SE2P_TRY(ws_get(ctx, WS_GAT_NODE, sizeof(int2) * (size_t)Ntar, &node));
-    SE2P_TRY(ws_get(ctx, WS_GAT_OFF, sizeof(double2) * (size_t)Ntar, &off));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &u));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_E1TAB, sizeof(double) * 64, &e1tab));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_KEY, sizeof(int) * (size_t)Ntar, &key));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_PERM, sizeof(int) * (size_t)Ntar, &perm));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &start));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &count));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_NODE, sizeof(int2) * (size_t)Ntar, &node));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_OFF, sizeof(double2) * (size_t)Ntar, &off));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &u));
     SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
     SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
     SE2P_TRY(launch_tile_keys(s, dpt, Ntar, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
                               (int*)count, (int2*)node, (double2*)off));
-    SE2P_TRY(sort_by_key(ctx, (int*)key, (int*)count, Ntar, ntiles, (int*)perm, (int*)start,
+    SE2P_TRY(sort_by_key(ctx, &ctx->ws, (int*)key, (int*)count, Ntar, ntiles, (int*)perm, (int*)start,
                          WS_SORT_TMP));
     SE2P_TRY(launch_gather(s, g, (int*)start, (int*)perm, (int2*)node, (double2*)off,
                            (double2*)dW, (double*)e1tab, nullptr, nullptr, (double*)u));
@@ -968,8 +943,8 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     SE2P_TRY(fft_axis_init(&ax, Mx));
     SE2P_TRY(fft_axis_init(&ay, My));
     double2 *twx, *twy;
-    SE2P_TRY(fft_twiddles(ctx, Mx, WS_TWIDDLE_X, &twx));
-    SE2P_TRY(fft_twiddles(ctx, My, WS_TWIDDLE_Y, &twy));
+    SE2P_TRY(fft_twiddles(ctx, &ctx->ws, Mx, WS_TWIDDLE_X, &twx));
+    SE2P_TRY(fft_twiddles(ctx, &ctx->ws, My, WS_TWIDDLE_Y, &twy));
     const size_t G = (size_t)Mx * My;
     std::vector<double> z(2 * G);
     for (size_t i = 0; i < G; ++i) {
@@ -1000,7 +975,7 @@ int se2p_e1(se2p_ctx* ctx, const double* x, int n, int tier, double* out)
     double* dx;
     void* dout;
     SE2P_TRY(upload(ctx, WS_PTS_A, x, (size_t)n, &dx));
-    SE2P_TRY(ws_get(ctx, WS_OUT_A, sizeof(double) * (size_t)n, &dout));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * (size_t)n, &dout));
     SE2P_TRY(launch_e1(ctx->stream, dx, n, tier, (double*)dout));
     return download(ctx, out, (double*)dout, (size_t)n);
 }

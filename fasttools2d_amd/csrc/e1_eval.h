@@ -45,13 +45,23 @@ struct E1Coeffs<kE1Fast> {
     static constexpr double farq[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARQ;
 };
 
+// 1/x from the hardware reciprocal estimate and two Newton steps (about 1 ulp).
+__device__ __forceinline__ double rcp64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double r = fma(-x, y, 1.0);
+    y = fma(y, r, y);
+    r = fma(-x, y, 1.0);
+    return fma(y, r, y);
+}
+
 // E1(x) given e = exp(-x) and t = 1/x, both already needed by the caller.
 template <int TIER>
 __device__ __forceinline__ double e1_eval(double x, double e, double t)
 {
     using C = E1Coeffs<TIER>;
     if (x < 1.0) return (-SE2P_EULER_GAMMA - log(x)) + x * horner(C::near, x);
-    return e * t * (horner(C::farp, t) / horner(C::farq, t));
+    return e * t * horner(C::farp, t) * rcp64(horner(C::farq, t));
 }
 
 }  // namespace se2p

@@ -101,6 +101,8 @@ int lines_per_block(int n, bool contiguous)
 
 }  // namespace
 
+int fft_prepare();
+
 int fft_axis_init(FftAxis* a, int n)
 {
     if (n < 1 || n > kFftMaxLen)
@@ -123,10 +125,11 @@ int fft_axis_init(FftAxis* a, int n)
     return SE2P_OK;
 }
 
-int fft_twiddles(se2p_ctx* ctx, int n, Slot slot, double2** tw)
+int fft_twiddles(se2p_ctx* ctx, Workspace* ws, int n, Slot slot, double2** tw)
 {
+    SE2P_TRY(fft_prepare());
     void* d = nullptr;
-    SE2P_TRY(ws_get(ctx, slot, sizeof(double2) * (size_t)n, &d));
+    SE2P_TRY(ws_get(ctx, ws, slot, sizeof(double2) * (size_t)n, &d));
     std::vector<double2> h(n);
     for (int k = 0; k < n; ++k) {
         long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / n;
@@ -140,8 +143,7 @@ int fft_twiddles(se2p_ctx* ctx, int n, Slot slot, double2** tw)
     return SE2P_OK;
 }
 
-int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
-          const double2* twx, const double2* twy, int inverse)
+int fft_prepare()
 {
     static bool attr_set = false;
     if (!attr_set) {
@@ -150,6 +152,13 @@ int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const Ff
                                            2 * kFftMaxLen * (int)sizeof(double2)));
         attr_set = true;
     }
+    return SE2P_OK;
+}
+
+int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
+          const double2* twx, const double2* twy, int inverse)
+{
+    if (!ctx->capturing) SE2P_TRY(fft_prepare());
     hipStream_t s = ctx->stream;
     // y lines: contiguous, Mx of them
     {

@@ -355,6 +355,32 @@ __global__ __launch_bounds__(256) void k_zero_mode_part(const double* __restrict
     }
 }
 
+__global__ __launch_bounds__(256) void k_zero_mode_rec_part(const double* __restrict__ rec, int n,
+                                                            double* __restrict__ part)
+{
+    __shared__ double sx[256], sy[256];
+    double ax = 0, ay = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double* r = rec + 8 * (size_t)i;
+        ax += r[6] * r[0];
+        ay += r[6] * r[1];
+    }
+    sx[threadIdx.x] = ax;
+    sy[threadIdx.x] = ay;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            sx[threadIdx.x] += sx[threadIdx.x + d];
+            sy[threadIdx.x] += sy[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sx[0];
+        part[2 * blockIdx.x + 1] = sy[0];
+    }
+}
+
 __global__ void k_zero_mode_final(const double* __restrict__ part, int nblk, double inv_area,
                                   double* __restrict__ out)
 {
@@ -463,6 +489,18 @@ int launch_dlp_zero_mode(hipStream_t s, const double* p, const double* f, const 
 {
     constexpr int nblk = 128;
     hipLaunchKernelGGL(k_zero_mode_part, dim3(nblk), dim3(256), 0, s, p, f, nrm, n, partial);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_mode_final, dim3(1), dim3(64), 0, s, partial, nblk,
+                       1.0 / (Lx * Ly), out);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_dlp_zero_mode_rec(hipStream_t s, const double* rec, int n, double Lx, double Ly,
+                             double* partial, double* out)
+{
+    constexpr int nblk = 128;
+    hipLaunchKernelGGL(k_zero_mode_rec_part, dim3(nblk), dim3(256), 0, s, rec, n, partial);
     SE2P_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_zero_mode_final, dim3(1), dim3(64), 0, s, partial, nblk,
                        1.0 / (Lx * Ly), out);

@@ -58,8 +58,14 @@ enum Slot {
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_PARTIAL, WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
-    WS_DENS_B, WS_OUT_A,
+    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U,
     WS_NUM_SLOTS
+};
+
+// Grow-only device buffers addressed by slot (one set per context, one per plan).
+struct Workspace {
+    void* ptr[WS_NUM_SLOTS] = {};
+    size_t size[WS_NUM_SLOTS] = {};
 };
 
 }  // namespace se2p
@@ -67,9 +73,9 @@ enum Slot {
 struct se2p_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    void* ws_ptr[se2p::WS_NUM_SLOTS] = {};
-    size_t ws_size[se2p::WS_NUM_SLOTS] = {};
+    se2p::Workspace ws;
     bool prof = false;
+    bool capturing = false;
     std::vector<se2p::ProfRecord> prof_pending;
     std::vector<se2p::ProfAcc> prof_acc;
     std::vector<hipEvent_t> event_pool;
@@ -79,7 +85,8 @@ struct se2p_ctx {
 namespace se2p {
 
 // Device buffer of at least `bytes` for a slot; contents are not preserved on growth.
-int ws_get(se2p_ctx* ctx, Slot slot, size_t bytes, void** out);
+int ws_get(se2p_ctx* ctx, Workspace* ws, Slot slot, size_t bytes, void** out);
+void ws_release(Workspace* ws);
 
 // Stage bracketing for per-stage event timing.
 int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec);
@@ -90,7 +97,7 @@ struct StageTimer {
     se2p_ctx* ctx;
     ProfRecord rec;
     bool on;
-    StageTimer(se2p_ctx* c, const char* name) : ctx(c), on(c->prof) {
+    StageTimer(se2p_ctx* c, const char* name) : ctx(c), on(c->prof && !c->capturing) {
         if (on) on = (prof_begin(ctx, name, &rec) == SE2P_OK);
     }
     ~StageTimer() {
@@ -112,9 +119,12 @@ int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly
                      int2* node, double2* off);
 // Stable counting sort of key[0..n) over nkeys buckets: perm lists the original
 // indices bucket by bucket in increasing index order; start has nkeys + 1 entries.
-int sort_by_key(se2p_ctx* ctx, const int* key, int* count, int n, int nkeys, int* perm,
-                int* start, Slot tmp_slot);
+int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n, int nkeys,
+                int* perm, int* start, Slot tmp_slot);
 int launch_fill_int(hipStream_t s, int* p, int n, int v);
+// Morton order inside each cell of a cell-sorted permutation (real-space clusters).
+int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
+                        double Lx, double Ly, int nsx, int nsy);
 // FindClosestNode for n points, unwrapped start nodes.
 int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, double Ly,
                          double h, int P, int* mx, int* my, double* px, double* py);
@@ -126,8 +136,8 @@ int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, doubl
 int launch_pack_sources(hipStream_t s, int kind, const int* perm, int n, const double* p,
                         const double* f, const double* nrm, double* out);
 int launch_pack_targets(hipStream_t s, const int* perm, int n, const double* p, double2* out);
-int launch_real(hipStream_t s, int kind, int nsx, int nsy, double Lx, double Ly, double xi,
-                const int* tstart, const int* tperm, const double2* tsorted,
+int launch_real(hipStream_t s, int kind, int tier, int nsx, int nsy, double Lx, double Ly,
+                double xi, const int* tstart, const int* tperm, const double2* tsorted,
                 const int* sstart, const double* ssorted, int mean_targets, double* u);
 // E1 on n values (tier 0 full, 1 reduced).
 int launch_e1(hipStream_t s, const double* x, int n, int tier, double* out);
@@ -155,6 +165,9 @@ int launch_gather(hipStream_t s, const KGeom& g, const int* tstart, const int* p
                   const double* add_first, const double* zero_mode, double* u);
 int launch_dlp_zero_mode(hipStream_t s, const double* p, const double* f, const double* nrm,
                          int n, double Lx, double Ly, double* partial, double* out);
+// The same sum from sorted DLP source records {x, y, f1, f2, n1, n2, f.n, 0}.
+int launch_dlp_zero_mode_rec(hipStream_t s, const double* rec, int n, double Lx, double Ly,
+                             double* partial, double* out);
 
 // ---------------------------------------------------------------------------
 // FFT (fft.hip)
@@ -167,11 +180,52 @@ struct FftAxis {
     int radix[kFftMaxStages];
 };
 int fft_axis_init(FftAxis* a, int n);
-int fft_twiddles(se2p_ctx* ctx, int n, Slot slot, double2** tw);
+int fft_twiddles(se2p_ctx* ctx, Workspace* ws, int n, Slot slot, double2** tw);
 // In-place 2D transform of a My x Mx column-major complex grid (H[ix*My + iy]).
 // Forward is unnormalised; inverse is unnormalised too (the 1/(Mx My) factor is
 // applied by the k-space filter).
 int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
           const double2* twx, const double2* twy, int inverse);
+
+// ---------------------------------------------------------------------------
+// Pipeline (pipeline.hip): geometry set-up once, density-dependent work per apply
+// ---------------------------------------------------------------------------
+struct Geom {
+    int kind = 0, Ns = 0, Nt = 0;
+    double Lx = 1, Ly = 1;
+    se2p_params prm{};
+    bool has_real = false, has_k = false;
+    int e1tier = 0;
+    double* nrm = nullptr;  // DLP normals, input order
+    // real space
+    int nbox = 0;
+    int *s_perm = nullptr, *s_start = nullptr, *t_perm = nullptr, *t_start = nullptr;
+    double2* t_sorted = nullptr;
+    double* s_rec = nullptr;
+    double *zm = nullptr, *zm_part = nullptr;
+    // k space
+    KGeom kg{};
+    FftAxis ax{}, ay{};
+    double2 *twx = nullptr, *twy = nullptr;
+    double* e1tab = nullptr;
+    int *sp_perm = nullptr, *sp_start = nullptr;
+    int2* sp_node = nullptr;
+    double2* sp_off = nullptr;
+    int *gt_perm = nullptr, *gt_start = nullptr;
+    int2* gt_node = nullptr;
+    double2* gt_off = nullptr;
+    double* partial = nullptr;
+    double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr;
+};
+int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
+               const double* d_ptar, const double* d_n, int Ns, int Nt, double Lx, double Ly,
+               const se2p_params& prm, bool want_real, bool want_k);
+// u = real-space part (set)
+int geom_apply_real(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u);
+// u = add_first + k-space part (+ zero mode when d_zm is given)
+int geom_apply_k(se2p_ctx* ctx, const Geom& G, const double* d_f, const double* d_add_first,
+                 const double* d_zm, double* d_u);
+// u = real + k (+ DLP zero mode)
+int geom_apply(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u);
 
 }  // namespace se2p

@@ -96,6 +96,7 @@ typedef struct {
     double xi, rc, w, eta;
     int nside_x, nside_y, Mx, My, P, kinf;
     int converged; /* 0 when a Newton solve hit its iteration cap */
+    int e1_tier;   /* real-space E1: 0 full FP64 (reference), 1 rel. error < 1e-12 */
 } se2p_params;
 
 void se2p_options_default(se2p_options* opt);
@@ -141,6 +142,26 @@ int se2p_ctx_sync(se2p_ctx* ctx);
 int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
                      const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
                      double Ly, const se2p_params* prm, double* d_u);
+
+/* ------------------------------------------------------------------------ */
+/* Plans: fixed geometry, repeated matvecs.  No counterpart in the reference, */
+/* which recomputes binning and Gaussian factors on every mex call           */
+/* (mex_stokes_slp_real.cpp:98-100, ewald_tools.cpp:125-127,152-159).        */
+/* ------------------------------------------------------------------------ */
+
+/* Bins sources and targets, orders them, computes the start nodes and keeps
+ * every buffer an evaluation needs.  Device pointers; d_n for SE2P_DLP only.
+ * The point arrays are read during creation and not retained. */
+int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+                     const double* d_n, int Nsrc, int Ntar, double Lx, double Ly,
+                     const se2p_params* prm, se2p_plan** out);
+/* d_u (2 x Ntar) = ur + uk (+ DLP zero mode) for density d_f (2 x Nsrc).
+ * Stream-ordered on the context's stream; allocates nothing. */
+int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u);
+/* As se2p_plan_apply, replayed from a HIP graph captured on the first call for
+ * this (d_f, d_u) pair; a new pair re-captures. */
+int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u);
+int se2p_plan_destroy(se2p_plan* plan);
 
 /* ------------------------------------------------------------------------ */
 /* Stage-level entry points (host buffers, synchronous): the helpers of      */

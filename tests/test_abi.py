@@ -75,3 +75,4 @@ def test_parameter_recipe_matches_oracle(oracle, kind, N, Lx, Ly, tol, P, Nb):
     for k in ("xi", "rc", "w", "eta"):
         assert abs(got[k] - ref[k]) <= 1e-14 * abs(ref[k]), k
     assert got["converged"] == int(ref["converged"])
+    assert got["e1_tier"] == 0
