@@ -172,6 +172,28 @@ def stage_profile(ctx, fn, steps):
     return stages
 
 
+def profile_only(ctx, dev, ps, p, f, args):
+    """Runs just one path (for rocprofv3 runs that should see only its kernels)."""
+    from fasttools2d_amd.plan import DeviceArray, Plan
+    from fasttools2d_amd.tune import make_params
+    if args.profile == "plain":
+        for _ in range(args.steps):
+            dev.run(ps)
+        ctx.sync()
+        print(json.dumps({"profile": "plain", "steps": args.steps}))
+        return
+    q = json.loads(args.params)
+    xi = q["c_r"] * q["nside"]
+    prm = make_params(xi, q["nside"], q["nside"], q["M"], q["M"], q["P"], 1.0, q.get("tier", 1))
+    plan = Plan(ctx, "slp", p, p, 1.0, 1.0, prm)
+    df = DeviceArray.from_host(ctx, f)
+    du = DeviceArray(ctx, (len(p), 2))
+    for _ in range(args.steps):
+        plan.apply_device(df.ptr, du.ptr, graph=False)
+    ctx.sync()
+    print(json.dumps({"profile": "tuned", "steps": args.steps, "params": prm}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +203,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile", choices=["plain", "tuned"],
+                    help="only run that path for --steps evaluations (for rocprofv3)")
+    ap.add_argument("--params", default=None,
+                    help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
@@ -200,6 +226,9 @@ def main():
     ctx = ft.Context(0)
     dev = DeviceProblem(ctx, p, f)
     ps = params_struct(prm)
+    if args.profile:
+        profile_only(ctx, dev, ps, p, f, args)
+        return
 
     # plain FP64 path: the reference recipe, everything recomputed per call
     ms_plain = time_steps(ctx, lambda: dev.run(ps), args.steps, args.warmup) * 1e3

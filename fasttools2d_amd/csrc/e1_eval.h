@@ -55,13 +55,27 @@ __device__ __forceinline__ double rcp64(double x)
     return fma(y, r, y);
 }
 
+// The two regimes separately: near for x < 1, far (given e = exp(-x), t = 1/x) for x >= 1.
+template <int TIER>
+__device__ __forceinline__ double e1_near(double x)
+{
+    using C = E1Coeffs<TIER>;
+    return (-SE2P_EULER_GAMMA - log(x)) + x * horner(C::near, x);
+}
+
+template <int TIER>
+__device__ __forceinline__ double e1_far(double e, double t)
+{
+    using C = E1Coeffs<TIER>;
+    return e * t * horner(C::farp, t) * rcp64(horner(C::farq, t));
+}
+
 // E1(x) given e = exp(-x) and t = 1/x, both already needed by the caller.
 template <int TIER>
 __device__ __forceinline__ double e1_eval(double x, double e, double t)
 {
-    using C = E1Coeffs<TIER>;
-    if (x < 1.0) return (-SE2P_EULER_GAMMA - log(x)) + x * horner(C::near, x);
-    return e * t * horner(C::farp, t) * rcp64(horner(C::farq, t));
+    if (x < 1.0) return e1_near<TIER>(x);
+    return e1_far<TIER>(e, t);
 }
 
 }  // namespace se2p

@@ -21,6 +21,167 @@ namespace se2p {
 
 namespace {
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// v * (sgn i)
+__device__ __forceinline__ double2 rot(double2 v, double sgn) { return make_double2(-sgn * v.y, sgn * v.x); }
+
+// In-register DFTs of length R with kernel exp(sgn 2 pi i r k / R), sgn = -1 forward.
+__device__ __forceinline__ void dft2(double2* v)
+{
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+__device__ __forceinline__ void dft4(double2& a, double2& b, double2& c, double2& d, double sgn)
+{
+    const double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = rot(csub(b, d), sgn);
+    a = cadd(t0, t2);
+    c = csub(t0, t2);
+    b = cadd(t1, t3);
+    d = csub(t1, t3);
+}
+
+__device__ __forceinline__ void dft3(double2* v, double sgn)
+{
+    const double h = 0.86602540378443864676;  // sin(2 pi/3)
+    const double2 a = v[0], s = cadd(v[1], v[2]), d = rot(csub(v[1], v[2]), sgn * h);
+    const double2 m = make_double2(a.x - 0.5 * s.x, a.y - 0.5 * s.y);
+    v[0] = cadd(a, s);
+    v[1] = cadd(m, d);
+    v[2] = csub(m, d);
+}
+
+__device__ __forceinline__ void dft5(double2* v, double sgn)
+{
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos(2pi/5), cos(4pi/5)
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin(2pi/5), sin(4pi/5)
+    const double2 a = v[0];
+    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const double2 p1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    const double2 p2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    const double2 q1 = rot(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sgn);
+    const double2 q2 = rot(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sgn);
+    v[0] = cadd(a, cadd(t1, t2));
+    v[1] = cadd(p1, q1);
+    v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2);
+    v[3] = csub(p2, q2);
+}
+
+__device__ __forceinline__ void dft8(double2* v, double sgn)
+{
+    const double r = 0.70710678118654752440;
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3, sgn);
+    dft4(o0, o1, o2, o3, sgn);
+    // W8^k = exp(sgn i pi k/4)
+    const double2 w1 = make_double2(r * (o1.x - sgn * o1.y), r * (o1.y + sgn * o1.x));
+    const double2 w2 = rot(o2, sgn);
+    const double2 w3 = make_double2(-r * (o3.x + sgn * o3.y), r * (sgn * o3.x - o3.y));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1);
+    v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2);
+    v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3);
+    v[7] = csub(e3, w3);
+}
+
+// Small odd radix with table twiddles: W_R^q = tw[q n/R].
+template <int R>
+__device__ __forceinline__ void dft_small(double2* v, const double2* __restrict__ tw, int n,
+                                          double sgn)
+{
+    double2 out[R];
+    const int step = n / R;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double2 acc = v[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const double2 w = tw[((r * k) % R) * step];
+            acc = cadd(acc, cmul(v[r], make_double2(w.x, -sgn * w.y)));
+        }
+        out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+}
+
+// One Stockham stage of radix R over nb lines of length n in LDS.
+template <int R>
+__device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst,
+                                      int n, int nb, int Ns, const double2* __restrict__ tw,
+                                      double sgn)
+{
+    const int m = n / R;
+    const int base = n / (Ns * R);
+    for (int q = threadIdx.x; q < nb * m; q += blockDim.x) {
+        const int b = q / m;
+        const int j = q - b * m;
+        const int jm = j % Ns;
+        const double2* in = src + b * n + j;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[r * m];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const double2 w = tw[r * jm * base];
+                v[r] = cmul(v[r], make_double2(w.x, -sgn * w.y));
+            }
+        }
+        if (R == 2) dft2(v);
+        else if (R == 3) dft3(v, sgn);
+        else if (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
+        else if (R == 5) dft5(v, sgn);
+        else if (R == 8) dft8(v, sgn);
+        else dft_small<R>(v, tw, n, sgn);
+        double2* out = dst + b * n + (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int k = 0; k < R; ++k) out[k * Ns] = v[k];
+    }
+}
+
+// Any radix: one output per thread, O(R) work per output.
+__device__ void stage_generic(const double2* __restrict__ src, double2* __restrict__ dst, int n,
+                              int nb, int Ns, int R, const double2* __restrict__ tw, double sgn)
+{
+    const int m = n / R;
+    const int base = n / (Ns * R);
+    for (int q = threadIdx.x; q < nb * n; q += blockDim.x) {
+        const int b = q / n;
+        const int qq = q - b * n;
+        const int k = qq / m;
+        const int j = qq - k * m;
+        const int jm = j % Ns;
+        const int step = (jm + k * Ns) * base;  // < n
+        const double2* in = src + b * n + j;
+        double re = 0, im = 0;
+        int e = 0;
+        for (int r = 0; r < R; ++r) {
+            const double2 v = in[r * m];
+            const double2 wv = tw[e];
+            const double wi = -sgn * wv.y;
+            re += v.x * wv.x - v.y * wi;
+            im += v.x * wi + v.y * wv.x;
+            e += step;
+            if (e >= n) e -= n;
+        }
+        dst[b * n + (j / Ns) * Ns * R + jm + k * Ns] = make_double2(re, im);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fft_lines(double2* __restrict__ data, int n, int nlines,
                                                    long elem_stride, long line_stride, int B,
                                                    FftAxis ax, const double2* __restrict__ tw,
@@ -45,32 +206,21 @@ __global__ __launch_bounds__(256) void k_fft_lines(double2* __restrict__ data, i
         src[b * n + e] = data[(line0 + b) * line_stride + e * elem_stride];
     }
     __syncthreads();
-    const double sgn = inverse ? -1.0 : 1.0;  // table holds exp(-2 pi i k/n)
+    // the table holds exp(-2 pi i k/n); sgn is the sign of the transform's exponent
+    const double sgn = inverse ? 1.0 : -1.0;
     int Ns = 1;
     for (int s = 0; s < ax.nst; ++s) {
         const int R = ax.radix[s];
-        const int m = n / R;
-        const int base = n / (Ns * R);
-        for (int q = threadIdx.x; q < total; q += blockDim.x) {
-            const int b = q / n;
-            const int qq = q - b * n;
-            const int k = qq / m;
-            const int j = qq - k * m;
-            const int jm = j % Ns;
-            const int step = (jm + k * Ns) * base;  // < n
-            const double2* in = src + b * n + j;
-            double re = 0, im = 0;
-            int e = 0;
-            for (int r = 0; r < R; ++r) {
-                const double2 v = in[r * m];
-                const double2 wv = tw[e];
-                const double wi = sgn * wv.y;
-                re += v.x * wv.x - v.y * wi;
-                im += v.x * wi + v.y * wv.x;
-                e += step;
-                if (e >= n) e -= n;
-            }
-            dst[b * n + (j / Ns) * Ns * R + jm + k * Ns] = make_double2(re, im);
+        switch (R) {
+            case 2: stage<2>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 3: stage<3>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 4: stage<4>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 5: stage<5>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 7: stage<7>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 8: stage<8>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 11: stage<11>(src, dst, n, nb, Ns, tw, sgn); break;
+            case 13: stage<13>(src, dst, n, nb, Ns, tw, sgn); break;
+            default: stage_generic(src, dst, n, nb, Ns, R, tw, sgn); break;
         }
         __syncthreads();
         double2* tmp = src;
@@ -110,7 +260,7 @@ int fft_axis_init(FftAxis* a, int n)
     a->n = n;
     a->nst = 0;
     int r = n;
-    const int small[] = {4, 2, 3, 5, 7};
+    const int small[] = {8, 4, 2, 3, 5, 7};
     for (int p : small)
         while (r % p == 0) {
             a->radix[a->nst++] = p;
