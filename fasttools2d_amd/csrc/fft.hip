@@ -305,6 +305,48 @@ int fft_prepare()
     return SE2P_OK;
 }
 
+namespace {
+
+// out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_transpose(const double2* __restrict__ in,
+                                                   double2* __restrict__ out, int rows, int cols)
+{
+    __shared__ double2 tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
+    }
+}
+
+}  // namespace
+
+int fft_lines_contig(se2p_ctx* ctx, double2* Z, int n, int count, const FftAxis& ax,
+                     const double2* tw, int inverse)
+{
+    if (!ctx->capturing) SE2P_TRY(fft_prepare());
+    const int B = lines_per_block(n, true);
+    const size_t lds = 2 * sizeof(double2) * (size_t)B * n;
+    hipLaunchKernelGGL(k_fft_lines, dim3((count + B - 1) / B), dim3(256), lds, ctx->stream, Z, n,
+                       count, 1L, (long)n, B, ax, tw, inverse);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int transpose(se2p_ctx* ctx, const double2* in, double2* out, int rows, int cols)
+{
+    hipLaunchKernelGGL(k_transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0,
+                       ctx->stream, in, out, rows, cols);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
 int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
           const double2* twx, const double2* twy, int inverse)
 {

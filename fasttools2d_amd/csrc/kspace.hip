@@ -219,17 +219,24 @@ __device__ __forceinline__ void dlp_bracket(double k1, double k2, double a, doub
 template <int KIND>
 __global__ void k_filter(const double2* __restrict__ Z0, const double2* __restrict__ Z1,
                          double2* __restrict__ W, int Mx, int My, double Lx, double Ly,
-                         double xi, double eta, double scale)
+                         double xi, double eta, double scale, int rowmajor)
 {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long)Mx * My) return;
-    const int ix = (int)(gid / My), iy = (int)(gid % My);
+    int ix, iy;
+    if (rowmajor) {
+        iy = (int)(gid / Mx);
+        ix = (int)(gid % Mx);
+    } else {
+        ix = (int)(gid / My);
+        iy = (int)(gid % My);
+    }
     if (ix == 0 && iy == 0) {
-        W[0] = make_double2(0.0, 0.0);
+        W[gid] = make_double2(0.0, 0.0);
         return;
     }
     const int jx = (Mx - ix) % Mx, jy = (My - iy) % My;
-    const long gm = (long)jx * My + jy;
+    const long gm = rowmajor ? (long)jy * Mx + jx : (long)jx * My + jy;
     const double k1 = wavenum(ix, Mx, Lx), k2 = wavenum(iy, My, Ly);
     const double m1 = wavenum(jx, Mx, Lx), m2 = wavenum(jy, My, Ly);
     const double a = 0.25 / (xi * xi);
@@ -451,17 +458,17 @@ int launch_merge(hipStream_t s, int /*ncomp*/, const KGeom& g, const double* par
 }
 
 int launch_filter(hipStream_t s, int kind, const KGeom& g, const double2* Z0,
-                  const double2* Z1, double2* W)
+                  const double2* Z1, double2* W, int rowmajor)
 {
     const long n = (long)g.Mx * g.My;
     const double scale = 1.0 / ((double)g.Mx * (double)g.My);
     dim3 gr((unsigned)((n + 255) / 256)), b(256);
     if (kind == SE2P_SLP)
         hipLaunchKernelGGL(k_filter<SE2P_SLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
-                           g.xi, g.eta, scale);
+                           g.xi, g.eta, scale, rowmajor);
     else
         hipLaunchKernelGGL(k_filter<SE2P_DLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
-                           g.xi, g.eta, scale);
+                           g.xi, g.eta, scale, rowmajor);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

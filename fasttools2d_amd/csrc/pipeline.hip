@@ -162,6 +162,10 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         }
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_C, sizeof(double2) * Gn, &p));
         G->W = (double2*)p;
+        if (kind == SE2P_DLP) {
+            SE2P_TRY(ws_get(ctx, ws, WS_GRID_D, sizeof(double2) * Gn, &p));
+            G->D = (double2*)p;
+        }
         StageTimer st(ctx, "bin_grid");
         SE2P_TRY(launch_e1_table(s, g, G->e1tab));
         SE2P_TRY(launch_fill_int(s, (int*)scount, ntiles, 0));
@@ -208,9 +212,15 @@ int geom_apply_k(se2p_ctx* ctx, const Geom& G, const double* d_f, const double* 
     hipStream_t s = ctx->stream;
     if (G.Nt == 0) return SE2P_OK;
     const KGeom& g = G.kg;
+    // Every FFT pass runs on contiguous lines: y lines in the spread layout
+    // (Z[ix*My + iy]), then a transpose to Z[iy*Mx + ix] for the x lines, the
+    // filter in that layout, and back.
+    //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
+    //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     const int nspread = G.kind == SE2P_SLP ? 1 : 2;
     for (int c = 0; c < nspread; ++c) {
         double2* Zc = c == 0 ? G.Z0 : G.Z1;
+        double2* Tc = c == 0 ? G.W : G.D;
         {
             StageTimer st(ctx, "spread");
             SE2P_TRY(launch_spread(s, G.kind == SE2P_SLP ? 0 : 1 + c, g, G.sp_start, G.sp_perm,
@@ -222,16 +232,20 @@ int geom_apply_k(se2p_ctx* ctx, const Geom& G, const double* d_f, const double* 
         }
         {
             StageTimer st(ctx, "fft");
-            SE2P_TRY(fft2d(ctx, Zc, g.Mx, g.My, G.ax, G.ay, G.twx, G.twy, 0));
+            SE2P_TRY(fft_lines_contig(ctx, Zc, g.My, g.Mx, G.ay, G.twy, 0));
+            SE2P_TRY(transpose(ctx, Zc, Tc, g.Mx, g.My));
+            SE2P_TRY(fft_lines_contig(ctx, Tc, g.Mx, g.My, G.ax, G.twx, 0));
         }
     }
     {
         StageTimer st(ctx, "filter");
-        SE2P_TRY(launch_filter(s, G.kind, g, G.Z0, G.Z1, G.W));
+        SE2P_TRY(launch_filter(s, G.kind, g, G.W, G.D, G.Z0, 1));
     }
     {
         StageTimer st(ctx, "fft");
-        SE2P_TRY(fft2d(ctx, G.W, g.Mx, g.My, G.ax, G.ay, G.twx, G.twy, 1));
+        SE2P_TRY(fft_lines_contig(ctx, G.Z0, g.Mx, g.My, G.ax, G.twx, 1));
+        SE2P_TRY(transpose(ctx, G.Z0, G.W, g.My, g.Mx));
+        SE2P_TRY(fft_lines_contig(ctx, G.W, g.My, g.Mx, G.ay, G.twy, 1));
     }
     StageTimer st(ctx, "gather");
     return launch_gather(s, g, G.gt_start, G.gt_perm, G.gt_node, G.gt_off, G.W, G.e1tab,

@@ -58,7 +58,7 @@ enum Slot {
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_PARTIAL, WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
-    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U,
+    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D,
     WS_NUM_SLOTS
 };
 
@@ -158,8 +158,10 @@ int launch_spread(hipStream_t s, int kind, const KGeom& g, const int* tstart, co
                   const double* e1tab, double* partial);
 int launch_merge(hipStream_t s, int ncomp, const KGeom& g, const double* partial,
                  double2* Z0, double2* Z1);
+// rowmajor = 0: grids are My x Mx column-major (Z[ix*My + iy]); 1: transposed
+// (Z[iy*Mx + ix]).
 int launch_filter(hipStream_t s, int kind, const KGeom& g, const double2* Z0,
-                  const double2* Z1, double2* W);
+                  const double2* Z1, double2* W, int rowmajor = 0);
 int launch_gather(hipStream_t s, const KGeom& g, const int* tstart, const int* perm,
                   const int2* node, const double2* off, const double2* W, const double* e1tab,
                   const double* add_first, const double* zero_mode, double* u);
@@ -184,6 +186,11 @@ int fft_twiddles(se2p_ctx* ctx, Workspace* ws, int n, Slot slot, double2** tw);
 // In-place 2D transform of a My x Mx column-major complex grid (H[ix*My + iy]).
 // Forward is unnormalised; inverse is unnormalised too (the 1/(Mx My) factor is
 // applied by the k-space filter).
+// count contiguous lines of length n (line stride n), in place.
+int fft_lines_contig(se2p_ctx* ctx, double2* Z, int n, int count, const FftAxis& ax,
+                     const double2* tw, int inverse);
+// out[c * rows + r] = in[r * cols + c]
+int transpose(se2p_ctx* ctx, const double2* in, double2* out, int rows, int cols);
 int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
           const double2* twx, const double2* twy, int inverse);
 
@@ -215,7 +222,7 @@ struct Geom {
     int2* gt_node = nullptr;
     double2* gt_off = nullptr;
     double* partial = nullptr;
-    double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr;
+    double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr, *D = nullptr;
 };
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
                const double* d_ptar, const double* d_n, int Ns, int Nt, double Lx, double Ly,
