@@ -125,6 +125,26 @@ def time_steps(ctx, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def measured_traffic(kernel, prm):
+    """HBM bytes per launch of `kernel` at parameters `prm`, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (each
+    counter scaled by what it reads for a device copy of known size), or None
+    when no record matches these parameters."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            recs = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for r in recs:
+        q = r["params"]
+        if (r["stage"] == kernel and q["nside"] == prm["nside_x"] and q["M"] == prm["Mx"]
+                and q["P"] == prm["P"] and q["tier"] == prm["e1_tier"]
+                and abs(q["c_r"] * q["nside"] - prm["xi"]) < 1e-6 * prm["xi"]):
+            return r["hbm_bytes"]
+    return None
+
+
 def cpu_baseline(p, f, prm, frac=0.05):
     """Oracle (C/OpenMP port of the reference stages) on the host cores, on a
     bounded sample: all sources, a random fraction of the targets; spreading and
@@ -182,6 +202,19 @@ def profile_only(ctx, dev, ps, p, f, args):
         ctx.sync()
         print(json.dumps({"profile": "plain", "steps": args.steps}))
         return
+    if args.profile == "copy":
+        # a plain device copy of known size (the runtime's copy kernel): the
+        # scale for the HBM byte counters
+        nbytes = 1 << 30
+        a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        for _ in range(args.steps):
+            ctx.d2d(b, a, nbytes)
+        ctx.sync()
+        ctx.free(a)
+        ctx.free(b)
+        print(json.dumps({"profile": "copy", "steps": args.steps, "bytes_read": nbytes,
+                          "bytes_written": nbytes}))
+        return
     q = json.loads(args.params)
     xi = q["c_r"] * q["nside"]
     prm = make_params(xi, q["nside"], q["nside"], q["M"], q["M"], q["P"], 1.0, q.get("tier", 1))
@@ -203,7 +236,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--profile", choices=["plain", "tuned"],
+    ap.add_argument("--profile", choices=["plain", "tuned", "copy"],
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
                     help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
@@ -263,6 +296,9 @@ def main():
     du = DeviceArray(ctx, (N, 2))
     ms_tuned = time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr, graph=True), args.steps,
                           args.warmup) * 1e3
+    again = [time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr, graph=True), args.steps,
+                        0) * 1e3 for _ in range(2)]
+    log(f"bench: tuned timed window {ms_tuned:.3f} ms/eval; two more windows {again}")
     u_tuned = du.download()
     err_tuned = rel_err(u_tuned, u_ref)
     stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps)
@@ -302,7 +338,8 @@ def main():
         "work_model_flops": wm,
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved,
                      "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / MI355X_FP64_TFLOPS,
+                     "traffic": measured_traffic(dom, tprm),
                      "per_launch_flops": wm[dom], "avg_launch_ms": dom_ms},
         "tune_report": report,
     }

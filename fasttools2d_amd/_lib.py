@@ -26,7 +26,7 @@ EXPORTS = (
     "se2p_device_count", "se2p_slp_real", "se2p_slp_kspace", "se2p_dlp_real",
     "se2p_dlp_kspace", "se2p_options_default", "se2p_params_ref", "se2p_slp_ewald",
     "se2p_dlp_ewald", "se2p_dev_alloc", "se2p_dev_free", "se2p_memcpy_h2d",
-    "se2p_memcpy_d2h", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
+    "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
     "se2p_prof_reset", "se2p_prof_get", "se2p_assign", "se2p_find_closest_node",
     "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1", "se2p_plan_create",
     "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy",
@@ -89,6 +89,7 @@ def lib():
             "se2p_dev_free": [P, P],
             "se2p_memcpy_h2d": [P, P, P, Z],
             "se2p_memcpy_d2h": [P, P, P, Z],
+            "se2p_memcpy_d2d": [P, P, P, Z],
             "se2p_ctx_sync": [P],
             "se2p_eval_device": [P, I, P, P, P, P, I, I, D, D, ctypes.POINTER(Params), P],
             "se2p_prof_enable": [P, I],
@@ -176,6 +177,9 @@ class Context:
     def d2h(self, arr, src):
         check(lib().se2p_memcpy_d2h(self.handle, arr.ctypes.data_as(ctypes.c_void_p), src,
                                     arr.nbytes))
+
+    def d2d(self, dst, src, nbytes):
+        check(lib().se2p_memcpy_d2d(self.handle, dst, src, int(nbytes)))
 
     def prof_enable(self, on=True):
         check(lib().se2p_prof_enable(self.handle, 1 if on else 0))

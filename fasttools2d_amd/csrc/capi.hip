@@ -648,6 +648,13 @@ int se2p_memcpy_d2h(se2p_ctx* ctx, void* dst, const void* src, size_t bytes)
     return SE2P_OK;
 }
 
+int se2p_memcpy_d2d(se2p_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    SE2P_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SE2P_OK;
+}
+
 int se2p_ctx_sync(se2p_ctx* ctx)
 {
     if (!ctx) return set_error(SE2P_ERR_ARG, "null context");

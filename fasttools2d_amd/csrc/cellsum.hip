@@ -12,14 +12,16 @@
 //
 // Mapping: one workgroup of 256 lanes per block of bs x bs cells, one lane per
 // target.  The sources of the block's window ((bs+2)^2 cells, image shifts
-// applied) are staged in LDS with a tag naming their cell position in the
-// window.  Candidates are tested 64 at a time in strided chunks (chunk c holds
-// staged sources c, c + nch, c + 2 nch, ...), so every chunk samples the whole
-// window and each lane's hit count per chunk is close to the mean.  Each lane
-// collects its hits as two 64-bit masks -- pairs with xi^2 r^2 >= 1 (E1 by
-// its rational fit) and below 1 (E1 by log and series) -- and then walks its
-// own masks, so no lane evaluates a pair it does not sum and the two E1
-// regimes do not diverge inside a wave.
+// applied) are staged in LDS, window column by window column, so the 3 x 3
+// stencil of any target cell is three contiguous runs of the staging buffer
+// (one per stencil column) and its own cell a sub-run of the middle one.  Each
+// lane walks only its own stencil, 64 candidates at a time: it tests them
+// (own cell, or r^2 < cutoff^2) into two 64-bit masks -- pairs with
+// xi^2 r^2 >= 1 (E1 by its rational fit) and below 1 (E1 by log and series) --
+// and then evaluates the pairs of each mask, so no lane evaluates a pair it
+// does not sum and the two E1 regimes do not diverge inside a wave.  Windows
+// larger than the staging buffer are staged in rounds; round r holds every
+// nround-th source, which keeps each stencil run contiguous in every round.
 #include "se2p_internal.h"
 #include "e1_eval.h"
 
@@ -50,6 +52,7 @@ struct SrcOf<SE2P_DLP> {
 constexpr int kBlkThreads = 256;
 constexpr int kMaxBlockSide = 4;
 constexpr int kMaxEntries = (kMaxBlockSide + 2) * (kMaxBlockSide + 2);
+constexpr int kWords = 2, kChunk = 64 * kWords;  // candidates per test chunk
 
 // 4 pi times the SLP pair term for r = target - source:
 //   E1(xi^2 r^2)/2 f + e^{-xi^2 r^2} ((r.f) r/r^2 - f)
@@ -88,8 +91,25 @@ __device__ __forceinline__ void dlp_pair(double r1, double r2, double rr, double
 
 __device__ __forceinline__ int wrapi(int q, int n) { return ((q % n) + n) % n; }
 
-template <int KIND, int TIER>
-__global__ __launch_bounds__(kBlkThreads) void k_real_blk(
+// Bits [lo, hi) of a 64-bit word (0 <= lo <= hi <= 64).
+__device__ __forceinline__ unsigned long long bit_range(int lo, int hi)
+{
+    const unsigned long long m = hi - lo >= 64 ? ~0ull : (1ull << (hi - lo)) - 1;
+    return lo < hi ? m << lo : 0ull;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// First staged index at or after window position g in round r of nround.
+__device__ __forceinline__ int round_index(int g, int r, int nround)
+{
+    return nround == 1 ? g : (g - r + nround - 1) / nround;
+}
+
+// UNI: blocks of one cell (bs = 1), where every lane has the same stencil and
+// the candidate bookkeeping is wave-uniform (scalar registers, LDS broadcast).
+template <int KIND, int TIER, bool UNI>
+__global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_real_stencil(
     int nsx, int nsy, int bs, int nbx, double Lx, double Ly, double xi2, double cut2,
     double self, const int* __restrict__ tstart, const int* __restrict__ tperm,
     const double2* __restrict__ tsorted, const int* __restrict__ sstart,
@@ -98,19 +118,17 @@ __global__ __launch_bounds__(kBlkThreads) void k_real_blk(
     using Rec = typename SrcOf<KIND>::T;
     constexpr int CAP = SrcOf<KIND>::cap;
     __shared__ Rec buf[CAP];
-    __shared__ int tagb[CAP];
-    __shared__ int e_src[kMaxEntries], e_off[kMaxEntries + 1], e_tag[kMaxEntries];
+    __shared__ int e_src[kMaxEntries], e_off[kMaxEntries + 1];
     __shared__ double e_ox[kMaxEntries], e_oy[kMaxEntries];
     __shared__ int c_t0[kMaxBlockSide * kMaxBlockSide], c_off[kMaxBlockSide * kMaxBlockSide + 1];
-    __shared__ int c_tag[kMaxBlockSide * kMaxBlockSide];
 
     const int tid = threadIdx.x;
     const int bx0 = (blockIdx.x % nbx) * bs, by0 = (blockIdx.x / nbx) * bs;
     const int bw = min(bs, nsx - bx0), bh = min(bs, nsy - by0);
-    const int W = bs + 2;
-    const int nent = (bw + 2) * (bh + 2), ncell = bw * bh;
+    const int H = bh + 2;  // window column height; entry (ux, uy) is (ux+1) H + (uy+1)
+    const int nent = (bw + 2) * H, ncell = bw * bh;
     if (tid < nent) {
-        const int ux = tid / (bh + 2) - 1, uy = tid % (bh + 2) - 1;
+        const int ux = tid / H - 1, uy = tid % H - 1;
         const int qx = bx0 + ux, qy = by0 + uy;
         const int wx = wrapi(qx, nsx), wy = wrapi(qy, nsy);
         const int cell = wy * nsx + wx;
@@ -118,14 +136,12 @@ __global__ __launch_bounds__(kBlkThreads) void k_real_blk(
         e_off[tid + 1] = sstart[cell + 1] - sstart[cell];  // count, prefixed below
         e_ox[tid] = (Lx * (qx - wx)) / nsx;
         e_oy[tid] = (Ly * (qy - wy)) / nsy;
-        e_tag[tid] = (ux + 1) * W + (uy + 1);
     }
     if (tid < ncell) {
         const int cx = tid / bh, cy = tid % bh;
         const int cell = (by0 + cy) * nsx + (bx0 + cx);
         c_t0[tid] = tstart[cell];
         c_off[tid + 1] = tstart[cell + 1] - tstart[cell];
-        c_tag[tid] = (cx + 1) * W + (cy + 1);
     }
     __syncthreads();
     if (tid == 0) {
@@ -137,23 +153,30 @@ __global__ __launch_bounds__(kBlkThreads) void k_real_blk(
     __syncthreads();
     const int total = e_off[nent], ntar = c_off[ncell];
     const double ixi2 = 1.0 / xi2;
+    const int nround = (total + CAP - 1) / CAP;
 
     for (int tb = 0; tb < ntar; tb += kBlkThreads) {
         const int i = tb + tid;
         const bool act = i < ntar;
-        int ti = 0, ttag = -1;
+        int ti = 0, cx = 0, cy = 0;
         double2 pt = make_double2(0.0, 0.0);
         if (act) {
             int c = 0;
             while (c + 1 < ncell && c_off[c + 1] <= i) ++c;
             ti = c_t0[c] + (i - c_off[c]);
-            ttag = c_tag[c];
+            cx = c / bh;
+            cy = c % bh;
             pt = tsorted[ti];
         }
+        // window positions of the three stencil columns and of the own cell
+        int g0[3], g1[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            g0[j] = e_off[(cx + j) * H + cy];
+            g1[j] = e_off[(cx + j) * H + cy + 3];
+        }
+        const int go0 = e_off[(cx + 1) * H + cy + 1], go1 = e_off[(cx + 1) * H + cy + 2];
         double a1 = 0, a2 = 0;
-        // Staging rounds take every nround-th source of the window, so each round
-        // (and each strided chunk inside it) samples the whole window.
-        const int nround = (total + CAP - 1) / CAP;
         for (int r = 0; r < nround; ++r) {
             const int rn = (total - r + nround - 1) / nround;
             __syncthreads();
@@ -165,35 +188,107 @@ __global__ __launch_bounds__(kBlkThreads) void k_real_blk(
                 rec.x += e_ox[e];
                 rec.y += e_oy[e];
                 buf[k] = rec;
-                tagb[k] = e_tag[e];
             }
             __syncthreads();
-            const int nch = (rn + 63) >> 6;
+            // this lane's candidates: staged runs [s_j, s_j + L_j), j = 0, 1, 2,
+            // numbered 0 .. n-1 in that order; the own cell is [o0, o1)
+            int s0 = round_index(g0[0], r, nround);
+            const int s1 = round_index(g0[1], r, nround);
+            const int s2 = round_index(g0[2], r, nround);
+            int L0 = round_index(g1[0], r, nround) - s0;
+            int L01 = L0 + round_index(g1[1], r, nround) - s1;
+            int n = L01 + round_index(g1[2], r, nround) - s2;
+            int o0 = round_index(go0, r, nround), o1 = round_index(go1, r, nround);
+            // candidate pos -> staged index: pos + s0, shifted at the two run ends
+            int j1 = s1 - L0 - s0, j2 = s2 - L01 - (s1 - L0);
+            if (UNI) {
+                // one cell: the window is the stencil, its three runs follow each
+                // other from 0, and candidate pos is staged source pos
+                s0 = j1 = j2 = 0;
+                L0 = __builtin_amdgcn_readfirstlane(L0);
+                L01 = __builtin_amdgcn_readfirstlane(L01);
+                n = __builtin_amdgcn_readfirstlane(n);
+                o0 = __builtin_amdgcn_readfirstlane(o0);
+                o1 = __builtin_amdgcn_readfirstlane(o1);
+            }
+            auto staged = [&](int pos) {
+                return pos + s0 + (pos >= L0 ? j1 : 0) + (pos >= L01 ? j2 : 0);
+            };
+            // Chunk c takes candidates c, c + nch, c + 2 nch, ... (nch the same
+            // for the whole wave), so each chunk samples all three runs and the
+            // hit counts of the lanes in a chunk stay close to their mean.
+            int nmax = n;
+            if (!UNI) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+            }
+            const int nch = (nmax + kChunk - 1) / kChunk;
+            const int kmax = nch ? (nmax + nch - 1) / nch : 0;  // <= kChunk
+            // own cell and valid candidates as candidate-position ranges: the own
+            // cell is a sub-run of run 1, which holds positions [L0, L01)
+            const int po0 = o0 - (s0 + j1), po1 = o1 - (s0 + j1);
             for (int c = 0; c < nch; ++c) {
-                const int kmax = (rn - c + nch - 1) / nch;  // <= 64
-                unsigned long long far = 0, near = 0;
-#pragma unroll 8
-                for (int k = 0; k < kmax; ++k) {
-                    const int idx = c + k * nch;
-                    const double2 sp = *reinterpret_cast<const double2*>(&buf[idx]);
-                    const double dx = pt.x - sp.x, dy = pt.y - sp.y;
-                    const double r2 = dx * dx + dy * dy;
-                    const bool own = tagb[idx] == ttag;
-                    const unsigned long long bit = 1ull << k;
+                // chunk candidates kg = 0 .. kc-1 at positions c + kg nch
+                const int kc = UNI ? (n - c + nch - 1) / nch : kmax;
+                const int kvalid = act ? (n - c + nch - 1) / nch : 0;
+                const int kown0 = nch == 1 ? po0 - c : (po0 - c + nch - 1) / nch;
+                const int kown1 = nch == 1 ? po1 - c : (po1 - c + nch - 1) / nch;
+                // per word: comparison bits shifted in (candidate k of the word, k <
+                // kw[w], lands on bit kw[w] - 1 - k), then combined with the range masks;
+                // kw[w] is rounded up to a multiple of 8 and the candidates past the
+                // chunk's end are masked off as not valid
+                unsigned long long fm[kWords], nm[kWords];
+                int kw[kWords];
+#pragma unroll
+                for (int w = 0; w < kWords; ++w) {
+                    kw[w] = (min(64, max(0, kc - 64 * w)) + 7) & ~7;
+                    unsigned long long cm = 0, zm = 0;  // r^2 < cut^2; SLP r^2 < 1/xi^2, DLP r^2 != 0
+                    for (int k0 = 0; k0 < kw[w]; k0 += 8) {
+                        double2 sp[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int pos = c + (64 * w + k0 + j) * nch;
+                            sp[j] = *reinterpret_cast<const double2*>(&buf[min(staged(pos), CAP - 1)]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const double dx = pt.x - sp[j].x, dy = pt.y - sp[j].y;
+                            const double r2 = dx * dx + dy * dy;
+                            cm = (cm << 1) | (r2 < cut2 ? 1ull : 0ull);
+                            if (KIND == SE2P_SLP)
+                                zm = (zm << 1) | (r2 < ixi2 ? 1ull : 0ull);
+                            else
+                                zm = (zm << 1) | (r2 != 0 ? 1ull : 0ull);
+                        }
+                    }
+                    const int K = kw[w];
+                    const unsigned long long valid = bit_range(K - clampi(kvalid - 64 * w, 0, K), K);
+                    const unsigned long long own = bit_range(K - clampi(kown1 - 64 * w, 0, K),
+                                                             K - clampi(kown0 - 64 * w, 0, K));
                     if (KIND == SE2P_SLP) {
-                        const bool hit = act & (own | (r2 < cut2));
-                        const bool nr = xi2 * r2 < 1.0;
-                        far |= (hit & !nr) ? bit : 0ull;
-                        near |= (hit & nr) ? bit : 0ull;
+                        // every pair with xi^2 r^2 < 1 is inside the cutoff
+                        const unsigned long long hit = (cm | own) & valid;
+                        fm[w] = hit & ~zm;
+                        nm[w] = hit & zm;
                     } else {
-                        const bool hit = act & (own ? (r2 != 0) : (r2 < cut2));
-                        far |= hit ? bit : 0ull;
+                        fm[w] = ((cm & ~own) | (own & zm)) & valid;
+                        nm[w] = 0;
                     }
                 }
-                while (far) {
-                    const int k = __builtin_ctzll(far);
-                    far &= far - 1;
-                    const Rec& s = buf[c + k * nch];
+                // candidate position of the lowest set bit of (lo, hi), which is cleared
+                auto next = [&](unsigned long long& lo, unsigned long long& hi) {
+                    const bool z = lo == 0;
+                    const unsigned long long m = z ? hi : lo;
+                    const int bit = __builtin_ctzll(m);
+                    const unsigned long long m2 = m & (m - 1);
+                    lo = z ? lo : m2;
+                    hi = z ? m2 : hi;
+                    const int k = z ? 64 + kw[1] - 1 - bit : kw[0] - 1 - bit;
+                    return c + k * nch;
+                };
+                unsigned long long lo = fm[0], hi = fm[1];
+                while (lo | hi) {
+                    const Rec& s = buf[staged(next(lo, hi))];
                     const double dx = pt.x - s.x, dy = pt.y - s.y;
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
@@ -204,14 +299,14 @@ __global__ __launch_bounds__(kBlkThreads) void k_real_blk(
                     }
                 }
                 if (KIND == SE2P_SLP) {
-                    while (near) {
-                        const int k = __builtin_ctzll(near);
-                        near &= near - 1;
-                        const int idx = c + k * nch;
+                    lo = nm[0];
+                    hi = nm[1];
+                    while (lo | hi) {
+                        const int idx = staged(next(lo, hi));
                         const double4& q = reinterpret_cast<const double4&>(buf[idx]);
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
-                        if (tagb[idx] == ttag && fabs(r2) < 1e-15) {
+                        if (idx >= o0 && idx < o1 && fabs(r2) < 1e-15) {
                             a1 += self * q.z;
                             a2 += self * q.w;
                         } else {
@@ -313,8 +408,16 @@ int launch_real(hipStream_t s, int kind, int tier, int nsx, int nsy, double Lx, 
     const int nbx = (nsx + bs - 1) / bs, nby = (nsy + bs - 1) / bs;
     dim3 gr(nbx * nby), bl(kBlkThreads);
 #define SE2P_LAUNCH_REAL(K, T, REC)                                                              \
-    hipLaunchKernelGGL((k_real_blk<K, T>), gr, bl, 0, s, nsx, nsy, bs, nbx, Lx, Ly, xi2, cut2,  \
-                       self, tstart, tperm, tsorted, sstart, (const REC*)ssorted, u)
+    do {                                                                                         \
+        if (bs == 1)                                                                             \
+            hipLaunchKernelGGL((k_real_stencil<K, T, true>), gr, bl, 0, s, nsx, nsy, bs, nbx, Lx, \
+                               Ly, xi2, cut2, self, tstart, tperm, tsorted, sstart,              \
+                               (const REC*)ssorted, u);                                          \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_real_stencil<K, T, false>), gr, bl, 0, s, nsx, nsy, bs, nbx,   \
+                               Lx, Ly, xi2, cut2, self, tstart, tperm, tsorted, sstart,          \
+                               (const REC*)ssorted, u);                                          \
+    } while (0)
     if (kind == SE2P_SLP && tier == kE1Full)
         SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full, double4);
     else if (kind == SE2P_SLP)

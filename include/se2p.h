@@ -132,6 +132,8 @@ int se2p_dev_alloc(se2p_ctx* ctx, size_t bytes, void** dptr);
 int se2p_dev_free(se2p_ctx* ctx, void* dptr);
 int se2p_memcpy_h2d(se2p_ctx* ctx, void* dst, const void* src, size_t bytes);
 int se2p_memcpy_d2h(se2p_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy, stream-ordered on the context's stream (no host sync). */
+int se2p_memcpy_d2d(se2p_ctx* ctx, void* dst, const void* src, size_t bytes);
 int se2p_ctx_sync(se2p_ctx* ctx);
 
 /* One complete evaluation on device pointers with explicit parameters: binning,
