@@ -151,7 +151,9 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         for (int c = 0; c < ncell; ++c) c_off[c + 1] += c_off[c];
     }
     __syncthreads();
-    const int total = e_off[nent], ntar = c_off[ncell];
+    // block-uniform: kept in scalar registers
+    const int total = __builtin_amdgcn_readfirstlane(e_off[nent]);
+    const int ntar = __builtin_amdgcn_readfirstlane(c_off[ncell]);
     const double ixi2 = 1.0 / xi2;
     const int nround = (total + CAP - 1) / CAP;
 
@@ -168,14 +170,6 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             cy = c % bh;
             pt = tsorted[ti];
         }
-        // window positions of the three stencil columns and of the own cell
-        int g0[3], g1[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            g0[j] = e_off[(cx + j) * H + cy];
-            g1[j] = e_off[(cx + j) * H + cy + 3];
-        }
-        const int go0 = e_off[(cx + 1) * H + cy + 1], go1 = e_off[(cx + 1) * H + cy + 2];
         double a1 = 0, a2 = 0;
         for (int r = 0; r < nround; ++r) {
             const int rn = (total - r + nround - 1) / nround;
@@ -190,15 +184,18 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 buf[k] = rec;
             }
             __syncthreads();
-            // this lane's candidates: staged runs [s_j, s_j + L_j), j = 0, 1, 2,
-            // numbered 0 .. n-1 in that order; the own cell is [o0, o1)
-            int s0 = round_index(g0[0], r, nround);
-            const int s1 = round_index(g0[1], r, nround);
-            const int s2 = round_index(g0[2], r, nround);
-            int L0 = round_index(g1[0], r, nround) - s0;
-            int L01 = L0 + round_index(g1[1], r, nround) - s1;
-            int n = L01 + round_index(g1[2], r, nround) - s2;
-            int o0 = round_index(go0, r, nround), o1 = round_index(go1, r, nround);
+            // this lane's candidates: staged runs [s_j, s_j + L_j), j = 0, 1, 2 (the
+            // stencil columns: window entries (cx + j) H + cy .. + 2), numbered
+            // 0 .. n-1 in that order; the own cell (entry (cx + 1) H + cy + 1) is [o0, o1)
+            const int e0 = cx * H + cy;
+            int s0 = round_index(e_off[e0], r, nround);
+            const int s1 = round_index(e_off[e0 + H], r, nround);
+            const int s2 = round_index(e_off[e0 + 2 * H], r, nround);
+            int L0 = round_index(e_off[e0 + 3], r, nround) - s0;
+            int L01 = L0 + round_index(e_off[e0 + H + 3], r, nround) - s1;
+            int n = L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2;
+            int o0 = round_index(e_off[e0 + H + 1], r, nround);
+            int o1 = round_index(e_off[e0 + H + 2], r, nround);
             // candidate pos -> staged index: pos + s0, shifted at the two run ends
             int j1 = s1 - L0 - s0, j2 = s2 - L01 - (s1 - L0);
             if (UNI) {

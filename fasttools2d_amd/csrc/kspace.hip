@@ -134,30 +134,37 @@ __global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__
 
 // Node-parallel halo merge: node (ix, iy) adds, in a fixed order, every tile
 // region that covers it (regions wrap periodically, possibly more than once
-// on grids smaller than R).
+// on grids smaller than R).  One block row per ix; indices wrap by a single
+// conditional add (tile and node differences stay within one period).
 __global__ void k_merge(const double* __restrict__ partial, int Mx, int My, int TX, int TY,
                         int ntx, int nty, int R, double2* __restrict__ Z)
 {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long)Mx * My) return;
-    const int ix = (int)(gid / My), iy = (int)(gid % My);
+    const int ix = blockIdx.y;
+    const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iy >= My) return;
     const int ndx = min(ntx, R / TX + 2), ndy = min(nty, R / TY + 2);
     const int bx = ix / TX, by = iy / TY;
     double s0 = 0, s1 = 0;
     for (int dx = 0; dx < ndx; ++dx) {
-        const int tx = ((bx - dx) % ntx + ntx) % ntx;
-        for (int lx = ((ix - tx * TX) % Mx + Mx) % Mx; lx < R; lx += Mx) {
+        int tx = bx - dx;
+        tx += tx < 0 ? ntx : 0;
+        int lx = ix - tx * TX;
+        lx += lx < 0 ? Mx : 0;
+        for (; lx < R; lx += Mx) {
             for (int dy = 0; dy < ndy; ++dy) {
-                const int ty = ((by - dy) % nty + nty) % nty;
+                int ty = by - dy;
+                ty += ty < 0 ? nty : 0;
                 const double* q = partial + (size_t)(ty * ntx + tx) * 2 * R * R + (size_t)lx * R;
-                for (int ly = ((iy - ty * TY) % My + My) % My; ly < R; ly += My) {
+                int ly = iy - ty * TY;
+                ly += ly < 0 ? My : 0;
+                for (; ly < R; ly += My) {
                     s0 += q[ly];
                     s1 += q[R * R + ly];
                 }
             }
         }
     }
-    Z[gid] = make_double2(s0, s1);
+    Z[(size_t)ix * My + iy] = make_double2(s0, s1);
 }
 
 // Wavenumber of grid index j: indices up to M/2 non-negative
@@ -450,9 +457,8 @@ int launch_spread(hipStream_t s, int mode, const KGeom& g, const int* tstart, co
 int launch_merge(hipStream_t s, int /*ncomp*/, const KGeom& g, const double* partial,
                  double2* Z0, double2* /*Z1*/)
 {
-    const long n = (long)g.Mx * g.My;
-    hipLaunchKernelGGL(k_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, g.Mx,
-                       g.My, g.TX, g.TY, g.ntx, g.nty, g.R, Z0);
+    hipLaunchKernelGGL(k_merge, dim3((unsigned)((g.My + 255) / 256), (unsigned)g.Mx), dim3(256), 0,
+                       s, partial, g.Mx, g.My, g.TX, g.TY, g.ntx, g.nty, g.R, Z0);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
