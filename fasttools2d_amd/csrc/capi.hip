@@ -14,6 +14,7 @@ struct se2p_plan {
     se2p_ctx* ctx = nullptr;
     se2p::Workspace ws;
     se2p::Geom G;
+    se2p::BatchExec B;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const double* graph_f = nullptr;
@@ -407,8 +408,11 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     double* duk = dur + 2 * (size_t)Nt;
     Geom G;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, true, true));
-    SE2P_TRY(geom_apply_real(ctx, G, df, dur));
-    SE2P_TRY(geom_apply_k(ctx, G, df, nullptr, kind == SE2P_DLP ? G.zm : nullptr, duk));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    SE2P_TRY(batch_apply_real(ctx, B, df, dur));
+    SE2P_TRY(batch_apply_k(ctx, B, df, duk, false, kind == SE2P_DLP));
     std::vector<double> h(4 * (size_t)Nt);
     SE2P_TRY(download(ctx, h.data(), dur, 4 * (size_t)Nt));
     for (int i = 0; i < Nt; ++i) {
@@ -444,10 +448,13 @@ int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const doubl
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &du));
     Geom G;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, real, !real));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
     if (real)
-        SE2P_TRY(geom_apply_real(ctx, G, df, (double*)du));
+        SE2P_TRY(batch_apply_real(ctx, B, df, (double*)du));
     else
-        SE2P_TRY(geom_apply_k(ctx, G, df, nullptr, nullptr, (double*)du));
+        SE2P_TRY(batch_apply_k(ctx, B, df, (double*)du, false, false));
     return download(ctx, out, (double*)du, 2 * (size_t)Nt);
 }
 
@@ -675,7 +682,10 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     Geom G;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, d_psrc, d_ptar, d_n, Nsrc, Ntar, Lx, Ly, *prm,
                         true, true));
-    return geom_apply(ctx, G, d_f, d_u);
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    return batch_apply(ctx, B, d_f, d_u);
 }
 
 int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
@@ -697,6 +707,10 @@ int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     int rc = geom_build(ctx, &pl->ws, &pl->G, kind, d_psrc, d_ptar, d_n, Nsrc, Ntar, Lx, Ly,
                         *prm, true, true);
     if (rc == SE2P_OK) {
+        const Geom* gp = &pl->G;
+        rc = batch_build(ctx, &pl->ws, &pl->B, &gp, 1);
+    }
+    if (rc == SE2P_OK) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = set_error(SE2P_ERR_HIP, "plan build: %s", hipGetErrorString(e));
     }
@@ -714,7 +728,8 @@ int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u)
     if (!plan) return set_error(SE2P_ERR_ARG, "null plan");
     SE2P_TRY(need_ptr(d_f, plan->G.Ns, "d_f"));
     SE2P_TRY(need_ptr(d_u, plan->G.Nt, "d_u"));
-    return geom_apply(plan->ctx, plan->G, d_f, d_u);
+    SE2P_HIP_CHECK(hipSetDevice(plan->ctx->device));
+    return batch_apply(plan->ctx, plan->B, d_f, d_u);
 }
 
 int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u)
@@ -734,7 +749,7 @@ int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u)
         }
         SE2P_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         ctx->capturing = true;
-        int rc = geom_apply(ctx, plan->G, d_f, d_u);
+        int rc = batch_apply(ctx, plan->B, d_f, d_u);
         ctx->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -862,36 +877,21 @@ int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, do
     SE2P_TRY(need_ptr(f, Nsrc, "f"));
     if (!H1 || !H2) return set_error(SE2P_ERR_ARG, "null grid");
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
-    KGeom g;
-    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
-    const int ntiles = g.ntx * g.nty;
-    const size_t G = (size_t)Mx * My;
     double *dps, *df;
-    void *e1tab, *key, *perm, *start, *count, *node, *off, *part, *Z;
     SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_E1TAB, sizeof(double) * 64, &e1tab));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_KEY, sizeof(int) * (size_t)Nsrc, &key));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_PERM, sizeof(int) * (size_t)Nsrc, &perm));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &start));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &count));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_NODE, sizeof(int2) * (size_t)Nsrc, &node));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SPR_OFF, sizeof(double2) * (size_t)Nsrc, &off));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &part));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_A, sizeof(double2) * G, &Z));
-    SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
-    SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
-    SE2P_TRY(launch_tile_keys(s, dps, Nsrc, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
-                              (int*)count, (int2*)node, (double2*)off));
-    SE2P_TRY(sort_by_key(ctx, &ctx->ws, (int*)key, (int*)count, Nsrc, ntiles, (int*)perm, (int*)start,
-                         WS_SORT_TMP));
-    SE2P_TRY(launch_spread(s, 0, g, (int*)start, (int*)perm, (int2*)node, (double2*)off, df,
-                           nullptr, (double*)e1tab, (double*)part));
-    SE2P_TRY(launch_merge(s, 2, g, (double*)part, (double2*)Z, nullptr));
-    std::vector<double> h(2 * G);
-    SE2P_TRY(download(ctx, h.data(), (double*)Z, 2 * G));
-    for (size_t i = 0; i < G; ++i) {
+    // k-space geometry of the sources only (no targets); spreading and merge of (f1, f2)
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dps, dps, nullptr, Nsrc, 0, Lx, Ly,
+                        k_only(xi, eta, Mx, My, w, P), false, true));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    SE2P_TRY(batch_spread_grid(ctx, B, df, 0));
+    const size_t Gn = (size_t)Mx * My;
+    std::vector<double> h(2 * Gn);
+    SE2P_TRY(download(ctx, h.data(), (double*)G.Z0, 2 * Gn));
+    for (size_t i = 0; i < Gn; ++i) {
         H1[i] = h[2 * i];
         H2[i] = h[2 * i + 1];
     }
@@ -908,33 +908,23 @@ int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, do
     if (!H) return set_error(SE2P_ERR_ARG, "null grid");
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
     if (Ntar == 0) return SE2P_OK;
-    hipStream_t s = ctx->stream;
-    KGeom g;
-    SE2P_TRY(kgeom_init(&g, Mx, My, P, Lx, Ly, xi, eta, w));
-    const int ntiles = g.ntx * g.nty;
-    const size_t G = (size_t)Mx * My;
-    std::vector<double> wz(2 * G, 0.0);
-    for (size_t i = 0; i < G; ++i) wz[2 * i] = H[i];
-    double *dpt, *dW;
-    void *e1tab, *key, *perm, *start, *count, *node, *off, *u;
+    double* dpt;
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    SE2P_TRY(upload(ctx, WS_GRID_C, wz.data(), 2 * G, &dW));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_E1TAB, sizeof(double) * 64, &e1tab));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_KEY, sizeof(int) * (size_t)Ntar, &key));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_PERM, sizeof(int) * (size_t)Ntar, &perm));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &start));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &count));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_NODE, sizeof(int2) * (size_t)Ntar, &node));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GAT_OFF, sizeof(double2) * (size_t)Ntar, &off));
+    // k-space geometry of the targets only (no sources); the grid goes into W
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dpt, dpt, nullptr, 0, Ntar, Lx, Ly,
+                        k_only(xi, eta, Mx, My, w, P), false, true));
+    const size_t Gn = (size_t)Mx * My;
+    std::vector<double> wz(2 * Gn, 0.0);
+    for (size_t i = 0; i < Gn; ++i) wz[2 * i] = H[i];
+    SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wz.data(), sizeof(double) * 2 * Gn, hipMemcpyHostToDevice,
+                                  ctx->stream));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));  // synchronises: wz may die after
+    void* u;
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Ntar, &u));
-    SE2P_TRY(launch_e1_table(s, g, (double*)e1tab));
-    SE2P_TRY(launch_fill_int(s, (int*)count, ntiles, 0));
-    SE2P_TRY(launch_tile_keys(s, dpt, Ntar, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx, (int*)key,
-                              (int*)count, (int2*)node, (double2*)off));
-    SE2P_TRY(sort_by_key(ctx, &ctx->ws, (int*)key, (int*)count, Ntar, ntiles, (int*)perm, (int*)start,
-                         WS_SORT_TMP));
-    SE2P_TRY(launch_gather(s, g, (int*)start, (int*)perm, (int2*)node, (double2*)off,
-                           (double2*)dW, (double*)e1tab, nullptr, nullptr, (double*)u));
+    SE2P_TRY(batch_gather(ctx, B, (double*)u));
     std::vector<double> h(2 * (size_t)Ntar);
     SE2P_TRY(download(ctx, h.data(), (double*)u, 2 * (size_t)Ntar));
     for (int i = 0; i < Ntar; ++i) out[i] = h[2 * i];
@@ -946,24 +936,55 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
     if (Mx < 1 || My < 1 || !re || !im) return set_error(SE2P_ERR_ARG, "bad grid");
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    FftAxis ax, ay;
-    SE2P_TRY(fft_axis_init(&ax, Mx));
-    SE2P_TRY(fft_axis_init(&ay, My));
+    hipStream_t s = ctx->stream;
+    SysDesc d{};
+    d.Mx = Mx;
+    d.My = My;
+    d.Bx = fft_lines_per_block(Mx);
+    d.By = fft_lines_per_block(My);
+    SE2P_TRY(fft_axis_init(&d.ax, Mx));
+    SE2P_TRY(fft_axis_init(&d.ay, My));
     double2 *twx, *twy;
     SE2P_TRY(fft_twiddles(ctx, &ctx->ws, Mx, WS_TWIDDLE_X, &twx));
     SE2P_TRY(fft_twiddles(ctx, &ctx->ws, My, WS_TWIDDLE_Y, &twy));
-    const size_t G = (size_t)Mx * My;
-    std::vector<double> z(2 * G);
-    for (size_t i = 0; i < G; ++i) {
+    d.twx = twx;
+    d.twy = twy;
+    const size_t Gn = (size_t)Mx * My;
+    std::vector<double> z(2 * Gn);
+    for (size_t i = 0; i < Gn; ++i) {
         z[2 * i] = re[i];
         z[2 * i + 1] = im[i];
     }
     double* dz;
-    SE2P_TRY(upload(ctx, WS_GRID_A, z.data(), 2 * G, &dz));
-    SE2P_TRY(fft2d(ctx, (double2*)dz, Mx, My, ax, ay, twx, twy, inverse ? 1 : 0));
-    SE2P_TRY(download(ctx, z.data(), dz, 2 * G));
+    void* tmp;
+    SE2P_TRY(upload(ctx, WS_GRID_A, z.data(), 2 * Gn, &dz));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_C, sizeof(double2) * Gn, &tmp));
+    d.grid[G_Z0] = (double2*)dz;
+    d.grid[G_W] = (double2*)tmp;
+    // y lines of the column-major grid, transpose, x lines, transpose back
+    const int ny = (Mx + d.By - 1) / d.By, nx = (My + d.Bx - 1) / d.Bx;
+    const int t0 = transpose_tiles(Mx, My), t1 = transpose_tiles(My, Mx);
+    std::vector<Job> J;
+    for (int b = 0; b < ny; ++b) J.push_back(make_int2(0, b));
+    for (int b = 0; b < t0; ++b) J.push_back(make_int2(0, b));
+    for (int b = 0; b < nx; ++b) J.push_back(make_int2(0, b));
+    for (int b = 0; b < t1; ++b) J.push_back(make_int2(0, b));
+    void *psys, *pjobs;
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SYS, sizeof(SysDesc), &psys));
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_JOBS, sizeof(Job) * J.size(), &pjobs));
+    SE2P_HIP_CHECK(hipMemcpyAsync(psys, &d, sizeof(SysDesc), hipMemcpyHostToDevice, s));
+    SE2P_HIP_CHECK(hipMemcpyAsync(pjobs, J.data(), sizeof(Job) * J.size(), hipMemcpyHostToDevice, s));
+    SE2P_TRY(fft_prepare());
+    const SysDesc* ds = (const SysDesc*)psys;
+    const Job* dj = (const Job*)pjobs;
+    const int inv = inverse ? 1 : 0;
+    SE2P_TRY(launch_fft_lines(s, 0, G_Z0, inv, fft_lds_bytes(My), ds, dj, ny));
+    SE2P_TRY(launch_transpose(s, G_Z0, G_W, 0, ds, dj + ny, t0));
+    SE2P_TRY(launch_fft_lines(s, 1, G_W, inv, fft_lds_bytes(Mx), ds, dj + ny + t0, nx));
+    SE2P_TRY(launch_transpose(s, G_W, G_Z0, 1, ds, dj + ny + t0 + nx, t1));
+    SE2P_TRY(download(ctx, z.data(), dz, 2 * Gn));
     const double sc = inverse ? 1.0 / ((double)Mx * (double)My) : 1.0;
-    for (size_t i = 0; i < G; ++i) {
+    for (size_t i = 0; i < Gn; ++i) {
         re[i] = z[2 * i] * sc;
         im[i] = z[2 * i + 1] * sc;
     }

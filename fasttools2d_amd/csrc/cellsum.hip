@@ -110,10 +110,7 @@ __device__ __forceinline__ int round_index(int g, int r, int nround)
 // the candidate bookkeeping is wave-uniform (scalar registers, LDS broadcast).
 template <int KIND, int TIER, bool UNI>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_real_stencil(
-    int nsx, int nsy, int bs, int nbx, double Lx, double Ly, double xi2, double cut2,
-    double self, const int* __restrict__ tstart, const int* __restrict__ tperm,
-    const double2* __restrict__ tsorted, const int* __restrict__ sstart,
-    const typename SrcOf<KIND>::T* __restrict__ ssorted, double* __restrict__ u)
+    const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, double* __restrict__ u_base)
 {
     using Rec = typename SrcOf<KIND>::T;
     constexpr int CAP = SrcOf<KIND>::cap;
@@ -122,8 +119,18 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ double e_ox[kMaxEntries], e_oy[kMaxEntries];
     __shared__ int c_t0[kMaxBlockSide * kMaxBlockSide], c_off[kMaxBlockSide * kMaxBlockSide + 1];
 
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int nsx = d.nsx, nsy = d.nsy, bs = d.bs, nbx = d.nbx;
+    const double Lx = d.Lx, Ly = d.Ly, xi2 = d.xi2, cut2 = d.cut2, self = d.self;
+    const int* __restrict__ tstart = d.t_start;
+    const int* __restrict__ tperm = d.t_perm;
+    const double2* __restrict__ tsorted = d.t_sorted;
+    const int* __restrict__ sstart = d.s_start;
+    const Rec* __restrict__ ssorted = reinterpret_cast<const Rec*>(d.s_rec);
+    double* __restrict__ u = u_base + d.u_off;
     const int tid = threadIdx.x;
-    const int bx0 = (blockIdx.x % nbx) * bs, by0 = (blockIdx.x / nbx) * bs;
+    const int bx0 = (jb.y % nbx) * bs, by0 = (jb.y / nbx) * bs;
     const int bw = min(bs, nsx - bx0), bh = min(bs, nsy - by0);
     const int H = bh + 2;  // window column height; entry (ux, uy) is (ux+1) H + (uy+1)
     const int nent = (bw + 2) * H, ncell = bw * bh;
@@ -390,37 +397,32 @@ int launch_pack_targets(hipStream_t s, const int* perm, int n, const double* p, 
     return SE2P_OK;
 }
 
-int launch_real(hipStream_t s, int kind, int tier, int nsx, int nsy, double Lx, double Ly,
-                double xi, const int* tstart, const int* tperm, const double2* tsorted,
-                const int* sstart, const double* ssorted, int mean_targets, double* u)
+int real_block_side(int nsx, int nsy, int mean_targets)
 {
-    const double xi2 = xi * xi;
-    const double cut2 = Lx * Ly / nsx / nsy;
-    // -(1 + gamma/2 + ln xi) (mex_stokes_slp_real.cpp:124)
-    const double self = -(1.0 + 0.5772156649015328606 / 2 + log(xi));
     // blocks of bs x bs cells holding about 256 targets
     int bs = (int)std::floor(std::sqrt(256.0 / (mean_targets > 1 ? mean_targets : 1)));
     bs = std::max(1, std::min(bs, kMaxBlockSide));
-    bs = std::min(bs, std::min(nsx, nsy));
-    const int nbx = (nsx + bs - 1) / bs, nby = (nsy + bs - 1) / bs;
-    dim3 gr(nbx * nby), bl(kBlkThreads);
-#define SE2P_LAUNCH_REAL(K, T, REC)                                                              \
-    do {                                                                                         \
-        if (bs == 1)                                                                             \
-            hipLaunchKernelGGL((k_real_stencil<K, T, true>), gr, bl, 0, s, nsx, nsy, bs, nbx, Lx, \
-                               Ly, xi2, cut2, self, tstart, tperm, tsorted, sstart,              \
-                               (const REC*)ssorted, u);                                          \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_real_stencil<K, T, false>), gr, bl, 0, s, nsx, nsy, bs, nbx,   \
-                               Lx, Ly, xi2, cut2, self, tstart, tperm, tsorted, sstart,          \
-                               (const REC*)ssorted, u);                                          \
+    return std::min(bs, std::min(nsx, nsy));
+}
+
+int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys, const Job* jobs,
+                int njobs, double* u)
+{
+    if (njobs <= 0) return SE2P_OK;
+    dim3 gr(njobs), bl(kBlkThreads);
+#define SE2P_LAUNCH_REAL(K, T)                                                                  \
+    do {                                                                                        \
+        if (uni)                                                                                \
+            hipLaunchKernelGGL((k_real_stencil<K, T, true>), gr, bl, 0, s, sys, jobs, u);        \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_real_stencil<K, T, false>), gr, bl, 0, s, sys, jobs, u);       \
     } while (0)
     if (kind == SE2P_SLP && tier == kE1Full)
-        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full, double4);
+        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full);
     else if (kind == SE2P_SLP)
-        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast, double4);
+        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
     else
-        SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full, DlpRec);
+        SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full);
 #undef SE2P_LAUNCH_REAL
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;

@@ -27,7 +27,6 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 // v * (sgn i)
 __device__ __forceinline__ double2 rot(double2 v, double sgn) { return make_double2(-sgn * v.y, sgn * v.x); }
 
@@ -182,35 +181,33 @@ __device__ void stage_generic(const double2* __restrict__ src, double2* __restri
     }
 }
 
-__global__ __launch_bounds__(256) void k_fft_lines(double2* __restrict__ data, int n, int nlines,
-                                                   long elem_stride, long line_stride, int B,
-                                                   FftAxis ax, const double2* __restrict__ tw,
+// Job (s, b): lines b*B .. b*B + B - 1 of grid `sel` of system s, along y
+// (axis 0: My-lines, the column-major grid) or x (axis 1: Mx-lines, the
+// transposed grid); every line is contiguous.
+__global__ __launch_bounds__(256) void k_fft_lines(const SysDesc* __restrict__ sys,
+                                                   const Job* __restrict__ jobs, int axis, int sel,
                                                    int inverse)
 {
     extern __shared__ double2 lds[];
-    const int line0 = blockIdx.x * B;
-    const int nb = min(B, nlines - line0);
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int n = axis ? d.Mx : d.My;
+    const int nlines = axis ? d.My : d.Mx;
+    const int B = axis ? d.Bx : d.By;
+    const FftAxis& ax = axis ? d.ax : d.ay;
+    const double2* __restrict__ tw = axis ? d.twx : d.twy;
+    double2* __restrict__ data = d.grid[sel] + (size_t)jb.y * B * n;
+    const int nb = min(B, nlines - jb.y * B);
     const int total = nb * n;
     double2* src = lds;
     double2* dst = lds + (size_t)B * n;
-    const bool contiguous = (elem_stride == 1);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int b, e;
-        if (contiguous) {
-            b = idx / n;
-            e = idx % n;
-        } else {
-            b = idx % nb;
-            e = idx / nb;
-        }
-        src[b * n + e] = data[(line0 + b) * line_stride + e * elem_stride];
-    }
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) src[idx] = data[idx];
     __syncthreads();
     // the table holds exp(-2 pi i k/n); sgn is the sign of the transform's exponent
     const double sgn = inverse ? 1.0 : -1.0;
     int Ns = 1;
-    for (int s = 0; s < ax.nst; ++s) {
-        const int R = ax.radix[s];
+    for (int st = 0; st < ax.nst; ++st) {
+        const int R = ax.radix[st];
         switch (R) {
             case 2: stage<2>(src, dst, n, nb, Ns, tw, sgn); break;
             case 3: stage<3>(src, dst, n, nb, Ns, tw, sgn); break;
@@ -228,30 +225,40 @@ __global__ __launch_bounds__(256) void k_fft_lines(double2* __restrict__ data, i
         dst = tmp;
         Ns *= R;
     }
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int b, e;
-        if (contiguous) {
-            b = idx / n;
-            e = idx % n;
-        } else {
-            b = idx % nb;
-            e = idx / nb;
-        }
-        data[(line0 + b) * line_stride + e * elem_stride] = src[b * n + e];
-    }
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) data[idx] = src[idx];
 }
 
-int lines_per_block(int n, bool contiguous)
+// out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows
+// = Mx, cols = My (column-major grid to transposed), dir 1 the reverse.
+__global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ sys,
+                                                   const Job* __restrict__ jobs, int in_sel,
+                                                   int out_sel, int dir)
 {
-    if (contiguous) return n >= 256 ? 1 : (256 + n - 1) / n;
-    int B = kFftMaxLen / n;
-    if (B > 16) B = 16;
-    return B < 1 ? 1 : B;
+    __shared__ double2 tile[32][33];
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int rows = dir ? d.My : d.Mx, cols = dir ? d.Mx : d.My;
+    const double2* __restrict__ in = d.grid[in_sel];
+    double2* __restrict__ out = d.grid[out_sel];
+    const int ntc = (cols + 31) / 32;
+    const int r0 = (jb.y / ntc) * 32, c0 = (jb.y % ntc) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
+    }
 }
 
 }  // namespace
 
-int fft_prepare();
+int fft_lines_per_block(int n) { return n >= 256 ? 1 : (256 + n - 1) / n; }
+
+size_t fft_lds_bytes(int n) { return 2 * sizeof(double2) * (size_t)fft_lines_per_block(n) * n; }
 
 int fft_axis_init(FftAxis* a, int n)
 {
@@ -305,69 +312,24 @@ int fft_prepare()
     return SE2P_OK;
 }
 
-namespace {
-
-// out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS.
-__global__ __launch_bounds__(256) void k_transpose(const double2* __restrict__ in,
-                                                   double2* __restrict__ out, int rows, int cols)
+int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, size_t max_lds,
+                     const SysDesc* sys, const Job* jobs, int njobs)
 {
-    __shared__ double2 tile[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int k = ty; k < 32; k += 8) {
-        const int r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
-    }
-    __syncthreads();
-    for (int k = ty; k < 32; k += 8) {
-        const int c = c0 + k, r = r0 + tx;
-        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
-    }
-}
-
-}  // namespace
-
-int fft_lines_contig(se2p_ctx* ctx, double2* Z, int n, int count, const FftAxis& ax,
-                     const double2* tw, int inverse)
-{
-    if (!ctx->capturing) SE2P_TRY(fft_prepare());
-    const int B = lines_per_block(n, true);
-    const size_t lds = 2 * sizeof(double2) * (size_t)B * n;
-    hipLaunchKernelGGL(k_fft_lines, dim3((count + B - 1) / B), dim3(256), lds, ctx->stream, Z, n,
-                       count, 1L, (long)n, B, ax, tw, inverse);
+    if (njobs <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_fft_lines, dim3(njobs), dim3(256), max_lds, s, sys, jobs, axis, sel,
+                       inverse);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int transpose(se2p_ctx* ctx, const double2* in, double2* out, int rows, int cols)
-{
-    hipLaunchKernelGGL(k_transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0,
-                       ctx->stream, in, out, rows, cols);
-    SE2P_HIP_CHECK(hipGetLastError());
-    return SE2P_OK;
-}
+int transpose_tiles(int rows, int cols) { return ((rows + 31) / 32) * ((cols + 31) / 32); }
 
-int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
-          const double2* twx, const double2* twy, int inverse)
+int launch_transpose(hipStream_t s, int in, int out, int dir, const SysDesc* sys,
+                     const Job* jobs, int njobs)
 {
-    if (!ctx->capturing) SE2P_TRY(fft_prepare());
-    hipStream_t s = ctx->stream;
-    // y lines: contiguous, Mx of them
-    {
-        const int B = lines_per_block(My, true);
-        const size_t lds = 2 * sizeof(double2) * (size_t)B * My;
-        hipLaunchKernelGGL(k_fft_lines, dim3((Mx + B - 1) / B), dim3(256), lds, s, Z, My, Mx, 1L,
-                           (long)My, B, ay, twy, inverse);
-        SE2P_HIP_CHECK(hipGetLastError());
-    }
-    // x lines: stride My, My of them
-    {
-        const int B = lines_per_block(Mx, false);
-        const size_t lds = 2 * sizeof(double2) * (size_t)B * Mx;
-        hipLaunchKernelGGL(k_fft_lines, dim3((My + B - 1) / B), dim3(256), lds, s, Z, Mx, My,
-                           (long)My, 1L, B, ax, twx, inverse);
-        SE2P_HIP_CHECK(hipGetLastError());
-    }
+    if (njobs <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_transpose, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out, dir);
+    SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 

@@ -45,16 +45,22 @@ __device__ __forceinline__ void point_density(int mode, int p, const double* __r
 }
 
 template <int BLK>
-__global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__ tstart,
-                                               const int* __restrict__ perm,
-                                               const int2* __restrict__ node,
-                                               const double2* __restrict__ off,
-                                               const double* __restrict__ f,
-                                               const double* __restrict__ nrm, int P, int TX,
-                                               int TY, int ntx, double t, double w, double h,
-                                               const double* __restrict__ e1tab,
-                                               double* __restrict__ partial)
+__global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
+                                               const Job* __restrict__ jobs, int c,
+                                               const double* __restrict__ f_base)
 {
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int mode = d.kind == SE2P_SLP ? 0 : 1 + c;
+    const int* __restrict__ tstart = d.sp_start;
+    const int* __restrict__ perm = d.sp_perm;
+    const int2* __restrict__ node = d.sp_node;
+    const double2* __restrict__ off = d.sp_off;
+    const double* __restrict__ f = f_base + d.f_off;
+    const double* __restrict__ nrm = d.nrm;
+    const int P = d.P, TX = d.TX, TY = d.TY, ntx = d.ntx;
+    const double t = d.t, w = d.w, h = d.h;
+    const double* __restrict__ e1tab = d.e1tab;
     constexpr int R = 8 * BLK;
     constexpr int KC = 16;
     __shared__ double A[KC][R];
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__
     __shared__ double B1[KC][R];
     __shared__ double e1s[kMaxP + 1];
 
-    const int tile = blockIdx.x;
+    const int tile = jb.y;
     const int tx = tile % ntx, ty = tile / ntx;
     const int lane = threadIdx.x;
     const int bi = (lane >> 3) * BLK, bj = (lane & 7) * BLK;
@@ -122,7 +128,7 @@ __global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__
                 }
         }
     }
-    double* base = partial + (size_t)tile * 2 * R * R;
+    double* base = d.partial + (size_t)tile * 2 * R * R;
 #pragma unroll
     for (int i = 0; i < BLK; ++i)
 #pragma unroll
@@ -134,14 +140,19 @@ __global__ __launch_bounds__(64) void k_spread(int mode, const int* __restrict__
 
 // Node-parallel halo merge: node (ix, iy) adds, in a fixed order, every tile
 // region that covers it (regions wrap periodically, possibly more than once
-// on grids smaller than R).  One block row per ix; indices wrap by a single
-// conditional add (tile and node differences stay within one period).
-__global__ void k_merge(const double* __restrict__ partial, int Mx, int My, int TX, int TY,
-                        int ntx, int nty, int R, double2* __restrict__ Z)
+// on grids smaller than R).  Job b covers node column ix = b / ceil(My/256)
+// and 256 nodes of it; indices wrap by a single conditional add (tile and node
+// differences stay within one period).
+__global__ void k_merge(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, int out)
 {
-    const int ix = blockIdx.y;
-    const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int Mx = d.Mx, My = d.My, TX = d.TX, TY = d.TY, ntx = d.ntx, nty = d.nty, R = d.R;
+    const int nyb = (My + 255) / 256;
+    const int ix = jb.y / nyb;
+    const int iy = (jb.y - ix * nyb) * 256 + threadIdx.x;
     if (iy >= My) return;
+    const double* __restrict__ partial = d.partial;
     const int ndx = min(ntx, R / TX + 2), ndy = min(nty, R / TY + 2);
     const int bx = ix / TX, by = iy / TY;
     double s0 = 0, s1 = 0;
@@ -164,7 +175,7 @@ __global__ void k_merge(const double* __restrict__ partial, int Mx, int My, int 
             }
         }
     }
-    Z[(size_t)ix * My + iy] = make_double2(s0, s1);
+    d.grid[out][(size_t)ix * My + iy] = make_double2(s0, s1);
 }
 
 // Wavenumber of grid index j: indices up to M/2 non-negative
@@ -224,29 +235,29 @@ __device__ __forceinline__ void dlp_bracket(double k1, double k2, double a, doub
 // inverse FFT returns g1 + i g2.  G(-k) uses the reference's wavenumbers at
 // index -k, which differ from -k only on Nyquist lines.
 template <int KIND>
-__global__ void k_filter(const double2* __restrict__ Z0, const double2* __restrict__ Z1,
-                         double2* __restrict__ W, int Mx, int My, double Lx, double Ly,
-                         double xi, double eta, double scale, int rowmajor)
+__global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
 {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int Mx = d.Mx, My = d.My;
+    const long gid = (long)jb.y * blockDim.x + threadIdx.x;
     if (gid >= (long)Mx * My) return;
-    int ix, iy;
-    if (rowmajor) {
-        iy = (int)(gid / Mx);
-        ix = (int)(gid % Mx);
-    } else {
-        ix = (int)(gid / My);
-        iy = (int)(gid % My);
-    }
+    // transposed layout Z[iy*Mx + ix]
+    const int iy = (int)(gid / Mx);
+    const int ix = (int)(gid - (long)iy * Mx);
+    const double2* __restrict__ Z0 = d.grid[G_W];
+    const double2* __restrict__ Z1 = d.grid[G_D];
+    double2* __restrict__ W = d.grid[G_Z0];
     if (ix == 0 && iy == 0) {
         W[gid] = make_double2(0.0, 0.0);
         return;
     }
-    const int jx = (Mx - ix) % Mx, jy = (My - iy) % My;
-    const long gm = rowmajor ? (long)jy * Mx + jx : (long)jx * My + jy;
-    const double k1 = wavenum(ix, Mx, Lx), k2 = wavenum(iy, My, Ly);
-    const double m1 = wavenum(jx, Mx, Lx), m2 = wavenum(jy, My, Ly);
-    const double a = 0.25 / (xi * xi);
+    const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
+    const long gm = (long)jy * Mx + jx;
+    const double k1 = wavenum(ix, Mx, d.Lx), k2 = wavenum(iy, My, d.Ly);
+    const double m1 = wavenum(jx, Mx, d.Lx), m2 = wavenum(jy, My, d.Ly);
+    const double a = 0.25 / (d.xi * d.xi), eta = d.eta;
+    const double scale = 1.0 / ((double)Mx * (double)My);
     cplx g1, g2;
     if (KIND == SE2P_SLP) {
         cplx q1, q2;
@@ -273,20 +284,26 @@ __global__ void k_filter(const double2* __restrict__ Z0, const double2* __restri
     W[gid] = make_double2((g1.re - g2.im) * scale, (g1.im + g2.re) * scale);
 }
 
-__global__ __launch_bounds__(64) void k_gather(const int* __restrict__ tstart,
-                                               const int* __restrict__ perm,
-                                               const int2* __restrict__ node,
-                                               const double2* __restrict__ off,
-                                               const double2* __restrict__ W, int Mx, int My,
-                                               int TX, int TY, int ntx, int P, int R, double t,
-                                               double w, double h,
-                                               const double* __restrict__ e1tab, double scale,
-                                               const double* add_first,
-                                               const double* __restrict__ zm, double* u)
+__global__ __launch_bounds__(64) void k_gather(const SysDesc* __restrict__ sys,
+                                               const Job* __restrict__ jobs,
+                                               double* __restrict__ u_base, int accumulate,
+                                               int use_zm)
 {
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int* __restrict__ tstart = d.gt_start;
+    const int* __restrict__ perm = d.gt_perm;
+    const int2* __restrict__ node = d.gt_node;
+    const double2* __restrict__ off = d.gt_off;
+    const double2* __restrict__ W = d.grid[G_W];
+    const int Mx = d.Mx, My = d.My, TX = d.TX, TY = d.TY, ntx = d.ntx, P = d.P, R = d.R;
+    const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
+    const double* __restrict__ e1tab = d.e1tab;
+    const double* zm = use_zm ? d.zm : nullptr;
+    double* u = u_base + d.u_off;
     extern __shared__ double2 reg[];
     __shared__ double e1s[kMaxP + 1];
-    const int tile = blockIdx.x;
+    const int tile = jb.y;
     const int tx = tile % ntx, ty = tile / ntx;
     const int X0 = tx * TX, Y0 = ty * TY;
     const int k0 = tstart[tile], k1 = tstart[tile + 1];
@@ -321,9 +338,9 @@ __global__ __launch_bounds__(64) void k_gather(const int* __restrict__ tstart,
             ex *= e3x;
         }
         double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
-        if (add_first) {
-            v0 = add_first[2 * p] + v0;
-            v1 = add_first[2 * p + 1] + v1;
+        if (accumulate) {
+            v0 = u[2 * p] + v0;
+            v1 = u[2 * p + 1] + v1;
         }
         u[2 * p] = v0;
         u[2 * p + 1] = v1;
@@ -340,41 +357,19 @@ __global__ void k_e1_table(double xi, double eta, double h, int P, double* tab)
     tab[j] = exp(c * q * q);
 }
 
-// Deterministic two-level reduction of sum_s (f.n)_s (x_s, y_s).
-__global__ __launch_bounds__(256) void k_zero_mode_part(const double* __restrict__ p,
-                                                        const double* __restrict__ f,
-                                                        const double* __restrict__ nrm, int n,
-                                                        double* __restrict__ part)
+// Deterministic two-level reduction of sum_s (f.n)_s (x_s, y_s) over the
+// sorted DLP records {x, y, f1, f2, n1, n2, f.n, 0}: job (s, b) sums records
+// b*256 + tid + k zm_nblk*256 of system s; then one job per system adds the
+// zm_nblk partials in order and divides by the box area.
+__global__ __launch_bounds__(256) void k_zero_mode_part(const SysDesc* __restrict__ sys,
+                                                        const Job* __restrict__ jobs)
 {
     __shared__ double sx[256], sy[256];
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const double* __restrict__ rec = d.s_rec;
     double ax = 0, ay = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double fn = f[2 * i] * nrm[2 * i] + f[2 * i + 1] * nrm[2 * i + 1];
-        ax += fn * p[2 * i];
-        ay += fn * p[2 * i + 1];
-    }
-    sx[threadIdx.x] = ax;
-    sy[threadIdx.x] = ay;
-    __syncthreads();
-    for (int d = 128; d > 0; d >>= 1) {
-        if ((int)threadIdx.x < d) {
-            sx[threadIdx.x] += sx[threadIdx.x + d];
-            sy[threadIdx.x] += sy[threadIdx.x + d];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sx[0];
-        part[2 * blockIdx.x + 1] = sy[0];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_zero_mode_rec_part(const double* __restrict__ rec, int n,
-                                                            double* __restrict__ part)
-{
-    __shared__ double sx[256], sy[256];
-    double ax = 0, ay = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = jb.y * 256 + threadIdx.x; i < d.Ns; i += d.zm_nblk * 256) {
         const double* r = rec + 8 * (size_t)i;
         ax += r[6] * r[0];
         ay += r[6] * r[1];
@@ -382,30 +377,56 @@ __global__ __launch_bounds__(256) void k_zero_mode_rec_part(const double* __rest
     sx[threadIdx.x] = ax;
     sy[threadIdx.x] = ay;
     __syncthreads();
-    for (int d = 128; d > 0; d >>= 1) {
-        if ((int)threadIdx.x < d) {
-            sx[threadIdx.x] += sx[threadIdx.x + d];
-            sy[threadIdx.x] += sy[threadIdx.x + d];
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            sx[threadIdx.x] += sx[threadIdx.x + k];
+            sy[threadIdx.x] += sy[threadIdx.x + k];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sx[0];
-        part[2 * blockIdx.x + 1] = sy[0];
+        d.zm_part[2 * jb.y] = sx[0];
+        d.zm_part[2 * jb.y + 1] = sy[0];
     }
 }
 
-__global__ void k_zero_mode_final(const double* __restrict__ part, int nblk, double inv_area,
-                                  double* __restrict__ out)
+__global__ void k_zero_mode_final(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
 {
     if (threadIdx.x != 0) return;
+    const SysDesc& d = sys[jobs[blockIdx.x].x];
     double sx = 0, sy = 0;
-    for (int b = 0; b < nblk; ++b) {
-        sx += part[2 * b];
-        sy += part[2 * b + 1];
+    for (int b = 0; b < d.zm_nblk; ++b) {
+        sx += d.zm_part[2 * b];
+        sy += d.zm_part[2 * b + 1];
     }
-    out[0] = sx * inv_area;
-    out[1] = sy * inv_area;
+    const double inv_area = 1.0 / (d.Lx * d.Ly);
+    d.zm[0] = sx * inv_area;
+    d.zm[1] = sy * inv_area;
+}
+
+// Density fill of the sorted source records (positions and normals are set at
+// build): SLP {x, y, f1, f2}; DLP {x, y, f1, f2, n1, n2, f.n, 0}.
+template <int KIND>
+__global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs,
+                            const double* __restrict__ f_base)
+{
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int i = jb.y * 256 + threadIdx.x;
+    if (i >= d.Ns) return;
+    const double* __restrict__ f = f_base + d.f_off;
+    const int o = d.s_perm[i];
+    const double f1 = f[2 * o], f2 = f[2 * o + 1];
+    if (KIND == SE2P_SLP) {
+        double4* rec = reinterpret_cast<double4*>(d.s_rec);
+        rec[i].z = f1;
+        rec[i].w = f2;
+    } else {
+        double* r = d.s_rec + 8 * (size_t)i;
+        r[2] = f1;
+        r[3] = f2;
+        r[6] = f1 * r[4] + f2 * r[5];
+    }
 }
 
 }  // namespace
@@ -438,85 +459,66 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
     return SE2P_OK;
 }
 
-int launch_spread(hipStream_t s, int mode, const KGeom& g, const int* tstart, const int* perm,
-                  const int2* node, const double2* off, const double* f, const double* nrm,
-                  const double* e1tab, double* partial)
+int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jobs, int njobs,
+                  const double* f)
 {
-    const int ntiles = g.ntx * g.nty;
-    const double t = -2 * g.xi * g.xi / g.eta;
-    if (g.R == 32)
-        hipLaunchKernelGGL(k_spread<4>, dim3(ntiles), dim3(64), 0, s, mode, tstart, perm, node,
-                           off, f, nrm, g.P, g.TX, g.TY, g.ntx, t, g.w, g.h, e1tab, partial);
+    if (njobs <= 0) return SE2P_OK;
+    if (R == 32)
+        hipLaunchKernelGGL(k_spread<4>, dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);
     else
-        hipLaunchKernelGGL(k_spread<6>, dim3(ntiles), dim3(64), 0, s, mode, tstart, perm, node,
-                           off, f, nrm, g.P, g.TX, g.TY, g.ntx, t, g.w, g.h, e1tab, partial);
+        hipLaunchKernelGGL(k_spread<6>, dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_merge(hipStream_t s, int /*ncomp*/, const KGeom& g, const double* partial,
-                 double2* Z0, double2* /*Z1*/)
+int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs)
 {
-    hipLaunchKernelGGL(k_merge, dim3((unsigned)((g.My + 255) / 256), (unsigned)g.Mx), dim3(256), 0,
-                       s, partial, g.Mx, g.My, g.TX, g.TY, g.ntx, g.nty, g.R, Z0);
+    if (njobs <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_merge, dim3(njobs), dim3(256), 0, s, sys, jobs, out);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_filter(hipStream_t s, int kind, const KGeom& g, const double2* Z0,
-                  const double2* Z1, double2* W, int rowmajor)
+int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs)
 {
-    const long n = (long)g.Mx * g.My;
-    const double scale = 1.0 / ((double)g.Mx * (double)g.My);
-    dim3 gr((unsigned)((n + 255) / 256)), b(256);
+    if (njobs <= 0) return SE2P_OK;
     if (kind == SE2P_SLP)
-        hipLaunchKernelGGL(k_filter<SE2P_SLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
-                           g.xi, g.eta, scale, rowmajor);
+        hipLaunchKernelGGL(k_filter<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
     else
-        hipLaunchKernelGGL(k_filter<SE2P_DLP>, gr, b, 0, s, Z0, Z1, W, g.Mx, g.My, g.Lx, g.Ly,
-                           g.xi, g.eta, scale, rowmajor);
+        hipLaunchKernelGGL(k_filter<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_gather(hipStream_t s, const KGeom& g, const int* tstart, const int* perm,
-                  const int2* node, const double2* off, const double2* W, const double* e1tab,
-                  const double* add_first, const double* zero_mode, double* u)
+int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
+                  double* u, int accumulate, int use_zm)
 {
-    const int ntiles = g.ntx * g.nty;
-    const double t = -2 * g.xi * g.xi / g.eta;
-    // (4 xi^2/eta)^2 h^2/pi/(4 pi)  (ewald_tools.cpp:267-269)
-    double sc = 4 * g.xi * g.xi / g.eta;
-    sc = sc * sc * g.h * g.h / kPi;
-    sc = sc / (4 * kPi);
-    const size_t lds = sizeof(double2) * (size_t)g.R * g.R;
-    hipLaunchKernelGGL(k_gather, dim3(ntiles), dim3(64), lds, s, tstart, perm, node, off, W, g.Mx,
-                       g.My, g.TX, g.TY, g.ntx, g.P, g.R, t, g.w, g.h, e1tab, sc, add_first,
-                       zero_mode, u);
+    if (njobs <= 0) return SE2P_OK;
+    const size_t lds = sizeof(double2) * (size_t)maxR * maxR;
+    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(64), lds, s, sys, jobs, u, accumulate, use_zm);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_dlp_zero_mode(hipStream_t s, const double* p, const double* f, const double* nrm,
-                         int n, double Lx, double Ly, double* partial, double* out)
+int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, int npart,
+                     const Job* sys_jobs, int nsys)
 {
-    constexpr int nblk = 128;
-    hipLaunchKernelGGL(k_zero_mode_part, dim3(nblk), dim3(256), 0, s, p, f, nrm, n, partial);
+    if (npart <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_zero_mode_part, dim3(npart), dim3(256), 0, s, sys, part_jobs);
     SE2P_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_zero_mode_final, dim3(1), dim3(64), 0, s, partial, nblk,
-                       1.0 / (Lx * Ly), out);
+    hipLaunchKernelGGL(k_zero_mode_final, dim3(nsys), dim3(64), 0, s, sys, sys_jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_dlp_zero_mode_rec(hipStream_t s, const double* rec, int n, double Lx, double Ly,
-                             double* partial, double* out)
+int launch_fill_dens(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs,
+                     const double* f)
 {
-    constexpr int nblk = 128;
-    hipLaunchKernelGGL(k_zero_mode_rec_part, dim3(nblk), dim3(256), 0, s, rec, n, partial);
-    SE2P_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_zero_mode_final, dim3(1), dim3(64), 0, s, partial, nblk,
-                       1.0 / (Lx * Ly), out);
+    if (njobs <= 0) return SE2P_OK;
+    if (kind == SE2P_SLP)
+        hipLaunchKernelGGL(k_fill_dens<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
+    else
+        hipLaunchKernelGGL(k_fill_dens<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

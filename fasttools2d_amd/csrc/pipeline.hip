@@ -15,33 +15,13 @@
 #include "se2p_internal.h"
 #include "e1_eval.h"
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 namespace se2p {
 
 namespace {
-
-__global__ void k_fill_slp_dens(const int* __restrict__ perm, int n, const double* __restrict__ f,
-                                double4* __restrict__ rec)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int o = perm[i];
-    rec[i].z = f[2 * o];
-    rec[i].w = f[2 * o + 1];
-}
-
-// DLP records {x, y, f1, f2, n1, n2, f.n, 0} (8 doubles): fill f1, f2, f.n.
-__global__ void k_fill_dlp_dens(const int* __restrict__ perm, int n, const double* __restrict__ f,
-                                double* __restrict__ rec)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int o = perm[i];
-    double* r = rec + 8 * (size_t)i;
-    const double f1 = f[2 * o], f2 = f[2 * o + 1];
-    r[2] = f1;
-    r[3] = f2;
-    r[6] = f1 * r[4] + f2 * r[5];
-}
 
 __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, long n)
 {
@@ -182,80 +162,266 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
     return SE2P_OK;
 }
 
-int geom_apply_real(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u)
+void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
 {
-    hipStream_t s = ctx->stream;
-    if (G.Nt == 0) return SE2P_OK;
-    {
-        StageTimer st(ctx, "pack");
-        if (G.Ns > 0) {
-            dim3 gr((G.Ns + 255) / 256), b(256);
-            if (G.kind == SE2P_SLP)
-                hipLaunchKernelGGL(k_fill_slp_dens, gr, b, 0, s, G.s_perm, G.Ns, d_f,
-                                   (double4*)G.s_rec);
-            else
-                hipLaunchKernelGGL(k_fill_dlp_dens, gr, b, 0, s, G.s_perm, G.Ns, d_f, G.s_rec);
-            SE2P_HIP_CHECK(hipGetLastError());
-        }
-        if (G.kind == SE2P_DLP)
-            SE2P_TRY(launch_dlp_zero_mode_rec(s, G.s_rec, G.Ns, G.Lx, G.Ly, G.zm_part, G.zm));
+    *d = SysDesc{};
+    d->kind = G.kind;
+    d->Ns = G.Ns;
+    d->Nt = G.Nt;
+    d->e1tier = G.e1tier;
+    d->f_off = f_off;
+    d->u_off = u_off;
+    d->Lx = G.Lx;
+    d->Ly = G.Ly;
+    d->nrm = G.nrm;
+    if (G.has_real) {
+        const int nsx = G.prm.nside_x, nsy = G.prm.nside_y;
+        const double xi = G.prm.xi;
+        d->nsx = nsx;
+        d->nsy = nsy;
+        d->bs = real_block_side(nsx, nsy, (G.Nt + G.nbox - 1) / G.nbox);
+        d->nbx = (nsx + d->bs - 1) / d->bs;
+        d->xi2 = xi * xi;
+        d->cut2 = G.Lx * G.Ly / nsx / nsy;
+        // -(1 + gamma/2 + ln xi) (mex_stokes_slp_real.cpp:124)
+        d->self = -(1.0 + 0.5772156649015328606 / 2 + log(xi));
+        d->t_start = G.t_start;
+        d->t_perm = G.t_perm;
+        d->s_start = G.s_start;
+        d->s_perm = G.s_perm;
+        d->t_sorted = G.t_sorted;
+        d->s_rec = G.s_rec;
+        d->zm = G.zm;
+        d->zm_part = G.zm_part;
+        d->zm_nblk = std::min(128, std::max(1, (G.Ns + 255) / 256));
     }
-    StageTimer st(ctx, "real");
-    return launch_real(s, G.kind, G.e1tier, G.prm.nside_x, G.prm.nside_y, G.Lx, G.Ly, G.prm.xi,
-                       G.t_start, G.t_perm, G.t_sorted, G.s_start, G.s_rec,
-                       (G.Nt + G.nbox - 1) / G.nbox, d_u);
+    if (G.has_k) {
+        const KGeom& g = G.kg;
+        d->Mx = g.Mx;
+        d->My = g.My;
+        d->P = g.P;
+        d->TX = g.TX;
+        d->TY = g.TY;
+        d->ntx = g.ntx;
+        d->nty = g.nty;
+        d->R = g.R;
+        d->Bx = fft_lines_per_block(g.Mx);
+        d->By = fft_lines_per_block(g.My);
+        d->t = -2 * g.xi * g.xi / g.eta;
+        d->w = g.w;
+        d->h = g.h;
+        d->xi = g.xi;
+        d->eta = g.eta;
+        // (4 xi^2/eta)^2 h^2/pi/(4 pi)  (ewald_tools.cpp:267-269)
+        double sc = 4 * g.xi * g.xi / g.eta;
+        sc = sc * sc * g.h * g.h / kPi;
+        d->gscale = sc / (4 * kPi);
+        d->sp_start = G.sp_start;
+        d->sp_perm = G.sp_perm;
+        d->gt_start = G.gt_start;
+        d->gt_perm = G.gt_perm;
+        d->sp_node = G.sp_node;
+        d->gt_node = G.gt_node;
+        d->sp_off = G.sp_off;
+        d->gt_off = G.gt_off;
+        d->e1tab = G.e1tab;
+        d->partial = G.partial;
+        d->grid[G_Z0] = G.Z0;
+        d->grid[G_Z1] = G.Z1;
+        d->grid[G_W] = G.W;
+        d->grid[G_D] = G.D;
+        d->twx = G.twx;
+        d->twy = G.twy;
+        d->ax = G.ax;
+        d->ay = G.ay;
+    }
 }
 
-int geom_apply_k(se2p_ctx* ctx, const Geom& G, const double* d_f, const double* d_add_first,
-                 const double* d_zm, double* d_u)
+int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* geoms, int nsys)
+{
+    *B = BatchExec{};
+    B->nsys = nsys;
+    std::vector<SysDesc> hs(nsys);
+    long foff = 0, uoff = 0;
+    for (int i = 0; i < nsys; ++i) {
+        const Geom& G = *geoms[i];
+        sysdesc_from_geom(G, foff, uoff, &hs[i]);
+        foff += 2L * G.Ns;
+        uoff += 2L * G.Nt;
+        B->has_real |= G.has_real;
+        B->has_k |= G.has_k;
+        B->any_dlp |= G.kind == SE2P_DLP;
+        if (G.has_k) {
+            B->lds_y = std::max(B->lds_y, fft_lds_bytes(G.kg.My));
+            B->lds_x = std::max(B->lds_x, fft_lds_bytes(G.kg.Mx));
+            B->maxR = std::max(B->maxR, G.kg.R);
+        }
+    }
+    std::vector<Job> J;
+    auto add = [&](JobRange& r, auto&& gen) {
+        r.first = (int)J.size();
+        for (int i = 0; i < nsys; ++i) gen(i, hs[i]);
+        r.count = (int)J.size() - r.first;
+    };
+    auto blocks = [&](int i, long n) {
+        for (long b = 0; b < n; ++b) J.push_back(make_int2(i, (int)b));
+    };
+    for (int kind = 0; kind < 2; ++kind) {
+        add(B->fill[kind], [&](int i, const SysDesc& d) {
+            if (geoms[i]->has_real && d.kind == kind) blocks(i, (d.Ns + 255) / 256);
+        });
+        add(B->filter[kind], [&](int i, const SysDesc& d) {
+            if (geoms[i]->has_k && d.kind == kind) blocks(i, ((long)d.Mx * d.My + 255) / 256);
+        });
+        for (int tier = 0; tier < 2; ++tier)
+            for (int uni = 0; uni < 2; ++uni)
+                add(B->real[kind][tier][uni], [&](int i, const SysDesc& d) {
+                    if (geoms[i]->has_real && d.kind == kind && d.e1tier == tier &&
+                        (d.bs == 1) == (uni == 1) && d.Nt > 0)
+                        blocks(i, (long)d.nbx * ((d.nsy + d.bs - 1) / d.bs));
+                });
+    }
+    add(B->zm_part, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_real && d.kind == SE2P_DLP) blocks(i, d.zm_nblk);
+    });
+    add(B->zm_sys, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_real && d.kind == SE2P_DLP) blocks(i, 1);
+    });
+    for (int c = 0; c < 2; ++c) {
+        auto want = [&](int i, const SysDesc& d) {
+            return geoms[i]->has_k && (c == 0 || d.kind == SE2P_DLP);
+        };
+        for (int big = 0; big < 2; ++big)
+            add(B->spread[c][big], [&](int i, const SysDesc& d) {
+                if (want(i, d) && (d.R == 48) == (big == 1)) blocks(i, (long)d.ntx * d.nty);
+            });
+        add(B->merge[c], [&](int i, const SysDesc& d) {
+            if (want(i, d)) blocks(i, (long)d.Mx * ((d.My + 255) / 256));
+        });
+        add(B->ffty[c], [&](int i, const SysDesc& d) {
+            if (want(i, d)) blocks(i, (d.Mx + d.By - 1) / d.By);
+        });
+        add(B->tr_fwd[c], [&](int i, const SysDesc& d) {
+            if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
+        });
+        add(B->fftx[c], [&](int i, const SysDesc& d) {
+            if (want(i, d)) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+        });
+    }
+    add(B->ifftx, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_k) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+    });
+    add(B->tr_inv, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_k) blocks(i, transpose_tiles(d.My, d.Mx));
+    });
+    add(B->iffty, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_k) blocks(i, (d.Mx + d.By - 1) / d.By);
+    });
+    add(B->gather, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_k && d.Nt > 0) blocks(i, (long)d.ntx * d.nty);
+    });
+    if (B->has_k) SE2P_TRY(fft_prepare());
+    void *psys, *pjobs;
+    SE2P_TRY(ws_get(ctx, ws, WS_SYS, sizeof(SysDesc) * (size_t)std::max(1, nsys), &psys));
+    SE2P_TRY(ws_get(ctx, ws, WS_JOBS, sizeof(Job) * std::max<size_t>(1, J.size()), &pjobs));
+    B->sys = (SysDesc*)psys;
+    B->jobs = (Job*)pjobs;
+    hipStream_t s = ctx->stream;
+    if (nsys)
+        SE2P_HIP_CHECK(hipMemcpyAsync(B->sys, hs.data(), sizeof(SysDesc) * (size_t)nsys,
+                                      hipMemcpyHostToDevice, s));
+    if (!J.empty())
+        SE2P_HIP_CHECK(hipMemcpyAsync(B->jobs, J.data(), sizeof(Job) * J.size(),
+                                      hipMemcpyHostToDevice, s));
+    // the host vectors die on return: wait for the copies
+    SE2P_HIP_CHECK(hipStreamSynchronize(s));
+    return SE2P_OK;
+}
+
+int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u)
 {
     hipStream_t s = ctx->stream;
-    if (G.Nt == 0) return SE2P_OK;
-    const KGeom& g = G.kg;
+    const Job* J = B.jobs;
+    {
+        StageTimer st(ctx, "pack");
+        for (int kind = 0; kind < 2; ++kind)
+            SE2P_TRY(launch_fill_dens(s, kind, B.sys, J + B.fill[kind].first, B.fill[kind].count,
+                                      d_f));
+        SE2P_TRY(launch_zero_mode(s, B.sys, J + B.zm_part.first, B.zm_part.count,
+                                  J + B.zm_sys.first, B.zm_sys.count));
+    }
+    StageTimer st(ctx, "real");
+    for (int kind = 0; kind < 2; ++kind)
+        for (int tier = 0; tier < 2; ++tier)
+            for (int uni = 0; uni < 2; ++uni) {
+                const JobRange& r = B.real[kind][tier][uni];
+                SE2P_TRY(launch_real(s, kind, tier, uni == 1, B.sys, J + r.first, r.count, d_u));
+            }
+    return SE2P_OK;
+}
+
+int batch_spread_grid(se2p_ctx* ctx, const BatchExec& B, const double* d_f, int c)
+{
+    hipStream_t s = ctx->stream;
+    const Job* J = B.jobs;
+    {
+        StageTimer st(ctx, "spread");
+        for (int big = 0; big < 2; ++big)
+            SE2P_TRY(launch_spread(s, big ? 48 : 32, c, B.sys, J + B.spread[c][big].first,
+                                   B.spread[c][big].count, d_f));
+    }
+    StageTimer st(ctx, "merge");
+    return launch_merge(s, c ? G_Z1 : G_Z0, B.sys, J + B.merge[c].first, B.merge[c].count);
+}
+
+int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
+{
+    StageTimer st(ctx, "gather");
+    return launch_gather(ctx->stream, B.maxR, B.sys, B.jobs + B.gather.first, B.gather.count, d_u,
+                         0, 0);
+}
+
+int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
+                  bool accumulate, bool use_zm)
+{
+    hipStream_t s = ctx->stream;
+    const Job* J = B.jobs;
     // Every FFT pass runs on contiguous lines: y lines in the spread layout
     // (Z[ix*My + iy]), then a transpose to Z[iy*Mx + ix] for the x lines, the
     // filter in that layout, and back.
     //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
-    const int nspread = G.kind == SE2P_SLP ? 1 : 2;
-    for (int c = 0; c < nspread; ++c) {
-        double2* Zc = c == 0 ? G.Z0 : G.Z1;
-        double2* Tc = c == 0 ? G.W : G.D;
-        {
-            StageTimer st(ctx, "spread");
-            SE2P_TRY(launch_spread(s, G.kind == SE2P_SLP ? 0 : 1 + c, g, G.sp_start, G.sp_perm,
-                                   G.sp_node, G.sp_off, d_f, G.nrm, G.e1tab, G.partial));
-        }
-        {
-            StageTimer st(ctx, "merge");
-            SE2P_TRY(launch_merge(s, 2, g, G.partial, Zc, nullptr));
-        }
-        {
-            StageTimer st(ctx, "fft");
-            SE2P_TRY(fft_lines_contig(ctx, Zc, g.My, g.Mx, G.ay, G.twy, 0));
-            SE2P_TRY(transpose(ctx, Zc, Tc, g.Mx, g.My));
-            SE2P_TRY(fft_lines_contig(ctx, Tc, g.Mx, g.My, G.ax, G.twx, 0));
-        }
+    for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
+        SE2P_TRY(batch_spread_grid(ctx, B, d_f, c));
+        StageTimer st(ctx, "fft");
+        const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
+        SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, B.lds_y, B.sys, J + B.ffty[c].first,
+                                  B.ffty[c].count));
+        SE2P_TRY(launch_transpose(s, Zc, Tc, 0, B.sys, J + B.tr_fwd[c].first, B.tr_fwd[c].count));
+        SE2P_TRY(launch_fft_lines(s, 1, Tc, 0, B.lds_x, B.sys, J + B.fftx[c].first,
+                                  B.fftx[c].count));
     }
     {
         StageTimer st(ctx, "filter");
-        SE2P_TRY(launch_filter(s, G.kind, g, G.W, G.D, G.Z0, 1));
+        for (int kind = 0; kind < 2; ++kind)
+            SE2P_TRY(launch_filter(s, kind, B.sys, J + B.filter[kind].first, B.filter[kind].count));
     }
     {
         StageTimer st(ctx, "fft");
-        SE2P_TRY(fft_lines_contig(ctx, G.Z0, g.Mx, g.My, G.ax, G.twx, 1));
-        SE2P_TRY(transpose(ctx, G.Z0, G.W, g.My, g.Mx));
-        SE2P_TRY(fft_lines_contig(ctx, G.W, g.My, g.Mx, G.ay, G.twy, 1));
+        SE2P_TRY(launch_fft_lines(s, 1, G_Z0, 1, B.lds_x, B.sys, J + B.ifftx.first, B.ifftx.count));
+        SE2P_TRY(launch_transpose(s, G_Z0, G_W, 1, B.sys, J + B.tr_inv.first, B.tr_inv.count));
+        SE2P_TRY(launch_fft_lines(s, 0, G_W, 1, B.lds_y, B.sys, J + B.iffty.first, B.iffty.count));
     }
     StageTimer st(ctx, "gather");
-    return launch_gather(s, g, G.gt_start, G.gt_perm, G.gt_node, G.gt_off, G.W, G.e1tab,
-                         d_add_first, d_zm, d_u);
+    return launch_gather(s, B.maxR, B.sys, J + B.gather.first, B.gather.count, d_u,
+                         accumulate ? 1 : 0, use_zm ? 1 : 0);
 }
 
-int geom_apply(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u)
+int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u)
 {
-    SE2P_TRY(geom_apply_real(ctx, G, d_f, d_u));
-    return geom_apply_k(ctx, G, d_f, d_u, G.kind == SE2P_DLP ? G.zm : nullptr, d_u);
+    if (B.has_real) SE2P_TRY(batch_apply_real(ctx, B, d_f, d_u));
+    if (B.has_k) SE2P_TRY(batch_apply_k(ctx, B, d_f, d_u, B.has_real, B.has_real));
+    return SE2P_OK;
 }
 
 }  // namespace se2p

@@ -58,7 +58,7 @@ enum Slot {
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_PARTIAL, WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
-    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D,
+    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS,
     WS_NUM_SLOTS
 };
 
@@ -130,20 +130,7 @@ int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, doubl
                          double h, int P, int* mx, int* my, double* px, double* py);
 
 // ---------------------------------------------------------------------------
-// Real space (realspace.hip)
-// ---------------------------------------------------------------------------
-// Sorted source records: SLP {x, y, f1, f2}; DLP {x, y, f1, f2, n1, n2, f.n, 0}.
-int launch_pack_sources(hipStream_t s, int kind, const int* perm, int n, const double* p,
-                        const double* f, const double* nrm, double* out);
-int launch_pack_targets(hipStream_t s, const int* perm, int n, const double* p, double2* out);
-int launch_real(hipStream_t s, int kind, int tier, int nsx, int nsy, double Lx, double Ly,
-                double xi, const int* tstart, const int* tperm, const double2* tsorted,
-                const int* sstart, const double* ssorted, int mean_targets, double* u);
-// E1 on n values (tier 0 full, 1 reduced).
-int launch_e1(hipStream_t s, const double* x, int n, int tier, double* out);
-
-// ---------------------------------------------------------------------------
-// k space (kspace.hip)
+// k-space tile geometry (kspace.hip)
 // ---------------------------------------------------------------------------
 struct KGeom {
     int Mx, My, P;
@@ -153,26 +140,9 @@ struct KGeom {
 int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi, double eta,
                double w);
 int launch_e1_table(hipStream_t s, const KGeom& g, double* tab);
-int launch_spread(hipStream_t s, int kind, const KGeom& g, const int* tstart, const int* perm,
-                  const int2* node, const double2* off, const double* f, const double* nrm,
-                  const double* e1tab, double* partial);
-int launch_merge(hipStream_t s, int ncomp, const KGeom& g, const double* partial,
-                 double2* Z0, double2* Z1);
-// rowmajor = 0: grids are My x Mx column-major (Z[ix*My + iy]); 1: transposed
-// (Z[iy*Mx + ix]).
-int launch_filter(hipStream_t s, int kind, const KGeom& g, const double2* Z0,
-                  const double2* Z1, double2* W, int rowmajor = 0);
-int launch_gather(hipStream_t s, const KGeom& g, const int* tstart, const int* perm,
-                  const int2* node, const double2* off, const double2* W, const double* e1tab,
-                  const double* add_first, const double* zero_mode, double* u);
-int launch_dlp_zero_mode(hipStream_t s, const double* p, const double* f, const double* nrm,
-                         int n, double Lx, double Ly, double* partial, double* out);
-// The same sum from sorted DLP source records {x, y, f1, f2, n1, n2, f.n, 0}.
-int launch_dlp_zero_mode_rec(hipStream_t s, const double* rec, int n, double Lx, double Ly,
-                             double* partial, double* out);
 
 // ---------------------------------------------------------------------------
-// FFT (fft.hip)
+// FFT axes (fft.hip)
 // ---------------------------------------------------------------------------
 constexpr int kFftMaxLen = 4096;
 constexpr int kFftMaxStages = 24;
@@ -183,16 +153,89 @@ struct FftAxis {
 };
 int fft_axis_init(FftAxis* a, int n);
 int fft_twiddles(se2p_ctx* ctx, Workspace* ws, int n, Slot slot, double2** tw);
-// In-place 2D transform of a My x Mx column-major complex grid (H[ix*My + iy]).
-// Forward is unnormalised; inverse is unnormalised too (the 1/(Mx My) factor is
-// applied by the k-space filter).
-// count contiguous lines of length n (line stride n), in place.
-int fft_lines_contig(se2p_ctx* ctx, double2* Z, int n, int count, const FftAxis& ax,
-                     const double2* tw, int inverse);
-// out[c * rows + r] = in[r * cols + c]
-int transpose(se2p_ctx* ctx, const double2* in, double2* out, int rows, int cols);
-int fft2d(se2p_ctx* ctx, double2* Z, int Mx, int My, const FftAxis& ax, const FftAxis& ay,
-          const double2* twx, const double2* twy, int inverse);
+// One-time kernel attributes (large dynamic LDS); not during graph capture.
+int fft_prepare();
+// Lines of length n per FFT workgroup.
+int fft_lines_per_block(int n);
+
+// ---------------------------------------------------------------------------
+// Systems and jobs.  Every density-dependent kernel reads a table of system
+// descriptors and a job list: job (s, b) is block b of system s, so one launch
+// covers every system of a batch (a single plan is a batch of one).
+// ---------------------------------------------------------------------------
+enum GridSel { G_Z0 = 0, G_Z1 = 1, G_W = 2, G_D = 3 };
+
+struct SysDesc {
+    int kind, Ns, Nt, e1tier;
+    long f_off, u_off;  // offsets (doubles) of this system in the apply's f and u
+    double Lx, Ly;
+    // real space
+    int nsx, nsy, bs, nbx;
+    double xi2, cut2, self;
+    const int *t_start, *t_perm, *s_start, *s_perm;
+    const double2* t_sorted;
+    double* s_rec;
+    double *zm, *zm_part;
+    int zm_nblk;
+    // k space
+    int Mx, My, P, TX, TY, ntx, nty, R, Bx, By;
+    double t, w, h, xi, eta, gscale;
+    const int *sp_start, *sp_perm, *gt_start, *gt_perm;
+    const int2 *sp_node, *gt_node;
+    const double2 *sp_off, *gt_off;
+    const double *e1tab, *nrm;
+    double* partial;
+    double2* grid[4];  // GridSel
+    const double2 *twx, *twy;
+    FftAxis ax, ay;
+};
+using Job = int2;
+
+// Job lists of one launch: a range of the job array.
+struct JobRange {
+    int first = 0, count = 0;
+};
+
+// Sorted source records: SLP {x, y, f1, f2}; DLP {x, y, f1, f2, n1, n2, f.n, 0}.
+int launch_pack_sources(hipStream_t s, int kind, const int* perm, int n, const double* p,
+                        const double* f, const double* nrm, double* out);
+int launch_pack_targets(hipStream_t s, const int* perm, int n, const double* p, double2* out);
+// E1 on n values (tier 0 full, 1 reduced).
+int launch_e1(hipStream_t s, const double* x, int n, int tier, double* out);
+
+// Real-space blocks of bs x bs cells for a mean cell occupancy.
+int real_block_side(int nsx, int nsy, int mean_targets);
+// k_real_stencil<kind, tier, one-cell> over the jobs (block indices).
+int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys, const Job* jobs,
+                int njobs, double* u);
+// Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into the tile partials;
+// R selects the register blocking (32 or 48).
+int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jobs, int njobs,
+                  const double* f);
+// Halo merge into grid `out`; job b = ix * ceil(My/256) + y block.
+int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs);
+// Filter: W (and D for the DLP) in transposed layout Z[iy*Mx + ix] -> Z0.
+int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
+// Gather from W into u (+= when accumulate; + the DLP zero mode when use_zm).
+int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
+                  double* u, int accumulate, int use_zm);
+// DLP zero mode sum_s (f.n)_s x_s / (Lx Ly) from the sorted records: partial
+// sums over jobs (s, b), b < zm_nblk, then one job per system.
+int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, int npart,
+                     const Job* sys_jobs, int nsys);
+// Density fill of the sorted source records.
+int launch_fill_dens(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs,
+                     const double* f);
+// Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
+// x (axis 1: Mx-lines of the transposed grid); max_lds from fft_lds_bytes.
+int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, size_t max_lds,
+                     const SysDesc* sys, const Job* jobs, int njobs);
+size_t fft_lds_bytes(int n);
+// Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
+// and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
+int launch_transpose(hipStream_t s, int in, int out, int dir, const SysDesc* sys,
+                     const Job* jobs, int njobs);
+int transpose_tiles(int rows, int cols);
 
 // ---------------------------------------------------------------------------
 // Pipeline (pipeline.hip): geometry set-up once, density-dependent work per apply
@@ -227,12 +270,38 @@ struct Geom {
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
                const double* d_ptar, const double* d_n, int Ns, int Nt, double Lx, double Ly,
                const se2p_params& prm, bool want_real, bool want_k);
+void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d);
+
+// The descriptors and job lists of a set of systems (device arrays in `ws`).
+struct BatchExec {
+    int nsys = 0;
+    bool has_real = false, has_k = false, any_dlp = false;
+    SysDesc* sys = nullptr;
+    Job* jobs = nullptr;
+    JobRange fill[2];     // by kind
+    JobRange zm_part, zm_sys;
+    JobRange real[2][2][2];  // [kind][tier][one-cell]
+    JobRange spread[2][2];   // [c][R == 48]
+    JobRange merge[2];       // [c]
+    JobRange ffty[2], fftx[2], tr_fwd[2];  // forward passes of component pair c
+    JobRange filter[2];      // by kind
+    JobRange ifftx, tr_inv, iffty, gather;
+    size_t lds_y = 0, lds_x = 0;
+    int maxR = 32;
+};
+// Builds descriptors (f and u offsets: systems back to back, 2 Ns and 2 Nt
+// doubles each) and job lists; synchronises the stream (host staging).
+int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* geoms, int nsys);
 // u = real-space part (set)
-int geom_apply_real(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u);
-// u = add_first + k-space part (+ zero mode when d_zm is given)
-int geom_apply_k(se2p_ctx* ctx, const Geom& G, const double* d_f, const double* d_add_first,
-                 const double* d_zm, double* d_u);
+int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u);
+// u (+)= k-space part (+ DLP zero mode when use_zm)
+int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
+                  bool accumulate, bool use_zm);
 // u = real + k (+ DLP zero mode)
-int geom_apply(se2p_ctx* ctx, const Geom& G, const double* d_f, double* d_u);
+int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u);
+// Spreading and merge of component pair c into Z0 (c = 0) or Z1 (c = 1) only.
+int batch_spread_grid(se2p_ctx* ctx, const BatchExec& B, const double* d_f, int c);
+// Gather from W only.
+int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u);
 
 }  // namespace se2p
