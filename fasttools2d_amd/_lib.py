@@ -29,7 +29,8 @@ EXPORTS = (
     "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
     "se2p_prof_reset", "se2p_prof_get", "se2p_assign", "se2p_find_closest_node",
     "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1", "se2p_plan_create",
-    "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy",
+    "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy", "se2p_batch_create",
+    "se2p_batch_sizes", "se2p_batch_apply", "se2p_batch_apply_graph", "se2p_batch_destroy",
 )
 
 
@@ -52,6 +53,13 @@ class Params(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class System(ctypes.Structure):
+    """se2p_system: one system of a batch (device pointers)."""
+    _fields_ = [("kind", ctypes.c_int), ("Nsrc", ctypes.c_int), ("Ntar", ctypes.c_int),
+                ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("d_psrc", ctypes.c_void_p),
+                ("d_ptar", ctypes.c_void_p), ("d_n", ctypes.c_void_p), ("prm", Params)]
 
 
 _lib = None
@@ -106,6 +114,12 @@ def lib():
             "se2p_plan_apply": [P, P, P],
             "se2p_plan_apply_graph": [P, P, P],
             "se2p_plan_destroy": [P],
+            "se2p_batch_create": [P, I, ctypes.POINTER(System), ctypes.POINTER(P)],
+            "se2p_batch_sizes": [P, ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_longlong)],
+            "se2p_batch_apply": [P, P, P],
+            "se2p_batch_apply_graph": [P, P, P],
+            "se2p_batch_destroy": [P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -8,17 +8,33 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
+
+// A captured apply for one (f, u) pair.
+struct GraphCache {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const double* f = nullptr;
+    double* u = nullptr;
+};
 
 struct se2p_plan {
     se2p_ctx* ctx = nullptr;
     se2p::Workspace ws;
     se2p::Geom G;
     se2p::BatchExec B;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const double* graph_f = nullptr;
-    double* graph_u = nullptr;
+    GraphCache gc;
+};
+
+struct se2p_batch {
+    se2p_ctx* ctx = nullptr;
+    std::vector<se2p::Geom> G;
+    std::vector<se2p::Workspace> ws;  // one per system
+    se2p::Workspace bws;              // descriptors and job lists
+    se2p::BatchExec B;
+    long long nf = 0, nu = 0;
+    GraphCache gc;
 };
 
 namespace se2p {
@@ -458,6 +474,41 @@ int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const doubl
     return download(ctx, out, (double*)du, 2 * (size_t)Nt);
 }
 
+void graph_release(GraphCache* gc)
+{
+    if (gc->exec) (void)hipGraphExecDestroy(gc->exec);
+    if (gc->graph) (void)hipGraphDestroy(gc->graph);
+    *gc = GraphCache{};
+}
+
+// batch_apply replayed from a graph captured on the first call for (d_f, d_u).
+int graph_apply(se2p_ctx* ctx, GraphCache* gc, const BatchExec& B, const double* d_f,
+                double* d_u)
+{
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (!gc->exec || gc->f != d_f || gc->u != d_u) {
+        graph_release(gc);
+        SE2P_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        int rc = batch_apply(ctx, B, d_f, d_u);
+        ctx->capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != SE2P_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return set_error(SE2P_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        gc->graph = g;
+        SE2P_HIP_CHECK(hipGraphInstantiate(&gc->exec, g, nullptr, nullptr, 0));
+        gc->f = d_f;
+        gc->u = d_u;
+    }
+    SE2P_HIP_CHECK(hipGraphLaunch(gc->exec, ctx->stream));
+    return SE2P_OK;
+}
+
 }  // namespace
 }  // namespace se2p
 
@@ -737,35 +788,7 @@ int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u)
     if (!plan) return set_error(SE2P_ERR_ARG, "null plan");
     SE2P_TRY(need_ptr(d_f, plan->G.Ns, "d_f"));
     SE2P_TRY(need_ptr(d_u, plan->G.Nt, "d_u"));
-    se2p_ctx* ctx = plan->ctx;
-    if (!plan->exec || plan->graph_f != d_f || plan->graph_u != d_u) {
-        if (plan->exec) {
-            SE2P_HIP_CHECK(hipGraphExecDestroy(plan->exec));
-            plan->exec = nullptr;
-        }
-        if (plan->graph) {
-            SE2P_HIP_CHECK(hipGraphDestroy(plan->graph));
-            plan->graph = nullptr;
-        }
-        SE2P_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        ctx->capturing = true;
-        int rc = batch_apply(ctx, plan->B, d_f, d_u);
-        ctx->capturing = false;
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (rc != SE2P_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess)
-            return set_error(SE2P_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-        plan->graph = g;
-        SE2P_HIP_CHECK(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
-        plan->graph_f = d_f;
-        plan->graph_u = d_u;
-    }
-    SE2P_HIP_CHECK(hipGraphLaunch(plan->exec, ctx->stream));
-    return SE2P_OK;
+    return graph_apply(plan->ctx, &plan->gc, plan->B, d_f, d_u);
 }
 
 int se2p_plan_destroy(se2p_plan* plan)
@@ -773,10 +796,90 @@ int se2p_plan_destroy(se2p_plan* plan)
     if (!plan) return SE2P_OK;
     (void)hipSetDevice(plan->ctx->device);
     (void)hipStreamSynchronize(plan->ctx->stream);
-    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
-    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    graph_release(&plan->gc);
     ws_release(&plan->ws);
     delete plan;
+    return SE2P_OK;
+}
+
+int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_batch** out)
+{
+    if (!out) return set_error(SE2P_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (nsys < 0 || (nsys > 0 && !systems)) return set_error(SE2P_ERR_ARG, "bad system list");
+    for (int i = 0; i < nsys; ++i) {
+        const se2p_system& y = systems[i];
+        if (y.kind != SE2P_SLP && y.kind != SE2P_DLP)
+            return set_error(SE2P_ERR_ARG, "system %d: bad kind", i);
+        int rc = check_common(ctx, y.Nsrc, y.Ntar, y.Lx, y.Ly);
+        if (rc == SE2P_OK) rc = check_params(&y.prm, y.Lx, y.Ly);
+        if (rc == SE2P_OK) rc = need_ptr(y.d_psrc, y.Nsrc, "d_psrc");
+        if (rc == SE2P_OK) rc = need_ptr(y.d_ptar, y.Ntar, "d_ptar");
+        if (rc == SE2P_OK && y.kind == SE2P_DLP) rc = need_ptr(y.d_n, y.Nsrc, "d_n");
+        if (rc != SE2P_OK)
+            return set_error(rc, "system %d: %s", i, std::string(se2p_last_error()).c_str());
+    }
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    se2p_batch* b = new (std::nothrow) se2p_batch();
+    if (!b) return set_error(SE2P_ERR_NOMEM, "batch allocation failed");
+    b->ctx = ctx;
+    b->G.resize(nsys);
+    b->ws.resize(nsys);
+    int rc = SE2P_OK;
+    std::vector<const Geom*> gp(nsys);
+    for (int i = 0; i < nsys && rc == SE2P_OK; ++i) {
+        const se2p_system& y = systems[i];
+        rc = geom_build(ctx, &b->ws[i], &b->G[i], y.kind, y.d_psrc, y.d_ptar, y.d_n, y.Nsrc,
+                        y.Ntar, y.Lx, y.Ly, y.prm, true, true);
+        gp[i] = &b->G[i];
+        b->nf += 2LL * y.Nsrc;
+        b->nu += 2LL * y.Ntar;
+    }
+    if (rc == SE2P_OK) rc = batch_build(ctx, &b->bws, &b->B, gp.data(), nsys);
+    if (rc != SE2P_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& w : b->ws) ws_release(&w);
+        ws_release(&b->bws);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SE2P_OK;
+}
+
+int se2p_batch_sizes(const se2p_batch* batch, long long* nf, long long* nu)
+{
+    if (!batch) return set_error(SE2P_ERR_ARG, "null batch");
+    if (nf) *nf = batch->nf;
+    if (nu) *nu = batch->nu;
+    return SE2P_OK;
+}
+
+int se2p_batch_apply(se2p_batch* batch, const double* d_f, double* d_u)
+{
+    if (!batch) return set_error(SE2P_ERR_ARG, "null batch");
+    if ((batch->nf && !d_f) || (batch->nu && !d_u)) return set_error(SE2P_ERR_ARG, "null array");
+    SE2P_HIP_CHECK(hipSetDevice(batch->ctx->device));
+    return batch_apply(batch->ctx, batch->B, d_f, d_u);
+}
+
+int se2p_batch_apply_graph(se2p_batch* batch, const double* d_f, double* d_u)
+{
+    if (!batch) return set_error(SE2P_ERR_ARG, "null batch");
+    if ((batch->nf && !d_f) || (batch->nu && !d_u)) return set_error(SE2P_ERR_ARG, "null array");
+    return graph_apply(batch->ctx, &batch->gc, batch->B, d_f, d_u);
+}
+
+int se2p_batch_destroy(se2p_batch* batch)
+{
+    if (!batch) return SE2P_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipStreamSynchronize(batch->ctx->stream);
+    graph_release(&batch->gc);
+    for (auto& w : batch->ws) ws_release(&w);
+    ws_release(&batch->bws);
+    delete batch;
     return SE2P_OK;
 }
 

@@ -39,6 +39,7 @@ extern "C" {
 
 typedef struct se2p_ctx se2p_ctx;
 typedef struct se2p_plan se2p_plan;
+typedef struct se2p_batch se2p_batch;
 
 /* ------------------------------------------------------------------------ */
 /* Context: one GPU, one HIP stream, a grow-only device workspace.           */
@@ -164,6 +165,33 @@ int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u);
  * this (d_f, d_u) pair; a new pair re-captures. */
 int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u);
 int se2p_plan_destroy(se2p_plan* plan);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent periodic systems (own points, box, kind and     */
+/* parameters each), fixed geometry, evaluated together: every stage runs as */
+/* one launch over the blocks of all systems.  No counterpart in the         */
+/* reference (one system per mex call, StokesSLP_ewald_2p.m:119,126).        */
+/* ------------------------------------------------------------------------ */
+typedef struct se2p_system {
+    int kind;             /* SE2P_SLP or SE2P_DLP */
+    int Nsrc, Ntar;
+    double Lx, Ly;
+    const double* d_psrc; /* 2 x Nsrc device array, read during creation only */
+    const double* d_ptar; /* 2 x Ntar */
+    const double* d_n;    /* 2 x Nsrc normals (SE2P_DLP), else NULL */
+    se2p_params prm;      /* e.g. from se2p_params_ref at the system's tolerance */
+} se2p_system;
+
+int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_batch** out);
+/* Doubles of the concatenated densities (sum of 2 Nsrc) and outputs (sum of 2 Ntar). */
+int se2p_batch_sizes(const se2p_batch* batch, long long* nf, long long* nu);
+/* d_u = u of every system, systems back to back (2 x Ntar each), for the
+ * densities d_f (2 x Nsrc each, same order).  Stream-ordered; allocates nothing. */
+int se2p_batch_apply(se2p_batch* batch, const double* d_f, double* d_u);
+/* As se2p_batch_apply, replayed from a HIP graph captured on the first call for
+ * this (d_f, d_u) pair; a new pair re-captures. */
+int se2p_batch_apply_graph(se2p_batch* batch, const double* d_f, double* d_u);
+int se2p_batch_destroy(se2p_batch* batch);
 
 /* ------------------------------------------------------------------------ */
 /* Stage-level entry points (host buffers, synchronous): the helpers of      */
