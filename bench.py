@@ -6,6 +6,7 @@ A step is one complete N-target matvec (binning, real space, spreading, FFTs,
 filter, gather) on inputs already resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+  python bench.py --batch [--nsys 512] [--steps K]   (batched serving workload)
 """
 import argparse
 import ctypes
@@ -227,6 +228,115 @@ def profile_only(ctx, dev, ps, p, f, args):
     print(json.dumps({"profile": "tuned", "steps": args.steps, "params": prm}))
 
 
+def capped_params(n, Lx, Ly, tol, P=24):
+    """Reference recipe for an SLP system with sources = targets, its points-per-
+    cell option raised until the grid fits the library's 4096-point FFT lines
+    (larger cells, smaller xi, coarser grid)."""
+    from fasttools2d_amd import ewald_params
+    prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, P=P)
+    nb = n / (prm["nside_x"] * prm["nside_y"])
+    while max(prm["Mx"], prm["My"]) > 4096:
+        nb *= 1.25
+        prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, Nb=nb, P=P)
+        if prm["nside_x"] < 1 or prm["nside_y"] < 1:
+            raise ValueError(f"no grid within 4096 per side for N={n}, box {Lx}x{Ly}, tol {tol}")
+    return prm
+
+
+def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
+    """The batched serving workload: box aspect 1..4 (half steps: the parameter
+    recipe needs a rational aspect, and its numerator multiplies the grid), N log-uniform in [nmin, nmax],
+    tolerance log-uniform in [1e-10, 1e-4], sources = targets.  Parameters per
+    system at its tolerance: recipe "rule" is tune.rule_params (the tuner's
+    calibrated constants as functions of the tolerance); "ref" is the reference
+    recipe with the fast E1 tier and its points-per-cell option raised where the
+    grid would exceed 4096 per side."""
+    from fasttools2d_amd.tune import rule_params
+    rng = np.random.default_rng(seed)
+    aspects = np.arange(2, 9) / 2.0
+    out = []
+    for _ in range(nsys):
+        Lx, Ly = float(rng.choice(aspects)), 1.0
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        tol = float(10 ** rng.uniform(-10, -4))
+        p = np.ascontiguousarray(rng.random((n, 2)) * [Lx, Ly] - [Lx / 2, Ly / 2])
+        if recipe == "rule":
+            prm = rule_params("slp", n, n, Lx, Ly, tol)
+        else:
+            prm = capped_params(n, Lx, Ly, tol)
+            prm["e1_tier"] = 1
+        out.append(dict(kind="slp", psrc=p, ptar=p, nrm=None, Lx=Lx, Ly=Ly, prm=prm, tol=tol))
+    return out
+
+
+def batch_bench(args):
+    """Serving workload: nsys independent periodic SLP systems in one batch,
+    `steps` graph-replayed matvecs with a fresh N(0,1) density each (copied into
+    the batch's density buffer inside the timed loop, as an iterative solver
+    writes its next vector)."""
+    import fasttools2d_amd as ft
+    from fasttools2d_amd.batch import Batch
+    from fasttools2d_amd.plan import DeviceArray, Plan
+    from fasttools2d_amd.tune import rel_err
+    systems = serving_systems(args.nsys, recipe=args.recipe)
+    ntot = sum(len(y["psrc"]) for y in systems)
+    log(f"bench-batch: {len(systems)} systems, {ntot} points")
+    ctx = ft.Context(0)
+    t0 = time.perf_counter()
+    b = Batch(ctx, systems)
+    ctx.sync()
+    t_create = time.perf_counter() - t0
+    log(f"bench-batch: created in {t_create:.2f} s")
+    rng = np.random.default_rng(5)
+    dens = [DeviceArray.from_host(ctx, rng.standard_normal(b.nf)) for _ in range(args.steps)]
+    nbytes = 8 * b.nf
+    for k in range(args.warmup):
+        ctx.d2d(b.f.ptr, dens[k % len(dens)].ptr, nbytes)
+        b.apply_device(b.f.ptr, b.u.ptr, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.d2d(b.f.ptr, dens[k].ptr, nbytes)
+        b.apply_device(b.f.ptr, b.u.ptr, graph=True)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    log(f"bench-batch: {dt / args.steps * 1e3:.2f} ms per batch matvec")
+    # accuracy of the last matvec on a sample of systems against tol-1e-13 plans
+    u_all = b.u.download()
+    f_last = dens[-1].download()
+    idx = sorted(set(np.linspace(0, len(systems) - 1, min(12, len(systems))).astype(int)))
+    worst = 0.0
+    for i in idx:
+        y = systems[i]
+        fi = f_last[b.f_off[i]:b.f_off[i + 1]].reshape(-1, 2)
+        ui = u_all[b.u_off[i]:b.u_off[i + 1]].reshape(-1, 2)
+        n = len(y["psrc"])
+        try:
+            ref = capped_params(n, y["Lx"], y["Ly"], 1e-13)
+        except ValueError:
+            ref = capped_params(n, y["Lx"], y["Ly"], y["tol"] * 1e-3)
+        pl = Plan(ctx, "slp", y["psrc"], y["ptar"], y["Lx"], y["Ly"], ref)
+        e = rel_err(ui, pl.apply(fi))
+        pl.close()
+        worst = max(worst, e / y["tol"])
+        log(f"bench-batch: system {i} N {n} tol {y['tol']:.1e} err {e:.2e}")
+    out = {
+        "metric": "batched serving workload: periodic SLP matvecs/s",
+        "value": len(systems) * args.steps / dt, "unit": "system-matvecs/s",
+        "points_per_s": ntot * args.steps / dt, "ms_per_batch": dt / args.steps * 1e3,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic: uniform points, N(0,1) densities, seed 11",
+        "config": {"workload": f"{len(systems)} independent periodic SLP systems, aspect 1-4, "
+                               "N log-uniform 500-200000, tol log-uniform 1e-10..1e-4",
+                   "points": ntot, "recipe": args.recipe},
+        "create_s": t_create, "worst_err_over_tol": worst,
+    }
+    for a in dens:
+        a.free()
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +346,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="run the batched serving workload instead (own JSON line)")
+    ap.add_argument("--nsys", type=int, default=512)
+    ap.add_argument("--recipe", choices=["rule", "ref"], default="rule",
+                    help="parameters of the batched systems")
     ap.add_argument("--profile", choices=["plain", "tuned", "copy"],
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
@@ -244,6 +359,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.batch:
+        batch_bench(args)
+        return
 
     import fasttools2d_amd as ft
     from fasttools2d_amd import ewald_params

@@ -190,3 +190,41 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         return prm, report
     finally:
         T.close()
+
+
+def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, margin=10.0):
+    """Parameters without measuring: the constants the tuner calibrates, as
+    functions of the tolerance, fitted to its N = 1e6, tol = 1e-8 result
+    (xi rc = 3.9, k_inf/xi = 8.0, P = 16, about 22 points per cell), with
+    L = ln(margin/tol) (the fit leaves no room on small systems, whose relative
+    errors run a few times higher):
+
+      xi rc      = sqrt(L - 3.2)        real-space truncation ~ e^{-(xi rc)^2}
+      k_inf / xi = 2 sqrt(L - 2.4)      k-space truncation ~ e^{-k_inf^2/(4 xi^2)}
+      P          = 2 ceil(0.4 L + 0.6)  Gaussian support
+
+    Cells hold about `per_cell` points; where the grid would exceed max_grid per
+    side the cells grow.  The fast E1 tier is used for tol >= 1e-10.  A fallback
+    for systems too many to tune one by one (batches); its accuracy is checked
+    against accurate solutions where it is used."""
+    A, B = _aspect(Lx, Ly)
+    L = math.log(margin / tol)
+    c_r = math.sqrt(max(L - 3.2, 1.0))
+    c_k = 2.0 * math.sqrt(max(L - 2.4, 1.0))
+    P = min(24, max(8, 2 * math.ceil(0.4 * L + 0.6)))
+    npts = max(Nsrc, Ntar)
+    a = max(1, int(round(math.sqrt(npts / (per_cell * A * B)))))
+    while True:
+        nsx, nsy = a * A, a * B
+        xi = c_r / (Lx / nsx)
+        M0 = c_k * xi * Lx / math.pi / A
+        if A * M0 <= max_grid and B * M0 <= max_grid:
+            try:
+                M0 = smooth_up(M0, max_grid // max(A, B))
+                break
+            except ValueError:
+                pass
+        if a == 1:
+            raise ValueError(f"no grid within {max_grid} per side for tol {tol}")
+        a -= 1
+    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0)
