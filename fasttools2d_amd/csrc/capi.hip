@@ -1081,9 +1081,10 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     const SysDesc* ds = (const SysDesc*)psys;
     const Job* dj = (const Job*)pjobs;
     const int inv = inverse ? 1 : 0;
-    SE2P_TRY(launch_fft_lines(s, 0, G_Z0, inv, fft_lds_bytes(My), ds, dj, ny));
+    SE2P_TRY(launch_fft_lines(s, 0, G_Z0, inv, fft_line_class(My), fft_lds_bytes(My), ds, dj, ny));
     SE2P_TRY(launch_transpose(s, G_Z0, G_W, 0, ds, dj + ny, t0));
-    SE2P_TRY(launch_fft_lines(s, 1, G_W, inv, fft_lds_bytes(Mx), ds, dj + ny + t0, nx));
+    SE2P_TRY(launch_fft_lines(s, 1, G_W, inv, fft_line_class(Mx), fft_lds_bytes(Mx), ds,
+                              dj + ny + t0, nx));
     SE2P_TRY(launch_transpose(s, G_W, G_Z0, 1, ds, dj + ny + t0 + nx, t1));
     SE2P_TRY(download(ctx, z.data(), dz, 2 * Gn));
     const double sc = inverse ? 1.0 / ((double)Mx * (double)My) : 1.0;

@@ -3,15 +3,16 @@
 // The reference transforms with MATLAB's fft2/ifft2 called back through
 // mexCallMATLAB (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_kspace.cpp:80-81,
 // 176-177).  This is the library's own transform: a mixed-radix Stockham FFT
-// with one workgroup per line (or per group of lines), the line held in LDS and
-// ping-ponged between two LDS buffers stage by stage.  Any length up to
-// kFftMaxLen works: radices 4 and 2 first, then 3, 5, 7 and any remaining prime
-// factor.  Stage s with radix R and Ns = product of the earlier radices maps
-// output (j, k), j < n/R, k < R, to
+// with one workgroup per line (or per group of short lines).  The first stage
+// reads the lines from HBM, the last writes them back, and in between they live
+// in one LDS buffer, each stage staging its butterflies through registers.  Any
+// length up to kFftMaxLen works: radices 8, 4 and 2 first, then 3, 5, 7 and any
+// remaining prime factor (2, 3, 4, 5, 8 as in-register butterflies, the others
+// by a direct O(R) sum per output).  Stage s with radix R and Ns = product of the earlier
+// radices maps output (j, k), j < n/R, k < R, to
 //   out[(j / Ns) Ns R + j % Ns + k Ns] = sum_r in[j + r n/R] W_n^{r ((j % Ns) + k Ns) n/(Ns R)}.
-// The grid is My x Mx column-major (Z[ix*My + iy]): y lines are contiguous,
-// x lines have stride My and are loaded B at a time so each global access
-// touches B consecutive elements.
+// Every pass works on contiguous lines: y lines of the My x Mx column-major
+// grid (Z[ix*My + iy]), and x lines after a tiled transpose.
 #include "se2p_internal.h"
 
 #include <cmath>
@@ -96,97 +97,192 @@ __device__ __forceinline__ void dft8(double2* v, double sgn)
     v[7] = csub(e3, w3);
 }
 
-// Small odd radix with table twiddles: W_R^q = tw[q n/R].
-template <int R>
-__device__ __forceinline__ void dft_small(double2* v, const double2* __restrict__ tw, int n,
-                                          double sgn)
+// Points per thread at most in a line pass: lines up to 2048 points run with 256
+// threads, longer ones with 512.
+constexpr int kFftPerThread = 8;
+
+// q = j / d, r = j % d for 0 <= j < 2^24, from a float reciprocal and one
+// correction step (no integer division in the butterfly loops).
+__device__ __forceinline__ void divmod(int j, int d, float rd, int& q, int& r)
 {
-    double2 out[R];
-    const int step = n / R;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        double2 acc = v[0];
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-            const double2 w = tw[((r * k) % R) * step];
-            acc = cadd(acc, cmul(v[r], make_double2(w.x, -sgn * w.y)));
-        }
-        out[k] = acc;
+    q = (int)((float)j * rd);
+    r = j - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
     }
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k] = out[k];
 }
 
-// One Stockham stage of radix R over nb lines of length n in LDS.
+// In-register DFT of radix R in {2, 3, 4, 5, 8}.
 template <int R>
-__device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst,
-                                      int n, int nb, int Ns, const double2* __restrict__ tw,
-                                      double sgn)
+__device__ __forceinline__ void butterfly(double2* v, double sgn)
 {
+    if (R == 2) dft2(v);
+    else if (R == 3) dft3(v, sgn);
+    else if (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
+    else if (R == 5) dft5(v, sgn);
+    else dft8(v, sgn);
+}
+
+// Pointers into LDS and global memory with their address space spelled out:
+// the stage functions are not inlined (each keeps its own register budget,
+// which the loop over runtime radices would otherwise merge), so the space
+// must travel in the type for LDS to be accessed with ds_* and HBM with
+// global_* instructions.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) dvec2 lds_d2;
+typedef __attribute__((address_space(1))) dvec2 glb_d2;
+
+template <typename Ptr>
+__device__ __forceinline__ double2 ld2(Ptr p, int i)
+{
+    const dvec2 v = p[i];
+    return make_double2(v.x, v.y);
+}
+
+template <typename Ptr>
+__device__ __forceinline__ void st2(Ptr p, int i, double2 a)
+{
+    dvec2 v;
+    v.x = a.x;
+    v.y = a.y;
+    p[i] = v;
+}
+
+// One Stockham stage of radix R over nb lines of length n.  Butterflies are
+// staged through registers, so the lines stay in one LDS buffer: every thread
+// reads its inputs, a barrier, every thread writes its outputs.  FIRST reads the
+// lines from HBM (gdata) and LAST writes them back there; the other stages read
+// and write the LDS copy.  Twiddles: one table read per butterfly, the other
+// R-2 as powers of it.
+template <int R, bool FIRST, bool LAST, int T>
+__device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
+                                                    glb_d2* __restrict__ gdata, int n, int nb,
+                                                    int Ns, const glb_d2* __restrict__ tw,
+                                                    double sgn)
+{
+    constexpr int BPT = (kFftPerThread + R - 1) / R;  // butterflies per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
-    for (int q = threadIdx.x; q < nb * m; q += blockDim.x) {
-        const int b = q / m;
-        const int j = q - b * m;
-        const int jm = j % Ns;
-        const double2* in = src + b * n + j;
-        double2 v[R];
+    const float rNs = 1.0f / (float)Ns, rm = 1.0f / (float)m;
+    const int total = nb * m;
+    double2 v[BPT][R];
+    int ob[BPT];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = in[r * m];
-        if (Ns > 1) {
+    for (int t = 0; t < BPT; ++t) {
+        const int q = threadIdx.x + t * T;
+        ob[t] = -1;
+        if (q < total) {
+            int b = 0, j = q;
+            if (nb > 1) divmod(q, m, rm, b, j);
+            int jq, jm;
+            divmod(j, Ns, rNs, jq, jm);
+            if constexpr (FIRST) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const double2 w = tw[r * jm * base];
-                v[r] = cmul(v[r], make_double2(w.x, -sgn * w.y));
+                for (int r = 0; r < R; ++r) v[t][r] = ld2(gdata, b * n + j + r * m);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[t][r] = ld2(lds, b * n + j + r * m);
+            }
+            if (Ns > 1) {
+                const double2 w0 = ld2(tw, jm * base);
+                const double2 w1 = make_double2(w0.x, -sgn * w0.y);
+                double2 w = w1;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    v[t][r] = cmul(v[t][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
+                }
+            }
+            butterfly<R>(v[t], sgn);
+            ob[t] = b * n + jq * Ns * R + jm;
+        }
+    }
+    if (FIRST == LAST) __syncthreads();  // inputs and outputs share memory
+#pragma unroll
+    for (int t = 0; t < BPT; ++t) {
+        if (ob[t] >= 0) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if constexpr (LAST)
+                    st2(gdata, ob[t] + k * Ns, v[t][k]);
+                else
+                    st2(lds, ob[t] + k * Ns, v[t][k]);
             }
         }
-        if (R == 2) dft2(v);
-        else if (R == 3) dft3(v, sgn);
-        else if (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
-        else if (R == 5) dft5(v, sgn);
-        else if (R == 8) dft8(v, sgn);
-        else dft_small<R>(v, tw, n, sgn);
-        double2* out = dst + b * n + (j / Ns) * Ns * R + jm;
-#pragma unroll
-        for (int k = 0; k < R; ++k) out[k * Ns] = v[k];
     }
+    if (!LAST) __syncthreads();
 }
 
-// Any radix: one output per thread, O(R) work per output.
-__device__ void stage_generic(const double2* __restrict__ src, double2* __restrict__ dst, int n,
-                              int nb, int Ns, int R, const double2* __restrict__ tw, double sgn)
+// Any radix: one output per (thread, slot), O(R) work per output.
+template <bool FIRST, bool LAST, int T>
+__device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds,
+                                                        glb_d2* __restrict__ gdata, int n, int nb,
+                                                        int Ns, int R,
+                                                        const glb_d2* __restrict__ tw, double sgn)
 {
+    constexpr int OPT = kFftPerThread;  // outputs per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
-    for (int q = threadIdx.x; q < nb * n; q += blockDim.x) {
-        const int b = q / n;
-        const int qq = q - b * n;
-        const int k = qq / m;
-        const int j = qq - k * m;
-        const int jm = j % Ns;
-        const int step = (jm + k * Ns) * base;  // < n
-        const double2* in = src + b * n + j;
-        double re = 0, im = 0;
-        int e = 0;
-        for (int r = 0; r < R; ++r) {
-            const double2 v = in[r * m];
-            const double2 wv = tw[e];
-            const double wi = -sgn * wv.y;
-            re += v.x * wv.x - v.y * wi;
-            im += v.x * wi + v.y * wv.x;
-            e += step;
-            if (e >= n) e -= n;
+    const float rn = 1.0f / (float)n, rm = 1.0f / (float)m, rNs = 1.0f / (float)Ns;
+    const int total = nb * n;
+    double2 res[OPT];
+    int ob[OPT];
+#pragma unroll
+    for (int t = 0; t < OPT; ++t) {
+        const int q = threadIdx.x + t * T;
+        ob[t] = -1;
+        if (q < total) {
+            int b, qq, k, j, jq, jm;
+            divmod(q, n, rn, b, qq);
+            divmod(qq, m, rm, k, j);
+            divmod(j, Ns, rNs, jq, jm);
+            const int step = (jm + k * Ns) * base;  // < n
+            const int i0 = b * n + j;
+            double re = 0, im = 0;
+            int e = 0;
+            for (int r = 0; r < R; ++r) {
+                double2 x;
+                if constexpr (FIRST)
+                    x = ld2(gdata, i0 + r * m);
+                else
+                    x = ld2(lds, i0 + r * m);
+                const double2 wv = ld2(tw, e);
+                const double wi = -sgn * wv.y;
+                re += x.x * wv.x - x.y * wi;
+                im += x.x * wi + x.y * wv.x;
+                e += step;
+                if (e >= n) e -= n;
+            }
+            res[t] = make_double2(re, im);
+            ob[t] = b * n + jq * Ns * R + jm + k * Ns;
         }
-        dst[b * n + (j / Ns) * Ns * R + jm + k * Ns] = make_double2(re, im);
     }
+    if (FIRST == LAST) __syncthreads();
+#pragma unroll
+    for (int t = 0; t < OPT; ++t) {
+        if (ob[t] >= 0) {
+            if constexpr (LAST)
+                st2(gdata, ob[t], res[t]);
+            else
+                st2(lds, ob[t], res[t]);
+        }
+    }
+    if (!LAST) __syncthreads();
 }
 
 // Job (s, b): lines b*B .. b*B + B - 1 of grid `sel` of system s, along y
 // (axis 0: My-lines, the column-major grid) or x (axis 1: Mx-lines, the
-// transposed grid); every line is contiguous.
-__global__ __launch_bounds__(256) void k_fft_lines(const SysDesc* __restrict__ sys,
-                                                   const Job* __restrict__ jobs, int axis, int sel,
-                                                   int inverse)
+// transposed grid); every line is contiguous.  The lines live in one LDS
+// buffer of B*n points between the first and the last stage.  T threads with
+// B*n <= kFftPerThread * T.
+template <int T>
+__global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys,
+                                                 const Job* __restrict__ jobs, int axis, int sel,
+                                                 int inverse)
 {
     extern __shared__ double2 lds[];
     const Job jb = jobs[blockIdx.x];
@@ -195,37 +291,38 @@ __global__ __launch_bounds__(256) void k_fft_lines(const SysDesc* __restrict__ s
     const int nlines = axis ? d.My : d.Mx;
     const int B = axis ? d.Bx : d.By;
     const FftAxis& ax = axis ? d.ax : d.ay;
-    const double2* __restrict__ tw = axis ? d.twx : d.twy;
-    double2* __restrict__ data = d.grid[sel] + (size_t)jb.y * B * n;
+    const glb_d2* __restrict__ tw = (const glb_d2*)(axis ? d.twx : d.twy);
+    glb_d2* __restrict__ data = (glb_d2*)(d.grid[sel] + (size_t)jb.y * B * n);
+    lds_d2* __restrict__ buf = (lds_d2*)lds;
     const int nb = min(B, nlines - jb.y * B);
-    const int total = nb * n;
-    double2* src = lds;
-    double2* dst = lds + (size_t)B * n;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) src[idx] = data[idx];
-    __syncthreads();
     // the table holds exp(-2 pi i k/n); sgn is the sign of the transform's exponent
     const double sgn = inverse ? 1.0 : -1.0;
+    const int nst = ax.nst;
     int Ns = 1;
-    for (int st = 0; st < ax.nst; ++st) {
+    for (int st = 0; st < nst; ++st) {
         const int R = ax.radix[st];
+        const bool first = st == 0, last = st == nst - 1;
+#define SE2P_STAGE(RR)                                                          \
+    if (first && last) stage_reg<RR, true, true, T>(buf, data, n, nb, Ns, tw, sgn);  \
+    else if (first) stage_reg<RR, true, false, T>(buf, data, n, nb, Ns, tw, sgn);   \
+    else if (last) stage_reg<RR, false, true, T>(buf, data, n, nb, Ns, tw, sgn);    \
+    else stage_reg<RR, false, false, T>(buf, data, n, nb, Ns, tw, sgn)
         switch (R) {
-            case 2: stage<2>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 3: stage<3>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 4: stage<4>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 5: stage<5>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 7: stage<7>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 8: stage<8>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 11: stage<11>(src, dst, n, nb, Ns, tw, sgn); break;
-            case 13: stage<13>(src, dst, n, nb, Ns, tw, sgn); break;
-            default: stage_generic(src, dst, n, nb, Ns, R, tw, sgn); break;
+            case 2: SE2P_STAGE(2); break;
+            case 3: SE2P_STAGE(3); break;
+            case 4: SE2P_STAGE(4); break;
+            case 5: SE2P_STAGE(5); break;
+            case 8: SE2P_STAGE(8); break;
+            default:
+                if (first && last) stage_generic<true, true, T>(buf, data, n, nb, Ns, R, tw, sgn);
+                else if (first) stage_generic<true, false, T>(buf, data, n, nb, Ns, R, tw, sgn);
+                else if (last) stage_generic<false, true, T>(buf, data, n, nb, Ns, R, tw, sgn);
+                else stage_generic<false, false, T>(buf, data, n, nb, Ns, R, tw, sgn);
+                break;
         }
-        __syncthreads();
-        double2* tmp = src;
-        src = dst;
-        dst = tmp;
+#undef SE2P_STAGE
         Ns *= R;
     }
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) data[idx] = src[idx];
 }
 
 // out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows
@@ -238,19 +335,19 @@ __global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ s
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int rows = dir ? d.My : d.Mx, cols = dir ? d.Mx : d.My;
-    const double2* __restrict__ in = d.grid[in_sel];
-    double2* __restrict__ out = d.grid[out_sel];
+    const glb_d2* __restrict__ in = (const glb_d2*)d.grid[in_sel];
+    glb_d2* __restrict__ out = (glb_d2*)d.grid[out_sel];
     const int ntc = (cols + 31) / 32;
     const int r0 = (jb.y / ntc) * 32, c0 = (jb.y % ntc) * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int k = ty; k < 32; k += 8) {
         const int r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
+        if (r < rows && c < cols) tile[k][tx] = ld2(in, r * cols + c);
     }
     __syncthreads();
     for (int k = ty; k < 32; k += 8) {
         const int c = c0 + k, r = r0 + tx;
-        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
+        if (r < rows && c < cols) st2(out, c * rows + r, tile[tx][k]);
     }
 }
 
@@ -258,7 +355,7 @@ __global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ s
 
 int fft_lines_per_block(int n) { return n >= 256 ? 1 : (256 + n - 1) / n; }
 
-size_t fft_lds_bytes(int n) { return 2 * sizeof(double2) * (size_t)fft_lines_per_block(n) * n; }
+size_t fft_lds_bytes(int n) { return sizeof(double2) * (size_t)fft_lines_per_block(n) * n; }
 
 int fft_axis_init(FftAxis* a, int n)
 {
@@ -304,20 +401,29 @@ int fft_prepare()
 {
     static bool attr_set = false;
     if (!attr_set) {
-        SE2P_HIP_CHECK(hipFuncSetAttribute((const void*)k_fft_lines,
+        SE2P_HIP_CHECK(hipFuncSetAttribute((const void*)k_fft_lines<256>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           2 * kFftMaxLen * (int)sizeof(double2)));
+                                           kFftMaxLen * (int)sizeof(double2)));
+        SE2P_HIP_CHECK(hipFuncSetAttribute((const void*)k_fft_lines<512>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           kFftMaxLen * (int)sizeof(double2)));
         attr_set = true;
     }
     return SE2P_OK;
 }
 
-int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, size_t max_lds,
+int fft_line_class(int n) { return n > 2048 ? 1 : 0; }
+
+int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_fft_lines, dim3(njobs), dim3(256), max_lds, s, sys, jobs, axis, sel,
-                       inverse);
+    if (cls == 0)
+        hipLaunchKernelGGL(k_fft_lines<256>, dim3(njobs), dim3(256), max_lds, s, sys, jobs, axis,
+                           sel, inverse);
+    else
+        hipLaunchKernelGGL(k_fft_lines<512>, dim3(njobs), dim3(512), max_lds, s, sys, jobs, axis,
+                           sel, inverse);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

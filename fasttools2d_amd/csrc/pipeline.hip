@@ -252,8 +252,10 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         B->has_k |= G.has_k;
         B->any_dlp |= G.kind == SE2P_DLP;
         if (G.has_k) {
-            B->lds_y = std::max(B->lds_y, fft_lds_bytes(G.kg.My));
-            B->lds_x = std::max(B->lds_x, fft_lds_bytes(G.kg.Mx));
+            size_t& ly = B->lds_y[fft_line_class(G.kg.My)];
+            size_t& lx = B->lds_x[fft_line_class(G.kg.Mx)];
+            ly = std::max(ly, fft_lds_bytes(G.kg.My));
+            lx = std::max(lx, fft_lds_bytes(G.kg.Mx));
             B->maxR = std::max(B->maxR, G.kg.R);
         }
     }
@@ -298,24 +300,28 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         add(B->merge[c], [&](int i, const SysDesc& d) {
             if (want(i, d)) blocks(i, (long)d.Mx * ((d.My + 255) / 256));
         });
-        add(B->ffty[c], [&](int i, const SysDesc& d) {
-            if (want(i, d)) blocks(i, (d.Mx + d.By - 1) / d.By);
-        });
+        for (int cls = 0; cls < 2; ++cls) {
+            add(B->ffty[c][cls], [&](int i, const SysDesc& d) {
+                if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
+            });
+            add(B->fftx[c][cls], [&](int i, const SysDesc& d) {
+                if (want(i, d) && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+            });
+        }
         add(B->tr_fwd[c], [&](int i, const SysDesc& d) {
             if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
         });
-        add(B->fftx[c], [&](int i, const SysDesc& d) {
-            if (want(i, d)) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+    }
+    for (int cls = 0; cls < 2; ++cls) {
+        add(B->ifftx[cls], [&](int i, const SysDesc& d) {
+            if (geoms[i]->has_k && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+        });
+        add(B->iffty[cls], [&](int i, const SysDesc& d) {
+            if (geoms[i]->has_k && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
         });
     }
-    add(B->ifftx, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k) blocks(i, (d.My + d.Bx - 1) / d.Bx);
-    });
     add(B->tr_inv, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_k) blocks(i, transpose_tiles(d.My, d.Mx));
-    });
-    add(B->iffty, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k) blocks(i, (d.Mx + d.By - 1) / d.By);
     });
     add(B->gather, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_k && d.Nt > 0) blocks(i, (long)d.ntx * d.nty);
@@ -395,11 +401,13 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
         SE2P_TRY(batch_spread_grid(ctx, B, d_f, c));
         StageTimer st(ctx, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
-        SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, B.lds_y, B.sys, J + B.ffty[c].first,
-                                  B.ffty[c].count));
+        for (int cls = 0; cls < 2; ++cls)
+            SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, cls, B.lds_y[cls], B.sys,
+                                      J + B.ffty[c][cls].first, B.ffty[c][cls].count));
         SE2P_TRY(launch_transpose(s, Zc, Tc, 0, B.sys, J + B.tr_fwd[c].first, B.tr_fwd[c].count));
-        SE2P_TRY(launch_fft_lines(s, 1, Tc, 0, B.lds_x, B.sys, J + B.fftx[c].first,
-                                  B.fftx[c].count));
+        for (int cls = 0; cls < 2; ++cls)
+            SE2P_TRY(launch_fft_lines(s, 1, Tc, 0, cls, B.lds_x[cls], B.sys,
+                                      J + B.fftx[c][cls].first, B.fftx[c][cls].count));
     }
     {
         StageTimer st(ctx, "filter");
@@ -408,9 +416,13 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
     }
     {
         StageTimer st(ctx, "fft");
-        SE2P_TRY(launch_fft_lines(s, 1, G_Z0, 1, B.lds_x, B.sys, J + B.ifftx.first, B.ifftx.count));
+        for (int cls = 0; cls < 2; ++cls)
+            SE2P_TRY(launch_fft_lines(s, 1, G_Z0, 1, cls, B.lds_x[cls], B.sys,
+                                      J + B.ifftx[cls].first, B.ifftx[cls].count));
         SE2P_TRY(launch_transpose(s, G_Z0, G_W, 1, B.sys, J + B.tr_inv.first, B.tr_inv.count));
-        SE2P_TRY(launch_fft_lines(s, 0, G_W, 1, B.lds_y, B.sys, J + B.iffty.first, B.iffty.count));
+        for (int cls = 0; cls < 2; ++cls)
+            SE2P_TRY(launch_fft_lines(s, 0, G_W, 1, cls, B.lds_y[cls], B.sys,
+                                      J + B.iffty[cls].first, B.iffty[cls].count));
     }
     StageTimer st(ctx, "gather");
     return launch_gather(s, B.maxR, B.sys, J + B.gather.first, B.gather.count, d_u,

@@ -227,8 +227,10 @@ int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, in
 int launch_fill_dens(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs,
                      const double* f);
 // Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
-// x (axis 1: Mx-lines of the transposed grid); max_lds from fft_lds_bytes.
-int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, size_t max_lds,
+// x (axis 1: Mx-lines of the transposed grid); every job's line length must be of
+// class `cls` (fft_line_class); max_lds from fft_lds_bytes.
+int fft_line_class(int n);
+int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs);
 size_t fft_lds_bytes(int n);
 // Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
@@ -283,10 +285,10 @@ struct BatchExec {
     JobRange real[2][2][2];  // [kind][tier][one-cell]
     JobRange spread[2][2];   // [c][R == 48]
     JobRange merge[2];       // [c]
-    JobRange ffty[2], fftx[2], tr_fwd[2];  // forward passes of component pair c
+    JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
     JobRange filter[2];      // by kind
-    JobRange ifftx, tr_inv, iffty, gather;
-    size_t lds_y = 0, lds_x = 0;
+    JobRange ifftx[2], iffty[2], tr_inv, gather;  // [line class]
+    size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
     int maxR = 32;
 };
 // Builds descriptors (f and u offsets: systems back to back, 2 Ns and 2 Nt
