@@ -284,7 +284,7 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     W[gid] = make_double2((g1.re - g2.im) * scale, (g1.im + g2.re) * scale);
 }
 
-__global__ __launch_bounds__(64) void k_gather(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs,
                                                double* __restrict__ u_base, int accumulate,
                                                int use_zm)
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64) void k_gather(const SysDesc* __restrict__ sys,
     const int2* __restrict__ node = d.gt_node;
     const double2* __restrict__ off = d.gt_off;
     const double2* __restrict__ W = d.grid[G_W];
-    const int Mx = d.Mx, My = d.My, TX = d.TX, TY = d.TY, ntx = d.ntx, P = d.P, R = d.R;
+    const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, P = d.P, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
     const double* __restrict__ e1tab = d.e1tab;
     const double* zm = use_zm ? d.zm : nullptr;
@@ -449,6 +449,12 @@ int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi,
     g->TX = g->TY = g->R - P;
     g->ntx = (Mx + g->TX - 1) / g->TX;
     g->nty = (My + g->TY - 1) / g->TY;
+    // gathering tiles: 32 x 32 start nodes (region up to 56 x 56 for P <= 24) for
+    // 256 lanes per tile
+    g->gTX = g->gTY = P <= 24 ? 32 : g->TX;
+    g->gR = g->gTX + P;
+    g->gntx = (Mx + g->gTX - 1) / g->gTX;
+    g->gnty = (My + g->gTY - 1) / g->gTY;
     return SE2P_OK;
 }
 
@@ -495,7 +501,7 @@ int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, 
 {
     if (njobs <= 0) return SE2P_OK;
     const size_t lds = sizeof(double2) * (size_t)maxR * maxR;
-    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(64), lds, s, sys, jobs, u, accumulate, use_zm);
+    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(256), lds, s, sys, jobs, u, accumulate, use_zm);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -107,7 +107,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(fft_twiddles(ctx, ws, Mx, WS_TWIDDLE_X, &G->twx));
         SE2P_TRY(fft_twiddles(ctx, ws, My, WS_TWIDDLE_Y, &G->twy));
         const KGeom& g = G->kg;
-        const int ntiles = g.ntx * g.nty;
+        const int ntiles = g.ntx * g.nty, gtiles = g.gntx * g.gnty;
         const size_t Gn = (size_t)Mx * My;
         void *skey, *scount, *tkey, *tcount;
         SE2P_TRY(ws_get(ctx, ws, WS_E1TAB, sizeof(double) * 64, &p));
@@ -123,10 +123,10 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_OFF, sizeof(double2) * (size_t)Ns, &p));
         G->sp_off = (double2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_KEY, sizeof(int) * (size_t)Nt, &tkey));
-        SE2P_TRY(ws_get(ctx, ws, WS_GAT_COUNT, sizeof(int) * (size_t)ntiles, &tcount));
+        SE2P_TRY(ws_get(ctx, ws, WS_GAT_COUNT, sizeof(int) * (size_t)gtiles, &tcount));
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_PERM, sizeof(int) * (size_t)Nt, &p));
         G->gt_perm = (int*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_GAT_START, sizeof(int) * (size_t)(ntiles + 1), &p));
+        SE2P_TRY(ws_get(ctx, ws, WS_GAT_START, sizeof(int) * (size_t)(gtiles + 1), &p));
         G->gt_start = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_NODE, sizeof(int2) * (size_t)Nt, &p));
         G->gt_node = (int2*)p;
@@ -153,10 +153,10 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)skey, (int*)scount, G->sp_node, G->sp_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, ntiles, G->sp_perm,
                              G->sp_start, WS_SORT_TMP));
-        SE2P_TRY(launch_fill_int(s, (int*)tcount, ntiles, 0));
-        SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
+        SE2P_TRY(launch_fill_int(s, (int*)tcount, gtiles, 0));
+        SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.gTX, g.gTY, g.gntx,
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
-        SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, ntiles, G->gt_perm,
+        SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, gtiles, G->gt_perm,
                              G->gt_start, WS_SORT_TMP));
     }
     return SE2P_OK;
@@ -205,6 +205,10 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->ntx = g.ntx;
         d->nty = g.nty;
         d->R = g.R;
+        d->gTX = g.gTX;
+        d->gTY = g.gTY;
+        d->gntx = g.gntx;
+        d->gR = g.gR;
         d->Bx = fft_lines_per_block(g.Mx);
         d->By = fft_lines_per_block(g.My);
         d->t = -2 * g.xi * g.xi / g.eta;
@@ -256,7 +260,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             size_t& lx = B->lds_x[fft_line_class(G.kg.Mx)];
             ly = std::max(ly, fft_lds_bytes(G.kg.My));
             lx = std::max(lx, fft_lds_bytes(G.kg.Mx));
-            B->maxR = std::max(B->maxR, G.kg.R);
+            B->maxR = std::max(B->maxR, G.kg.gR);
         }
     }
     std::vector<Job> J;
@@ -324,7 +328,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         if (geoms[i]->has_k) blocks(i, transpose_tiles(d.My, d.Mx));
     });
     add(B->gather, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k && d.Nt > 0) blocks(i, (long)d.ntx * d.nty);
+        if (geoms[i]->has_k && d.Nt > 0) blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
     });
     if (B->has_k) SE2P_TRY(fft_prepare());
     void *psys, *pjobs;

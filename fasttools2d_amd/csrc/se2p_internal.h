@@ -135,7 +135,8 @@ int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, doubl
 struct KGeom {
     int Mx, My, P;
     double Lx, Ly, h, xi, eta, w;
-    int TX, TY, ntx, nty, R;  // spread/gather tiles (TX x TY start nodes), region R x R
+    int TX, TY, ntx, nty, R;  // spreading tiles (TX x TY start nodes), region R x R
+    int gTX, gTY, gntx, gnty, gR;  // gathering tiles, region gR x gR
 };
 int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi, double eta,
                double w);
@@ -179,6 +180,7 @@ struct SysDesc {
     int zm_nblk;
     // k space
     int Mx, My, P, TX, TY, ntx, nty, R, Bx, By;
+    int gTX, gTY, gntx, gR;
     double t, w, h, xi, eta, gscale;
     const int *sp_start, *sp_perm, *gt_start, *gt_perm;
     const int2 *sp_node, *gt_node;
@@ -216,7 +218,8 @@ int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jo
 int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs);
 // Filter: W (and D for the DLP) in transposed layout Z[iy*Mx + ix] -> Z0.
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
-// Gather from W into u (+= when accumulate; + the DLP zero mode when use_zm).
+// Gather from W into u (+= when accumulate; + the DLP zero mode when use_zm);
+// maxR: the largest gathering region side of the jobs' systems.
 int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
                   double* u, int accumulate, int use_zm);
 // DLP zero mode sum_s (f.n)_s x_s / (Lx Ly) from the sorted records: partial
@@ -289,7 +292,7 @@ struct BatchExec {
     JobRange filter[2];      // by kind
     JobRange ifftx[2], iffty[2], tr_inv, gather;  // [line class]
     size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
-    int maxR = 32;
+    int maxR = 32;   // largest gathering region side
 };
 // Builds descriptors (f and u offsets: systems back to back, 2 Ns and 2 Nt
 // doubles each) and job lists; synchronises the stream (host staging).
