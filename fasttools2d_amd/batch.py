@@ -21,12 +21,12 @@ class Batch:
         self.ctx = ctx
         self.ns = [len(y["psrc"]) for y in systems]
         self.nt = [len(y["ptar"]) for y in systems]
+        self.nc = [_lib.out_comps(y.get("kind", "slp")) for y in systems]
         tmp = []
         arr = (System * max(1, len(systems)))()
         try:
             for i, y in enumerate(systems):
-                kind = _lib.SE2P_SLP if y.get("kind", "slp") in ("slp", _lib.SE2P_SLP) \
-                    else _lib.SE2P_DLP
+                kind = _lib.kind_code(y.get("kind", "slp"))
                 dps = DeviceArray.from_host(ctx, y["psrc"])
                 dpt = DeviceArray.from_host(ctx, y["ptar"])
                 dn = DeviceArray.from_host(ctx, y["nrm"]) if kind == _lib.SE2P_DLP else None
@@ -46,7 +46,8 @@ class Batch:
         self.f = DeviceArray(ctx, (max(1, self.nf),))
         self.u = DeviceArray(ctx, (max(1, self.nu),))
         self.f_off = np.concatenate([[0], np.cumsum(2 * np.array(self.ns, dtype=np.int64))])
-        self.u_off = np.concatenate([[0], np.cumsum(2 * np.array(self.nt, dtype=np.int64))])
+        self.u_off = np.concatenate([[0], np.cumsum(np.array(self.nc, dtype=np.int64) *
+                                                    np.array(self.nt, dtype=np.int64))])
 
     def apply_device(self, d_f, d_u, graph=False):
         fn = lib().se2p_batch_apply_graph if graph else lib().se2p_batch_apply
@@ -59,7 +60,8 @@ class Batch:
                                           for f in fs]))
         self.apply_device(self.f.ptr, self.u.ptr, graph)
         u = self.u.download() if self.nu else np.zeros(0)
-        return [u[self.u_off[i]:self.u_off[i + 1]].reshape(-1, 2) for i in range(len(self.nt))]
+        return [u[self.u_off[i]:self.u_off[i + 1]].reshape(-1, 2) if self.nc[i] == 2
+                else u[self.u_off[i]:self.u_off[i + 1]] for i in range(len(self.nt))]
 
     def close(self):
         if getattr(self, "_h", None):

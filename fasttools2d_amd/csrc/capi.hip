@@ -137,6 +137,11 @@ namespace {
 // ---------------------------------------------------------------------------
 // Validation
 // ---------------------------------------------------------------------------
+bool valid_kind(int kind)
+{
+    return kind == SE2P_SLP || kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE;
+}
+
 int check_common(se2p_ctx* ctx, int Ns, int Nt, double Lx, double Ly)
 {
     if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
@@ -448,6 +453,66 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     return SE2P_OK;
 }
 
+// StokesSLP_pressure_ewald_2p.m:1-138: wrap, the SLP recipe, pr + pk with the
+// zero mode -sum(f1 x + f2 y)/(2 Lx Ly) added to pk (:127-128).
+int pressure_driver(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
+                    const double* ytar, const double* f1, const double* f2, int Ns, int Nt,
+                    double Lx, double Ly, const se2p_options* opt, double* p_out,
+                    double* pr_out, double* pk_out, double* xi_out)
+{
+    SE2P_TRY(check_common(ctx, Ns, Nt, Lx, Ly));
+    SE2P_TRY(need_ptr(xsrc, Ns, "xsrc"));
+    SE2P_TRY(need_ptr(ysrc, Ns, "ysrc"));
+    SE2P_TRY(need_ptr(f1, Ns, "f1"));
+    SE2P_TRY(need_ptr(f2, Ns, "f2"));
+    SE2P_TRY(need_ptr(xtar, Nt, "xtar"));
+    SE2P_TRY(need_ptr(ytar, Nt, "ytar"));
+    SE2P_TRY(need_ptr(p_out, Nt, "p"));
+    std::vector<double> ps(2 * (size_t)Ns), pt(2 * (size_t)Nt), f(2 * (size_t)Ns);
+    double q1 = 0, q2 = 0;
+    for (int i = 0; i < Ns; ++i) {
+        ps[2 * i] = wrap1(xsrc[i], Lx);
+        ps[2 * i + 1] = wrap1(ysrc[i], Ly);
+        f[2 * i] = f1[i];
+        f[2 * i + 1] = f2[i];
+        q1 += f1[i] * f1[i];
+        q2 += f2[i] * f2[i];
+    }
+    for (int i = 0; i < Nt; ++i) {
+        pt[2 * i] = wrap1(xtar[i], Lx);
+        pt[2 * i + 1] = wrap1(ytar[i], Ly);
+    }
+    se2p_params prm;
+    SE2P_TRY(params_ref(SE2P_SLP, Ns, Nt, (q1 + q2) + 1, Lx, Ly, opt, &prm));
+    SE2P_TRY(check_params(&prm, Lx, Ly));
+    if (xi_out) *xi_out = prm.xi;
+    if (Nt == 0) return SE2P_OK;
+    double *dps, *dpt, *df;
+    SE2P_TRY(upload(ctx, WS_PTS_A, ps.data(), ps.size(), &dps));
+    SE2P_TRY(upload(ctx, WS_PTS_B, pt.data(), pt.size(), &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f.data(), f.size(), &df));
+    void* out = nullptr;
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &out));
+    double* dpr = (double*)out;
+    double* dpk = dpr + Nt;
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP_PRESSURE, dps, dpt, nullptr, Ns, Nt, Lx, Ly,
+                        prm, true, true));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    SE2P_TRY(batch_apply_real(ctx, B, df, dpr));
+    SE2P_TRY(batch_apply_k(ctx, B, df, dpk, false, true));
+    std::vector<double> h(2 * (size_t)Nt);
+    SE2P_TRY(download(ctx, h.data(), dpr, 2 * (size_t)Nt));
+    for (int i = 0; i < Nt; ++i) {
+        p_out[i] = h[i] + h[Nt + i];
+        if (pr_out) pr_out[i] = h[i];
+        if (pk_out) pk_out[i] = h[Nt + i];
+    }
+    return SE2P_OK;
+}
+
 // One mex call: upload, build the half that is needed, apply, download.
 int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const double* ptar,
              const double* f, const double* n, int Ns, int Nt, double Lx, double Ly,
@@ -461,7 +526,8 @@ int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const doubl
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Nt, &dpt));
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Ns, &df));
     if (kind == SE2P_DLP) SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Ns, &dn));
-    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &du));
+    const size_t nout = (size_t)out_comps(kind) * Nt;
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * nout, &du));
     Geom G;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, real, !real));
     BatchExec B;
@@ -471,7 +537,7 @@ int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const doubl
         SE2P_TRY(batch_apply_real(ctx, B, df, (double*)du));
     else
         SE2P_TRY(batch_apply_k(ctx, B, df, (double*)du, false, false));
-    return download(ctx, out, (double*)du, 2 * (size_t)Nt);
+    return download(ctx, out, (double*)du, nout);
 }
 
 void graph_release(GraphCache* gc)
@@ -627,6 +693,45 @@ int se2p_dlp_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, doubl
                     k_only(xi, eta, Mx, My, w, P), uk);
 }
 
+int se2p_slp_pressure_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, int Nsrc, int Ntar, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, double* pr)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, xi));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(pr, Ntar, "pr"));
+    return mex_call(ctx, SE2P_SLP_PRESSURE, true, psrc, ptar, f, nullptr, Nsrc, Ntar, Lx, Ly,
+                    real_only(xi, nside_x, nside_y), pr);
+}
+
+int se2p_slp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                             const double* f, double xi, double eta, int Nsrc, int Ntar, int Mx,
+                             int My, double Lx, double Ly, double w, int P, double* pk)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(pk, Ntar, "pk"));
+    return mex_call(ctx, SE2P_SLP_PRESSURE, false, psrc, ptar, f, nullptr, Nsrc, Ntar, Lx, Ly,
+                    k_only(xi, eta, Mx, My, w, P), pk);
+}
+
+int se2p_slp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                            const double* xtar, const double* ytar, const double* f1,
+                            const double* f2, int Nsrc, int Ntar, double Lx, double Ly,
+                            const se2p_options* opt, double* p, double* pr, double* pk,
+                            double* xi)
+{
+    if (ctx) SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return pressure_driver(ctx, xsrc, ysrc, xtar, ytar, f1, f2, Nsrc, Ntar, Lx, Ly, opt, p, pr,
+                           pk, xi);
+}
+
 void se2p_options_default(se2p_options* opt)
 {
     if (!opt) return;
@@ -640,9 +745,11 @@ int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly
                     const se2p_options* opt, se2p_params* out)
 {
     if (!out) return set_error(SE2P_ERR_ARG, "null output");
-    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
     if (!(Lx > 0) || !(Ly > 0)) return set_error(SE2P_ERR_ARG, "box lengths must be positive");
-    return params_ref(kind, Nsrc, Ntar, Q, Lx, Ly, opt, out);
+    // the pressure driver uses the velocity driver's recipe
+    // (StokesSLP_pressure_ewald_2p.m:83-104 = StokesSLP_ewald_2p.m:83-104)
+    return params_ref(kind == SE2P_DLP ? SE2P_DLP : SE2P_SLP, Nsrc, Ntar, Q, Lx, Ly, opt, out);
 }
 
 int se2p_slp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
@@ -725,7 +832,7 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
                      double Ly, const se2p_params* prm, double* d_u)
 {
     SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
-    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
     SE2P_TRY(check_params(prm, Lx, Ly));
     if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
@@ -746,7 +853,7 @@ int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     if (!out) return set_error(SE2P_ERR_ARG, "null output pointer");
     *out = nullptr;
     SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
-    if (kind != SE2P_SLP && kind != SE2P_DLP) return set_error(SE2P_ERR_ARG, "bad kind");
+    if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
     SE2P_TRY(check_params(prm, Lx, Ly));
     SE2P_TRY(need_ptr(d_psrc, Nsrc, "d_psrc"));
     SE2P_TRY(need_ptr(d_ptar, Ntar, "d_ptar"));
@@ -810,8 +917,7 @@ int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_
     if (nsys < 0 || (nsys > 0 && !systems)) return set_error(SE2P_ERR_ARG, "bad system list");
     for (int i = 0; i < nsys; ++i) {
         const se2p_system& y = systems[i];
-        if (y.kind != SE2P_SLP && y.kind != SE2P_DLP)
-            return set_error(SE2P_ERR_ARG, "system %d: bad kind", i);
+        if (!valid_kind(y.kind)) return set_error(SE2P_ERR_ARG, "system %d: bad kind", i);
         int rc = check_common(ctx, y.Nsrc, y.Ntar, y.Lx, y.Ly);
         if (rc == SE2P_OK) rc = check_params(&y.prm, y.Lx, y.Ly);
         if (rc == SE2P_OK) rc = need_ptr(y.d_psrc, y.Nsrc, "d_psrc");
@@ -834,7 +940,7 @@ int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_
                         y.Ntar, y.Lx, y.Ly, y.prm, true, true);
         gp[i] = &b->G[i];
         b->nf += 2LL * y.Nsrc;
-        b->nu += 2LL * y.Ntar;
+        b->nu += (long long)out_comps(y.kind) * y.Ntar;
     }
     if (rc == SE2P_OK) rc = batch_build(ctx, &b->bws, &b->B, gp.data(), nsys);
     if (rc != SE2P_OK) {

@@ -6,7 +6,10 @@
 // r^2 < 1e-15 take the self term -(1 + gamma/2 + ln xi) f; DLP: coincident
 // points are skipped) and every source of the 8 neighbour cells (periodic
 // images) with r^2 < Lx Ly/(nside_x nside_y).  Output scaled by 1/(4 pi) and
-// written in input order.  Sources further than one cell away are never within
+// written in input order.  The Stokeslet pressure
+// (mex_stokes_slp_pressure_real.cpp:60-160) uses the same pair set as the DLP
+// (coincident points of the own cell skipped) with the pair term
+// (r.f) e^{-xi^2 r^2}/r^2 and one output scaled by 1/(2 pi).  Sources further than one cell away are never within
 // the cutoff, so "own cell, or r^2 < cutoff^2" over any window that contains
 // the 3 x 3 stencil selects exactly the reference's pairs.
 //
@@ -47,6 +50,11 @@ template <>
 struct SrcOf<SE2P_DLP> {
     using T = DlpRec;
     static constexpr int cap = 512;
+};
+template <>
+struct SrcOf<SE2P_SLP_PRESSURE> {
+    using T = double4;
+    static constexpr int cap = 1024;
 };
 
 constexpr int kBlkThreads = 256;
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             cm = (cm << 1) | (r2 < cut2 ? 1ull : 0ull);
                             if (KIND == SE2P_SLP)
                                 zm = (zm << 1) | (r2 < ixi2 ? 1ull : 0ull);
-                            else
+                            else  // coincident points of the own cell are skipped
                                 zm = (zm << 1) | (r2 != 0 ? 1ull : 0ull);
                         }
                     }
@@ -298,8 +306,11 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     if (KIND == SE2P_SLP) {
                         const double4& q = reinterpret_cast<const double4&>(s);
                         slp_pair<TIER, false>(dx, dy, r2, xi2, ixi2, q.z, q.w, a1, a2);
-                    } else {
+                    } else if (KIND == SE2P_DLP) {
                         dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
+                    } else {
+                        const double4& q = reinterpret_cast<const double4&>(s);
+                        a1 += (dx * q.z + dy * q.w) * exp(-xi2 * r2) * rcp64(r2);
                     }
                 }
                 if (KIND == SE2P_SLP) {
@@ -322,8 +333,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         }
         if (act) {
             const int o = tperm[ti];
-            u[2 * o] = a1 / (4 * kPi);
-            u[2 * o + 1] = a2 / (4 * kPi);
+            if (KIND == SE2P_SLP_PRESSURE) {
+                u[o] = a1 / (2 * kPi);
+            } else {
+                u[2 * o] = a1 / (4 * kPi);
+                u[2 * o + 1] = a2 / (4 * kPi);
+            }
         }
     }
 }
@@ -381,7 +396,7 @@ int launch_pack_sources(hipStream_t s, int kind, const int* perm, int n, const d
 {
     if (n <= 0) return SE2P_OK;
     dim3 g((n + 255) / 256), b(256);
-    if (kind == SE2P_SLP)
+    if (rec_layout(kind) == 0)
         hipLaunchKernelGGL(k_pack_slp, g, b, 0, s, perm, n, p, f, (double4*)out);
     else
         hipLaunchKernelGGL(k_pack_dlp, g, b, 0, s, perm, n, p, f, nrm, (DlpRec*)out);
@@ -421,8 +436,10 @@ int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys,
         SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full);
     else if (kind == SE2P_SLP)
         SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
-    else
+    else if (kind == SE2P_DLP)
         SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full);
+    else
+        SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE, kE1Full);
 #undef SE2P_LAUNCH_REAL
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;

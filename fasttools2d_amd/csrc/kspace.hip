@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
 {
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
-    const int mode = d.kind == SE2P_SLP ? 0 : 1 + c;
+    const int mode = d.kind == SE2P_DLP ? 1 + c : 0;
     const int* __restrict__ tstart = d.sp_start;
     const int* __restrict__ perm = d.sp_perm;
     const int2* __restrict__ node = d.sp_node;
@@ -267,6 +267,19 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
         slp_apply(m1, m2, a, eta, q1, q2, r1, r2);
         g1 = cscale(cadd(p1, r1), 0.5);
         g2 = cscale(cadd(p2, r2), 0.5);
+    } else if (KIND == SE2P_SLP_PRESSURE) {
+        // P(k) = -i (k.q) e^{-(1-eta) K^2/(4 xi^2)}/K^2
+        // (mex_stokes_slp_pressure_kspace.cpp:120-150); its Hermitian part with
+        // k' the index -k's wavenumbers is -i ((k.q) g(k) - (k'.q) g(k'))/2.
+        cplx q1, q2;
+        unpack(Z0[gid], Z0[gm], q1, q2);
+        const double K2 = k1 * k1 + k2 * k2, M2 = m1 * m1 + m2 * m2;
+        const double gk = exp(-(1 - eta) * a * K2) / K2, gm_ = exp(-(1 - eta) * a * M2) / M2;
+        const cplx dk = cadd(cscale(q1, k1 * gk), cscale(q2, k2 * gk));
+        const cplx dm = cadd(cscale(q1, m1 * gm_), cscale(q2, m2 * gm_));
+        const cplx dd = cscale(csub(dk, dm), 0.5);
+        g1 = {dd.im, -dd.re};
+        g2 = {0.0, 0.0};
     } else {
         cplx q11, q21, q12, q22;
         unpack(Z0[gid], Z0[gm], q11, q21);  // spread (f1 n1, f2 n1)
@@ -338,6 +351,10 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
             ex *= e3x;
         }
         double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
+        if (d.nc == 1) {  // one real field (pressure): the real part of W
+            u[p] = accumulate ? u[p] + v0 : v0;
+            continue;
+        }
         if (accumulate) {
             v0 = u[2 * p] + v0;
             v1 = u[2 * p + 1] + v1;
@@ -368,11 +385,17 @@ __global__ __launch_bounds__(256) void k_zero_mode_part(const SysDesc* __restric
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const double* __restrict__ rec = d.s_rec;
+    const bool dlp = d.kind == SE2P_DLP;
     double ax = 0, ay = 0;
     for (int i = jb.y * 256 + threadIdx.x; i < d.Ns; i += d.zm_nblk * 256) {
-        const double* r = rec + 8 * (size_t)i;
-        ax += r[6] * r[0];
-        ay += r[6] * r[1];
+        if (dlp) {  // {x, y, f1, f2, n1, n2, f.n, 0}: (f.n) x
+            const double* r = rec + 8 * (size_t)i;
+            ax += r[6] * r[0];
+            ay += r[6] * r[1];
+        } else {  // {x, y, f1, f2}: f.x
+            const double* r = rec + 4 * (size_t)i;
+            ax += r[2] * r[0] + r[3] * r[1];
+        }
     }
     sx[threadIdx.x] = ax;
     sy[threadIdx.x] = ay;
@@ -400,13 +423,18 @@ __global__ void k_zero_mode_final(const SysDesc* __restrict__ sys, const Job* __
         sy += d.zm_part[2 * b + 1];
     }
     const double inv_area = 1.0 / (d.Lx * d.Ly);
-    d.zm[0] = sx * inv_area;
-    d.zm[1] = sy * inv_area;
+    if (d.kind == SE2P_DLP) {  // StokesDLP_ewald_2p.m:149-151
+        d.zm[0] = sx * inv_area;
+        d.zm[1] = sy * inv_area;
+    } else {  // StokesSLP_pressure_ewald_2p.m:127-128
+        d.zm[0] = -sx * inv_area / 2;
+        d.zm[1] = 0;
+    }
 }
 
 // Density fill of the sorted source records (positions and normals are set at
-// build): SLP {x, y, f1, f2}; DLP {x, y, f1, f2, n1, n2, f.n, 0}.
-template <int KIND>
+// build): layout 0 {x, y, f1, f2}; layout 1 (DLP) {x, y, f1, f2, n1, n2, f.n, 0}.
+template <int LAYOUT>
 __global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs,
                             const double* __restrict__ f_base)
 {
@@ -417,7 +445,7 @@ __global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restri
     const double* __restrict__ f = f_base + d.f_off;
     const int o = d.s_perm[i];
     const double f1 = f[2 * o], f2 = f[2 * o + 1];
-    if (KIND == SE2P_SLP) {
+    if (LAYOUT == 0) {
         double4* rec = reinterpret_cast<double4*>(d.s_rec);
         rec[i].z = f1;
         rec[i].w = f2;
@@ -490,8 +518,10 @@ int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, 
     if (njobs <= 0) return SE2P_OK;
     if (kind == SE2P_SLP)
         hipLaunchKernelGGL(k_filter<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
-    else
+    else if (kind == SE2P_DLP)
         hipLaunchKernelGGL(k_filter<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
+    else
+        hipLaunchKernelGGL(k_filter<SE2P_SLP_PRESSURE>, dim3(njobs), dim3(256), 0, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
@@ -517,14 +547,14 @@ int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, in
     return SE2P_OK;
 }
 
-int launch_fill_dens(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs,
+int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* jobs, int njobs,
                      const double* f)
 {
     if (njobs <= 0) return SE2P_OK;
-    if (kind == SE2P_SLP)
-        hipLaunchKernelGGL(k_fill_dens<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
+    if (layout == 0)
+        hipLaunchKernelGGL(k_fill_dens<0>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
     else
-        hipLaunchKernelGGL(k_fill_dens<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
+        hipLaunchKernelGGL(k_fill_dens<1>, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -73,7 +73,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->t_perm = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_TAR_START, sizeof(int) * (size_t)(nbox + 1), &p));
         G->t_start = (int*)p;
-        const size_t rec = kind == SE2P_SLP ? 32 : 64;
+        const size_t rec = rec_layout(kind) == 0 ? 32 : 64;
         SE2P_TRY(ws_get(ctx, ws, WS_SRC_SORTED, rec * (size_t)Ns, &p));
         G->s_rec = (double*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_TAR_SORTED, sizeof(double2) * (size_t)Nt, &p));
@@ -92,7 +92,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
-        if (kind == SE2P_DLP) {
+        if (kind != SE2P_SLP) {  // zero mode of the DLP and of the SLP pressure
             SE2P_TRY(ws_get(ctx, ws, WS_REDUCE, sizeof(double) * 512, &p));
             G->zm_part = (double*)p;
             SE2P_TRY(ws_get(ctx, ws, WS_ZM, sizeof(double) * 2, &p));
@@ -166,6 +166,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
 {
     *d = SysDesc{};
     d->kind = G.kind;
+    d->nc = out_comps(G.kind);
     d->Ns = G.Ns;
     d->Nt = G.Nt;
     d->e1tier = G.e1tier;
@@ -251,7 +252,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         const Geom& G = *geoms[i];
         sysdesc_from_geom(G, foff, uoff, &hs[i]);
         foff += 2L * G.Ns;
-        uoff += 2L * G.Nt;
+        uoff += (long)out_comps(G.kind) * G.Nt;
         B->has_real |= G.has_real;
         B->has_k |= G.has_k;
         B->any_dlp |= G.kind == SE2P_DLP;
@@ -272,10 +273,11 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     auto blocks = [&](int i, long n) {
         for (long b = 0; b < n; ++b) J.push_back(make_int2(i, (int)b));
     };
-    for (int kind = 0; kind < 2; ++kind) {
-        add(B->fill[kind], [&](int i, const SysDesc& d) {
-            if (geoms[i]->has_real && d.kind == kind) blocks(i, (d.Ns + 255) / 256);
+    for (int layout = 0; layout < 2; ++layout)
+        add(B->fill[layout], [&](int i, const SysDesc& d) {
+            if (geoms[i]->has_real && rec_layout(d.kind) == layout) blocks(i, (d.Ns + 255) / 256);
         });
+    for (int kind = 0; kind < kNumKinds; ++kind) {
         add(B->filter[kind], [&](int i, const SysDesc& d) {
             if (geoms[i]->has_k && d.kind == kind) blocks(i, ((long)d.Mx * d.My + 255) / 256);
         });
@@ -288,10 +290,10 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 });
     }
     add(B->zm_part, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && d.kind == SE2P_DLP) blocks(i, d.zm_nblk);
+        if (geoms[i]->has_real && d.kind != SE2P_SLP) blocks(i, d.zm_nblk);
     });
     add(B->zm_sys, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && d.kind == SE2P_DLP) blocks(i, 1);
+        if (geoms[i]->has_real && d.kind != SE2P_SLP) blocks(i, 1);
     });
     for (int c = 0; c < 2; ++c) {
         auto want = [&](int i, const SysDesc& d) {
@@ -354,14 +356,14 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
     const Job* J = B.jobs;
     {
         StageTimer st(ctx, "pack");
-        for (int kind = 0; kind < 2; ++kind)
-            SE2P_TRY(launch_fill_dens(s, kind, B.sys, J + B.fill[kind].first, B.fill[kind].count,
-                                      d_f));
+        for (int layout = 0; layout < 2; ++layout)
+            SE2P_TRY(launch_fill_dens(s, layout, B.sys, J + B.fill[layout].first,
+                                      B.fill[layout].count, d_f));
         SE2P_TRY(launch_zero_mode(s, B.sys, J + B.zm_part.first, B.zm_part.count,
                                   J + B.zm_sys.first, B.zm_sys.count));
     }
     StageTimer st(ctx, "real");
-    for (int kind = 0; kind < 2; ++kind)
+    for (int kind = 0; kind < kNumKinds; ++kind)
         for (int tier = 0; tier < 2; ++tier)
             for (int uni = 0; uni < 2; ++uni) {
                 const JobRange& r = B.real[kind][tier][uni];
@@ -415,7 +417,7 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
     }
     {
         StageTimer st(ctx, "filter");
-        for (int kind = 0; kind < 2; ++kind)
+        for (int kind = 0; kind < kNumKinds; ++kind)
             SE2P_TRY(launch_filter(s, kind, B.sys, J + B.filter[kind].first, B.filter[kind].count));
     }
     {

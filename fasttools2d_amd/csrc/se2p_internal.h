@@ -14,6 +14,13 @@ namespace se2p {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Kinds: SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE.
+constexpr int kNumKinds = 3;
+// Values per target: 2 (velocity) or 1 (pressure).
+inline int out_comps(int kind) { return kind == SE2P_SLP_PRESSURE ? 1 : 2; }
+// Sorted source records: DLP {x, y, f1, f2, n1, n2, f.n, 0}, others {x, y, f1, f2}.
+inline int rec_layout(int kind) { return kind == SE2P_DLP ? 1 : 0; }
+
 // ---------------------------------------------------------------------------
 // Status helpers
 // ---------------------------------------------------------------------------
@@ -167,7 +174,7 @@ int fft_lines_per_block(int n);
 enum GridSel { G_Z0 = 0, G_Z1 = 1, G_W = 2, G_D = 3 };
 
 struct SysDesc {
-    int kind, Ns, Nt, e1tier;
+    int kind, Ns, Nt, e1tier, nc;  // nc: values per target
     long f_off, u_off;  // offsets (doubles) of this system in the apply's f and u
     double Lx, Ly;
     // real space
@@ -222,12 +229,13 @@ int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, 
 // maxR: the largest gathering region side of the jobs' systems.
 int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
                   double* u, int accumulate, int use_zm);
-// DLP zero mode sum_s (f.n)_s x_s / (Lx Ly) from the sorted records: partial
-// sums over jobs (s, b), b < zm_nblk, then one job per system.
+// Zero modes from the sorted records (DLP: sum_s (f.n)_s x_s / (Lx Ly); SLP
+// pressure: -sum_s f_s.x_s / (2 Lx Ly)): partial sums over jobs (s, b), b <
+// zm_nblk, then one job per system.
 int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, int npart,
                      const Job* sys_jobs, int nsys);
-// Density fill of the sorted source records.
-int launch_fill_dens(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs,
+// Density fill of the sorted source records (layout: rec_layout).
+int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* jobs, int njobs,
                      const double* f);
 // Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
 // x (axis 1: Mx-lines of the transposed grid); every job's line length must be of
@@ -283,13 +291,13 @@ struct BatchExec {
     bool has_real = false, has_k = false, any_dlp = false;
     SysDesc* sys = nullptr;
     Job* jobs = nullptr;
-    JobRange fill[2];     // by kind
+    JobRange fill[2];     // by record layout
     JobRange zm_part, zm_sys;
-    JobRange real[2][2][2];  // [kind][tier][one-cell]
+    JobRange real[kNumKinds][2][2];  // [kind][tier][one-cell]
     JobRange spread[2][2];   // [c][R == 48]
     JobRange merge[2];       // [c]
     JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
-    JobRange filter[2];      // by kind
+    JobRange filter[kNumKinds];  // by kind
     JobRange ifftx[2], iffty[2], tr_inv, gather;  // [line class]
     size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
     int maxR = 32;   // largest gathering region side

@@ -94,6 +94,50 @@ def mex_stokes_dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P, ctx=N
     return out.T.copy()
 
 
+def mex_stokes_slp_pressure_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, ctx=None):
+    """pr = mex_stokes_slp_pressure_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)
+    (mex_stokes_slp_pressure_real.cpp:17-160); 1 x Ntar."""
+    ps, pt, ff = _as_2xn(psrc, "psrc"), _as_2xn(ptar, "ptar"), _as_2xn(f, "f")
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    out = np.zeros(pt.shape[0])
+    check(lib().se2p_slp_pressure_real(_ctx(ctx), _p(ps), _p(pt), _p(ff), ps.shape[0],
+                                       pt.shape[0], float(xi), int(nside_x), int(nside_y),
+                                       float(Lx), float(Ly), _p(out)))
+    return out[None, :].copy()
+
+
+def mex_stokes_slp_pressure_kspace(psrc, ptar, f, xi, eta, Mx, My, Lx, Ly, w, P, ctx=None):
+    """pk = mex_stokes_slp_pressure_kspace(psrc, ptar, f, xi, eta, Mx, My, Lx, Ly, w, P)
+    (mex_stokes_slp_pressure_kspace.cpp:7-190, f before xi); 1 x Ntar, no zero mode."""
+    ps, pt, ff = _as_2xn(psrc, "psrc"), _as_2xn(ptar, "ptar"), _as_2xn(f, "f")
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    out = np.zeros(pt.shape[0])
+    check(lib().se2p_slp_pressure_kspace(_ctx(ctx), _p(ps), _p(pt), _p(ff), float(xi),
+                                         float(eta), ps.shape[0], pt.shape[0], int(Mx), int(My),
+                                         float(Lx), float(Ly), float(w), int(P), _p(out)))
+    return out[None, :].copy()
+
+
+def StokesSLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, P=24, Nb=None,
+                                tol=1e-16, verbose=0, ctx=None):
+    """[p, pr, pk, xi] = StokesSLP_pressure_ewald_2p(...) (StokesSLP_pressure_ewald_2p.m:1-138)."""
+    xs, ys, xt, yt = (_vec(v, n) for v, n in ((xsrc, "xsrc"), (ysrc, "ysrc"), (xtar, "xtar"),
+                                                 (ytar, "ytar")))
+    a1, a2 = _vec(f1, "f1"), _vec(f2, "f2")
+    Ns, Nt = xs.size, xt.size
+    if ys.size != Ns or a1.size != Ns or a2.size != Ns or yt.size != Nt:
+        raise ValueError("inconsistent input lengths")
+    p, pr, pk = np.zeros(Nt), np.zeros(Nt), np.zeros(Nt)
+    xi = ctypes.c_double()
+    o = _options(P, Nb, tol, verbose)
+    check(lib().se2p_slp_pressure_ewald(_ctx(ctx), _p(xs), _p(ys), _p(xt), _p(yt), _p(a1),
+                                        _p(a2), Ns, Nt, float(Lx), float(Ly), ctypes.byref(o),
+                                        _p(p), _p(pr), _p(pk), ctypes.byref(xi)))
+    return p, pr, pk, xi.value
+
+
 def _options(P=24, Nb=None, tol=1e-16, verbose=0):
     o = Options()
     lib().se2p_options_default(ctypes.byref(o))
@@ -105,10 +149,11 @@ def _options(P=24, Nb=None, tol=1e-16, verbose=0):
 
 
 def ewald_params(kind, Nsrc, Ntar, Q, Lx, Ly, P=24, Nb=None, tol=1e-16):
-    """Reference parameter recipe (StokesSLP_ewald_2p.m:84-104 / StokesDLP_ewald_2p.m:104-125)."""
+    """Reference parameter recipe (StokesSLP_ewald_2p.m:84-104 / StokesDLP_ewald_2p.m:104-125;
+    the pressure uses the SLP recipe, StokesSLP_pressure_ewald_2p.m:83-104)."""
     o = _options(P, Nb, tol)
     prm = Params()
-    k = _lib.SE2P_SLP if kind in ("slp", _lib.SE2P_SLP) else _lib.SE2P_DLP
+    k = _lib.kind_code(kind)
     check(lib().se2p_params_ref(k, int(Nsrc), int(Ntar), float(Q), float(Lx), float(Ly),
                                 ctypes.byref(o), ctypes.byref(prm)))
     return prm.as_dict()

@@ -58,7 +58,7 @@ class Plan:
 
     def __init__(self, ctx, kind, psrc, ptar, Lx, Ly, prm, nrm=None):
         self.ctx = ctx
-        self.kind = _lib.SE2P_SLP if kind in ("slp", _lib.SE2P_SLP) else _lib.SE2P_DLP
+        self.kind = _lib.kind_code(kind)
         self.Ns, self.Nt = len(psrc), len(ptar)
         self.prm = params_struct(prm) if isinstance(prm, dict) else prm
         dps = DeviceArray.from_host(ctx, psrc)
@@ -76,7 +76,8 @@ class Plan:
             if dn:
                 dn.free()
         self.f = DeviceArray(ctx, (self.Ns, 2))
-        self.u = DeviceArray(ctx, (self.Nt, 2))
+        nc = _lib.out_comps(self.kind)
+        self.u = DeviceArray(ctx, (self.Nt, 2) if nc == 2 else (self.Nt,))
 
     def apply_device(self, d_f, d_u, graph=False):
         fn = lib().se2p_plan_apply_graph if graph else lib().se2p_plan_apply

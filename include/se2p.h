@@ -33,6 +33,7 @@ extern "C" {
 
 #define SE2P_SLP 0 /* Stokeslet, single-layer potential */
 #define SE2P_DLP 1 /* stresslet, double-layer potential */
+#define SE2P_SLP_PRESSURE 2 /* pressure of the Stokeslet (one value per target) */
 
 /* Arithmetic of a plan (only FP64 in this build). */
 #define SE2P_FP64 0
@@ -100,6 +101,31 @@ typedef struct {
     int e1_tier;   /* real-space E1: 0 full FP64 (reference), 1 rel. error < 1e-12 */
 } se2p_params;
 
+/* ------------------------------------------------------------------------ */
+/* Derived quantity: pressure of the Stokeslet.                              */
+/* ------------------------------------------------------------------------ */
+/* Replaces mex_stokes_slp_pressure_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_pressure_real.cpp:17-160).
+ * pr: 1 x Ntar. */
+int se2p_slp_pressure_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, int Nsrc, int Ntar, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, double* pr);
+/* Replaces mex_stokes_slp_pressure_kspace(psrc, ptar, f, xi, eta, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_slp_pressure_kspace.cpp:7-190; note f before xi).  pk: 1 x Ntar,
+ * without the zero mode (the driver adds it). */
+int se2p_slp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                             const double* f, double xi, double eta, int Nsrc, int Ntar, int Mx,
+                             int My, double Lx, double Ly, double w, int P, double* pk);
+/* StokesSLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, ...)
+ * (StokesSLP_pressure_ewald_2p.m:1-138): wrapping, the SLP parameter recipe, p =
+ * pr + pk with the zero mode -sum(f1 x + f2 y)/(2 Lx Ly) in pk.  p, pr, pk: Ntar
+ * values each (pr, pk and xi may be NULL). */
+int se2p_slp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                            const double* xtar, const double* ytar, const double* f1,
+                            const double* f2, int Nsrc, int Ntar, double Lx, double Ly,
+                            const se2p_options* opt, double* p, double* pr, double* pk,
+                            double* xi);
+
 void se2p_options_default(se2p_options* opt);
 
 /* Parameter recipe of StokesSLP_ewald_2p.m:27-37,84-104,147-202 (kind SE2P_SLP)
@@ -139,7 +165,8 @@ int se2p_ctx_sync(se2p_ctx* ctx);
 
 /* One complete evaluation on device pointers with explicit parameters: binning,
  * real space and k space recomputed on every call, as the reference does.
- * d_n must be given for SE2P_DLP (NULL for SLP).  d_u (2 x Ntar) receives
+ * d_n must be given for SE2P_DLP (NULL otherwise).  d_u (2 x Ntar; Ntar values
+ * for SE2P_SLP_PRESSURE) receives
  * ur + uk (+ the DLP zero mode).  Stream-ordered on the context's stream; call
  * se2p_ctx_sync() before reading d_u. */
 int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
@@ -173,7 +200,7 @@ int se2p_plan_destroy(se2p_plan* plan);
 /* reference (one system per mex call, StokesSLP_ewald_2p.m:119,126).        */
 /* ------------------------------------------------------------------------ */
 typedef struct se2p_system {
-    int kind;             /* SE2P_SLP or SE2P_DLP */
+    int kind;             /* SE2P_SLP, SE2P_DLP or SE2P_SLP_PRESSURE */
     int Nsrc, Ntar;
     double Lx, Ly;
     const double* d_psrc; /* 2 x Nsrc device array, read during creation only */

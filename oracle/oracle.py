@@ -55,6 +55,11 @@ def lib():
             "orc_dlp_zero_mode": [_P, _P, _P, _I, _D, _D, _P],
             "orc_params": [_I, _I, _D, _D, _D, _I, _D, _D, _P, _P],
             "orc_rat": [_D, _P, _P],
+            "orc_slp_pressure_real": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+            "orc_slp_pressure_filter": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D],
+            "orc_slp_pressure_kspace": [_P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I, _P],
+            "orc_slp_pressure_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
+            "orc_slp_pressure_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -63,6 +68,8 @@ def lib():
         L.orc_params.restype = _I
         L.orc_e1.argtypes = [_D]
         L.orc_e1.restype = _D
+        L.orc_slp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
+        L.orc_slp_pressure_zero_mode.restype = _D
         _lib = L
     return _lib
 
@@ -209,6 +216,54 @@ def dlp_zero_mode(psrc, f, n, Lx, Ly):
     out = np.zeros(2)
     lib().orc_dlp_zero_mode(_ptr(psrc), _ptr(f), _ptr(n), psrc.shape[0], Lx, Ly, _ptr(out))
     return out
+
+
+def slp_pressure_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly):
+    psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_slp_pressure_real(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0], ptar.shape[0],
+                                xi, nside_x, nside_y, Lx, Ly, _ptr(p))
+    return p
+
+
+def slp_pressure_filter(H1hat, H2hat, Lx, Ly, xi, eta):
+    """Pressure filter of the two spread spectra (Mx x My arrays, [ix, iy]);
+    returns the filtered spectrum."""
+    a, b = _c(H1hat.real), _c(H1hat.imag)
+    c, d = _c(H2hat.real), _c(H2hat.imag)
+    Mx, My = a.shape
+    lib().orc_slp_pressure_filter(_ptr(a), _ptr(b), _ptr(c), _ptr(d), Mx, My, Lx, Ly, xi, eta)
+    return a + 1j * b
+
+
+def slp_pressure_kspace(psrc, ptar, f, xi, eta, Mx, My, Lx, Ly, w, P):
+    psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_slp_pressure_kspace(_ptr(psrc), _ptr(ptar), _ptr(f), xi, eta, psrc.shape[0],
+                                  ptar.shape[0], Mx, My, Lx, Ly, w, P, _ptr(p))
+    return p
+
+
+def slp_pressure_zero_mode(psrc, f, Lx, Ly):
+    psrc, f = _c(psrc), _c(f)
+    return lib().orc_slp_pressure_zero_mode(_ptr(psrc), _ptr(f), psrc.shape[0], Lx, Ly)
+
+
+def slp_pressure_real_ds(psrc, ptar, f, Lx, Ly, xi):
+    psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_slp_pressure_real_ds(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0],
+                                   ptar.shape[0], Lx, Ly, xi, _ptr(p))
+    return p
+
+
+def slp_pressure_kspace_ds(psrc, ptar, f, Lx, Ly, xi, kinf):
+    """Direct k-space sum including the zero mode (stokes_slp_pressure_kspace_ds.m)."""
+    psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_slp_pressure_kspace_ds(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0],
+                                     ptar.shape[0], Lx, Ly, xi, kinf, _ptr(p))
+    return p
 
 
 def params(kind, npts, Q, Lx, Ly, P, Nb, tol):

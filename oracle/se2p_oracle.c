@@ -905,3 +905,176 @@ int orc_params(int kind, int npts, double Q, double Lx, double Ly, int P, double
     iout[6] = B;
     return (ok1 && ok2) ? 0 : 1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Derived quantity: SLP pressure                                            */
+/* ------------------------------------------------------------------------ */
+/* mex_stokes_slp_pressure_real.cpp:58-160: the velocity's cells and stencil;
+ * coincident points of the own cell skipped (r^2 == 0, :104), neighbours with
+ * r^2 < Lx Ly/(nside_x nside_y) (:139); pair term (r.f) e^{-xi^2 r^2}/r^2
+ * (:107-108), output / (2 pi) (:150-152). */
+void orc_slp_pressure_real(const double* psrc, const double* ptar, const double* f, int Ns,
+                           int Nt, double xi, int nside_x, int nside_y, double Lx, double Ly,
+                           double* p)
+{
+    cells_t c;
+    cells_build(&c, psrc, Ns, ptar, Nt, Lx, Ly, nside_x, nside_y);
+    const int nbox = nside_x * nside_y;
+    const double xi2 = xi * xi;
+    const double cut2 = Lx * Ly / nside_x / nside_y;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int bx = b % nside_x, by = b / nside_x;
+        for (int it = c.bo_t[b]; it < c.bo_t[b + 1]; ++it) {
+            int t = c.po_t[it];
+            double xt = ptar[2 * t], yt = ptar[2 * t + 1];
+            double a = 0;
+            for (int is = c.bo_s[b]; is < c.bo_s[b + 1]; ++is) {
+                int s = c.po_s[is];
+                double r1 = xt - psrc[2 * s], r2 = yt - psrc[2 * s + 1];
+                double rr = r1 * r1 + r2 * r2;
+                if (rr == 0) continue;
+                a += (r1 * f[2 * s] + r2 * f[2 * s + 1]) * exp(-xi2 * rr) / rr;
+            }
+            for (int j = 0; j < 8; ++j) {
+                int qx = bx + nb_dx[j], qy = by + nb_dy[j];
+                int wx = (qx + nside_x) % nside_x, wy = (qy + nside_y) % nside_y;
+                int sb = wy * nside_x + wx;
+                double ox = (Lx * (qx - wx)) / nside_x;
+                double oy = (Ly * (qy - wy)) / nside_y;
+                for (int is = c.bo_s[sb]; is < c.bo_s[sb + 1]; ++is) {
+                    int s = c.po_s[is];
+                    double r1 = xt - (psrc[2 * s] + ox), r2 = yt - (psrc[2 * s + 1] + oy);
+                    double rr = r1 * r1 + r2 * r2;
+                    if (rr < cut2) a += (r1 * f[2 * s] + r2 * f[2 * s + 1]) * exp(-xi2 * rr) / rr;
+                }
+            }
+            p[t] = a / (2 * ORC_PI);
+        }
+    }
+    cells_free(&c);
+}
+
+/* mex_stokes_slp_pressure_kspace.cpp:100-150: with Q the spread (f1, f2)
+ * spectra, P(k) = -i (k.Q) exp(-(1-eta) K^2/(4 xi^2)) / K^2 into h1; k = 0 set
+ * to zero (:152-154). */
+void orc_slp_pressure_filter(double* h1re, double* h1im, const double* h2re,
+                             const double* h2im, int Mx, int My, double Lx, double Ly,
+                             double xi, double eta)
+{
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < Mx; ++ix) {
+        double k1 = wavenumber(ix, Mx, Lx);
+        for (int iy = 0; iy < My; ++iy) {
+            size_t o = (size_t)ix * My + iy;
+            if (ix == 0 && iy == 0) {
+                h1re[o] = h1im[o] = 0;
+                continue;
+            }
+            double k2 = wavenumber(iy, My, Ly);
+            double K2 = k1 * k1 + k2 * k2;
+            double e = exp(-(1 - eta) * a * K2);
+            double dr = k1 * h1re[o] + k2 * h2re[o];
+            double di = k1 * h1im[o] + k2 * h2im[o];
+            h1re[o] = di * e / K2;
+            h1im[o] = -dr * e / K2;
+        }
+    }
+}
+
+/* mex_stokes_slp_pressure_kspace.cpp:7-190: spread (f1, f2), two forward DFTs,
+ * filter, inverse DFT, gather of the real part.  The zero mode is not added
+ * (the driver adds it, StokesSLP_pressure_ewald_2p.m:127-128). */
+void orc_slp_pressure_kspace(const double* psrc, const double* ptar, const double* f,
+                             double xi, double eta, int Ns, int Nt, int Mx, int My, double Lx,
+                             double Ly, double w, int P, double* pk)
+{
+    size_t G = (size_t)Mx * My;
+    double* b = (double*)calloc(4 * G, sizeof(double));
+    double *h1r = b, *h1i = b + G, *h2r = b + 2 * G, *h2i = b + 3 * G;
+    orc_spread(psrc, f, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h1r, h2r);
+    orc_fft2(h1r, h1i, My, Mx, 0);
+    orc_fft2(h2r, h2i, My, Mx, 0);
+    orc_slp_pressure_filter(h1r, h1i, h2r, h2i, Mx, My, Lx, Ly, xi, eta);
+    orc_fft2(h1r, h1i, My, Mx, 1);
+    for (int m = 0; m < Nt; ++m) pk[m] = 0;
+    orc_gather(h1r, 1, 0, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, pk);
+    free(b);
+}
+
+/* StokesSLP_pressure_ewald_2p.m:127-128: -sum_s (f1 x + f2 y)_s / (2 Lx Ly). */
+double orc_slp_pressure_zero_mode(const double* psrc, const double* f, int Ns, double Lx,
+                                  double Ly)
+{
+    double s = 0;
+    for (int i = 0; i < Ns; ++i) s += f[2 * i] * psrc[2 * i] + f[2 * i + 1] * psrc[2 * i + 1];
+    return -s / (2 * Lx * Ly);
+}
+
+/* stokes_slp_pressure_real_ds.m: all 3 x 3 images, |r| < 1e-13 skipped, / (2 pi). */
+void orc_slp_pressure_real_ds(const double* psrc, const double* ptar, const double* f, int Ns,
+                              int Nt, double Lx, double Ly, double xi, double* p)
+{
+    const double xi2 = xi * xi;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double a = 0;
+        for (int s = 0; s < Ns; ++s)
+            for (int jx = -1; jx <= 1; ++jx)
+                for (int jy = -1; jy <= 1; ++jy) {
+                    double r1 = ptar[2 * m] - (psrc[2 * s] + jx * Lx);
+                    double r2 = ptar[2 * m + 1] - (psrc[2 * s + 1] + jy * Ly);
+                    double rr = r1 * r1 + r2 * r2;
+                    if (sqrt(rr) < 1e-13) continue;
+                    a += (r1 * f[2 * s] + r2 * f[2 * s + 1]) * exp(-xi2 * rr) / rr;
+                }
+        p[m] = a / (2 * ORC_PI);
+    }
+}
+
+/* stokes_slp_pressure_kspace_ds.m: real part of
+ *   sum_{k != 0, |j_i| <= kinf} -i (k.f_s) e^{-K^2/(4 xi^2)}/K^2 e^{i k.(x_t - x_s)} / (Lx Ly)
+ * plus the zero mode, through the structure factors S_c(k) = sum_s f_c e^{-i k.x_s}. */
+void orc_slp_pressure_kspace_ds(const double* psrc, const double* ptar, const double* f,
+                                int Ns, int Nt, double Lx, double Ly, double xi, int kinf,
+                                double* p)
+{
+    const int K = 2 * kinf;
+    const size_t nm = (size_t)K * K;
+    double* S = (double*)calloc(nm * 4, sizeof(double));
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)nm; ++q) {
+        int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+        double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+        double* s = S + 4 * q;
+        for (int i = 0; i < Ns; ++i) {
+            double ph = -(k1 * psrc[2 * i] + k2 * psrc[2 * i + 1]);
+            double c = cos(ph), sn = sin(ph);
+            s[0] += f[2 * i] * c;
+            s[1] += f[2 * i] * sn;
+            s[2] += f[2 * i + 1] * c;
+            s[3] += f[2 * i + 1] * sn;
+        }
+    }
+    const double zm = orc_slp_pressure_zero_mode(psrc, f, Ns, Lx, Ly);
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double acc = 0;
+        for (size_t q = 0; q < nm; ++q) {
+            int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+            if (j1 == 0 && j2 == 0) continue;
+            double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+            double K2 = k1 * k1 + k2 * k2;
+            double g = exp(-a * K2) / K2;
+            const double* s = S + 4 * q;
+            double Ar = k1 * s[0] + k2 * s[2], Ai = k1 * s[1] + k2 * s[3];
+            double ph = k1 * ptar[2 * m] + k2 * ptar[2 * m + 1];
+            /* Re(-i (Ar + i Ai) e^{i ph}) = Ai cos ph + Ar sin ph */
+            acc += g * (Ai * cos(ph) + Ar * sin(ph));
+        }
+        p[m] = acc / (Lx * Ly) + zm;
+    }
+    free(S);
+}

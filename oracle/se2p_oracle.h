@@ -93,7 +93,28 @@ int orc_params(int kind, int npts, double Q, double Lx, double Ly, int P, double
 /* MATLAB rat(X) with its default tolerance 1e-6*norm(X,1). */
 void orc_rat(double X, int* N, int* D);
 
+/* Derived quantity: SLP pressure (mex_stokes_slp_pressure_{real,kspace}.cpp,
+ * StokesSLP_pressure_ewald_2p.m, tests/direct_sums/matlab_sums/
+ * stokes_slp_pressure_{real,kspace}_ds.m).  One value per target. */
+void orc_slp_pressure_real(const double* psrc, const double* ptar, const double* f, int Ns,
+                           int Nt, double xi, int nside_x, int nside_y, double Lx, double Ly,
+                           double* p);
+void orc_slp_pressure_filter(double* h1re, double* h1im, const double* h2re,
+                             const double* h2im, int Mx, int My, double Lx, double Ly,
+                             double xi, double eta);
+void orc_slp_pressure_kspace(const double* psrc, const double* ptar, const double* f,
+                             double xi, double eta, int Ns, int Nt, int Mx, int My, double Lx,
+                             double Ly, double w, int P, double* pk);
+double orc_slp_pressure_zero_mode(const double* psrc, const double* f, int Ns, double Lx,
+                                  double Ly);
+void orc_slp_pressure_real_ds(const double* psrc, const double* ptar, const double* f, int Ns,
+                              int Nt, double Lx, double Ly, double xi, double* p);
+void orc_slp_pressure_kspace_ds(const double* psrc, const double* ptar, const double* f,
+                                int Ns, int Nt, double Lx, double Ly, double xi, int kinf,
+                                double* p);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif
