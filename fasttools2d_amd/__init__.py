@@ -5,17 +5,19 @@ include/se2p.h); this package is its Python front end, mirroring the
 reference's PeriodicEwald functions.
 """
 from ._lib import (Context, Se2pError, default_context, device_count, LIB_PATH,  # noqa: F401
-                   SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE)
-from .periodic_ewald import (StokesDLP_ewald_2p, StokesSLP_ewald_2p,  # noqa: F401
-                             StokesSLP_pressure_ewald_2p, ewald_params, mex_stokes_dlp_kspace,
-                             mex_stokes_dlp_real, mex_stokes_slp_kspace,
-                             mex_stokes_slp_pressure_kspace, mex_stokes_slp_pressure_real,
-                             mex_stokes_slp_real)
+                   SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE, SE2P_DLP_PRESSURE)
+from .periodic_ewald import (StokesDLP_ewald_2p, StokesDLP_pressure_ewald_2p,  # noqa: F401
+                             StokesSLP_ewald_2p, StokesSLP_pressure_ewald_2p, ewald_params,
+                             mex_stokes_dlp_kspace, mex_stokes_dlp_pressure_kspace,
+                             mex_stokes_dlp_pressure_real, mex_stokes_dlp_real,
+                             mex_stokes_slp_kspace, mex_stokes_slp_pressure_kspace,
+                             mex_stokes_slp_pressure_real, mex_stokes_slp_real)
 
 __all__ = [
     "Context", "Se2pError", "default_context", "device_count", "LIB_PATH", "SE2P_SLP",
-    "SE2P_DLP", "SE2P_SLP_PRESSURE", "StokesSLP_ewald_2p", "StokesDLP_ewald_2p",
-    "StokesSLP_pressure_ewald_2p", "ewald_params", "mex_stokes_slp_real",
-    "mex_stokes_slp_kspace", "mex_stokes_dlp_real", "mex_stokes_dlp_kspace",
-    "mex_stokes_slp_pressure_real", "mex_stokes_slp_pressure_kspace",
+    "SE2P_DLP", "SE2P_SLP_PRESSURE", "SE2P_DLP_PRESSURE", "StokesSLP_ewald_2p",
+    "StokesDLP_ewald_2p", "StokesSLP_pressure_ewald_2p", "StokesDLP_pressure_ewald_2p",
+    "ewald_params", "mex_stokes_slp_real", "mex_stokes_slp_kspace", "mex_stokes_dlp_real",
+    "mex_stokes_dlp_kspace", "mex_stokes_slp_pressure_real", "mex_stokes_slp_pressure_kspace",
+    "mex_stokes_dlp_pressure_real", "mex_stokes_dlp_pressure_kspace",
 ]

@@ -21,12 +21,14 @@ SE2P_ERR_NODEV = -5
 SE2P_SLP = 0
 SE2P_DLP = 1
 SE2P_SLP_PRESSURE = 2
+SE2P_DLP_PRESSURE = 3
 
-_KINDS = {"slp": SE2P_SLP, "dlp": SE2P_DLP, "slp_pressure": SE2P_SLP_PRESSURE}
+_KINDS = {"slp": SE2P_SLP, "dlp": SE2P_DLP, "slp_pressure": SE2P_SLP_PRESSURE,
+          "dlp_pressure": SE2P_DLP_PRESSURE}
 
 
 def kind_code(kind):
-    """'slp' / 'dlp' / 'slp_pressure' (or the integer code) -> se2p kind."""
+    """'slp' / 'dlp' / 'slp_pressure' / 'dlp_pressure' (or the integer code) -> se2p kind."""
     if isinstance(kind, str):
         return _KINDS[kind]
     if kind not in _KINDS.values():
@@ -36,7 +38,12 @@ def kind_code(kind):
 
 def out_comps(kind):
     """Values per target: 1 for the pressure, 2 for velocities."""
-    return 1 if kind_code(kind) == SE2P_SLP_PRESSURE else 2
+    return 1 if kind_code(kind) >= SE2P_SLP_PRESSURE else 2
+
+
+def has_normals(kind):
+    """Stresslet kinds carry source normals."""
+    return kind_code(kind) in (SE2P_DLP, SE2P_DLP_PRESSURE)
 
 EXPORTS = (
     "se2p_ctx_create", "se2p_ctx_destroy", "se2p_last_error", "se2p_version",
@@ -49,6 +56,7 @@ EXPORTS = (
     "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy", "se2p_batch_create",
     "se2p_batch_sizes", "se2p_batch_apply", "se2p_batch_apply_graph", "se2p_batch_destroy",
     "se2p_slp_pressure_real", "se2p_slp_pressure_kspace", "se2p_slp_pressure_ewald",
+    "se2p_dlp_pressure_real", "se2p_dlp_pressure_kspace", "se2p_dlp_pressure_ewald",
 )
 
 
@@ -142,6 +150,10 @@ def lib():
             "se2p_slp_pressure_kspace": [P, P, P, P, D, D, I, I, I, I, D, D, D, I, P],
             "se2p_slp_pressure_ewald": [P, P, P, P, P, P, P, I, I, D, D, ctypes.POINTER(Options),
                                         P, P, P, P],
+            "se2p_dlp_pressure_real": [P, P, P, P, P, I, I, D, I, I, D, D, P],
+            "se2p_dlp_pressure_kspace": [P, P, P, P, P, D, D, I, I, I, I, D, D, D, I, P],
+            "se2p_dlp_pressure_ewald": [P, P, P, P, P, P, P, P, P, I, I, D, D,
+                                        ctypes.POINTER(Options), P, P, P, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

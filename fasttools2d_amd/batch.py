@@ -14,8 +14,9 @@ from .plan import DeviceArray, params_struct
 
 
 class Batch:
-    """systems: list of dicts with keys kind ("slp"/"dlp"), psrc, ptar (N x 2 host
-    arrays), nrm (DLP), Lx, Ly, prm (parameter dict or se2p_params)."""
+    """systems: list of dicts with keys kind ("slp", "dlp", "slp_pressure",
+    "dlp_pressure"), psrc, ptar (N x 2 host arrays), nrm (stresslet kinds), Lx, Ly,
+    prm (parameter dict or se2p_params)."""
 
     def __init__(self, ctx, systems):
         self.ctx = ctx
@@ -29,7 +30,7 @@ class Batch:
                 kind = _lib.kind_code(y.get("kind", "slp"))
                 dps = DeviceArray.from_host(ctx, y["psrc"])
                 dpt = DeviceArray.from_host(ctx, y["ptar"])
-                dn = DeviceArray.from_host(ctx, y["nrm"]) if kind == _lib.SE2P_DLP else None
+                dn = DeviceArray.from_host(ctx, y["nrm"]) if _lib.has_normals(kind) else None
                 tmp += [a for a in (dps, dpt, dn) if a is not None]
                 prm = y["prm"]
                 arr[i] = System(kind, self.ns[i], self.nt[i], float(y["Lx"]), float(y["Ly"]),

@@ -139,7 +139,8 @@ namespace {
 // ---------------------------------------------------------------------------
 bool valid_kind(int kind)
 {
-    return kind == SE2P_SLP || kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE;
+    return kind == SE2P_SLP || kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE ||
+           kind == SE2P_DLP_PRESSURE;
 }
 
 int check_common(se2p_ctx* ctx, int Ns, int Nt, double Lx, double Ly)
@@ -453,10 +454,12 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     return SE2P_OK;
 }
 
-// StokesSLP_pressure_ewald_2p.m:1-138: wrap, the SLP recipe, pr + pk with the
-// zero mode -sum(f1 x + f2 y)/(2 Lx Ly) added to pk (:127-128).
-int pressure_driver(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
-                    const double* ytar, const double* f1, const double* f2, int Ns, int Nt,
+// StokesSLP_pressure_ewald_2p.m:1-138 / StokesDLP_pressure_ewald_2p.m: wrap, the
+// SLP or DLP recipe, pr + pk with the zero mode added to pk (SLP: -sum(f1 x +
+// f2 y)/(2 Lx Ly), :127-128; DLP: -sum(n1 f1 + n2 f2)/(2 Lx Ly), :156-157).
+int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc,
+                    const double* xtar, const double* ytar, const double* n1,
+                    const double* n2, const double* f1, const double* f2, int Ns, int Nt,
                     double Lx, double Ly, const se2p_options* opt, double* p_out,
                     double* pr_out, double* pk_out, double* xi_out)
 {
@@ -468,7 +471,18 @@ int pressure_driver(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const
     SE2P_TRY(need_ptr(xtar, Nt, "xtar"));
     SE2P_TRY(need_ptr(ytar, Nt, "ytar"));
     SE2P_TRY(need_ptr(p_out, Nt, "p"));
-    std::vector<double> ps(2 * (size_t)Ns), pt(2 * (size_t)Nt), f(2 * (size_t)Ns);
+    if (kind == SE2P_DLP_PRESSURE) {
+        SE2P_TRY(need_ptr(n1, Ns, "n1"));
+        SE2P_TRY(need_ptr(n2, Ns, "n2"));
+    }
+    std::vector<double> ps(2 * (size_t)Ns), pt(2 * (size_t)Nt), f(2 * (size_t)Ns), nn;
+    if (kind == SE2P_DLP_PRESSURE) {
+        nn.resize(2 * (size_t)Ns);
+        for (int i = 0; i < Ns; ++i) {
+            nn[2 * i] = n1[i];
+            nn[2 * i + 1] = n2[i];
+        }
+    }
     double q1 = 0, q2 = 0;
     for (int i = 0; i < Ns; ++i) {
         ps[2 * i] = wrap1(xsrc[i], Lx);
@@ -483,21 +497,22 @@ int pressure_driver(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const
         pt[2 * i + 1] = wrap1(ytar[i], Ly);
     }
     se2p_params prm;
-    SE2P_TRY(params_ref(SE2P_SLP, Ns, Nt, (q1 + q2) + 1, Lx, Ly, opt, &prm));
+    SE2P_TRY(params_ref(kind == SE2P_DLP_PRESSURE ? SE2P_DLP : SE2P_SLP, Ns, Nt, (q1 + q2) + 1,
+                        Lx, Ly, opt, &prm));
     SE2P_TRY(check_params(&prm, Lx, Ly));
     if (xi_out) *xi_out = prm.xi;
     if (Nt == 0) return SE2P_OK;
-    double *dps, *dpt, *df;
+    double *dps, *dpt, *df, *dn = nullptr;
     SE2P_TRY(upload(ctx, WS_PTS_A, ps.data(), ps.size(), &dps));
     SE2P_TRY(upload(ctx, WS_PTS_B, pt.data(), pt.size(), &dpt));
     SE2P_TRY(upload(ctx, WS_DENS_A, f.data(), f.size(), &df));
+    if (kind == SE2P_DLP_PRESSURE) SE2P_TRY(upload(ctx, WS_DENS_B, nn.data(), nn.size(), &dn));
     void* out = nullptr;
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &out));
     double* dpr = (double*)out;
     double* dpk = dpr + Nt;
     Geom G;
-    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP_PRESSURE, dps, dpt, nullptr, Ns, Nt, Lx, Ly,
-                        prm, true, true));
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, kind, dps, dpt, dn, Ns, Nt, Lx, Ly, prm, true, true));
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
@@ -525,7 +540,7 @@ int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const doubl
     SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Ns, &dps));
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Nt, &dpt));
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Ns, &df));
-    if (kind == SE2P_DLP) SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Ns, &dn));
+    if (dlp_family(kind)) SE2P_TRY(upload(ctx, WS_DENS_B, n, 2 * (size_t)Ns, &dn));
     const size_t nout = (size_t)out_comps(kind) * Nt;
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * nout, &du));
     Geom G;
@@ -728,8 +743,50 @@ int se2p_slp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysr
                             double* xi)
 {
     if (ctx) SE2P_HIP_CHECK(hipSetDevice(ctx->device));
-    return pressure_driver(ctx, xsrc, ysrc, xtar, ytar, f1, f2, Nsrc, Ntar, Lx, Ly, opt, p, pr,
-                           pk, xi);
+    return pressure_driver(ctx, SE2P_SLP_PRESSURE, xsrc, ysrc, xtar, ytar, nullptr, nullptr, f1,
+                           f2, Nsrc, Ntar, Lx, Ly, opt, p, pr, pk, xi);
+}
+
+int se2p_dlp_pressure_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, const double* n, int Nsrc, int Ntar, double xi,
+                           int nside_x, int nside_y, double Lx, double Ly, double* pr)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, xi));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(pr, Ntar, "pr"));
+    return mex_call(ctx, SE2P_DLP_PRESSURE, true, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    real_only(xi, nside_x, nside_y), pr);
+}
+
+int se2p_dlp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                             const double* f, const double* n, double xi, double eta, int Nsrc,
+                             int Ntar, int Mx, int My, double Lx, double Ly, double w, int P,
+                             double* pk)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(pk, Ntar, "pk"));
+    return mex_call(ctx, SE2P_DLP_PRESSURE, false, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    k_only(xi, eta, Mx, My, w, P), pk);
+}
+
+int se2p_dlp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                            const double* xtar, const double* ytar, const double* n1,
+                            const double* n2, const double* f1, const double* f2, int Nsrc,
+                            int Ntar, double Lx, double Ly, const se2p_options* opt, double* p,
+                            double* pr, double* pk, double* xi)
+{
+    if (ctx) SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return pressure_driver(ctx, SE2P_DLP_PRESSURE, xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Nsrc,
+                           Ntar, Lx, Ly, opt, p, pr, pk, xi);
 }
 
 void se2p_options_default(se2p_options* opt)
@@ -749,7 +806,7 @@ int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly
     if (!(Lx > 0) || !(Ly > 0)) return set_error(SE2P_ERR_ARG, "box lengths must be positive");
     // the pressure driver uses the velocity driver's recipe
     // (StokesSLP_pressure_ewald_2p.m:83-104 = StokesSLP_ewald_2p.m:83-104)
-    return params_ref(kind == SE2P_DLP ? SE2P_DLP : SE2P_SLP, Nsrc, Ntar, Q, Lx, Ly, opt, out);
+    return params_ref(dlp_family(kind) ? SE2P_DLP : SE2P_SLP, Nsrc, Ntar, Q, Lx, Ly, opt, out);
 }
 
 int se2p_slp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,
@@ -834,7 +891,7 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
     if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
     SE2P_TRY(check_params(prm, Lx, Ly));
-    if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
+    if (dlp_family(kind)) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
     if (Ntar == 0) return SE2P_OK;
     Geom G;
@@ -857,7 +914,7 @@ int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     SE2P_TRY(check_params(prm, Lx, Ly));
     SE2P_TRY(need_ptr(d_psrc, Nsrc, "d_psrc"));
     SE2P_TRY(need_ptr(d_ptar, Ntar, "d_ptar"));
-    if (kind == SE2P_DLP) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
+    if (dlp_family(kind)) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
     se2p_plan* pl = new (std::nothrow) se2p_plan();
     if (!pl) return set_error(SE2P_ERR_NOMEM, "plan allocation failed");
@@ -922,7 +979,7 @@ int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_
         if (rc == SE2P_OK) rc = check_params(&y.prm, y.Lx, y.Ly);
         if (rc == SE2P_OK) rc = need_ptr(y.d_psrc, y.Nsrc, "d_psrc");
         if (rc == SE2P_OK) rc = need_ptr(y.d_ptar, y.Ntar, "d_ptar");
-        if (rc == SE2P_OK && y.kind == SE2P_DLP) rc = need_ptr(y.d_n, y.Nsrc, "d_n");
+        if (rc == SE2P_OK && dlp_family(y.kind)) rc = need_ptr(y.d_n, y.Nsrc, "d_n");
         if (rc != SE2P_OK)
             return set_error(rc, "system %d: %s", i, std::string(se2p_last_error()).c_str());
     }

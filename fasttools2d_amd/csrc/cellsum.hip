@@ -9,7 +9,8 @@
 // written in input order.  The Stokeslet pressure
 // (mex_stokes_slp_pressure_real.cpp:60-160) uses the same pair set as the DLP
 // (coincident points of the own cell skipped) with the pair term
-// (r.f) e^{-xi^2 r^2}/r^2 and one output scaled by 1/(2 pi).  Sources further than one cell away are never within
+// (r.f) e^{-xi^2 r^2}/r^2 and one output scaled by 1/(2 pi); the stresslet
+// pressure (mex_stokes_dlp_pressure_real.cpp:100-192) likewise with its own pair term.  Sources further than one cell away are never within
 // the cutoff, so "own cell, or r^2 < cutoff^2" over any window that contains
 // the 3 x 3 stencil selects exactly the reference's pairs.
 //
@@ -56,6 +57,11 @@ struct SrcOf<SE2P_SLP_PRESSURE> {
     using T = double4;
     static constexpr int cap = 1024;
 };
+template <>
+struct SrcOf<SE2P_DLP_PRESSURE> {
+    using T = DlpRec;
+    static constexpr int cap = 512;
+};
 
 constexpr int kBlkThreads = 256;
 constexpr int kMaxBlockSide = 4;
@@ -95,6 +101,17 @@ __device__ __forceinline__ void dlp_pair(double r1, double r2, double rr, double
     const double c = 2.0 * xi2;
     a1 += e * (b * r1 + c * (s.f1 * rn + s.n1 * rf + s.fn * r1));
     a2 += e * (b * r2 + c * (s.f2 * rn + s.n2 * rf + s.fn * r2));
+}
+
+// 2 pi times the stresslet pressure pair term for r = target - source
+// (mex_stokes_dlp_pressure_real.cpp:113-116):
+//   -e^{-xi^2 r^2} ((f.n - 2 xi^2 (r.n)(r.f))/r^2 - 2 (r.n)(r.f)/r^4)
+__device__ __forceinline__ double dlp_pressure_pair(double r1, double r2, double rr, double xi2,
+                                                    const DlpRec& s)
+{
+    const double ir = rcp64(rr);
+    const double rf = r1 * s.f1 + r2 * s.f2, rn = r1 * s.n1 + r2 * s.n2;
+    return -exp(-xi2 * rr) * ((s.fn - 2 * xi2 * rn * rf) * ir - 2 * rn * rf * ir * ir);
 }
 
 __device__ __forceinline__ int wrapi(int q, int n) { return ((q % n) + n) % n; }
@@ -308,9 +325,11 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         slp_pair<TIER, false>(dx, dy, r2, xi2, ixi2, q.z, q.w, a1, a2);
                     } else if (KIND == SE2P_DLP) {
                         dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
-                    } else {
+                    } else if (KIND == SE2P_SLP_PRESSURE) {
                         const double4& q = reinterpret_cast<const double4&>(s);
                         a1 += (dx * q.z + dy * q.w) * exp(-xi2 * r2) * rcp64(r2);
+                    } else {
+                        a1 += dlp_pressure_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s));
                     }
                 }
                 if (KIND == SE2P_SLP) {
@@ -333,7 +352,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         }
         if (act) {
             const int o = tperm[ti];
-            if (KIND == SE2P_SLP_PRESSURE) {
+            if (KIND == SE2P_SLP_PRESSURE || KIND == SE2P_DLP_PRESSURE) {
                 u[o] = a1 / (2 * kPi);
             } else {
                 u[2 * o] = a1 / (4 * kPi);
@@ -438,8 +457,10 @@ int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys,
         SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
     else if (kind == SE2P_DLP)
         SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full);
-    else
+    else if (kind == SE2P_SLP_PRESSURE)
         SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE, kE1Full);
+    else
+        SE2P_LAUNCH_REAL(SE2P_DLP_PRESSURE, kE1Full);
 #undef SE2P_LAUNCH_REAL
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
 {
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
-    const int mode = d.kind == SE2P_DLP ? 1 + c : 0;
+    const int mode = dlp_family(d.kind) ? 1 + c : 0;
     const int* __restrict__ tstart = d.sp_start;
     const int* __restrict__ perm = d.sp_perm;
     const int2* __restrict__ node = d.sp_node;
@@ -280,6 +280,22 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
         const cplx dd = cscale(csub(dk, dm), 0.5);
         g1 = {dd.im, -dd.re};
         g2 = {0.0, 0.0};
+    } else if (KIND == SE2P_DLP_PRESSURE) {
+        // P(k) = -(k.Q.k) e^{-(1-eta) K^2/(4 xi^2)}/K^2 with Q_ij the spread f_i n_j
+        // (mex_stokes_dlp_pressure_kspace.cpp:150-190); a real operator, so its
+        // Hermitian part is the mean of the operators at k and k'.
+        cplx q11, q21, q12, q22;
+        unpack(Z0[gid], Z0[gm], q11, q21);  // spread (f1 n1, f2 n1)
+        unpack(Z1[gid], Z1[gm], q12, q22);  // spread (f1 n2, f2 n2)
+        const cplx qs = cadd(q12, q21);
+        const double K2 = k1 * k1 + k2 * k2, M2 = m1 * m1 + m2 * m2;
+        const double gk = -exp(-(1 - eta) * a * K2) / K2, gm_ = -exp(-(1 - eta) * a * M2) / M2;
+        const cplx pk = cscale(cadd(cadd(cscale(q11, k1 * k1), cscale(qs, k1 * k2)),
+                                    cscale(q22, k2 * k2)), gk);
+        const cplx pm = cscale(cadd(cadd(cscale(q11, m1 * m1), cscale(qs, m1 * m2)),
+                                    cscale(q22, m2 * m2)), gm_);
+        g1 = cscale(cadd(pk, pm), 0.5);
+        g2 = {0.0, 0.0};
     } else {
         cplx q11, q21, q12, q22;
         unpack(Z0[gid], Z0[gm], q11, q21);  // spread (f1 n1, f2 n1)
@@ -385,13 +401,14 @@ __global__ __launch_bounds__(256) void k_zero_mode_part(const SysDesc* __restric
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const double* __restrict__ rec = d.s_rec;
-    const bool dlp = d.kind == SE2P_DLP;
     double ax = 0, ay = 0;
     for (int i = jb.y * 256 + threadIdx.x; i < d.Ns; i += d.zm_nblk * 256) {
-        if (dlp) {  // {x, y, f1, f2, n1, n2, f.n, 0}: (f.n) x
+        if (d.kind == SE2P_DLP) {  // {x, y, f1, f2, n1, n2, f.n, 0}: (f.n) x
             const double* r = rec + 8 * (size_t)i;
             ax += r[6] * r[0];
             ay += r[6] * r[1];
+        } else if (d.kind == SE2P_DLP_PRESSURE) {  // f.n
+            ax += rec[8 * (size_t)i + 6];
         } else {  // {x, y, f1, f2}: f.x
             const double* r = rec + 4 * (size_t)i;
             ax += r[2] * r[0] + r[3] * r[1];
@@ -426,7 +443,7 @@ __global__ void k_zero_mode_final(const SysDesc* __restrict__ sys, const Job* __
     if (d.kind == SE2P_DLP) {  // StokesDLP_ewald_2p.m:149-151
         d.zm[0] = sx * inv_area;
         d.zm[1] = sy * inv_area;
-    } else {  // StokesSLP_pressure_ewald_2p.m:127-128
+    } else {  // StokesSLP_pressure_ewald_2p.m:127-128, StokesDLP_pressure_ewald_2p.m:156-157
         d.zm[0] = -sx * inv_area / 2;
         d.zm[1] = 0;
     }
@@ -520,8 +537,10 @@ int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, 
         hipLaunchKernelGGL(k_filter<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
     else if (kind == SE2P_DLP)
         hipLaunchKernelGGL(k_filter<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
-    else
+    else if (kind == SE2P_SLP_PRESSURE)
         hipLaunchKernelGGL(k_filter<SE2P_SLP_PRESSURE>, dim3(njobs), dim3(256), 0, s, sys, jobs);
+    else
+        hipLaunchKernelGGL(k_filter<SE2P_DLP_PRESSURE>, dim3(njobs), dim3(256), 0, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -47,7 +47,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
     G->has_k = want_k;
     G->e1tier = prm.e1_tier == 1 ? kE1Fast : kE1Full;
     void* p;
-    if (kind == SE2P_DLP && Ns > 0) {
+    if (dlp_family(kind) && Ns > 0) {
         // normals are geometry: keep a copy in input order for the spreading passes
         SE2P_TRY(ws_get(ctx, ws, WS_NRM, sizeof(double) * 2 * (size_t)Ns, &p));
         G->nrm = (double*)p;
@@ -136,13 +136,13 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->partial = (double*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_A, sizeof(double2) * Gn, &p));
         G->Z0 = (double2*)p;
-        if (kind == SE2P_DLP) {
+        if (dlp_family(kind)) {
             SE2P_TRY(ws_get(ctx, ws, WS_GRID_B, sizeof(double2) * Gn, &p));
             G->Z1 = (double2*)p;
         }
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_C, sizeof(double2) * Gn, &p));
         G->W = (double2*)p;
-        if (kind == SE2P_DLP) {
+        if (dlp_family(kind)) {
             SE2P_TRY(ws_get(ctx, ws, WS_GRID_D, sizeof(double2) * Gn, &p));
             G->D = (double2*)p;
         }
@@ -255,7 +255,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         uoff += (long)out_comps(G.kind) * G.Nt;
         B->has_real |= G.has_real;
         B->has_k |= G.has_k;
-        B->any_dlp |= G.kind == SE2P_DLP;
+        B->any_dlp |= dlp_family(G.kind);
         if (G.has_k) {
             size_t& ly = B->lds_y[fft_line_class(G.kg.My)];
             size_t& lx = B->lds_x[fft_line_class(G.kg.Mx)];
@@ -297,7 +297,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     });
     for (int c = 0; c < 2; ++c) {
         auto want = [&](int i, const SysDesc& d) {
-            return geoms[i]->has_k && (c == 0 || d.kind == SE2P_DLP);
+            return geoms[i]->has_k && (c == 0 || dlp_family(d.kind));
         };
         for (int big = 0; big < 2; ++big)
             add(B->spread[c][big], [&](int i, const SysDesc& d) {

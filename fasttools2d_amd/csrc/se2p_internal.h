@@ -14,12 +14,17 @@ namespace se2p {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Kinds: SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE.
-constexpr int kNumKinds = 3;
+// Kinds: SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE, SE2P_DLP_PRESSURE.
+constexpr int kNumKinds = 4;
+// Stresslet kinds: normals, two spreading passes (f n1, f n2).
+__host__ __device__ inline bool dlp_family(int kind)
+{
+    return kind == SE2P_DLP || kind == SE2P_DLP_PRESSURE;
+}
 // Values per target: 2 (velocity) or 1 (pressure).
-inline int out_comps(int kind) { return kind == SE2P_SLP_PRESSURE ? 1 : 2; }
-// Sorted source records: DLP {x, y, f1, f2, n1, n2, f.n, 0}, others {x, y, f1, f2}.
-inline int rec_layout(int kind) { return kind == SE2P_DLP ? 1 : 0; }
+inline int out_comps(int kind) { return kind >= SE2P_SLP_PRESSURE ? 1 : 2; }
+// Sorted source records: stresslet {x, y, f1, f2, n1, n2, f.n, 0}, else {x, y, f1, f2}.
+__host__ __device__ inline int rec_layout(int kind) { return dlp_family(kind) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
 // Status helpers

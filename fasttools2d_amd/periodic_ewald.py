@@ -138,6 +138,58 @@ def StokesSLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, P=24, Nb
     return p, pr, pk, xi.value
 
 
+def mex_stokes_dlp_pressure_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly, ctx=None):
+    """pr = mex_stokes_dlp_pressure_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly)
+    (mex_stokes_dlp_pressure_real.cpp:19-63); 1 x Ntar."""
+    ps, pt = _as_2xn(psrc, "psrc"), _as_2xn(ptar, "ptar")
+    ff, nn = _as_2xn(f, "f"), _as_2xn(n, "n")
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    out = np.zeros(pt.shape[0])
+    check(lib().se2p_dlp_pressure_real(_ctx(ctx), _p(ps), _p(pt), _p(ff), _p(nn), ps.shape[0],
+                                       pt.shape[0], float(xi), int(nside_x), int(nside_y),
+                                       float(Lx), float(Ly), _p(out)))
+    return out[None, :].copy()
+
+
+def mex_stokes_dlp_pressure_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P, ctx=None):
+    """pk = mex_stokes_dlp_pressure_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P)
+    (mex_stokes_dlp_pressure_kspace.cpp:10-55); 1 x Ntar, no zero mode."""
+    ps, pt = _as_2xn(psrc, "psrc"), _as_2xn(ptar, "ptar")
+    ff, nn = _as_2xn(f, "f"), _as_2xn(n, "n")
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    out = np.zeros(pt.shape[0])
+    check(lib().se2p_dlp_pressure_kspace(_ctx(ctx), _p(ps), _p(pt), _p(ff), _p(nn), float(xi),
+                                         float(eta), ps.shape[0], pt.shape[0], int(Mx), int(My),
+                                         float(Lx), float(Ly), float(w), int(P), _p(out)))
+    return out[None, :].copy()
+
+
+def StokesDLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, P=24, Nb=None,
+                                tol=1e-16, verbose=0, ctx=None):
+    """[p, pr, pk, xi] = StokesDLP_pressure_ewald_2p(...) (StokesDLP_pressure_ewald_2p.m:1-160)."""
+    xs, ys, xt, yt = (_vec(v, n) for v, n in ((xsrc, "xsrc"), (ysrc, "ysrc"), (xtar, "xtar"),
+                                                 (ytar, "ytar")))
+    b1, b2 = _vec(n1, "n1"), _vec(n2, "n2")
+    a1, a2 = _vec(f1, "f1"), _vec(f2, "f2")
+    Ns, Nt = xs.size, xt.size
+    if (ys.size != Ns or a1.size != Ns or a2.size != Ns or b1.size != Ns or b2.size != Ns
+            or yt.size != Nt):
+        raise ValueError("inconsistent input lengths")
+    p, pr, pk = np.zeros(Nt), np.zeros(Nt), np.zeros(Nt)
+    xi = ctypes.c_double()
+    o = _options(P, Nb, tol, verbose)
+    check(lib().se2p_dlp_pressure_ewald(_ctx(ctx), _p(xs), _p(ys), _p(xt), _p(yt), _p(b1),
+                                        _p(b2), _p(a1), _p(a2), Ns, Nt, float(Lx), float(Ly),
+                                        ctypes.byref(o), _p(p), _p(pr), _p(pk), ctypes.byref(xi)))
+    return p, pr, pk, xi.value
+
+
 def _options(P=24, Nb=None, tol=1e-16, verbose=0):
     o = Options()
     lib().se2p_options_default(ctypes.byref(o))
@@ -150,7 +202,8 @@ def _options(P=24, Nb=None, tol=1e-16, verbose=0):
 
 def ewald_params(kind, Nsrc, Ntar, Q, Lx, Ly, P=24, Nb=None, tol=1e-16):
     """Reference parameter recipe (StokesSLP_ewald_2p.m:84-104 / StokesDLP_ewald_2p.m:104-125;
-    the pressure uses the SLP recipe, StokesSLP_pressure_ewald_2p.m:83-104)."""
+    each pressure uses its family's recipe, StokesSLP_pressure_ewald_2p.m:83-104 and
+    StokesDLP_pressure_ewald_2p.m:111-134)."""
     o = _options(P, Nb, tol)
     prm = Params()
     k = _lib.kind_code(kind)

@@ -34,6 +34,7 @@ extern "C" {
 #define SE2P_SLP 0 /* Stokeslet, single-layer potential */
 #define SE2P_DLP 1 /* stresslet, double-layer potential */
 #define SE2P_SLP_PRESSURE 2 /* pressure of the Stokeslet (one value per target) */
+#define SE2P_DLP_PRESSURE 3 /* pressure of the stresslet (one value per target) */
 
 /* Arithmetic of a plan (only FP64 in this build). */
 #define SE2P_FP64 0
@@ -126,6 +127,30 @@ int se2p_slp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysr
                             const se2p_options* opt, double* p, double* pr, double* pk,
                             double* xi);
 
+/* ------------------------------------------------------------------------ */
+/* Derived quantity: pressure of the stresslet.                              */
+/* ------------------------------------------------------------------------ */
+/* Replaces mex_stokes_dlp_pressure_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_pressure_real.cpp:17-195).
+ * pr: 1 x Ntar. */
+int se2p_dlp_pressure_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, const double* n, int Nsrc, int Ntar, double xi,
+                           int nside_x, int nside_y, double Lx, double Ly, double* pr);
+/* Replaces mex_stokes_dlp_pressure_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_dlp_pressure_kspace.cpp:10-230).  pk: 1 x Ntar, without the zero mode. */
+int se2p_dlp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                             const double* f, const double* n, double xi, double eta, int Nsrc,
+                             int Ntar, int Mx, int My, double Lx, double Ly, double w, int P,
+                             double* pk);
+/* StokesDLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, ...)
+ * (StokesDLP_pressure_ewald_2p.m): wrapping, the DLP parameter recipe, p = pr + pk
+ * with the zero mode -sum(n1 f1 + n2 f2)/(2 Lx Ly) in pk. */
+int se2p_dlp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
+                            const double* xtar, const double* ytar, const double* n1,
+                            const double* n2, const double* f1, const double* f2, int Nsrc,
+                            int Ntar, double Lx, double Ly, const se2p_options* opt, double* p,
+                            double* pr, double* pk, double* xi);
+
 void se2p_options_default(se2p_options* opt);
 
 /* Parameter recipe of StokesSLP_ewald_2p.m:27-37,84-104,147-202 (kind SE2P_SLP)
@@ -200,12 +225,12 @@ int se2p_plan_destroy(se2p_plan* plan);
 /* reference (one system per mex call, StokesSLP_ewald_2p.m:119,126).        */
 /* ------------------------------------------------------------------------ */
 typedef struct se2p_system {
-    int kind;             /* SE2P_SLP, SE2P_DLP or SE2P_SLP_PRESSURE */
+    int kind;             /* SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE or SE2P_DLP_PRESSURE */
     int Nsrc, Ntar;
     double Lx, Ly;
     const double* d_psrc; /* 2 x Nsrc device array, read during creation only */
     const double* d_ptar; /* 2 x Ntar */
-    const double* d_n;    /* 2 x Nsrc normals (SE2P_DLP), else NULL */
+    const double* d_n;    /* 2 x Nsrc normals (SE2P_DLP, SE2P_DLP_PRESSURE), else NULL */
     se2p_params prm;      /* e.g. from se2p_params_ref at the system's tolerance */
 } se2p_system;
 

@@ -60,6 +60,12 @@ def lib():
             "orc_slp_pressure_kspace": [_P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I, _P],
             "orc_slp_pressure_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
             "orc_slp_pressure_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+            "orc_dlp_pressure_real": [_P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+            "orc_dlp_pressure_filter": [_P] * 8 + [_I, _I, _D, _D, _D, _D],
+            "orc_dlp_pressure_kspace": [_P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
+                                        _P],
+            "orc_dlp_pressure_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
+            "orc_dlp_pressure_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -70,6 +76,8 @@ def lib():
         L.orc_e1.restype = _D
         L.orc_slp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
         L.orc_slp_pressure_zero_mode.restype = _D
+        L.orc_dlp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
+        L.orc_dlp_pressure_zero_mode.restype = _D
         _lib = L
     return _lib
 
@@ -262,6 +270,53 @@ def slp_pressure_kspace_ds(psrc, ptar, f, Lx, Ly, xi, kinf):
     psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
     p = np.zeros(ptar.shape[0])
     lib().orc_slp_pressure_kspace_ds(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0],
+                                     ptar.shape[0], Lx, Ly, xi, kinf, _ptr(p))
+    return p
+
+
+def dlp_pressure_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly):
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _c(n)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_dlp_pressure_real(_ptr(psrc), _ptr(ptar), _ptr(f), _ptr(n), psrc.shape[0],
+                                ptar.shape[0], xi, nside_x, nside_y, Lx, Ly, _ptr(p))
+    return p
+
+
+def dlp_pressure_filter(H1, H2, H3, H4, Lx, Ly, xi, eta):
+    """Pressure filter of the four spread spectra (f1 n1, f2 n1, f1 n2, f2 n2),
+    Mx x My arrays [ix, iy]; returns the filtered spectrum."""
+    arrs = [_c(x) for h in (H1, H2, H3, H4) for x in (h.real, h.imag)]
+    Mx, My = arrs[0].shape
+    lib().orc_dlp_pressure_filter(*[_ptr(a) for a in arrs], Mx, My, Lx, Ly, xi, eta)
+    return arrs[0] + 1j * arrs[1]
+
+
+def dlp_pressure_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P):
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _c(n)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_dlp_pressure_kspace(_ptr(psrc), _ptr(ptar), _ptr(f), _ptr(n), xi, eta,
+                                  psrc.shape[0], ptar.shape[0], Mx, My, Lx, Ly, w, P, _ptr(p))
+    return p
+
+
+def dlp_pressure_zero_mode(f, n, Lx, Ly):
+    f, n = _c(f), _c(n)
+    return lib().orc_dlp_pressure_zero_mode(_ptr(f), _ptr(n), f.shape[0], Lx, Ly)
+
+
+def dlp_pressure_real_ds(psrc, ptar, f, n, Lx, Ly, xi):
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _c(n)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_dlp_pressure_real_ds(_ptr(psrc), _ptr(ptar), _ptr(f), _ptr(n), psrc.shape[0],
+                                   ptar.shape[0], Lx, Ly, xi, _ptr(p))
+    return p
+
+
+def dlp_pressure_kspace_ds(psrc, ptar, f, n, Lx, Ly, xi, kinf):
+    """Direct k-space sum including the zero mode (stokes_dlp_pressure_kspace_ds.m)."""
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _c(n)
+    p = np.zeros(ptar.shape[0])
+    lib().orc_dlp_pressure_kspace_ds(_ptr(psrc), _ptr(ptar), _ptr(f), _ptr(n), psrc.shape[0],
                                      ptar.shape[0], Lx, Ly, xi, kinf, _ptr(p))
     return p
 

@@ -1078,3 +1078,198 @@ void orc_slp_pressure_kspace_ds(const double* psrc, const double* ptar, const do
     }
     free(S);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Derived quantity: DLP (stresslet) pressure                                */
+/* ------------------------------------------------------------------------ */
+static inline double dlp_pressure_pair(double r1, double r2, double rr, double xi2, double f1,
+                                       double f2, double n1, double n2)
+{
+    double rf = r1 * f1 + r2 * f2, rn = r1 * n1 + r2 * n2, fn = f1 * n1 + f2 * n2;
+    return -exp(-xi2 * rr) * ((fn - 2 * xi2 * rn * rf) / rr - 2 * rn * rf / rr / rr);
+}
+
+/* mex_stokes_dlp_pressure_real.cpp:100-192: the DLP's cells and pair set
+ * (coincident points of the own cell skipped), pair term
+ * -e^{-xi^2 r^2} ((f.n - 2 xi^2 (r.n)(r.f))/r^2 - 2 (r.n)(r.f)/r^4), / (2 pi). */
+void orc_dlp_pressure_real(const double* psrc, const double* ptar, const double* f,
+                           const double* n, int Ns, int Nt, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, double* p)
+{
+    cells_t c;
+    cells_build(&c, psrc, Ns, ptar, Nt, Lx, Ly, nside_x, nside_y);
+    const int nbox = nside_x * nside_y;
+    const double xi2 = xi * xi;
+    const double cut2 = Lx * Ly / nside_x / nside_y;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int bx = b % nside_x, by = b / nside_x;
+        for (int it = c.bo_t[b]; it < c.bo_t[b + 1]; ++it) {
+            int t = c.po_t[it];
+            double xt = ptar[2 * t], yt = ptar[2 * t + 1];
+            double a = 0;
+            for (int is = c.bo_s[b]; is < c.bo_s[b + 1]; ++is) {
+                int s = c.po_s[is];
+                double r1 = xt - psrc[2 * s], r2 = yt - psrc[2 * s + 1];
+                double rr = r1 * r1 + r2 * r2;
+                if (rr == 0) continue;
+                a += dlp_pressure_pair(r1, r2, rr, xi2, f[2 * s], f[2 * s + 1], n[2 * s],
+                                       n[2 * s + 1]);
+            }
+            for (int j = 0; j < 8; ++j) {
+                int qx = bx + nb_dx[j], qy = by + nb_dy[j];
+                int wx = (qx + nside_x) % nside_x, wy = (qy + nside_y) % nside_y;
+                int sb = wy * nside_x + wx;
+                double ox = (Lx * (qx - wx)) / nside_x;
+                double oy = (Ly * (qy - wy)) / nside_y;
+                for (int is = c.bo_s[sb]; is < c.bo_s[sb + 1]; ++is) {
+                    int s = c.po_s[is];
+                    double r1 = xt - (psrc[2 * s] + ox), r2 = yt - (psrc[2 * s + 1] + oy);
+                    double rr = r1 * r1 + r2 * r2;
+                    if (rr < cut2)
+                        a += dlp_pressure_pair(r1, r2, rr, xi2, f[2 * s], f[2 * s + 1],
+                                               n[2 * s], n[2 * s + 1]);
+                }
+            }
+            p[t] = a / (2 * ORC_PI);
+        }
+    }
+    cells_free(&c);
+}
+
+/* mex_stokes_dlp_pressure_kspace.cpp:140-200: with h1 = f1 n1, h2 = f2 n1,
+ * h3 = f1 n2, h4 = f2 n2 (:86-89), P(k) = -(k1^2 h1 + k1 k2 (h2 + h3) + k2^2 h4)
+ * exp(-(1-eta) K^2/(4 xi^2)) / K^2 into h1; k = 0 set to zero. */
+void orc_dlp_pressure_filter(double* h1re, double* h1im, const double* h2re,
+                             const double* h2im, const double* h3re, const double* h3im,
+                             const double* h4re, const double* h4im, int Mx, int My, double Lx,
+                             double Ly, double xi, double eta)
+{
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < Mx; ++ix) {
+        double k1 = wavenumber(ix, Mx, Lx);
+        for (int iy = 0; iy < My; ++iy) {
+            size_t o = (size_t)ix * My + iy;
+            if (ix == 0 && iy == 0) {
+                h1re[o] = h1im[o] = 0;
+                continue;
+            }
+            double k2 = wavenumber(iy, My, Ly);
+            double K2 = k1 * k1 + k2 * k2;
+            double e = exp(-(1 - eta) * a * K2);
+            double qr = k1 * k1 * h1re[o] + k1 * k2 * (h2re[o] + h3re[o]) + k2 * k2 * h4re[o];
+            double qi = k1 * k1 * h1im[o] + k1 * k2 * (h2im[o] + h3im[o]) + k2 * k2 * h4im[o];
+            h1re[o] = -qr * e / K2;
+            h1im[o] = -qi * e / K2;
+        }
+    }
+}
+
+/* mex_stokes_dlp_pressure_kspace.cpp:10-230: spread v1 = (f1 n1, f2 n1) and
+ * v2 = (f1 n2, f2 n2), four forward DFTs, filter, inverse DFT, gather.  The
+ * zero mode is not added (StokesDLP_pressure_ewald_2p.m:156-157 adds it). */
+void orc_dlp_pressure_kspace(const double* psrc, const double* ptar, const double* f,
+                             const double* n, double xi, double eta, int Ns, int Nt, int Mx,
+                             int My, double Lx, double Ly, double w, int P, double* pk)
+{
+    size_t G = (size_t)Mx * My;
+    double* b = (double*)calloc(8 * G, sizeof(double));
+    double* v1 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+    double* v2 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+    for (int i = 0; i < Ns; ++i) {
+        v1[2 * i] = f[2 * i] * n[2 * i];
+        v1[2 * i + 1] = f[2 * i + 1] * n[2 * i];
+        v2[2 * i] = f[2 * i] * n[2 * i + 1];
+        v2[2 * i + 1] = f[2 * i + 1] * n[2 * i + 1];
+    }
+    double *h1r = b, *h1i = b + G, *h2r = b + 2 * G, *h2i = b + 3 * G;
+    double *h3r = b + 4 * G, *h3i = b + 5 * G, *h4r = b + 6 * G, *h4i = b + 7 * G;
+    orc_spread(psrc, v1, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h1r, h2r);
+    orc_spread(psrc, v2, Ns, Lx, Ly, xi, w, eta, P, Mx, My, h3r, h4r);
+    orc_fft2(h1r, h1i, My, Mx, 0);
+    orc_fft2(h2r, h2i, My, Mx, 0);
+    orc_fft2(h3r, h3i, My, Mx, 0);
+    orc_fft2(h4r, h4i, My, Mx, 0);
+    orc_dlp_pressure_filter(h1r, h1i, h2r, h2i, h3r, h3i, h4r, h4i, Mx, My, Lx, Ly, xi, eta);
+    orc_fft2(h1r, h1i, My, Mx, 1);
+    for (int m = 0; m < Nt; ++m) pk[m] = 0;
+    orc_gather(h1r, 1, 0, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, pk);
+    free(v1);
+    free(v2);
+    free(b);
+}
+
+/* StokesDLP_pressure_ewald_2p.m:156-157: -sum_s (n.f)_s / (2 Lx Ly). */
+double orc_dlp_pressure_zero_mode(const double* f, const double* n, int Ns, double Lx,
+                                  double Ly)
+{
+    double s = 0;
+    for (int i = 0; i < Ns; ++i) s += n[2 * i] * f[2 * i] + n[2 * i + 1] * f[2 * i + 1];
+    return -s / (2 * Lx * Ly);
+}
+
+/* stokes_dlp_pressure_real_ds.m: all 3 x 3 images, |r| < 1e-13 skipped, / (2 pi). */
+void orc_dlp_pressure_real_ds(const double* psrc, const double* ptar, const double* f,
+                              const double* n, int Ns, int Nt, double Lx, double Ly, double xi,
+                              double* p)
+{
+    const double xi2 = xi * xi;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double a = 0;
+        for (int s = 0; s < Ns; ++s)
+            for (int jx = -1; jx <= 1; ++jx)
+                for (int jy = -1; jy <= 1; ++jy) {
+                    double r1 = ptar[2 * m] - (psrc[2 * s] + jx * Lx);
+                    double r2 = ptar[2 * m + 1] - (psrc[2 * s + 1] + jy * Ly);
+                    double rr = r1 * r1 + r2 * r2;
+                    if (sqrt(rr) < 1e-13) continue;
+                    a += dlp_pressure_pair(r1, r2, rr, xi2, f[2 * s], f[2 * s + 1], n[2 * s],
+                                           n[2 * s + 1]);
+                }
+        p[m] = a / (2 * ORC_PI);
+    }
+}
+
+/* stokes_dlp_pressure_kspace_ds.m: real part of
+ *   sum_{k != 0} -(k.f_s)(k.n_s) e^{-K^2/(4 xi^2)}/K^2 e^{i k.(x_t - x_s)} / (Lx Ly)
+ * plus the zero mode, through S(k) = sum_s (k.f_s)(k.n_s) e^{-i k.x_s}. */
+void orc_dlp_pressure_kspace_ds(const double* psrc, const double* ptar, const double* f,
+                                const double* n, int Ns, int Nt, double Lx, double Ly,
+                                double xi, int kinf, double* p)
+{
+    const int K = 2 * kinf;
+    const size_t nm = (size_t)K * K;
+    double* S = (double*)calloc(nm * 2, sizeof(double));
+    const double a = 0.25 / (xi * xi);
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)nm; ++q) {
+        int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+        double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+        double* s = S + 2 * q;
+        for (int i = 0; i < Ns; ++i) {
+            double ph = -(k1 * psrc[2 * i] + k2 * psrc[2 * i + 1]);
+            double w = (k1 * f[2 * i] + k2 * f[2 * i + 1]) * (k1 * n[2 * i] + k2 * n[2 * i + 1]);
+            s[0] += w * cos(ph);
+            s[1] += w * sin(ph);
+        }
+    }
+    const double zm = orc_dlp_pressure_zero_mode(f, n, Ns, Lx, Ly);
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double acc = 0;
+        for (size_t q = 0; q < nm; ++q) {
+            int j1 = (int)(q / K) - kinf + 1, j2 = (int)(q % K) - kinf + 1;
+            if (j1 == 0 && j2 == 0) continue;
+            double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+            double K2 = k1 * k1 + k2 * k2;
+            double ph = k1 * ptar[2 * m] + k2 * ptar[2 * m + 1];
+            const double* s = S + 2 * q;
+            /* Re(-S e^{i ph}) g */
+            acc -= exp(-a * K2) / K2 * (s[0] * cos(ph) - s[1] * sin(ph));
+        }
+        p[m] = acc / (Lx * Ly) + zm;
+    }
+    free(S);
+}

@@ -113,6 +113,28 @@ void orc_slp_pressure_kspace_ds(const double* psrc, const double* ptar, const do
                                 int Ns, int Nt, double Lx, double Ly, double xi, int kinf,
                                 double* p);
 
+
+/* Derived quantity: DLP pressure (mex_stokes_dlp_pressure_{real,kspace}.cpp,
+ * StokesDLP_pressure_ewald_2p.m, stokes_dlp_pressure_{real,kspace}_ds.m). */
+void orc_dlp_pressure_real(const double* psrc, const double* ptar, const double* f,
+                           const double* n, int Ns, int Nt, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, double* p);
+void orc_dlp_pressure_filter(double* h1re, double* h1im, const double* h2re,
+                             const double* h2im, const double* h3re, const double* h3im,
+                             const double* h4re, const double* h4im, int Mx, int My, double Lx,
+                             double Ly, double xi, double eta);
+void orc_dlp_pressure_kspace(const double* psrc, const double* ptar, const double* f,
+                             const double* n, double xi, double eta, int Ns, int Nt, int Mx,
+                             int My, double Lx, double Ly, double w, int P, double* pk);
+double orc_dlp_pressure_zero_mode(const double* f, const double* n, int Ns, double Lx,
+                                  double Ly);
+void orc_dlp_pressure_real_ds(const double* psrc, const double* ptar, const double* f,
+                              const double* n, int Ns, int Nt, double Lx, double Ly, double xi,
+                              double* p);
+void orc_dlp_pressure_kspace_ds(const double* psrc, const double* ptar, const double* f,
+                                const double* n, int Ns, int Nt, double Lx, double Ly,
+                                double xi, int kinf, double* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,3 +125,107 @@ def test_pressure_filter_closed_form(oracle):
             want = 0 if K2 == 0 else -1j * (k1 * H1[ix, iy] + k2 * H2[ix, iy]) * \
                 np.exp(-(1 - eta) * K2 / (4 * xi * xi)) / K2
             assert abs(out[ix, iy] - want) <= 1e-14 * max(1.0, abs(want))
+
+
+# ------------------------------------------------------------ DLP pressure
+def _brute_dlp_pressure_real(ps, pt, f, n, xi, nsx, nsy, Lx, Ly):
+    """As _brute_pressure_real with the stresslet pressure pair term
+    -e^{-xi^2 r^2} ((f.n - 2 xi^2 (r.n)(r.f))/r^2 - 2 (r.n)(r.f)/r^4) / (2 pi)."""
+    cut2 = Lx * Ly / nsx / nsy
+
+    def cell(p):
+        bx = np.clip(np.floor(nsx * (p[:, 0] / Lx + 0.5)).astype(int), 0, nsx - 1)
+        by = np.clip(np.floor(nsy * (p[:, 1] / Ly + 0.5)).astype(int), 0, nsy - 1)
+        return bx, by
+
+    sx, sy = cell(ps)
+    tx, ty = cell(pt)
+    out = np.zeros(len(pt))
+    for t in range(len(pt)):
+        ddx = (sx - tx[t] + 1) % nsx - 1
+        ddy = (sy - ty[t] + 1) % nsy - 1
+        inside = (np.abs(ddx) <= 1) & (np.abs(ddy) <= 1)
+        shx = (tx[t] + ddx - sx) * Lx / nsx
+        shy = (ty[t] + ddy - sy) * Ly / nsy
+        rx = pt[t, 0] - (ps[:, 0] + shx)
+        ry = pt[t, 1] - (ps[:, 1] + shy)
+        r2 = rx * rx + ry * ry
+        same = (ddx == 0) & (ddy == 0)
+        k = inside & np.where(same, r2 != 0, r2 < cut2)
+        rf = rx[k] * f[k, 0] + ry[k] * f[k, 1]
+        rn = rx[k] * n[k, 0] + ry[k] * n[k, 1]
+        fn = f[k, 0] * n[k, 0] + f[k, 1] * n[k, 1]
+        q = r2[k]
+        out[t] = (-np.exp(-xi * xi * q) * ((fn - 2 * xi * xi * rn * rf) / q - 2 * rn * rf / q ** 2)).sum()
+    return out / (2 * math.pi)
+
+
+def _normals(n, seed):
+    th = np.random.default_rng(seed).random(n) * 2 * np.pi
+    return np.stack([np.cos(th), np.sin(th)], 1)
+
+
+def _dlp_pressure(oracle, ps, pt, f, n, Lx, Ly, tol, Nb=None, P=24):
+    npts = len(ps) + len(pt)
+    Q = (f ** 2).sum() + 1
+    prm = oracle.params("dlp", npts, Q, Lx, Ly, P, Nb if Nb else oracle.default_Nb(npts, P), tol)
+    pr = oracle.dlp_pressure_real(ps, pt, f, n, prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly)
+    pk = oracle.dlp_pressure_kspace(ps, pt, f, n, prm["xi"], prm["eta"], prm["Mx"], prm["My"],
+                                    Lx, Ly, prm["w"], P)
+    return pr, pk + oracle.dlp_pressure_zero_mode(f, n, Lx, Ly), prm
+
+
+@pytest.mark.parametrize("nsx,nsy,Lx,Ly", [(4, 4, 1.0, 1.0), (6, 3, 2.0, 1.0)])
+def test_dlp_pressure_real_matches_numpy_restatement(oracle, nsx, nsy, Lx, Ly):
+    ps = uniform_points(400, Lx, Ly, seed=181)
+    pt = np.concatenate([uniform_points(200, Lx, Ly, seed=182), ps[:30]])
+    f = normal_density(400, seed=183)
+    n = _normals(400, 184)
+    xi = 3.2 * nsx / Lx
+    p = oracle.dlp_pressure_real(ps, pt, f, n, xi, nsx, nsy, Lx, Ly)
+    assert _rel(p, _brute_dlp_pressure_real(ps, pt, f, n, xi, nsx, nsy, Lx, Ly)) < 1e-13
+
+
+def test_dlp_pressure_direct_sums_like_reference(oracle):
+    """direct_sums_pressure_test.m (stresslet part): 20 sources, 20 targets, tol 1e-16."""
+    Lx = Ly = 1.0
+    rng = np.random.default_rng(191)
+    f = 10 * rng.random((20, 2))
+    n1 = rng.random(20)
+    n = np.stack([n1, np.sqrt(1 - n1 ** 2)], 1)
+    ps = oracle.wrap(rng.random((20, 2)), Lx, Ly)
+    pt = oracle.wrap(rng.random((20, 2)), Lx, Ly)
+    pr, pk, prm = _dlp_pressure(oracle, ps, pt, f, n, Lx, Ly, 1e-16)
+    assert np.abs(pr - oracle.dlp_pressure_real_ds(ps, pt, f, n, Lx, Ly, prm["xi"])).max() < 1e-11
+    assert np.abs(pk - oracle.dlp_pressure_kspace_ds(ps, pt, f, n, Lx, Ly, prm["xi"], 70)).max() < 1e-9
+
+
+def test_dlp_pressure_independent_of_ewald_split(oracle):
+    Lx, Ly = 1.0, 2.0
+    ps = uniform_points(300, Lx, Ly, seed=195)
+    pt = uniform_points(150, Lx, Ly, seed=196)
+    f = 10 * np.random.default_rng(197).random((300, 2))
+    n = _normals(300, 198)
+    tot = []
+    for Nb in (3, 9, 27):
+        pr, pk, _ = _dlp_pressure(oracle, ps, pt, f, n, Lx, Ly, 1e-14, Nb=Nb)
+        tot.append(pr + pk)
+    scale = np.abs(tot[0]).max()
+    for p in tot[1:]:
+        assert np.abs(p - tot[0]).max() < 1e-11 * scale
+
+
+def test_dlp_pressure_filter_closed_form(oracle):
+    Mx, My, Lx, Ly, xi, eta = 8, 6, 2.0, 1.5, 3.0, 0.4
+    rng = np.random.default_rng(8)
+    H = [rng.standard_normal((Mx, My)) + 1j * rng.standard_normal((Mx, My)) for _ in range(4)]
+    out = oracle.dlp_pressure_filter(*H, Lx, Ly, xi, eta)
+    for ix in range(Mx):
+        for iy in range(My):
+            k1 = 2 * np.pi / Lx * (ix if ix <= Mx // 2 else ix - Mx)
+            k2 = 2 * np.pi / Ly * (iy if iy <= My // 2 else iy - My)
+            K2 = k1 * k1 + k2 * k2
+            kqk = k1 * k1 * H[0][ix, iy] + k1 * k2 * (H[1][ix, iy] + H[2][ix, iy]) + \
+                k2 * k2 * H[3][ix, iy]
+            want = 0 if K2 == 0 else -kqk * np.exp(-(1 - eta) * K2 / (4 * xi * xi)) / K2
+            assert abs(out[ix, iy] - want) <= 1e-14 * max(1.0, abs(want))
