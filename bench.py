@@ -201,7 +201,10 @@ def profile_only(ctx, dev, ps, p, f, args):
         for _ in range(args.steps):
             dev.run(ps)
         ctx.sync()
-        print(json.dumps({"profile": "plain", "steps": args.steps}))
+        # the same stages timed with HIP events in this process, to set beside
+        # the profiler's kernel durations
+        stages = stage_profile(ctx, lambda: dev.run(ps), args.steps)
+        print(json.dumps({"profile": "plain", "steps": args.steps, "stage_ms": stages}))
         return
     if args.profile == "copy":
         # a plain device copy of known size (the runtime's copy kernel): the
@@ -225,7 +228,9 @@ def profile_only(ctx, dev, ps, p, f, args):
     for _ in range(args.steps):
         plan.apply_device(df.ptr, du.ptr, graph=False)
     ctx.sync()
-    print(json.dumps({"profile": "tuned", "steps": args.steps, "params": prm}))
+    stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps)
+    print(json.dumps({"profile": "tuned", "steps": args.steps, "params": prm,
+                      "stage_ms": stages}))
 
 
 def capped_params(n, Lx, Ly, tol, P=24):
