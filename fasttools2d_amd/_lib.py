@@ -22,13 +22,29 @@ SE2P_SLP = 0
 SE2P_DLP = 1
 SE2P_SLP_PRESSURE = 2
 SE2P_DLP_PRESSURE = 3
+SE2P_SLP_GRADIENT = 4
+SE2P_SLP_PRESSURE_GRAD = 5
+SE2P_SLP_VORTICITY = 6
+SE2P_SLP_STRESS = 7
+SE2P_DLP_GRADIENT = 8
+SE2P_DLP_PRESSURE_GRAD = 9
+SE2P_DLP_VORTICITY = 10
+SE2P_DLP_STRESS = 11
 
 _KINDS = {"slp": SE2P_SLP, "dlp": SE2P_DLP, "slp_pressure": SE2P_SLP_PRESSURE,
-          "dlp_pressure": SE2P_DLP_PRESSURE}
+          "dlp_pressure": SE2P_DLP_PRESSURE, "slp_gradient": SE2P_SLP_GRADIENT,
+          "slp_pressure_grad": SE2P_SLP_PRESSURE_GRAD, "slp_vorticity": SE2P_SLP_VORTICITY,
+          "slp_stress": SE2P_SLP_STRESS, "dlp_gradient": SE2P_DLP_GRADIENT,
+          "dlp_pressure_grad": SE2P_DLP_PRESSURE_GRAD, "dlp_vorticity": SE2P_DLP_VORTICITY,
+          "dlp_stress": SE2P_DLP_STRESS}
+_NCOMP = {SE2P_SLP: 2, SE2P_DLP: 2, SE2P_SLP_PRESSURE: 1, SE2P_DLP_PRESSURE: 1,
+          SE2P_SLP_GRADIENT: 4, SE2P_SLP_PRESSURE_GRAD: 2, SE2P_SLP_VORTICITY: 1,
+          SE2P_SLP_STRESS: 4, SE2P_DLP_GRADIENT: 4, SE2P_DLP_PRESSURE_GRAD: 2,
+          SE2P_DLP_VORTICITY: 1, SE2P_DLP_STRESS: 4}
 
 
 def kind_code(kind):
-    """'slp' / 'dlp' / 'slp_pressure' / 'dlp_pressure' (or the integer code) -> se2p kind."""
+    """Kind name ('slp', 'dlp', 'slp_pressure', ..., 'dlp_stress') or integer code -> code."""
     if isinstance(kind, str):
         return _KINDS[kind]
     if kind not in _KINDS.values():
@@ -37,13 +53,15 @@ def kind_code(kind):
 
 
 def out_comps(kind):
-    """Values per target: 1 for the pressure, 2 for velocities."""
-    return 1 if kind_code(kind) >= SE2P_SLP_PRESSURE else 2
+    """Values per target: 2 (velocity, pressure gradient), 1 (pressure, vorticity)
+    or 4 (velocity gradient, stress)."""
+    return _NCOMP[kind_code(kind)]
 
 
 def has_normals(kind):
     """Stresslet kinds carry source normals."""
-    return kind_code(kind) in (SE2P_DLP, SE2P_DLP_PRESSURE)
+    k = kind_code(kind)
+    return k in (SE2P_DLP, SE2P_DLP_PRESSURE) or k >= SE2P_DLP_GRADIENT
 
 EXPORTS = (
     "se2p_ctx_create", "se2p_ctx_destroy", "se2p_last_error", "se2p_version",
@@ -57,7 +75,9 @@ EXPORTS = (
     "se2p_batch_sizes", "se2p_batch_apply", "se2p_batch_apply_graph", "se2p_batch_destroy",
     "se2p_slp_pressure_real", "se2p_slp_pressure_kspace", "se2p_slp_pressure_ewald",
     "se2p_dlp_pressure_real", "se2p_dlp_pressure_kspace", "se2p_dlp_pressure_ewald",
-)
+) + tuple(f"se2p_{layer}_{q}_{part}" for layer in ("slp", "dlp")
+          for q in ("gradient", "pressure_grad", "vorticity", "stress")
+          for part in ("real", "kspace"))
 
 
 class Se2pError(RuntimeError):
@@ -154,6 +174,19 @@ def lib():
             "se2p_dlp_pressure_kspace": [P, P, P, P, P, D, D, I, I, I, I, D, D, D, I, P],
             "se2p_dlp_pressure_ewald": [P, P, P, P, P, P, P, P, P, I, I, D, D,
                                         ctypes.POINTER(Options), P, P, P, P],
+            # derived quantities: real parts (psrc, ptar, f[, n], Ns, Nt, xi, nsx, nsy,
+            # Lx, Ly, out); k-space parts in each mex function's argument order
+            **{f"se2p_slp_{q}_real": [P, P, P, P, I, I, D, I, I, D, D, P]
+               for q in ("gradient", "pressure_grad", "vorticity", "stress")},
+            **{f"se2p_dlp_{q}_real": [P, P, P, P, P, I, I, D, I, I, D, D, P]
+               for q in ("gradient", "pressure_grad", "vorticity", "stress")},
+            **{f"se2p_slp_{q}_kspace": [P, P, P, D, D, P, I, I, I, I, D, D, D, I, P]
+               for q in ("gradient", "vorticity", "stress")},
+            "se2p_slp_pressure_grad_kspace": [P, P, P, P, D, D, I, I, I, I, D, D, D, I, P],
+            **{f"se2p_dlp_{q}_kspace": [P, P, P, D, D, P, P, I, I, I, I, D, D, D, I, P]
+               for q in ("gradient", "stress")},
+            **{f"se2p_dlp_{q}_kspace": [P, P, P, P, P, D, D, I, I, I, I, D, D, D, I, P]
+               for q in ("pressure_grad", "vorticity")},
         }
         for name, args in sig.items():
             fn = getattr(L, name)

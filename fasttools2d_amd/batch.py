@@ -61,7 +61,7 @@ class Batch:
                                           for f in fs]))
         self.apply_device(self.f.ptr, self.u.ptr, graph)
         u = self.u.download() if self.nu else np.zeros(0)
-        return [u[self.u_off[i]:self.u_off[i + 1]].reshape(-1, 2) if self.nc[i] == 2
+        return [u[self.u_off[i]:self.u_off[i + 1]].reshape(-1, self.nc[i]) if self.nc[i] > 1
                 else u[self.u_off[i]:self.u_off[i + 1]] for i in range(len(self.nt))]
 
     def close(self):

@@ -8,7 +8,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 // A captured apply for one (f, u) pair.
@@ -137,10 +140,14 @@ namespace {
 // ---------------------------------------------------------------------------
 // Validation
 // ---------------------------------------------------------------------------
-bool valid_kind(int kind)
+bool valid_kind(int kind) { return kind >= 0 && kind < SE2P_NUM_KINDS; }
+
+// Recipe of a kind's MATLAB driver: the SLP one for the Stokeslet quantities and
+// the stresslet vorticity (StokesDLP_vorticity_ewald_2p.m:113-131 and its
+// find_xi/find_kinfb at :170-224 are the SLP recipe), the DLP one otherwise.
+int recipe_of(int kind)
 {
-    return kind == SE2P_SLP || kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE ||
-           kind == SE2P_DLP_PRESSURE;
+    return dlp_family(kind) && kind != SE2P_DLP_VORTICITY ? SE2P_DLP : SE2P_SLP;
 }
 
 int check_common(se2p_ctx* ctx, int Ns, int Nt, double Lx, double Ly)
@@ -457,6 +464,14 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
 // StokesSLP_pressure_ewald_2p.m:1-138 / StokesDLP_pressure_ewald_2p.m: wrap, the
 // SLP or DLP recipe, pr + pk with the zero mode added to pk (SLP: -sum(f1 x +
 // f2 y)/(2 Lx Ly), :127-128; DLP: -sum(n1 f1 + n2 f2)/(2 Lx Ly), :156-157).
+// Hash of a coordinate pair; -0.0 is folded into +0.0 by the callers (+ 0.0).
+struct PairHash {
+    size_t operator()(const std::pair<double, double>& p) const
+    {
+        return std::hash<double>()(p.first) * 1000003u ^ std::hash<double>()(p.second);
+    }
+};
+
 int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc,
                     const double* xtar, const double* ytar, const double* n1,
                     const double* n2, const double* f1, const double* f2, int Ns, int Nt,
@@ -525,6 +540,20 @@ int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* y
         if (pr_out) pr_out[i] = h[i];
         if (pk_out) pk_out[i] = h[Nt + i];
     }
+    if (kind == SE2P_DLP_PRESSURE) {
+        // StokesDLP_pressure_ewald_2p.m:64-70,165-172: a source that is also a
+        // target (same raw coordinates) adds xi^2 (f.n)/(2 pi) to p there.  The
+        // reference adds the k-th coincident source's term to the k-th target,
+        // the coincident one when the targets are the sources.
+        std::unordered_map<std::pair<double, double>, std::vector<int>, PairHash> where;
+        for (int j = 0; j < Nt; ++j) where[{xtar[j] + 0.0, ytar[j] + 0.0}].push_back(j);
+        const double c = prm.xi * prm.xi / (2 * kPi);
+        for (int i = 0; i < Ns; ++i) {
+            auto it = where.find({xsrc[i] + 0.0, ysrc[i] + 0.0});
+            if (it == where.end()) continue;
+            for (int j : it->second) p_out[j] += c * (f1[i] * n1[i] + f2[i] * n2[i]);
+        }
+    }
     return SE2P_OK;
 }
 
@@ -588,6 +617,36 @@ int graph_apply(se2p_ctx* ctx, GraphCache* gc, const BatchExec& B, const double*
     }
     SE2P_HIP_CHECK(hipGraphLaunch(gc->exec, ctx->stream));
     return SE2P_OK;
+}
+
+int derived_real(se2p_ctx* ctx, int kind, const double* psrc, const double* ptar,
+                 const double* f, const double* n, int Nsrc, int Ntar, double xi, int nside_x,
+                 int nside_y, double Lx, double Ly, double* out)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_real(nside_x, nside_y, xi));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    if (dlp_family(kind)) SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(out, Ntar, "out"));
+    return mex_call(ctx, kind, true, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    real_only(xi, nside_x, nside_y), out);
+}
+
+int derived_kspace(se2p_ctx* ctx, int kind, const double* psrc, const double* ptar,
+                   const double* f, const double* n, double xi, double eta, int Nsrc, int Ntar,
+                   int Mx, int My, double Lx, double Ly, double w, int P, double* out)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    SE2P_TRY(check_kspace(Mx, My, Lx, Ly, xi, eta, w, P));
+    SE2P_TRY(need_ptr(psrc, Nsrc, "psrc"));
+    SE2P_TRY(need_ptr(f, Nsrc, "f"));
+    if (dlp_family(kind)) SE2P_TRY(need_ptr(n, Nsrc, "n"));
+    SE2P_TRY(need_ptr(ptar, Ntar, "ptar"));
+    SE2P_TRY(need_ptr(out, Ntar, "out"));
+    return mex_call(ctx, kind, false, psrc, ptar, f, n, Nsrc, Ntar, Lx, Ly,
+                    k_only(xi, eta, Mx, My, w, P), out);
 }
 
 }  // namespace
@@ -778,6 +837,99 @@ int se2p_dlp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* pt
                     k_only(xi, eta, Mx, My, w, P), pk);
 }
 
+// ---------------------------------------------------------------------------
+// Derived quantities: velocity gradient, pressure gradient, vorticity, stress
+// ---------------------------------------------------------------------------
+
+#define SE2P_SLP_DERIVED(NAME, KIND)                                                           \
+    int se2p_slp_##NAME##_real(se2p_ctx* ctx, const double* psrc, const double* ptar,        \
+                               const double* f, int Nsrc, int Ntar, double xi, int nside_x,  \
+                               int nside_y, double Lx, double Ly, double* out)               \
+    {                                                                                        \
+        return derived_real(ctx, KIND, psrc, ptar, f, nullptr, Nsrc, Ntar, xi, nside_x, \
+                                  nside_y, Lx, Ly, out);                                     \
+    }
+#define SE2P_DLP_DERIVED(NAME, KIND)                                                           \
+    int se2p_dlp_##NAME##_real(se2p_ctx* ctx, const double* psrc, const double* ptar,        \
+                               const double* f, const double* n, int Nsrc, int Ntar,         \
+                               double xi, int nside_x, int nside_y, double Lx, double Ly,    \
+                               double* out)                                                  \
+    {                                                                                        \
+        return derived_real(ctx, KIND, psrc, ptar, f, n, Nsrc, Ntar, xi, nside_x,      \
+                                  nside_y, Lx, Ly, out);                                     \
+    }
+SE2P_SLP_DERIVED(gradient, SE2P_SLP_GRADIENT)
+SE2P_SLP_DERIVED(pressure_grad, SE2P_SLP_PRESSURE_GRAD)
+SE2P_SLP_DERIVED(vorticity, SE2P_SLP_VORTICITY)
+SE2P_SLP_DERIVED(stress, SE2P_SLP_STRESS)
+SE2P_DLP_DERIVED(gradient, SE2P_DLP_GRADIENT)
+SE2P_DLP_DERIVED(pressure_grad, SE2P_DLP_PRESSURE_GRAD)
+SE2P_DLP_DERIVED(vorticity, SE2P_DLP_VORTICITY)
+SE2P_DLP_DERIVED(stress, SE2P_DLP_STRESS)
+#undef SE2P_SLP_DERIVED
+#undef SE2P_DLP_DERIVED
+
+// k-space parts, each with its mex function's argument order
+int se2p_slp_gradient_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                             double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                             double Lx, double Ly, double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_SLP_GRADIENT, psrc, ptar, f, nullptr, xi, eta, Nsrc,
+                                Ntar, Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_slp_pressure_grad_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                  const double* f, double xi, double eta, int Nsrc, int Ntar,
+                                  int Mx, int My, double Lx, double Ly, double w, int P,
+                                  double* out)
+{
+    return derived_kspace(ctx, SE2P_SLP_PRESSURE_GRAD, psrc, ptar, f, nullptr, xi, eta,
+                                Nsrc, Ntar, Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_slp_vorticity_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                              double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                              double Lx, double Ly, double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_SLP_VORTICITY, psrc, ptar, f, nullptr, xi, eta, Nsrc,
+                                Ntar, Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_slp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                           double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                           double Lx, double Ly, double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_SLP_STRESS, psrc, ptar, f, nullptr, xi, eta, Nsrc,
+                                Ntar, Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_dlp_gradient_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                             double eta, const double* f, const double* n, int Nsrc, int Ntar,
+                             int Mx, int My, double Lx, double Ly, double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_DLP_GRADIENT, psrc, ptar, f, n, xi, eta, Nsrc, Ntar,
+                                Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_dlp_pressure_grad_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                  const double* f, const double* n, double xi, double eta,
+                                  int Nsrc, int Ntar, int Mx, int My, double Lx, double Ly,
+                                  double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_DLP_PRESSURE_GRAD, psrc, ptar, f, n, xi, eta, Nsrc,
+                                Ntar, Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_dlp_vorticity_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                              const double* f, const double* n, double xi, double eta, int Nsrc,
+                              int Ntar, int Mx, int My, double Lx, double Ly, double w, int P,
+                              double* out)
+{
+    return derived_kspace(ctx, SE2P_DLP_VORTICITY, psrc, ptar, f, n, xi, eta, Nsrc, Ntar,
+                                Mx, My, Lx, Ly, w, P, out);
+}
+int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                           double eta, const double* f, const double* n, int Nsrc, int Ntar,
+                           int Mx, int My, double Lx, double Ly, double w, int P, double* out)
+{
+    return derived_kspace(ctx, SE2P_DLP_STRESS, psrc, ptar, f, n, xi, eta, Nsrc, Ntar, Mx,
+                                My, Lx, Ly, w, P, out);
+}
+
 int se2p_dlp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
                             const double* xtar, const double* ytar, const double* n1,
                             const double* n2, const double* f1, const double* f2, int Nsrc,
@@ -804,9 +956,9 @@ int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly
     if (!out) return set_error(SE2P_ERR_ARG, "null output");
     if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
     if (!(Lx > 0) || !(Ly > 0)) return set_error(SE2P_ERR_ARG, "box lengths must be positive");
-    // the pressure driver uses the velocity driver's recipe
+    // e.g. the pressure driver uses the velocity driver's recipe
     // (StokesSLP_pressure_ewald_2p.m:83-104 = StokesSLP_ewald_2p.m:83-104)
-    return params_ref(dlp_family(kind) ? SE2P_DLP : SE2P_SLP, Nsrc, Ntar, Q, Lx, Ly, opt, out);
+    return params_ref(recipe_of(kind), Nsrc, Ntar, Q, Lx, Ly, opt, out);
 }
 
 int se2p_slp_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc, const double* xtar,

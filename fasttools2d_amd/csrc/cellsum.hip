@@ -10,7 +10,13 @@
 // (mex_stokes_slp_pressure_real.cpp:60-160) uses the same pair set as the DLP
 // (coincident points of the own cell skipped) with the pair term
 // (r.f) e^{-xi^2 r^2}/r^2 and one output scaled by 1/(2 pi); the stresslet
-// pressure (mex_stokes_dlp_pressure_real.cpp:100-192) likewise with its own pair term.  Sources further than one cell away are never within
+// pressure (mex_stokes_dlp_pressure_real.cpp:100-192) likewise with its own pair
+// term.  The other derived quantities (velocity gradient, pressure gradient,
+// vorticity and stress of both layers, mex_stokes_{slp,dlp}_{gradient,
+// pressure_grad,vorticity,stress}_real.cpp) share that pair set too (the
+// stresslet vorticity skips own-cell pairs with r^2 < 1e-13,
+// mex_stokes_dlp_vorticity_real.cpp:128) and write 1, 2 or 4 values per
+// target.  Sources further than one cell away are never within
 // the cutoff, so "own cell, or r^2 < cutoff^2" over any window that contains
 // the 3 x 3 stencil selects exactly the reference's pairs.
 //
@@ -31,6 +37,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace se2p {
 
@@ -41,27 +48,21 @@ struct alignas(64) DlpRec {
 };
 
 template <int KIND>
-struct SrcOf;
-template <>
-struct SrcOf<SE2P_SLP> {
-    using T = double4;
-    static constexpr int cap = 1024;  // 32 KB of staged records: 4 workgroups per CU
+struct SrcOf {
+    using T = typename std::conditional<dlp_family(KIND), DlpRec, double4>::type;
+    // 32 KB of staged records: 4 workgroups per CU
+    static constexpr int cap = dlp_family(KIND) ? 512 : 1024;
 };
-template <>
-struct SrcOf<SE2P_DLP> {
-    using T = DlpRec;
-    static constexpr int cap = 512;
-};
-template <>
-struct SrcOf<SE2P_SLP_PRESSURE> {
-    using T = double4;
-    static constexpr int cap = 1024;
-};
-template <>
-struct SrcOf<SE2P_DLP_PRESSURE> {
-    using T = DlpRec;
-    static constexpr int cap = 512;
-};
+
+// Divisor of the accumulated sums at output.
+constexpr double out_div(int kind)
+{
+    return (kind == SE2P_SLP_PRESSURE || kind == SE2P_DLP_PRESSURE || kind == SE2P_SLP_PRESSURE_GRAD ||
+            kind == SE2P_DLP_PRESSURE_GRAD || kind == SE2P_SLP_VORTICITY)
+               ? 2 * kPi
+           : kind == SE2P_DLP_VORTICITY ? kPi
+                                        : 4 * kPi;
+}
 
 constexpr int kBlkThreads = 256;
 constexpr int kMaxBlockSide = 4;
@@ -112,6 +113,117 @@ __device__ __forceinline__ double dlp_pressure_pair(double r1, double r2, double
     const double ir = rcp64(rr);
     const double rf = r1 * s.f1 + r2 * s.f2, rn = r1 * s.n1 + r2 * s.n2;
     return -exp(-xi2 * rr) * ((s.fn - 2 * xi2 * rn * rf) * ir - 2 * rn * rf * ir * ir);
+}
+
+// Derived quantities, r = target - source, before the output divisor.  Tensor
+// slot 2p + j holds the derivative along p of component j.
+//   SLP gradient (mex_stokes_slp_gradient_real.cpp:118-132):
+//     e [2 xi^2 r_p f_j + (d_pj (r.f) + r_j f_p - r_p f_j)/r^2 - 2 r_p r_j (r.f)(xi^2 + 1/r^2)/r^2]
+//   SLP pressure gradient (mex_stokes_slp_pressure_grad_real.cpp:120-121):
+//     -e [(f_p - 2 xi^2 (r.f) r_p)/r^2 - 2 (r.f) r_p/r^4]
+//   SLP vorticity (mex_stokes_slp_vorticity_real.cpp:118-121):
+//     e (1/r^2 - xi^2)(f1 r2 - f2 r1)
+//   SLP stress (mex_stokes_slp_stress_real.cpp:118-129):
+//     e [2 xi^2 (d_pj (r.f) + r_p f_j + r_j f_p) - 4 r_p r_j (r.f)(1 + xi^2 r^2)/r^4]
+//   DLP gradient (mex_stokes_dlp_gradient_real.cpp:130-145), C = 8 xi^4/r^2 +
+//   16 xi^2/r^4 + 16/r^6:
+//     e [r_p r_j (r.f)(r.n) C - 4 (1 + xi^2 r^2)(d_pj (r.f)(r.n) + r_j f_p (r.n) + r_j n_p (r.f))/r^4
+//        + 2 xi^2 (f_p n_j + n_p f_j + d_pj (f.n) - 2 xi^2 r_p (f_j (r.n) + n_j (r.f) + r_j (f.n)))]
+//   DLP pressure gradient (mex_stokes_dlp_pressure_grad_real.cpp:122-128), with
+//   a_p = r_p (f.n) + f_p (r.n) + n_p (r.f):
+//     e [2 xi^2 (a_p - 2 xi^2 r_p (r.f)(r.n))/r^2 + 2 (a_p - 4 xi^2 r_p (r.n)(r.f))/r^4
+//        - 8 r_p (r.n)(r.f)/r^6]
+//   DLP vorticity (mex_stokes_dlp_vorticity_real.cpp:136-145), with
+//   v = (r.f)(n1 r2 - n2 r1) + (r.n)(f1 r2 - f2 r1):
+//     e [-(1 + xi^2 r^2) v/r^4 + xi^4 v]
+//   DLP stress (mex_stokes_dlp_stress_real.cpp:132-151, mu = 1), twice
+//     e [d_pj ((f.n) - 2 xi^2 (r.f)(r.n))/r^2 - 2 d_pj (r.f)(r.n)/r^4 + r_p r_j (r.f)(r.n) C
+//        - 2 (1 + xi^2 r^2)(f_p r_j (r.n) + n_p r_j (r.f) + r_p f_j (r.n) + r_p n_j (r.f)
+//                            + 2 d_pj (r.f)(r.n))/r^4
+//        + 2 xi^2 (d_pj (f.n) + n_p f_j + f_p n_j
+//                  - xi^2 (2 r_p r_j (f.n) + r_p f_j (r.n) + r_p n_j (r.f) + f_p r_j (r.n) + n_p r_j (r.f)))]
+template <int KIND>
+__device__ __forceinline__ void derived_pair(double r1, double r2, double rr, double xi2,
+                                             const double* __restrict__ s, double* acc)
+{
+    const double e = exp(-xi2 * rr);
+    const double ir = rcp64(rr);
+    const double f[2] = {s[2], s[3]};
+    const double r[2] = {r1, r2};
+    const double rf = r1 * f[0] + r2 * f[1];
+    if (KIND == SE2P_SLP_GRADIENT) {
+        const double c = 2 * (xi2 + ir) * ir * rf;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[2 * p + j] += e * (2 * xi2 * r[p] * f[j] +
+                                       ((p == j ? rf : 0.0) + r[j] * f[p] - r[p] * f[j]) * ir -
+                                       r[p] * r[j] * c);
+    } else if (KIND == SE2P_SLP_PRESSURE_GRAD) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+            acc[p] -= e * ((f[p] - 2 * xi2 * rf * r[p]) * ir - 2 * rf * r[p] * ir * ir);
+    } else if (KIND == SE2P_SLP_VORTICITY) {
+        acc[0] += e * (ir - xi2) * (f[0] * r2 - f[1] * r1);
+    } else if (KIND == SE2P_SLP_STRESS) {
+        const double c = 4 * rf * (1 + xi2 * rr) * ir * ir;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[2 * p + j] += e * (2 * xi2 * ((p == j ? rf : 0.0) + r[p] * f[j] + r[j] * f[p]) -
+                                       r[p] * r[j] * c);
+    } else {
+        const double n[2] = {s[4], s[5]};
+        const double fn = s[6];
+        const double rn = r1 * n[0] + r2 * n[1];
+        const double ir2 = ir * ir;
+        if (KIND == SE2P_DLP_GRADIENT) {
+            const double C = (8 * xi2 * xi2 + (16 * xi2 + 16 * ir) * ir) * ir;
+            const double B = 4 * (1 + xi2 * rr) * ir2;
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[2 * p + j] +=
+                        e * (r[p] * r[j] * rf * rn * C -
+                             B * ((p == j ? rf * rn : 0.0) + r[j] * f[p] * rn + r[j] * n[p] * rf) +
+                             2 * xi2 *
+                                 (f[p] * n[j] + n[p] * f[j] + (p == j ? fn : 0.0) -
+                                  2 * xi2 * r[p] * (f[j] * rn + n[j] * rf + r[j] * fn)));
+        } else if (KIND == SE2P_DLP_PRESSURE_GRAD) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const double a = r[p] * fn + f[p] * rn + n[p] * rf;
+                acc[p] += e * (2 * xi2 * (a - 2 * xi2 * r[p] * rf * rn) * ir +
+                               2 * (a - 4 * xi2 * r[p] * rn * rf) * ir2 -
+                               8 * r[p] * rn * rf * ir2 * ir);
+            }
+        } else if (KIND == SE2P_DLP_VORTICITY) {
+            const double v = rf * (n[0] * r2 - n[1] * r1) + rn * (f[0] * r2 - f[1] * r1);
+            acc[0] += e * (-(1 + xi2 * rr) * v * ir2 + xi2 * xi2 * v);
+        } else {  // SE2P_DLP_STRESS
+            const double C = (8 * xi2 * xi2 + (16 * xi2 + 16 * ir) * ir) * ir;
+            const double B = 2 * (1 + xi2 * rr) * ir2;
+            const double P = (fn - 2 * xi2 * rf * rn) * ir - 2 * rf * rn * ir2;
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const double d = p == j ? 1.0 : 0.0;
+                    const double G =
+                        r[p] * r[j] * rf * rn * C -
+                        B * (f[p] * r[j] * rn + n[p] * r[j] * rf + r[p] * f[j] * rn +
+                             r[p] * n[j] * rf + 2 * d * rf * rn) +
+                        2 * xi2 *
+                            (d * fn + n[p] * f[j] + f[p] * n[j] -
+                             xi2 * (2 * r[p] * r[j] * fn + r[p] * f[j] * rn + r[p] * n[j] * rf +
+                                    f[p] * r[j] * rn + n[p] * r[j] * rf));
+                    acc[2 * p + j] += 2 * e * (d * P + G);
+                }
+        }
+    }
 }
 
 __device__ __forceinline__ int wrapi(int q, int n) { return ((q % n) + n) % n; }
@@ -202,7 +314,9 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             cy = c % bh;
             pt = tsorted[ti];
         }
-        double a1 = 0, a2 = 0;
+        constexpr int NC = out_comps(KIND);
+        double a1 = 0, a2 = 0;                // velocity and pressure kinds
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // derived kinds
         for (int r = 0; r < nround; ++r) {
             const int rn = (total - r + nround - 1) / nround;
             __syncthreads();
@@ -286,6 +400,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             cm = (cm << 1) | (r2 < cut2 ? 1ull : 0ull);
                             if (KIND == SE2P_SLP)
                                 zm = (zm << 1) | (r2 < ixi2 ? 1ull : 0ull);
+                            else if (KIND == SE2P_DLP_VORTICITY)  // own-cell pairs r^2 < 1e-13 skipped
+                                zm = (zm << 1) | (r2 >= 1e-13 ? 1ull : 0ull);
                             else  // coincident points of the own cell are skipped
                                 zm = (zm << 1) | (r2 != 0 ? 1ull : 0ull);
                         }
@@ -328,8 +444,10 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     } else if (KIND == SE2P_SLP_PRESSURE) {
                         const double4& q = reinterpret_cast<const double4&>(s);
                         a1 += (dx * q.z + dy * q.w) * exp(-xi2 * r2) * rcp64(r2);
-                    } else {
+                    } else if (KIND == SE2P_DLP_PRESSURE) {
                         a1 += dlp_pressure_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s));
+                    } else {
+                        derived_pair<KIND>(dx, dy, r2, xi2, reinterpret_cast<const double*>(&s), acc);
                     }
                 }
                 if (KIND == SE2P_SLP) {
@@ -352,11 +470,15 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         }
         if (act) {
             const int o = tperm[ti];
-            if (KIND == SE2P_SLP_PRESSURE || KIND == SE2P_DLP_PRESSURE) {
-                u[o] = a1 / (2 * kPi);
+            constexpr double dv = out_div(KIND);
+            if (KIND >= SE2P_SLP_GRADIENT) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) u[NC * o + c] = acc[c] / dv;
+            } else if (NC == 1) {
+                u[o] = a1 / dv;
             } else {
-                u[2 * o] = a1 / (4 * kPi);
-                u[2 * o + 1] = a2 / (4 * kPi);
+                u[2 * o] = a1 / dv;
+                u[2 * o + 1] = a2 / dv;
             }
         }
     }
@@ -451,16 +573,25 @@ int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys,
         else                                                                                    \
             hipLaunchKernelGGL((k_real_stencil<K, T, false>), gr, bl, 0, s, sys, jobs, u);       \
     } while (0)
-    if (kind == SE2P_SLP && tier == kE1Full)
-        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full);
-    else if (kind == SE2P_SLP)
-        SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
-    else if (kind == SE2P_DLP)
-        SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full);
-    else if (kind == SE2P_SLP_PRESSURE)
-        SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE, kE1Full);
-    else
-        SE2P_LAUNCH_REAL(SE2P_DLP_PRESSURE, kE1Full);
+    switch (kind) {
+    case SE2P_SLP:
+        if (tier == kE1Full)
+            SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full);
+        else
+            SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
+        break;
+    case SE2P_DLP: SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full); break;
+    case SE2P_SLP_PRESSURE: SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE, kE1Full); break;
+    case SE2P_DLP_PRESSURE: SE2P_LAUNCH_REAL(SE2P_DLP_PRESSURE, kE1Full); break;
+    case SE2P_SLP_GRADIENT: SE2P_LAUNCH_REAL(SE2P_SLP_GRADIENT, kE1Full); break;
+    case SE2P_SLP_PRESSURE_GRAD: SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE_GRAD, kE1Full); break;
+    case SE2P_SLP_VORTICITY: SE2P_LAUNCH_REAL(SE2P_SLP_VORTICITY, kE1Full); break;
+    case SE2P_SLP_STRESS: SE2P_LAUNCH_REAL(SE2P_SLP_STRESS, kE1Full); break;
+    case SE2P_DLP_GRADIENT: SE2P_LAUNCH_REAL(SE2P_DLP_GRADIENT, kE1Full); break;
+    case SE2P_DLP_PRESSURE_GRAD: SE2P_LAUNCH_REAL(SE2P_DLP_PRESSURE_GRAD, kE1Full); break;
+    case SE2P_DLP_VORTICITY: SE2P_LAUNCH_REAL(SE2P_DLP_VORTICITY, kE1Full); break;
+    default: SE2P_LAUNCH_REAL(SE2P_DLP_STRESS, kE1Full); break;
+    }
 #undef SE2P_LAUNCH_REAL
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;

@@ -228,6 +228,88 @@ __device__ __forceinline__ void dlp_bracket(double k1, double k2, double a, doub
     v2 = cscale(csub(cadd(cadd(cscale(q22, 2 * k2), cscale(qs, k1)), cscale(tr, k2)), cscale(m, 2 * k2)), e);
 }
 
+// Derived-quantity filters at (k1, k2), with e0 = exp(-(1 - eta) K^2/(4 xi^2))
+// and a = 1/(4 xi^2).  Each output spectrum is i^s R with R real-coefficient in
+// the spread spectra: s = 1 for the Stokeslet gradient, vorticity and stress
+// and the stresslet pressure gradient, s = 0 for the rest.  Slot 2p + j: the
+// derivative along p of component j.  Stokeslet, q = spectra of f:
+//   gradient (mex_stokes_slp_gradient_kspace.cpp:143-172):
+//     R = k_p (K^2 q_j - k_j (k.q)) (1/K^4 + a/K^2) e0
+//   pressure gradient (mex_stokes_slp_pressure_grad_kspace.cpp:126-135):
+//     R = -k_p (k.q) e0/K^2
+//   vorticity (mex_stokes_slp_vorticity_kspace.cpp:142-156):
+//     R = -(q1 k2 - q2 k1)(1/K^2 + a) e0
+//   stress (mex_stokes_slp_stress_kspace.cpp:170-185):
+//     R = (d_pj (k.q) + q_j k_p + q_p k_j - 2 k_p k_j (k.q)/K^2) e0 (1 + a K^2)/K^2
+// Stresslet, with Q the spectra of f_i n_j entering only through A = Q11,
+// S = Q12 + Q21, D = Q22, T = A + D and kQk = k1^2 A + k1 k2 S + k2^2 D:
+//   gradient (mex_stokes_dlp_gradient_kspace.cpp:167-206): R = -k_p v_j with v
+//     the velocity bracket of dlp_bracket
+//   pressure gradient (mex_stokes_dlp_pressure_grad_kspace.cpp:170-173):
+//     R = -k_p kQk e0/K^2
+//   vorticity (mex_stokes_dlp_vorticity_kspace.cpp:160-169):
+//     R = (2 k1 k2 (A - D) + (k2^2 - k1^2) S)(1/K^2 + a) e0
+//   stress (mex_stokes_dlp_stress_kspace.cpp:166-201, mu = 1), c = 1/K^2 + a:
+//     R_0 = e0 (kQk/K^2 - c (2 k1^2 T + 4 k1^2 A + 2 k1 k2 S - 4 k1^2 kQk/K^2))
+//     R_1 = R_2 = -e0 c (2 k1 k2 (T + A + D) + K^2 S - 4 k1 k2 kQk/K^2)
+//     R_3 = e0 (kQk/K^2 - c (2 k2^2 T + 4 k2^2 D + 2 k1 k2 S - 4 k2^2 kQk/K^2))
+template <int KIND>
+__device__ __forceinline__ void derived_op(double k1, double k2, double a, double eta,
+                                           const cplx* q, cplx* R)
+{
+    const double K2 = k1 * k1 + k2 * k2, iK2 = 1.0 / K2;
+    const double e0 = exp(-(1 - eta) * a * K2);
+    const double k[2] = {k1, k2};
+    if (!dlp_family(KIND)) {
+        const cplx kq = cadd(cscale(q[0], k1), cscale(q[1], k2));
+        if (KIND == SE2P_SLP_GRADIENT) {
+            const double c = (iK2 * iK2 + a * iK2) * e0;
+            for (int p = 0; p < 2; ++p)
+                for (int j = 0; j < 2; ++j)
+                    R[2 * p + j] = cscale(csub(cscale(q[j], K2), cscale(kq, k[j])), k[p] * c);
+        } else if (KIND == SE2P_SLP_PRESSURE_GRAD) {
+            for (int p = 0; p < 2; ++p) R[p] = cscale(kq, -k[p] * e0 * iK2);
+        } else if (KIND == SE2P_SLP_VORTICITY) {
+            R[0] = cscale(csub(cscale(q[0], k2), cscale(q[1], k1)), -(iK2 + a) * e0);
+        } else {  // SE2P_SLP_STRESS
+            const double e = e0 * (1 + a * K2) * iK2;
+            for (int p = 0; p < 2; ++p)
+                for (int j = 0; j < 2; ++j) {
+                    cplx v = cadd(cscale(q[j], k[p]), cscale(q[p], k[j]));
+                    v = cadd(v, cscale(kq, (p == j ? 1.0 : 0.0) - 2 * k[p] * k[j] * iK2));
+                    R[2 * p + j] = cscale(v, e);
+                }
+        }
+        return;
+    }
+    const cplx A = q[0], S = cadd(q[1], q[2]), D = q[3];
+    const cplx kQk = cadd(cadd(cscale(A, k1 * k1), cscale(S, k1 * k2)), cscale(D, k2 * k2));
+    if (KIND == SE2P_DLP_GRADIENT) {
+        cplx v[2];
+        dlp_bracket(k1, k2, a, eta, A, S, D, v[0], v[1]);
+        for (int p = 0; p < 2; ++p)
+            for (int j = 0; j < 2; ++j) R[2 * p + j] = cscale(v[j], -k[p]);
+    } else if (KIND == SE2P_DLP_PRESSURE_GRAD) {
+        for (int p = 0; p < 2; ++p) R[p] = cscale(kQk, -k[p] * e0 * iK2);
+    } else if (KIND == SE2P_DLP_VORTICITY) {
+        R[0] = cscale(cadd(cscale(csub(A, D), 2 * k1 * k2), cscale(S, k2 * k2 - k1 * k1)),
+                      (iK2 + a) * e0);
+    } else {  // SE2P_DLP_STRESS
+        const double c = iK2 + a;
+        const cplx T = cadd(A, D);
+        const cplx m = cscale(kQk, iK2);
+        cplx b0 = cadd(cadd(cscale(T, 2 * k1 * k1), cscale(A, 4 * k1 * k1)), cscale(S, 2 * k1 * k2));
+        b0 = csub(b0, cscale(m, 4 * k1 * k1));
+        cplx b3 = cadd(cadd(cscale(T, 2 * k2 * k2), cscale(D, 4 * k2 * k2)), cscale(S, 2 * k1 * k2));
+        b3 = csub(b3, cscale(m, 4 * k2 * k2));
+        cplx b1 = cadd(cscale(T, 4 * k1 * k2), cscale(S, K2));  // 2 k1 k2 (T + A + D)
+        b1 = csub(b1, cscale(m, 4 * k1 * k2));
+        R[0] = cscale(csub(m, cscale(b0, c)), e0);
+        R[1] = R[2] = cscale(b1, -e0 * c);
+        R[3] = cscale(csub(m, cscale(b3, c)), e0);
+    }
+}
+
 // Filter, k = 0 removal and the 1/(Mx My) of the inverse transform.  The
 // reference keeps only the real part of each inverse transform
 // (mex_stokes_slp_kspace.cpp:180-181), i.e. the Hermitian part of each filtered
@@ -250,6 +332,7 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     double2* __restrict__ W = d.grid[G_Z0];
     if (ix == 0 && iy == 0) {
         W[gid] = make_double2(0.0, 0.0);
+        if (out_grids(KIND) == 2) d.grid[G_Z1][gid] = make_double2(0.0, 0.0);
         return;
     }
     const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
@@ -258,6 +341,31 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     const double m1 = wavenum(jx, Mx, d.Lx), m2 = wavenum(jy, My, d.Ly);
     const double a = 0.25 / (d.xi * d.xi), eta = d.eta;
     const double scale = 1.0 / ((double)Mx * (double)My);
+    if (KIND >= SE2P_SLP_GRADIENT) {
+        constexpr int NC = out_comps(KIND);
+        constexpr bool imag = KIND == SE2P_SLP_GRADIENT || KIND == SE2P_SLP_VORTICITY ||
+                              KIND == SE2P_SLP_STRESS || KIND == SE2P_DLP_PRESSURE_GRAD;
+        cplx q[4];
+        unpack(Z0[gid], Z0[gm], q[0], q[1]);
+        if (dlp_family(KIND)) unpack(Z1[gid], Z1[gm], q[2], q[3]);  // (f1 n2, f2 n2)
+        cplx Rk[4], Rm[4], G[4];
+        derived_op<KIND>(k1, k2, a, eta, q, Rk);
+        derived_op<KIND>(m1, m2, a, eta, q, Rm);
+        // Hermitian part of i^s R: i^s (R(k) + (-1)^s R(k'))/2
+        for (int c = 0; c < NC; ++c) {
+            const cplx h = cscale(imag ? csub(Rk[c], Rm[c]) : cadd(Rk[c], Rm[c]), 0.5);
+            G[c] = imag ? cplx{-h.im, h.re} : h;
+        }
+        if (NC == 1) {
+            W[gid] = make_double2(G[0].re * scale, G[0].im * scale);
+        } else {
+            W[gid] = make_double2((G[0].re - G[1].im) * scale, (G[0].im + G[1].re) * scale);
+            if (NC == 4)
+                d.grid[G_Z1][gid] =
+                    make_double2((G[2].re - G[3].im) * scale, (G[2].im + G[3].re) * scale);
+        }
+        return;
+    }
     cplx g1, g2;
     if (KIND == SE2P_SLP) {
         cplx q1, q2;
@@ -315,8 +423,8 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
 
 __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs,
-                                               double* __restrict__ u_base, int accumulate,
-                                               int use_zm)
+                                               double* __restrict__ u_base, int g,
+                                               int accumulate, int use_zm)
 {
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
@@ -324,7 +432,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const int* __restrict__ perm = d.gt_perm;
     const int2* __restrict__ node = d.gt_node;
     const double2* __restrict__ off = d.gt_off;
-    const double2* __restrict__ W = d.grid[G_W];
+    const double2* __restrict__ W = d.grid[g ? G_D : G_W];
     const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, P = d.P, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
     const double* __restrict__ e1tab = d.e1tab;
@@ -367,16 +475,17 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
             ex *= e3x;
         }
         double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
-        if (d.nc == 1) {  // one real field (pressure): the real part of W
+        if (d.nc == 1) {  // one real field: the real part of W
             u[p] = accumulate ? u[p] + v0 : v0;
             continue;
         }
+        double* up = u + (size_t)d.nc * p + 2 * g;
         if (accumulate) {
-            v0 = u[2 * p] + v0;
-            v1 = u[2 * p + 1] + v1;
+            v0 = up[0] + v0;
+            v1 = up[1] + v1;
         }
-        u[2 * p] = v0;
-        u[2 * p + 1] = v1;
+        up[0] = v0;
+        up[1] = v1;
     }
 }
 
@@ -409,6 +518,9 @@ __global__ __launch_bounds__(256) void k_zero_mode_part(const SysDesc* __restric
             ay += r[6] * r[1];
         } else if (d.kind == SE2P_DLP_PRESSURE) {  // f.n
             ax += rec[8 * (size_t)i + 6];
+        } else if (d.kind == SE2P_SLP_VORTICITY) {  // {x, y, f1, f2}: f1 y - f2 x
+            const double* r = rec + 4 * (size_t)i;
+            ax += r[2] * r[1] - r[3] * r[0];
         } else {  // {x, y, f1, f2}: f.x
             const double* r = rec + 4 * (size_t)i;
             ax += r[2] * r[0] + r[3] * r[1];
@@ -443,7 +555,8 @@ __global__ void k_zero_mode_final(const SysDesc* __restrict__ sys, const Job* __
     if (d.kind == SE2P_DLP) {  // StokesDLP_ewald_2p.m:149-151
         d.zm[0] = sx * inv_area;
         d.zm[1] = sy * inv_area;
-    } else {  // StokesSLP_pressure_ewald_2p.m:127-128, StokesDLP_pressure_ewald_2p.m:156-157
+    } else {  // StokesSLP_pressure_ewald_2p.m:127-128, StokesDLP_pressure_ewald_2p.m:156-157,
+              // StokesSLP_vorticity_ewald_2p.m:145
         d.zm[0] = -sx * inv_area / 2;
         d.zm[1] = 0;
     }
@@ -533,24 +646,34 @@ int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, in
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    if (kind == SE2P_SLP)
-        hipLaunchKernelGGL(k_filter<SE2P_SLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
-    else if (kind == SE2P_DLP)
-        hipLaunchKernelGGL(k_filter<SE2P_DLP>, dim3(njobs), dim3(256), 0, s, sys, jobs);
-    else if (kind == SE2P_SLP_PRESSURE)
-        hipLaunchKernelGGL(k_filter<SE2P_SLP_PRESSURE>, dim3(njobs), dim3(256), 0, s, sys, jobs);
-    else
-        hipLaunchKernelGGL(k_filter<SE2P_DLP_PRESSURE>, dim3(njobs), dim3(256), 0, s, sys, jobs);
+#define SE2P_LAUNCH_FILTER(K) \
+    hipLaunchKernelGGL(k_filter<K>, dim3(njobs), dim3(256), 0, s, sys, jobs)
+    switch (kind) {
+    case SE2P_SLP: SE2P_LAUNCH_FILTER(SE2P_SLP); break;
+    case SE2P_DLP: SE2P_LAUNCH_FILTER(SE2P_DLP); break;
+    case SE2P_SLP_PRESSURE: SE2P_LAUNCH_FILTER(SE2P_SLP_PRESSURE); break;
+    case SE2P_DLP_PRESSURE: SE2P_LAUNCH_FILTER(SE2P_DLP_PRESSURE); break;
+    case SE2P_SLP_GRADIENT: SE2P_LAUNCH_FILTER(SE2P_SLP_GRADIENT); break;
+    case SE2P_SLP_PRESSURE_GRAD: SE2P_LAUNCH_FILTER(SE2P_SLP_PRESSURE_GRAD); break;
+    case SE2P_SLP_VORTICITY: SE2P_LAUNCH_FILTER(SE2P_SLP_VORTICITY); break;
+    case SE2P_SLP_STRESS: SE2P_LAUNCH_FILTER(SE2P_SLP_STRESS); break;
+    case SE2P_DLP_GRADIENT: SE2P_LAUNCH_FILTER(SE2P_DLP_GRADIENT); break;
+    case SE2P_DLP_PRESSURE_GRAD: SE2P_LAUNCH_FILTER(SE2P_DLP_PRESSURE_GRAD); break;
+    case SE2P_DLP_VORTICITY: SE2P_LAUNCH_FILTER(SE2P_DLP_VORTICITY); break;
+    default: SE2P_LAUNCH_FILTER(SE2P_DLP_STRESS); break;
+    }
+#undef SE2P_LAUNCH_FILTER
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
+int launch_gather(hipStream_t s, int maxR, int g, const SysDesc* sys, const Job* jobs, int njobs,
                   double* u, int accumulate, int use_zm)
 {
     if (njobs <= 0) return SE2P_OK;
     const size_t lds = sizeof(double2) * (size_t)maxR * maxR;
-    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(256), lds, s, sys, jobs, u, accumulate, use_zm);
+    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(256), lds, s, sys, jobs, u, g, accumulate,
+                       use_zm);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

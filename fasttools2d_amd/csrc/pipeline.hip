@@ -92,7 +92,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
-        if (kind != SE2P_SLP) {  // zero mode of the DLP and of the SLP pressure
+        if (has_zero_mode(kind)) {
             SE2P_TRY(ws_get(ctx, ws, WS_REDUCE, sizeof(double) * 512, &p));
             G->zm_part = (double*)p;
             SE2P_TRY(ws_get(ctx, ws, WS_ZM, sizeof(double) * 2, &p));
@@ -136,13 +136,15 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->partial = (double*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_A, sizeof(double2) * Gn, &p));
         G->Z0 = (double2*)p;
-        if (dlp_family(kind)) {
+        // second grid pair: stresslet spreading (f n2) or a second output grid
+        const bool two = dlp_family(kind) || out_grids(kind) == 2;
+        if (two) {
             SE2P_TRY(ws_get(ctx, ws, WS_GRID_B, sizeof(double2) * Gn, &p));
             G->Z1 = (double2*)p;
         }
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_C, sizeof(double2) * Gn, &p));
         G->W = (double2*)p;
-        if (dlp_family(kind)) {
+        if (two) {
             SE2P_TRY(ws_get(ctx, ws, WS_GRID_D, sizeof(double2) * Gn, &p));
             G->D = (double2*)p;
         }
@@ -256,6 +258,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         B->has_real |= G.has_real;
         B->has_k |= G.has_k;
         B->any_dlp |= dlp_family(G.kind);
+        B->any_g1 |= G.has_k && out_grids(G.kind) == 2;
         if (G.has_k) {
             size_t& ly = B->lds_y[fft_line_class(G.kg.My)];
             size_t& lx = B->lds_x[fft_line_class(G.kg.Mx)];
@@ -290,10 +293,10 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 });
     }
     add(B->zm_part, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && d.kind != SE2P_SLP) blocks(i, d.zm_nblk);
+        if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, d.zm_nblk);
     });
     add(B->zm_sys, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && d.kind != SE2P_SLP) blocks(i, 1);
+        if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, 1);
     });
     for (int c = 0; c < 2; ++c) {
         auto want = [&](int i, const SysDesc& d) {
@@ -318,20 +321,25 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
         });
     }
-    for (int cls = 0; cls < 2; ++cls) {
-        add(B->ifftx[cls], [&](int i, const SysDesc& d) {
-            if (geoms[i]->has_k && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+    for (int g = 0; g < 2; ++g) {
+        auto want = [&](int i, const SysDesc& d) {
+            return geoms[i]->has_k && (g == 0 || out_grids(d.kind) == 2);
+        };
+        for (int cls = 0; cls < 2; ++cls) {
+            add(B->ifftx[g][cls], [&](int i, const SysDesc& d) {
+                if (want(i, d) && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
+            });
+            add(B->iffty[g][cls], [&](int i, const SysDesc& d) {
+                if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
+            });
+        }
+        add(B->tr_inv[g], [&](int i, const SysDesc& d) {
+            if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
         });
-        add(B->iffty[cls], [&](int i, const SysDesc& d) {
-            if (geoms[i]->has_k && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
+        add(B->gather[g], [&](int i, const SysDesc& d) {
+            if (want(i, d) && d.Nt > 0) blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
         });
     }
-    add(B->tr_inv, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k) blocks(i, transpose_tiles(d.My, d.Mx));
-    });
-    add(B->gather, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k && d.Nt > 0) blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
-    });
     if (B->has_k) SE2P_TRY(fft_prepare());
     void *psys, *pjobs;
     SE2P_TRY(ws_get(ctx, ws, WS_SYS, sizeof(SysDesc) * (size_t)std::max(1, nsys), &psys));
@@ -389,8 +397,8 @@ int batch_spread_grid(se2p_ctx* ctx, const BatchExec& B, const double* d_f, int 
 int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
 {
     StageTimer st(ctx, "gather");
-    return launch_gather(ctx->stream, B.maxR, B.sys, B.jobs + B.gather.first, B.gather.count, d_u,
-                         0, 0);
+    return launch_gather(ctx->stream, B.maxR, 0, B.sys, B.jobs + B.gather[0].first,
+                         B.gather[0].count, d_u, 0, 0);
 }
 
 int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
@@ -403,6 +411,7 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
     // filter in that layout, and back.
     //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
+    // Kinds with 4 values per target also filter into Z1 and return it through D.
     for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
         SE2P_TRY(batch_spread_grid(ctx, B, d_f, c));
         StageTimer st(ctx, "fft");
@@ -420,19 +429,22 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
         for (int kind = 0; kind < kNumKinds; ++kind)
             SE2P_TRY(launch_filter(s, kind, B.sys, J + B.filter[kind].first, B.filter[kind].count));
     }
-    {
+    for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g) {
         StageTimer st(ctx, "fft");
+        const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
         for (int cls = 0; cls < 2; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 1, G_Z0, 1, cls, B.lds_x[cls], B.sys,
-                                      J + B.ifftx[cls].first, B.ifftx[cls].count));
-        SE2P_TRY(launch_transpose(s, G_Z0, G_W, 1, B.sys, J + B.tr_inv.first, B.tr_inv.count));
+            SE2P_TRY(launch_fft_lines(s, 1, Zg, 1, cls, B.lds_x[cls], B.sys,
+                                      J + B.ifftx[g][cls].first, B.ifftx[g][cls].count));
+        SE2P_TRY(launch_transpose(s, Zg, Tg, 1, B.sys, J + B.tr_inv[g].first, B.tr_inv[g].count));
         for (int cls = 0; cls < 2; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 0, G_W, 1, cls, B.lds_y[cls], B.sys,
-                                      J + B.iffty[cls].first, B.iffty[cls].count));
+            SE2P_TRY(launch_fft_lines(s, 0, Tg, 1, cls, B.lds_y[cls], B.sys,
+                                      J + B.iffty[g][cls].first, B.iffty[g][cls].count));
     }
     StageTimer st(ctx, "gather");
-    return launch_gather(s, B.maxR, B.sys, J + B.gather.first, B.gather.count, d_u,
-                         accumulate ? 1 : 0, use_zm ? 1 : 0);
+    for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g)
+        SE2P_TRY(launch_gather(s, B.maxR, g, B.sys, J + B.gather[g].first, B.gather[g].count, d_u,
+                               accumulate ? 1 : 0, use_zm && g == 0 ? 1 : 0));
+    return SE2P_OK;
 }
 
 int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u)

@@ -14,17 +14,35 @@ namespace se2p {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Kinds: SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE, SE2P_DLP_PRESSURE.
-constexpr int kNumKinds = 4;
+// Kinds SE2P_SLP .. SE2P_DLP_STRESS.
+constexpr int kNumKinds = SE2P_NUM_KINDS;
 // Stresslet kinds: normals, two spreading passes (f n1, f n2).
-__host__ __device__ inline bool dlp_family(int kind)
+__host__ __device__ constexpr bool dlp_family(int kind)
 {
-    return kind == SE2P_DLP || kind == SE2P_DLP_PRESSURE;
+    return kind == SE2P_DLP || kind == SE2P_DLP_PRESSURE || kind >= SE2P_DLP_GRADIENT;
 }
-// Values per target: 2 (velocity) or 1 (pressure).
-inline int out_comps(int kind) { return kind >= SE2P_SLP_PRESSURE ? 1 : 2; }
+// Values per target: 2 (velocity, pressure gradient), 1 (pressure,
+// vorticity) or 4 (velocity gradient, stress).
+__host__ __device__ constexpr int out_comps(int kind)
+{
+    return (kind == SE2P_SLP_PRESSURE || kind == SE2P_DLP_PRESSURE || kind == SE2P_SLP_VORTICITY ||
+            kind == SE2P_DLP_VORTICITY)
+               ? 1
+           : (kind == SE2P_SLP_GRADIENT || kind == SE2P_SLP_STRESS || kind == SE2P_DLP_GRADIENT ||
+              kind == SE2P_DLP_STRESS)
+               ? 4
+               : 2;
+}
+// Kinds whose full sum carries a zero mode that is the same for every target.
+__host__ __device__ constexpr bool has_zero_mode(int kind)
+{
+    return kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE || kind == SE2P_DLP_PRESSURE ||
+           kind == SE2P_SLP_VORTICITY;
+}
 // Sorted source records: stresslet {x, y, f1, f2, n1, n2, f.n, 0}, else {x, y, f1, f2}.
-__host__ __device__ inline int rec_layout(int kind) { return dlp_family(kind) ? 1 : 0; }
+__host__ __device__ constexpr int rec_layout(int kind) { return dlp_family(kind) ? 1 : 0; }
+// Complex grids of the k-space output (two real values per grid).
+__host__ __device__ constexpr int out_grids(int kind) { return out_comps(kind) == 4 ? 2 : 1; }
 
 // ---------------------------------------------------------------------------
 // Status helpers
@@ -228,15 +246,18 @@ int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jo
                   const double* f);
 // Halo merge into grid `out`; job b = ix * ceil(My/256) + y block.
 int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs);
-// Filter: W (and D for the DLP) in transposed layout Z[iy*Mx + ix] -> Z0.
+// Filter: W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
+// Z0 (and Z1 for kinds with 4 values per target).
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
-// Gather from W into u (+= when accumulate; + the DLP zero mode when use_zm);
-// maxR: the largest gathering region side of the jobs' systems.
-int launch_gather(hipStream_t s, int maxR, const SysDesc* sys, const Job* jobs, int njobs,
+// Gather of output grid g (W for g = 0, D for g = 1) into values 2g, 2g + 1 of
+// each target (+= when accumulate; + the zero mode when use_zm); maxR: the
+// largest gathering region side of the jobs' systems.
+int launch_gather(hipStream_t s, int maxR, int g, const SysDesc* sys, const Job* jobs, int njobs,
                   double* u, int accumulate, int use_zm);
 // Zero modes from the sorted records (DLP: sum_s (f.n)_s x_s / (Lx Ly); SLP
-// pressure: -sum_s f_s.x_s / (2 Lx Ly)): partial sums over jobs (s, b), b <
-// zm_nblk, then one job per system.
+// pressure: -sum_s f_s.x_s / (2 Lx Ly); DLP pressure: -sum_s (f.n)_s / (2 Lx Ly);
+// SLP vorticity: -sum_s (f1 y - f2 x)_s / (2 Lx Ly)): partial sums over jobs
+// (s, b), b < zm_nblk, then one job per system.
 int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, int npart,
                      const Job* sys_jobs, int nsys);
 // Density fill of the sorted source records (layout: rec_layout).
@@ -293,7 +314,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d);
 // The descriptors and job lists of a set of systems (device arrays in `ws`).
 struct BatchExec {
     int nsys = 0;
-    bool has_real = false, has_k = false, any_dlp = false;
+    bool has_real = false, has_k = false, any_dlp = false, any_g1 = false;
     SysDesc* sys = nullptr;
     Job* jobs = nullptr;
     JobRange fill[2];     // by record layout
@@ -303,7 +324,8 @@ struct BatchExec {
     JobRange merge[2];       // [c]
     JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
     JobRange filter[kNumKinds];  // by kind
-    JobRange ifftx[2], iffty[2], tr_inv, gather;  // [line class]
+    // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
+    JobRange ifftx[2][2], iffty[2][2], tr_inv[2], gather[2];  // [g][line class]
     size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
     int maxR = 32;   // largest gathering region side
 };

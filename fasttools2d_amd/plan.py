@@ -77,7 +77,7 @@ class Plan:
                 dn.free()
         self.f = DeviceArray(ctx, (self.Ns, 2))
         nc = _lib.out_comps(self.kind)
-        self.u = DeviceArray(ctx, (self.Nt, 2) if nc == 2 else (self.Nt,))
+        self.u = DeviceArray(ctx, (self.Nt, nc) if nc > 1 else (self.Nt,))
 
     def apply_device(self, d_f, d_u, graph=False):
         fn = lib().se2p_plan_apply_graph if graph else lib().se2p_plan_apply

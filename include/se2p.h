@@ -35,6 +35,18 @@ extern "C" {
 #define SE2P_DLP 1 /* stresslet, double-layer potential */
 #define SE2P_SLP_PRESSURE 2 /* pressure of the Stokeslet (one value per target) */
 #define SE2P_DLP_PRESSURE 3 /* pressure of the stresslet (one value per target) */
+/* Further derived quantities.  Tensors are 4 values per target in the order
+ * [d/dx of component 1, d/dx of component 2, d/dy of component 1, d/dy of
+ * component 2] (e.g. mex_stokes_slp_gradient_real.cpp:118-132). */
+#define SE2P_SLP_GRADIENT 4      /* Stokeslet velocity gradient (4 per target) */
+#define SE2P_SLP_PRESSURE_GRAD 5 /* Stokeslet pressure gradient (2 per target) */
+#define SE2P_SLP_VORTICITY 6     /* Stokeslet vorticity (1 per target) */
+#define SE2P_SLP_STRESS 7        /* Stokeslet stress tensor (4 per target) */
+#define SE2P_DLP_GRADIENT 8      /* stresslet velocity gradient (4 per target) */
+#define SE2P_DLP_PRESSURE_GRAD 9 /* stresslet pressure gradient (2 per target) */
+#define SE2P_DLP_VORTICITY 10    /* stresslet vorticity (1 per target) */
+#define SE2P_DLP_STRESS 11       /* stresslet stress tensor (4 per target) */
+#define SE2P_NUM_KINDS 12
 
 /* Arithmetic of a plan (only FP64 in this build). */
 #define SE2P_FP64 0
@@ -144,18 +156,111 @@ int se2p_dlp_pressure_kspace(se2p_ctx* ctx, const double* psrc, const double* pt
                              double* pk);
 /* StokesDLP_pressure_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, ...)
  * (StokesDLP_pressure_ewald_2p.m): wrapping, the DLP parameter recipe, p = pr + pk
- * with the zero mode -sum(n1 f1 + n2 f2)/(2 Lx Ly) in pk. */
+ * with the zero mode -sum(n1 f1 + n2 f2)/(2 Lx Ly) in pk, plus xi^2 (f.n)/(2 pi)
+ * of each source at the targets with its exact (unwrapped) coordinates
+ * (:64-70,165-172). */
 int se2p_dlp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,
                             const double* xtar, const double* ytar, const double* n1,
                             const double* n2, const double* f1, const double* f2, int Nsrc,
                             int Ntar, double Lx, double Ly, const se2p_options* opt, double* p,
                             double* pr, double* pk, double* xi);
 
+/* ------------------------------------------------------------------------ */
+/* Derived quantities: velocity gradient, pressure gradient, vorticity and   */
+/* stress of both layers.  Each pair replaces the reference's real-space and */
+/* k-space mex functions with their argument order; outputs are written in  */
+/* full (nc x Ntar, nc = 4, 2, 1, 4).  The k-space parts hold no zero mode   */
+/* and no self term: the MATLAB drivers add those.  SLP arguments: psrc,     */
+/* ptar, f; DLP arguments add the normals n after f.                         */
+/* ------------------------------------------------------------------------ */
+/* mex_stokes_slp_gradient_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_gradient_real.cpp:9-217) and
+ * mex_stokes_slp_gradient_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_slp_gradient_kspace.cpp:3-288); out 4 x Ntar. */
+int se2p_slp_gradient_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, int Nsrc, int Ntar, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, double* out);
+int se2p_slp_gradient_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                             double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                             double Lx, double Ly, double w, int P, double* out);
+/* mex_stokes_slp_pressure_grad_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)
+ * (mex_stokes_slp_pressure_grad_real.cpp:17-186) and
+ * mex_stokes_slp_pressure_grad_kspace(psrc, ptar, f, xi, eta, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_slp_pressure_grad_kspace.cpp:7-195, f before xi); out 2 x Ntar. */
+int se2p_slp_pressure_grad_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                const double* f, int Nsrc, int Ntar, double xi, int nside_x,
+                                int nside_y, double Lx, double Ly, double* out);
+int se2p_slp_pressure_grad_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                  const double* f, double xi, double eta, int Nsrc, int Ntar,
+                                  int Mx, int My, double Lx, double Ly, double w, int P,
+                                  double* out);
+/* mex_stokes_slp_vorticity_real (mex_stokes_slp_vorticity_real.cpp:9-187) and
+ * mex_stokes_slp_vorticity_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_slp_vorticity_kspace.cpp:3-203); out 1 x Ntar. */
+int se2p_slp_vorticity_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                            const double* f, int Nsrc, int Ntar, double xi, int nside_x,
+                            int nside_y, double Lx, double Ly, double* out);
+int se2p_slp_vorticity_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                              double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                              double Lx, double Ly, double w, int P, double* out);
+/* mex_stokes_slp_stress_real (mex_stokes_slp_stress_real.cpp:9-217) and
+ * mex_stokes_slp_stress_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_slp_stress_kspace.cpp:3-293); out 4 x Ntar. */
+int se2p_slp_stress_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                         int Nsrc, int Ntar, double xi, int nside_x, int nside_y, double Lx,
+                         double Ly, double* out);
+int se2p_slp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                           double eta, const double* f, int Nsrc, int Ntar, int Mx, int My,
+                           double Lx, double Ly, double w, int P, double* out);
+/* mex_stokes_dlp_gradient_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly)
+ * (PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_gradient_real.cpp:9-240) and
+ * mex_stokes_dlp_gradient_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_dlp_gradient_kspace.cpp:3-345); out 4 x Ntar. */
+int se2p_dlp_gradient_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                           const double* f, const double* n, int Nsrc, int Ntar, double xi,
+                           int nside_x, int nside_y, double Lx, double Ly, double* out);
+int se2p_dlp_gradient_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                             double eta, const double* f, const double* n, int Nsrc, int Ntar,
+                             int Mx, int My, double Lx, double Ly, double w, int P, double* out);
+/* mex_stokes_dlp_pressure_grad_real (mex_stokes_dlp_pressure_grad_real.cpp:17-207) and
+ * mex_stokes_dlp_pressure_grad_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_dlp_pressure_grad_kspace.cpp:10-269); out 2 x Ntar. */
+int se2p_dlp_pressure_grad_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                const double* f, const double* n, int Nsrc, int Ntar, double xi,
+                                int nside_x, int nside_y, double Lx, double Ly, double* out);
+int se2p_dlp_pressure_grad_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                                  const double* f, const double* n, double xi, double eta,
+                                  int Nsrc, int Ntar, int Mx, int My, double Lx, double Ly,
+                                  double w, int P, double* out);
+/* mex_stokes_dlp_vorticity_real (mex_stokes_dlp_vorticity_real.cpp:19-221) and
+ * mex_stokes_dlp_vorticity_kspace(psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_dlp_vorticity_kspace.cpp:3-252); out 1 x Ntar. */
+int se2p_dlp_vorticity_real(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                            const double* f, const double* n, int Nsrc, int Ntar, double xi,
+                            int nside_x, int nside_y, double Lx, double Ly, double* out);
+int se2p_dlp_vorticity_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar,
+                              const double* f, const double* n, double xi, double eta, int Nsrc,
+                              int Ntar, int Mx, int My, double Lx, double Ly, double w, int P,
+                              double* out);
+/* mex_stokes_dlp_stress_real (mex_stokes_dlp_stress_real.cpp:9-248) and
+ * mex_stokes_dlp_stress_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P)
+ * (mex_stokes_dlp_stress_kspace.cpp:3-340); out 4 x Ntar. */
+int se2p_dlp_stress_real(se2p_ctx* ctx, const double* psrc, const double* ptar, const double* f,
+                         const double* n, int Nsrc, int Ntar, double xi, int nside_x,
+                         int nside_y, double Lx, double Ly, double* out);
+int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar, double xi,
+                           double eta, const double* f, const double* n, int Nsrc, int Ntar,
+                           int Mx, int My, double Lx, double Ly, double w, int P, double* out);
+
 void se2p_options_default(se2p_options* opt);
 
 /* Parameter recipe of StokesSLP_ewald_2p.m:27-37,84-104,147-202 (kind SE2P_SLP)
  * or StokesDLP_ewald_2p.m:28-38,104-125,171-225 (kind SE2P_DLP).  Q is
- * sum(f.^2) + 1 over both components of the density. */
+ * sum(f.^2) + 1 over both components of the density.  A derived kind takes the
+ * recipe its MATLAB driver uses: the SLP recipe for every Stokeslet quantity
+ * and for the stresslet vorticity (StokesDLP_vorticity_ewald_2p.m:113-131,
+ * whose find_xi/find_kinfb are the SLP ones), the DLP recipe for the other
+ * stresslet quantities. */
 int se2p_params_ref(int kind, int Nsrc, int Ntar, double Q, double Lx, double Ly,
                     const se2p_options* opt, se2p_params* out);
 
@@ -190,10 +295,12 @@ int se2p_ctx_sync(se2p_ctx* ctx);
 
 /* One complete evaluation on device pointers with explicit parameters: binning,
  * real space and k space recomputed on every call, as the reference does.
- * d_n must be given for SE2P_DLP (NULL otherwise).  d_u (2 x Ntar; Ntar values
- * for SE2P_SLP_PRESSURE) receives
- * ur + uk (+ the DLP zero mode).  Stream-ordered on the context's stream; call
- * se2p_ctx_sync() before reading d_u. */
+ * d_n must be given for the stresslet kinds (NULL otherwise).  d_u (nc x Ntar,
+ * nc the kind's values per target) receives ur + uk, plus the zero mode for the
+ * kinds whose driver adds one that is the same for every target: DLP velocity
+ * (StokesDLP_ewald_2p.m:149-151), both pressures and the SLP vorticity
+ * (StokesSLP_vorticity_ewald_2p.m:145).  Stream-ordered on the context's
+ * stream; call se2p_ctx_sync() before reading d_u. */
 int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
                      const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
                      double Ly, const se2p_params* prm, double* d_u);
@@ -205,13 +312,14 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
 /* ------------------------------------------------------------------------ */
 
 /* Bins sources and targets, orders them, computes the start nodes and keeps
- * every buffer an evaluation needs.  Device pointers; d_n for SE2P_DLP only.
+ * every buffer an evaluation needs.  Device pointers; d_n for stresslet kinds.
  * The point arrays are read during creation and not retained. */
 int se2p_plan_create(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
                      const double* d_n, int Nsrc, int Ntar, double Lx, double Ly,
                      const se2p_params* prm, se2p_plan** out);
-/* d_u (2 x Ntar) = ur + uk (+ DLP zero mode) for density d_f (2 x Nsrc).
- * Stream-ordered on the context's stream; allocates nothing. */
+/* d_u (nc x Ntar) = ur + uk (+ the zero mode, as for se2p_eval_device) for
+ * density d_f (2 x Nsrc).  Stream-ordered on the context's stream; allocates
+ * nothing. */
 int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u);
 /* As se2p_plan_apply, replayed from a HIP graph captured on the first call for
  * this (d_f, d_u) pair; a new pair re-captures. */
@@ -225,19 +333,19 @@ int se2p_plan_destroy(se2p_plan* plan);
 /* reference (one system per mex call, StokesSLP_ewald_2p.m:119,126).        */
 /* ------------------------------------------------------------------------ */
 typedef struct se2p_system {
-    int kind;             /* SE2P_SLP, SE2P_DLP, SE2P_SLP_PRESSURE or SE2P_DLP_PRESSURE */
+    int kind;             /* SE2P_SLP .. SE2P_DLP_STRESS */
     int Nsrc, Ntar;
     double Lx, Ly;
     const double* d_psrc; /* 2 x Nsrc device array, read during creation only */
     const double* d_ptar; /* 2 x Ntar */
-    const double* d_n;    /* 2 x Nsrc normals (SE2P_DLP, SE2P_DLP_PRESSURE), else NULL */
+    const double* d_n;    /* 2 x Nsrc normals (stresslet kinds), else NULL */
     se2p_params prm;      /* e.g. from se2p_params_ref at the system's tolerance */
 } se2p_system;
 
 int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_batch** out);
-/* Doubles of the concatenated densities (sum of 2 Nsrc) and outputs (sum of 2 Ntar). */
+/* Doubles of the concatenated densities (sum of 2 Nsrc) and outputs (sum of nc Ntar). */
 int se2p_batch_sizes(const se2p_batch* batch, long long* nf, long long* nu);
-/* d_u = u of every system, systems back to back (2 x Ntar each), for the
+/* d_u = u of every system, systems back to back (nc x Ntar each), for the
  * densities d_f (2 x Nsrc each, same order).  Stream-ordered; allocates nothing. */
 int se2p_batch_apply(se2p_batch* batch, const double* d_f, double* d_u);
 /* As se2p_batch_apply, replayed from a HIP graph captured on the first call for

@@ -66,6 +66,11 @@ def lib():
                                         _P],
             "orc_dlp_pressure_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
             "orc_dlp_pressure_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+            "orc_derived_real": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+            "orc_derived_kspace": [_I, _P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
+                                   _P],
+            "orc_derived_real_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
+            "orc_derived_kspace_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -319,6 +324,67 @@ def dlp_pressure_kspace_ds(psrc, ptar, f, n, Lx, Ly, xi, kinf):
     lib().orc_dlp_pressure_kspace_ds(_ptr(psrc), _ptr(ptar), _ptr(f), _ptr(n), psrc.shape[0],
                                      ptar.shape[0], Lx, Ly, xi, kinf, _ptr(p))
     return p
+
+
+# Derived quantities (kind codes of the C oracle; the same numbers as the
+# product's SE2P_* kinds).
+DERIVED = {"slp_gradient": 4, "slp_pressure_grad": 5, "slp_vorticity": 6, "slp_stress": 7,
+           "dlp_gradient": 8, "dlp_pressure_grad": 9, "dlp_vorticity": 10, "dlp_stress": 11}
+NCOMP = {4: 4, 5: 2, 6: 1, 7: 4, 8: 4, 9: 2, 10: 1, 11: 4}
+
+
+def _kind(kind):
+    return DERIVED[kind] if isinstance(kind, str) else int(kind)
+
+
+def _nullable(a):
+    return None if a is None else _c(a)
+
+
+def derived_real(kind, psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly):
+    """Real-space mex restatement; returns (Nt, nc) (rows in the mex order)."""
+    k = _kind(kind)
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _nullable(n)
+    out = np.zeros((ptar.shape[0], NCOMP[k]))
+    lib().orc_derived_real(k, _ptr(psrc), _ptr(ptar), _ptr(f), None if n is None else _ptr(n),
+                           psrc.shape[0], ptar.shape[0], xi, nside_x, nside_y, Lx, Ly,
+                           _ptr(out))
+    return out
+
+
+def derived_kspace(kind, psrc, ptar, f, n, xi, eta, Mx, My, Lx, Ly, w, P):
+    """k-space mex restatement (no zero mode); returns (Nt, nc)."""
+    k = _kind(kind)
+    psrc, ptar, f, n = _c(psrc), _c(ptar), _c(f), _nullable(n)
+    out = np.zeros((ptar.shape[0], NCOMP[k]))
+    lib().orc_derived_kspace(k, _ptr(psrc), _ptr(ptar), _ptr(f),
+                             None if n is None else _ptr(n), xi, eta, psrc.shape[0],
+                             ptar.shape[0], Mx, My, Lx, Ly, w, P, _ptr(out))
+    return out
+
+
+def derived_real_ds(kind, psrc, ptar, f, n, b, Lx, Ly, xi):
+    """stokes_*_real_ds.m: (Nt, 2) (vorticity (Nt,)), gradient/stress projected on b."""
+    k = _kind(kind)
+    psrc, ptar, f, n, b = _c(psrc), _c(ptar), _c(f), _nullable(n), _nullable(b)
+    no = 1 if k in (6, 10) else 2
+    out = np.zeros((ptar.shape[0], no))
+    lib().orc_derived_real_ds(k, _ptr(psrc), _ptr(ptar), _ptr(f),
+                              None if n is None else _ptr(n), None if b is None else _ptr(b),
+                              psrc.shape[0], ptar.shape[0], Lx, Ly, xi, _ptr(out))
+    return out[:, 0] if no == 1 else out
+
+
+def derived_kspace_ds(kind, psrc, ptar, f, n, b, Lx, Ly, xi, kinf):
+    """stokes_*_kspace_ds.m (pair by pair: small cases only)."""
+    k = _kind(kind)
+    psrc, ptar, f, n, b = _c(psrc), _c(ptar), _c(f), _nullable(n), _nullable(b)
+    no = 1 if k in (6, 10) else 2
+    out = np.zeros((ptar.shape[0], no))
+    lib().orc_derived_kspace_ds(k, _ptr(psrc), _ptr(ptar), _ptr(f),
+                                None if n is None else _ptr(n), None if b is None else _ptr(b),
+                                psrc.shape[0], ptar.shape[0], Lx, Ly, xi, kinf, _ptr(out))
+    return out[:, 0] if no == 1 else out
 
 
 def params(kind, npts, Q, Lx, Ly, P, Nb, tol):

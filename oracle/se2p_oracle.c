@@ -12,6 +12,7 @@
  */
 #include "se2p_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1272,4 +1273,464 @@ void orc_dlp_pressure_kspace_ds(const double* psrc, const double* ptar, const do
         p[m] = acc / (Lx * Ly) + zm;
     }
     free(S);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Derived quantities: velocity gradient, pressure gradient, vorticity and   */
+/* stress of both layers.  Kind codes 4..11 (gradient, pressure gradient,    */
+/* vorticity, stress; Stokeslet then stresslet).  Tensors hold 4 values per  */
+/* target in the mex order Ts[4j..4j+3].                                     */
+/* ------------------------------------------------------------------------ */
+int orc_kind_ncomp(int kind)
+{
+    switch (kind) {
+    case 4: case 7: case 8: case 11: return 4;
+    case 5: case 9: return 2;
+    case 6: case 10: return 1;
+    default: return 0;
+    }
+}
+
+static int kind_is_dlp(int kind) { return kind >= 8; }
+
+/* Pair terms of the real-space mex files, r = target - source, before the
+ * output scaling; each line restates the file's own expression. */
+static void derived_pair_orc(int kind, double r1, double r2, double rr, double xi, double f1,
+                             double f2, double n1, double n2, double* T)
+{
+    const double xi2 = xi * xi;
+    const double e2 = exp(-xi2 * rr);
+    const double rdotf = r1 * f1 + r2 * f2;
+    switch (kind) {
+    case 4: /* mex_stokes_slp_gradient_real.cpp:118-132 */
+        T[0] += e2 * (2 * xi * xi * r1 * f1 + rdotf / rr - 2 * r1 * r1 * rdotf * (xi * xi + 1 / rr) / rr);
+        T[1] += e2 * (2 * xi * xi * r1 * f2 + (-r1 * f2 + r2 * f1) / rr - 2 * r1 * r2 * rdotf * (xi * xi + 1 / rr) / rr);
+        T[2] += e2 * (2 * xi * xi * r2 * f1 + (r1 * f2 - r2 * f1) / rr - 2 * r1 * r2 * rdotf * (xi * xi + 1 / rr) / rr);
+        T[3] += e2 * (2 * xi * xi * r2 * f2 + rdotf / rr - 2 * r2 * r2 * rdotf * (xi * xi + 1 / rr) / rr);
+        break;
+    case 5: /* mex_stokes_slp_pressure_grad_real.cpp:120-121 */
+        T[0] -= e2 * ((f1 - 2 * xi * xi * rdotf * r1) / rr - 2 * rdotf * r1 / rr / rr);
+        T[1] -= e2 * ((f2 - 2 * xi * xi * rdotf * r2) / rr - 2 * rdotf * r2 / rr / rr);
+        break;
+    case 6: /* mex_stokes_slp_vorticity_real.cpp:118-121 */
+        T[0] += e2 * (1 / rr - xi2) * (f1 * r2 - f2 * r1);
+        break;
+    case 7: /* mex_stokes_slp_stress_real.cpp:118-129 */
+        T[0] += e2 * (2 * xi2 * (rdotf + r1 * f1 + r1 * f1) - 4 * r1 * r1 * rdotf / (rr * rr) * (1 + xi2 * rr));
+        T[1] += e2 * (2 * xi2 * (r2 * f1 + r1 * f2) - 4 * r1 * r2 * rdotf / (rr * rr) * (1 + xi2 * rr));
+        T[2] += e2 * (2 * xi2 * (r1 * f2 + r2 * f1) - 4 * r2 * r1 * rdotf / (rr * rr) * (1 + xi2 * rr));
+        T[3] += e2 * (2 * xi2 * (rdotf + r2 * f2 + r2 * f2) - 4 * r2 * r2 * rdotf / (rr * rr) * (1 + xi2 * rr));
+        break;
+    default: {
+        const double rdotn = r1 * n1 + r2 * n2, fdotn = f1 * n1 + f2 * n2;
+        const double c6 = 8 * xi2 * xi2 / rr + 16 * xi2 / rr / rr + 16 / rr / rr / rr;
+        if (kind == 8) { /* mex_stokes_dlp_gradient_real.cpp:130-145 */
+            T[0] += e2 * (r1 * r1 * rdotf * rdotn * c6 - 4 * (1 + xi2 * rr) * (rdotf * rdotn + r1 * f1 * rdotn + r1 * n1 * rdotf) / rr / rr
+                          + 2 * xi2 * (2 * f1 * n1 + fdotn - 2 * xi2 * r1 * (f1 * rdotn + n1 * rdotf + r1 * fdotn)));
+            T[1] += e2 * (r1 * r2 * rdotf * rdotn * c6 - 4 * (1 + xi2 * rr) * (r2 * f1 * rdotn + r2 * n1 * rdotf) / rr / rr
+                          + 2 * xi2 * (f1 * n2 + n1 * f2 - 2 * xi2 * r1 * (f2 * rdotn + n2 * rdotf + r2 * fdotn)));
+            T[2] += e2 * (r1 * r2 * rdotf * rdotn * c6 - 4 * (1 + xi2 * rr) * (r1 * f2 * rdotn + r1 * n2 * rdotf) / rr / rr
+                          + 2 * xi2 * (f2 * n1 + f1 * n2 - 2 * xi2 * r2 * (f1 * rdotn + n1 * rdotf + r1 * fdotn)));
+            T[3] += e2 * (r2 * r2 * rdotf * rdotn * c6 - 4 * (1 + xi2 * rr) * (rdotf * rdotn + r2 * f2 * rdotn + r2 * n2 * rdotf) / rr / rr
+                          + 2 * xi2 * (2 * f2 * n2 + fdotn - 2 * xi2 * r2 * (f2 * rdotn + n2 * rdotf + r2 * fdotn)));
+        } else if (kind == 9) { /* mex_stokes_dlp_pressure_grad_real.cpp:122-128 */
+            T[0] += e2 * (2 * xi2 * (r1 * fdotn + f1 * rdotn + n1 * rdotf - 2 * xi2 * r1 * rdotf * rdotn) / rr
+                          + 2 * (r1 * fdotn + f1 * rdotn + n1 * rdotf - 4 * xi2 * r1 * rdotn * rdotf) / rr / rr
+                          - 8 * r1 * rdotn * rdotf / rr / rr / rr);
+            T[1] += e2 * (2 * xi2 * (r2 * fdotn + f2 * rdotn + n2 * rdotf - 2 * xi2 * r2 * rdotf * rdotn) / rr
+                          + 2 * (r2 * fdotn + f2 * rdotn + n2 * rdotf - 4 * xi2 * r2 * rdotn * rdotf) / rr / rr
+                          - 8 * r2 * rdotn * rdotf / rr / rr / rr);
+        } else if (kind == 10) { /* mex_stokes_dlp_vorticity_real.cpp:136-145 */
+            double rdot = r1 * (n2 * rdotf + f2 * rdotn) - r2 * (n1 * rdotf + f1 * rdotn);
+            double ndot = n1 * (r2 * rdotf) - n2 * (r1 * rdotf);
+            double fdot = f1 * (r2 * rdotn) - f2 * (r1 * rdotn);
+            T[0] += e2 * ((1 + xi2 * rr) * rdot / (rr * rr) + xi2 * xi2 * (ndot + fdot));
+        } else { /* mex_stokes_dlp_stress_real.cpp:132-151, mu = 1 */
+            const double mu = 1, pi = ORC_PI;
+            T[0] += e2 * (1 / (2 * pi) * ((fdotn - 2 * xi2 * rdotf * rdotn) / rr - 2 * rdotf * rdotn / (rr * rr))
+                          + 1 / (4 * pi) * 2 * mu * (r1 * r1 * rdotf * rdotn * c6
+                          - 2 * (1 + xi2 * rr) * (f1 * r1 * rdotn + n1 * r1 * rdotf + r1 * f1 * rdotn + r1 * n1 * rdotf + 2 * rdotf * rdotn) / (rr * rr)
+                          + 2 * xi2 * (fdotn + n1 * f1 + f1 * n1 - xi2 * (2 * r1 * r1 * fdotn + r1 * f1 * rdotn + r1 * n1 * rdotf + f1 * r1 * rdotn + n1 * r1 * rdotf))));
+            T[1] += 1 / (4 * pi) * 2 * mu * e2 * (r1 * r2 * rdotf * rdotn * c6
+                          - 2 * (1 + xi2 * rr) * (f1 * r2 * rdotn + n1 * r2 * rdotf + r1 * f2 * rdotn + r1 * n2 * rdotf) / (rr * rr)
+                          + 2 * xi2 * (n1 * f2 + f1 * n2 - xi2 * (2 * r1 * r2 * fdotn + r1 * f2 * rdotn + r1 * n2 * rdotf + f1 * r2 * rdotn + n1 * r2 * rdotf)));
+            T[2] += 1 / (4 * pi) * 2 * mu * e2 * (r2 * r1 * rdotf * rdotn * c6
+                          - 2 * (1 + xi2 * rr) * (f2 * r1 * rdotn + n2 * r1 * rdotf + r2 * f1 * rdotn + r2 * n1 * rdotf) / (rr * rr)
+                          + 2 * xi2 * (n2 * f1 + f2 * n1 - xi2 * (2 * r2 * r1 * fdotn + r2 * f1 * rdotn + r2 * n1 * rdotf + f2 * r1 * rdotn + n2 * r1 * rdotf)));
+            T[3] += e2 * (1 / (2 * pi) * ((fdotn - 2 * xi2 * rdotf * rdotn) / rr - 2 * rdotf * rdotn / (rr * rr))
+                          + 1 / (4 * pi) * 2 * mu * (r2 * r2 * rdotf * rdotn * c6
+                          - 2 * (1 + xi2 * rr) * (f2 * r2 * rdotn + n2 * r2 * rdotf + r2 * f2 * rdotn + r2 * n2 * rdotf + 2 * rdotf * rdotn) / (rr * rr)
+                          + 2 * xi2 * (fdotn + n2 * f2 + f2 * n2 - xi2 * (2 * r2 * r2 * fdotn + r2 * f2 * rdotn + r2 * n2 * rdotf + f2 * r2 * rdotn + n2 * r2 * rdotf))));
+        }
+    }
+    }
+}
+
+/* Output scaling at the end of each real-space mex (e.g. /4/pi at
+ * mex_stokes_slp_gradient_real.cpp:202-205, /(2*pi) at
+ * mex_stokes_slp_pressure_grad_real.cpp:173-174, /pi at
+ * mex_stokes_dlp_vorticity_real.cpp:209; the stresslet stress keeps the
+ * factors inside its pair term). */
+static double derived_out_scale(int kind)
+{
+    switch (kind) {
+    case 4: case 7: case 8: return 1 / (4 * ORC_PI);
+    case 5: case 6: case 9: return 1 / (2 * ORC_PI);
+    case 10: return 1 / ORC_PI;
+    default: return 1;
+    }
+}
+
+/* The real-space mex files of the derived quantities: the cells, stencil and
+ * cutoff of mex_stokes_dlp_real.cpp (own cell uncut, coincident points skipped;
+ * the stresslet vorticity skips own-cell pairs with r^2 < 1e-13,
+ * mex_stokes_dlp_vorticity_real.cpp:128).  out: nc x Nt. */
+void orc_derived_real(int kind, const double* psrc, const double* ptar, const double* f,
+                      const double* n, int Ns, int Nt, double xi, int nside_x, int nside_y,
+                      double Lx, double Ly, double* out)
+{
+    const int nc = orc_kind_ncomp(kind);
+    const int dlp = kind_is_dlp(kind);
+    cells_t c;
+    cells_build(&c, psrc, Ns, ptar, Nt, Lx, Ly, nside_x, nside_y);
+    const int nbox = nside_x * nside_y;
+    const double cut2 = Lx * Ly / nside_x / nside_y;
+    const double own_skip = kind == 10 ? 1e-13 : 0.0;
+    const double sc = derived_out_scale(kind);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int bx = b % nside_x, by = b / nside_x;
+        for (int it = c.bo_t[b]; it < c.bo_t[b + 1]; ++it) {
+            int t = c.po_t[it];
+            double xt = ptar[2 * t], yt = ptar[2 * t + 1];
+            double T[4] = {0, 0, 0, 0};
+            for (int is = c.bo_s[b]; is < c.bo_s[b + 1]; ++is) {
+                int s = c.po_s[is];
+                double r1 = xt - psrc[2 * s], r2 = yt - psrc[2 * s + 1];
+                double rr = r1 * r1 + r2 * r2;
+                if (kind == 10 ? rr < own_skip : rr == 0) continue;
+                derived_pair_orc(kind, r1, r2, rr, xi, f[2 * s], f[2 * s + 1],
+                                 dlp ? n[2 * s] : 0, dlp ? n[2 * s + 1] : 0, T);
+            }
+            for (int j = 0; j < 8; ++j) {
+                int qx = bx + nb_dx[j], qy = by + nb_dy[j];
+                int wx = (qx + nside_x) % nside_x, wy = (qy + nside_y) % nside_y;
+                int sb = wy * nside_x + wx;
+                double ox = (Lx * (qx - wx)) / nside_x;
+                double oy = (Ly * (qy - wy)) / nside_y;
+                for (int is = c.bo_s[sb]; is < c.bo_s[sb + 1]; ++is) {
+                    int s = c.po_s[is];
+                    double r1 = xt - (psrc[2 * s] + ox), r2 = yt - (psrc[2 * s + 1] + oy);
+                    double rr = r1 * r1 + r2 * r2;
+                    if (rr < cut2)
+                        derived_pair_orc(kind, r1, r2, rr, xi, f[2 * s], f[2 * s + 1],
+                                         dlp ? n[2 * s] : 0, dlp ? n[2 * s + 1] : 0, T);
+                }
+            }
+            for (int q = 0; q < nc; ++q) out[(size_t)nc * t + q] = T[q] * sc;
+        }
+    }
+    cells_free(&c);
+}
+
+/* The k-space filters of the derived-quantity mex files, on one mode.  in:
+ * the forward spectra as the mex names them (Stokeslet: Hhat1, Hhat2 = spread
+ * f1, f2; stresslet: Hhat1..4 = spread f1 n1, f2 n1, f1 n2, f2 n2, which the
+ * files call f1n1, f1n2, f2n1, f2n2 -- the middle two enter only through their
+ * sum).  out: the nc filtered spectra. */
+static void derived_filter_mode(int kind, double k1, double k2, double xi, double eta,
+                                const double complex* H, double complex* o)
+{
+    const double Ksq = k1 * k1 + k2 * k2;
+    const double xi2 = xi * xi;
+    const double complex I1 = I;
+    if (!kind_is_dlp(kind)) {
+        const double complex q1 = H[0], q2 = H[1];
+        const double complex kdotq = k1 * q1 + k2 * q2;
+        if (kind == 4) { /* mex_stokes_slp_gradient_kspace.cpp:143-172 */
+            double e = (1.0 / (Ksq * Ksq) + 0.25 / (Ksq * xi * xi)) * exp(-0.25 * (1 - eta) / (xi * xi) * Ksq);
+            o[0] = I1 * k1 * (Ksq * q1 - k1 * kdotq) * e;
+            o[1] = I1 * k1 * (Ksq * q2 - k2 * kdotq) * e;
+            o[2] = I1 * k2 * (Ksq * q1 - k1 * kdotq) * e;
+            o[3] = I1 * k2 * (Ksq * q2 - k2 * kdotq) * e;
+        } else if (kind == 5) { /* mex_stokes_slp_pressure_grad_kspace.cpp:126-135 */
+            double e = exp(-Ksq * (1 - eta) / (4 * xi * xi));
+            o[0] = -k1 * kdotq * e / Ksq;
+            o[1] = -k2 * kdotq * e / Ksq;
+        } else if (kind == 6) { /* mex_stokes_slp_vorticity_kspace.cpp:142-156 */
+            double e = (1.0 / Ksq + 0.25 / (xi * xi)) * exp(-0.25 * (1 - eta) / (xi * xi) * Ksq);
+            double complex fdotkperp = q1 * k2 - q2 * k1;
+            o[0] = -I1 * fdotkperp * e;
+        } else { /* mex_stokes_slp_stress_kspace.cpp:170-185 */
+            double e = exp(-0.25 * (1 - eta) / xi2 * Ksq) * (1.0 + Ksq / (4 * xi2)) / Ksq;
+            o[0] = I1 * (kdotq + q1 * k1 + q1 * k1 - 2 * (k1 * k1 * kdotq) / Ksq) * e;
+            o[1] = I1 * (q2 * k1 + q1 * k2 - 2 * (k2 * k1 * kdotq) / Ksq) * e;
+            o[2] = I1 * (q1 * k2 + q2 * k1 - 2 * (k1 * k2 * kdotq) / Ksq) * e;
+            o[3] = I1 * (kdotq + q2 * k2 + q2 * k2 - 2 * (k2 * k2 * kdotq) / Ksq) * e;
+        }
+        return;
+    }
+    const double complex f1n1 = H[0], f1n2 = H[1], f2n1 = H[2], f2n2 = H[3];
+    const double complex kqk = k1 * k1 * f1n1 + k1 * k2 * (f1n2 + f2n1) + k2 * k2 * f2n2;
+    if (kind == 8) { /* mex_stokes_dlp_gradient_kspace.cpp:167-206 */
+        double e = exp(-Ksq * (1 - eta) / (4 * xi * xi)) * (1 + Ksq / (4 * xi * xi)) / Ksq;
+        double complex v1 = 2 * f1n1 * k1 + k2 * (f1n2 + f2n1) + k1 * (f1n1 + f2n2) - 2 * k1 * kqk / Ksq;
+        double complex v2 = 2 * f2n2 * k2 + k1 * (f1n2 + f2n1) + k2 * (f1n1 + f2n2) - 2 * k2 * kqk / Ksq;
+        o[0] = -k1 * v1 * e;
+        o[1] = -k1 * v2 * e;
+        o[2] = -k2 * v1 * e;
+        o[3] = -k2 * v2 * e;
+    } else if (kind == 9) { /* mex_stokes_dlp_pressure_grad_kspace.cpp:170-173 */
+        double e = exp(-Ksq * (1 - eta) / (4 * xi * xi));
+        o[0] = -I1 * k1 * kqk * e / Ksq;
+        o[1] = -I1 * k2 * kqk * e / Ksq;
+    } else if (kind == 10) { /* mex_stokes_dlp_vorticity_kspace.cpp:160-169 */
+        double e = (1.0 / Ksq + 1 / (4 * xi2)) * exp(-Ksq * (1 - eta) / (4 * xi2));
+        double complex t1 = k1 * k2 * f1n1 + k2 * k2 * f1n2 - (k1 * k1 * f2n1 + k1 * k2 * f2n2);
+        double complex t2 = k1 * k2 * f1n1 + k2 * k2 * f2n1 - (k1 * k1 * f1n2 + k1 * k2 * f2n2);
+        o[0] = (t1 + t2) * e;
+    } else { /* mex_stokes_dlp_stress_kspace.cpp:166-201, mu = 1 */
+        const double mu = 1;
+        double e = exp(-Ksq * (1 - eta) / (4 * xi * xi));
+        double c = 1 / Ksq + 0.25 / (xi * xi);
+        double complex a1 = k1 * f1n1 + k2 * f1n2, a2 = k1 * f1n1 + k2 * f2n1;
+        double complex b1 = k1 * f2n1 + k2 * f2n2, b2 = k1 * f1n2 + k2 * f2n2;
+        o[0] = e * (kqk / Ksq - mu * c * (2 * k1 * k1 * (f1n1 + f2n2) + k1 * a1 + k1 * a2 + k1 * a1 + k1 * a2
+                                          - 4 * k1 * k1 * kqk / Ksq));
+        o[1] = -e * mu * c * (2 * k2 * k1 * (f1n1 + f2n2) + k2 * a1 + k2 * a2 + k1 * b1 + k1 * b2
+                              - 4 * k2 * k1 * kqk / Ksq);
+        o[2] = -e * mu * c * (2 * k1 * k2 * (f1n1 + f2n2) + k1 * b1 + k1 * b2 + k2 * a1 + k2 * a2
+                              - 4 * k1 * k2 * kqk / Ksq);
+        o[3] = e * (kqk / Ksq - mu * c * (2 * k2 * k2 * (f1n1 + f2n2) + k2 * b1 + k2 * b2 + k2 * b1 + k2 * b2
+                                          - 4 * k2 * k2 * kqk / Ksq));
+    }
+}
+
+/* The k-space mex files of the derived quantities: spread (f, or f n1 and
+ * f n2), forward DFT of each real grid, the filter on every mode with k = 0
+ * set to zero, inverse DFT of each output, gather of its real part into
+ * component c (Gather(Ht_c, nc, c, ...)).  No zero mode.  out: nc x Nt. */
+void orc_derived_kspace(int kind, const double* psrc, const double* ptar, const double* f,
+                        const double* n, double xi, double eta, int Ns, int Nt, int Mx, int My,
+                        double Lx, double Ly, double w, int P, double* out)
+{
+    const int nc = orc_kind_ncomp(kind);
+    const int nin = kind_is_dlp(kind) ? 4 : 2;
+    const size_t G = (size_t)Mx * My;
+    double* hin = (double*)calloc(2 * nin * G, sizeof(double));
+    double* hout = (double*)calloc(2 * nc * G, sizeof(double));
+    if (kind_is_dlp(kind)) {
+        double* v1 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+        double* v2 = (double*)malloc(sizeof(double) * 2 * (size_t)(Ns > 0 ? Ns : 1));
+        for (int i = 0; i < Ns; ++i) {
+            v1[2 * i] = f[2 * i] * n[2 * i];
+            v1[2 * i + 1] = f[2 * i + 1] * n[2 * i];
+            v2[2 * i] = f[2 * i] * n[2 * i + 1];
+            v2[2 * i + 1] = f[2 * i + 1] * n[2 * i + 1];
+        }
+        orc_spread(psrc, v1, Ns, Lx, Ly, xi, w, eta, P, Mx, My, hin, hin + 2 * G);
+        orc_spread(psrc, v2, Ns, Lx, Ly, xi, w, eta, P, Mx, My, hin + 4 * G, hin + 6 * G);
+        free(v1);
+        free(v2);
+    } else {
+        orc_spread(psrc, f, Ns, Lx, Ly, xi, w, eta, P, Mx, My, hin, hin + 2 * G);
+    }
+    for (int q = 0; q < nin; ++q) orc_fft2(hin + 2 * q * G, hin + (2 * q + 1) * G, My, Mx, 0);
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < Mx; ++ix) {
+        double k1 = wavenumber(ix, Mx, Lx);
+        for (int iy = 0; iy < My; ++iy) {
+            size_t o = (size_t)ix * My + iy;
+            if (ix == 0 && iy == 0) continue; /* zero frequency removed */
+            double k2 = wavenumber(iy, My, Ly);
+            double complex H[4], out4[4];
+            for (int q = 0; q < nin; ++q) H[q] = hin[2 * q * G + o] + I * hin[(2 * q + 1) * G + o];
+            derived_filter_mode(kind, k1, k2, xi, eta, H, out4);
+            for (int q = 0; q < nc; ++q) {
+                hout[2 * q * G + o] = creal(out4[q]);
+                hout[(2 * q + 1) * G + o] = cimag(out4[q]);
+            }
+        }
+    }
+    for (int m = 0; m < nc * Nt; ++m) out[m] = 0;
+    for (int q = 0; q < nc; ++q) {
+        orc_fft2(hout + 2 * q * G, hout + (2 * q + 1) * G, My, Mx, 1);
+        orc_gather(hout + 2 * q * G, nc, q, ptar, Nt, Lx, Ly, xi, w, eta, P, Mx, My, out);
+    }
+    free(hin);
+    free(hout);
+}
+
+/* Real-space direct sums, tests/direct_sums/matlab_sums/stokes_*_real_ds.m:
+ * all 3 x 3 images, |r| < 1e-13 skipped; gradient and stress projected on the
+ * per-target vector b (2 x Nt), the others as they are.  out: 2 x Nt (1 x Nt
+ * for the vorticity). */
+void orc_derived_real_ds(int kind, const double* psrc, const double* ptar, const double* f,
+                         const double* n, const double* b, int Ns, int Nt, double Lx, double Ly,
+                         double xi, double* out)
+{
+    const double xi2 = xi * xi, pi = ORC_PI;
+    const int no = kind == 6 || kind == 10 ? 1 : 2;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double acc[2] = {0, 0};
+        const double b1 = b ? b[2 * m] : 0, b2 = b ? b[2 * m + 1] : 0;
+        for (int s = 0; s < Ns; ++s)
+            for (int jx = -1; jx <= 1; ++jx)
+                for (int jy = -1; jy <= 1; ++jy) {
+                    double r1 = ptar[2 * m] - (psrc[2 * s] + jx * Lx);
+                    double r2 = ptar[2 * m + 1] - (psrc[2 * s + 1] + jy * Ly);
+                    double rr = r1 * r1 + r2 * r2, r = sqrt(rr);
+                    if (r < 1e-13) continue;
+                    double f1 = f[2 * s], f2 = f[2 * s + 1];
+                    double n1 = n ? n[2 * s] : 0, n2 = n ? n[2 * s + 1] : 0;
+                    double rdotf = r1 * f1 + r2 * f2, rdotb = r1 * b1 + r2 * b2;
+                    double bdotf = f1 * b1 + f2 * b2, e = exp(-xi2 * rr);
+                    double rdotn = r1 * n1 + r2 * n2, fdotn = n1 * f1 + n2 * f2;
+                    double bdotn = n1 * b1 + n2 * b2;
+                    switch (kind) {
+                    case 4: /* stokes_slp_gradient_real_ds.m:59-75 */
+                        acc[0] += e * (2 * xi2 * rdotb * f1 + (b1 * rdotf + r1 * bdotf - f1 * rdotb) / rr
+                                       - 2 * r1 * rdotb * rdotf * (xi2 + 1 / rr) / rr);
+                        acc[1] += e * (2 * xi2 * rdotb * f2 + (b2 * rdotf + r2 * bdotf - f2 * rdotb) / rr
+                                       - 2 * r2 * rdotb * rdotf * (xi2 + 1 / rr) / rr);
+                        break;
+                    case 5: /* stokes_slp_pressure_grad_real_ds.m:57-64 */
+                        acc[0] += e * ((f1 - r1 * 2 * xi2 * rdotf) / rr - 2 * r1 * rdotf / (rr * rr));
+                        acc[1] += e * ((f2 - r2 * 2 * xi2 * rdotf) / rr - 2 * r2 * rdotf / (rr * rr));
+                        break;
+                    case 6: /* stokes_slp_vorticity_real_ds.m:55-63 */
+                        acc[0] += e * (1 / rr - xi2) * (f1 * r2 - f2 * r1) / (2 * pi);
+                        break;
+                    case 7: /* stokes_slp_stress_real_ds.m:61-78 */
+                        acc[0] += e * (2 * xi2 * (b1 * rdotf + f1 * rdotb + r1 * bdotf)
+                                       - (4 * r1 * rdotb * rdotf / (rr * rr)) * (1 + xi2 * rr));
+                        acc[1] += e * (2 * xi2 * (b2 * rdotf + f2 * rdotb + r2 * bdotf)
+                                       - (4 * r2 * rdotb * rdotf / (rr * rr)) * (1 + xi2 * rr));
+                        break;
+                    case 8: { /* stokes_dlp_gradient_real_ds.m:61-82 */
+                        double c6 = 8 * xi2 * xi2 / rr + 16 * xi2 / (rr * rr) + 16 / (rr * rr * rr);
+                        acc[0] += e * (r1 * rdotf * rdotn * rdotb * c6
+                                       - 4 * (1 + xi2 * rr) * (b1 * rdotf * rdotn + r1 * bdotf * rdotn + r1 * bdotn * rdotf) / (rr * rr)
+                                       + 2 * xi2 * (f1 * bdotn + n1 * bdotf + b1 * fdotn - 2 * xi2 * rdotb * (f1 * rdotn + n1 * rdotf + r1 * fdotn)));
+                        acc[1] += e * (r2 * rdotf * rdotn * rdotb * c6
+                                       - 4 * (1 + xi2 * rr) * (b2 * rdotf * rdotn + r2 * bdotf * rdotn + r2 * bdotn * rdotf) / (rr * rr)
+                                       + 2 * xi2 * (f2 * bdotn + n2 * bdotf + b2 * fdotn - 2 * xi2 * rdotb * (f2 * rdotn + n2 * rdotf + r2 * fdotn)));
+                        break;
+                    }
+                    case 9: /* stokes_dlp_pressure_grad_real_ds.m:63-76 */
+                        acc[0] += e * (2 * xi2 * (r1 * fdotn + f1 * rdotn + n1 * rdotf - 2 * xi2 * r1 * rdotn * rdotf) / rr
+                                       + 2 * (-4 * xi2 * r1 * rdotn * rdotf + r1 * fdotn + f1 * rdotn + n1 * rdotf) / (rr * rr)
+                                       - 8 * r1 * rdotn * rdotf / (rr * rr * rr));
+                        acc[1] += e * (2 * xi2 * (r2 * fdotn + f2 * rdotn + n2 * rdotf - 2 * xi2 * r2 * rdotn * rdotf) / rr
+                                       + 2 * (-4 * xi2 * r2 * rdotn * rdotf + r2 * fdotn + f2 * rdotn + n2 * rdotf) / (rr * rr)
+                                       - 8 * r2 * rdotn * rdotf / (rr * rr * rr));
+                        break;
+                    case 10: { /* stokes_dlp_vorticity_real_ds.m:57-86 */
+                        double rdot = r1 * (n2 * rdotf + f2 * rdotn) - r2 * (n1 * rdotf + f1 * rdotn);
+                        double ndot = n1 * (r2 * rdotf) - n2 * (r1 * rdotf);
+                        double fdot = f1 * (r2 * rdotn) - f2 * (r1 * rdotn);
+                        acc[0] += e * ((1 + xi2 * rr) * rdot / (rr * rr) + xi2 * xi2 * (ndot + fdot)) / pi;
+                        break;
+                    }
+                    default: { /* stokes_dlp_stress_real_ds.m:58-84, mu = 1 */
+                        double fdotb = bdotf, mu = 1;
+                        double c6 = (8 * xi2 * xi2) / rr + (16 * xi2) / (rr * rr) + 16 / (rr * rr * rr);
+                        double ptmp = (fdotn - 2 * xi2 * rdotf * rdotn) / rr - 2 * rdotf * rdotn / (rr * rr);
+                        acc[0] += e * (b1 / (2 * pi) * ptmp + 1 / (4 * pi) * 2 * mu * (r1 * rdotb * rdotf * rdotn * c6
+                                  - 2 * (1 + xi2 * rr) * (f1 * rdotb * rdotn + n1 * rdotb * rdotf + r1 * fdotb * rdotn + r1 * bdotn * rdotf + 2 * b1 * rdotf * rdotn) / (rr * rr)
+                                  + 2 * xi2 * (b1 * fdotn + n1 * fdotb + f1 * bdotn - xi2 * (2 * r1 * rdotb * fdotn + r1 * rdotn * fdotb + r1 * rdotf * bdotn + f1 * rdotb * rdotn + n1 * rdotb * rdotf))));
+                        acc[1] += e * (b2 / (2 * pi) * ptmp + 1 / (4 * pi) * 2 * mu * (r2 * rdotb * rdotf * rdotn * c6
+                                  - 2 * (1 + xi2 * rr) * (f2 * rdotb * rdotn + n2 * rdotb * rdotf + r2 * fdotb * rdotn + r2 * bdotn * rdotf + 2 * b2 * rdotf * rdotn) / (rr * rr)
+                                  + 2 * xi2 * (b2 * fdotn + n2 * fdotb + f2 * bdotn - xi2 * (2 * r2 * rdotb * fdotn + r2 * rdotn * fdotb + r2 * rdotf * bdotn + f2 * rdotb * rdotn + n2 * rdotb * rdotf))));
+                    }
+                    }
+                }
+        double sc = kind == 4 || kind == 7 || kind == 8 ? 1 / (4 * pi)
+                    : kind == 5 ? -1 / (2 * pi) : kind == 9 ? 1 / (2 * pi) : 1.0;
+        for (int q = 0; q < no; ++q) out[no * m + q] = acc[q] * sc;
+    }
+}
+
+/* k-space direct sums, stokes_*_kspace_ds.m: modes |j_i| <= kinf (j from
+ * -kinf+1 to kinf), k = 0 skipped, real part / (Lx Ly); the stresslet stress
+ * adds sum(n.f)/(2 Lx Ly) to both values (stokes_dlp_stress_kspace_ds.m:67).
+ * Evaluated pair by pair (Ns Nt (2 kinf)^2 terms): small cases only. */
+void orc_derived_kspace_ds(int kind, const double* psrc, const double* ptar, const double* f,
+                           const double* n, const double* b, int Ns, int Nt, double Lx,
+                           double Ly, double xi, int kinf, double* out)
+{
+    const double xi2 = xi * xi;
+    const int no = kind == 6 || kind == 10 ? 1 : 2;
+    double zm = 0;
+    if (kind == 11)
+        for (int s = 0; s < Ns; ++s) zm += n[2 * s] * f[2 * s] + n[2 * s + 1] * f[2 * s + 1];
+    zm /= 2 * Lx * Ly;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < Nt; ++m) {
+        double complex acc[2] = {0, 0};
+        const double b1 = b ? b[2 * m] : 0, b2 = b ? b[2 * m + 1] : 0;
+        for (int s = 0; s < Ns; ++s) {
+            double r1 = ptar[2 * m] - psrc[2 * s], r2 = ptar[2 * m + 1] - psrc[2 * s + 1];
+            double f1 = f[2 * s], f2 = f[2 * s + 1];
+            double n1 = n ? n[2 * s] : 0, n2 = n ? n[2 * s + 1] : 0;
+            for (int j1 = -kinf + 1; j1 <= kinf; ++j1)
+                for (int j2 = -kinf + 1; j2 <= kinf; ++j2) {
+                    if (j1 == 0 && j2 == 0) continue;
+                    double k1 = 2 * ORC_PI / Lx * j1, k2 = 2 * ORC_PI / Ly * j2;
+                    double kSq = k1 * k1 + k2 * k2;
+                    double kdotf = k1 * f1 + k2 * f2, kdotb = k1 * b1 + k2 * b2;
+                    double kdotn = k1 * n1 + k2 * n2, fdotn = f1 * n1 + f2 * n2;
+                    double bdotf = f1 * b1 + f2 * b2, ndotb = n1 * b1 + n2 * b2;
+                    double g = exp(-kSq / (4 * xi2));
+                    double complex t[2] = {0, 0};
+                    switch (kind) {
+                    case 4: /* stokes_slp_gradient_kspace_ds.m:66-81 */
+                        t[0] = I * (f1 * kdotb / kSq - k1 * kdotb * kdotf / (kSq * kSq)) * (1 + kSq / (4 * xi2)) * g;
+                        t[1] = I * (f2 * kdotb / kSq - k2 * kdotb * kdotf / (kSq * kSq)) * (1 + kSq / (4 * xi2)) * g;
+                        break;
+                    case 5: /* stokes_slp_pressure_grad_kspace_ds.m:64-72 */
+                        t[0] = -k1 * kdotf * g / kSq;
+                        t[1] = -k2 * kdotf * g / kSq;
+                        break;
+                    case 6: /* stokes_slp_vorticity_kspace_ds.m:64-70 */
+                        t[0] = -I * (f1 * k2 - f2 * k1) * (1 / kSq + 1 / (4 * xi2)) * g;
+                        break;
+                    case 7: { /* stokes_slp_stress_kspace_ds.m:66-86 */
+                        double e = g * (1 + kSq / (4 * xi2)) / kSq;
+                        t[0] = I * (b1 * kdotf + f1 * kdotb + k1 * bdotf - 2 * (k1 * kdotb * kdotf) / kSq) * e;
+                        t[1] = I * (b2 * kdotf + f2 * kdotb + k2 * bdotf - 2 * (k2 * kdotb * kdotf) / kSq) * e;
+                        break;
+                    }
+                    case 8: /* stokes_dlp_gradient_kspace_ds.m:76-92 */
+                        t[0] = -kdotb * (f1 * kdotn + n1 * kdotf + k1 * fdotn - 2 * k1 * kdotn * kdotf / kSq) * (1 + kSq / (4 * xi2)) * g / kSq;
+                        t[1] = -kdotb * (f2 * kdotn + n2 * kdotf + k2 * fdotn - 2 * k2 * kdotn * kdotf / kSq) * (1 + kSq / (4 * xi2)) * g / kSq;
+                        break;
+                    case 9: /* stokes_dlp_pressure_grad_kspace_ds.m:69-81 */
+                        t[0] = -I * k1 * kdotn * kdotf * g / kSq;
+                        t[1] = -I * k2 * kdotn * kdotf * g / kSq;
+                        break;
+                    case 10: { /* stokes_dlp_vorticity_kspace_ds.m:67-95 */
+                        double fdot = f1 * k2 * kdotn - f2 * k1 * kdotn;
+                        double ndot = n1 * k2 * kdotf - n2 * k1 * kdotf;
+                        t[0] = (fdot + ndot) * (1 / kSq + 1 / (4 * xi2)) * g;
+                        break;
+                    }
+                    default: { /* stokes_dlp_stress_kspace_ds.m:71-97, mu = 1 */
+                        double mu = 1, fdotb = bdotf;
+                        t[0] = g * (b1 * kdotf * kdotn / kSq - mu * (1 / kSq + 1 / (4 * xi2)) * (2 * k1 * kdotb * fdotn
+                                    + k1 * kdotn * fdotb + k1 * kdotf * ndotb + f1 * kdotb * kdotn + n1 * kdotb * kdotf
+                                    - 4 * k1 * kdotb * kdotf * kdotn / kSq));
+                        t[1] = g * (b2 * kdotf * kdotn / kSq - mu * (1 / kSq + 1 / (4 * xi2)) * (2 * k2 * kdotb * fdotn
+                                    + k2 * kdotn * fdotb + k2 * kdotf * ndotb + f2 * kdotb * kdotn + n2 * kdotb * kdotf
+                                    - 4 * k2 * kdotb * kdotf * kdotn / kSq));
+                    }
+                    }
+                    double complex ph = cexp(I * (k1 * r1 + k2 * r2));
+                    acc[0] += t[0] * ph;
+                    acc[1] += t[1] * ph;
+                }
+        }
+        for (int q = 0; q < no; ++q) out[no * m + q] = creal(acc[q]) / (Lx * Ly) + zm;
+    }
 }

@@ -135,6 +135,27 @@ void orc_dlp_pressure_kspace_ds(const double* psrc, const double* ptar, const do
                                 const double* n, int Ns, int Nt, double Lx, double Ly,
                                 double xi, int kinf, double* p);
 
+/* Derived quantities, kinds 4..11: velocity gradient, pressure gradient,
+ * vorticity, stress of the Stokeslet, then of the stresslet (mex_stokes_{slp,dlp}_
+ * {gradient,pressure_grad,vorticity,stress}_{real,kspace}.cpp and the direct
+ * sums stokes_*_{real,kspace}_ds.m).  out: nc x Nt with nc = orc_kind_ncomp;
+ * n is ignored for Stokeslet kinds.  The direct sums project the gradient and
+ * the stress on the per-target vector b (2 x Nt) and write 2 x Nt (1 x Nt for
+ * the vorticity). */
+int orc_kind_ncomp(int kind);
+void orc_derived_real(int kind, const double* psrc, const double* ptar, const double* f,
+                      const double* n, int Ns, int Nt, double xi, int nside_x, int nside_y,
+                      double Lx, double Ly, double* out);
+void orc_derived_kspace(int kind, const double* psrc, const double* ptar, const double* f,
+                        const double* n, double xi, double eta, int Ns, int Nt, int Mx, int My,
+                        double Lx, double Ly, double w, int P, double* out);
+void orc_derived_real_ds(int kind, const double* psrc, const double* ptar, const double* f,
+                         const double* n, const double* b, int Ns, int Nt, double Lx, double Ly,
+                         double xi, double* out);
+void orc_derived_kspace_ds(int kind, const double* psrc, const double* ptar, const double* f,
+                           const double* n, const double* b, int Ns, int Nt, double Lx,
+                           double Ly, double xi, int kinf, double* out);
+
 #ifdef __cplusplus
 }
 #endif
