@@ -81,7 +81,7 @@ __device__ __forceinline__ void slp_pair(double dx, double dy, double r2, double
 {
     const double ir2 = rcp64(r2);
     const double x = xi2 * r2;
-    const double e = exp(-x);
+    const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
     const double E = NEAR ? e1_near<TIER>(x) : e1_far<TIER>(e, ir2 * ixi2);
     const double c = e * ir2;
     const double cr = dx * f2 - dy * f1;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "e1_coeffs.h"
+#include "exp_coeffs.h"
 
 namespace se2p {
 
@@ -44,6 +45,31 @@ struct E1Coeffs<kE1Fast> {
     static constexpr double farp[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARP;
     static constexpr double farq[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARQ;
 };
+
+// e^{-x} for 0 <= x < 700: Cody-Waite reduction r = n ln 2 - x with n =
+// rint(x log2 e), |r| <= ln(2)/2, e^r by the degree-10 Chebyshev-interpolated
+// polynomial of exp_coeffs.h (max rel. error 3e-16 before rounding), scaled by
+// 2^{-n}.  No special-case handling: the callers pass finite x in range.
+// a * b + c as one three-operand v_fma_f64.  The compiler otherwise turns each
+// Horner step on coefficients held in vector registers into a copy plus a
+// two-operand v_fmac_f64, twice the instructions.
+__device__ __forceinline__ double fma3(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+__device__ __forceinline__ double exp_neg(double x)
+{
+    constexpr double c[SE2P_EXP_DEG + 1] = SE2P_EXP_POLY;
+    const double n = __builtin_rint(x * SE2P_LOG2E);
+    const double r = fma(n, SE2P_LN2_LO, fma(n, SE2P_LN2_HI, -x));
+    double p = c[SE2P_EXP_DEG];
+#pragma unroll
+    for (int i = SE2P_EXP_DEG - 1; i >= 0; --i) p = fma3(p, r, c[i]);
+    return ldexp(p, -(int)n);
+}
 
 // 1/x from the hardware reciprocal estimate and two Newton steps (about 1 ulp).
 __device__ __forceinline__ double rcp64(double x)
