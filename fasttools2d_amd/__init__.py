@@ -10,6 +10,7 @@ from ._lib import (Context, Se2pError, default_context, device_count, LIB_PATH, 
                    SE2P_DLP_GRADIENT, SE2P_DLP_PRESSURE_GRAD, SE2P_DLP_VORTICITY,
                    SE2P_DLP_STRESS)
 from . import periodic_ewald as _pe
+from .freespace import stokesDLPfmm, stokesSLPfmm  # noqa: F401
 from .periodic_ewald import (StokesDLP_ewald_2p, StokesDLP_pressure_ewald_2p,  # noqa: F401
                              StokesSLP_ewald_2p, StokesSLP_pressure_ewald_2p, ewald_params,
                              mex_stokes_dlp_kspace, mex_stokes_dlp_pressure_kspace,
@@ -36,4 +37,5 @@ for _layer in ("slp", "dlp"):
 for _k in ("SLP_GRADIENT", "SLP_PRESSURE_GRAD", "SLP_VORTICITY", "SLP_STRESS", "DLP_GRADIENT",
            "DLP_PRESSURE_GRAD", "DLP_VORTICITY", "DLP_STRESS"):
     __all__.append("SE2P_" + _k)
+__all__ += ["stokesSLPfmm", "stokesDLPfmm"]
 del _layer, _q, _name, _k

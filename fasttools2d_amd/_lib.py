@@ -77,7 +77,7 @@ EXPORTS = (
     "se2p_dlp_pressure_real", "se2p_dlp_pressure_kspace", "se2p_dlp_pressure_ewald",
 ) + tuple(f"se2p_{layer}_{q}_{part}" for layer in ("slp", "dlp")
           for q in ("gradient", "pressure_grad", "vorticity", "stress")
-          for part in ("real", "kspace"))
+          for part in ("real", "kspace")) + ("se2p_free_slp", "se2p_free_dlp")
 
 
 class Se2pError(RuntimeError):
@@ -187,6 +187,8 @@ def lib():
                for q in ("gradient", "stress")},
             **{f"se2p_dlp_{q}_kspace": [P, P, P, P, P, D, D, I, I, I, I, D, D, D, I, P]
                for q in ("pressure_grad", "vorticity")},
+            "se2p_free_slp": [P, P, P, P, P, I, P, P, I, I, I, P, P],
+            "se2p_free_dlp": [P, P, P, P, P, P, P, I, P, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

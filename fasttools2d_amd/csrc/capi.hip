@@ -649,6 +649,49 @@ int derived_kspace(se2p_ctx* ctx, int kind, const double* psrc, const double* pt
                     k_only(xi, eta, Mx, My, w, P), out);
 }
 
+int free_eval(se2p_ctx* ctx, bool dlp, const double* f1, const double* f2, const double* xs,
+              const double* ys, const double* n1, const double* n2, int Ns, const double* xt,
+              const double* yt, int Nt, bool self, double* u1, double* u2)
+{
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Nt == 0) return SE2P_OK;
+    const int rec = dlp ? 6 : 4;
+    std::vector<double> hs((size_t)rec * std::max(Ns, 1)), ht(2 * (size_t)Nt);
+    for (int i = 0; i < Ns; ++i) {
+        double* r = hs.data() + (size_t)rec * i;
+        r[0] = xs[i];
+        r[1] = ys[i];
+        r[2] = f1[i];
+        r[3] = f2[i];
+        if (dlp) {
+            r[4] = n1[i];
+            r[5] = n2[i];
+        }
+    }
+    for (int i = 0; i < Nt; ++i) {
+        ht[2 * i] = xt[i];
+        ht[2 * i + 1] = yt[i];
+    }
+    double *ds, *dt;
+    SE2P_TRY(upload(ctx, WS_PTS_A, hs.data(), (size_t)rec * Ns, &ds));
+    SE2P_TRY(upload(ctx, WS_PTS_B, ht.data(), ht.size(), &dt));
+    const int nslice = free_slices(Ns, Nt);
+    void *out, *part = nullptr;
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &out));
+    if (nslice > 1)
+        SE2P_TRY(ws_get(ctx, &ctx->ws, WS_FREE_PART, sizeof(double) * 2 * (size_t)Nt * nslice,
+                        &part));
+    SE2P_TRY(launch_free(ctx->stream, dlp, ds, Ns, dt, Nt, self, nslice, (double*)part,
+                         (double*)out));
+    std::vector<double> h(2 * (size_t)Nt);
+    SE2P_TRY(download(ctx, h.data(), (double*)out, h.size()));
+    for (int i = 0; i < Nt; ++i) {
+        u1[i] = h[2 * i];
+        u2[i] = h[2 * i + 1];
+    }
+    return SE2P_OK;
+}
+
 }  // namespace
 }  // namespace se2p
 
@@ -928,6 +971,51 @@ int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar
 {
     return derived_kspace(ctx, SE2P_DLP_STRESS, psrc, ptar, f, n, xi, eta, Nsrc, Ntar, Mx,
                                 My, Lx, Ly, w, P, out);
+}
+
+// ---------------------------------------------------------------------------
+// Free-space layer potentials (the FMM mex functions, computed directly)
+// ---------------------------------------------------------------------------
+
+int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
+                  const double* ysrc, int Nsrc, const double* xtar, const double* ytar, int Ntar,
+                  int iequal, int iprec, double* u1, double* u2)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (Nsrc < 0 || Ntar < 0) return set_error(SE2P_ERR_ARG, "negative size");
+    if (iequal != 0 && iequal != 1) return set_error(SE2P_ERR_ARG, "iequal must be 0 or 1");
+    if (iprec < -2 || iprec > 5) return set_error(SE2P_ERR_ARG, "iprec must be in -2..5");
+    if (iequal && Ntar != Nsrc)
+        return set_error(SE2P_ERR_ARG, "iequal = 1 needs as many targets as sources");
+    for (const double* p : {f1, f2, xsrc, ysrc})
+        if (Nsrc > 0 && !p) return set_error(SE2P_ERR_ARG, "null source array");
+    for (const double* p : {(const double*)u1, (const double*)u2})
+        if (Ntar > 0 && !p) return set_error(SE2P_ERR_ARG, "null output array");
+    // with iequal = 1 the targets are the sources (stokesSLP.f: rfmm2dpart at the sources)
+    const double* xt = iequal ? xsrc : xtar;
+    const double* yt = iequal ? ysrc : ytar;
+    if (Ntar > 0 && (!xt || !yt)) return set_error(SE2P_ERR_ARG, "null target array");
+    return free_eval(ctx, false, f1, f2, xsrc, ysrc, nullptr, nullptr, Nsrc, xt, yt, Ntar,
+                     iequal == 1, u1, u2);
+}
+
+int se2p_free_dlp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
+                  const double* ysrc, const double* n1, const double* n2, int N, double* u1,
+                  double* u2)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (N < 0) return set_error(SE2P_ERR_ARG, "negative size");
+    for (const double* p : {f1, f2, xsrc, ysrc, n1, n2})
+        if (N > 0 && !p) return set_error(SE2P_ERR_ARG, "null array");
+    if (N > 0 && (!u1 || !u2)) return set_error(SE2P_ERR_ARG, "null output array");
+    // stokesDLP.f:31: a density with sum |f1| + |f2| <= 1e-14 gives zero
+    double zc = 0;
+    for (int i = 0; i < N; ++i) zc += std::fabs(f1[i]) + std::fabs(f2[i]);
+    if (zc <= 1e-14) {
+        for (int i = 0; i < N; ++i) u1[i] = u2[i] = 0;
+        return SE2P_OK;
+    }
+    return free_eval(ctx, true, f1, f2, xsrc, ysrc, n1, n2, N, xsrc, ysrc, N, true, u1, u2);
 }
 
 int se2p_dlp_pressure_ewald(se2p_ctx* ctx, const double* xsrc, const double* ysrc,

@@ -88,7 +88,7 @@ enum Slot {
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_PARTIAL, WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
-    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS,
+    WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
     WS_NUM_SLOTS
 };
 
@@ -260,6 +260,12 @@ int launch_gather(hipStream_t s, int maxR, int g, const SysDesc* sys, const Job*
 // (s, b), b < zm_nblk, then one job per system.
 int launch_zero_mode(hipStream_t s, const SysDesc* sys, const Job* part_jobs, int npart,
                      const Job* sys_jobs, int nsys);
+// Free-space direct sums (SLP or DLP records, see freespace.hip) of Ns sources
+// at Nt targets, split into nslice source slices (free_slices) whose partial
+// sums (part: nslice x 2 Nt doubles when nslice > 1) are added in order.
+int free_slices(int Ns, int Nt);
+int launch_free(hipStream_t s, bool dlp, const double* src, int Ns, const double* tar, int Nt,
+                bool self, int nslice, double* part, double* out);
 // Density fill of the sorted source records (layout: rec_layout).
 int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* jobs, int njobs,
                      const double* f);

@@ -252,6 +252,29 @@ int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar
                            double eta, const double* f, const double* n, int Nsrc, int Ntar,
                            int Mx, int My, double Lx, double Ly, double w, int P, double* out);
 
+/* ------------------------------------------------------------------------ */
+/* Free-space (non-periodic) layer potentials.  The reference evaluates them */
+/* with a Fortran FMM; these compute the same sums directly on the GPU in    */
+/* FP64 (exact to rounding, O(Nsrc Ntar) work).                              */
+/* ------------------------------------------------------------------------ */
+/* [u1, u2] = stokesSLPfmm(f1, f2, xsrc, ysrc, xtar, ytar, iequal, iprec)
+ * (FMM/src/StokesSLP/stokesSLPfmm.F:10-149, stokesSLP.f): u = sum_s (-1/2
+ * log(rho^2) f + (r.f) r/rho^2)/(4 pi), r = x - x_s (the direct sum of
+ * FMM/tests/stokes_SLP_FMM_test.m:29-51).  iequal = 1: the targets are the
+ * sources (Ntar == Nsrc; xtar, ytar may be NULL) and each source's own term is
+ * left out.  iprec (-2..5) is the FMM's accuracy level, accepted and not
+ * needed.  Coincident source-target pairs are skipped. */
+int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
+                  const double* ysrc, int Nsrc, const double* xtar, const double* ytar, int Ntar,
+                  int iequal, int iprec, double* u1, double* u2);
+/* [u1, u2] = stokesDLPfmm(f1, f2, xsrc, ysrc, n1, n2) (FMM/src/StokesDLP/
+ * stokesDLPfmm.F, stokesDLP.f): at the N sources, u = sum_{s != t} (r.n)(r.f)
+ * r/rho^4 / pi (FMM/tests/stokes_DLP_FMM_test.m:17-35); zero when sum |f1| +
+ * |f2| <= 1e-14 (stokesDLP.f:31). */
+int se2p_free_dlp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
+                  const double* ysrc, const double* n1, const double* n2, int N, double* u1,
+                  double* u2);
+
 void se2p_options_default(se2p_options* opt);
 
 /* Parameter recipe of StokesSLP_ewald_2p.m:27-37,84-104,147-202 (kind SE2P_SLP)

@@ -71,6 +71,8 @@ def lib():
                                    _P],
             "orc_derived_real_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
             "orc_derived_kspace_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+            "orc_free_slp": [_P, _P, _I, _P, _I, _I, _P],
+            "orc_free_dlp": [_P, _P, _P, _I, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -385,6 +387,23 @@ def derived_kspace_ds(kind, psrc, ptar, f, n, b, Lx, Ly, xi, kinf):
                                 None if n is None else _ptr(n), None if b is None else _ptr(b),
                                 psrc.shape[0], ptar.shape[0], Lx, Ly, xi, kinf, _ptr(out))
     return out[:, 0] if no == 1 else out
+
+
+def free_slp(f, psrc, ptar, self=False):
+    """Free-space SLP direct sum (stokes_SLP_FMM_test.m:29-51); (Nt, 2)."""
+    f, psrc, ptar = _c(f), _c(psrc), _c(ptar)
+    u = np.zeros((ptar.shape[0], 2))
+    lib().orc_free_slp(_ptr(f), _ptr(psrc), psrc.shape[0], _ptr(ptar), ptar.shape[0],
+                       int(bool(self)), _ptr(u))
+    return u
+
+
+def free_dlp(f, n, p):
+    """Free-space DLP direct sum at the sources (stokes_DLP_FMM_test.m:17-35); (N, 2)."""
+    f, n, p = _c(f), _c(n), _c(p)
+    u = np.zeros((p.shape[0], 2))
+    lib().orc_free_dlp(_ptr(f), _ptr(n), _ptr(p), p.shape[0], _ptr(u))
+    return u
 
 
 def params(kind, npts, Q, Lx, Ly, P, Nb, tol):

@@ -1734,3 +1734,52 @@ void orc_derived_kspace_ds(int kind, const double* psrc, const double* ptar, con
         for (int q = 0; q < no; ++q) out[no * m + q] = creal(acc[q]) / (Lx * Ly) + zm;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Free-space layer potentials: the direct sums the reference's FMM tests    */
+/* check against (FMM/tests/stokes_SLP_FMM_test.m:29-51,                     */
+/* stokes_DLP_FMM_test.m:17-35).                                             */
+/* ------------------------------------------------------------------------ */
+/* u_t = sum_s (-1/2 log rho^2 f_s + (r.f_s) r/rho^2)/(4 pi), r = x_t - x_s;
+ * self: leave out s = t (the test's srcEqualsTar); pairs with rho = 0 skipped. */
+void orc_free_slp(const double* f, const double* psrc, int Ns, const double* ptar, int Nt,
+                  int self, double* u)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < Nt; ++k) {
+        double ud = 0, vd = 0;
+        for (int s = 0; s < Ns; ++s) {
+            if (self && s == k) continue;
+            double rx = ptar[2 * k] - psrc[2 * s], ry = ptar[2 * k + 1] - psrc[2 * s + 1];
+            double rho2 = rx * rx + ry * ry;
+            if (rho2 == 0) continue;
+            double rdotf = rx * f[2 * s] + ry * f[2 * s + 1];
+            ud += -0.5 * log(rho2) * f[2 * s] + rdotf / rho2 * rx;
+            vd += -0.5 * log(rho2) * f[2 * s + 1] + rdotf / rho2 * ry;
+        }
+        u[2 * k] = ud / 4 / ORC_PI;
+        u[2 * k + 1] = vd / 4 / ORC_PI;
+    }
+}
+
+/* u_t = 4 sum_{s != t} (r.n_s)(r.f_s) r/rho^4 / (4 pi) at the sources. */
+void orc_free_dlp(const double* f, const double* n, const double* p, int N, double* u)
+{
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k) {
+        double ud = 0, vd = 0;
+        for (int s = 0; s < N; ++s) {
+            if (s == k) continue;
+            double rx = p[2 * k] - p[2 * s], ry = p[2 * k + 1] - p[2 * s + 1];
+            double rho2 = rx * rx + ry * ry;
+            if (rho2 == 0) continue;
+            double rho4 = rho2 * rho2;
+            double rdotf = rx * f[2 * s] + ry * f[2 * s + 1];
+            double rdotn = rx * n[2 * s] + ry * n[2 * s + 1];
+            ud += rdotn * rdotf / rho4 * rx;
+            vd += rdotn * rdotf / rho4 * ry;
+        }
+        u[2 * k] = 4 * ud / 4 / ORC_PI;
+        u[2 * k + 1] = 4 * vd / 4 / ORC_PI;
+    }
+}

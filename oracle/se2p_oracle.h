@@ -156,6 +156,13 @@ void orc_derived_kspace_ds(int kind, const double* psrc, const double* ptar, con
                            const double* n, const double* b, int Ns, int Nt, double Lx,
                            double Ly, double xi, int kinf, double* out);
 
+/* Free-space direct sums (FMM/tests/stokes_{SLP,DLP}_FMM_test.m): SLP at Nt
+ * targets (self: targets are the sources, own term left out), DLP at the
+ * sources; u is 2 x Nt interleaved. */
+void orc_free_slp(const double* f, const double* psrc, int Ns, const double* ptar, int Nt,
+                  int self, double* u);
+void orc_free_dlp(const double* f, const double* n, const double* p, int N, double* u);
+
 #ifdef __cplusplus
 }
 #endif
