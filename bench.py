@@ -342,6 +342,51 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def free_bench(args):
+    """Free-space Stokeslet sums (the replacement of the FMM mex stokesSLPfmm):
+    N sources = targets uniform in [-1, 1]^2, f = 10 U(0, 1) (FMM/tests/
+    stokes_SLP_FMM_test.m:9-20), own terms left out.  value = pair
+    interactions per second of the GPU kernel (HIP events); the host call adds
+    the PCIe copies, reported beside it."""
+    import fasttools2d_amd as ft
+    N = args.free_n
+    rng = np.random.default_rng(21)
+    p = rng.uniform(-1, 1, (N, 2))
+    f = 10 * rng.random((N, 2))
+    ctx = ft.Context(0)
+    call = lambda: ft.stokesSLPfmm(f[:, 0], f[:, 1], p[:, 0], p[:, 1], p[:, 0], p[:, 1], 1, 5,  # noqa: E731
+                                   ctx=ctx)
+    for _ in range(args.warmup):
+        call()
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    host_s = (time.perf_counter() - t0) / args.steps
+    ms, cnt = ctx.prof_get("free")
+    ctx.prof_enable(False)
+    kern_s = ms * 1e-3 / max(cnt, 1)
+    pairs = float(N) * (N - 1)
+    out = {"metric": "free-space Stokeslet pair interactions/s, N=1e5 sources=targets",
+           "value": pairs / kern_s, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": kern_s * 1e3, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic: uniform in [-1, 1]^2, f = 10 U(0,1), seed 21",
+           "config": {"workload": f"free-space SLP, N={N}, iequal=1"},
+           "host_call_ms": host_s * 1e3}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        sample = min(N, 2000)
+        t0 = time.perf_counter()
+        O.free_slp(f, p, p[:sample], self=False)
+        cpu_s = (time.perf_counter() - t0) * N / sample
+        out["cpu_baseline"] = {"value": pairs / cpu_s, "unit": "pairs/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port",
+                               "sample": f"{sample} targets x {N} sources, scaled to {N}"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,12 +405,18 @@ def main():
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
                     help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
+    ap.add_argument("--free", action="store_true",
+                    help="run the free-space (FMM replacement) benchmark instead")
+    ap.add_argument("--free-n", type=int, default=100_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.batch:
         batch_bench(args)
+        return
+    if args.free:
+        free_bench(args)
         return
 
     import fasttools2d_amd as ft

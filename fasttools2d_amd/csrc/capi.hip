@@ -681,8 +681,11 @@ int free_eval(se2p_ctx* ctx, bool dlp, const double* f1, const double* f2, const
     if (nslice > 1)
         SE2P_TRY(ws_get(ctx, &ctx->ws, WS_FREE_PART, sizeof(double) * 2 * (size_t)Nt * nslice,
                         &part));
-    SE2P_TRY(launch_free(ctx->stream, dlp, ds, Ns, dt, Nt, self, nslice, (double*)part,
-                         (double*)out));
+    {
+        StageTimer st(ctx, "free");
+        SE2P_TRY(launch_free(ctx->stream, dlp, ds, Ns, dt, Nt, self, nslice, (double*)part,
+                             (double*)out));
+    }
     std::vector<double> h(2 * (size_t)Nt);
     SE2P_TRY(download(ctx, h.data(), (double*)out, h.size()));
     for (int i = 0; i < Nt; ++i) {
