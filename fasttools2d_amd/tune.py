@@ -27,12 +27,14 @@ from .plan import DeviceArray, Plan
 
 
 def smooth_up(n, max_n=4096):
-    """Smallest even n' >= n whose prime factors are in {2, 3, 5, 7}."""
+    """Smallest even n' >= n whose prime factors are in {2, 3, 5}: the radices
+    the line FFT runs as in-register butterflies (7 and larger primes take its
+    slower generic stage)."""
     n = max(4, int(math.ceil(n)))
     while n <= max_n:
         if n % 2 == 0:
             m = n
-            for p in (2, 3, 5, 7):
+            for p in (2, 3, 5):
                 while m % p == 0:
                     m //= p
             if m == 1:
@@ -105,7 +107,7 @@ def _aspect(Lx, Ly):
 
 
 def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
-             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4), reps=3):
+             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=3):
     """Returns (params, report).  ps, pt, f, nrm are (N, 2) host arrays."""
     log = log or (lambda s: None)
     A, B = _aspect(Lx, Ly)
