@@ -440,8 +440,7 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
-    SE2P_TRY(batch_apply_real(ctx, B, df, dur));
-    SE2P_TRY(batch_apply_k(ctx, B, df, duk, false, kind == SE2P_DLP));
+    SE2P_TRY(batch_apply_parts(ctx, B, df, dur, duk, false, kind == SE2P_DLP));
     std::vector<double> h(4 * (size_t)Nt);
     SE2P_TRY(download(ctx, h.data(), dur, 4 * (size_t)Nt));
     for (int i = 0; i < Nt; ++i) {
@@ -531,8 +530,7 @@ int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* y
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
-    SE2P_TRY(batch_apply_real(ctx, B, df, dpr));
-    SE2P_TRY(batch_apply_k(ctx, B, df, dpk, false, true));
+    SE2P_TRY(batch_apply_parts(ctx, B, df, dpr, dpk, false, true));
     std::vector<double> h(2 * (size_t)Nt);
     SE2P_TRY(download(ctx, h.data(), dpr, 2 * (size_t)Nt));
     for (int i = 0; i < Nt; ++i) {
@@ -695,6 +693,15 @@ int free_eval(se2p_ctx* ctx, bool dlp, const double* f1, const double* f2, const
     return SE2P_OK;
 }
 
+void destroy_streams(se2p_ctx* c)
+{
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
 }  // namespace
 }  // namespace se2p
 
@@ -728,10 +735,15 @@ int se2p_ctx_create(int device, se2p_ctx** out)
     se2p_ctx* c = new (std::nothrow) se2p_ctx();
     if (!c) return set_error(SE2P_ERR_NOMEM, "context allocation failed");
     c->device = device;
+    // both streams at the default priority
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
+        destroy_streams(c);
         delete c;
-        return set_error(SE2P_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        return set_error(SE2P_ERR_HIP, "stream set-up: %s", hipGetErrorString(e));
     }
     *out = c;
     return SE2P_OK;
@@ -750,7 +762,7 @@ int se2p_ctx_destroy(se2p_ctx* ctx)
         (void)hipEventDestroy(r.stop);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(ctx->stream);
+    destroy_streams(ctx);
     delete ctx;
     return SE2P_OK;
 }
@@ -1396,7 +1408,7 @@ int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, do
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
-    SE2P_TRY(batch_spread_grid(ctx, B, df, 0));
+    SE2P_TRY(batch_spread_grid(ctx, ctx->stream, B, df, 0));
     const size_t Gn = (size_t)Mx * My;
     std::vector<double> h(2 * Gn);
     SE2P_TRY(download(ctx, h.data(), (double*)G.Z0, 2 * Gn));

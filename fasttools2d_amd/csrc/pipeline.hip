@@ -380,9 +380,9 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
     return SE2P_OK;
 }
 
-int batch_spread_grid(se2p_ctx* ctx, const BatchExec& B, const double* d_f, int c)
+int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
+                      int c)
 {
-    hipStream_t s = ctx->stream;
     const Job* J = B.jobs;
     {
         StageTimer st(ctx, "spread");
@@ -401,10 +401,10 @@ int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
                          B.gather[0].count, d_u, 0, 0);
 }
 
-int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
-                  bool accumulate, bool use_zm)
+// Stage timers record on ctx->stream, so a pass on another stream is only
+// timed when it is ctx->stream (profiling runs keep every stage there).
+int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f)
 {
-    hipStream_t s = ctx->stream;
     const Job* J = B.jobs;
     // Every FFT pass runs on contiguous lines: y lines in the spread layout
     // (Z[ix*My + iy]), then a transpose to Z[iy*Mx + ix] for the x lines, the
@@ -413,7 +413,7 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
-        SE2P_TRY(batch_spread_grid(ctx, B, d_f, c));
+        SE2P_TRY(batch_spread_grid(ctx, s, B, d_f, c));
         StageTimer st(ctx, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
         for (int cls = 0; cls < 2; ++cls)
@@ -440,17 +440,53 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
             SE2P_TRY(launch_fft_lines(s, 0, Tg, 1, cls, B.lds_y[cls], B.sys,
                                       J + B.iffty[g][cls].first, B.iffty[g][cls].count));
     }
+    return SE2P_OK;
+}
+
+int batch_k_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u, bool accumulate, bool use_zm)
+{
     StageTimer st(ctx, "gather");
     for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g)
-        SE2P_TRY(launch_gather(s, B.maxR, g, B.sys, J + B.gather[g].first, B.gather[g].count, d_u,
-                               accumulate ? 1 : 0, use_zm && g == 0 ? 1 : 0));
+        SE2P_TRY(launch_gather(ctx->stream, B.maxR, g, B.sys, B.jobs + B.gather[g].first,
+                               B.gather[g].count, d_u, accumulate ? 1 : 0,
+                               use_zm && g == 0 ? 1 : 0));
     return SE2P_OK;
+}
+
+int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
+                  bool accumulate, bool use_zm)
+{
+    SE2P_TRY(batch_k_grids(ctx, ctx->stream, B, d_f));
+    return batch_k_gather(ctx, B, d_u, accumulate, use_zm);
+}
+
+int batch_apply_parts(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_ur,
+                      double* d_uk, bool accumulate, bool use_zm)
+{
+    if (ctx->prof) {
+        // one stream, so that each stage's events bracket that stage alone
+        SE2P_TRY(batch_apply_real(ctx, B, d_f, d_ur));
+        return batch_apply_k(ctx, B, d_f, d_uk, accumulate, use_zm);
+    }
+    // The grid stages read only d_f and geometry and write only the grids; the
+    // real-space stages write d_ur and the zero mode.  The gather, which reads
+    // the grids and the zero mode and adds into d_uk, waits for both streams, so
+    // every value is computed as in the one-stream order.
+    hipStream_t s = ctx->stream, side = ctx->side;
+    SE2P_HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+    SE2P_HIP_CHECK(hipStreamWaitEvent(side, ctx->ev_fork, 0));
+    SE2P_TRY(batch_k_grids(ctx, side, B, d_f));
+    SE2P_TRY(batch_apply_real(ctx, B, d_f, d_ur));
+    SE2P_HIP_CHECK(hipEventRecord(ctx->ev_join, side));
+    SE2P_HIP_CHECK(hipStreamWaitEvent(s, ctx->ev_join, 0));
+    return batch_k_gather(ctx, B, d_uk, accumulate, use_zm);
 }
 
 int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u)
 {
-    if (B.has_real) SE2P_TRY(batch_apply_real(ctx, B, d_f, d_u));
-    if (B.has_k) SE2P_TRY(batch_apply_k(ctx, B, d_f, d_u, B.has_real, B.has_real));
+    if (B.has_real && B.has_k) return batch_apply_parts(ctx, B, d_f, d_u, d_u, true, true);
+    if (B.has_real) return batch_apply_real(ctx, B, d_f, d_u);
+    if (B.has_k) return batch_apply_k(ctx, B, d_f, d_u, false, false);
     return SE2P_OK;
 }
 

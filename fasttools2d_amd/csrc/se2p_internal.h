@@ -103,6 +103,10 @@ struct Workspace {
 struct se2p_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for the k-space grid passes, forked from and joined back
+    // into `stream` with these two events inside every apply
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     se2p::Workspace ws;
     bool prof = false;
     bool capturing = false;
@@ -343,10 +347,19 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
 // u (+)= k-space part (+ DLP zero mode when use_zm)
 int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u,
                   bool accumulate, bool use_zm);
+// k-space passes on stream s up to the inverse FFTs (no gather)
+int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f);
+// u (+)= gather of the inverse-transformed grids (+ DLP zero mode when use_zm)
+int batch_k_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u, bool accumulate, bool use_zm);
+// ur = real-space part, uk (+)= k-space part: the two run concurrently on the
+// context's two streams (serially while stage profiling is on)
+int batch_apply_parts(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_ur,
+                      double* d_uk, bool accumulate, bool use_zm);
 // u = real + k (+ DLP zero mode)
 int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u);
 // Spreading and merge of component pair c into Z0 (c = 0) or Z1 (c = 1) only.
-int batch_spread_grid(se2p_ctx* ctx, const BatchExec& B, const double* d_f, int c);
+int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
+                      int c);
 // Gather from W only.
 int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u);
 
