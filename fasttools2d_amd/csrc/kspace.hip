@@ -28,8 +28,6 @@ namespace se2p {
 
 namespace {
 
-constexpr int kMaxP = 40;
-
 __device__ __forceinline__ void point_density(int mode, int p, const double* __restrict__ f,
                                               const double* __restrict__ nrm, double& d0,
                                               double& d1)
@@ -203,6 +201,19 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     if (out_grids(KIND) == 2) d.grid[G_Z1][gid] = o1;
 }
 
+// Gathering.  One 256-lane workgroup per tile of gTX x gTY start nodes: the
+// tile's gR x gR node region of the grid is staged in LDS, then every target of
+// the tile sums its (P+1)^2 nodes from there.  At node step (i, j) a lane reads
+// region entry b + i gR + j with b = ox gR + oy its start offset, so two lanes
+// whose b differ modulo 16 read different 16-byte LDS bank groups at every
+// step.  Each chunk of up to 256 targets is therefore handed to the lanes
+// interleaved by b mod 16 (ranks by wave ballots, no atomics): the k-th target
+// of residue r goes to position ~16k + r, so the 8- and 16-lane groups an LDS
+// read is served in mostly see distinct residues.  Which lane takes a target
+// does not change its sum, whose order is fixed: row sums over j with the y
+// weights held in registers (P is a template parameter, so the row unrolls and
+// its LDS reads are issued together), then the rows over i.
+template <int P>
 __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs,
                                                double* __restrict__ u_base, int g,
@@ -215,59 +226,108 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const int2* __restrict__ node = d.gt_node;
     const double2* __restrict__ off = d.gt_off;
     const double2* __restrict__ W = d.grid[g ? G_D : G_W];
-    const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, P = d.P, R = d.gR;
+    const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
     const double* __restrict__ e1tab = d.e1tab;
     const double* zm = use_zm ? d.zm : nullptr;
     double* u = u_base + d.u_off;
     extern __shared__ double2 reg[];
-    __shared__ double e1s[kMaxP + 1];
+    __shared__ double e1s[P + 1];
+    __shared__ int wcnt[4][16];
+    __shared__ int order[256];
     const int tile = jb.y;
     const int tx = tile % ntx, ty = tile / ntx;
     const int X0 = tx * TX, Y0 = ty * TY;
     const int k0 = tstart[tile], k1 = tstart[tile + 1];
     if (k0 == k1) return;
-    for (int i = threadIdx.x; i <= P; i += blockDim.x) e1s[i] = e1tab[i];
-    for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
-        const int lx = idx / R, ly = idx % R;
-        reg[idx] = W[(size_t)((X0 + lx) % Mx) * My + (Y0 + ly) % My];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i <= P; i += blockDim.x) e1s[i] = e1tab[i];
+    // region rows (periodic wrap; a grid may be narrower than the region)
+    for (int lx = wv; lx < R; lx += 4) {
+        int gx = X0 + lx;
+        while (gx >= Mx) gx -= Mx;
+        const double2* __restrict__ col = W + (size_t)gx * My;
+        for (int ly = lane; ly < R; ly += 64) {
+            int gy = Y0 + ly;
+            while (gy >= My) gy -= My;
+            reg[lx * R + ly] = col[gy];
+        }
     }
-    __syncthreads();
     const double z0 = zm ? zm[0] : 0.0, z1 = zm ? zm[1] : 0.0;
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const int p = perm[k];
-        const int2 m = node[p];
-        const double2 o = off[p];
-        const int ox = m.x - X0, oy = m.y - Y0;
-        double ex = exp(t * (o.x * o.x + 2 * w * o.x));
-        const double ey0 = exp(t * (o.y * o.y + 2 * w * o.y));
-        const double e3x = exp(-2 * t * h * o.x);
-        const double e3y = exp(-2 * t * h * o.y);
-        double s0 = 0, s1 = 0;
-        for (int i = 0; i <= P; ++i) {
-            double ey = ex * ey0 * e1s[i];
-            const double2* row = reg + (ox + i) * R + oy;
-            for (int j = 0; j <= P; ++j) {
-                const double g = ey * e1s[j];
-                const double2 v = row[j];
-                s0 += g * v.x;
-                s1 += g * v.y;
-                ey *= e3y;
+    for (int kb = k0; kb < k1; kb += 256) {
+        const int cnt = min(256, k1 - kb);
+        int key = -1;
+        if (tid < cnt) {
+            const int2 m = node[perm[kb + tid]];
+            key = ((m.x - X0) * R + (m.y - Y0)) & 15;
+        }
+        int rank = 0;
+        for (int b = 0; b < 16; ++b) {
+            const unsigned long long m = __ballot(key == b);
+            if (key == b)
+                rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (lane == 0) wcnt[wv][b] = __popcll(m);
+        }
+        __syncthreads();  // also orders the region and e1s stores before their reads
+        if (key >= 0) {
+            for (int q = 0; q < wv; ++q) rank += wcnt[q][key];
+            // occupied slots (r, b), r < count of b, that come before (rank, key)
+            // in the order 16 r + b
+            int pos = 0;
+            for (int b = 0; b < 16; ++b) {
+                const int tot = wcnt[0][b] + wcnt[1][b] + wcnt[2][b] + wcnt[3][b];
+                pos += min(tot, rank + (b < key ? 1 : 0));
             }
-            ex *= e3x;
+            order[pos] = tid;
         }
-        double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
-        if (d.nc == 1) {  // one real field: the real part of W
-            u[p] = accumulate ? u[p] + v0 : v0;
-            continue;
+        __syncthreads();
+        if (tid < cnt) {
+            const int p = perm[kb + order[tid]];
+            const int2 m = node[p];
+            const double2 o = off[p];
+            const int ox = m.x - X0, oy = m.y - Y0;
+            double wy[P + 1];
+            {
+                double ey = exp(t * (o.y * o.y + 2 * w * o.y));
+                const double e3y = exp(-2 * t * h * o.y);
+#pragma unroll
+                for (int j = 0; j <= P; ++j) {
+                    wy[j] = ey * e1s[j];
+                    ey *= e3y;
+                }
+            }
+            double ex = exp(t * (o.x * o.x + 2 * w * o.x));
+            const double e3x = exp(-2 * t * h * o.x);
+            double s0 = 0, s1 = 0;
+            const double2* row = reg + ox * R + oy;
+            for (int i = 0; i <= P; ++i, row += R) {
+                double r0 = 0, r1 = 0;
+#pragma unroll
+                for (int j = 0; j <= P; ++j) {
+                    const double2 v = row[j];
+                    r0 = fma(wy[j], v.x, r0);
+                    r1 = fma(wy[j], v.y, r1);
+                }
+                const double gx = ex * e1s[i];
+                s0 = fma(gx, r0, s0);
+                s1 = fma(gx, r1, s1);
+                ex *= e3x;
+            }
+            double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
+            if (d.nc == 1) {  // one real field: the real part of W
+                u[p] = accumulate ? u[p] + v0 : v0;
+            } else {
+                double* up = u + (size_t)d.nc * p + 2 * g;
+                if (accumulate) {
+                    v0 = up[0] + v0;
+                    v1 = up[1] + v1;
+                }
+                up[0] = v0;
+                up[1] = v1;
+            }
         }
-        double* up = u + (size_t)d.nc * p + 2 * g;
-        if (accumulate) {
-            v0 = up[0] + v0;
-            v1 = up[1] + v1;
-        }
-        up[0] = v0;
-        up[1] = v1;
+        __syncthreads();  // order[] and wcnt[] are rewritten by the next chunk
     }
 }
 
@@ -449,13 +509,25 @@ int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, 
     return SE2P_OK;
 }
 
-int launch_gather(hipStream_t s, int maxR, int g, const SysDesc* sys, const Job* jobs, int njobs,
-                  double* u, int accumulate, int use_zm)
+int launch_gather(hipStream_t s, int maxR, int P, int g, const SysDesc* sys, const Job* jobs,
+                  int njobs, double* u, int accumulate, int use_zm)
 {
     if (njobs <= 0) return SE2P_OK;
     const size_t lds = sizeof(double2) * (size_t)maxR * maxR;
-    hipLaunchKernelGGL(k_gather, dim3(njobs), dim3(256), lds, s, sys, jobs, u, g, accumulate,
-                       use_zm);
+    switch (P) {
+#define SE2P_GATHER_P(PP)                                                                         \
+    case PP:                                                                                      \
+        hipLaunchKernelGGL(k_gather<PP>, dim3(njobs), dim3(256), lds, s, sys, jobs, u, g,        \
+                           accumulate, use_zm);                                                   \
+        break;
+        SE2P_GATHER_P(2) SE2P_GATHER_P(4) SE2P_GATHER_P(6) SE2P_GATHER_P(8) SE2P_GATHER_P(10)
+        SE2P_GATHER_P(12) SE2P_GATHER_P(14) SE2P_GATHER_P(16) SE2P_GATHER_P(18) SE2P_GATHER_P(20)
+        SE2P_GATHER_P(22) SE2P_GATHER_P(24) SE2P_GATHER_P(26) SE2P_GATHER_P(28) SE2P_GATHER_P(30)
+        SE2P_GATHER_P(32) SE2P_GATHER_P(34) SE2P_GATHER_P(36) SE2P_GATHER_P(38) SE2P_GATHER_P(40)
+#undef SE2P_GATHER_P
+    default: return set_error(SE2P_ERR_UNSUPPORTED, "gather: P = %d", P);
+    }
+    static_assert(kMaxP == 40, "gather instantiations cover every even P up to kMaxP");
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

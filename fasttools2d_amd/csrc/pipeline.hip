@@ -336,9 +336,11 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         add(B->tr_inv[g], [&](int i, const SysDesc& d) {
             if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
         });
-        add(B->gather[g], [&](int i, const SysDesc& d) {
-            if (want(i, d) && d.Nt > 0) blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
-        });
+        for (int q = 0; q < kNumP; ++q)
+            add(B->gather[g][q], [&](int i, const SysDesc& d) {
+                if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1))
+                    blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
+            });
     }
     if (B->has_k) SE2P_TRY(fft_prepare());
     void *psys, *pjobs;
@@ -397,8 +399,10 @@ int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const do
 int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
 {
     StageTimer st(ctx, "gather");
-    return launch_gather(ctx->stream, B.maxR, 0, B.sys, B.jobs + B.gather[0].first,
-                         B.gather[0].count, d_u, 0, 0);
+    for (int q = 0; q < kNumP; ++q)
+        SE2P_TRY(launch_gather(ctx->stream, B.maxR, 2 * (q + 1), 0, B.sys,
+                               B.jobs + B.gather[0][q].first, B.gather[0][q].count, d_u, 0, 0));
+    return SE2P_OK;
 }
 
 // Stage timers record on ctx->stream, so a pass on another stream is only
@@ -447,9 +451,10 @@ int batch_k_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u, bool accumula
 {
     StageTimer st(ctx, "gather");
     for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g)
-        SE2P_TRY(launch_gather(ctx->stream, B.maxR, g, B.sys, B.jobs + B.gather[g].first,
-                               B.gather[g].count, d_u, accumulate ? 1 : 0,
-                               use_zm && g == 0 ? 1 : 0));
+        for (int q = 0; q < kNumP; ++q)
+            SE2P_TRY(launch_gather(ctx->stream, B.maxR, 2 * (q + 1), g, B.sys,
+                                   B.jobs + B.gather[g][q].first, B.gather[g][q].count, d_u,
+                                   accumulate ? 1 : 0, use_zm && g == 0 ? 1 : 0));
     return SE2P_OK;
 }
 

@@ -16,6 +16,9 @@ constexpr double kPi = 3.14159265358979323846;
 
 // Kinds SE2P_SLP .. SE2P_DLP_STRESS.
 constexpr int kNumKinds = SE2P_NUM_KINDS;
+// Gaussian support: P even in [2, kMaxP] (P + 1 grid nodes per direction)
+constexpr int kMaxP = 40;
+constexpr int kNumP = kMaxP / 2;
 // Stresslet kinds: normals, two spreading passes (f n1, f n2).
 __host__ __device__ constexpr bool dlp_family(int kind)
 {
@@ -255,9 +258,9 @@ int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, in
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
 // Gather of output grid g (W for g = 0, D for g = 1) into values 2g, 2g + 1 of
 // each target (+= when accumulate; + the zero mode when use_zm); maxR: the
-// largest gathering region side of the jobs' systems.
-int launch_gather(hipStream_t s, int maxR, int g, const SysDesc* sys, const Job* jobs, int njobs,
-                  double* u, int accumulate, int use_zm);
+// largest gathering region side of the jobs' systems, all of support P.
+int launch_gather(hipStream_t s, int maxR, int P, int g, const SysDesc* sys, const Job* jobs,
+                  int njobs, double* u, int accumulate, int use_zm);
 // Zero modes from the sorted records (DLP: sum_s (f.n)_s x_s / (Lx Ly); SLP
 // pressure: -sum_s f_s.x_s / (2 Lx Ly); DLP pressure: -sum_s (f.n)_s / (2 Lx Ly);
 // SLP vorticity: -sum_s (f1 y - f2 x)_s / (2 Lx Ly)): partial sums over jobs
@@ -335,7 +338,8 @@ struct BatchExec {
     JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
     JobRange filter[kNumKinds];  // by kind
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
-    JobRange ifftx[2][2], iffty[2][2], tr_inv[2], gather[2];  // [g][line class]
+    JobRange ifftx[2][2], iffty[2][2], tr_inv[2];  // [g][line class]
+    JobRange gather[2][kNumP];  // [g][P / 2 - 1]
     size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
     int maxR = 32;   // largest gathering region side
 };
