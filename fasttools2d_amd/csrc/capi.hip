@@ -1401,7 +1401,7 @@ int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, do
     double *dps, *df;
     SE2P_TRY(upload(ctx, WS_PTS_A, psrc, 2 * (size_t)Nsrc, &dps));
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
-    // k-space geometry of the sources only (no targets); spreading and merge of (f1, f2)
+    // k-space geometry of the sources only (no targets); spreading of (f1, f2)
     Geom G;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dps, dps, nullptr, Nsrc, 0, Lx, Ly,
                         k_only(xi, eta, Mx, My, w, P), false, true));

@@ -1,5 +1,5 @@
-// k-space part of the spectral Ewald sum: FGG spreading, halo merge, Stokes
-// filters, FGG gather.
+// k-space part of the spectral Ewald sum: FGG spreading, Stokes filters, FGG
+// gather.
 //
 // Follows PeriodicEwald/src/mex/common/ewald_tools.cpp:120-271 (Spread,
 // Gather) and mex_stokes_{slp,dlp}_kspace.cpp (filters).  The Gaussian
@@ -8,15 +8,11 @@
 // fast-Gaussian-gridding recurrence wx[i] = exp(t(px^2 + 2 w px)) e3x^i e1[i],
 // e3x = exp(-2 t h px), e1[i] = exp(t h^2 (i - P/2)^2), t = -2 xi^2/eta.
 //
-// Spreading.  Points are binned by the tile (TX x TY start nodes) their start
-// node falls in, in increasing index order.  A tile's points touch an R x R
-// node region (R = TX + P).  Its contribution to the two real grids is the
-// product  H_c = A^T diag(d_c) B  with A[k][gx] = wx_k(gx), B[k][gy] = wy_k(gy)
-// (zero outside the support), evaluated as a register-blocked FP64 outer-
-// product accumulation: 64 lanes, each owning a (R/8) x (R/8) block of nodes
-// of both grids.  Every node sums its points in index order, so the result is
-// bit-reproducible.  The R x R partial regions are then merged into the grid
-// by a node-parallel pass that adds the overlapping tiles in a fixed order.
+// Spreading.  Points are binned by the tile (kSB x kSB start nodes) their
+// start node falls in, in increasing index order.  Each kSB x kSB block of the
+// grid sums the points whose support reaches it as the matrix product
+// H_c = A^T diag(d_c) B, A[k][x] = wx_k(x), B[k][y] = wy_k(y) (zero outside
+// the support), on the FP64 matrix cores (k_spread below).
 //
 // Two real grids travel as one complex grid Z = H_0 + i H_1, so the SLP needs
 // one forward and one inverse complex FFT; the DLP spreads (f1 n1, f2 n1) and
@@ -43,138 +39,187 @@ __device__ __forceinline__ void point_density(int mode, int p, const double* __r
     }
 }
 
-template <int BLK>
-__global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
-                                               const Job* __restrict__ jobs, int c,
-                                               const double* __restrict__ f_base)
+// Spreading.  One 64-lane wave per kSB x kSB block of grid nodes, which it
+// owns.  It visits every point whose support reaches the block -- the points
+// of the spreading tiles (kSB x kSB start nodes, binned at build) that cover
+// the block's window of start nodes [X0 - P, X0 + kSB) x [Y0 - P, Y0 + kSB),
+// one periodic image at a time (several on grids narrower than the window) --
+// and accumulates
+//     H_c(x, y) = sum_k wx_k(x) d_c,k wy_k(y)
+// on the FP64 matrix cores: each v_mfma_f64_16x16x4f64 step adds four points,
+// A = their wx rows over the block's 16 x nodes, B = their d_c wy rows over its
+// 16 y nodes.  The weight rows of a chunk of kSC points are staged in LDS (the
+// first 32 lanes fill the x rows, the other 32 the y rows).  The block is
+// written to the grid once, zeros included.  Points are visited in a fixed
+// order (image, tile, binned order), so every node sum is bit-reproducible.
+constexpr int kSC = 32;       // points staged per chunk
+constexpr int kRow = kSB + 2;  // padded LDS row: consecutive lanes' rows start in different banks
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// row[x], x in [0, kSB): the weight of support node i = x - o (0 <= i <= P) of a
+// point whose start node lies at block offset o, e e3^i e1[i] with one factor
+// e3 per node as in the reference's recurrence; 0 off the support.  Support
+// nodes outside the block go to the row's padding slot kSB, so the stores
+// need no branch.
+template <int P>
+__device__ __forceinline__ void fill_row(double* row, int o, double e, double e3,
+                                         const double (&e1)[P + 1])
 {
-    const Job jb = jobs[blockIdx.x];
-    const SysDesc& d = sys[jb.x];
-    const int mode = dlp_family(d.kind) ? 1 + c : 0;
-    const int* __restrict__ tstart = d.sp_start;
-    const int* __restrict__ perm = d.sp_perm;
-    const int2* __restrict__ node = d.sp_node;
-    const double2* __restrict__ off = d.sp_off;
-    const double* __restrict__ f = f_base + d.f_off;
-    const double* __restrict__ nrm = d.nrm;
-    const int P = d.P, TX = d.TX, TY = d.TY, ntx = d.ntx;
-    const double t = d.t, w = d.w, h = d.h;
-    const double* __restrict__ e1tab = d.e1tab;
-    constexpr int R = 8 * BLK;
-    constexpr int KC = 16;
-    __shared__ double A[KC][R];
-    __shared__ double B0[KC][R];
-    __shared__ double B1[KC][R];
-    __shared__ double e1s[kMaxP + 1];
-
-    const int tile = jb.y;
-    const int tx = tile % ntx, ty = tile / ntx;
-    const int lane = threadIdx.x;
-    const int bi = (lane >> 3) * BLK, bj = (lane & 7) * BLK;
-    for (int i = lane; i <= P; i += 64) e1s[i] = e1tab[i];
-
-    double acc0[BLK][BLK], acc1[BLK][BLK];
 #pragma unroll
-    for (int i = 0; i < BLK; ++i)
+    for (int x = 0; x < kSB; x += 2) *reinterpret_cast<double2*>(row + x) = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int j = 0; j < BLK; ++j) acc0[i][j] = acc1[i][j] = 0;
-
-    const int k0 = tstart[tile], k1 = tstart[tile + 1];
-    for (int kb = k0; kb < k1; kb += KC) {
-        const int cnt = min(KC, k1 - kb);
-        __syncthreads();
-        for (int i = lane; i < KC * R; i += 64) {
-            (&A[0][0])[i] = 0;
-            (&B0[0][0])[i] = 0;
-            (&B1[0][0])[i] = 0;
-        }
-        __syncthreads();
-        if (lane < cnt) {
-            const int p = perm[kb + lane];
-            const int2 m = node[p];
-            const double2 o = off[p];
-            const int ox = m.x - tx * TX, oy = m.y - ty * TY;
-            double d0, d1;
-            point_density(mode, p, f, nrm, d0, d1);
-            double ex = exp(t * (o.x * o.x + 2 * w * o.x));
-            double ey = exp(t * (o.y * o.y + 2 * w * o.y));
-            const double e3x = exp(-2 * t * h * o.x);
-            const double e3y = exp(-2 * t * h * o.y);
-            for (int i = 0; i <= P; ++i) {
-                A[lane][ox + i] = ex * e1s[i];
-                const double wy = ey * e1s[i];
-                B0[lane][oy + i] = d0 * wy;
-                B1[lane][oy + i] = d1 * wy;
-                ex *= e3x;
-                ey *= e3y;
-            }
-        }
-        __syncthreads();
-        for (int k = 0; k < cnt; ++k) {
-            double a[BLK], b0[BLK], b1[BLK];
-#pragma unroll
-            for (int i = 0; i < BLK; ++i) {
-                a[i] = A[k][bi + i];
-                b0[i] = B0[k][bj + i];
-                b1[i] = B1[k][bj + i];
-            }
-#pragma unroll
-            for (int i = 0; i < BLK; ++i)
-#pragma unroll
-                for (int j = 0; j < BLK; ++j) {
-                    acc0[i][j] = fma(a[i], b0[j], acc0[i][j]);
-                    acc1[i][j] = fma(a[i], b1[j], acc1[i][j]);
-                }
-        }
+    for (int i = 0; i <= P; ++i) {
+        const int x = o + i;
+        row[x >= 0 && x < kSB ? x : kSB] = e * e1[i];
+        e *= e3;
     }
-    double* base = d.partial + (size_t)tile * 2 * R * R;
-#pragma unroll
-    for (int i = 0; i < BLK; ++i)
-#pragma unroll
-        for (int j = 0; j < BLK; ++j) {
-            base[(bi + i) * R + bj + j] = acc0[i][j];
-            base[R * R + (bi + i) * R + bj + j] = acc1[i][j];
-        }
 }
 
-// Node-parallel halo merge: node (ix, iy) adds, in a fixed order, every tile
-// region that covers it (regions wrap periodically, possibly more than once
-// on grids smaller than R).  Job b covers node column ix = b / ceil(My/256)
-// and 256 nodes of it; indices wrap by a single conditional add (tile and node
-// differences stay within one period).
-__global__ void k_merge(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, int out)
+template <int P>
+__global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
+                                               const Job* __restrict__ jobs, int c)
 {
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
-    const int Mx = d.Mx, My = d.My, TX = d.TX, TY = d.TY, ntx = d.ntx, nty = d.nty, R = d.R;
-    const int nyb = (My + 255) / 256;
-    const int ix = jb.y / nyb;
-    const int iy = (jb.y - ix * nyb) * 256 + threadIdx.x;
-    if (iy >= My) return;
-    const double* __restrict__ partial = d.partial;
-    const int ndx = min(ntx, R / TX + 2), ndy = min(nty, R / TY + 2);
-    const int bx = ix / TX, by = iy / TY;
-    double s0 = 0, s1 = 0;
-    for (int dx = 0; dx < ndx; ++dx) {
-        int tx = bx - dx;
-        tx += tx < 0 ? ntx : 0;
-        int lx = ix - tx * TX;
-        lx += lx < 0 ? Mx : 0;
-        for (; lx < R; lx += Mx) {
-            for (int dy = 0; dy < ndy; ++dy) {
-                int ty = by - dy;
-                ty += ty < 0 ? nty : 0;
-                const double* q = partial + (size_t)(ty * ntx + tx) * 2 * R * R + (size_t)lx * R;
-                int ly = iy - ty * TY;
-                ly += ly < 0 ? My : 0;
-                for (; ly < R; ly += My) {
-                    s0 += q[ly];
-                    s1 += q[R * R + ly];
+    const int* __restrict__ tstart = d.sp_start;
+    const int2* __restrict__ snode = d.sp_snode;
+    const double4* __restrict__ fac = d.sp_fac;
+    const double2* __restrict__ dens = d.sp_dens + (size_t)c * d.Ns;
+    const int Mx = d.Mx, My = d.My, ntx = d.ntx;
+    double e1[P + 1];  // the same for every lane: held in scalar registers
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const double v = d.e1tab[i];
+        const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+        const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+        e1[i] = __hiloint2double(hi, lo);
+    }
+    __shared__ __align__(16) double WX[kSC][kRow];
+    __shared__ __align__(16) double WY[kSC][kRow];
+    __shared__ double4 SF[kSC];
+    __shared__ double2 DD[kSC];
+    __shared__ int OX[kSC], OY[kSC];
+
+    const int lane = threadIdx.x;
+    // rows past a chunk's count are read (and multiplied by a zero x weight) in
+    // its last matrix step: keep every y row and density finite from the start
+    for (int i = lane; i < kSC * kRow; i += 64) (&WY[0][0])[i] = 0.0;
+    if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
+    const int X0 = (jb.y % ntx) * kSB, Y0 = (jb.y / ntx) * kSB;
+    const int XE = min(X0 + kSB, Mx), YE = min(Y0 + kSB, My);
+    const int q = lane & (kSC - 1);
+    const int row = lane >> 4, col = lane & 15;  // MFMA operand: point 4 s + row, node col
+    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // image kx: start node m.x - kx Mx; the window reaches images while X0 - P + kx Mx < Mx
+    for (int kx = 0; X0 - P + kx * Mx < Mx; ++kx) {
+        const int lox = max(0, X0 - P + kx * Mx), hix = min(Mx - 1, XE - 1 + kx * Mx);
+        for (int ky = 0; Y0 - P + ky * My < My; ++ky) {
+            const int loy = max(0, Y0 - P + ky * My), hiy = min(My - 1, YE - 1 + ky * My);
+            for (int ty = loy / kSB; ty <= hiy / kSB; ++ty)
+                for (int tx = lox / kSB; tx <= hix / kSB; ++tx) {
+                    const int tile = ty * ntx + tx;
+                    const int k0 = tstart[tile], k1 = tstart[tile + 1];
+                    for (int kb = k0; kb < k1; kb += kSC) {
+                        // one round of independent, contiguous loads per chunk
+                        const int k = kb + lane;
+                        const bool cand = lane < kSC && k < k1;
+                        int2 m = make_int2(-1, -1);
+                        double4 fk = make_double4(0.0, 0.0, 0.0, 0.0);
+                        double2 dk = make_double2(0.0, 0.0);
+                        if (cand) {
+                            m = snode[k];
+                            fk = fac[k];
+                            dk = dens[k];
+                        }
+                        const bool keep = cand && m.x >= lox && m.x <= hix && m.y >= loy &&
+                                          m.y <= hiy;
+                        const unsigned long long bal = __ballot(keep);
+                        if (!bal) continue;
+                        const int n = __popcll(bal);
+                        __syncthreads();  // the previous chunk's rows have been read
+                        if (keep) {
+                            const int pos = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                            OX[pos] = m.x - kx * Mx - X0;
+                            OY[pos] = m.y - ky * My - Y0;
+                            SF[pos] = fk;
+                            DD[pos] = dk;
+                        }
+                        __syncthreads();
+                        if (q < n) {  // lanes 0-31: x rows, 32-63: y rows, one code path
+                            const bool xr = lane < kSC;
+                            const double4 f4 = SF[q];
+                            fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q], xr ? f4.x : f4.z,
+                                        xr ? f4.y : f4.w, e1);
+                        }
+                        __syncthreads();
+                        // all operands of the chunk first, then the matrix steps back to back
+                        double a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
+#pragma unroll
+                        for (int st = 0; st < kSC / 4; ++st) {
+                            const int kk = 4 * st + row;
+                            const double wx = WX[kk][col];
+                            a[st] = kk < n ? wx : 0.0;
+                            const double wy = WY[kk][col];
+                            const double2 dd = DD[kk];
+                            b0[st] = dd.x * wy;
+                            b1[st] = dd.y * wy;
+                        }
+#pragma unroll
+                        for (int st = 0; st < kSC / 4; ++st)
+                            if (4 * st < n) {
+                                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st], acc0, 0,
+                                                                            0, 0);
+                                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st], acc1, 0,
+                                                                            0, 0);
+                            }
+                    }
                 }
-            }
         }
     }
-    d.grid[out][(size_t)ix * My + iy] = make_double2(s0, s1);
+    // the product's element (x, y) sits in lane y + 16 (x % 4), register x / 4
+    double2* __restrict__ out = d.grid[c ? G_Z1 : G_Z0];
+    const int iy = Y0 + col;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ix = X0 + 4 * r + row;
+        if (ix < Mx && iy < My) out[(size_t)ix * My + iy] = make_double2(acc0[r], acc1[r]);
+    }
+}
+
+__global__ void k_spread_records(const int* __restrict__ perm, const int2* __restrict__ node,
+                                 const double2* __restrict__ off, int n, double t, double w,
+                                 double h, int2* __restrict__ snode, double4* __restrict__ fac)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = perm[k];
+    const double2 o = off[p];
+    snode[k] = node[p];
+    fac[k] = make_double4(exp(t * (o.x * o.x + 2 * w * o.x)), exp(-2 * t * h * o.x),
+                          exp(t * (o.y * o.y + 2 * w * o.y)), exp(-2 * t * h * o.y));
+}
+
+__global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__ sys,
+                                                     const Job* __restrict__ jobs,
+                                                     const double* __restrict__ f_base)
+{
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int k = jb.y * 256 + threadIdx.x;
+    if (k >= d.Ns) return;
+    const int p = d.sp_perm[k];
+    const double* __restrict__ f = f_base + d.f_off;
+    double d0, d1;
+    if (!dlp_family(d.kind)) {
+        point_density(0, p, f, d.nrm, d0, d1);
+        d.sp_dens[k] = make_double2(d0, d1);
+        return;
+    }
+    point_density(1, p, f, d.nrm, d0, d1);
+    d.sp_dens[k] = make_double2(d0, d1);
+    point_density(2, p, f, d.nrm, d0, d1);
+    d.sp_dens[(size_t)d.Ns + k] = make_double2(d0, d1);
 }
 
 template <int KIND>
@@ -445,13 +490,13 @@ int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi,
     g->xi = xi;
     g->eta = eta;
     g->w = w;
-    g->R = P <= 24 ? 32 : 48;
-    g->TX = g->TY = g->R - P;
-    g->ntx = (Mx + g->TX - 1) / g->TX;
-    g->nty = (My + g->TY - 1) / g->TY;
+    // spreading tiles and blocks: kSB x kSB start nodes / grid nodes
+    g->TX = g->TY = kSB;
+    g->ntx = (Mx + kSB - 1) / kSB;
+    g->nty = (My + kSB - 1) / kSB;
     // gathering tiles: 32 x 32 start nodes (region up to 56 x 56 for P <= 24) for
     // 256 lanes per tile
-    g->gTX = g->gTY = P <= 24 ? 32 : g->TX;
+    g->gTX = g->gTY = P <= 24 ? 32 : 16;
     g->gR = g->gTX + P;
     g->gntx = (Mx + g->gTX - 1) / g->gTX;
     g->gnty = (My + g->gTY - 1) / g->gTY;
@@ -465,22 +510,39 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
     return SE2P_OK;
 }
 
-int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jobs, int njobs,
-                  const double* f)
+int launch_spread(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    if (R == 32)
-        hipLaunchKernelGGL(k_spread<4>, dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);
-    else
-        hipLaunchKernelGGL(k_spread<6>, dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);
+    switch (P) {
+#define SE2P_SPREAD_P(PP)                                                                         \
+    case PP: hipLaunchKernelGGL(k_spread<PP>, dim3(njobs), dim3(64), 0, s, sys, jobs, c); break;
+        SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
+        SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
+        SE2P_SPREAD_P(22) SE2P_SPREAD_P(24) SE2P_SPREAD_P(26) SE2P_SPREAD_P(28) SE2P_SPREAD_P(30)
+        SE2P_SPREAD_P(32) SE2P_SPREAD_P(34) SE2P_SPREAD_P(36) SE2P_SPREAD_P(38) SE2P_SPREAD_P(40)
+#undef SE2P_SPREAD_P
+    default: return set_error(SE2P_ERR_UNSUPPORTED, "spread: P = %d", P);
+    }
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs)
+int launch_spread_records(hipStream_t s, const KGeom& g, const int* perm, const int2* node,
+                          const double2* off, int n, int2* snode, double4* fac)
+{
+    if (n <= 0) return SE2P_OK;
+    const double t = -2 * g.xi * g.xi / g.eta;
+    hipLaunchKernelGGL(k_spread_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
+                       n, t, g.w, g.h, snode, fac);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
+                       const double* f)
 {
     if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_merge, dim3(njobs), dim3(256), 0, s, sys, jobs, out);
+    hipLaunchKernelGGL(k_spread_dens, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -122,6 +122,13 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->sp_node = (int2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_OFF, sizeof(double2) * (size_t)Ns, &p));
         G->sp_off = (double2*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_SNODE, sizeof(int2) * (size_t)Ns, &p));
+        G->sp_snode = (int2*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_FAC, sizeof(double4) * (size_t)Ns, &p));
+        G->sp_fac = (double4*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
+                        sizeof(double2) * (dlp_family(kind) ? 2 : 1) * (size_t)Ns, &p));
+        G->sp_dens = (double2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_KEY, sizeof(int) * (size_t)Nt, &tkey));
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_COUNT, sizeof(int) * (size_t)gtiles, &tcount));
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_PERM, sizeof(int) * (size_t)Nt, &p));
@@ -132,8 +139,6 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->gt_node = (int2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_OFF, sizeof(double2) * (size_t)Nt, &p));
         G->gt_off = (double2*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_PARTIAL, sizeof(double) * 2 * (size_t)g.R * g.R * ntiles, &p));
-        G->partial = (double*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GRID_A, sizeof(double2) * Gn, &p));
         G->Z0 = (double2*)p;
         // second grid pair: stresslet spreading (f n2) or a second output grid
@@ -155,6 +160,8 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)skey, (int*)scount, G->sp_node, G->sp_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, ntiles, G->sp_perm,
                              G->sp_start, WS_SORT_TMP));
+        SE2P_TRY(launch_spread_records(s, g, G->sp_perm, G->sp_node, G->sp_off, Ns, G->sp_snode,
+                                       G->sp_fac));
         SE2P_TRY(launch_fill_int(s, (int*)tcount, gtiles, 0));
         SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.gTX, g.gTY, g.gntx,
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
@@ -207,7 +214,6 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->TY = g.TY;
         d->ntx = g.ntx;
         d->nty = g.nty;
-        d->R = g.R;
         d->gTX = g.gTX;
         d->gTY = g.gTY;
         d->gntx = g.gntx;
@@ -227,12 +233,12 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->sp_perm = G.sp_perm;
         d->gt_start = G.gt_start;
         d->gt_perm = G.gt_perm;
-        d->sp_node = G.sp_node;
+        d->sp_snode = G.sp_snode;
+        d->sp_fac = G.sp_fac;
+        d->sp_dens = G.sp_dens;
         d->gt_node = G.gt_node;
-        d->sp_off = G.sp_off;
         d->gt_off = G.gt_off;
         d->e1tab = G.e1tab;
-        d->partial = G.partial;
         d->grid[G_Z0] = G.Z0;
         d->grid[G_Z1] = G.Z1;
         d->grid[G_W] = G.W;
@@ -298,17 +304,17 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     add(B->zm_sys, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, 1);
     });
+    add(B->sdens, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_k) blocks(i, (d.Ns + 255) / 256);
+    });
     for (int c = 0; c < 2; ++c) {
         auto want = [&](int i, const SysDesc& d) {
             return geoms[i]->has_k && (c == 0 || dlp_family(d.kind));
         };
-        for (int big = 0; big < 2; ++big)
-            add(B->spread[c][big], [&](int i, const SysDesc& d) {
-                if (want(i, d) && (d.R == 48) == (big == 1)) blocks(i, (long)d.ntx * d.nty);
+        for (int q = 0; q < kNumP; ++q)
+            add(B->spread[c][q], [&](int i, const SysDesc& d) {
+                if (want(i, d) && d.P == 2 * (q + 1)) blocks(i, (long)d.ntx * d.nty);
             });
-        add(B->merge[c], [&](int i, const SysDesc& d) {
-            if (want(i, d)) blocks(i, (long)d.Mx * ((d.My + 255) / 256));
-        });
         for (int cls = 0; cls < 2; ++cls) {
             add(B->ffty[c][cls], [&](int i, const SysDesc& d) {
                 if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
@@ -385,15 +391,14 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
 int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
                       int c)
 {
-    const Job* J = B.jobs;
-    {
-        StageTimer st(ctx, "spread");
-        for (int big = 0; big < 2; ++big)
-            SE2P_TRY(launch_spread(s, big ? 48 : 32, c, B.sys, J + B.spread[c][big].first,
-                                   B.spread[c][big].count, d_f));
-    }
-    StageTimer st(ctx, "merge");
-    return launch_merge(s, c ? G_Z1 : G_Z0, B.sys, J + B.merge[c].first, B.merge[c].count);
+    StageTimer st(ctx, "spread");
+    // the densities of every pair are packed with the first
+    if (c == 0)
+        SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
+    for (int q = 0; q < kNumP; ++q)
+        SE2P_TRY(launch_spread(s, 2 * (q + 1), c, B.sys, B.jobs + B.spread[c][q].first,
+                               B.spread[c][q].count));
+    return SE2P_OK;
 }
 
 int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)

@@ -19,6 +19,9 @@ constexpr int kNumKinds = SE2P_NUM_KINDS;
 // Gaussian support: P even in [2, kMaxP] (P + 1 grid nodes per direction)
 constexpr int kMaxP = 40;
 constexpr int kNumP = kMaxP / 2;
+// spreading: tiles of kSB x kSB start nodes, each grid block of kSB x kSB nodes
+// summed by one wave (the 16 x 16 output of an FP64 matrix-core step)
+constexpr int kSB = 16;
 // Stresslet kinds: normals, two spreading passes (f n1, f n2).
 __host__ __device__ constexpr bool dlp_family(int kind)
 {
@@ -88,8 +91,9 @@ enum Slot {
     WS_TAR_KEY, WS_TAR_PERM, WS_TAR_START, WS_TAR_COUNT,
     WS_SRC_SORTED, WS_TAR_SORTED,
     WS_SPR_KEY, WS_SPR_PERM, WS_SPR_START, WS_SPR_COUNT, WS_SPR_NODE, WS_SPR_OFF,
+    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS,
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
-    WS_PARTIAL, WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
+    WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
     WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
     WS_NUM_SLOTS
@@ -172,7 +176,7 @@ int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, doubl
 struct KGeom {
     int Mx, My, P;
     double Lx, Ly, h, xi, eta, w;
-    int TX, TY, ntx, nty, R;  // spreading tiles (TX x TY start nodes), region R x R
+    int TX, TY, ntx, nty;  // spreading tiles (TX x TY start nodes) = grid blocks
     int gTX, gTY, gntx, gnty, gR;  // gathering tiles, region gR x gR
 };
 int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi, double eta,
@@ -216,14 +220,15 @@ struct SysDesc {
     double *zm, *zm_part;
     int zm_nblk;
     // k space
-    int Mx, My, P, TX, TY, ntx, nty, R, Bx, By;
+    int Mx, My, P, TX, TY, ntx, nty, Bx, By;
     int gTX, gTY, gntx, gR;
     double t, w, h, xi, eta, gscale;
     const int *sp_start, *sp_perm, *gt_start, *gt_perm;
-    const int2 *sp_node, *gt_node;
-    const double2 *sp_off, *gt_off;
+    const int2 *sp_snode, *gt_node;  // spreading: start nodes in binned order
+    const double4* sp_fac;  // spreading: FGG factors (ex, e3x, ey, e3y) in binned order
+    double2* sp_dens;       // spreading: (d0, d1) of pair c at c Ns + k, binned order
+    const double2* gt_off;
     const double *e1tab, *nrm;
-    double* partial;
     double2* grid[4];  // GridSel
     const double2 *twx, *twy;
     FftAxis ax, ay;
@@ -247,12 +252,17 @@ int real_block_side(int nsx, int nsy, int mean_targets);
 // k_real_stencil<kind, tier, one-cell> over the jobs (block indices).
 int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys, const Job* jobs,
                 int njobs, double* u);
-// Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into the tile partials;
-// R selects the register blocking (32 or 48).
-int launch_spread(hipStream_t s, int R, int c, const SysDesc* sys, const Job* jobs, int njobs,
-                  const double* f);
-// Halo merge into grid `out`; job b = ix * ceil(My/256) + y block.
-int launch_merge(hipStream_t s, int out, const SysDesc* sys, const Job* jobs, int njobs);
+// Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
+// or Z1 (c = 1); job b = grid block (b % ntx, b / ntx) of kSB x kSB nodes; all
+// jobs' systems of support P.
+int launch_spread(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs, int njobs);
+// Spreading records of a geometry: start node and FGG factors of the k-th binned point.
+int launch_spread_records(hipStream_t s, const KGeom& g, const int* perm, const int2* node,
+                          const double2* off, int n, int2* snode, double4* fac);
+// Densities of the binned points of every spreading pair (job (s, b): points
+// b*256 .. b*256+255 of system s).
+int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
+                       const double* f);
 // Filter: W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
 // Z0 (and Z1 for kinds with 4 values per target).
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
@@ -313,10 +323,12 @@ struct Geom {
     int *sp_perm = nullptr, *sp_start = nullptr;
     int2* sp_node = nullptr;
     double2* sp_off = nullptr;
+    int2* sp_snode = nullptr;
+    double4* sp_fac = nullptr;
+    double2* sp_dens = nullptr;
     int *gt_perm = nullptr, *gt_start = nullptr;
     int2* gt_node = nullptr;
     double2* gt_off = nullptr;
-    double* partial = nullptr;
     double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr, *D = nullptr;
 };
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
@@ -332,9 +344,9 @@ struct BatchExec {
     Job* jobs = nullptr;
     JobRange fill[2];     // by record layout
     JobRange zm_part, zm_sys;
+    JobRange sdens;       // spreading densities: blocks of 256 binned points
     JobRange real[kNumKinds][2][2];  // [kind][tier][one-cell]
-    JobRange spread[2][2];   // [c][R == 48]
-    JobRange merge[2];       // [c]
+    JobRange spread[2][kNumP];  // [c][P / 2 - 1]
     JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
     JobRange filter[kNumKinds];  // by kind
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
@@ -361,7 +373,7 @@ int batch_apply_parts(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doub
                       double* d_uk, bool accumulate, bool use_zm);
 // u = real + k (+ DLP zero mode)
 int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u);
-// Spreading and merge of component pair c into Z0 (c = 0) or Z1 (c = 1) only.
+// Spreading of component pair c into Z0 (c = 0) or Z1 (c = 1) only.
 int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
                       int c);
 // Gather from W only.
