@@ -178,17 +178,28 @@ def cpu_baseline(p, f, prm, frac=0.05):
             "stage_s": {"real_sample": t_real, "grid": t_grid, "gather_sample": t_gather}}
 
 
-def stage_profile(ctx, fn, steps):
-    ctx.prof_enable(True)
-    ctx.prof_reset()
-    for _ in range(steps):
-        fn()
+STAGES = ("bin", "bin_grid", "pack", "real", "spread", "fft", "filter", "gather")
+
+
+def stage_collect(ctx, steps):
+    """Per-apply milliseconds of each stage recorded since prof_reset."""
     ctx.sync()
     stages = {}
-    for st in ("bin", "bin_grid", "pack", "real", "spread", "merge", "fft", "filter", "gather"):
+    for st in STAGES:
         ms, cnt = ctx.prof_get(st)
         if cnt:
             stages[st] = ms / steps
+    return stages
+
+
+def stage_profile(ctx, fn, steps, serial=False):
+    """Stage times over `steps` calls of fn: each stage bracketed by HIP events
+    on the stream it runs on; serial runs every stage on one stream in order."""
+    ctx.prof_enable(True, serial=serial)
+    ctx.prof_reset()
+    for _ in range(steps):
+        fn()
+    stages = stage_collect(ctx, steps)
     ctx.prof_enable(False)
     return stages
 
@@ -198,12 +209,10 @@ def profile_only(ctx, dev, ps, p, f, args):
     from fasttools2d_amd.plan import DeviceArray, Plan
     from fasttools2d_amd.tune import make_params
     if args.profile == "plain":
-        for _ in range(args.steps):
-            dev.run(ps)
-        ctx.sync()
-        # the same stages timed with HIP events in this process, to set beside
-        # the profiler's kernel durations
-        stages = stage_profile(ctx, lambda: dev.run(ps), args.steps)
+        # every stage on one stream, warm-up included; the same stages timed with
+        # HIP events in this process, to set beside the profiler's kernel durations
+        stage_profile(ctx, lambda: dev.run(ps), args.steps, serial=True)
+        stages = stage_profile(ctx, lambda: dev.run(ps), args.steps, serial=True)
         print(json.dumps({"profile": "plain", "steps": args.steps, "stage_ms": stages}))
         return
     if args.profile == "copy":
@@ -225,10 +234,10 @@ def profile_only(ctx, dev, ps, p, f, args):
     plan = Plan(ctx, "slp", p, p, 1.0, 1.0, prm)
     df = DeviceArray.from_host(ctx, f)
     du = DeviceArray(ctx, (len(p), 2))
-    for _ in range(args.steps):
-        plan.apply_device(df.ptr, du.ptr, graph=False)
-    ctx.sync()
-    stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps)
+    # every stage on one stream, warm-up included, so the profiler's kernel
+    # durations are the ones the HIP events below time
+    stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps, serial=True)
+    stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps, serial=True)
     print(json.dumps({"profile": "tuned", "steps": args.steps, "params": prm,
                       "stage_ms": stages}))
 
@@ -449,7 +458,7 @@ def main():
     ms_plain = time_steps(ctx, lambda: dev.run(ps), args.steps, args.warmup) * 1e3
     log(f"bench: plain {ms_plain:.3f} ms/eval")
     u_plain = dev.result()
-    stages_plain = stage_profile(ctx, lambda: dev.run(ps), args.steps)
+    stages_plain = stage_profile(ctx, lambda: dev.run(ps), args.steps, serial=True)
     log(f"bench: plain stage ms {json.dumps(stages_plain)}")
 
     # accurate reference: tol 1e-13 with two different Ewald splits
@@ -477,17 +486,20 @@ def main():
     plan = Plan(ctx, "slp", p, p, 1.0, 1.0, tprm)
     df = DeviceArray.from_host(ctx, f)
     du = DeviceArray(ctx, (N, 2))
-    ms_tuned = time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps,
-                          args.warmup) * 1e3
-    again = [time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps, 0) * 1e3
-             for _ in range(2)]
+    apply = lambda: plan.apply_device(df.ptr, du.ptr)  # noqa: E731
+    ms_tuned = time_steps(ctx, apply, args.steps, args.warmup) * 1e3
+    again = [time_steps(ctx, apply, args.steps, 0) * 1e3 for _ in range(2)]
     ms_graph = time_steps(ctx, lambda: plan.apply_device(df.ptr, du.ptr, graph=True), args.steps,
                           args.warmup) * 1e3
     log(f"bench: tuned timed window {ms_tuned:.3f} ms/eval; two more windows {again}; "
         f"graph replay {ms_graph:.3f} ms/eval")
     u_tuned = du.download()
     err_tuned = rel_err(u_tuned, u_ref)
-    stages = stage_profile(ctx, lambda: plan.apply_device(df.ptr, du.ptr), args.steps)
+    # Stage launch durations: HIP events on each stage's stream over a window of
+    # the same applies with every stage on one stream.  Events recorded between
+    # the launches of the two concurrent streams perturb their schedule (a window
+    # with them ran 1.33 instead of 1.11 ms), so each stage is timed alone.
+    stages = stage_profile(ctx, apply, args.steps, serial=True)
     log(f"bench: tuned {ms_tuned:.3f} ms/eval err {err_tuned:.2e} stages {json.dumps(stages)}")
 
     cand, evald = real_space_work(p, tprm)

@@ -264,8 +264,12 @@ class Context:
     def d2d(self, dst, src, nbytes):
         check(lib().se2p_memcpy_d2d(self.handle, dst, src, int(nbytes)))
 
-    def prof_enable(self, on=True):
-        check(lib().se2p_prof_enable(self.handle, 1 if on else 0))
+    def prof_enable(self, on=True, serial=False):
+        """Per-stage HIP-event timing.  serial: every stage of an apply on one
+        stream in order, so each is timed alone; otherwise the real-space and
+        k-space passes keep running concurrently and each stage is timed on
+        its own stream."""
+        check(lib().se2p_prof_enable(self.handle, (2 if serial else 1) if on else 0))
 
     def prof_reset(self):
         check(lib().se2p_prof_reset(self.handle))

@@ -87,7 +87,7 @@ void ws_release(Workspace* ws)
     }
 }
 
-int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec)
+int prof_begin(se2p_ctx* ctx, hipStream_t s, const char* name, ProfRecord* rec)
 {
     auto take = [&](hipEvent_t* e) -> int {
         if (!ctx->event_pool.empty()) {
@@ -101,13 +101,13 @@ int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec)
     rec->name = name;
     SE2P_TRY(take(&rec->start));
     SE2P_TRY(take(&rec->stop));
-    SE2P_HIP_CHECK(hipEventRecord(rec->start, ctx->stream));
+    SE2P_HIP_CHECK(hipEventRecord(rec->start, s));
     return SE2P_OK;
 }
 
-int prof_end(se2p_ctx* ctx, ProfRecord* rec)
+int prof_end(se2p_ctx* ctx, hipStream_t s, ProfRecord* rec)
 {
-    SE2P_HIP_CHECK(hipEventRecord(rec->stop, ctx->stream));
+    SE2P_HIP_CHECK(hipEventRecord(rec->stop, s));
     ctx->prof_pending.push_back(*rec);
     return SE2P_OK;
 }
@@ -1304,7 +1304,9 @@ int se2p_batch_destroy(se2p_batch* batch)
 int se2p_prof_enable(se2p_ctx* ctx, int on)
 {
     if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (on < 0 || on > 2) return set_error(SE2P_ERR_ARG, "profiling mode %d: need 0, 1 or 2", on);
     ctx->prof = on != 0;
+    ctx->prof_serial = on == 2;
     return SE2P_OK;
 }
 

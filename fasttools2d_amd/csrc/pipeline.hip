@@ -391,7 +391,7 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
 int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
                       int c)
 {
-    StageTimer st(ctx, "spread");
+    StageTimer st(ctx, s, "spread");
     // the densities of every pair are packed with the first
     if (c == 0)
         SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
@@ -410,8 +410,6 @@ int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
     return SE2P_OK;
 }
 
-// Stage timers record on ctx->stream, so a pass on another stream is only
-// timed when it is ctx->stream (profiling runs keep every stage there).
 int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f)
 {
     const Job* J = B.jobs;
@@ -423,7 +421,7 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
         SE2P_TRY(batch_spread_grid(ctx, s, B, d_f, c));
-        StageTimer st(ctx, "fft");
+        StageTimer st(ctx, s, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
         for (int cls = 0; cls < 2; ++cls)
             SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, cls, B.lds_y[cls], B.sys,
@@ -434,12 +432,12 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
                                       J + B.fftx[c][cls].first, B.fftx[c][cls].count));
     }
     {
-        StageTimer st(ctx, "filter");
+        StageTimer st(ctx, s, "filter");
         for (int kind = 0; kind < kNumKinds; ++kind)
             SE2P_TRY(launch_filter(s, kind, B.sys, J + B.filter[kind].first, B.filter[kind].count));
     }
     for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g) {
-        StageTimer st(ctx, "fft");
+        StageTimer st(ctx, s, "fft");
         const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
         for (int cls = 0; cls < 2; ++cls)
             SE2P_TRY(launch_fft_lines(s, 1, Zg, 1, cls, B.lds_x[cls], B.sys,
@@ -473,7 +471,7 @@ int batch_apply_k(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* 
 int batch_apply_parts(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_ur,
                       double* d_uk, bool accumulate, bool use_zm)
 {
-    if (ctx->prof) {
+    if (ctx->prof && ctx->prof_serial) {
         // one stream, so that each stage's events bracket that stage alone
         SE2P_TRY(batch_apply_real(ctx, B, d_f, d_ur));
         return batch_apply_k(ctx, B, d_f, d_uk, accumulate, use_zm);

@@ -116,6 +116,7 @@ struct se2p_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     se2p::Workspace ws;
     bool prof = false;
+    bool prof_serial = false;  // while profiling: every stage on `stream`, in order
     bool capturing = false;
     std::vector<se2p::ProfRecord> prof_pending;
     std::vector<se2p::ProfAcc> prof_acc;
@@ -129,20 +130,23 @@ namespace se2p {
 int ws_get(se2p_ctx* ctx, Workspace* ws, Slot slot, size_t bytes, void** out);
 void ws_release(Workspace* ws);
 
-// Stage bracketing for per-stage event timing.
-int prof_begin(se2p_ctx* ctx, const char* name, ProfRecord* rec);
-int prof_end(se2p_ctx* ctx, ProfRecord* rec);
+// Stage bracketing for per-stage event timing on the stage's stream.
+int prof_begin(se2p_ctx* ctx, hipStream_t s, const char* name, ProfRecord* rec);
+int prof_end(se2p_ctx* ctx, hipStream_t s, ProfRecord* rec);
 int prof_flush(se2p_ctx* ctx);
 
 struct StageTimer {
     se2p_ctx* ctx;
+    hipStream_t s;
     ProfRecord rec;
     bool on;
-    StageTimer(se2p_ctx* c, const char* name) : ctx(c), on(c->prof && !c->capturing) {
-        if (on) on = (prof_begin(ctx, name, &rec) == SE2P_OK);
+    StageTimer(se2p_ctx* c, const char* name) : StageTimer(c, c->stream, name) {}
+    StageTimer(se2p_ctx* c, hipStream_t st, const char* name)
+        : ctx(c), s(st), on(c->prof && !c->capturing) {
+        if (on) on = (prof_begin(ctx, s, name, &rec) == SE2P_OK);
     }
     ~StageTimer() {
-        if (on) prof_end(ctx, &rec);
+        if (on) prof_end(ctx, s, &rec);
     }
 };
 

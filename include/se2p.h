@@ -413,12 +413,16 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
 int se2p_e1(se2p_ctx* ctx, const double* x, int n, int tier, double* out);
 
 /* ------------------------------------------------------------------------ */
-/* Per-stage timing with HIP events on the context's stream.                 */
+/* Per-stage timing with HIP events, each stage bracketed on the stream it   */
+/* runs on.  on = 0: off; 1: on, the real-space and k-space passes of an     */
+/* apply still run concurrently on the context's two streams (a stage's time */
+/* includes what the other stream takes from it); 2: on, every stage on one  */
+/* stream in order, so each stage is timed alone.                            */
 /* ------------------------------------------------------------------------ */
 int se2p_prof_enable(se2p_ctx* ctx, int on);
 int se2p_prof_reset(se2p_ctx* ctx);
 /* Sum of event-measured milliseconds and launch count for a stage name
- * ("bin", "real", "spread", "merge", "fft", "filter", "gather", ...). */
+ * ("bin", "pack", "real", "spread", "fft", "filter", "gather", "free"). */
 int se2p_prof_get(se2p_ctx* ctx, const char* stage, double* total_ms, int* count);
 
 #ifdef __cplusplus

@@ -63,9 +63,10 @@ def test_plan_matches_one_shot_and_graph_replays(gpu_ctx, oracle, kind):
 @pytest.mark.parametrize("kind", ["slp", "dlp", "slp_vorticity", "slp_stress", "dlp_stress"])
 def test_two_stream_apply_matches_one_stream_order(gpu_ctx, kind):
     """An apply runs the real-space sum and the k-space grid passes on two streams
-    joined before the gather; with stage profiling on they run on one stream in
-    order.  Both orders give the same bits (kinds with a zero mode and with a
-    second output grid included), eager and graph-replayed, call after call."""
+    joined before the gather; with one-stream stage profiling they run on one
+    stream in order.  Both orders give the same bits (kinds with a zero mode and
+    with a second output grid included), eager and graph-replayed, call after
+    call, and so does an apply timed on both streams."""
     Lx, Ly = 1.0, 1.5
     ps = uniform_points(5000, Lx, Ly, seed=91)
     pt = uniform_points(3000, Lx, Ly, seed=92)
@@ -77,7 +78,7 @@ def test_two_stream_apply_matches_one_stream_order(gpu_ctx, kind):
     try:
         for seed in (4, 5):
             f = normal_density(5000, seed=seed)
-            gpu_ctx.prof_enable(True)
+            gpu_ctx.prof_enable(True, serial=True)
             try:
                 one = plan.apply(f)
             finally:
@@ -85,5 +86,10 @@ def test_two_stream_apply_matches_one_stream_order(gpu_ctx, kind):
             for _ in range(2):
                 np.testing.assert_array_equal(plan.apply(f), one)
                 np.testing.assert_array_equal(plan.apply(f, graph=True), one)
+            gpu_ctx.prof_enable(True)
+            try:
+                np.testing.assert_array_equal(plan.apply(f), one)
+            finally:
+                gpu_ctx.prof_enable(False)
     finally:
         plan.close()
