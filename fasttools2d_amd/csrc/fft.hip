@@ -97,9 +97,11 @@ __device__ __forceinline__ void dft8(double2* v, double sgn)
     v[7] = csub(e3, w3);
 }
 
-// Points per thread at most in a line pass: lines up to 2048 points run with 256
-// threads, longer ones with 512.
-constexpr int kFftPerThread = 8;
+// Line classes: T threads, at most PPT points per thread (every butterfly's
+// inputs are held in registers across the stage's barrier, so PPT sets the
+// registers and with them how many lines a CU holds at once):
+//   class 0: n <= 2048, T = 256, PPT = 8;  class 1: n <= 2560, T = 256, PPT = 10;
+//   class 2: n <= kFftMaxLen, T = 512, PPT = 8.
 
 // q = j / d, r = j % d for 0 <= j < 2^24, from a float reciprocal and one
 // correction step (no integer division in the butterfly loops).
@@ -158,13 +160,13 @@ __device__ __forceinline__ void st2(Ptr p, int i, double2 a)
 // lines from HBM (gdata) and LAST writes them back there; the other stages read
 // and write the LDS copy.  Twiddles: one table read per butterfly, the other
 // R-2 as powers of it.
-template <int R, bool FIRST, bool LAST, int T>
+template <int R, bool FIRST, bool LAST, int T, int PPT>
 __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
                                                     glb_d2* __restrict__ gdata, int n, int nb,
                                                     int Ns, const glb_d2* __restrict__ tw,
                                                     double sgn)
 {
-    constexpr int BPT = (kFftPerThread + R - 1) / R;  // butterflies per thread at most
+    constexpr int BPT = (PPT + R - 1) / R;  // butterflies per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
     const float rNs = 1.0f / (float)Ns, rm = 1.0f / (float)m;
@@ -218,13 +220,13 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
 }
 
 // Any radix: one output per (thread, slot), O(R) work per output.
-template <bool FIRST, bool LAST, int T>
+template <bool FIRST, bool LAST, int T, int PPT>
 __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds,
                                                         glb_d2* __restrict__ gdata, int n, int nb,
                                                         int Ns, int R,
                                                         const glb_d2* __restrict__ tw, double sgn)
 {
-    constexpr int OPT = kFftPerThread;  // outputs per thread at most
+    constexpr int OPT = PPT;  // outputs per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
     const float rn = 1.0f / (float)n, rm = 1.0f / (float)m, rNs = 1.0f / (float)Ns;
@@ -278,8 +280,8 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
 // (axis 0: My-lines, the column-major grid) or x (axis 1: Mx-lines, the
 // transposed grid); every line is contiguous.  The lines live in one LDS
 // buffer of B*n points between the first and the last stage.  T threads with
-// B*n <= kFftPerThread * T.
-template <int T>
+// B*n <= PPT * T.
+template <int T, int PPT>
 __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys,
                                                  const Job* __restrict__ jobs, int axis, int sel,
                                                  int inverse)
@@ -303,10 +305,10 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
         const int R = ax.radix[st];
         const bool first = st == 0, last = st == nst - 1;
 #define SE2P_STAGE(RR)                                                          \
-    if (first && last) stage_reg<RR, true, true, T>(buf, data, n, nb, Ns, tw, sgn);  \
-    else if (first) stage_reg<RR, true, false, T>(buf, data, n, nb, Ns, tw, sgn);   \
-    else if (last) stage_reg<RR, false, true, T>(buf, data, n, nb, Ns, tw, sgn);    \
-    else stage_reg<RR, false, false, T>(buf, data, n, nb, Ns, tw, sgn)
+    if (first && last) stage_reg<RR, true, true, T, PPT>(buf, data, n, nb, Ns, tw, sgn);  \
+    else if (first) stage_reg<RR, true, false, T, PPT>(buf, data, n, nb, Ns, tw, sgn);   \
+    else if (last) stage_reg<RR, false, true, T, PPT>(buf, data, n, nb, Ns, tw, sgn);    \
+    else stage_reg<RR, false, false, T, PPT>(buf, data, n, nb, Ns, tw, sgn)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
@@ -314,10 +316,14 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
             case 5: SE2P_STAGE(5); break;
             case 8: SE2P_STAGE(8); break;
             default:
-                if (first && last) stage_generic<true, true, T>(buf, data, n, nb, Ns, R, tw, sgn);
-                else if (first) stage_generic<true, false, T>(buf, data, n, nb, Ns, R, tw, sgn);
-                else if (last) stage_generic<false, true, T>(buf, data, n, nb, Ns, R, tw, sgn);
-                else stage_generic<false, false, T>(buf, data, n, nb, Ns, R, tw, sgn);
+                if (first && last)
+                    stage_generic<true, true, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                else if (first)
+                    stage_generic<true, false, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                else if (last)
+                    stage_generic<false, true, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                else
+                    stage_generic<false, false, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
                 break;
         }
 #undef SE2P_STAGE
@@ -401,29 +407,31 @@ int fft_prepare()
 {
     static bool attr_set = false;
     if (!attr_set) {
-        SE2P_HIP_CHECK(hipFuncSetAttribute((const void*)k_fft_lines<256>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           kFftMaxLen * (int)sizeof(double2)));
-        SE2P_HIP_CHECK(hipFuncSetAttribute((const void*)k_fft_lines<512>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           kFftMaxLen * (int)sizeof(double2)));
+        const void* kernels[] = {(const void*)k_fft_lines<256, 8>, (const void*)k_fft_lines<256, 10>,
+                                 (const void*)k_fft_lines<512, 8>};
+        for (const void* k : kernels)
+            SE2P_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               kFftMaxLen * (int)sizeof(double2)));
         attr_set = true;
     }
     return SE2P_OK;
 }
 
-int fft_line_class(int n) { return n > 2048 ? 1 : 0; }
+int fft_line_class(int n) { return n <= 2048 ? 0 : n <= 2560 ? 1 : 2; }
 
 int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
     if (cls == 0)
-        hipLaunchKernelGGL(k_fft_lines<256>, dim3(njobs), dim3(256), max_lds, s, sys, jobs, axis,
-                           sel, inverse);
+        hipLaunchKernelGGL((k_fft_lines<256, 8>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
+                           axis, sel, inverse);
+    else if (cls == 1)
+        hipLaunchKernelGGL((k_fft_lines<256, 10>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
+                           axis, sel, inverse);
     else
-        hipLaunchKernelGGL(k_fft_lines<512>, dim3(njobs), dim3(512), max_lds, s, sys, jobs, axis,
-                           sel, inverse);
+        hipLaunchKernelGGL((k_fft_lines<512, 8>), dim3(njobs), dim3(512), max_lds, s, sys, jobs,
+                           axis, sel, inverse);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

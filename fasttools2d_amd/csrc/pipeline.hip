@@ -315,7 +315,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             add(B->spread[c][q], [&](int i, const SysDesc& d) {
                 if (want(i, d) && d.P == 2 * (q + 1)) blocks(i, (long)d.ntx * d.nty);
             });
-        for (int cls = 0; cls < 2; ++cls) {
+        for (int cls = 0; cls < kFftClasses; ++cls) {
             add(B->ffty[c][cls], [&](int i, const SysDesc& d) {
                 if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
             });
@@ -331,7 +331,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         auto want = [&](int i, const SysDesc& d) {
             return geoms[i]->has_k && (g == 0 || out_grids(d.kind) == 2);
         };
-        for (int cls = 0; cls < 2; ++cls) {
+        for (int cls = 0; cls < kFftClasses; ++cls) {
             add(B->ifftx[g][cls], [&](int i, const SysDesc& d) {
                 if (want(i, d) && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
             });
@@ -423,11 +423,11 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
         SE2P_TRY(batch_spread_grid(ctx, s, B, d_f, c));
         StageTimer st(ctx, s, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
-        for (int cls = 0; cls < 2; ++cls)
+        for (int cls = 0; cls < kFftClasses; ++cls)
             SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, cls, B.lds_y[cls], B.sys,
                                       J + B.ffty[c][cls].first, B.ffty[c][cls].count));
         SE2P_TRY(launch_transpose(s, Zc, Tc, 0, B.sys, J + B.tr_fwd[c].first, B.tr_fwd[c].count));
-        for (int cls = 0; cls < 2; ++cls)
+        for (int cls = 0; cls < kFftClasses; ++cls)
             SE2P_TRY(launch_fft_lines(s, 1, Tc, 0, cls, B.lds_x[cls], B.sys,
                                       J + B.fftx[c][cls].first, B.fftx[c][cls].count));
     }
@@ -439,11 +439,11 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
     for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g) {
         StageTimer st(ctx, s, "fft");
         const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
-        for (int cls = 0; cls < 2; ++cls)
+        for (int cls = 0; cls < kFftClasses; ++cls)
             SE2P_TRY(launch_fft_lines(s, 1, Zg, 1, cls, B.lds_x[cls], B.sys,
                                       J + B.ifftx[g][cls].first, B.ifftx[g][cls].count));
         SE2P_TRY(launch_transpose(s, Zg, Tg, 1, B.sys, J + B.tr_inv[g].first, B.tr_inv[g].count));
-        for (int cls = 0; cls < 2; ++cls)
+        for (int cls = 0; cls < kFftClasses; ++cls)
             SE2P_TRY(launch_fft_lines(s, 0, Tg, 1, cls, B.lds_y[cls], B.sys,
                                       J + B.iffty[g][cls].first, B.iffty[g][cls].count));
     }

@@ -293,6 +293,7 @@ int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* j
 // Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
 // x (axis 1: Mx-lines of the transposed grid); every job's line length must be of
 // class `cls` (fft_line_class); max_lds from fft_lds_bytes.
+constexpr int kFftClasses = 3;
 int fft_line_class(int n);
 int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs);
@@ -351,12 +352,12 @@ struct BatchExec {
     JobRange sdens;       // spreading densities: blocks of 256 binned points
     JobRange real[kNumKinds][2][2];  // [kind][tier][one-cell]
     JobRange spread[2][kNumP];  // [c][P / 2 - 1]
-    JobRange ffty[2][2], fftx[2][2], tr_fwd[2];  // [c][line class]: forward passes of pair c
+    JobRange ffty[2][kFftClasses], fftx[2][kFftClasses], tr_fwd[2];  // [c][line class]
     JobRange filter[kNumKinds];  // by kind
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
-    JobRange ifftx[2][2], iffty[2][2], tr_inv[2];  // [g][line class]
+    JobRange ifftx[2][kFftClasses], iffty[2][kFftClasses], tr_inv[2];  // [g][line class]
     JobRange gather[2][kNumP];  // [g][P / 2 - 1]
-    size_t lds_y[2] = {0, 0}, lds_x[2] = {0, 0};  // [line class]
+    size_t lds_y[kFftClasses] = {}, lds_x[kFftClasses] = {};  // [line class]
     int maxR = 32;   // largest gathering region side
 };
 // Builds descriptors (f and u offsets: systems back to back, 2 Ns and 2 Nt
