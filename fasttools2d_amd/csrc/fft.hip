@@ -14,6 +14,7 @@
 // Every pass works on contiguous lines: y lines of the My x Mx column-major
 // grid (Z[ix*My + iy]), and x lines after a tiled transpose.
 #include "se2p_internal.h"
+#include "fft_roots.h"
 
 #include <cmath>
 #include <vector>
@@ -118,15 +119,63 @@ __device__ __forceinline__ void divmod(int j, int d, float rd, int& q, int& r)
     }
 }
 
-// In-register DFT of radix R in {2, 3, 4, 5, 8}.
+template <int R>
+__device__ __forceinline__ void butterfly(double2* v, double sgn);
+
+// DFT of length R = R1 R2 in registers: with r = r1 + R1 r2 and k = k2 + R2 k1,
+//   X[k] = sum_r1 W_R1^(r1 k1) W_R^(r1 k2) sum_r2 W_R2^(r2 k2) x[r1 + R1 r2],
+// i.e. R1 DFTs of length R2, the twiddles W_R^(r1 k2) (compile-time constants),
+// then R2 DFTs of length R1.
+template <int R1, int R2>
+__device__ __forceinline__ void dft_split(double2* v, double sgn)
+{
+    constexpr int R = R1 * R2;
+    double2 y[R1][R2];
+#pragma unroll
+    for (int r1 = 0; r1 < R1; ++r1) {
+#pragma unroll
+        for (int r2 = 0; r2 < R2; ++r2) y[r1][r2] = v[r1 + R1 * r2];
+        butterfly<R2>(y[r1], sgn);
+    }
+#pragma unroll
+    for (int r1 = 1; r1 < R1; ++r1)
+#pragma unroll
+        for (int k2 = 1; k2 < R2; ++k2) {
+            const int m = (r1 * k2) % R;
+            y[r1][k2] = cmul(y[r1][k2], make_double2(FftRoots<R>::c[m], sgn * FftRoots<R>::s[m]));
+        }
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+        double2 z[R1];
+#pragma unroll
+        for (int r1 = 0; r1 < R1; ++r1) z[r1] = y[r1][k2];
+        butterfly<R1>(z, sgn);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) v[k2 + R2 * k1] = z[k1];
+    }
+}
+
+// In-register DFT of radix R in {2, 3, 4, 5, 8} directly, {6, 9, 10, 12, 15,
+// 16, 20, 25} split into two of those.
 template <int R>
 __device__ __forceinline__ void butterfly(double2* v, double sgn)
 {
-    if (R == 2) dft2(v);
-    else if (R == 3) dft3(v, sgn);
-    else if (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
-    else if (R == 5) dft5(v, sgn);
-    else dft8(v, sgn);
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 3) dft3(v, sgn);
+    else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
+    else if constexpr (R == 5) dft5(v, sgn);
+    else if constexpr (R == 8) dft8(v, sgn);
+    else if constexpr (R == 6) dft_split<2, 3>(v, sgn);
+    else if constexpr (R == 9) dft_split<3, 3>(v, sgn);
+    else if constexpr (R == 10) dft_split<2, 5>(v, sgn);
+    else if constexpr (R == 12) dft_split<4, 3>(v, sgn);
+    else if constexpr (R == 15) dft_split<3, 5>(v, sgn);
+    else if constexpr (R == 16) dft_split<4, 4>(v, sgn);
+    else if constexpr (R == 20) dft_split<4, 5>(v, sgn);
+    else dft_split<5, 5>(v, sgn);
+    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 6 || R == 9 ||
+                      R == 10 || R == 12 || R == 15 || R == 16 || R == 20 || R == 25,
+                  "unsupported in-register radix");
 }
 
 // Pointers into LDS and global memory with their address space spelled out:
@@ -144,6 +193,8 @@ __device__ __forceinline__ double2 ld2(Ptr p, int i)
     const dvec2 v = p[i];
     return make_double2(v.x, v.y);
 }
+
+__device__ __forceinline__ double2 conj_if(double2 a, int c) { return make_double2(a.x, c ? -a.y : a.y); }
 
 template <typename Ptr>
 __device__ __forceinline__ void st2(Ptr p, int i, double2 a)
@@ -164,7 +215,7 @@ template <int R, bool FIRST, bool LAST, int T, int PPT>
 __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
                                                     glb_d2* __restrict__ gdata, int n, int nb,
                                                     int Ns, const glb_d2* __restrict__ tw,
-                                                    double sgn)
+                                                    int inverse)
 {
     constexpr int BPT = (PPT + R - 1) / R;  // butterflies per thread at most
     const int m = n / R;
@@ -184,14 +235,13 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
             divmod(j, Ns, rNs, jq, jm);
             if constexpr (FIRST) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[t][r] = ld2(gdata, b * n + j + r * m);
+                for (int r = 0; r < R; ++r) v[t][r] = conj_if(ld2(gdata, b * n + j + r * m), inverse);
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[t][r] = ld2(lds, b * n + j + r * m);
             }
             if (Ns > 1) {
-                const double2 w0 = ld2(tw, jm * base);
-                const double2 w1 = make_double2(w0.x, -sgn * w0.y);
+                const double2 w1 = ld2(tw, jm * base);
                 double2 w = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -199,7 +249,7 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
                     if (r + 1 < R) w = cmul(w, w1);
                 }
             }
-            butterfly<R>(v[t], sgn);
+            butterfly<R>(v[t], -1.0);
             ob[t] = b * n + jq * Ns * R + jm;
         }
     }
@@ -210,7 +260,7 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 if constexpr (LAST)
-                    st2(gdata, ob[t] + k * Ns, v[t][k]);
+                    st2(gdata, ob[t] + k * Ns, conj_if(v[t][k], inverse));
                 else
                     st2(lds, ob[t] + k * Ns, v[t][k]);
             }
@@ -224,7 +274,7 @@ template <bool FIRST, bool LAST, int T, int PPT>
 __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds,
                                                         glb_d2* __restrict__ gdata, int n, int nb,
                                                         int Ns, int R,
-                                                        const glb_d2* __restrict__ tw, double sgn)
+                                                        const glb_d2* __restrict__ tw, int inverse)
 {
     constexpr int OPT = PPT;  // outputs per thread at most
     const int m = n / R;
@@ -249,11 +299,11 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
             for (int r = 0; r < R; ++r) {
                 double2 x;
                 if constexpr (FIRST)
-                    x = ld2(gdata, i0 + r * m);
+                    x = conj_if(ld2(gdata, i0 + r * m), inverse);
                 else
                     x = ld2(lds, i0 + r * m);
                 const double2 wv = ld2(tw, e);
-                const double wi = -sgn * wv.y;
+                const double wi = wv.y;
                 re += x.x * wv.x - x.y * wi;
                 im += x.x * wi + x.y * wv.x;
                 e += step;
@@ -268,7 +318,7 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
     for (int t = 0; t < OPT; ++t) {
         if (ob[t] >= 0) {
             if constexpr (LAST)
-                st2(gdata, ob[t], res[t]);
+                st2(gdata, ob[t], conj_if(res[t], inverse));
             else
                 st2(lds, ob[t], res[t]);
         }
@@ -281,7 +331,7 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
 // transposed grid); every line is contiguous.  The lines live in one LDS
 // buffer of B*n points between the first and the last stage.  T threads with
 // B*n <= PPT * T.
-template <int T, int PPT>
+template <int T, int PPT, bool COMP>
 __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys,
                                                  const Job* __restrict__ jobs, int axis, int sel,
                                                  int inverse)
@@ -297,33 +347,41 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
     glb_d2* __restrict__ data = (glb_d2*)(d.grid[sel] + (size_t)jb.y * B * n);
     lds_d2* __restrict__ buf = (lds_d2*)lds;
     const int nb = min(B, nlines - jb.y * B);
-    // the table holds exp(-2 pi i k/n); sgn is the sign of the transform's exponent
-    const double sgn = inverse ? 1.0 : -1.0;
+    // The table holds exp(-2 pi i k/n) and every stage runs the forward
+    // transform; the inverse is conj(FFT(conj(x))): the first stage conjugates
+    // what it loads and the last what it stores.
     const int nst = ax.nst;
     int Ns = 1;
     for (int st = 0; st < nst; ++st) {
         const int R = ax.radix[st];
         const bool first = st == 0, last = st == nst - 1;
 #define SE2P_STAGE(RR)                                                          \
-    if (first && last) stage_reg<RR, true, true, T, PPT>(buf, data, n, nb, Ns, tw, sgn);  \
-    else if (first) stage_reg<RR, true, false, T, PPT>(buf, data, n, nb, Ns, tw, sgn);   \
-    else if (last) stage_reg<RR, false, true, T, PPT>(buf, data, n, nb, Ns, tw, sgn);    \
-    else stage_reg<RR, false, false, T, PPT>(buf, data, n, nb, Ns, tw, sgn)
+    if (first && last) stage_reg<RR, true, true, T, PPT>(buf, data, n, nb, Ns, tw, inverse);  \
+    else if (first) stage_reg<RR, true, false, T, PPT>(buf, data, n, nb, Ns, tw, inverse);   \
+    else if (last) stage_reg<RR, false, true, T, PPT>(buf, data, n, nb, Ns, tw, inverse);    \
+    else stage_reg<RR, false, false, T, PPT>(buf, data, n, nb, Ns, tw, inverse)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
             case 4: SE2P_STAGE(4); break;
             case 5: SE2P_STAGE(5); break;
             case 8: SE2P_STAGE(8); break;
+#define SE2P_COMPOSITE(RR)             \
+    case RR:                           \
+        if constexpr (COMP) SE2P_STAGE(RR); \
+        break;
+            SE2P_COMPOSITE(6) SE2P_COMPOSITE(9) SE2P_COMPOSITE(10) SE2P_COMPOSITE(12)
+            SE2P_COMPOSITE(15) SE2P_COMPOSITE(16)
+#undef SE2P_COMPOSITE
             default:
                 if (first && last)
-                    stage_generic<true, true, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                    stage_generic<true, true, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
                 else if (first)
-                    stage_generic<true, false, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                    stage_generic<true, false, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
                 else if (last)
-                    stage_generic<false, true, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                    stage_generic<false, true, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
                 else
-                    stage_generic<false, false, T, PPT>(buf, data, n, nb, Ns, R, tw, sgn);
+                    stage_generic<false, false, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
                 break;
         }
 #undef SE2P_STAGE
@@ -369,14 +427,37 @@ int fft_axis_init(FftAxis* a, int n)
         return set_error(SE2P_ERR_UNSUPPORTED, "FFT length %d outside [1, %d]", n, kFftMaxLen);
     a->n = n;
     a->nst = 0;
-    int r = n;
-    const int small[] = {8, 4, 2, 3, 5, 7};
-    for (int p : small)
+    // The 2-3-5-smooth part of n in as few in-register stages as possible (each
+    // stage is an LDS round trip and two barriers), larger radices first; the
+    // composite radices only in the class whose kernel has them (their
+    // registers would cost the other classes a line per CU).  Every other prime
+    // factor becomes a stage of its own (7 and up run as direct sums).
+    const bool comp = fft_line_class(n) == 1;
+    static const int rad_comp[] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+    static const int rad_base[] = {8, 5, 4, 3, 2};
+    int smooth = 1, r = n;
+    for (int p : {2, 3, 5})
         while (r % p == 0) {
-            a->radix[a->nst++] = p;
+            smooth *= p;
             r /= p;
         }
-    for (int p = 11; r > 1; p += 2)
+    std::vector<int> best(smooth + 1, 1 << 20), pick(smooth + 1, 0);
+    best[1] = 0;
+    for (int d = 2; d <= smooth; ++d) {
+        if (smooth % d) continue;
+        auto consider = [&](int q) {
+            if (d % q == 0 && best[d / q] + 1 < best[d]) {
+                best[d] = best[d / q] + 1;
+                pick[d] = q;
+            }
+        };
+        if (comp)
+            for (int q : rad_comp) consider(q);
+        else
+            for (int q : rad_base) consider(q);
+    }
+    for (int d = smooth; d > 1; d /= pick[d]) a->radix[a->nst++] = pick[d];
+    for (int p = 7; r > 1; p += 2)
         while (r % p == 0) {
             if (a->nst >= kFftMaxStages) return set_error(SE2P_ERR_UNSUPPORTED, "FFT length %d", n);
             a->radix[a->nst++] = p;
@@ -407,8 +488,9 @@ int fft_prepare()
 {
     static bool attr_set = false;
     if (!attr_set) {
-        const void* kernels[] = {(const void*)k_fft_lines<256, 8>, (const void*)k_fft_lines<256, 10>,
-                                 (const void*)k_fft_lines<512, 8>};
+        const void* kernels[] = {(const void*)k_fft_lines<256, 8, false>,
+                                 (const void*)k_fft_lines<256, 10, true>,
+                                 (const void*)k_fft_lines<512, 8, false>};
         for (const void* k : kernels)
             SE2P_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                kFftMaxLen * (int)sizeof(double2)));
@@ -424,13 +506,13 @@ int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, siz
 {
     if (njobs <= 0) return SE2P_OK;
     if (cls == 0)
-        hipLaunchKernelGGL((k_fft_lines<256, 8>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
+        hipLaunchKernelGGL((k_fft_lines<256, 8, false>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
                            axis, sel, inverse);
     else if (cls == 1)
-        hipLaunchKernelGGL((k_fft_lines<256, 10>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
+        hipLaunchKernelGGL((k_fft_lines<256, 10, true>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
                            axis, sel, inverse);
     else
-        hipLaunchKernelGGL((k_fft_lines<512, 8>), dim3(njobs), dim3(512), max_lds, s, sys, jobs,
+        hipLaunchKernelGGL((k_fft_lines<512, 8, false>), dim3(njobs), dim3(512), max_lds, s, sys, jobs,
                            axis, sel, inverse);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;

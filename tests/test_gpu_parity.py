@@ -111,7 +111,12 @@ def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
 
 
 @pytest.mark.parametrize("My,Mx", [(8, 8), (64, 64), (902, 902), (366, 732), (41, 82), (2112, 16),
-                                   (30, 22), (4096, 4)])
+                                   (30, 22), (4096, 4),
+                                   # lines of 2049..2560 points run composite radices
+                                   # (2160 = 16 15 9, 2560 = 16 16 10, 2250 = 15 15 10,
+                                   # 2304 = 16 16 9, 2400 = 16 15 10, 2106 = 6 9 ... 13)
+                                   (2160, 24), (24, 2560), (2250, 6), (2304, 10), (2400, 12),
+                                   (2106, 4)])
 def test_fft2_matches_numpy(gpu_ctx, My, Mx):
     rng = np.random.default_rng(My + Mx)
     Z = rng.standard_normal((My, Mx)) + 1j * rng.standard_normal((My, Mx))
