@@ -116,7 +116,9 @@ def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
                                    # (2160 = 16 15 9, 2560 = 16 16 10, 2250 = 15 15 10,
                                    # 2304 = 16 16 9, 2400 = 16 15 10, 2106 = 6 9 ... 13)
                                    (2160, 24), (24, 2560), (2250, 6), (2304, 10), (2400, 12),
-                                   (2106, 4)])
+                                   (2106, 4),
+                                   # longer lines (512 threads, radices up to 8)
+                                   (3000, 6), (8, 4096), (3072, 2), (2592, 8)])
 def test_fft2_matches_numpy(gpu_ctx, My, Mx):
     rng = np.random.default_rng(My + Mx)
     Z = rng.standard_normal((My, Mx)) + 1j * rng.standard_normal((My, Mx))
