@@ -96,6 +96,29 @@ def test_spread_matches_oracle(gpu_ctx, oracle, N, M, P, L):
     assert _rel(H1.T, R1) < RTOL and _rel(H2.T, R2) < RTOL
 
 
+@pytest.mark.parametrize("N,Mx,My,P", [(300, 8, 8, 16),    # window over 4 periodic images
+                                       (200, 2, 4, 2),     # the smallest grids
+                                       (400, 10, 6, 40),   # largest support, 7+ images
+                                       (900, 50, 34, 16),  # last blocks partial (not 16 | M)
+                                       (1500, 96, 48, 40),
+                                       (2000, 130, 66, 24)])
+def test_spread_block_edges_match_oracle(gpu_ctx, oracle, N, Mx, My, P):
+    """The block spreading's edge cases: grids narrower than a block or than the
+    window of start nodes (a point reaches a block through several periodic
+    images), partial last blocks, non-square grids, P up to 40."""
+    Lx = 1.0
+    Ly = Lx * My / Mx
+    xi = 4.0 * Mx / 64 + 2.0
+    h = Lx / Mx
+    w = P * h / 2
+    eta = (2 * xi * w / (0.95 * math.sqrt(math.pi * P))) ** 2
+    p = uniform_points(N, Lx, Ly, seed=N + Mx)
+    f = normal_density(N, seed=N + My)
+    H1, H2 = pe.Spread(p.T, f.T, Lx, Ly, xi, w, eta, P, Mx, My)
+    R1, R2 = oracle.spread(p, f, Lx, Ly, xi, w, eta, P, Mx, My)
+    assert _rel(H1.T, R1) < RTOL and _rel(H2.T, R2) < RTOL
+
+
 @pytest.mark.parametrize("N,Mx,My,P,Lx,Ly", [(700, 64, 64, 24, 1.0, 1.0), (900, 92, 46, 16, 2.0, 1.0),
                                               (300, 26, 26, 24, 1.0, 1.0)])
 def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
