@@ -222,6 +222,11 @@ __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__
     d.sp_dens[(size_t)d.Ns + k] = make_double2(d0, d1);
 }
 
+// Each mode and its Hermitian partner (-k) are filtered together from the same
+// two loads: rows iy = 0 .. My/2 of the transposed layout Z[iy*Mx + ix] pair
+// with rows My - iy, and in the rows that are their own partners (0, My/2) the
+// modes ix <= Mx - ix take both.  Each output is the one filter_mode gives for
+// its mode alone.
 template <int KIND>
 __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
 {
@@ -229,21 +234,26 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     const SysDesc& d = sys[jb.x];
     const int Mx = d.Mx, My = d.My;
     const long gid = (long)jb.y * blockDim.x + threadIdx.x;
-    if (gid >= (long)Mx * My) return;
-    // transposed layout Z[iy*Mx + ix]
+    if (gid >= (long)(My / 2 + 1) * Mx) return;
     const int iy = (int)(gid / Mx);
     const int ix = (int)(gid - (long)iy * Mx);
     const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
-    const long gm = (long)jy * Mx + jx;
+    if (jy == iy && jx < ix) return;  // taken with its partner
+    const long gk = gid, gm = (long)jy * Mx + jx;
     const double2* __restrict__ W = d.grid[G_W];
     const double2* __restrict__ D = d.grid[G_D];
     const double2 z = make_double2(0.0, 0.0);
     const bool two_in = dlp_family(KIND);
+    const double2 wk = W[gk], wm = W[gm];
+    const double2 dk = two_in ? D[gk] : z, dm = two_in ? D[gm] : z;
     double2 o0, o1;
-    kf::filter_mode<KIND>(d, ix, iy, W[gid], W[gm], two_in ? D[gid] : z, two_in ? D[gm] : z, o0,
-                          o1);
-    d.grid[G_Z0][gid] = o0;
-    if (out_grids(KIND) == 2) d.grid[G_Z1][gid] = o1;
+    kf::filter_mode<KIND>(d, ix, iy, wk, wm, dk, dm, o0, o1);
+    d.grid[G_Z0][gk] = o0;
+    if (out_grids(KIND) == 2) d.grid[G_Z1][gk] = o1;
+    if (gm == gk) return;
+    kf::filter_mode<KIND>(d, jx, jy, wm, wk, dm, dk, o0, o1);
+    d.grid[G_Z0][gm] = o0;
+    if (out_grids(KIND) == 2) d.grid[G_Z1][gm] = o1;
 }
 
 // Gathering.  One 256-lane workgroup per tile of gTX x gTY start nodes: the

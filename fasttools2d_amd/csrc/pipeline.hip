@@ -288,7 +288,8 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         });
     for (int kind = 0; kind < kNumKinds; ++kind) {
         add(B->filter[kind], [&](int i, const SysDesc& d) {
-            if (geoms[i]->has_k && d.kind == kind) blocks(i, ((long)d.Mx * d.My + 255) / 256);
+            if (geoms[i]->has_k && d.kind == kind)
+                blocks(i, ((long)d.Mx * (d.My / 2 + 1) + 255) / 256);
         });
         for (int tier = 0; tier < 2; ++tier)
             for (int uni = 0; uni < 2; ++uni)

@@ -267,7 +267,8 @@ int launch_spread_records(hipStream_t s, const KGeom& g, const int* perm, const 
 // b*256 .. b*256+255 of system s).
 int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
                        const double* f);
-// Filter: W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
+// Filter (job b: pair modes b*256 .. of rows 0 .. My/2, each with its partner):
+// W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
 // Z0 (and Z1 for kinds with 4 values per target).
 int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
 // Gather of output grid g (W for g = 0, D for g = 1) into values 2g, 2g + 1 of
