@@ -82,11 +82,45 @@ __device__ __forceinline__ double rcp64(double x)
 }
 
 // The two regimes separately: near for x < 1, far (given e = exp(-x), t = 1/x) for x >= 1.
+// ln(x) for finite x > 0 (normal or not): x = m 2^k with m in [sqrt(1/2),
+// sqrt(2)), ln(m) = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, summed as
+// 2 s sum_j s^(2j) / (2j + 1) for j <= 10 (the series' tail is below 3e-18
+// relative), then k ln 2 with ln 2 split into a 32-bit head and a tail (the
+// reduction of W. J. Cody and W. Waite, Software Manual for the Elementary
+// Functions, 1980).  About 25 instructions; within 2 ulp of ln.
+__device__ __forceinline__ double log_pos(double x)
+{
+    int k = __builtin_amdgcn_frexp_exp(x);
+    double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        k -= 1;
+    }
+    const double s = (m - 1.0) * rcp64(m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 21;
+    p = fma(p, z, 1.0 / 19);
+    p = fma(p, z, 1.0 / 17);
+    p = fma(p, z, 1.0 / 15);
+    p = fma(p, z, 1.0 / 13);
+    p = fma(p, z, 1.0 / 11);
+    p = fma(p, z, 1.0 / 9);
+    p = fma(p, z, 1.0 / 7);
+    p = fma(p, z, 1.0 / 5);
+    p = fma(p, z, 1.0 / 3);
+    const double lm = 2.0 * fma(s * z, p, s);
+    const double kd = (double)k;
+    return fma(kd, SE2P_LN2_HI, fma(kd, SE2P_LN2_LO, lm));
+}
+
 template <int TIER>
 __device__ __forceinline__ double e1_near(double x)
 {
     using C = E1Coeffs<TIER>;
-    return (-SE2P_EULER_GAMMA - log(x)) + x * horner(C::near, x);
+    // the fast tier takes the library's own log (within 2 ulp, ~25 instructions
+    // against ~80 for the math library's compensated one)
+    const double lx = TIER == kE1Fast ? log_pos(x) : log(x);
+    return (-SE2P_EULER_GAMMA - lx) + x * horner(C::near, x);
 }
 
 template <int TIER>

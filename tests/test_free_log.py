@@ -1,8 +1,9 @@
-"""The free-space kernel's natural log (fasttools2d_amd/csrc/freespace.hip,
-log_pos): the reduction x = m 2^k, m in [sqrt(1/2), sqrt(2)), and the atanh
-series, restated in numpy and checked against numpy's log over the range the
-kernel sees (squared distances from 1e-300 to 1e300, dense near 1, powers of
-two exact).  The device code itself is pinned by the free-space parity tests."""
+"""The library's natural log (fasttools2d_amd/csrc/e1_eval.h, log_pos; used by
+the free-space sums and the fast tier's E1 near 1/xi): the reduction x = m 2^k,
+m in [sqrt(1/2), sqrt(2)), and the atanh series, restated in numpy and checked
+against numpy's log over the range the kernels see (squared distances from
+1e-300 to 1e300, dense near 1, powers of two exact).  The device code itself is
+pinned by the free-space and real-space parity tests."""
 import numpy as np
 
 LN2_HI = 6.931471803691238164901733e-1
