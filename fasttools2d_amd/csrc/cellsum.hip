@@ -339,7 +339,10 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int s2 = round_index(e_off[e0 + 2 * H], r, nround);
             int L0 = round_index(e_off[e0 + 3], r, nround) - s0;
             int L01 = L0 + round_index(e_off[e0 + H + 3], r, nround) - s1;
-            int n = L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2;
+            // a lane without a target has no candidates, so a wave without
+            // targets (a block's second pass past 256 targets, or its last wave)
+            // skips the candidate tests
+            int n = act ? L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2 : 0;
             int o0 = round_index(e_off[e0 + H + 1], r, nround);
             int o1 = round_index(e_off[e0 + H + 2], r, nround);
             // candidate pos -> staged index: pos + s0, shifted at the two run ends

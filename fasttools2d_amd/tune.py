@@ -105,7 +105,7 @@ def _aspect(Lx, Ly):
 
 
 def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
-             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=3):
+             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=8):
     """Returns (params, report).  ps, pt, f, nrm are (N, 2) host arrays."""
     log = log or (lambda s: None)
     A, B = _aspect(Lx, Ly)
@@ -167,24 +167,36 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         report.update(P=P, c_k=c_k)
         # 4. split sweep, timed
         tier = 1 if tol >= 1e-10 else 0
-        results = []
-        for s in nside_scales:
-            a = max(1, int(round(a0 * s)))
+        results, tried = [], set()
+
+        def timed(a):
+            if a in tried:
+                return
+            tried.add(a)
             try:
                 prm = prm_for(a, c_r, c_k, P, tier)
             except ValueError:
-                continue
+                return
             if prm["Mx"] > 4096 or prm["My"] > 4096:
-                continue
+                return
             u, t = T.run(prm, reps=reps)
             e = rel_err(u, u_ref)
             log(f"tune: nside {prm['nside_x']} xi {prm['xi']:.1f} M {prm['Mx']} P {P} -> "
                 f"{t * 1e3:.3f} ms err {e:.2e}")
-            results.append((t, e, prm))
+            results.append((t, e, prm, a))
+
+        for s in nside_scales:
+            timed(max(1, int(round(a0 * s))))
+        # 5. refine around the fastest split that met the tolerance
+        ok = [r for r in results if r[1] <= tol]
+        if ok:
+            a_best = min(ok, key=lambda r: r[0])[3]
+            for s in (0.88, 0.94, 1.06, 1.12):
+                timed(max(1, int(round(a_best * s))))
         ok = [r for r in results if r[1] <= tol]
         if not ok:
             ok = sorted(results, key=lambda r: r[1])[:1]
-        t, e, prm = min(ok, key=lambda r: r[0])
+        t, e, prm, _ = min(ok, key=lambda r: r[0])
         report.update(ms=t * 1e3, err=e, sweep=[(r[2]["nside_x"], r[2]["Mx"], r[0] * 1e3, r[1])
                                                 for r in results])
         return prm, report
