@@ -146,7 +146,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             c_r = c_exact_r
         report["c_r"] = c_r
         # 3. k-space constants: cheapest (P, c_k) meeting tol/3 by a cost model
-        best = None
+        best, kcands = None, []
         for P in (8, 10, 12, 14, 16, 18, 20, 24):
             for ck in (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0):
                 prm = prm_for(a0, c_exact_r, ck, P)
@@ -158,6 +158,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                     G = prm["Mx"] * prm["My"]
                     cost = (Ns + Nt) * (P + 1) ** 2 * 2.0 + 25.0 * G * math.log2(G)
                     log(f"tune: P {P} c_k {ck:.1f} M {prm['Mx']} -> err {e:.2e} cost {cost:.3e}")
+                    kcands.append((cost, P, ck))
                     if best is None or cost < best[0]:
                         best = (cost, P, ck)
                     break
@@ -169,10 +170,10 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         tier = 1 if tol >= 1e-10 else 0
         results, tried = [], set()
 
-        def timed(a):
-            if a in tried:
+        def timed(a, P=P, c_k=c_k):
+            if (a, P) in tried:
                 return
-            tried.add(a)
+            tried.add((a, P))
             try:
                 prm = prm_for(a, c_r, c_k, P, tier)
             except ValueError:
@@ -193,11 +194,17 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             a_best = min(ok, key=lambda r: r[0])[3]
             for s in (0.88, 0.94, 1.06, 1.12):
                 timed(max(1, int(round(a_best * s))))
+        # 6. the next cheapest supports by the cost model, timed at the best split
+        ok = [r for r in results if r[1] <= tol]
+        if ok:
+            a_best = min(ok, key=lambda r: r[0])[3]
+            for _, P2, ck2 in sorted(kcands)[1:3]:
+                timed(a_best, P2, ck2)
         ok = [r for r in results if r[1] <= tol]
         if not ok:
             ok = sorted(results, key=lambda r: r[1])[:1]
         t, e, prm, _ = min(ok, key=lambda r: r[0])
-        report.update(ms=t * 1e3, err=e, sweep=[(r[2]["nside_x"], r[2]["Mx"], r[0] * 1e3, r[1])
+        report.update(ms=t * 1e3, err=e, P_timed=prm["P"], sweep=[(r[2]["nside_x"], r[2]["Mx"], r[0] * 1e3, r[1])
                                                 for r in results])
         return prm, report
     finally:
