@@ -208,16 +208,20 @@ __global__ __launch_bounds__(256) void k_sort_big(int* __restrict__ perm,
 
 // Reorders each cell's indices along a Morton curve over a 16 x 16 sub-grid of
 // the cell (ties keep index order), so consecutive runs of 8 points are
-// spatially compact.  Cells larger than kSortCap keep index order.
+// spatially compact -- or, with yorder, by y to 2^-20 of the cell height, so a
+// column of stacked cells is ordered by y.  Cells larger than kSortCap keep
+// index order (with yorder they, and points outside their cell, set *unsorted).
 __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
                                                       const int* __restrict__ start,
                                                       const double* __restrict__ p, double Lx,
-                                                      double Ly, int nsx, int nsy)
+                                                      double Ly, int nsx, int nsy, int yorder,
+                                                      int* __restrict__ unsorted)
 {
     __shared__ unsigned key[kSortCap];
     __shared__ int old[kSortCap];
     const int b = blockIdx.x;
     const int lo = start[b], size = start[b + 1] - lo;
+    if (size > kSortCap && yorder && threadIdx.x == 0) *unsorted = 1;
     if (size <= 1 || size > kSortCap) return;
     const int bx = b % nsx, by = b / nsx;
     int np = 2;
@@ -228,11 +232,20 @@ __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
             old[i] = o;
             double fx = (p[2 * o] / Lx + 0.5) * nsx - bx;
             double fy = (p[2 * o + 1] / Ly + 0.5) * nsy - by;
-            int qx = min(15, max(0, (int)(fx * 16))), qy = min(15, max(0, (int)(fy * 16)));
-            unsigned code = 0;
-            for (int k = 0; k < 4; ++k)
-                code |= (((qx >> k) & 1u) << (2 * k)) | (((qy >> k) & 1u) << (2 * k + 1));
-            key[i] = (code << 12) | (unsigned)i;
+            if (yorder) {
+                // y inside the cell to 2^-20 of its height; a point outside its
+                // cell (outside the box) leaves the cell's order unusable for
+                // trimming by y
+                if (!(fy >= 0.0 && fy < 1.0)) *unsorted = 1;
+                const unsigned qy = (unsigned)min(1048575, max(0, (int)(fy * 1048576.0)));
+                key[i] = (qy << 12) | (unsigned)i;
+            } else {
+                int qx = min(15, max(0, (int)(fx * 16))), qy = min(15, max(0, (int)(fy * 16)));
+                unsigned code = 0;
+                for (int k = 0; k < 4; ++k)
+                    code |= (((qx >> k) & 1u) << (2 * k)) | (((qy >> k) & 1u) << (2 * k + 1));
+                key[i] = (code << 12) | (unsigned)i;
+            }
         } else {
             key[i] = 0xFFFFFFFFu;
         }
@@ -329,10 +342,10 @@ int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n,
 }
 
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
-                        double Lx, double Ly, int nsx, int nsy)
+                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted)
 {
     hipLaunchKernelGGL(k_morton_cells, dim3(nbox), dim3(256), 0, s, perm, start, p, Lx, Ly, nsx,
-                       nsy);
+                       nsy, yorder, unsorted);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

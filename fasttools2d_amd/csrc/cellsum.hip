@@ -300,6 +300,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int ntar = __builtin_amdgcn_readfirstlane(c_off[ncell]);
     const double ixi2 = 1.0 / xi2;
     const int nround = (total + CAP - 1) / CAP;
+    const double trim_w = d.trim_w;
+    const bool trim = !UNI && trim_w > 0 && d.unsorted && *d.unsorted == 0;
 
     for (int tb = 0; tb < ntar; tb += kBlkThreads) {
         const int i = tb + tid;
@@ -335,8 +337,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             // 0 .. n-1 in that order; the own cell (entry (cx + 1) H + cy + 1) is [o0, o1)
             const int e0 = cx * H + cy;
             int s0 = round_index(e_off[e0], r, nround);
-            const int s1 = round_index(e_off[e0 + H], r, nround);
-            const int s2 = round_index(e_off[e0 + 2 * H], r, nround);
+            int s1 = round_index(e_off[e0 + H], r, nround);
+            int s2 = round_index(e_off[e0 + 2 * H], r, nround);
             int L0 = round_index(e_off[e0 + 3], r, nround) - s0;
             int L01 = L0 + round_index(e_off[e0 + H + 3], r, nround) - s1;
             // a lane without a target has no candidates, so a wave without
@@ -345,6 +347,35 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             int n = act ? L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2 : 0;
             int o0 = round_index(e_off[e0 + H + 1], r, nround);
             int o1 = round_index(e_off[e0 + H + 2], r, nround);
+            if (trim && act) {
+                // Each run is ordered by y (sources by y inside their cells, the
+                // cells of a column stacked, the periodic shift staged with y):
+                // keep the part within trim_w of the target's y, found by binary
+                // search.  Every pair the sums take lies in it.
+                const double ylo = pt.y - trim_w, yhi = pt.y + trim_w;
+                auto first_at = [&](int a, int b, double v) {  // first of [a, b) with y >= v
+                    while (a < b) {
+                        const int mid = (a + b) >> 1;
+                        if (buf[mid].y < v)
+                            a = mid + 1;
+                        else
+                            b = mid;
+                    }
+                    return a;
+                };
+                const int e0b = s0 + L0, e1b = s1 + (L01 - L0), e2b = s2 + (n - L01);
+                const int a0 = first_at(s0, e0b, ylo), b0 = first_at(a0, e0b, yhi);
+                const int a1 = first_at(s1, e1b, ylo), b1 = first_at(a1, e1b, yhi);
+                const int a2 = first_at(s2, e2b, ylo), b2 = first_at(a2, e2b, yhi);
+                s0 = a0;
+                s1 = a1;
+                s2 = a2;
+                L0 = b0 - a0;
+                L01 = L0 + (b1 - a1);
+                n = L01 + (b2 - a2);
+                o0 = min(max(o0, a1), b1);
+                o1 = min(max(o1, o0), b1);
+            }
             // candidate pos -> staged index: pos + s0, shifted at the two run ends
             int j1 = s1 - L0 - s0, j2 = s2 - L01 - (s1 - L0);
             if (UNI) {

@@ -78,6 +78,8 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->s_rec = (double*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_TAR_SORTED, sizeof(double2) * (size_t)Nt, &p));
         G->t_sorted = (double2*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_UNSORTED, sizeof(int), &p));
+        G->unsorted = (int*)p;
         StageTimer st(ctx, "bin");
         SE2P_TRY(launch_fill_int(s, (int*)scount, nbox, 0));
         SE2P_TRY(launch_cell_keys(s, d_psrc, Ns, Lx, Ly, nsx, nsy, (int*)skey, (int*)scount));
@@ -87,8 +89,13 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, nbox, G->t_perm, G->t_start,
                              WS_SORT_TMP));
-        SE2P_TRY(launch_morton_cells(s, G->s_perm, G->s_start, nbox, d_psrc, Lx, Ly, nsx, nsy));
-        SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx, nsy));
+        // sources by y inside their cells (each stencil column is then sorted by
+        // y), targets along a Morton curve (neighbouring lanes, similar stencils)
+        SE2P_TRY(launch_fill_int(s, G->unsorted, 1, 0));
+        SE2P_TRY(launch_morton_cells(s, G->s_perm, G->s_start, nbox, d_psrc, Lx, Ly, nsx, nsy, 1,
+                                     G->unsorted));
+        SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx, nsy, 0,
+                                     G->unsorted));
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
@@ -193,6 +200,11 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->nbx = (nsx + d->bs - 1) / d->bs;
         d->xi2 = xi * xi;
         d->cut2 = G.Lx * G.Ly / nsx / nsy;
+        // every neighbour candidate within the cutoff and every own-cell source
+        // lies within max(rc, cell height) in y; the margin covers the 2^-20
+        // resolution of the in-cell y order
+        d->trim_w = std::max(std::sqrt(d->cut2), G.Ly / nsy) * (1.0 + 1e-5);
+        d->unsorted = G.unsorted;
         // -(1 + gamma/2 + ln xi) (mex_stokes_slp_real.cpp:124)
         d->self = -(1.0 + 0.5772156649015328606 / 2 + log(xi));
         d->t_start = G.t_start;

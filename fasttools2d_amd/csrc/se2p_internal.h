@@ -91,7 +91,7 @@ enum Slot {
     WS_TAR_KEY, WS_TAR_PERM, WS_TAR_START, WS_TAR_COUNT,
     WS_SRC_SORTED, WS_TAR_SORTED,
     WS_SPR_KEY, WS_SPR_PERM, WS_SPR_START, WS_SPR_COUNT, WS_SPR_NODE, WS_SPR_OFF,
-    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS,
+    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED,
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
@@ -167,9 +167,10 @@ int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly
 int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n, int nkeys,
                 int* perm, int* start, Slot tmp_slot);
 int launch_fill_int(hipStream_t s, int* p, int n, int v);
-// Morton order inside each cell of a cell-sorted permutation (real-space clusters).
+// Morton order (yorder 0) or y order (yorder 1; sets *unsorted where a cell cannot be
+// ordered) inside each cell of a cell-sorted permutation.
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
-                        double Lx, double Ly, int nsx, int nsy);
+                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted);
 // FindClosestNode for n points, unwrapped start nodes.
 int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, double Ly,
                          double h, int P, int* mx, int* my, double* px, double* py);
@@ -218,6 +219,10 @@ struct SysDesc {
     // real space
     int nsx, nsy, bs, nbx;
     double xi2, cut2, self;
+    // half-width in y of the candidates a target keeps from each stencil column
+    // (sources y-ordered in their cells); unusable while *unsorted != 0
+    double trim_w;
+    const int* unsorted;
     const int *t_start, *t_perm, *s_start, *s_perm;
     const double2* t_sorted;
     double* s_rec;
@@ -321,6 +326,7 @@ struct Geom {
     double2* t_sorted = nullptr;
     double* s_rec = nullptr;
     double *zm = nullptr, *zm_part = nullptr;
+    int* unsorted = nullptr;  // set by the source ordering when a cell is not y-ordered
     // k space
     KGeom kg{};
     FftAxis ax{}, ay{};
