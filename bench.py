@@ -490,8 +490,11 @@ def main():
     replay = lambda: plan.apply_device(df.ptr, du.ptr, graph=True)  # noqa: E731
     # launch mode: stream launches or the captured graph, whichever a short
     # comparison finds faster; the timed window then runs that mode alone
-    pre = {"streams": time_steps(ctx, apply, 10, args.warmup),
-           "graph": time_steps(ctx, replay, 10, args.warmup)}
+    pre = {"streams": 0.0, "graph": 0.0}
+    time_steps(ctx, replay, 1, args.warmup)  # capture the graph
+    for _ in range(3):  # alternating, so a drift of the clock hits both alike
+        pre["streams"] += time_steps(ctx, apply, 10, 2)
+        pre["graph"] += time_steps(ctx, replay, 10, 2)
     mode = min(pre, key=pre.get)
     run = replay if mode == "graph" else apply
     ms_tuned = time_steps(ctx, run, args.steps, args.warmup) * 1e3

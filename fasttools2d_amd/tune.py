@@ -7,12 +7,14 @@ method and measures instead:
 
 1. an accurate reference solution u_ref (reference recipe at tol 1e-13, P = 24);
 2. calibration of the real-space constant c_r = xi * rc: the smallest c_r whose
-   real-space truncation alone stays below tol/3 (k space kept exact);
+   real-space truncation alone stays below tol/2 (k space kept exact);
 3. calibration of the k-space constants: k_inf / xi (giving M) and the Gaussian
-   support P, the cheapest pair whose error alone stays below tol/3;
+   support P, the cheapest pair whose error alone stays below tol/2;
 4. a sweep over the split (nside, hence rc, xi = c_r / rc and M) timed with the
-   plan path; the fastest configuration whose measured error against u_ref is
-   below tol wins.
+   plan path, refined around its fastest point, then the next supports at that
+   split; the fastest configuration whose measured error against u_ref is below
+   0.8 tol wins (the two halves add to at most tol; the 20% margin covers what
+   the split changes in them).
 
 Errors are relative L2 norms over all targets.  Timings use se2p_plan_apply on
 device-resident inputs.
@@ -135,17 +137,17 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
 
         # 2. real-space constant
         c_r = None
-        for c in (3.0, 3.3, 3.6, 3.9, 4.2, 4.6, 5.0, 5.5):
+        for c in (3.0, 3.3, 3.5, 3.6, 3.7, 3.8, 3.9, 4.0, 4.2, 4.6, 5.0, 5.5):
             u, _ = T.run(prm_for(a0, c, c_exact_k, P_exact, 1 if tol >= 1e-10 else 0))
             e = rel_err(u, u_ref)
             log(f"tune: c_r {c:.2f} -> err {e:.2e}")
-            if e < tol / 3:
+            if e < tol / 2:
                 c_r = c
                 break
         if c_r is None:
             c_r = c_exact_r
         report["c_r"] = c_r
-        # 3. k-space constants: cheapest (P, c_k) meeting tol/3 by a cost model
+        # 3. k-space constants: cheapest (P, c_k) meeting tol/2 by a cost model
         best, kcands = None, []
         for P in (8, 10, 12, 14, 16, 18, 20, 24):
             for ck in (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0):
@@ -154,7 +156,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                     continue
                 u, _ = T.run(prm)
                 e = rel_err(u, u_ref)
-                if e < tol / 3:
+                if e < tol / 2:
                     G = prm["Mx"] * prm["My"]
                     cost = (Ns + Nt) * (P + 1) ** 2 * 2.0 + 25.0 * G * math.log2(G)
                     log(f"tune: P {P} c_k {ck:.1f} M {prm['Mx']} -> err {e:.2e} cost {cost:.3e}")
@@ -189,18 +191,18 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         for s in nside_scales:
             timed(max(1, int(round(a0 * s))))
         # 5. refine around the fastest split that met the tolerance
-        ok = [r for r in results if r[1] <= tol]
+        ok = [r for r in results if r[1] <= 0.8 * tol]
         if ok:
             a_best = min(ok, key=lambda r: r[0])[3]
             for s in (0.88, 0.94, 1.06, 1.12):
                 timed(max(1, int(round(a_best * s))))
         # 6. the next cheapest supports by the cost model, timed at the best split
-        ok = [r for r in results if r[1] <= tol]
+        ok = [r for r in results if r[1] <= 0.8 * tol]
         if ok:
             a_best = min(ok, key=lambda r: r[0])[3]
             for _, P2, ck2 in sorted(kcands)[1:3]:
                 timed(a_best, P2, ck2)
-        ok = [r for r in results if r[1] <= tol]
+        ok = [r for r in results if r[1] <= 0.8 * tol]
         if not ok:
             ok = sorted(results, key=lambda r: r[1])[:1]
         t, e, prm, _ = min(ok, key=lambda r: r[0])
