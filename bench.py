@@ -230,7 +230,8 @@ def profile_only(ctx, dev, ps, p, f, args):
         return
     q = json.loads(args.params)
     xi = q["c_r"] * q["nside"]
-    prm = make_params(xi, q["nside"], q["nside"], q["M"], q["M"], q["P"], 1.0, q.get("tier", 1))
+    prm = make_params(xi, q["nside"], q["nside"], q["M"], q["M"], q["P"], 1.0, q.get("tier", 1),
+                      q.get("symmetric", 0))
     plan = Plan(ctx, "slp", p, p, 1.0, 1.0, prm)
     df = DeviceArray.from_host(ctx, f)
     du = DeviceArray(ctx, (len(p), 2))

@@ -95,7 +95,8 @@ class Params(ctypes.Structure):
     _fields_ = [("xi", ctypes.c_double), ("rc", ctypes.c_double), ("w", ctypes.c_double),
                 ("eta", ctypes.c_double), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
-                ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int)]
+                ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int),
+                ("symmetric", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -29,9 +29,12 @@ class Batch:
             for i, y in enumerate(systems):
                 kind = _lib.kind_code(y.get("kind", "slp"))
                 dps = DeviceArray.from_host(ctx, y["psrc"])
-                dpt = DeviceArray.from_host(ctx, y["ptar"])
+                # the same array for sources and targets: one upload, so the
+                # library sees one pointer (pair-symmetric sums when asked for)
+                same = y["ptar"] is y["psrc"]
+                dpt = dps if same else DeviceArray.from_host(ctx, y["ptar"])
                 dn = DeviceArray.from_host(ctx, y["nrm"]) if _lib.has_normals(kind) else None
-                tmp += [a for a in (dps, dpt, dn) if a is not None]
+                tmp += [a for a in (dps, None if same else dpt, dn) if a is not None]
                 prm = y["prm"]
                 arr[i] = System(kind, self.ns[i], self.nt[i], float(y["Lx"]), float(y["Ly"]),
                                 dps.ptr, dpt.ptr, dn.ptr if dn else None,

@@ -215,14 +215,16 @@ __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
                                                       const int* __restrict__ start,
                                                       const double* __restrict__ p, double Lx,
                                                       double Ly, int nsx, int nsy, int yorder,
-                                                      int* __restrict__ unsorted)
+                                                      int* __restrict__ unsorted,
+                                                      int* __restrict__ ykey)
 {
     __shared__ unsigned key[kSortCap];
     __shared__ int old[kSortCap];
     const int b = blockIdx.x;
     const int lo = start[b], size = start[b + 1] - lo;
     if (size > kSortCap && yorder && threadIdx.x == 0) *unsorted = 1;
-    if (size <= 1 || size > kSortCap) return;
+    // a single point is in order, but its y key is still wanted
+    if (size > kSortCap || size == 0 || (size == 1 && !(yorder && ykey))) return;
     const int bx = b % nsx, by = b / nsx;
     int np = 2;
     while (np < size) np <<= 1;
@@ -267,7 +269,10 @@ __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < size; i += blockDim.x) perm[lo + i] = old[key[i] & 0xFFFu];
+    for (int i = threadIdx.x; i < size; i += blockDim.x) {
+        perm[lo + i] = old[key[i] & 0xFFFu];
+        if (yorder && ykey) ykey[lo + i] = (int)(key[i] >> 12);
+    }
 }
 
 inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
@@ -342,10 +347,11 @@ int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n,
 }
 
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
-                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted)
+                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted,
+                        int* ykey)
 {
     hipLaunchKernelGGL(k_morton_cells, dim3(nbox), dim3(256), 0, s, perm, start, p, Lx, Ly, nsx,
-                       nsy, yorder, unsorted);
+                       nsy, yorder, unsorted, ykey);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -190,6 +190,8 @@ int check_params(const se2p_params* prm, double Lx, double Ly)
     SE2P_TRY(check_kspace(prm->Mx, prm->My, Lx, Ly, prm->xi, prm->eta, prm->w, prm->P));
     if (prm->e1_tier != 0 && prm->e1_tier != 1)
         return set_error(SE2P_ERR_ARG, "e1_tier must be 0 or 1");
+    if (prm->symmetric != 0 && prm->symmetric != 1)
+        return set_error(SE2P_ERR_ARG, "symmetric must be 0 or 1");
     return SE2P_OK;
 }
 
@@ -359,6 +361,7 @@ int params_ref(int kind, int Ns, int Nt, double Q, double Lx, double Ly, const s
     out->eta = std::pow(2 * xi * out->w / m, 2);
     out->converged = ok1 && ok2;
     out->e1_tier = 0;  // the reference evaluates E1 to full FP64 accuracy
+    out->symmetric = 0;
     if (o->verbose)
         fprintf(stderr,
                 "se2p %s: xi %.6g rc %.6g kinf %d Mx %d My %d nside %d x %d w %.6g eta %.6g\n",

@@ -89,6 +89,27 @@ __device__ __forceinline__ void slp_pair(double dx, double dy, double r2, double
     a2 += 0.5 * E * f2 - c * dx * cr;
 }
 
+// slp_pair for a pair evaluated once for both points: (a1, a2) += the term for
+// density f at the target, (b1, b2) = the term for density g at the source (the
+// pair term is even in r).
+template <int TIER, bool NEAR>
+__device__ __forceinline__ void slp_pair2(double dx, double dy, double r2, double xi2,
+                                          double ixi2, double f1, double f2, double g1,
+                                          double g2, double& a1, double& a2, double& b1,
+                                          double& b2)
+{
+    const double ir2 = rcp64(r2);
+    const double x = xi2 * r2;
+    const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
+    const double hE = 0.5 * (NEAR ? e1_near<TIER>(x) : e1_far<TIER>(e, ir2 * ixi2));
+    const double c = e * ir2;
+    const double cf = dx * f2 - dy * f1, cg = dx * g2 - dy * g1;
+    a1 += hE * f1 + c * dy * cf;
+    a2 += hE * f2 - c * dx * cf;
+    b1 = hE * g1 + c * dy * cg;
+    b2 = hE * g2 - c * dx * cg;
+}
+
 // 4 pi times the DLP pair term for r = target - source:
 //   e^{-xi^2 r^2} [ -4 (1 + xi^2 r^2)/r^4 r (r.f)(r.n) + 2 xi^2 (f (r.n) + n (r.f) + r (f.n)) ]
 __device__ __forceinline__ void dlp_pair(double r1, double r2, double rr, double xi2,
@@ -245,13 +266,29 @@ __device__ __forceinline__ int round_index(int g, int r, int nround)
 
 // UNI: blocks of one cell (bs = 1), where every lane has the same stencil and
 // the candidate bookkeeping is wave-uniform (scalar registers, LDS broadcast).
-template <int KIND, int TIER, bool UNI>
+//
+// SYM (Stokeslet, sources = targets in one y order): each pair is evaluated
+// once, by the point the other lies "after": the source's image in the stencil
+// row above, or in the same row with a larger in-cell y key (ties: the image in
+// the column to the right; in the own cell: later in the sorted order).  This
+// order is antisymmetric for every pair of images, so each pair is taken once.
+// A target's candidates are then, per stencil column, the part of the row-y
+// cell after its key (own cell: after itself) and the row above up to y +
+// trim_w.  The window holds rows 0 .. bh only; the term for the source is
+// accumulated in LDS (racc) and added after each round, the target's own sum
+// after the last round, with atomics into sacc, which follows the sorted order
+// (so a cell's atomics share cache lines); k_sym_out writes u from it.
+template <int KIND, int TIER, bool UNI, bool SYM = false>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_real_stencil(
     const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, double* __restrict__ u_base)
 {
+    static_assert(!SYM || (KIND == SE2P_SLP && !UNI), "pair-symmetric sums: Stokeslet blocks");
     using Rec = typename SrcOf<KIND>::T;
-    constexpr int CAP = SrcOf<KIND>::cap;
+    // with the reflected sums and keys, 4 workgroups per CU still fit in LDS
+    constexpr int CAP = SYM ? 768 : SrcOf<KIND>::cap;
     __shared__ Rec buf[CAP];
+    __shared__ double2 racc[SYM ? CAP : 1];
+    __shared__ int rkey[SYM ? CAP : 1];
     __shared__ int e_src[kMaxEntries], e_off[kMaxEntries + 1];
     __shared__ double e_ox[kMaxEntries], e_oy[kMaxEntries];
     __shared__ int c_t0[kMaxBlockSide * kMaxBlockSide], c_off[kMaxBlockSide * kMaxBlockSide + 1];
@@ -269,10 +306,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int tid = threadIdx.x;
     const int bx0 = (jb.y % nbx) * bs, by0 = (jb.y / nbx) * bs;
     const int bw = min(bs, nsx - bx0), bh = min(bs, nsy - by0);
-    const int H = bh + 2;  // window column height; entry (ux, uy) is (ux+1) H + (uy+1)
+    // window column height; entry (ux, uy) is (ux+1) H + (uy+1), or (ux+1) H + uy
+    // for SYM, whose window starts at row 0
+    const int H = SYM ? bh + 1 : bh + 2;
     const int nent = (bw + 2) * H, ncell = bw * bh;
     if (tid < nent) {
-        const int ux = tid / H - 1, uy = tid % H - 1;
+        const int ux = tid / H - 1, uy = tid % H - (SYM ? 0 : 1);
         const int qx = bx0 + ux, qy = by0 + uy;
         const int wx = wrapi(qx, nsx), wy = wrapi(qy, nsy);
         const int cell = wy * nsx + wx;
@@ -306,15 +345,24 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     for (int tb = 0; tb < ntar; tb += kBlkThreads) {
         const int i = tb + tid;
         const bool act = i < ntar;
-        int ti = 0, cx = 0, cy = 0;
+        int ti = 0, cx = 0, cy = 0, kin = 0;
         double2 pt = make_double2(0.0, 0.0);
+        double fi1 = 0, fi2 = 0;  // SYM: the target's density and y key
+        int qyi = 0;
         if (act) {
             int c = 0;
             while (c + 1 < ncell && c_off[c + 1] <= i) ++c;
-            ti = c_t0[c] + (i - c_off[c]);
+            kin = i - c_off[c];
+            ti = c_t0[c] + kin;
             cx = c / bh;
             cy = c % bh;
             pt = tsorted[ti];
+            if constexpr (SYM) {
+                const double4 q = ssorted[ti];
+                fi1 = q.z;
+                fi2 = q.w;
+                qyi = d.ykey[ti];
+            }
         }
         constexpr int NC = out_comps(KIND);
         double a1 = 0, a2 = 0;                // velocity and pressure kinds
@@ -326,33 +374,25 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 const int g = r + k * nround;
                 int e = 0;
                 while (e_off[e + 1] <= g) ++e;
-                Rec rec = ssorted[e_src[e] + (g - e_off[e])];
+                const int sk = e_src[e] + (g - e_off[e]);
+                Rec rec = ssorted[sk];
                 rec.x += e_ox[e];
                 rec.y += e_oy[e];
                 buf[k] = rec;
+                if constexpr (SYM) {
+                    rkey[k] = d.ykey[sk];
+                    racc[k] = make_double2(0.0, 0.0);
+                }
             }
             __syncthreads();
             // this lane's candidates: staged runs [s_j, s_j + L_j), j = 0, 1, 2 (the
             // stencil columns: window entries (cx + j) H + cy .. + 2), numbered
             // 0 .. n-1 in that order; the own cell (entry (cx + 1) H + cy + 1) is [o0, o1)
             const int e0 = cx * H + cy;
-            int s0 = round_index(e_off[e0], r, nround);
-            int s1 = round_index(e_off[e0 + H], r, nround);
-            int s2 = round_index(e_off[e0 + 2 * H], r, nround);
-            int L0 = round_index(e_off[e0 + 3], r, nround) - s0;
-            int L01 = L0 + round_index(e_off[e0 + H + 3], r, nround) - s1;
-            // a lane without a target has no candidates, so a wave without
-            // targets (a block's second pass past 256 targets, or its last wave)
-            // skips the candidate tests
-            int n = act ? L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2 : 0;
-            int o0 = round_index(e_off[e0 + H + 1], r, nround);
-            int o1 = round_index(e_off[e0 + H + 2], r, nround);
-            if (trim && act) {
-                // Each run is ordered by y (sources by y inside their cells, the
-                // cells of a column stacked, the periodic shift staged with y):
-                // keep the part within trim_w of the target's y, found by binary
-                // search.  Every pair the sums take lies in it.
-                const double ylo = pt.y - trim_w, yhi = pt.y + trim_w;
+            int s0, s1, s2, L0, L01, n, o0, o1;
+            if constexpr (SYM) {
+                // column j: its row-y cell is entry e0 + j H, the row above e0 + j H + 1
+                const double yhi = pt.y + trim_w;
                 auto first_at = [&](int a, int b, double v) {  // first of [a, b) with y >= v
                     while (a < b) {
                         const int mid = (a + b) >> 1;
@@ -363,18 +403,72 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     }
                     return a;
                 };
-                const int e0b = s0 + L0, e1b = s1 + (L01 - L0), e2b = s2 + (n - L01);
-                const int a0 = first_at(s0, e0b, ylo), b0 = first_at(a0, e0b, yhi);
-                const int a1 = first_at(s1, e1b, ylo), b1 = first_at(a1, e1b, yhi);
-                const int a2 = first_at(s2, e2b, ylo), b2 = first_at(a2, e2b, yhi);
-                s0 = a0;
-                s1 = a1;
-                s2 = a2;
-                L0 = b0 - a0;
-                L01 = L0 + (b1 - a1);
-                n = L01 + (b2 - a2);
-                o0 = min(max(o0, a1), b1);
-                o1 = min(max(o1, o0), b1);
+                auto first_key = [&](int a, int b, int v) {  // first of [a, b) with key >= v
+                    while (a < b) {
+                        const int mid = (a + b) >> 1;
+                        if (rkey[mid] < v)
+                            a = mid + 1;
+                        else
+                            b = mid;
+                    }
+                    return a;
+                };
+                const int eo = e0 + H;
+                const int w1 = round_index(e_off[e0 + 1], r, nround);
+                const int m1 = round_index(e_off[eo + 1], r, nround);
+                const int q1 = round_index(e_off[e0 + 2 * H + 1], r, nround);
+                s0 = first_key(round_index(e_off[e0], r, nround), w1, qyi + 1);
+                s1 = round_index(e_off[eo] + kin + 1, r, nround);
+                s2 = first_key(round_index(e_off[e0 + 2 * H], r, nround), q1, qyi);
+                const int b0 = first_at(w1, round_index(e_off[e0 + 2], r, nround), yhi);
+                const int b1 = first_at(m1, round_index(e_off[eo + 2], r, nround), yhi);
+                const int b2 = first_at(q1, round_index(e_off[e0 + 2 * H + 2], r, nround), yhi);
+                L0 = b0 - s0;
+                L01 = L0 + (b1 - s1);
+                n = act ? L01 + (b2 - s2) : 0;
+                o0 = s1;  // the own cell after the target
+                o1 = m1;
+            } else {
+                s0 = round_index(e_off[e0], r, nround);
+                s1 = round_index(e_off[e0 + H], r, nround);
+                s2 = round_index(e_off[e0 + 2 * H], r, nround);
+                L0 = round_index(e_off[e0 + 3], r, nround) - s0;
+                L01 = L0 + round_index(e_off[e0 + H + 3], r, nround) - s1;
+                // a lane without a target has no candidates, so a wave without
+                // targets (a block's second pass past 256 targets, or its last wave)
+                // skips the candidate tests
+                n = act ? L01 + round_index(e_off[e0 + 2 * H + 3], r, nround) - s2 : 0;
+                o0 = round_index(e_off[e0 + H + 1], r, nround);
+                o1 = round_index(e_off[e0 + H + 2], r, nround);
+                if (trim && act) {
+                    // Each run is ordered by y (sources by y inside their cells, the
+                    // cells of a column stacked, the periodic shift staged with y):
+                    // keep the part within trim_w of the target's y, found by binary
+                    // search.  Every pair the sums take lies in it.
+                    const double ylo = pt.y - trim_w, yhi = pt.y + trim_w;
+                    auto first_at = [&](int a, int b, double v) {  // first of [a, b) with y >= v
+                        while (a < b) {
+                            const int mid = (a + b) >> 1;
+                            if (buf[mid].y < v)
+                                a = mid + 1;
+                            else
+                                b = mid;
+                        }
+                        return a;
+                    };
+                    const int e0b = s0 + L0, e1b = s1 + (L01 - L0), e2b = s2 + (n - L01);
+                    const int a0 = first_at(s0, e0b, ylo), b0 = first_at(a0, e0b, yhi);
+                    const int a1 = first_at(s1, e1b, ylo), b1 = first_at(a1, e1b, yhi);
+                    const int a2 = first_at(s2, e2b, ylo), b2 = first_at(a2, e2b, yhi);
+                    s0 = a0;
+                    s1 = a1;
+                    s2 = a2;
+                    L0 = b0 - a0;
+                    L01 = L0 + (b1 - a1);
+                    n = L01 + (b2 - a2);
+                    o0 = min(max(o0, a1), b1);
+                    o1 = min(max(o1, o0), b1);
+                }
             }
             // candidate pos -> staged index: pos + s0, shifted at the two run ends
             int j1 = s1 - L0 - s0, j2 = s2 - L01 - (s1 - L0);
@@ -466,8 +560,31 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     return c + k * nch;
                 };
                 unsigned long long lo = fm[0], hi = fm[1];
+                if constexpr (SYM) {
+                    // the next hit's record is read before this hit's LDS atomics
+                    // are issued, so waiting for it does not wait for them
+                    bool more = (lo | hi) != 0;
+                    int idx = more ? staged(next(lo, hi)) : 0;
+                    double4 q = buf[idx];
+                    while (more) {
+                        more = (lo | hi) != 0;
+                        const int nidx = more ? staged(next(lo, hi)) : idx;
+                        const double4 qn = buf[nidx];
+                        const double dx = pt.x - q.x, dy = pt.y - q.y;
+                        const double r2 = dx * dx + dy * dy;
+                        double b1, b2;
+                        slp_pair2<TIER, false>(dx, dy, r2, xi2, ixi2, q.z, q.w, fi1, fi2, a1, a2,
+                                               b1, b2);
+                        atomicAdd(&racc[idx].x, b1);
+                        atomicAdd(&racc[idx].y, b2);
+                        idx = nidx;
+                        q = qn;
+                    }
+                    lo = hi = 0;
+                }
                 while (lo | hi) {
-                    const Rec& s = buf[staged(next(lo, hi))];
+                    const int idx = staged(next(lo, hi));
+                    const Rec& s = buf[idx];
                     const double dx = pt.x - s.x, dy = pt.y - s.y;
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
@@ -495,9 +612,34 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         if (idx >= o0 && idx < o1 && fabs(r2) < 1e-15) {
                             a1 += self * q.z;
                             a2 += self * q.w;
+                            if (SYM) {
+                                atomicAdd(&racc[idx].x, self * fi1);
+                                atomicAdd(&racc[idx].y, self * fi2);
+                            }
+                        } else if (SYM) {
+                            double b1, b2;
+                            slp_pair2<TIER, true>(dx, dy, r2, xi2, ixi2, q.z, q.w, fi1, fi2, a1,
+                                                  a2, b1, b2);
+                            atomicAdd(&racc[idx].x, b1);
+                            atomicAdd(&racc[idx].y, b2);
                         } else {
                             slp_pair<TIER, true>(dx, dy, r2, xi2, ixi2, q.z, q.w, a1, a2);
                         }
+                    }
+                }
+            }
+            if constexpr (SYM) {
+                // the sources' sums of this round, before the next round restages
+                __syncthreads();
+                for (int k = tid; k < rn; k += kBlkThreads) {
+                    const double2 v = racc[k];
+                    if (v.x != 0.0 || v.y != 0.0) {
+                        const int g = r + k * nround;
+                        int e = 0;
+                        while (e_off[e + 1] <= g) ++e;
+                        double* a = &d.sacc[e_src[e] + (g - e_off[e])].x;
+                        unsafeAtomicAdd(a, v.x);
+                        unsafeAtomicAdd(a + 1, v.y);
                     }
                 }
             }
@@ -505,7 +647,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         if (act) {
             const int o = tperm[ti];
             constexpr double dv = out_div(KIND);
-            if (KIND >= SE2P_SLP_GRADIENT) {
+            if (SYM) {
+                // the pair of the target with itself: the self term
+                double* a = &d.sacc[ti].x;
+                unsafeAtomicAdd(a, a1 + self * fi1);
+                unsafeAtomicAdd(a + 1, a2 + self * fi2);
+            } else if (KIND >= SE2P_SLP_GRADIENT) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) u[NC * o + c] = acc[c] / dv;
             } else if (NC == 1) {
@@ -555,6 +702,29 @@ __global__ void k_pack_tar(const int* __restrict__ perm, int n, const double* __
     out[i] = make_double2(p[2 * o], p[2 * o + 1]);
 }
 
+// Pair-symmetric sums: zeroing, and the output in input order (points
+// jb.y * 256 .. + 255 of system jb.x in sorted order).
+__global__ void k_sym_zero(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
+{
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int k = jb.y * 256 + threadIdx.x;
+    if (k < d.Nt) d.sacc[k] = make_double2(0.0, 0.0);
+}
+
+__global__ void k_sym_out(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs,
+                          double* __restrict__ u)
+{
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int k = jb.y * 256 + threadIdx.x;
+    if (k >= d.Nt) return;
+    const double2 v = d.sacc[k];
+    const int o = d.t_perm[k];
+    u[d.u_off + 2 * o] = v.x / out_div(SE2P_SLP);
+    u[d.u_off + 2 * o + 1] = v.y / out_div(SE2P_SLP);
+}
+
 template <int TIER>
 __global__ void k_e1(const double* __restrict__ x, int n, double* __restrict__ out)
 {
@@ -595,11 +765,24 @@ int real_block_side(int nsx, int nsy, int mean_targets)
     return std::min(bs, std::min(nsx, nsy));
 }
 
-int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys, const Job* jobs,
+int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys, const Job* jobs,
                 int njobs, double* u)
 {
     if (njobs <= 0) return SE2P_OK;
     dim3 gr(njobs), bl(kBlkThreads);
+    const bool uni = mode == RM_ONE_CELL;
+    if (mode == RM_SYM) {
+        if (kind != SE2P_SLP)
+            return set_error(SE2P_ERR_UNSUPPORTED, "pair-symmetric sums are Stokeslet only");
+        if (tier == kE1Full)
+            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Full, false, true>), gr, bl, 0, s, sys,
+                               jobs, u);
+        else
+            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Fast, false, true>), gr, bl, 0, s, sys,
+                               jobs, u);
+        SE2P_HIP_CHECK(hipGetLastError());
+        return SE2P_OK;
+    }
 #define SE2P_LAUNCH_REAL(K, T)                                                                  \
     do {                                                                                        \
         if (uni)                                                                                \
@@ -627,6 +810,22 @@ int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys,
     default: SE2P_LAUNCH_REAL(SE2P_DLP_STRESS, kE1Full); break;
     }
 #undef SE2P_LAUNCH_REAL
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs)
+{
+    if (njobs <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_sym_zero, dim3(njobs), dim3(256), 0, s, sys, jobs);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_sym_out(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs, double* u)
+{
+    if (njobs <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_sym_out, dim3(njobs), dim3(256), 0, s, sys, jobs, u);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

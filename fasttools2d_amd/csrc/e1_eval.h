@@ -113,13 +113,29 @@ __device__ __forceinline__ double log_pos(double x)
     return fma(kd, SE2P_LN2_HI, fma(kd, SE2P_LN2_LO, lm));
 }
 
+// ln(x) for normal x > 0 to about 1e-15 absolute: x = m 2^k with m in [1/2, 1),
+// y0 = ln m from the single-precision hardware log2 (absolute error about
+// 1e-7), one Newton step for e^y = m, y1 = y0 + m e^{-y0} - 1 (quadratic: the
+// error becomes about y0's squared over 2), then k ln 2 with the split ln 2.
+// The exponential is exp_neg, whose coefficients the caller already holds in
+// registers, so this log adds no constants of its own (log_pos adds ten).
+__device__ __forceinline__ double log_newton(double x)
+{
+    const int k = __builtin_amdgcn_frexp_exp(x);
+    const double m = __builtin_amdgcn_frexp_mant(x);
+    const double y0 = (double)__builtin_amdgcn_logf((float)m) * SE2P_LN2_HI;
+    const double lm = y0 + fma(m, exp_neg(y0), -1.0);
+    const double kd = (double)k;
+    return fma(kd, SE2P_LN2_HI, fma(kd, SE2P_LN2_LO, lm));
+}
+
 template <int TIER>
 __device__ __forceinline__ double e1_near(double x)
 {
     using C = E1Coeffs<TIER>;
-    // the fast tier takes the library's own log (within 2 ulp, ~25 instructions
-    // against ~80 for the math library's compensated one)
-    const double lx = TIER == kE1Fast ? log_pos(x) : log(x);
+    // the fast tier takes the library's own log (~25 instructions against ~80
+    // for the math library's compensated one)
+    const double lx = TIER == kE1Fast ? log_newton(x) : log(x);
     return (-SE2P_EULER_GAMMA - lx) + x * horner(C::near, x);
 }
 

@@ -69,10 +69,22 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->s_perm = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &p));
         G->s_start = (int*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_TAR_PERM, sizeof(int) * (size_t)Nt, &p));
-        G->t_perm = (int*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_TAR_START, sizeof(int) * (size_t)(nbox + 1), &p));
-        G->t_start = (int*)p;
+        // pair-symmetric Stokeslet sums need the targets to be the sources; they
+        // then share the sources' order
+        const bool sym = prm.symmetric == 1 && kind == SE2P_SLP && d_psrc == d_ptar && Ns == Nt;
+        if (sym) {
+            G->t_perm = G->s_perm;
+            G->t_start = G->s_start;
+            SE2P_TRY(ws_get(ctx, ws, WS_SRC_YKEY, sizeof(int) * (size_t)Ns, &p));
+            G->ykey = (int*)p;
+            SE2P_TRY(ws_get(ctx, ws, WS_SYM_ACC, sizeof(double2) * (size_t)Ns, &p));
+            G->sacc = (double2*)p;
+        } else {
+            SE2P_TRY(ws_get(ctx, ws, WS_TAR_PERM, sizeof(int) * (size_t)Nt, &p));
+            G->t_perm = (int*)p;
+            SE2P_TRY(ws_get(ctx, ws, WS_TAR_START, sizeof(int) * (size_t)(nbox + 1), &p));
+            G->t_start = (int*)p;
+        }
         const size_t rec = rec_layout(kind) == 0 ? 32 : 64;
         SE2P_TRY(ws_get(ctx, ws, WS_SRC_SORTED, rec * (size_t)Ns, &p));
         G->s_rec = (double*)p;
@@ -85,17 +97,28 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(launch_cell_keys(s, d_psrc, Ns, Lx, Ly, nsx, nsy, (int*)skey, (int*)scount));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, nbox, G->s_perm, G->s_start,
                              WS_SORT_TMP));
-        SE2P_TRY(launch_fill_int(s, (int*)tcount, nbox, 0));
-        SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
-        SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, nbox, G->t_perm, G->t_start,
-                             WS_SORT_TMP));
+        if (!sym) {
+            SE2P_TRY(launch_fill_int(s, (int*)tcount, nbox, 0));
+            SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
+            SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, nbox, G->t_perm,
+                                 G->t_start, WS_SORT_TMP));
+        }
         // sources by y inside their cells (each stencil column is then sorted by
         // y), targets along a Morton curve (neighbouring lanes, similar stencils)
         SE2P_TRY(launch_fill_int(s, G->unsorted, 1, 0));
         SE2P_TRY(launch_morton_cells(s, G->s_perm, G->s_start, nbox, d_psrc, Lx, Ly, nsx, nsy, 1,
-                                     G->unsorted));
-        SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx, nsy, 0,
-                                     G->unsorted));
+                                     G->unsorted, G->ykey));
+        if (!sym)
+            SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx,
+                                         nsy, 0, G->unsorted));
+        if (sym) {
+            // the symmetric sums rely on the y order; without it (a cell too large
+            // to sort, points outside the box) each target sums its own pairs
+            int uns = 1;
+            SE2P_HIP_CHECK(hipMemcpyAsync(&uns, G->unsorted, sizeof(int), hipMemcpyDeviceToHost, s));
+            SE2P_HIP_CHECK(hipStreamSynchronize(s));
+            G->sym = uns == 0;
+        }
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
@@ -205,6 +228,9 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         // resolution of the in-cell y order
         d->trim_w = std::max(std::sqrt(d->cut2), G.Ly / nsy) * (1.0 + 1e-5);
         d->unsorted = G.unsorted;
+        d->sym = G.sym ? 1 : 0;
+        d->ykey = G.ykey;
+        d->sacc = G.sacc;
         // -(1 + gamma/2 + ln xi) (mex_stokes_slp_real.cpp:124)
         d->self = -(1.0 + 0.5772156649015328606 / 2 + log(xi));
         d->t_start = G.t_start;
@@ -304,13 +330,17 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 blocks(i, ((long)d.Mx * (d.My / 2 + 1) + 255) / 256);
         });
         for (int tier = 0; tier < 2; ++tier)
-            for (int uni = 0; uni < 2; ++uni)
-                add(B->real[kind][tier][uni], [&](int i, const SysDesc& d) {
-                    if (geoms[i]->has_real && d.kind == kind && d.e1tier == tier &&
-                        (d.bs == 1) == (uni == 1) && d.Nt > 0)
+            for (int mode = 0; mode < kRealModes; ++mode)
+                add(B->real[kind][tier][mode], [&](int i, const SysDesc& d) {
+                    const int m = d.sym ? RM_SYM : d.bs == 1 ? RM_ONE_CELL : RM_GENERAL;
+                    if (geoms[i]->has_real && d.kind == kind && d.e1tier == tier && m == mode &&
+                        d.Nt > 0)
                         blocks(i, (long)d.nbx * ((d.nsy + d.bs - 1) / d.bs));
                 });
     }
+    add(B->sym_pts, [&](int i, const SysDesc& d) {
+        if (geoms[i]->has_real && d.sym) blocks(i, ((long)d.Nt + 255) / 256);
+    });
     add(B->zm_part, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, d.zm_nblk);
     });
@@ -390,14 +420,16 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
                                       B.fill[layout].count, d_f));
         SE2P_TRY(launch_zero_mode(s, B.sys, J + B.zm_part.first, B.zm_part.count,
                                   J + B.zm_sys.first, B.zm_sys.count));
+        SE2P_TRY(launch_sym_zero(s, B.sys, J + B.sym_pts.first, B.sym_pts.count));
     }
     StageTimer st(ctx, "real");
     for (int kind = 0; kind < kNumKinds; ++kind)
         for (int tier = 0; tier < 2; ++tier)
-            for (int uni = 0; uni < 2; ++uni) {
-                const JobRange& r = B.real[kind][tier][uni];
-                SE2P_TRY(launch_real(s, kind, tier, uni == 1, B.sys, J + r.first, r.count, d_u));
+            for (int mode = 0; mode < kRealModes; ++mode) {
+                const JobRange& r = B.real[kind][tier][mode];
+                SE2P_TRY(launch_real(s, kind, tier, mode, B.sys, J + r.first, r.count, d_u));
             }
+    SE2P_TRY(launch_sym_out(s, B.sys, J + B.sym_pts.first, B.sym_pts.count, d_u));
     return SE2P_OK;
 }
 

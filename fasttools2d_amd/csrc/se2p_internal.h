@@ -91,7 +91,7 @@ enum Slot {
     WS_TAR_KEY, WS_TAR_PERM, WS_TAR_START, WS_TAR_COUNT,
     WS_SRC_SORTED, WS_TAR_SORTED,
     WS_SPR_KEY, WS_SPR_PERM, WS_SPR_START, WS_SPR_COUNT, WS_SPR_NODE, WS_SPR_OFF,
-    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED,
+    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED, WS_SRC_YKEY, WS_SYM_ACC,
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
@@ -168,9 +168,11 @@ int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n,
                 int* perm, int* start, Slot tmp_slot);
 int launch_fill_int(hipStream_t s, int* p, int n, int v);
 // Morton order (yorder 0) or y order (yorder 1; sets *unsorted where a cell cannot be
-// ordered) inside each cell of a cell-sorted permutation.
+// ordered) inside each cell of a cell-sorted permutation.  With yorder and a
+// non-null ykey, ykey[k] is the 20-bit in-cell y key of the k-th sorted point.
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
-                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted);
+                        double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted,
+                        int* ykey = nullptr);
 // FindClosestNode for n points, unwrapped start nodes.
 int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, double Ly,
                          double h, int P, int* mx, int* my, double* px, double* py);
@@ -223,6 +225,11 @@ struct SysDesc {
     // (sources y-ordered in their cells); unusable while *unsorted != 0
     double trim_w;
     const int* unsorted;
+    // pair-symmetric sums (sources are the targets, same sorted order): ykey
+    // holds the in-cell y key of each sorted source, sacc the sums in sorted order
+    int sym;
+    const int* ykey;
+    double2* sacc;
     const int *t_start, *t_perm, *s_start, *s_perm;
     const double2* t_sorted;
     double* s_rec;
@@ -258,9 +265,17 @@ int launch_e1(hipStream_t s, const double* x, int n, int tier, double* out);
 
 // Real-space blocks of bs x bs cells for a mean cell occupancy.
 int real_block_side(int nsx, int nsy, int mean_targets);
-// k_real_stencil<kind, tier, one-cell> over the jobs (block indices).
-int launch_real(hipStream_t s, int kind, int tier, bool uni, const SysDesc* sys, const Job* jobs,
+// Real-space launch modes: general blocks, one-cell blocks, pair-symmetric blocks.
+enum RealMode { RM_GENERAL = 0, RM_ONE_CELL = 1, RM_SYM = 2, kRealModes = 3 };
+// k_real_stencil<kind, tier, mode> over the jobs (block indices).  RM_GENERAL
+// and RM_ONE_CELL set u; RM_SYM adds into sacc (zeroed by launch_sym_zero) with
+// atomics, and launch_sym_out then sets u from it.
+int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys, const Job* jobs,
                 int njobs, double* u);
+// sacc = 0, job (system, block of 256 sorted points).
+int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs);
+// u (input order) = sacc (sorted order) / 4 pi, job (system, block of 256 sorted points).
+int launch_sym_out(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs, double* u);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
 // or Z1 (c = 1); job b = grid block (b % ntx, b / ntx) of kSB x kSB nodes; all
 // jobs' systems of support P.
@@ -327,6 +342,9 @@ struct Geom {
     double* s_rec = nullptr;
     double *zm = nullptr, *zm_part = nullptr;
     int* unsorted = nullptr;  // set by the source ordering when a cell is not y-ordered
+    bool sym = false;         // pair-symmetric real space (targets alias the sources)
+    int* ykey = nullptr;      // in-cell y keys of the sorted sources (sym only)
+    double2* sacc = nullptr;  // pair-symmetric sums in sorted order (sym only)
     // k space
     KGeom kg{};
     FftAxis ax{}, ay{};
@@ -357,7 +375,8 @@ struct BatchExec {
     JobRange fill[2];     // by record layout
     JobRange zm_part, zm_sys;
     JobRange sdens;       // spreading densities: blocks of 256 binned points
-    JobRange real[kNumKinds][2][2];  // [kind][tier][one-cell]
+    JobRange real[kNumKinds][2][kRealModes];  // [kind][tier][RealMode]
+    JobRange sym_pts;     // pair-symmetric systems: blocks of 256 sorted points
     JobRange spread[2][kNumP];  // [c][P / 2 - 1]
     JobRange ffty[2][kFftClasses], fftx[2][kFftClasses], tr_fwd[2];  // [c][line class]
     JobRange filter[kNumKinds];  // by kind

@@ -61,8 +61,11 @@ class Plan:
         self.kind = _lib.kind_code(kind)
         self.Ns, self.Nt = len(psrc), len(ptar)
         self.prm = params_struct(prm) if isinstance(prm, dict) else prm
+        # the same array for sources and targets: one upload, so the library sees
+        # one pointer (pair-symmetric sums when prm asks for them)
+        same = ptar is psrc
         dps = DeviceArray.from_host(ctx, psrc)
-        dpt = DeviceArray.from_host(ctx, ptar)
+        dpt = dps if same else DeviceArray.from_host(ctx, ptar)
         dn = DeviceArray.from_host(ctx, nrm) if nrm is not None else None
         self._h = ctypes.c_void_p()
         try:
@@ -72,7 +75,8 @@ class Plan:
                                          ctypes.byref(self._h)))
         finally:
             dps.free()
-            dpt.free()
+            if not same:
+                dpt.free()
             if dn:
                 dn.free()
         self.f = DeviceArray(ctx, (self.Ns, 2))

@@ -1,0 +1,177 @@
+"""Pair-symmetric Stokeslet real space (sources = targets, se2p_params.symmetric).
+
+Each pair is evaluated once and added to both points with atomics, so results
+are compared to tolerance, not bit for bit: against the fixed-order sums of the
+same plan with symmetric = 0 (which the parity suite pins to the oracle) and
+against the oracle itself.  A pair taken twice or missed moves the result by
+about 1e-2 relative; the bound is 1e-13.  The cases cover the orders the pair
+rule has to break: exact duplicates (the self term across coincident points),
+equal y keys inside a cell and across neighbouring cells, boxes one and two
+cells wide (a cell's own periodic images as neighbours), windows staged in
+several rounds and blocks of more than 256 targets (a dense cluster), one-cell
+blocks, and inputs the y order cannot handle (fallback to the fixed order).
+"""
+import numpy as np
+import pytest
+
+import fasttools2d_amd as ft
+from fasttools2d_amd.batch import Batch
+from fasttools2d_amd.plan import Plan
+from tests.inputs import normal_density, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-13
+
+
+def _prm(N, Lx, Ly, nsx, nsy, tol=1e-10):
+    prm = dict(ft.ewald_params("slp", N, N, 2 * N + 1.0, Lx, Ly, tol=tol))
+    prm["nside_x"], prm["nside_y"] = nsx, nsy
+    prm["rc"] = Lx / nsx
+    return prm
+
+
+def _apply(ctx, p, f, Lx, Ly, prm, sym, graph=False, reps=1):
+    q = dict(prm, symmetric=1 if sym else 0)
+    plan = Plan(ctx, "slp", p, p, Lx, Ly, q)
+    try:
+        return [plan.apply(f, graph=graph) for _ in range(reps)]
+    finally:
+        plan.close()
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _check(ctx, p, f, Lx, Ly, prm, oracle=None):
+    ref = _apply(ctx, p, f, Lx, Ly, prm, False)[0]
+    for u in _apply(ctx, p, f, Lx, Ly, prm, True, reps=2):
+        assert _rel(u, ref) < REL
+        assert np.max(np.abs(u - ref)) < REL * np.max(np.abs(ref)) * 10
+        # the pairs were summed in the other order (the symmetric path ran)
+        assert np.any(u != ref)
+    if oracle is not None:
+        r = oracle.slp_real(p, p, f, prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly)
+        r += oracle.slp_kspace(p, p, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], Lx, Ly,
+                               prm["w"], prm["P"])
+        assert _rel(_apply(ctx, p, f, Lx, Ly, prm, True)[0], r) < REL
+
+
+@pytest.mark.parametrize("N,nside", [(6000, 18), (20000, 33), (3000, 4)])
+def test_uniform_matches_fixed_order_and_oracle(gpu_ctx, oracle, N, nside):
+    p = uniform_points(N, 1.0, 1.0, seed=N)
+    f = normal_density(N, seed=N + 1)
+    _check(gpu_ctx, p, f, 1.0, 1.0, _prm(N, 1.0, 1.0, nside, nside), oracle)
+
+
+def test_duplicates_and_equal_keys(gpu_ctx, oracle):
+    """Exact duplicates, a lattice with shared y values (equal keys within and
+    across cells) and points on cell edges."""
+    Lx = Ly = 1.0
+    rng = np.random.default_rng(5)
+    base = uniform_points(2500, Lx, Ly, seed=6)
+    dup = base[rng.choice(2500, 300, replace=False)]
+    g = (np.arange(24) + 0.5) / 24 - 0.5
+    lat = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2)
+    edges = np.stack([np.arange(-6, 6) / 12.0, np.full(12, 0.25)], 1)
+    p = np.concatenate([base, dup, lat, edges])
+    f = normal_density(len(p), seed=7)
+    _check(gpu_ctx, p, f, Lx, Ly, _prm(len(p), Lx, Ly, 12, 12), oracle)
+
+
+@pytest.mark.parametrize("Lx,Ly,nsx,nsy", [(1.0, 1.0, 3, 3), (2.0, 1.0, 2, 1), (1.0, 3.0, 1, 3),
+                                           (4.0, 1.0, 4, 1)])
+def test_narrow_boxes(gpu_ctx, oracle, Lx, Ly, nsx, nsy):
+    N = 1500
+    p = uniform_points(N, Lx, Ly, seed=11)
+    f = normal_density(N, seed=12)
+    prm = ft.ewald_params("slp", N, N, 2 * N + 1.0, Lx, Ly, tol=1e-10)
+    prm = dict(prm, nside_x=nsx, nside_y=nsy)
+    _check(gpu_ctx, p, f, Lx, Ly, prm, oracle)
+
+
+def test_cluster_rounds_and_wide_blocks(gpu_ctx, oracle):
+    """A dense cluster in a sparse background: windows beyond the staging buffer
+    (several rounds) and blocks of more than 256 targets."""
+    Lx = Ly = 1.0
+    rng = np.random.default_rng(21)
+    bg = uniform_points(4000, Lx, Ly, seed=22)
+    cl = np.clip(rng.normal(0.1, 0.01, (2500, 2)), -0.49, 0.49)
+    p = np.concatenate([bg, cl])
+    f = normal_density(len(p), seed=23)
+    _check(gpu_ctx, p, f, Lx, Ly, _prm(len(p), Lx, Ly, 20, 20), oracle)
+
+
+def test_one_cell_blocks(gpu_ctx):
+    N = 30000
+    p = uniform_points(N, 1.0, 1.0, seed=31)
+    f = normal_density(N, seed=32)
+    _check(gpu_ctx, p, f, 1.0, 1.0, _prm(N, 1.0, 1.0, 20, 20))
+
+
+def test_unordered_input_falls_back_to_fixed_order(gpu_ctx):
+    """A point outside the box leaves its cell without a y order: the plan then
+    sums in the fixed order, bit for bit."""
+    N = 4000
+    p = uniform_points(N, 1.0, 1.0, seed=41)
+    p[7] = [0.1, 0.7]
+    f = normal_density(N, seed=42)
+    prm = _prm(N, 1.0, 1.0, 14, 14)
+    np.testing.assert_array_equal(_apply(gpu_ctx, p, f, 1.0, 1.0, prm, True)[0],
+                                  _apply(gpu_ctx, p, f, 1.0, 1.0, prm, False)[0])
+
+
+def test_graph_replays_and_distinct_arrays(gpu_ctx):
+    """Graph replays agree with eager applies; separate source and target arrays
+    (equal values, different pointers) keep the fixed order."""
+    N = 8000
+    p = uniform_points(N, 1.0, 1.0, seed=51)
+    f = normal_density(N, seed=52)
+    prm = _prm(N, 1.0, 1.0, 21, 21)
+    ref = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, False)[0]
+    for u in _apply(gpu_ctx, p, f, 1.0, 1.0, prm, True, graph=True, reps=4):
+        assert _rel(u, ref) < REL
+    plan = Plan(gpu_ctx, "slp", p, p.copy(), 1.0, 1.0, dict(prm, symmetric=1))
+    try:
+        np.testing.assert_array_equal(plan.apply(f), ref)
+    finally:
+        plan.close()
+
+
+def test_batch_mixes_symmetric_and_fixed_order_systems(gpu_ctx):
+    systems, fs = [], []
+    for i, (N, Lx, Ly, ns) in enumerate([(5000, 1.0, 1.0, 16), (3000, 2.0, 1.0, 12)]):
+        p = uniform_points(N, Lx, Ly, seed=60 + i)
+        prm = _prm(N, Lx, Ly, ns * (2 if Lx > Ly else 1), ns)
+        systems.append({"kind": "slp", "psrc": p, "ptar": p, "Lx": Lx, "Ly": Ly,
+                        "prm": dict(prm, symmetric=1 - i)})
+        fs.append(normal_density(N, seed=70 + i))
+    b = Batch(gpu_ctx, systems)
+    try:
+        got = b.apply(fs)
+        got_g = b.apply(fs, graph=True)
+    finally:
+        b.close()
+    for y, f, u, ug in zip(systems, fs, got, got_g):
+        ref = _apply(gpu_ctx, y["psrc"], f, y["Lx"], y["Ly"], y["prm"], False)[0]
+        assert _rel(u, ref) < REL and _rel(ug, ref) < REL
+        if not y["prm"]["symmetric"]:
+            np.testing.assert_array_equal(u, ref)
+
+
+def test_stresslet_ignores_the_flag(gpu_ctx):
+    N = 3000
+    p = uniform_points(N, 1.0, 1.0, seed=81)
+    th = np.random.default_rng(82).random(N) * 2 * np.pi
+    n = np.stack([np.cos(th), np.sin(th)], 1)
+    f = normal_density(N, seed=83)
+    prm = ft.ewald_params("dlp", N, N, 2 * N + 1.0, 1.0, 1.0, tol=1e-10)
+    out = []
+    for sym in (0, 1):
+        plan = Plan(gpu_ctx, "dlp", p, p, 1.0, 1.0, dict(prm, symmetric=sym), nrm=n)
+        try:
+            out.append(plan.apply(f))
+        finally:
+            plan.close()
+    np.testing.assert_array_equal(out[0], out[1])
