@@ -71,12 +71,18 @@ def real_space_work(p, prm, sample=4000, seed=3):
 
 
 def work_model(prm, N, cand, evald, kind="slp"):
-    """Flops per matvec of the fixed work model: 6 cand + 60 E + spread + gather + FFT."""
+    """Flops per matvec of the fixed work model: 6 cand + 60 E + spread + gather + FFT.
+    cand and evald count ordered (target, source) pairs; pair-symmetric sums
+    (prm["symmetric"]) test and evaluate each unordered pair once, and an
+    evaluation there also adds the source's term (8 flops more)."""
     P1 = (prm["P"] + 1) ** 2
     G = prm["Mx"] * prm["My"]
     cs = 2 if kind == "slp" else 4
     nfft = 4 if kind == "slp" else 6
-    real = 6 * cand + (60 if kind == "slp" else 45) * evald
+    if kind == "slp" and prm.get("symmetric", 0):
+        real = (6 * cand + 68 * evald) / 2
+    else:
+        real = 6 * cand + (60 if kind == "slp" else 45) * evald
     spread = 2 * cs * N * P1
     gather = 2 * 2 * N * P1
     fft = 5 * nfft * G * math.log2(G)
@@ -276,7 +282,7 @@ def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
         tol = float(10 ** rng.uniform(-10, -4))
         p = np.ascontiguousarray(rng.random((n, 2)) * [Lx, Ly] - [Lx / 2, Ly / 2])
         if recipe == "rule":
-            prm = rule_params("slp", n, n, Lx, Ly, tol)
+            prm = rule_params("slp", n, n, Lx, Ly, tol, symmetric=1)
         else:
             prm = capped_params(n, Lx, Ly, tol)
             prm["e1_tier"] = 1
@@ -493,9 +499,9 @@ def main():
     # comparison finds faster; the timed window then runs that mode alone
     pre = {"streams": 0.0, "graph": 0.0}
     time_steps(ctx, replay, 1, args.warmup)  # capture the graph
-    for _ in range(3):  # alternating, so a drift of the clock hits both alike
-        pre["streams"] += time_steps(ctx, apply, 10, 2)
-        pre["graph"] += time_steps(ctx, replay, 10, 2)
+    for _ in range(5):  # alternating, so a drift of the clock hits both alike
+        pre["streams"] += time_steps(ctx, apply, 20, 5)
+        pre["graph"] += time_steps(ctx, replay, 20, 5)
     mode = min(pre, key=pre.get)
     run = replay if mode == "graph" else apply
     ms_tuned = time_steps(ctx, run, args.steps, args.warmup) * 1e3
@@ -535,9 +541,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic: uniform points in the unit box, f ~ N(0,1), seed 1",
         "config": {"workload": "SLP spectral Ewald, N=1e6 sources=targets, unit box, tol 1e-8",
-                   "path": "tuned: autotuned split, plan reuse, real-space and k-space "
-                           f"passes on two streams ({mode}), FP64",
-                   "params": {k: tprm[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier")}},
+                   "path": "tuned: autotuned split, plan reuse, pair-symmetric real space, "
+                           f"real-space and k-space passes on two streams ({mode}), FP64",
+                   "params": {k: tprm[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier",
+                                                   "symmetric")}},
         "rel_err": err_tuned,
         "plain": {"evals_per_s": 1e3 / ms_plain, "ms": ms_plain, "rel_err": err_plain,
                   "params": {k: prm[k] for k in ("xi", "nside_x", "Mx", "P")},

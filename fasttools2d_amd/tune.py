@@ -16,6 +16,10 @@ method and measures instead:
    0.8 tol wins (the two halves add to at most tol; the 20% margin covers what
    the split changes in them).
 
+A Stokeslet system whose targets are its sources (the same array) is tuned with
+pair-symmetric real-space sums (se2p_params.symmetric), which halve the pair
+work and so move the best split towards larger cells.
+
 Errors are relative L2 norms over all targets.  Timings use se2p_plan_apply on
 device-resident inputs.
 """
@@ -112,6 +116,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
     log = log or (lambda s: None)
     A, B = _aspect(Lx, Ly)
     Ns, Nt = len(ps), len(pt)
+    sym = 1 if _lib.kind_code(kind) == _lib.SE2P_SLP and pt is ps else 0
     T = Tuner(ctx, kind, ps, pt, f, Lx, Ly, nrm=nrm, log=log)
     report = {"tol": tol}
     try:
@@ -133,7 +138,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             rc = Lx / nsx
             xi = c_r / rc
             M0 = smooth_up(c_k * xi * Lx / math.pi / A)
-            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier)
+            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym)
 
         # 2. real-space constant
         c_r = None
@@ -213,7 +218,8 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         T.close()
 
 
-def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, margin=10.0):
+def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, margin=10.0,
+                symmetric=0):
     """Parameters without measuring: the constants the tuner calibrates, as
     functions of the tolerance, fitted to its N = 1e6, tol = 1e-8 result
     (xi rc = 3.9, k_inf/xi = 8.0, P = 16, about 22 points per cell), with
@@ -227,7 +233,9 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
     Cells hold about `per_cell` points; where the grid would exceed max_grid per
     side the cells grow.  The fast E1 tier is used for tol >= 1e-10.  A fallback
     for systems too many to tune one by one (batches); its accuracy is checked
-    against accurate solutions where it is used."""
+    against accurate solutions where it is used.  symmetric is passed through
+    (pair-symmetric real space for Stokeslet systems whose targets are their
+    sources)."""
     A, B = _aspect(Lx, Ly)
     L = math.log(margin / tol)
     c_r = math.sqrt(max(L - 3.2, 1.0))
@@ -248,4 +256,4 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
         if a == 1:
             raise ValueError(f"no grid within {max_grid} per side for tol {tol}")
         a -= 1
-    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0)
+    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric)
