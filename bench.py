@@ -147,6 +147,7 @@ def measured_traffic(kernel, prm):
         q = r["params"]
         if (r["stage"] == kernel and q["nside"] == prm["nside_x"] and q["M"] == prm["Mx"]
                 and q["P"] == prm["P"] and q["tier"] == prm["e1_tier"]
+                and q.get("symmetric", 0) == prm.get("symmetric", 0)
                 and abs(q["c_r"] * q["nside"] - prm["xi"]) < 1e-6 * prm["xi"]):
             return r["hbm_bytes"]
     return None

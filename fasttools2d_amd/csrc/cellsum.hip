@@ -69,6 +69,28 @@ constexpr int kMaxBlockSide = 4;
 constexpr int kMaxEntries = (kMaxBlockSide + 2) * (kMaxBlockSide + 2);
 constexpr int kWords = 2, kChunk = 64 * kWords;  // candidates per test chunk
 
+// E = E1(xi^2 r^2) and c = e^{-xi^2 r^2}/r^2.  In the far regime (x = xi^2 r^2 >=
+// 1) the fit E1 = e^{-x} t P(t)/Q(t), t = 1/x, is evaluated as e^{-x} P~(x)/(x
+// Q~(x)) with the reversed polynomials P~(x) = x^d P(1/x), Q~(x) = x^d Q(1/x)
+// (positive coefficients, so Horner in x has no cancellation), and 1/r^2 =
+// xi^2 Q~/(x Q~): one reciprocal serves both.
+template <int TIER, bool NEAR>
+__device__ __forceinline__ void slp_radial(double r2, double xi2, double& E, double& c)
+{
+    const double x = xi2 * r2;
+    const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
+    if (NEAR) {
+        E = e1_near<TIER>(x);
+        c = e * rcp64(r2);
+    } else {
+        using C = E1Coeffs<TIER>;
+        const double pr = horner_rev(C::farp, x), qr = horner_rev(C::farq, x);
+        const double eD = e * rcp64(x * qr);
+        E = eD * pr;
+        c = eD * qr * xi2;
+    }
+}
+
 // 4 pi times the SLP pair term for r = target - source:
 //   E1(xi^2 r^2)/2 f + e^{-xi^2 r^2} ((r.f) r/r^2 - f)
 // in the cancellation-free form u1 += E/2 f1 + (e/r^2) dy (dx f2 - dy f1),
@@ -79,11 +101,8 @@ __device__ __forceinline__ void slp_pair(double dx, double dy, double r2, double
                                          double ixi2, double f1, double f2, double& a1,
                                          double& a2)
 {
-    const double ir2 = rcp64(r2);
-    const double x = xi2 * r2;
-    const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
-    const double E = NEAR ? e1_near<TIER>(x) : e1_far<TIER>(e, ir2 * ixi2);
-    const double c = e * ir2;
+    double E, c;
+    slp_radial<TIER, NEAR>(r2, xi2, E, c);
     const double cr = dx * f2 - dy * f1;
     a1 += 0.5 * E * f1 + c * dy * cr;
     a2 += 0.5 * E * f2 - c * dx * cr;
@@ -98,11 +117,9 @@ __device__ __forceinline__ void slp_pair2(double dx, double dy, double r2, doubl
                                           double g2, double& a1, double& a2, double& b1,
                                           double& b2)
 {
-    const double ir2 = rcp64(r2);
-    const double x = xi2 * r2;
-    const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
-    const double hE = 0.5 * (NEAR ? e1_near<TIER>(x) : e1_far<TIER>(e, ir2 * ixi2));
-    const double c = e * ir2;
+    double E, c;
+    slp_radial<TIER, NEAR>(r2, xi2, E, c);
+    const double hE = 0.5 * E;
     const double cf = dx * f2 - dy * f1, cg = dx * g2 - dy * g1;
     a1 += hE * f1 + c * dy * cf;
     a2 += hE * f2 - c * dx * cf;
@@ -277,7 +294,7 @@ __device__ __forceinline__ int round_index(int g, int r, int nround)
 // trim_w.  The window holds rows 0 .. bh only; the term for the source is
 // accumulated in LDS (racc) and added after each round, the target's own sum
 // after the last round, with atomics into sacc, which follows the sorted order
-// (so a cell's atomics share cache lines); k_sym_out writes u from it.
+// (so a cell's atomics share cache lines); the gather adds it to u.
 template <int KIND, int TIER, bool UNI, bool SYM = false>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_real_stencil(
     const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, double* __restrict__ u_base)
@@ -287,6 +304,17 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     // with the reflected sums and keys, 4 workgroups per CU still fit in LDS
     constexpr int CAP = SYM ? 768 : SrcOf<KIND>::cap;
     __shared__ Rec buf[CAP];
+    // Stokeslet records are staged as two arrays, positions and densities, so a
+    // candidate test reads 16-byte entries at a 16-byte stride
+    constexpr bool SOA = KIND == SE2P_SLP;
+    double2* const bxy = reinterpret_cast<double2*>(buf);
+    double2* const bfd = bxy + CAP;
+    auto rec_xy = [&](int i) -> double2 {
+        if constexpr (SOA)
+            return bxy[i];
+        else
+            return *reinterpret_cast<const double2*>(&buf[i]);
+    };
     __shared__ double2 racc[SYM ? CAP : 1];
     __shared__ int rkey[SYM ? CAP : 1];
     __shared__ int e_src[kMaxEntries], e_off[kMaxEntries + 1];
@@ -310,30 +338,55 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     // for SYM, whose window starts at row 0
     const int H = SYM ? bh + 1 : bh + 2;
     const int nent = (bw + 2) * H, ncell = bw * bh;
-    if (tid < nent) {
-        const int ux = tid / H - 1, uy = tid % H - (SYM ? 0 : 1);
-        const int qx = bx0 + ux, qy = by0 + uy;
-        const int wx = wrapi(qx, nsx), wy = wrapi(qy, nsy);
-        const int cell = wy * nsx + wx;
-        e_src[tid] = sstart[cell];
-        e_off[tid + 1] = sstart[cell + 1] - sstart[cell];  // count, prefixed below
-        e_ox[tid] = (Lx * (qx - wx)) / nsx;
-        e_oy[tid] = (Ly * (qy - wy)) / nsy;
-    }
-    if (tid < ncell) {
-        const int cx = tid / bh, cy = tid % bh;
-        const int cell = (by0 + cy) * nsx + (bx0 + cx);
-        c_t0[tid] = tstart[cell];
-        c_off[tid + 1] = tstart[cell + 1] - tstart[cell];
+    static_assert(kMaxEntries <= 64, "the window entries are set up by one wave");
+    if (tid < 64) {
+        // wave 0: one lane per window entry and per target cell, then the
+        // offsets as inclusive scans of the counts across the wave
+        int ecnt = 0, ccnt = 0;
+        if (tid < nent) {
+            const int ux = tid / H - 1, uy = tid % H - (SYM ? 0 : 1);
+            const int qx = bx0 + ux, qy = by0 + uy;
+            const int wx = wrapi(qx, nsx), wy = wrapi(qy, nsy);
+            const int cell = wy * nsx + wx;
+            e_src[tid] = sstart[cell];
+            ecnt = sstart[cell + 1] - sstart[cell];
+            e_ox[tid] = (Lx * (qx - wx)) / nsx;
+            e_oy[tid] = (Ly * (qy - wy)) / nsy;
+        }
+        if (tid < ncell) {
+            const int cx = tid / bh, cy = tid % bh;
+            const int cell = (by0 + cy) * nsx + (bx0 + cx);
+            c_t0[tid] = tstart[cell];
+            ccnt = tstart[cell + 1] - tstart[cell];
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int a = __shfl_up(ecnt, o), b = __shfl_up(ccnt, o);
+            if (tid >= o) {
+                ecnt += a;
+                ccnt += b;
+            }
+        }
+        if (tid < nent) e_off[tid + 1] = ecnt;
+        if (tid < ncell) c_off[tid + 1] = ccnt;
+        if (tid == 0) {
+            e_off[0] = 0;
+            c_off[0] = 0;
+        }
     }
     __syncthreads();
-    if (tid == 0) {
-        e_off[0] = 0;
-        for (int e = 0; e < nent; ++e) e_off[e + 1] += e_off[e];
-        c_off[0] = 0;
-        for (int c = 0; c < ncell; ++c) c_off[c + 1] += c_off[c];
-    }
-    __syncthreads();
+    // window entry of window position g (the last entry starting at or before g)
+    auto entry_of = [&](int g) {
+        int lo = 0, hi = nent;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (e_off[mid] <= g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
     // block-uniform: kept in scalar registers
     const int total = __builtin_amdgcn_readfirstlane(e_off[nent]);
     const int ntar = __builtin_amdgcn_readfirstlane(c_off[ncell]);
@@ -372,13 +425,17 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             __syncthreads();
             for (int k = tid; k < rn; k += kBlkThreads) {
                 const int g = r + k * nround;
-                int e = 0;
-                while (e_off[e + 1] <= g) ++e;
+                const int e = entry_of(g);
                 const int sk = e_src[e] + (g - e_off[e]);
                 Rec rec = ssorted[sk];
                 rec.x += e_ox[e];
                 rec.y += e_oy[e];
-                buf[k] = rec;
+                if constexpr (SOA) {
+                    bxy[k] = make_double2(rec.x, rec.y);
+                    bfd[k] = make_double2(rec.z, rec.w);
+                } else {
+                    buf[k] = rec;
+                }
                 if constexpr (SYM) {
                     rkey[k] = d.ykey[sk];
                     racc[k] = make_double2(0.0, 0.0);
@@ -396,7 +453,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 auto first_at = [&](int a, int b, double v) {  // first of [a, b) with y >= v
                     while (a < b) {
                         const int mid = (a + b) >> 1;
-                        if (buf[mid].y < v)
+                        if (rec_xy(mid).y < v)
                             a = mid + 1;
                         else
                             b = mid;
@@ -449,7 +506,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     auto first_at = [&](int a, int b, double v) {  // first of [a, b) with y >= v
                         while (a < b) {
                             const int mid = (a + b) >> 1;
-                            if (buf[mid].y < v)
+                            if (rec_xy(mid).y < v)
                                 a = mid + 1;
                             else
                                 b = mid;
@@ -519,7 +576,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             const int pos = c + (64 * w + k0 + j) * nch;
-                            sp[j] = *reinterpret_cast<const double2*>(&buf[min(staged(pos), CAP - 1)]);
+                            sp[j] = rec_xy(min(staged(pos), CAP - 1));
                         }
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
@@ -565,31 +622,33 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     // are issued, so waiting for it does not wait for them
                     bool more = (lo | hi) != 0;
                     int idx = more ? staged(next(lo, hi)) : 0;
-                    double4 q = buf[idx];
+                    double2 q = bxy[idx], g = bfd[idx];
                     while (more) {
                         more = (lo | hi) != 0;
                         const int nidx = more ? staged(next(lo, hi)) : idx;
-                        const double4 qn = buf[nidx];
+                        const double2 qn = bxy[nidx], gn = bfd[nidx];
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
                         double b1, b2;
-                        slp_pair2<TIER, false>(dx, dy, r2, xi2, ixi2, q.z, q.w, fi1, fi2, a1, a2,
+                        slp_pair2<TIER, false>(dx, dy, r2, xi2, ixi2, g.x, g.y, fi1, fi2, a1, a2,
                                                b1, b2);
                         atomicAdd(&racc[idx].x, b1);
                         atomicAdd(&racc[idx].y, b2);
                         idx = nidx;
                         q = qn;
+                        g = gn;
                     }
                     lo = hi = 0;
                 }
                 while (lo | hi) {
                     const int idx = staged(next(lo, hi));
                     const Rec& s = buf[idx];
-                    const double dx = pt.x - s.x, dy = pt.y - s.y;
+                    const double2 sxy = rec_xy(idx);
+                    const double dx = pt.x - sxy.x, dy = pt.y - sxy.y;
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
-                        const double4& q = reinterpret_cast<const double4&>(s);
-                        slp_pair<TIER, false>(dx, dy, r2, xi2, ixi2, q.z, q.w, a1, a2);
+                        const double2 g = bfd[idx];
+                        slp_pair<TIER, false>(dx, dy, r2, xi2, ixi2, g.x, g.y, a1, a2);
                     } else if (KIND == SE2P_DLP) {
                         dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
                     } else if (KIND == SE2P_SLP_PRESSURE) {
@@ -606,24 +665,24 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     hi = nm[1];
                     while (lo | hi) {
                         const int idx = staged(next(lo, hi));
-                        const double4& q = reinterpret_cast<const double4&>(buf[idx]);
+                        const double2 q = rec_xy(idx), g = bfd[idx];
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
                         if (idx >= o0 && idx < o1 && fabs(r2) < 1e-15) {
-                            a1 += self * q.z;
-                            a2 += self * q.w;
+                            a1 += self * g.x;
+                            a2 += self * g.y;
                             if (SYM) {
                                 atomicAdd(&racc[idx].x, self * fi1);
                                 atomicAdd(&racc[idx].y, self * fi2);
                             }
                         } else if (SYM) {
                             double b1, b2;
-                            slp_pair2<TIER, true>(dx, dy, r2, xi2, ixi2, q.z, q.w, fi1, fi2, a1,
+                            slp_pair2<TIER, true>(dx, dy, r2, xi2, ixi2, g.x, g.y, fi1, fi2, a1,
                                                   a2, b1, b2);
                             atomicAdd(&racc[idx].x, b1);
                             atomicAdd(&racc[idx].y, b2);
                         } else {
-                            slp_pair<TIER, true>(dx, dy, r2, xi2, ixi2, q.z, q.w, a1, a2);
+                            slp_pair<TIER, true>(dx, dy, r2, xi2, ixi2, g.x, g.y, a1, a2);
                         }
                     }
                 }
@@ -635,8 +694,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     const double2 v = racc[k];
                     if (v.x != 0.0 || v.y != 0.0) {
                         const int g = r + k * nround;
-                        int e = 0;
-                        while (e_off[e + 1] <= g) ++e;
+                        const int e = entry_of(g);
                         double* a = &d.sacc[e_src[e] + (g - e_off[e])].x;
                         unsafeAtomicAdd(a, v.x);
                         unsafeAtomicAdd(a + 1, v.y);
@@ -702,8 +760,8 @@ __global__ void k_pack_tar(const int* __restrict__ perm, int n, const double* __
     out[i] = make_double2(p[2 * o], p[2 * o + 1]);
 }
 
-// Pair-symmetric sums: zeroing, and the output in input order (points
-// jb.y * 256 .. + 255 of system jb.x in sorted order).
+// Pair-symmetric sums: zeroing (points jb.y * 256 .. + 255 of system jb.x in
+// sorted order).
 __global__ void k_sym_zero(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
 {
     const Job jb = jobs[blockIdx.x];
@@ -712,17 +770,10 @@ __global__ void k_sym_zero(const SysDesc* __restrict__ sys, const Job* __restric
     if (k < d.Nt) d.sacc[k] = make_double2(0.0, 0.0);
 }
 
-__global__ void k_sym_out(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs,
-                          double* __restrict__ u)
+__global__ void k_inverse_perm(const int* __restrict__ perm, int n, int* __restrict__ pos)
 {
-    const Job jb = jobs[blockIdx.x];
-    const SysDesc& d = sys[jb.x];
-    const int k = jb.y * 256 + threadIdx.x;
-    if (k >= d.Nt) return;
-    const double2 v = d.sacc[k];
-    const int o = d.t_perm[k];
-    u[d.u_off + 2 * o] = v.x / out_div(SE2P_SLP);
-    u[d.u_off + 2 * o + 1] = v.y / out_div(SE2P_SLP);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) pos[perm[k]] = k;
 }
 
 template <int TIER>
@@ -814,18 +865,18 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
     return SE2P_OK;
 }
 
-int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs)
+int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos)
 {
-    if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_sym_zero, dim3(njobs), dim3(256), 0, s, sys, jobs);
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, pos);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_sym_out(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs, double* u)
+int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_sym_out, dim3(njobs), dim3(256), 0, s, sys, jobs, u);
+    hipLaunchKernelGGL(k_sym_zero, dim3(njobs), dim3(256), 0, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

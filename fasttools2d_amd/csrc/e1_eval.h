@@ -29,6 +29,16 @@ __device__ __forceinline__ double horner(const double (&c)[N], double x)
     return r;
 }
 
+// sum_i c[i] x^(N-1-i): the polynomial with the coefficients in reverse order
+template <int N>
+__device__ __forceinline__ double horner_rev(const double (&c)[N], double x)
+{
+    double r = c[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) r = fma(r, x, c[i]);
+    return r;
+}
+
 template <int TIER>
 struct E1Coeffs;
 

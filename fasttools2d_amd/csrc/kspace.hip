@@ -285,6 +285,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
     const double* __restrict__ e1tab = d.e1tab;
     const double* zm = use_zm ? d.zm : nullptr;
+    const bool sym = d.sym && g == 0;
     double* u = u_base + d.u_off;
     extern __shared__ double2 reg[];
     __shared__ double e1s[P + 1];
@@ -374,7 +375,12 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                 u[p] = accumulate ? u[p] + v0 : v0;
             } else {
                 double* up = u + (size_t)d.nc * p + 2 * g;
-                if (accumulate) {
+                if (accumulate && sym) {
+                    // the pair-symmetric real-space sums, in sorted order
+                    const double2 a = d.sacc[d.spos[p]];
+                    v0 = a.x / (4 * kPi) + v0;
+                    v1 = a.y / (4 * kPi) + v1;
+                } else if (accumulate) {
                     v0 = up[0] + v0;
                     v1 = up[1] + v1;
                 }

@@ -69,9 +69,10 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->s_perm = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &p));
         G->s_start = (int*)p;
-        // pair-symmetric Stokeslet sums need the targets to be the sources; they
-        // then share the sources' order
-        const bool sym = prm.symmetric == 1 && kind == SE2P_SLP && d_psrc == d_ptar && Ns == Nt;
+        // pair-symmetric Stokeslet sums need the targets to be the sources (they
+        // then share the sources' order) and a k-space part (its gather adds them)
+        const bool sym = prm.symmetric == 1 && kind == SE2P_SLP && d_psrc == d_ptar && Ns == Nt &&
+                         want_k;
         if (sym) {
             G->t_perm = G->s_perm;
             G->t_start = G->s_start;
@@ -79,6 +80,8 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
             G->ykey = (int*)p;
             SE2P_TRY(ws_get(ctx, ws, WS_SYM_ACC, sizeof(double2) * (size_t)Ns, &p));
             G->sacc = (double2*)p;
+            SE2P_TRY(ws_get(ctx, ws, WS_SYM_POS, sizeof(int) * (size_t)Ns, &p));
+            G->spos = (int*)p;
         } else {
             SE2P_TRY(ws_get(ctx, ws, WS_TAR_PERM, sizeof(int) * (size_t)Nt, &p));
             G->t_perm = (int*)p;
@@ -112,6 +115,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
             SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx,
                                          nsy, 0, G->unsorted));
         if (sym) {
+            SE2P_TRY(launch_inverse_perm(s, G->s_perm, Ns, G->spos));
             // the symmetric sums rely on the y order; without it (a cell too large
             // to sort, points outside the box) each target sums its own pairs
             int uns = 1;
@@ -231,6 +235,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->sym = G.sym ? 1 : 0;
         d->ykey = G.ykey;
         d->sacc = G.sacc;
+        d->spos = G.spos;
         // -(1 + gamma/2 + ln xi) (mex_stokes_slp_real.cpp:124)
         d->self = -(1.0 + 0.5772156649015328606 / 2 + log(xi));
         d->t_start = G.t_start;
@@ -341,6 +346,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     add(B->sym_pts, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && d.sym) blocks(i, ((long)d.Nt + 255) / 256);
     });
+
     add(B->zm_part, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, d.zm_nblk);
     });
@@ -429,7 +435,6 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
                 const JobRange& r = B.real[kind][tier][mode];
                 SE2P_TRY(launch_real(s, kind, tier, mode, B.sys, J + r.first, r.count, d_u));
             }
-    SE2P_TRY(launch_sym_out(s, B.sys, J + B.sym_pts.first, B.sym_pts.count, d_u));
     return SE2P_OK;
 }
 

@@ -91,7 +91,7 @@ enum Slot {
     WS_TAR_KEY, WS_TAR_PERM, WS_TAR_START, WS_TAR_COUNT,
     WS_SRC_SORTED, WS_TAR_SORTED,
     WS_SPR_KEY, WS_SPR_PERM, WS_SPR_START, WS_SPR_COUNT, WS_SPR_NODE, WS_SPR_OFF,
-    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED, WS_SRC_YKEY, WS_SYM_ACC,
+    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED, WS_SRC_YKEY, WS_SYM_ACC, WS_SYM_POS,
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
@@ -230,6 +230,7 @@ struct SysDesc {
     int sym;
     const int* ykey;
     double2* sacc;
+    const int* spos;  // sorted position of each point (the inverse of t_perm)
     const int *t_start, *t_perm, *s_start, *s_perm;
     const double2* t_sorted;
     double* s_rec;
@@ -269,13 +270,14 @@ int real_block_side(int nsx, int nsy, int mean_targets);
 enum RealMode { RM_GENERAL = 0, RM_ONE_CELL = 1, RM_SYM = 2, kRealModes = 3 };
 // k_real_stencil<kind, tier, mode> over the jobs (block indices).  RM_GENERAL
 // and RM_ONE_CELL set u; RM_SYM adds into sacc (zeroed by launch_sym_zero) with
-// atomics, and launch_sym_out then sets u from it.
+// atomics, and the gather adds sacc to the k-space part (only systems with a
+// k-space part take pair-symmetric sums).
 int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys, const Job* jobs,
                 int njobs, double* u);
 // sacc = 0, job (system, block of 256 sorted points).
 int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs);
-// u (input order) = sacc (sorted order) / 4 pi, job (system, block of 256 sorted points).
-int launch_sym_out(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs, double* u);
+// pos[perm[k]] = k for k < n.
+int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
 // or Z1 (c = 1); job b = grid block (b % ntx, b / ntx) of kSB x kSB nodes; all
 // jobs' systems of support P.
@@ -345,6 +347,7 @@ struct Geom {
     bool sym = false;         // pair-symmetric real space (targets alias the sources)
     int* ykey = nullptr;      // in-cell y keys of the sorted sources (sym only)
     double2* sacc = nullptr;  // pair-symmetric sums in sorted order (sym only)
+    int* spos = nullptr;      // sorted position of each point (sym only)
     // k space
     KGeom kg{};
     FftAxis ax{}, ay{};
