@@ -606,6 +606,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     }
                 }
                 // candidate position of the lowest set bit of (lo, hi), which is cleared
+                // candidate position of bit 0 of each word
+                const int pos_w0 = c + (kw[0] - 1) * nch, pos_w1 = c + (64 + kw[1] - 1) * nch;
                 auto next = [&](unsigned long long& lo, unsigned long long& hi) {
                     const bool z = lo == 0;
                     const unsigned long long m = z ? hi : lo;
@@ -613,8 +615,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     const unsigned long long m2 = m & (m - 1);
                     lo = z ? lo : m2;
                     hi = z ? m2 : hi;
-                    const int k = z ? 64 + kw[1] - 1 - bit : kw[0] - 1 - bit;
-                    return c + k * nch;
+                    return (z ? pos_w1 : pos_w0) - bit * nch;
                 };
                 unsigned long long lo = fm[0], hi = fm[1];
                 if constexpr (SYM) {

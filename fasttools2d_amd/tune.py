@@ -11,8 +11,9 @@ method and measures instead:
 3. calibration of the k-space constants: k_inf / xi (giving M) and the Gaussian
    support P, the cheapest pair whose error alone stays below tol/2;
 4. a sweep over the split (nside, hence rc, xi = c_r / rc and M) timed with the
-   plan path, refined around its fastest point, then the next supports at that
-   split; the fastest configuration whose measured error against u_ref is below
+   plan path, refined around its fastest point, then the next supports and the
+   larger grids whose line FFTs take fewer stages at that split; the fastest
+   configuration whose measured error against u_ref is below
    0.8 tol wins (the two halves add to at most tol; the 20% margin covers what
    the split changes in them).
 
@@ -47,6 +48,37 @@ def smooth_up(n, max_n=4096):
                 return n
         n += 1
     raise ValueError("grid beyond the supported size")
+
+
+# Radices of the line FFT per line class (fasttools2d_amd/csrc/fft.hip,
+# fft_axis_init and fft_line_class): composite radices only for 2049..2560 points.
+_RADICES = {0: (8, 5, 4, 3, 2), 1: (16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2), 2: (8, 5, 4, 3, 2)}
+
+
+def fft_stages(n):
+    """In-register stages of the line FFT's radix plan for a 2-3-5-smooth n (the
+    planner takes the fewest); None when n has another prime factor."""
+    cls = 0 if n <= 2048 else 1 if n <= 2560 else 2
+    best = {1: 0}
+    for d in range(2, n + 1):
+        if n % d:
+            continue
+        c = [best[d // q] + 1 for q in _RADICES[cls] if d % q == 0 and d // q in best]
+        if c:
+            best[d] = min(c)
+    return best.get(n)
+
+
+def fft_alternatives(m, max_n=4096, slack=1.12):
+    """Even 2-3-5-smooth grid sizes above m, up to slack * m, whose line FFT takes
+    fewer stages than m's (a larger grid only lowers the k-space error); the
+    tuner times them, since a stage's cost depends on its radices and line class."""
+    out, st0 = [], fft_stages(m)
+    for n in range(m + 2, min(max_n, int(slack * m)) + 1, 2):
+        st = fft_stages(n)
+        if st is not None and st < st0 and all(fft_stages(o) > st for o in out):
+            out.append(n)
+    return out
 
 
 def eta_w(xi, P, L, M):
@@ -133,11 +165,13 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         # exact-enough k space for calibrating the real space, and vice versa
         c_exact_r, c_exact_k, P_exact = 6.0, 2 * 6.0, 24
 
-        def prm_for(a, c_r, c_k, P, tier=0):
+        def prm_for(a, c_r, c_k, P, tier=0, M1=None):
             nsx, nsy = a * A, a * B
             rc = Lx / nsx
             xi = c_r / rc
             M0 = smooth_up(c_k * xi * Lx / math.pi / A)
+            if M1 is not None:
+                M0 = M1
             return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym)
 
         # 2. real-space constant
@@ -177,12 +211,12 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         tier = 1 if tol >= 1e-10 else 0
         results, tried = [], set()
 
-        def timed(a, P=P, c_k=c_k):
-            if (a, P) in tried:
+        def timed(a, P=P, c_k=c_k, M1=None):
+            if (a, P, M1) in tried:
                 return
-            tried.add((a, P))
+            tried.add((a, P, M1))
             try:
-                prm = prm_for(a, c_r, c_k, P, tier)
+                prm = prm_for(a, c_r, c_k, P, tier, M1)
             except ValueError:
                 return
             if prm["Mx"] > 4096 or prm["My"] > 4096:
@@ -207,6 +241,13 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             a_best = min(ok, key=lambda r: r[0])[3]
             for _, P2, ck2 in sorted(kcands)[1:3]:
                 timed(a_best, P2, ck2)
+        # 7. larger grids whose line FFTs take fewer stages, at the best point
+        ok = [r for r in results if r[1] <= 0.8 * tol]
+        if ok:
+            _, _, pb, a_best = min(ok, key=lambda r: r[0])
+            ck_b = next((ck for _, P2, ck in kcands if P2 == pb["P"]), c_k)
+            for m in fft_alternatives(pb["Mx"] // A, 4096 // max(A, B)):
+                timed(a_best, pb["P"], ck_b, m)
         ok = [r for r in results if r[1] <= 0.8 * tol]
         if not ok:
             ok = sorted(results, key=lambda r: r[1])[:1]
