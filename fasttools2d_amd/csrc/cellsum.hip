@@ -761,16 +761,6 @@ __global__ void k_pack_tar(const int* __restrict__ perm, int n, const double* __
     out[i] = make_double2(p[2 * o], p[2 * o + 1]);
 }
 
-// Pair-symmetric sums: zeroing (points jb.y * 256 .. + 255 of system jb.x in
-// sorted order).
-__global__ void k_sym_zero(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
-{
-    const Job jb = jobs[blockIdx.x];
-    const SysDesc& d = sys[jb.x];
-    const int k = jb.y * 256 + threadIdx.x;
-    if (k < d.Nt) d.sacc[k] = make_double2(0.0, 0.0);
-}
-
 __global__ void k_inverse_perm(const int* __restrict__ perm, int n, int* __restrict__ pos)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -870,14 +860,6 @@ int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos)
 {
     if (n <= 0) return SE2P_OK;
     hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, pos);
-    SE2P_HIP_CHECK(hipGetLastError());
-    return SE2P_OK;
-}
-
-int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs)
-{
-    if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_sym_zero, dim3(njobs), dim3(256), 0, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

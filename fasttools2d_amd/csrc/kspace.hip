@@ -482,6 +482,9 @@ __global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restri
         double4* rec = reinterpret_cast<double4*>(d.s_rec);
         rec[i].z = f1;
         rec[i].w = f2;
+        // pair-symmetric sums (sources = targets, one sorted order): clear the
+        // accumulator the real-space pass adds into
+        if (d.sym) d.sacc[i] = make_double2(0.0, 0.0);
     } else {
         double* r = d.s_rec + 8 * (size_t)i;
         r[2] = f1;

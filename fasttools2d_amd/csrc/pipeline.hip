@@ -343,9 +343,6 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                         blocks(i, (long)d.nbx * ((d.nsy + d.bs - 1) / d.bs));
                 });
     }
-    add(B->sym_pts, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && d.sym) blocks(i, ((long)d.Nt + 255) / 256);
-    });
 
     add(B->zm_part, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, d.zm_nblk);
@@ -426,7 +423,6 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
                                       B.fill[layout].count, d_f));
         SE2P_TRY(launch_zero_mode(s, B.sys, J + B.zm_part.first, B.zm_part.count,
                                   J + B.zm_sys.first, B.zm_sys.count));
-        SE2P_TRY(launch_sym_zero(s, B.sys, J + B.sym_pts.first, B.sym_pts.count));
     }
     StageTimer st(ctx, "real");
     for (int kind = 0; kind < kNumKinds; ++kind)

@@ -269,13 +269,11 @@ int real_block_side(int nsx, int nsy, int mean_targets);
 // Real-space launch modes: general blocks, one-cell blocks, pair-symmetric blocks.
 enum RealMode { RM_GENERAL = 0, RM_ONE_CELL = 1, RM_SYM = 2, kRealModes = 3 };
 // k_real_stencil<kind, tier, mode> over the jobs (block indices).  RM_GENERAL
-// and RM_ONE_CELL set u; RM_SYM adds into sacc (zeroed by launch_sym_zero) with
+// and RM_ONE_CELL set u; RM_SYM adds into sacc (zeroed by the density fill) with
 // atomics, and the gather adds sacc to the k-space part (only systems with a
 // k-space part take pair-symmetric sums).
 int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys, const Job* jobs,
                 int njobs, double* u);
-// sacc = 0, job (system, block of 256 sorted points).
-int launch_sym_zero(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs);
 // pos[perm[k]] = k for k < n.
 int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
@@ -379,7 +377,6 @@ struct BatchExec {
     JobRange zm_part, zm_sys;
     JobRange sdens;       // spreading densities: blocks of 256 binned points
     JobRange real[kNumKinds][2][kRealModes];  // [kind][tier][RealMode]
-    JobRange sym_pts;     // pair-symmetric systems: blocks of 256 sorted points
     JobRange spread[2][kNumP];  // [c][P / 2 - 1]
     JobRange ffty[2][kFftClasses], fftx[2][kFftClasses], tr_fwd[2];  // [c][line class]
     JobRange filter[kNumKinds];  // by kind
