@@ -98,7 +98,7 @@ __device__ __forceinline__ void slp_radial(double r2, double xi2, double& E, dou
 // (xi^2 r^2 < 1), decided by the caller.
 template <int TIER, bool NEAR>
 __device__ __forceinline__ void slp_pair(double dx, double dy, double r2, double xi2,
-                                         double ixi2, double f1, double f2, double& a1,
+                                         double f1, double f2, double& a1,
                                          double& a2)
 {
     double E, c;
@@ -113,7 +113,7 @@ __device__ __forceinline__ void slp_pair(double dx, double dy, double r2, double
 // pair term is even in r).
 template <int TIER, bool NEAR>
 __device__ __forceinline__ void slp_pair2(double dx, double dy, double r2, double xi2,
-                                          double ixi2, double f1, double f2, double g1,
+                                          double f1, double f2, double g1,
                                           double g2, double& a1, double& a2, double& b1,
                                           double& b2)
 {
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
                         double b1, b2;
-                        slp_pair2<TIER, false>(dx, dy, r2, xi2, ixi2, g.x, g.y, fi1, fi2, a1, a2,
+                        slp_pair2<TIER, false>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1, a2,
                                                b1, b2);
                         atomicAdd(&racc[idx].x, b1);
                         atomicAdd(&racc[idx].y, b2);
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
                         const double2 g = bfd[idx];
-                        slp_pair<TIER, false>(dx, dy, r2, xi2, ixi2, g.x, g.y, a1, a2);
+                        slp_pair<TIER, false>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
                     } else if (KIND == SE2P_DLP) {
                         dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
                     } else if (KIND == SE2P_SLP_PRESSURE) {
@@ -678,12 +678,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             }
                         } else if (SYM) {
                             double b1, b2;
-                            slp_pair2<TIER, true>(dx, dy, r2, xi2, ixi2, g.x, g.y, fi1, fi2, a1,
+                            slp_pair2<TIER, true>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1,
                                                   a2, b1, b2);
                             atomicAdd(&racc[idx].x, b1);
                             atomicAdd(&racc[idx].y, b2);
                         } else {
-                            slp_pair<TIER, true>(dx, dy, r2, xi2, ixi2, g.x, g.y, a1, a2);
+                            slp_pair<TIER, true>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
                         }
                     }
                 }

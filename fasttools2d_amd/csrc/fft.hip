@@ -368,7 +368,9 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
             case 8: SE2P_STAGE(8); break;
 #define SE2P_COMPOSITE(RR)             \
     case RR:                           \
-        if constexpr (COMP) SE2P_STAGE(RR); \
+        if constexpr (COMP) {          \
+            SE2P_STAGE(RR);            \
+        }                              \
         break;
             SE2P_COMPOSITE(6) SE2P_COMPOSITE(9) SE2P_COMPOSITE(10) SE2P_COMPOSITE(12)
             SE2P_COMPOSITE(15) SE2P_COMPOSITE(16)
