@@ -175,3 +175,20 @@ def test_stresslet_ignores_the_flag(gpu_ctx):
         finally:
             plan.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_full_size_tuned_split_matches_fixed_order(gpu_ctx):
+    """The benchmark's configuration at its full size (N = 1e6 sources = targets,
+    fast E1 tier, the tuner's split): pair-symmetric and fixed-order sums agree to
+    rounding, eagerly and replayed from a graph."""
+    from fasttools2d_amd.tune import make_params
+    N = 1_000_000
+    rng = np.random.default_rng(1)
+    p = rng.random((N, 2)) - 0.5
+    f = rng.standard_normal((N, 2))
+    prm = make_params(3.8 * 201, 201, 201, 2048, 2048, 14, 1.0, 1)
+    ref = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, False)[0]
+    for g in (False, True):
+        u = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, True, graph=g, reps=2)
+        for v in u:
+            assert _rel(v, ref) < REL
