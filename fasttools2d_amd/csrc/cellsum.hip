@@ -81,11 +81,11 @@ __device__ __forceinline__ void slp_radial(double r2, double xi2, double& E, dou
     const double e = TIER == kE1Fast ? exp_neg(x) : exp(-x);
     if (NEAR) {
         E = e1_near<TIER>(x);
-        c = e * rcp64(r2);
+        c = e * (TIER == kE1Fast ? rcp64_1(r2) : rcp64(r2));
     } else {
         using C = E1Coeffs<TIER>;
         const double pr = horner_rev(C::farp, x), qr = horner_rev(C::farq, x);
-        const double eD = e * rcp64(x * qr);
+        const double eD = e * (TIER == kE1Fast ? rcp64_1(x * qr) : rcp64(x * qr));
         E = eD * pr;
         c = eD * qr * xi2;
     }

@@ -91,6 +91,14 @@ __device__ __forceinline__ double rcp64(double x)
     return fma(y, r, y);
 }
 
+// 1/x to about 1e-14 relative: the hardware estimate (about 2^-23) and one
+// Newton step; for the fast tier, whose fits carry 1e-12.
+__device__ __forceinline__ double rcp64_1(double x)
+{
+    const double y = __builtin_amdgcn_rcp(x);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+
 // The two regimes separately: near for x < 1, far (given e = exp(-x), t = 1/x) for x >= 1.
 // ln(x) for finite x > 0 (normal or not): x = m 2^k with m in [sqrt(1/2),
 // sqrt(2)), ln(m) = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, summed as

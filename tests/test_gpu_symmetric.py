@@ -192,3 +192,21 @@ def test_full_size_tuned_split_matches_fixed_order(gpu_ctx):
         u = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, True, graph=g, reps=2)
         for v in u:
             assert _rel(v, ref) < REL
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_fast_e1_tier_within_its_bound(gpu_ctx, oracle, sym):
+    """The fast E1 tier (library log, exponential and reciprocals, E1 fits with
+    relative error below 1e-12) against the full tier and the oracle on the same
+    split: the sums agree to 1e-12, near (xi r < 1) and far pairs both present."""
+    N = 20000
+    p = uniform_points(N, 1.0, 1.0, seed=91)
+    f = normal_density(N, seed=92)
+    prm = _prm(N, 1.0, 1.0, 33, 33)
+    full = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, sym)[0]
+    fast = _apply(gpu_ctx, p, f, 1.0, 1.0, dict(prm, e1_tier=1), sym)[0]
+    assert _rel(fast, full) < 1e-12
+    r = oracle.slp_real(p, p, f, prm["xi"], 33, 33, 1.0, 1.0)
+    r += oracle.slp_kspace(p, p, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], 1.0, 1.0,
+                           prm["w"], prm["P"])
+    assert _rel(fast, r) < 1e-12
