@@ -153,6 +153,28 @@ def measured_traffic(kernel, prm):
     return None
 
 
+def oracle_check(p, f, sub, Lx=1.0, Ly=1.0, tol=1e-13):
+    """The checker: the CPU oracle's spectral Ewald sum (C restatement of the
+    reference stages) at tolerance `tol`, all sources, at the targets p[sub].
+    Parameters from the reference recipe, the grid raised to the next
+    2-3-5-smooth size (a finer grid only lowers the oracle's own error; the
+    oracle's FFT is slow on large prime factors)."""
+    from oracle import oracle as O
+    from fasttools2d_amd.tune import smooth_up
+    n = len(p)
+    prm = O.params("slp", 2 * n, O.density_q(f), Lx, Ly, 24, O.default_Nb(2 * n), tol)
+    M = smooth_up(prm["Mx"], 8192)
+    P = prm["P"]
+    w = P * Lx / M / 2
+    eta = (2 * prm["xi"] * w / (0.95 * math.sqrt(math.pi * P))) ** 2
+    pt = p[sub]
+    t0 = time.perf_counter()
+    u = O.slp_real(p, pt, f, prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly)
+    u += O.slp_kspace(p, pt, prm["xi"], eta, f, M, M, Lx, Ly, w, P)
+    return u, {"xi": float(prm["xi"]), "nside": prm["nside_x"], "M": M, "P": P, "tol": tol,
+               "targets": int(len(sub)), "s": time.perf_counter() - t0}
+
+
 def cpu_baseline(p, f, prm, frac=0.05):
     """Oracle (C/OpenMP port of the reference stages) on the host cores, on a
     bounded sample: all sources, a random fraction of the targets; spreading and
@@ -250,44 +272,22 @@ def profile_only(ctx, dev, ps, p, f, args):
                       "stage_ms": stages}))
 
 
-def capped_params(n, Lx, Ly, tol, P=24):
-    """Reference recipe for an SLP system with sources = targets, its points-per-
-    cell option raised until the grid fits the library's 4096-point FFT lines
-    (larger cells, smaller xi, coarser grid)."""
-    from fasttools2d_amd import ewald_params
-    prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, P=P)
-    nb = n / (prm["nside_x"] * prm["nside_y"])
-    while max(prm["Mx"], prm["My"]) > 4096:
-        nb *= 1.25
-        prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, Nb=nb, P=P)
-        if prm["nside_x"] < 1 or prm["nside_y"] < 1:
-            raise ValueError(f"no grid within 4096 per side for N={n}, box {Lx}x{Ly}, tol {tol}")
-    return prm
-
-
 def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
-    """The batched serving workload: box aspect 1..4 (half steps: the parameter
-    recipe needs a rational aspect, and its numerator multiplies the grid), N log-uniform in [nmin, nmax],
-    tolerance log-uniform in [1e-10, 1e-4], sources = targets.  Parameters per
-    system at its tolerance: recipe "rule" is tune.rule_params (the tuner's
-    calibrated constants as functions of the tolerance); "ref" is the reference
-    recipe with the fast E1 tier and its points-per-cell option raised where the
-    grid would exceed 4096 per side."""
-    from fasttools2d_amd.tune import rule_params
-    rng = np.random.default_rng(seed)
-    aspects = np.arange(2, 9) / 2.0
-    out = []
-    for _ in range(nsys):
-        Lx, Ly = float(rng.choice(aspects)), 1.0
-        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
-        tol = float(10 ** rng.uniform(-10, -4))
-        p = np.ascontiguousarray(rng.random((n, 2)) * [Lx, Ly] - [Lx / 2, Ly / 2])
+    """The batched serving workload (fasttools2d_amd.workloads.serving_systems)
+    with parameters per system at its tolerance: recipe "rule" is
+    tune.rule_params (the tuner's calibrated constants as functions of the
+    tolerance); "ref" is the reference recipe with the fast E1 tier and its
+    points-per-cell option raised where the grid would exceed 4096 per side."""
+    from fasttools2d_amd.tune import capped_params, rule_params
+    from fasttools2d_amd.workloads import serving_systems as gen
+    out = gen(nsys, seed=seed, nmin=nmin, nmax=nmax)
+    for y in out:
+        n = len(y["psrc"])
         if recipe == "rule":
-            prm = rule_params("slp", n, n, Lx, Ly, tol, symmetric=1)
+            y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1)
         else:
-            prm = capped_params(n, Lx, Ly, tol)
-            prm["e1_tier"] = 1
-        out.append(dict(kind="slp", psrc=p, ptar=p, nrm=None, Lx=Lx, Ly=Ly, prm=prm, tol=tol))
+            y["prm"] = capped_params(n, y["Lx"], y["Ly"], y["tol"])
+            y["prm"]["e1_tier"] = 1
     return out
 
 
@@ -300,7 +300,7 @@ def batch_bench(args):
     import fasttools2d_amd as ft
     from fasttools2d_amd.batch import Batch
     from fasttools2d_amd.plan import DeviceArray, Plan
-    from fasttools2d_amd.tune import rel_err
+    from fasttools2d_amd.tune import capped_params, rel_err
     systems = serving_systems(args.nsys, recipe=args.recipe)
     ntot = sum(len(y["psrc"]) for y in systems)
     log(f"bench-batch: {len(systems)} systems, {ntot} points")
@@ -452,7 +452,7 @@ def main():
     N = args.n
     log(f"bench: building workload N={N}")
     p, f = workload(N)
-    Q = float((f ** 2).sum() + 1)
+    Q = ft.density_q(f[:, 0], f[:, 1])
     prm = ewald_params("slp", N, N, Q, 1.0, 1.0, P=24, tol=args.tol)
     log(f"bench: reference recipe at tol {args.tol}: {prm}")
     ctx = ft.Context(0)
@@ -492,6 +492,8 @@ def main():
         tprm, report = autotune(ctx, "slp", p, p, f, 1.0, 1.0, args.tol, u_ref=u_ref, log=log)
     log(f"bench: tuned parameters {tprm} report {json.dumps(report)}")
     plan = Plan(ctx, "slp", p, p, 1.0, 1.0, tprm)
+    used = plan.params_in_use()
+    log(f"bench: pair-symmetric sums in use: {used['symmetric']}")
     df = DeviceArray.from_host(ctx, f)
     du = DeviceArray(ctx, (N, 2))
     apply = lambda: plan.apply_device(df.ptr, du.ptr)  # noqa: E731
@@ -513,6 +515,14 @@ def main():
         f"streams {ms_streams:.3f}, graph replay {ms_graph:.3f} ms/eval")
     u_tuned = du.download()
     err_tuned = rel_err(u_tuned, u_ref)
+    # a second, held-out density (the tuner saw only f)
+    f2 = workload(N, seed=2)[1]
+    df2 = DeviceArray.from_host(ctx, f2)
+    du2 = DeviceArray(ctx, (N, 2))
+    plan.apply_device(df2.ptr, du2.ptr)
+    u_tuned2 = du2.download()
+    df2.free()
+    du2.free()
     # Stage launch durations: HIP events on each stage's stream over a window of
     # the same applies with every stage on one stream.  Events recorded between
     # the launches of the two concurrent streams perturb their schedule (a window
@@ -544,9 +554,11 @@ def main():
         "config": {"workload": "SLP spectral Ewald, N=1e6 sources=targets, unit box, tol 1e-8",
                    "path": "tuned: autotuned split, plan reuse, pair-symmetric real space, "
                            f"real-space and k-space passes on two streams ({mode}), FP64",
-                   "params": {k: tprm[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier",
+                   "params": {k: used[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier",
                                                    "symmetric")}},
         "rel_err": err_tuned,
+        "rel_err_vs": "GPU plan at tol 1e-13 (all targets)",
+        "rel_err_gpu_reference": err_tuned,
         "plain": {"evals_per_s": 1e3 / ms_plain, "ms": ms_plain, "rel_err": err_plain,
                   "params": {k: prm[k] for k in ("xi", "nside_x", "Mx", "P")},
                   "stage_ms": stages_plain, "real_tflops": plain_real_tf},
@@ -565,6 +577,21 @@ def main():
         "tune_report": report,
     }
     if not args.no_cpu:
+        # the checker: CPU oracle at tol 1e-13 on a 1e4-target subset, all sources,
+        # for the tuned density and the held-out one
+        sub = np.sort(np.random.default_rng(7).choice(N, size=min(N, 10_000), replace=False))
+        o1, oinfo = oracle_check(p, f, sub)
+        o2, _ = oracle_check(p, f2, sub)
+        oracle = {"subset_targets": int(len(sub)), "oracle": oinfo,
+                  "tuned": rel_err(u_tuned[sub], o1), "tuned_heldout": rel_err(u_tuned2[sub], o2),
+                  "plain": rel_err(u_plain[sub], o1),
+                  "gpu_reference": rel_err(u_ref[sub], o1)}
+        log(f"bench: errors vs the CPU oracle {json.dumps(oracle)}")
+        out["rel_err"] = max(oracle["tuned"], oracle["tuned_heldout"])
+        out["rel_err_vs"] = "CPU oracle (tol 1e-13, all sources, 1e4-target subset), worst of " \
+                            "two densities"
+        out["oracle_errors"] = oracle
+        out["plain"]["rel_err"] = oracle["plain"]
         log("bench: CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(p, f, prm)
         log(f"bench: cpu {out['cpu_baseline']}")

@@ -12,7 +12,8 @@ from ._lib import (Context, Se2pError, default_context, device_count, LIB_PATH, 
 from . import periodic_ewald as _pe
 from .freespace import stokesDLPfmm, stokesSLPfmm  # noqa: F401
 from .periodic_ewald import (StokesDLP_ewald_2p, StokesDLP_pressure_ewald_2p,  # noqa: F401
-                             StokesSLP_ewald_2p, StokesSLP_pressure_ewald_2p, ewald_params,
+                             StokesSLP_ewald_2p, StokesSLP_pressure_ewald_2p, density_q,
+                             ewald_params,
                              mex_stokes_dlp_kspace, mex_stokes_dlp_pressure_kspace,
                              mex_stokes_dlp_pressure_real, mex_stokes_dlp_real,
                              mex_stokes_slp_kspace, mex_stokes_slp_pressure_kspace,
@@ -22,7 +23,7 @@ __all__ = [
     "Context", "Se2pError", "default_context", "device_count", "LIB_PATH", "SE2P_SLP",
     "SE2P_DLP", "SE2P_SLP_PRESSURE", "SE2P_DLP_PRESSURE", "StokesSLP_ewald_2p",
     "StokesDLP_ewald_2p", "StokesSLP_pressure_ewald_2p", "StokesDLP_pressure_ewald_2p",
-    "ewald_params", "mex_stokes_slp_real", "mex_stokes_slp_kspace", "mex_stokes_dlp_real",
+    "density_q", "ewald_params", "mex_stokes_slp_real", "mex_stokes_slp_kspace", "mex_stokes_dlp_real",
     "mex_stokes_dlp_kspace", "mex_stokes_slp_pressure_real", "mex_stokes_slp_pressure_kspace",
     "mex_stokes_dlp_pressure_real", "mex_stokes_dlp_pressure_kspace",
 ]

@@ -71,8 +71,9 @@ EXPORTS = (
     "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
     "se2p_prof_reset", "se2p_prof_get", "se2p_assign", "se2p_find_closest_node",
     "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1", "se2p_plan_create",
-    "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy", "se2p_batch_create",
-    "se2p_batch_sizes", "se2p_batch_apply", "se2p_batch_apply_graph", "se2p_batch_destroy",
+    "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy", "se2p_plan_params",
+    "se2p_batch_create", "se2p_batch_sizes", "se2p_batch_apply", "se2p_batch_apply_graph",
+    "se2p_batch_destroy", "se2p_batch_params",
     "se2p_slp_pressure_real", "se2p_slp_pressure_kspace", "se2p_slp_pressure_ewald",
     "se2p_dlp_pressure_real", "se2p_dlp_pressure_kspace", "se2p_dlp_pressure_ewald",
 ) + tuple(f"se2p_{layer}_{q}_{part}" for layer in ("slp", "dlp")
@@ -161,12 +162,14 @@ def lib():
             "se2p_plan_apply": [P, P, P],
             "se2p_plan_apply_graph": [P, P, P],
             "se2p_plan_destroy": [P],
+            "se2p_plan_params": [P, ctypes.POINTER(Params)],
             "se2p_batch_create": [P, I, ctypes.POINTER(System), ctypes.POINTER(P)],
             "se2p_batch_sizes": [P, ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_longlong)],
             "se2p_batch_apply": [P, P, P],
             "se2p_batch_apply_graph": [P, P, P],
             "se2p_batch_destroy": [P],
+            "se2p_batch_params": [P, I, ctypes.POINTER(Params)],
             "se2p_slp_pressure_real": [P, P, P, P, I, I, D, I, I, D, D, P],
             "se2p_slp_pressure_kspace": [P, P, P, P, D, D, I, I, I, I, D, D, D, I, P],
             "se2p_slp_pressure_ewald": [P, P, P, P, P, P, P, I, I, D, D, ctypes.POINTER(Options),

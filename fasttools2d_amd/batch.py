@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import System, check, lib
+from ._lib import Params, System, check, lib
 from .plan import DeviceArray, params_struct
 
 
@@ -66,6 +66,13 @@ class Batch:
         u = self.u.download() if self.nu else np.zeros(0)
         return [u[self.u_off[i]:self.u_off[i + 1]].reshape(-1, self.nc[i]) if self.nc[i] > 1
                 else u[self.u_off[i]:self.u_off[i + 1]] for i in range(len(self.nt))]
+
+    def params_in_use(self, i):
+        """Parameters system i runs with (se2p_batch_params); `symmetric` is 1
+        only when its pair-symmetric real-space sums are in use."""
+        out = Params()
+        check(lib().se2p_batch_params(self._h, int(i), ctypes.byref(out)))
+        return out.as_dict()
 
     def close(self):
         if getattr(self, "_h", None):

@@ -379,6 +379,17 @@ double wrap1(double x, double L)
     return v - L / 2;
 }
 
+// Q = sum(sum(f.^2)) + 1 with f = [f1'; f2'] (2 x N, StokesSLP_ewald_2p.m:82,87):
+// sum over a matrix adds down each column, f1(i)^2 + f2(i)^2, and the outer sum
+// adds those N column sums in index order.
+double density_q(const double* f1, const double* f2, int n)
+{
+#pragma clang fp contract(off)
+    double q = 0;
+    for (int i = 0; i < n; ++i) q += f1[i] * f1[i] + f2[i] * f2[i];
+    return q + 1;
+}
+
 int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc,
                  const double* xtar, const double* ytar, const double* n1, const double* n2,
                  const double* f1, const double* f2, int Ns, int Nt, double Lx, double Ly,
@@ -405,13 +416,7 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
         f[2 * i] = f1[i];
         f[2 * i + 1] = f2[i];
     }
-    // Q = sum(sum(f.^2)) + 1: MATLAB sums each row, then the two row sums.
-    double q1 = 0, q2 = 0;
-    for (int i = 0; i < Ns; ++i) {
-        q1 += f1[i] * f1[i];
-        q2 += f2[i] * f2[i];
-    }
-    const double Q = (q1 + q2) + 1;
+    const double Q = density_q(f1, f2, Ns);
     for (int i = 0; i < Nt; ++i) {
         pt[2 * i] = wrap1(xtar[i], Lx);
         pt[2 * i + 1] = wrap1(ytar[i], Ly);
@@ -500,22 +505,19 @@ int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* y
             nn[2 * i + 1] = n2[i];
         }
     }
-    double q1 = 0, q2 = 0;
     for (int i = 0; i < Ns; ++i) {
         ps[2 * i] = wrap1(xsrc[i], Lx);
         ps[2 * i + 1] = wrap1(ysrc[i], Ly);
         f[2 * i] = f1[i];
         f[2 * i + 1] = f2[i];
-        q1 += f1[i] * f1[i];
-        q2 += f2[i] * f2[i];
     }
     for (int i = 0; i < Nt; ++i) {
         pt[2 * i] = wrap1(xtar[i], Lx);
         pt[2 * i + 1] = wrap1(ytar[i], Ly);
     }
     se2p_params prm;
-    SE2P_TRY(params_ref(kind == SE2P_DLP_PRESSURE ? SE2P_DLP : SE2P_SLP, Ns, Nt, (q1 + q2) + 1,
-                        Lx, Ly, opt, &prm));
+    SE2P_TRY(params_ref(kind == SE2P_DLP_PRESSURE ? SE2P_DLP : SE2P_SLP, Ns, Nt,
+                        density_q(f1, f2, Ns), Lx, Ly, opt, &prm));
     SE2P_TRY(check_params(&prm, Lx, Ly));
     if (xi_out) *xi_out = prm.xi;
     if (Nt == 0) return SE2P_OK;
@@ -1224,6 +1226,21 @@ int se2p_plan_destroy(se2p_plan* plan)
     return SE2P_OK;
 }
 
+// Parameters of a geometry as its evaluations run them: symmetric reports
+// whether pair-symmetric real-space sums are in use.
+static void params_in_use(const Geom& G, se2p_params* used)
+{
+    *used = G.prm;
+    used->symmetric = G.sym ? 1 : 0;
+}
+
+int se2p_plan_params(const se2p_plan* plan, se2p_params* used)
+{
+    if (!plan || !used) return set_error(SE2P_ERR_ARG, "null plan or output");
+    params_in_use(plan->G, used);
+    return SE2P_OK;
+}
+
 int se2p_batch_create(se2p_ctx* ctx, int nsys, const se2p_system* systems, se2p_batch** out)
 {
     if (!out) return set_error(SE2P_ERR_ARG, "null output pointer");
@@ -1290,6 +1307,15 @@ int se2p_batch_apply_graph(se2p_batch* batch, const double* d_f, double* d_u)
     if (!batch) return set_error(SE2P_ERR_ARG, "null batch");
     if ((batch->nf && !d_f) || (batch->nu && !d_u)) return set_error(SE2P_ERR_ARG, "null array");
     return graph_apply(batch->ctx, &batch->gc, batch->B, d_f, d_u);
+}
+
+int se2p_batch_params(const se2p_batch* batch, int sys, se2p_params* used)
+{
+    if (!batch || !used) return set_error(SE2P_ERR_ARG, "null batch or output");
+    if (sys < 0 || sys >= (int)batch->G.size())
+        return set_error(SE2P_ERR_ARG, "system %d of %d", sys, (int)batch->G.size());
+    params_in_use(batch->G[sys], used);
+    return SE2P_OK;
 }
 
 int se2p_batch_destroy(se2p_batch* batch)

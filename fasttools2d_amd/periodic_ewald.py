@@ -200,6 +200,18 @@ def _options(P=24, Nb=None, tol=1e-16, verbose=0):
     return o
 
 
+def density_q(f1, f2):
+    """Q = sum(sum(f.^2)) + 1 of the MATLAB drivers, f = [f1'; f2'] (2 x N;
+    StokesSLP_ewald_2p.m:82,87): each column's f1(i)^2 + f2(i)^2, then those
+    sums added in index order (a sequential sum, as the library's C++ drivers
+    add them)."""
+    a1 = np.asarray(f1, dtype=np.float64).ravel()
+    a2 = np.asarray(f2, dtype=np.float64).ravel()
+    if a1.size == 0:
+        return 1.0
+    return float(np.cumsum(a1 * a1 + a2 * a2)[-1] + 1.0)
+
+
 def ewald_params(kind, Nsrc, Ntar, Q, Lx, Ly, P=24, Nb=None, tol=1e-16):
     """Reference parameter recipe (StokesSLP_ewald_2p.m:84-104 / StokesDLP_ewald_2p.m:104-125;
     each pressure uses its family's recipe, StokesSLP_pressure_ewald_2p.m:83-104 and
@@ -517,7 +529,7 @@ def _derived_driver(kind, xsrc, ysrc, xtar, ytar, f1, f2, n1, n2, Lx, Ly, P, Nb,
     ps = np.ascontiguousarray(np.stack([_mwrap(xs, Lx), _mwrap(ys, Ly)], 1))
     pt = np.ascontiguousarray(np.stack([_mwrap(xt, Lx), _mwrap(yt, Ly)], 1))
     ff = np.ascontiguousarray(np.stack([a1, a2], 1))
-    prm = ewald_params(kind, Ns, Nt, float((ff ** 2).sum() + 1), Lx, Ly, P=P, Nb=Nb, tol=tol)
+    prm = ewald_params(kind, Ns, Nt, density_q(a1, a2), Lx, Ly, P=P, Nb=Nb, tol=tol)
     R = _derived(kind, "real", ps, pt, ff, nn,
                  (prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly), ctx)
     K = _derived(kind, "kspace", ps, pt, ff, nn,

@@ -92,6 +92,13 @@ class Plan:
         self.apply_device(self.f.ptr, self.u.ptr, graph)
         return self.u.download()
 
+    def params_in_use(self):
+        """The parameters evaluations run with (se2p_plan_params): `symmetric`
+        is 1 only when pair-symmetric real-space sums are in use."""
+        out = Params()
+        check(lib().se2p_plan_params(self._h, ctypes.byref(out)))
+        return out.as_dict()
+
     def close(self):
         if self._h:
             lib().se2p_plan_destroy(self._h)

@@ -69,18 +69,6 @@ def fft_stages(n):
     return best.get(n)
 
 
-def fft_alternatives(m, max_n=4096, slack=1.12):
-    """Even 2-3-5-smooth grid sizes above m, up to slack * m, whose line FFT takes
-    fewer stages than m's (a larger grid only lowers the k-space error); the
-    tuner times them, since a stage's cost depends on its radices and line class."""
-    out, st0 = [], fft_stages(m)
-    for n in range(m + 2, min(max_n, int(slack * m)) + 1, 2):
-        st = fft_stages(n)
-        if st is not None and st < st0 and all(fft_stages(o) > st for o in out):
-            out.append(n)
-    return out
-
-
 def eta_w(xi, P, L, M):
     """Gaussian parameters of the reference recipe (StokesSLP_ewald_2p.m:90,103-104)."""
     w = P * L / M / 2
@@ -110,25 +98,48 @@ class Tuner:
         self.df = DeviceArray.from_host(ctx, f)
         self.du = DeviceArray(ctx, (len(pt), 2))
 
-    def run(self, prm, reps=0):
+    def run(self, prm, reps=0, f=None):
+        """(u, seconds per apply or None); f: another density (host array)."""
         plan = Plan(self.ctx, self.kind, self.ps, self.pt, self.Lx, self.Ly, prm, nrm=self.nrm)
         try:
-            plan.apply_device(self.df.ptr, self.du.ptr)
+            self.used_symmetric = plan.params_in_use()["symmetric"]
+            df = self.df if f is None else DeviceArray.from_host(self.ctx, f)
+            plan.apply_device(df.ptr, self.du.ptr)
             self.ctx.sync()
             u = self.du.download()
+            if f is not None:
+                df.free()
             t = None
             if reps:
-                # stream launches: on this runtime they replay faster than a HIP graph
-                plan.apply_device(self.df.ptr, self.du.ptr)
-                self.ctx.sync()
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    plan.apply_device(self.df.ptr, self.du.ptr)
-                self.ctx.sync()
-                t = (time.perf_counter() - t0) / reps
+                t = self._time(plan, reps)
         finally:
             plan.close()
         return u, t
+
+    def _time(self, plan, reps):
+        # stream launches: on this runtime they replay faster than a HIP graph
+        plan.apply_device(self.df.ptr, self.du.ptr)
+        self.ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.apply_device(self.df.ptr, self.du.ptr)
+        self.ctx.sync()
+        return (time.perf_counter() - t0) / reps
+
+    def retime(self, prms, reps, rounds):
+        """Median seconds per apply of each parameter set, timed in `rounds`
+        alternating windows (a drift of the clock hits every candidate alike)."""
+        plans = [Plan(self.ctx, self.kind, self.ps, self.pt, self.Lx, self.Ly, q, nrm=self.nrm)
+                 for q in prms]
+        try:
+            ts = [[] for _ in plans]
+            for _ in range(rounds):
+                for i, pl in enumerate(plans):
+                    ts[i].append(self._time(pl, reps))
+            return [float(np.median(t)) for t in ts]
+        finally:
+            for pl in plans:
+                pl.close()
 
     def close(self):
         self.df.free()
@@ -142,22 +153,60 @@ def _aspect(Lx, Ly):
     return fr.numerator, fr.denominator
 
 
+def fft_alternatives(m, max_n=4096, slack=1.12, low=0.9):
+    """Even 2-3-5-smooth grid sizes near m (from low * m to slack * m) whose
+    line FFT takes the fewest stages in its line class (up to two per class,
+    the nearest to m): a larger grid only lowers the k-space error, a smaller
+    one is cheaper and the tuner measures its error, which decides.  Stage
+    counts of different classes are not comparable (their radices differ), so
+    every class present in the range contributes; the tuner times them."""
+    by_cls = {}
+    for n in range(max(4, int(math.ceil(low * m))), min(max_n, int(slack * m)) + 1):
+        if n % 2 or n == m:
+            continue
+        st = fft_stages(n)
+        if st is None:
+            continue
+        cls = 0 if n <= 2048 else 1 if n <= 2560 else 2
+        by_cls.setdefault(cls, []).append((st, abs(n - m), n))
+    out = []
+    for cls in sorted(by_cls):
+        c = by_cls[cls]
+        smin = min(st for st, _, _ in c)
+        out += [n for _, _, n in sorted(x for x in c if x[0] == smin)[:2]]
+    return out
+
+
 def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
-             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=8):
-    """Returns (params, report).  ps, pt, f, nrm are (N, 2) host arrays."""
+             nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=8, max_grid=4096,
+             heldout_seed=12345, final_rounds=5):
+    """Returns (params, report).  ps, pt, f, nrm are (N, 2) host arrays.
+
+    max_grid caps the grid per side.  After the sweep the fastest few accepted
+    configurations are timed again in alternating windows (median of
+    final_rounds) and the fastest by that median is checked on a held-out
+    density (a fresh N(0,1) draw from heldout_seed, scored against its own
+    accurate solution): the calibration and the sweep only ever see f, so this
+    is the out-of-sample error a new density gets.  A candidate whose held-out
+    error exceeds tol is passed over for the next one.  report records the
+    sweep, both errors, whether pair-symmetric sums were in use, and the re-timed
+    medians."""
     log = log or (lambda s: None)
     A, B = _aspect(Lx, Ly)
     Ns, Nt = len(ps), len(pt)
     sym = 1 if _lib.kind_code(kind) == _lib.SE2P_SLP and pt is ps else 0
     T = Tuner(ctx, kind, ps, pt, f, Lx, Ly, nrm=nrm, log=log)
-    report = {"tol": tol}
+    report = {"tol": tol, "max_grid": max_grid}
+    from .periodic_ewald import density_q, ewald_params
+
+    def reference_params(dens):
+        return ewald_params(kind, Ns, Nt, density_q(dens[:, 0], dens[:, 1]), Lx, Ly, P=24,
+                            tol=1e-13)
+
     try:
         # 1. reference solution
         if u_ref is None:
-            from .periodic_ewald import ewald_params
-            Q = float((f ** 2).sum() + 1)
-            ref_prm = ewald_params(kind, Ns, Nt, Q, Lx, Ly, P=24, tol=1e-13)
-            u_ref, _ = T.run(ref_prm)
+            u_ref, _ = T.run(reference_params(f))
         npts = Ns + Nt
         # base split: the reference's default cell occupancy
         nb = min(24 * round(math.log2(npts)), npts / 4)
@@ -169,11 +218,18 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             nsx, nsy = a * A, a * B
             rc = Lx / nsx
             xi = c_r / rc
-            M0 = smooth_up(c_k * xi * Lx / math.pi / A)
+            M0 = smooth_up(c_k * xi * Lx / math.pi / A, 4096)
             if M1 is not None:
                 M0 = M1
             return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym)
 
+        def fits(prm):
+            return prm["Mx"] <= max_grid and prm["My"] <= max_grid
+
+        # the calibration runs keep the grid within max_grid by lowering the
+        # base split where the exact k space would not fit
+        while a0 > 1 and not fits(prm_for(a0, c_exact_r, c_exact_k, P_exact)):
+            a0 -= 1
         # 2. real-space constant (and the errors of the next two, for step 8)
         c_r, rerr = None, {}
         for c in (3.0, 3.3, 3.5, 3.6, 3.7, 3.8, 3.9, 4.0, 4.2, 4.6, 5.0, 5.5):
@@ -193,7 +249,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         for P in (8, 10, 12, 14, 16, 18, 20, 24):
             for ck in (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0):
                 prm = prm_for(a0, c_exact_r, ck, P)
-                if prm["Mx"] > 4096:
+                if not fits(prm):
                     continue
                 u, _ = T.run(prm)
                 e = rel_err(u, u_ref)
@@ -222,42 +278,48 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                 prm = prm_for(a, cr, c_k, P, tier, M1)
             except ValueError:
                 return
-            if prm["Mx"] > 4096 or prm["My"] > 4096:
+            if not fits(prm):
                 return
             u, t = T.run(prm, reps=reps)
             e = rel_err(u, u_ref)
             log(f"tune: nside {prm['nside_x']} xi {prm['xi']:.1f} M {prm['Mx']} P {P} -> "
-                f"{t * 1e3:.3f} ms err {e:.2e}")
-            results.append((t, e, prm, a))
+                f"{t * 1e3:.3f} ms err {e:.2e} symmetric {T.used_symmetric}")
+            results.append((t, e, prm, a, T.used_symmetric))
+
+        def accepted():
+            return [r for r in results if r[1] <= 0.8 * tol]
 
         for s in nside_scales:
             timed(max(1, int(round(a0 * s))))
         # 5. refine around the fastest split that met the tolerance
-        ok = [r for r in results if r[1] <= 0.8 * tol]
-        if ok:
-            a_best = min(ok, key=lambda r: r[0])[3]
+        if accepted():
+            a_best = min(accepted(), key=lambda r: r[0])[3]
             for s in (0.88, 0.94, 1.06, 1.12):
                 timed(max(1, int(round(a_best * s))))
         # 6. the next cheapest supports by the cost model, timed at the best split
-        ok = [r for r in results if r[1] <= 0.8 * tol]
-        if ok:
-            a_best = min(ok, key=lambda r: r[0])[3]
+        if accepted():
+            a_best = min(accepted(), key=lambda r: r[0])[3]
             for _, P2, ck2 in sorted(kcands)[1:3]:
                 timed(a_best, P2, ck2)
-        # 7. larger grids whose line FFTs take fewer stages, at the best point
-        ok = [r for r in results if r[1] <= 0.8 * tol]
-        if ok:
-            _, _, pb, a_best = min(ok, key=lambda r: r[0])
+        # 7. grids near the best one whose line FFTs take fewer stages, at the
+        # best point and at the runner-up split
+        ok = sorted(accepted(), key=lambda r: r[0])
+        seen_a = []
+        for _, _, pb, a_b, _ in ok:
+            if a_b in seen_a:
+                continue
+            seen_a.append(a_b)
             ck_b = next((ck for _, P2, ck in kcands if P2 == pb["P"]), c_k)
-            for m in fft_alternatives(pb["Mx"] // A, 4096 // max(A, B)):
-                timed(a_best, pb["P"], ck_b, m)
+            for m in fft_alternatives(pb["Mx"] // A, max_grid // max(A, B)):
+                timed(a_b, pb["P"], ck_b, m)
+            if len(seen_a) == 2:
+                break
         # 8. the error budget split the other way: a smaller support whose k-space
         # error alone is below 0.7 tol, with the smallest calibrated c_r whose
         # error, added in quadrature, keeps the estimate below 0.7 tol (the
         # measured total still decides)
-        ok = [r for r in results if r[1] <= 0.8 * tol]
-        if ok:
-            _, _, pb, a_best = min(ok, key=lambda r: r[0])
+        if accepted():
+            _, _, pb, a_best, _ = min(accepted(), key=lambda r: r[0])
             for P2 in sorted({q for q, _ in kerr if q < pb["P"]}, reverse=True)[:2]:
                 cks = sorted(ck for q, ck in kerr if q == P2 and kerr[(q, ck)] < 0.7 * tol)
                 if not cks:
@@ -266,13 +328,39 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                 crs = sorted(c for c, er in rerr.items() if math.hypot(er, ek) < 0.7 * tol)
                 if crs:
                     timed(a_best, P2, cks[0], None, crs[0])
-        ok = [r for r in results if r[1] <= 0.8 * tol]
+        ok = sorted(accepted(), key=lambda r: r[0])
         if not ok:
             ok = sorted(results, key=lambda r: r[1])[:1]
-        t, e, prm, _ = min(ok, key=lambda r: r[0])
-        report.update(ms=t * 1e3, err=e, P_timed=prm["P"],
-                      c_r_timed=prm["xi"] * Lx / prm["nside_x"], sweep=[(r[2]["nside_x"], r[2]["Mx"], r[0] * 1e3, r[1])
-                                                for r in results])
+        # 9. the fastest few again, alternating windows, median
+        top = ok[:3]
+        med = T.retime([r[2] for r in top], reps, final_rounds) if len(top) > 1 else \
+            [top[0][0]]
+        order = sorted(range(len(top)), key=lambda i: med[i])
+        log("tune: re-timed " + ", ".join(
+            f"nside {top[i][2]['nside_x']} M {top[i][2]['Mx']}: {med[i] * 1e3:.3f} ms"
+            for i in order))
+        # 10. held-out density
+        f2 = np.random.default_rng(heldout_seed).standard_normal(f.shape)
+        u2_ref, _ = T.run(reference_params(f2), f=f2)
+        chosen, e_held = None, None
+        for i in order + [j for j in range(len(top), len(ok))]:
+            r = top[i] if i < len(top) else ok[i]
+            u2, _ = T.run(r[2], f=f2)
+            e2 = rel_err(u2, u2_ref)
+            log(f"tune: held-out nside {r[2]['nside_x']} M {r[2]['Mx']} -> err {e2:.2e}")
+            if e2 <= tol:
+                chosen, e_held = (r, med[i] if i < len(top) else r[0]), e2
+                break
+        if chosen is None:  # nothing generalises: the most accurate measured
+            r = min(ok, key=lambda q: q[1])
+            chosen, e_held = (r, r[0]), None
+        (t_sweep, e, prm, _, used_sym), t_med = chosen
+        report.update(ms=t_med * 1e3, ms_sweep=t_sweep * 1e3, err=e, err_heldout=e_held,
+                      symmetric_in_use=int(used_sym), P_timed=prm["P"],
+                      c_r_timed=prm["xi"] * Lx / prm["nside_x"],
+                      retimed=[(top[i][2]["nside_x"], top[i][2]["Mx"], med[i] * 1e3)
+                               for i in order],
+                      sweep=[(r[2]["nside_x"], r[2]["Mx"], r[0] * 1e3, r[1]) for r in results])
         return prm, report
     finally:
         T.close()
@@ -317,3 +405,18 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
             raise ValueError(f"no grid within {max_grid} per side for tol {tol}")
         a -= 1
     return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric)
+
+
+def capped_params(n, Lx, Ly, tol, P=24):
+    """Reference recipe for an SLP system with sources = targets, its points-per-
+    cell option raised until the grid fits the library's 4096-point FFT lines
+    (larger cells, smaller xi, coarser grid)."""
+    from .periodic_ewald import ewald_params
+    prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, P=P)
+    nb = n / (prm["nside_x"] * prm["nside_y"])
+    while max(prm["Mx"], prm["My"]) > 4096:
+        nb *= 1.25
+        prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, Nb=nb, P=P)
+        if prm["nside_x"] < 1 or prm["nside_y"] < 1:
+            raise ValueError(f"no grid within 4096 per side for N={n}, box {Lx}x{Ly}, tol {tol}")
+    return prm

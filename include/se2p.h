@@ -354,6 +354,12 @@ int se2p_plan_apply(se2p_plan* plan, const double* d_f, double* d_u);
  * this (d_f, d_u) pair; a new pair re-captures. */
 int se2p_plan_apply_graph(se2p_plan* plan, const double* d_f, double* d_u);
 int se2p_plan_destroy(se2p_plan* plan);
+/* The parameters a plan's evaluations run with: a copy of those it was created
+ * with, except that `symmetric` is 1 only when pair-symmetric sums are in use
+ * (they need the targets to be the sources and every cell ordered by y; a cell
+ * of more than 4096 points or a point outside the box selects the fixed order). */
+int se2p_plan_params(const se2p_plan* plan, se2p_params* used);
+
 
 /* ------------------------------------------------------------------------ */
 /* Batches: many independent periodic systems (own points, box, kind and     */
@@ -380,6 +386,8 @@ int se2p_batch_apply(se2p_batch* batch, const double* d_f, double* d_u);
 /* As se2p_batch_apply, replayed from a HIP graph captured on the first call for
  * this (d_f, d_u) pair; a new pair re-captures. */
 int se2p_batch_apply_graph(se2p_batch* batch, const double* d_f, double* d_u);
+/* As se2p_plan_params for system `sys` (0-based, creation order) of a batch. */
+int se2p_batch_params(const se2p_batch* batch, int sys, se2p_params* used);
 int se2p_batch_destroy(se2p_batch* batch);
 
 /* ------------------------------------------------------------------------ */

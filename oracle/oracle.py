@@ -406,6 +406,16 @@ def free_dlp(f, n, p):
     return u
 
 
+def density_q(f):
+    """Q = sum(sum(f.^2)) + 1 for f = [f1'; f2'] (StokesSLP_ewald_2p.m:82,87):
+    the column sums f1(i)^2 + f2(i)^2 added sequentially in index order."""
+    f = np.asarray(f, dtype=np.float64).reshape(-1, 2)
+    q = 0.0
+    for v in (f[:, 0] * f[:, 0] + f[:, 1] * f[:, 1]).tolist():
+        q += v
+    return q + 1.0
+
+
 def params(kind, npts, Q, Lx, Ly, P, Nb, tol):
     """Reference parameter recipe; kind 'slp' or 'dlp'.  Returns a dict."""
     out = np.zeros(4)

@@ -57,7 +57,7 @@ def test_derived_kspace_matches_oracle(gpu_ctx, oracle, kind, Lx, Ly, tol):
     f = normal_density(N, seed=413)
     dlp = kind.startswith("dlp")
     n = _normals(N, seed=414) if dlp else None
-    p = ft.ewald_params(kind, N, 3000, (f ** 2).sum() + 1, Lx, Ly, tol=tol)
+    p = ft.ewald_params(kind, N, 3000, ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=tol)
     fn = _mex(kind, "kspace")
     tail = (p["Mx"], p["My"], Lx, Ly, p["w"], p["P"])
     if kind in ("slp_pressure_grad", "dlp_pressure_grad", "dlp_vorticity"):  # f (n) before xi
@@ -110,7 +110,7 @@ def test_derived_plans_graphs_and_batch_bitwise(gpu_ctx, oracle):
 
 # ----------------------------------------------------------------- drivers
 def _oracle_parts(oracle, kind, ps, pt, f, n, Lx, Ly, tol):
-    p = ft.ewald_params(kind, len(ps), len(pt), (f ** 2).sum() + 1, Lx, Ly, tol=tol)
+    p = ft.ewald_params(kind, len(ps), len(pt), ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=tol)
     psw, ptw = oracle.wrap(ps, Lx, Ly), oracle.wrap(pt, Lx, Ly)
     code = CODE[kind]
     R = oracle.derived_real(code, psw, ptw, f, n, p["xi"], p["nside_x"], p["nside_y"], Lx, Ly)

@@ -241,7 +241,7 @@ def test_slp_driver_matches_oracle(gpu_ctx, oracle, N, Lx, Ly, tol):
                                                f[:, 1], Lx, Ly, tol=tol)
     wps, wpt = oracle.wrap(ps, Lx, Ly), oracle.wrap(pt, Lx, Ly)
     npts = len(ps) + len(pt)
-    prm = oracle.params("slp", npts, (f ** 2).sum() + 1, Lx, Ly, 24, oracle.default_Nb(npts), tol)
+    prm = oracle.params("slp", npts, oracle.density_q(f), Lx, Ly, 24, oracle.default_Nb(npts), tol)
     assert xi == prm["xi"]
     rr = oracle.slp_real(wps, wpt, f, prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly)
     rk = oracle.slp_kspace(wps, wpt, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], Lx, Ly,
@@ -258,7 +258,7 @@ def test_dlp_driver_circles_matches_oracle(gpu_ctx, oracle):
     u1, u2, ur, uk, xi = ft.StokesDLP_ewald_2p(ps[:, 0], ps[:, 1], pt[:, 0], pt[:, 1], n[:, 0],
                                                n[:, 1], f[:, 0], f[:, 1], Lx, Ly, tol=1e-10)
     npts = len(ps) + len(pt)
-    prm = oracle.params("dlp", npts, (f ** 2).sum() + 1, Lx, Ly, 24, oracle.default_Nb(npts),
+    prm = oracle.params("dlp", npts, oracle.density_q(f), Lx, Ly, 24, oracle.default_Nb(npts),
                         1e-10)
     rr = oracle.dlp_real(ps, pt, f, n, prm["xi"], prm["nside_x"], prm["nside_y"], Lx, Ly)
     rk = oracle.dlp_kspace(ps, pt, prm["xi"], prm["eta"], f, n, prm["Mx"], prm["My"], Lx, Ly,

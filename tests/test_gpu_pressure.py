@@ -35,7 +35,7 @@ def test_pressure_kspace_matches_oracle(gpu_ctx, oracle, Lx, Ly, tol):
     ps = uniform_points(N, Lx, Ly, seed=211)
     pt = uniform_points(3000, Lx, Ly, seed=212)
     f = normal_density(N, seed=213)
-    prm = ft.ewald_params("slp_pressure", N, 3000, (f ** 2).sum() + 1, Lx, Ly, tol=tol)
+    prm = ft.ewald_params("slp_pressure", N, 3000, ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=tol)
     got = ft.mex_stokes_slp_pressure_kspace(ps.T, pt.T, f.T, prm["xi"], prm["eta"], prm["Mx"],
                                             prm["My"], Lx, Ly, prm["w"], prm["P"], ctx=gpu_ctx)
     ref = oracle.slp_pressure_kspace(ps, pt, f, prm["xi"], prm["eta"], prm["Mx"], prm["My"], Lx,
@@ -53,7 +53,7 @@ def test_pressure_driver_matches_oracle(gpu_ctx, oracle):
                                                    ctx=gpu_ctx)
     np.testing.assert_array_equal(p, pr + pk)
     psw = oracle.wrap(ps, Lx, Ly)
-    prm = ft.ewald_params("slp_pressure", 5000, 3000, (f ** 2).sum() + 1, Lx, Ly, tol=1e-12)
+    prm = ft.ewald_params("slp_pressure", 5000, 3000, ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=1e-12)
     assert abs(prm["xi"] - xi) <= 1e-12 * xi  # Q summed in another order
     r_r = oracle.slp_pressure_real(psw, pt, f, xi, prm["nside_x"], prm["nside_y"], Lx, Ly)
     r_k = oracle.slp_pressure_kspace(psw, pt, f, xi, prm["eta"], prm["Mx"], prm["My"], Lx, Ly,
@@ -119,7 +119,7 @@ def test_dlp_pressure_kspace_matches_oracle(gpu_ctx, oracle, Lx, Ly, tol):
     pt = uniform_points(3000, Lx, Ly, seed=312)
     f = normal_density(N, seed=313)
     n = _unit_normals(N, seed=314)
-    prm = ft.ewald_params("dlp_pressure", N, 3000, (f ** 2).sum() + 1, Lx, Ly, tol=tol)
+    prm = ft.ewald_params("dlp_pressure", N, 3000, ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=tol)
     got = ft.mex_stokes_dlp_pressure_kspace(ps.T, pt.T, f.T, n.T, prm["xi"], prm["eta"],
                                             prm["Mx"], prm["My"], Lx, Ly, prm["w"], prm["P"],
                                             ctx=gpu_ctx)
@@ -139,7 +139,7 @@ def test_dlp_pressure_driver_matches_oracle(gpu_ctx, oracle):
     np.testing.assert_array_equal(p, pr + pk)
     psw = oracle.wrap(ps, Lx, Ly)
     Ns = len(ps)
-    prm = ft.ewald_params("dlp_pressure", Ns, 3000, (f ** 2).sum() + 1, Lx, Ly, tol=1e-12)
+    prm = ft.ewald_params("dlp_pressure", Ns, 3000, ft.density_q(f[:, 0], f[:, 1]), Lx, Ly, tol=1e-12)
     assert abs(prm["xi"] - xi) <= 1e-12 * xi
     r_r = oracle.dlp_pressure_real(psw, pt, f, n, xi, prm["nside_x"], prm["nside_y"], Lx, Ly)
     r_k = oracle.dlp_pressure_kspace(psw, pt, f, n, xi, prm["eta"], prm["Mx"], prm["My"], Lx, Ly,

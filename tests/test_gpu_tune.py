@@ -26,18 +26,51 @@ def test_autotune_meets_tolerance(gpu_ctx, same):
     p = uniform_points(N, 1.0, 1.0, seed=5)
     pt = p if same else p.copy()
     f = normal_density(N, seed=6)
-    prm, rep = autotune(gpu_ctx, "slp", p, pt, f, 1.0, 1.0, tol, reps=2)
+    prm, rep = autotune(gpu_ctx, "slp", p, pt, f, 1.0, 1.0, tol, reps=2, final_rounds=2)
     assert prm["symmetric"] == (1 if same else 0)
+    assert rep["symmetric_in_use"] == (1 if same else 0)
     assert rep["err"] <= 0.8 * tol
-    # an accurate solution from the reference recipe with other constants
+    assert rep["err_heldout"] is not None and rep["err_heldout"] <= tol
+    # an accurate solution from the reference recipe with other constants, for a
+    # density the tuner never saw
+    g = normal_density(N, seed=99)
     ref = ft.ewald_params("slp", N, N, 2 * N + 1.0, 1.0, 1.0, P=24, tol=1e-14)
-    u_ref = _solve(gpu_ctx, p, p.copy(), f, ref)
-    assert rel_err(_solve(gpu_ctx, p, pt, f, prm), u_ref) <= tol
+    u_ref = _solve(gpu_ctx, p, p.copy(), g, ref)
+    assert rel_err(_solve(gpu_ctx, p, pt, g, prm), u_ref) <= tol
+
+
+@pytest.mark.gpu
+def test_autotune_stresslet_circles(gpu_ctx):
+    """The tuner on a stresslet system (sources on circles, separate targets),
+    with the grid capped: held-out error within tolerance and checked again
+    against the CPU oracle on a subset."""
+    from oracle import oracle as O
+    from tests.inputs import circles, oracle_se, rel
+    Lx, Ly, tol = 2.0, 1.0, 1e-8
+    ps, n, f = circles(6, 500, Lx, Ly, seed=31)
+    pt = uniform_points(4000, Lx, Ly, seed=32)
+    prm, rep = autotune(gpu_ctx, "dlp", ps, pt, f, Lx, Ly, tol, nrm=n, reps=2, final_rounds=2,
+                        max_grid=512)
+    assert prm["Mx"] <= 512 and prm["My"] <= 512
+    assert rep["err"] <= 0.8 * tol and rep["err_heldout"] <= tol
+    g = normal_density(len(ps), seed=33) * f.std()
+    pl = Plan(gpu_ctx, "dlp", ps, pt, Lx, Ly, prm, nrm=n)
+    try:
+        u = pl.apply(g)
+    finally:
+        pl.close()
+    sub = np.arange(0, len(pt), 4)
+    assert rel(u[sub], oracle_se(O, "dlp", ps, pt[sub], g, n, Lx, Ly)) <= tol
 
 
 def test_fft_grid_alternatives():
-    # fewer stages only, each alternative shorter than the one before
     assert fft_stages(2000) == 5 and fft_stages(2048) == 4 and fft_stages(2160) == 3
-    assert fft_alternatives(2000) == [2048, 2160]
-    assert fft_alternatives(2160) == []
     assert fft_stages(2002) is None  # 7 and 11 are no radices
+    # per line class (<= 2048, 2049-2560, longer) the sizes near m with that
+    # class's fewest stages, nearest first, both above and below m
+    assert fft_alternatives(2000) == [2048, 2160]
+    assert fft_alternatives(2250) == [2048, 2304, 2160]
+    assert fft_alternatives(902) == [960, 1000]  # 902 = 2 11 41 has no radix plan
+    for m in (1200, 2250, 3000):
+        for n in fft_alternatives(m):
+            assert n % 2 == 0 and 0.9 * m <= n <= 1.12 * m and fft_stages(n) is not None

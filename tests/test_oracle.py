@@ -336,3 +336,27 @@ def test_params_reference_recipe_values(oracle):
     p = oracle.params("slp", 8_000_000, 8e6 + 1, 1.0, 1.0, 24, oracle.default_Nb(8e6), 1e-10)
     assert (p["nside_x"], p["Mx"]) == (121, 2112)
     assert abs(p["xi"] - 632) < 1.0
+
+
+def test_density_q_sums_columns_in_order(oracle):
+    """Q = sum(sum(f.^2)) + 1 for f = [f1'; f2']: MATLAB's sum of a 2 x N matrix
+    adds down each column, then the outer sum adds the N column sums in order.
+    A hand-written sequential loop pins it; for this density the row-first
+    order differs in the last bits and moves the recipe's xi."""
+    rng = np.random.default_rng(16)
+    f = rng.standard_normal((1000, 2)) * np.exp(rng.uniform(-20, 20, (1000, 2)))
+    q = 0.0
+    for i in range(len(f)):
+        col = f[i, 0] * f[i, 0] + f[i, 1] * f[i, 1]
+        q = q + col
+    assert oracle.density_q(f) == q + 1.0
+    import fasttools2d_amd.periodic_ewald as pe
+    assert pe.density_q(f[:, 0], f[:, 1]) == q + 1.0
+    q1 = q2 = 0.0
+    for i in range(len(f)):
+        q1 += f[i, 0] * f[i, 0]
+        q2 += f[i, 1] * f[i, 1]
+    assert (q1 + q2) + 1 != q + 1.0
+    a = oracle.params("slp", 2000, q + 1.0, 1.0, 1.0, 24, oracle.default_Nb(2000), 1e-10)
+    b = oracle.params("slp", 2000, (q1 + q2) + 1, 1.0, 1.0, 24, oracle.default_Nb(2000), 1e-10)
+    assert a["xi"] != b["xi"]
