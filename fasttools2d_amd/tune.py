@@ -223,17 +223,32 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                 M0 = M1
             return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym)
 
-        def fits(prm):
-            return prm["Mx"] <= max_grid and prm["My"] <= max_grid
+        def fits(prm, cap=None):
+            cap = max_grid if cap is None else cap
+            return prm["Mx"] <= cap and prm["My"] <= cap
 
-        # the calibration runs keep the grid within max_grid by lowering the
-        # base split where the exact k space would not fit
-        while a0 > 1 and not fits(prm_for(a0, c_exact_r, c_exact_k, P_exact)):
-            a0 -= 1
+        def fits_try(a, c_r, c_k, P, cap=None):
+            try:
+                return fits(prm_for(a, c_r, c_k, P), cap)
+            except ValueError:  # beyond the FFT's longest line
+                return False
+
+        # Calibration split: the constants are dimensionless, so they are
+        # measured where the real space is cheaper, at the finest split whose
+        # exact k space fits the FFT's 4096-point lines (max_grid binds only the
+        # configurations that are timed), with at least 64 points (sources and
+        # targets) per cell on average: the real-space truncation error grows
+        # with the occupancy, so calibrating on sparser cells than the sweep
+        # visits would under-estimate it.  Clustered inputs gain most: their
+        # dense cells cost the square of their occupancy.
+        a_cal = max(a0, int(math.sqrt(npts / (64.0 * A * B))))
+        while a_cal > 1 and not fits_try(a_cal, c_exact_r, c_exact_k, P_exact, 4096):
+            a_cal -= 1
+        report["calibration_nside"] = a_cal * A
         # 2. real-space constant (and the errors of the next two, for step 8)
         c_r, rerr = None, {}
         for c in (3.0, 3.3, 3.5, 3.6, 3.7, 3.8, 3.9, 4.0, 4.2, 4.6, 5.0, 5.5):
-            u, _ = T.run(prm_for(a0, c, c_exact_k, P_exact, 1 if tol >= 1e-10 else 0))
+            u, _ = T.run(prm_for(a_cal, c, c_exact_k, P_exact, 1 if tol >= 1e-10 else 0))
             e = rel_err(u, u_ref)
             rerr[c] = e
             log(f"tune: c_r {c:.2f} -> err {e:.2e}")
@@ -244,24 +259,43 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         if c_r is None:
             c_r = c_exact_r
         report["c_r"] = c_r
-        # 3. k-space constants: cheapest (P, c_k) meeting tol/2 by a cost model
+        # 3. k-space constants: cheapest (P, c_k) meeting tol/2 by a cost model.
+        # The error falls as c_k grows, so each support's smallest passing c_k is
+        # found by bisection; a support that passes at the smallest c_k ends the
+        # search (larger supports cost more at the same grid).
         best, kcands, kerr = None, [], {}
+        cks = (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0)
+
+        def k_error(P, ck):
+            if (P, ck) not in kerr:
+                if not fits_try(a_cal, c_exact_r, ck, P, 4096):
+                    kerr[(P, ck)] = math.inf
+                else:
+                    u, _ = T.run(prm_for(a_cal, c_exact_r, ck, P))
+                    kerr[(P, ck)] = rel_err(u, u_ref)
+                log(f"tune: P {P} c_k {ck:.1f} -> err {kerr[(P, ck)]:.2e}")
+            return kerr[(P, ck)]
+
         for P in (8, 10, 12, 14, 16, 18, 20, 24):
-            for ck in (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0):
-                prm = prm_for(a0, c_exact_r, ck, P)
-                if not fits(prm):
-                    continue
-                u, _ = T.run(prm)
-                e = rel_err(u, u_ref)
-                kerr[(P, ck)] = e
-                if e < tol / 2:
-                    G = prm["Mx"] * prm["My"]
-                    cost = (Ns + Nt) * (P + 1) ** 2 * 2.0 + 25.0 * G * math.log2(G)
-                    log(f"tune: P {P} c_k {ck:.1f} M {prm['Mx']} -> err {e:.2e} cost {cost:.3e}")
-                    kcands.append((cost, P, ck))
-                    if best is None or cost < best[0]:
-                        best = (cost, P, ck)
-                    break
+            if k_error(P, cks[-1]) >= tol / 2:
+                continue
+            lo, hi = 0, len(cks) - 1
+            while lo < hi:
+                mid = (lo + hi) // 2
+                if k_error(P, cks[mid]) < tol / 2:
+                    hi = mid
+                else:
+                    lo = mid + 1
+            ck = cks[hi]
+            prm = prm_for(a_cal, c_exact_r, ck, P)
+            G = prm["Mx"] * prm["My"]
+            cost = (Ns + Nt) * (P + 1) ** 2 * 2.0 + 25.0 * G * math.log2(G)
+            log(f"tune: P {P} passes from c_k {ck:.1f} (cost {cost:.3e})")
+            kcands.append((cost, P, ck))
+            if best is None or cost < best[0]:
+                best = (cost, P, ck)
+            if hi == 0:
+                break
         if best is None:
             best = (0, 24, c_exact_k)
         _, P, c_k = best
@@ -284,34 +318,53 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             e = rel_err(u, u_ref)
             log(f"tune: nside {prm['nside_x']} xi {prm['xi']:.1f} M {prm['Mx']} P {P} -> "
                 f"{t * 1e3:.3f} ms err {e:.2e} symmetric {T.used_symmetric}")
-            results.append((t, e, prm, a, T.used_symmetric))
+            results.append((t, e, prm, a, T.used_symmetric, cr))
 
         def accepted():
             return [r for r in results if r[1] <= 0.8 * tol]
 
+        a_fit = a0
+        while a_fit < 8 * a0 and fits_try(a_fit + 1, c_r, c_k, P):
+            a_fit += 1
+        while a_fit > 1 and not fits_try(a_fit, c_r, c_k, P):
+            a_fit -= 1
         for s in nside_scales:
-            timed(max(1, int(round(a0 * s))))
+            a = max(1, int(round(a0 * s)))
+            if a <= a_fit:
+                timed(a)
+        timed(a_fit)
         # 5. refine around the fastest split that met the tolerance
         if accepted():
             a_best = min(accepted(), key=lambda r: r[0])[3]
             for s in (0.88, 0.94, 1.06, 1.12):
                 timed(max(1, int(round(a_best * s))))
+        # 5b. nothing met the tolerance (the calibration's occupancy differs
+        # from the sweep's): the next calibrated real-space constants at the
+        # fastest splits
+        if not accepted() and results:
+            fastest = sorted({r[3] for r in sorted(results, key=lambda r: r[0])[:2]})
+            for c2 in sorted(c for c in rerr if c > c_r)[:3]:
+                for a in fastest:
+                    timed(a, cr=c2)
+                if accepted():
+                    c_r = c2
+                    break
         # 6. the next cheapest supports by the cost model, timed at the best split
         if accepted():
             a_best = min(accepted(), key=lambda r: r[0])[3]
             for _, P2, ck2 in sorted(kcands)[1:3]:
-                timed(a_best, P2, ck2)
+                timed(a_best, P2, ck2, cr=c_r)
         # 7. grids near the best one whose line FFTs take fewer stages, at the
         # best point and at the runner-up split
         ok = sorted(accepted(), key=lambda r: r[0])
         seen_a = []
-        for _, _, pb, a_b, _ in ok:
+        for _, _, pb, a_b, _, cr_b in ok:
             if a_b in seen_a:
                 continue
             seen_a.append(a_b)
             ck_b = next((ck for _, P2, ck in kcands if P2 == pb["P"]), c_k)
             for m in fft_alternatives(pb["Mx"] // A, max_grid // max(A, B)):
-                timed(a_b, pb["P"], ck_b, m)
+                timed(a_b, pb["P"], ck_b, m, cr_b)
             if len(seen_a) == 2:
                 break
         # 8. the error budget split the other way: a smaller support whose k-space
@@ -319,7 +372,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         # error, added in quadrature, keeps the estimate below 0.7 tol (the
         # measured total still decides)
         if accepted():
-            _, _, pb, a_best, _ = min(accepted(), key=lambda r: r[0])
+            _, _, pb, a_best, _, _ = min(accepted(), key=lambda r: r[0])
             for P2 in sorted({q for q, _ in kerr if q < pb["P"]}, reverse=True)[:2]:
                 cks = sorted(ck for q, ck in kerr if q == P2 and kerr[(q, ck)] < 0.7 * tol)
                 if not cks:
@@ -354,7 +407,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         if chosen is None:  # nothing generalises: the most accurate measured
             r = min(ok, key=lambda q: q[1])
             chosen, e_held = (r, r[0]), None
-        (t_sweep, e, prm, _, used_sym), t_med = chosen
+        (t_sweep, e, prm, _, used_sym, _), t_med = chosen
         report.update(ms=t_med * 1e3, ms_sweep=t_sweep * 1e3, err=e, err_heldout=e_held,
                       symmetric_in_use=int(used_sym), P_timed=prm["P"],
                       c_r_timed=prm["xi"] * Lx / prm["nside_x"],

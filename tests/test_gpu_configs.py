@@ -99,8 +99,8 @@ def test_c3_clustered_fp64_autotuned(gpu_ctx, oracle, c3_points, kind):
     tol = 1e-10
     dens, nrm = (f, None) if kind == "slp" else (g, n)
     prm, rep = autotune(gpu_ctx, kind, p, p, dens, 1.0, 1.0, tol, nrm=nrm, reps=2,
-                        final_rounds=2, max_grid=2048,
-                        nside_scales=(1.0, 1.4, 1.9, 2.6, 3.4))
+                        final_rounds=2, max_grid=2048, nside_scales=(1.0, 1.4),
+                        log=lambda m: print(m, flush=True))
     print(f"C3 {kind}: tuned {prm} ms {rep['ms']:.2f} err {rep['err']:.2e} "
           f"held-out {rep['err_heldout']}")
     assert prm["Mx"] <= 2048 and rep["err"] <= 0.8 * tol
