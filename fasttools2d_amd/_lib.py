@@ -68,7 +68,8 @@ EXPORTS = (
     "se2p_device_count", "se2p_slp_real", "se2p_slp_kspace", "se2p_dlp_real",
     "se2p_dlp_kspace", "se2p_options_default", "se2p_params_ref", "se2p_slp_ewald",
     "se2p_dlp_ewald", "se2p_dev_alloc", "se2p_dev_free", "se2p_memcpy_h2d",
-    "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device", "se2p_prof_enable",
+    "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device",
+    "se2p_eval_device_noalloc", "se2p_prof_enable",
     "se2p_prof_reset", "se2p_prof_get", "se2p_assign", "se2p_find_closest_node",
     "se2p_spread", "se2p_gather", "se2p_fft2", "se2p_e1", "se2p_plan_create",
     "se2p_plan_apply", "se2p_plan_apply_graph", "se2p_plan_destroy", "se2p_plan_params",
@@ -148,6 +149,7 @@ def lib():
             "se2p_memcpy_d2d": [P, P, P, Z],
             "se2p_ctx_sync": [P],
             "se2p_eval_device": [P, I, P, P, P, P, I, I, D, D, ctypes.POINTER(Params), P],
+            "se2p_eval_device_noalloc": [P, I, P, P, P, P, I, I, D, D, ctypes.POINTER(Params), P],
             "se2p_prof_enable": [P, I],
             "se2p_prof_reset": [P],
             "se2p_prof_get": [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(I)],

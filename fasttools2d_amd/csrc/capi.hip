@@ -8,7 +8,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -40,6 +42,14 @@ struct se2p_batch {
     GraphCache gc;
 };
 
+// Scratch memory of the allocation-free evaluation (se2p_eval_device_noalloc):
+// one array per GPU, held until the process ends.  Host code never passes this
+// name to a kernel or a copy: on the host it is only a stand-in for the symbol.
+// The device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device at context creation with that device current.
+static constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace se2p {
 
 namespace {
@@ -58,6 +68,20 @@ int set_error(int code, const char* fmt, ...)
 int ws_get(se2p_ctx* ctx, Workspace* ws, Slot slot, size_t bytes, void** out)
 {
     if (bytes < 256) bytes = 256;
+    if (ws->arena) {
+        if (ws->size[slot] < bytes) {
+            const size_t off = (ws->arena_used + 255) & ~(size_t)255;
+            if (off > ws->arena_cap || bytes > ws->arena_cap - off)
+                return set_error(SE2P_ERR_UNSUPPORTED,
+                                 "scratch of %zu bytes exhausted (%zu more needed)",
+                                 ws->arena_cap, off + bytes - ws->arena_cap);
+            ws->ptr[slot] = ws->arena + off;
+            ws->size[slot] = bytes;
+            ws->arena_used = off + bytes;
+        }
+        *out = ws->ptr[slot];
+        return SE2P_OK;
+    }
     if (ws->size[slot] < bytes) {
         if (ctx->capturing)
             return set_error(SE2P_ERR_ARG, "allocation requested while a graph is captured");
@@ -81,7 +105,7 @@ int ws_get(se2p_ctx* ctx, Workspace* ws, Slot slot, size_t bytes, void** out)
 void ws_release(Workspace* ws)
 {
     for (int i = 0; i < WS_NUM_SLOTS; ++i) {
-        if (ws->ptr[i]) (void)hipFree(ws->ptr[i]);
+        if (ws->ptr[i] && !ws->arena) (void)hipFree(ws->ptr[i]);
         ws->ptr[i] = nullptr;
         ws->size[i] = 0;
     }
@@ -712,6 +736,34 @@ void destroy_streams(se2p_ctx* c)
 
 using namespace se2p;
 
+namespace se2p {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// This GPU's scratch address, resolved on the first call for the device (the
+// caller has made the device current).
+int scratch_address(int device, char** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(SE2P_ERR_UNSUPPORTED, "device %d: scratch table holds %d devices", device,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(SE2P_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+                             device, hipGetErrorString(e));
+        g_scratch[device] = static_cast<char*>(p);
+    }
+    *out = g_scratch[device];
+    return SE2P_OK;
+}
+}  // namespace
+}  // namespace se2p
+
 extern "C" {
 
 int se2p_version(void) { return 2; }
@@ -726,6 +778,7 @@ int se2p_device_count(int* count)
     *count = (e == hipSuccess) ? n : 0;
     return SE2P_OK;
 }
+
 
 int se2p_ctx_create(int device, se2p_ctx** out)
 {
@@ -749,6 +802,12 @@ int se2p_ctx_create(int device, se2p_ctx** out)
         destroy_streams(c);
         delete c;
         return set_error(SE2P_ERR_HIP, "stream set-up: %s", hipGetErrorString(e));
+    }
+    int rc = scratch_address(device, &c->scratch);
+    if (rc != SE2P_OK) {
+        destroy_streams(c);
+        delete c;
+        return rc;
     }
     *out = c;
     return SE2P_OK;
@@ -1160,6 +1219,76 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    return batch_apply(ctx, B, d_f, d_u);
+}
+
+// An upper bound of the scratch bytes one evaluation takes: every workspace
+// request of geom_build, sort_by_key, fft_twiddles and batch_build, each counted
+// as a region of its own rounded up to 256 bytes.
+static size_t noalloc_bytes(int kind, int Ns, int Nt, const se2p_params& prm)
+{
+    auto r = [](double b) { return ((size_t)std::max(256.0, std::ceil(b)) + 255) & ~(size_t)255; };
+    const double ns = Ns, nt = Nt;
+    const double nbox = (double)prm.nside_x * prm.nside_y;
+    const int P = prm.P;
+    const double ntile = std::ceil(prm.Mx / 16.0) * std::ceil(prm.My / 16.0);
+    const int gT = P <= 24 ? 32 : 16;
+    const double gtile = std::ceil((double)prm.Mx / gT) * std::ceil((double)prm.My / gT);
+    const double G = (double)prm.Mx * prm.My;
+    const bool dlp = dlp_family(kind), two = dlp || out_grids(kind) == 2;
+    size_t b = 0;
+    if (dlp) b += r(16 * ns);  // normals
+    // real space: keys, counts, permutations, starts, y keys, accumulator,
+    // positions, sorted records and targets, flag, zero-mode partials
+    b += r(4 * ns) + r(4 * nbox) + r(4 * nt) + r(4 * nbox) + r(4 * ns) + r(4 * (nbox + 1)) +
+         r(4 * nt) + r(4 * (nbox + 1)) + r(4 * ns) + r(16 * ns) + r(4 * ns) +
+         r((dlp ? 64 : 32) * ns) + r(16 * nt) + r(4) + r(8 * 512) + r(16);
+    // k space: table, keys, counts, permutations, starts, nodes, offsets,
+    // records, densities, grids, twiddles
+    b += r(512) + r(4 * ns) + r(4 * ntile) + r(4 * ns) + r(4 * (ntile + 1)) + r(8 * ns) +
+         r(16 * ns) + r(8 * ns) + r(32 * ns) + r((dlp ? 32 : 16) * ns) + r(4 * nt) +
+         r(4 * gtile) + r(4 * nt) + r(4 * (gtile + 1)) + r(8 * nt) + r(16 * nt) +
+         r(16 * G) * (two ? 4 : 2) + r(16.0 * prm.Mx) + r(16.0 * prm.My);
+    // four stable sorts (source cells, target cells, spreading and gathering
+    // tiles): scratch of n and a list of nkeys + 1 each
+    for (double n : {ns, nt, ns, nt}) b += r(4 * n);
+    for (double k : {nbox, nbox, ntile, gtile}) b += r(4 * (k + 1));
+    // descriptor and job list (every job range of one system, generously)
+    b += r(sizeof(SysDesc)) +
+         r(sizeof(Job) * (8 * (ns + nt) / 256 + 8 * nbox + 8 * ntile + 4 * gtile + 4 * G / 256 +
+                          4.0 * (prm.Mx + prm.My) + 64));
+    return b;
+}
+
+int se2p_eval_device_noalloc(se2p_ctx* ctx, int kind, const double* d_psrc, const double* d_ptar,
+                             const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
+                             double Ly, const se2p_params* prm, double* d_u)
+{
+    SE2P_TRY(check_common(ctx, Nsrc, Ntar, Lx, Ly));
+    if (!valid_kind(kind)) return set_error(SE2P_ERR_ARG, "bad kind");
+    SE2P_TRY(check_params(prm, Lx, Ly));
+    if (dlp_family(kind)) SE2P_TRY(need_ptr(d_n, Nsrc, "d_n"));
+    const size_t need = noalloc_bytes(kind, Nsrc, Ntar, *prm);
+    if (need > kScratchBytes)
+        return set_error(SE2P_ERR_UNSUPPORTED,
+                         "se2p_eval_device_noalloc: Nsrc %d, Ntar %d, grid %d x %d need up to "
+                         "%zu bytes of scratch, more than its fixed %zu; use se2p_eval_device",
+                         Nsrc, Ntar, prm->Mx, prm->My, need, kScratchBytes);
+    if (!ctx->scratch) return set_error(SE2P_ERR_HIP, "scratch address not resolved");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    if (Ntar == 0) return SE2P_OK;
+    // Every workspace request is carved from this GPU's scratch array: no
+    // allocation.  The kernels clear what they need cleared (counts, flags,
+    // accumulators) themselves, as with fresh allocations.
+    Workspace ws;
+    ws.arena = ctx->scratch;
+    ws.arena_cap = kScratchBytes;
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ws, &G, kind, d_psrc, d_ptar, d_n, Nsrc, Ntar, Lx, Ly, *prm, true,
+                        true));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ws, &B, &gp, 1));
     return batch_apply(ctx, B, d_f, d_u);
 }
 

@@ -100,9 +100,15 @@ enum Slot {
 };
 
 // Grow-only device buffers addressed by slot (one set per context, one per plan).
+// With `arena` set, slots are carved from [arena, arena + arena_cap) instead of
+// allocated: a request larger than its slot's region takes a new region past
+// the last one (the old one is abandoned), and a request past arena_cap fails
+// with SE2P_ERR_UNSUPPORTED.  Nothing is ever freed.
 struct Workspace {
     void* ptr[WS_NUM_SLOTS] = {};
     size_t size[WS_NUM_SLOTS] = {};
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
 
 }  // namespace se2p
@@ -122,6 +128,8 @@ struct se2p_ctx {
     std::vector<se2p::ProfAcc> prof_acc;
     std::vector<hipEvent_t> event_pool;
     std::vector<void*> user_allocs;
+    // device address of this GPU's scratch array (allocation-free evaluation)
+    char* scratch = nullptr;
 };
 
 namespace se2p {

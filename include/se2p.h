@@ -334,6 +334,23 @@ int se2p_eval_device(se2p_ctx* ctx, int kind, const double* d_psrc, const double
                      const double* d_f, const double* d_n, int Nsrc, int Ntar, double Lx,
                      double Ly, const se2p_params* prm, double* d_u);
 
+/* As se2p_eval_device, allocating nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * workspace buffer of the evaluation is carved from a fixed scratch array of
+ * 256 MiB per GPU (a static device array the library holds until the process
+ * ends); a problem that needs more (about 200 bytes per source or target plus
+ * 32 bytes per grid node for a Stokeslet, so roughly a million points in all)
+ * is rejected with SE2P_ERR_UNSUPPORTED before any work is queued.
+ * All calls on one GPU share its array and it keeps the last call's contents
+ * (the kernels clear what must start at zero): calls that overlap on one GPU,
+ * from two contexts or threads, race.  Results are those of se2p_eval_device,
+ * bit for bit. */
+int se2p_eval_device_noalloc(se2p_ctx* ctx, int kind, const double* d_psrc,
+                             const double* d_ptar, const double* d_f, const double* d_n,
+                             int Nsrc, int Ntar, double Lx, double Ly, const se2p_params* prm,
+                             double* d_u);
+
 /* ------------------------------------------------------------------------ */
 /* Plans: fixed geometry, repeated matvecs.  No counterpart in the reference, */
 /* which recomputes binning and Gaussian factors on every mex call           */
