@@ -30,6 +30,8 @@ SE2P_DLP_GRADIENT = 8
 SE2P_DLP_PRESSURE_GRAD = 9
 SE2P_DLP_VORTICITY = 10
 SE2P_DLP_STRESS = 11
+SE2P_FP64 = 0
+SE2P_FP32C = 1
 
 _KINDS = {"slp": SE2P_SLP, "dlp": SE2P_DLP, "slp_pressure": SE2P_SLP_PRESSURE,
           "dlp_pressure": SE2P_DLP_PRESSURE, "slp_gradient": SE2P_SLP_GRADIENT,
@@ -65,7 +67,7 @@ def has_normals(kind):
 
 EXPORTS = (
     "se2p_ctx_create", "se2p_ctx_destroy", "se2p_last_error", "se2p_version",
-    "se2p_device_count", "se2p_slp_real", "se2p_slp_kspace", "se2p_dlp_real",
+    "se2p_device_count", "se2p_ctx_set_precision", "se2p_slp_real", "se2p_slp_kspace", "se2p_dlp_real",
     "se2p_dlp_kspace", "se2p_options_default", "se2p_params_ref", "se2p_slp_ewald",
     "se2p_dlp_ewald", "se2p_dev_alloc", "se2p_dev_free", "se2p_memcpy_h2d",
     "se2p_memcpy_d2h", "se2p_memcpy_d2d", "se2p_ctx_sync", "se2p_eval_device",
@@ -98,7 +100,7 @@ class Params(ctypes.Structure):
                 ("eta", ctypes.c_double), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
                 ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int),
-                ("symmetric", ctypes.c_int)]
+                ("symmetric", ctypes.c_int), ("precision", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -133,6 +135,7 @@ def lib():
             "se2p_ctx_create": [I, ctypes.POINTER(P)],
             "se2p_ctx_destroy": [P],
             "se2p_device_count": [ctypes.POINTER(I)],
+            "se2p_ctx_set_precision": [P, I],
             "se2p_slp_real": [P, P, P, P, I, I, D, I, I, D, D, P],
             "se2p_slp_kspace": [P, P, P, D, D, P, I, I, I, I, D, D, D, I, P],
             "se2p_dlp_real": [P, P, P, P, P, I, I, D, I, I, D, D, P],
@@ -249,6 +252,11 @@ class Context:
 
     def sync(self):
         check(lib().se2p_ctx_sync(self.handle))
+
+    def set_precision(self, precision):
+        """SE2P_FP64 (0) or SE2P_FP32C (1) for this context's one-shot calls (the
+        mex equivalents and drivers)."""
+        check(lib().se2p_ctx_set_precision(self.handle, int(precision)))
 
     # device memory owned by the caller
     def alloc(self, nbytes):

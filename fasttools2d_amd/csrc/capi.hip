@@ -216,6 +216,8 @@ int check_params(const se2p_params* prm, double Lx, double Ly)
         return set_error(SE2P_ERR_ARG, "e1_tier must be 0 or 1");
     if (prm->symmetric != 0 && prm->symmetric != 1)
         return set_error(SE2P_ERR_ARG, "symmetric must be 0 or 1");
+    if (prm->precision != SE2P_FP64 && prm->precision != SE2P_FP32C)
+        return set_error(SE2P_ERR_ARG, "precision must be SE2P_FP64 or SE2P_FP32C");
     return SE2P_OK;
 }
 
@@ -386,6 +388,7 @@ int params_ref(int kind, int Ns, int Nt, double Q, double Lx, double Ly, const s
     out->converged = ok1 && ok2;
     out->e1_tier = 0;  // the reference evaluates E1 to full FP64 accuracy
     out->symmetric = 0;
+    out->precision = SE2P_FP64;
     if (o->verbose)
         fprintf(stderr,
                 "se2p %s: xi %.6g rc %.6g kinf %d Mx %d My %d nside %d x %d w %.6g eta %.6g\n",
@@ -454,6 +457,7 @@ int ewald_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* ysrc
     }
     se2p_params prm;
     SE2P_TRY(params_ref(kind, Ns, Nt, Q, Lx, Ly, opt, &prm));
+    prm.precision = ctx->precision;
     SE2P_TRY(check_params(&prm, Lx, Ly));
     if (xi_out) *xi_out = prm.xi;
     if (Nt == 0) return SE2P_OK;
@@ -587,8 +591,10 @@ int pressure_driver(se2p_ctx* ctx, int kind, const double* xsrc, const double* y
 // One mex call: upload, build the half that is needed, apply, download.
 int mex_call(se2p_ctx* ctx, int kind, bool real, const double* psrc, const double* ptar,
              const double* f, const double* n, int Ns, int Nt, double Lx, double Ly,
-             const se2p_params& prm, double* out)
+             const se2p_params& prm_in, double* out)
 {
+    se2p_params prm = prm_in;
+    prm.precision = ctx->precision;
     SE2P_HIP_CHECK(hipSetDevice(ctx->device));
     if (Nt == 0) return SE2P_OK;
     double *dps, *dpt, *df, *dn = nullptr;
@@ -1361,6 +1367,7 @@ static void params_in_use(const Geom& G, se2p_params* used)
 {
     *used = G.prm;
     used->symmetric = G.sym ? 1 : 0;
+    used->precision = G.prec ? SE2P_FP32C : SE2P_FP64;
 }
 
 int se2p_plan_params(const se2p_plan* plan, se2p_params* used)
@@ -1456,6 +1463,15 @@ int se2p_batch_destroy(se2p_batch* batch)
     for (auto& w : batch->ws) ws_release(&w);
     ws_release(&batch->bws);
     delete batch;
+    return SE2P_OK;
+}
+
+int se2p_ctx_set_precision(se2p_ctx* ctx, int precision)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (precision != SE2P_FP64 && precision != SE2P_FP32C)
+        return set_error(SE2P_ERR_ARG, "precision must be SE2P_FP64 or SE2P_FP32C");
+    ctx->precision = precision;
     return SE2P_OK;
 }
 
@@ -1563,13 +1579,26 @@ int se2p_spread(se2p_ctx* ctx, const double* psrc, const double* f, int Nsrc, do
     SE2P_TRY(upload(ctx, WS_DENS_A, f, 2 * (size_t)Nsrc, &df));
     // k-space geometry of the sources only (no targets); spreading of (f1, f2)
     Geom G;
-    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dps, dps, nullptr, Nsrc, 0, Lx, Ly,
-                        k_only(xi, eta, Mx, My, w, P), false, true));
+    se2p_params kp = k_only(xi, eta, Mx, My, w, P);
+    kp.precision = ctx->precision;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dps, dps, nullptr, Nsrc, 0, Lx, Ly, kp, false,
+                        true));
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
     SE2P_TRY(batch_spread_grid(ctx, ctx->stream, B, df, 0));
     const size_t Gn = (size_t)Mx * My;
+    if (G.prec) {  // FP32 tier: a float2 grid
+        std::vector<float2> h(Gn);
+        SE2P_HIP_CHECK(hipMemcpyAsync(h.data(), G.Z0, sizeof(float2) * Gn, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        SE2P_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < Gn; ++i) {
+            H1[i] = h[i].x;
+            H2[i] = h[i].y;
+        }
+        return SE2P_OK;
+    }
     std::vector<double> h(2 * Gn);
     SE2P_TRY(download(ctx, h.data(), (double*)G.Z0, 2 * Gn));
     for (size_t i = 0; i < Gn; ++i) {
@@ -1593,13 +1622,23 @@ int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, do
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
     // k-space geometry of the targets only (no sources); the grid goes into W
     Geom G;
-    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dpt, dpt, nullptr, 0, Ntar, Lx, Ly,
-                        k_only(xi, eta, Mx, My, w, P), false, true));
+    se2p_params kp = k_only(xi, eta, Mx, My, w, P);
+    kp.precision = ctx->precision;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dpt, dpt, nullptr, 0, Ntar, Lx, Ly, kp, false,
+                        true));
     const size_t Gn = (size_t)Mx * My;
     std::vector<double> wz(2 * Gn, 0.0);
-    for (size_t i = 0; i < Gn; ++i) wz[2 * i] = H[i];
-    SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wz.data(), sizeof(double) * 2 * Gn, hipMemcpyHostToDevice,
-                                  ctx->stream));
+    std::vector<float2> wf;
+    if (G.prec) {  // FP32 tier: the grid travels as float2
+        wf.assign(Gn, make_float2(0.f, 0.f));
+        for (size_t i = 0; i < Gn; ++i) wf[i].x = (float)H[i];
+        SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wf.data(), sizeof(float2) * Gn, hipMemcpyHostToDevice,
+                                      ctx->stream));
+    } else {
+        for (size_t i = 0; i < Gn; ++i) wz[2 * i] = H[i];
+        SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wz.data(), sizeof(double) * 2 * Gn,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    }
     BatchExec B;
     const Geom* gp = &G;
     SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));  // synchronises: wz may die after
@@ -1631,6 +1670,7 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     d.twx = twx;
     d.twy = twy;
     const size_t Gn = (size_t)Mx * My;
+    const int prec = ctx->precision == SE2P_FP32C ? 1 : 0;
     std::vector<double> z(2 * Gn);
     for (size_t i = 0; i < Gn; ++i) {
         z[2 * i] = re[i];
@@ -1638,7 +1678,17 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     }
     double* dz;
     void* tmp;
-    SE2P_TRY(upload(ctx, WS_GRID_A, z.data(), 2 * Gn, &dz));
+    std::vector<float2> zf;
+    if (prec) {  // FP32 tier: float2 lines
+        zf.resize(Gn);
+        for (size_t i = 0; i < Gn; ++i) zf[i] = make_float2((float)re[i], (float)im[i]);
+        void* p = nullptr;
+        SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_A, sizeof(double2) * Gn, &p));
+        dz = (double*)p;
+        SE2P_HIP_CHECK(hipMemcpyAsync(dz, zf.data(), sizeof(float2) * Gn, hipMemcpyHostToDevice, s));
+    } else {
+        SE2P_TRY(upload(ctx, WS_GRID_A, z.data(), 2 * Gn, &dz));
+    }
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_C, sizeof(double2) * Gn, &tmp));
     d.grid[G_Z0] = (double2*)dz;
     d.grid[G_W] = (double2*)tmp;
@@ -1659,12 +1709,22 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     const SysDesc* ds = (const SysDesc*)psys;
     const Job* dj = (const Job*)pjobs;
     const int inv = inverse ? 1 : 0;
-    SE2P_TRY(launch_fft_lines(s, 0, G_Z0, inv, fft_line_class(My), fft_lds_bytes(My), ds, dj, ny));
-    SE2P_TRY(launch_transpose(s, G_Z0, G_W, 0, ds, dj + ny, t0));
-    SE2P_TRY(launch_fft_lines(s, 1, G_W, inv, fft_line_class(Mx), fft_lds_bytes(Mx), ds,
-                              dj + ny + t0, nx));
-    SE2P_TRY(launch_transpose(s, G_W, G_Z0, 1, ds, dj + ny + t0 + nx, t1));
-    SE2P_TRY(download(ctx, z.data(), dz, 2 * Gn));
+    SE2P_TRY(launch_fft_lines(s, prec, 0, G_Z0, inv, fft_line_class(My), fft_lds_bytes(My, prec),
+                              ds, dj, ny));
+    SE2P_TRY(launch_transpose(s, prec, G_Z0, G_W, 0, ds, dj + ny, t0));
+    SE2P_TRY(launch_fft_lines(s, prec, 1, G_W, inv, fft_line_class(Mx), fft_lds_bytes(Mx, prec),
+                              ds, dj + ny + t0, nx));
+    SE2P_TRY(launch_transpose(s, prec, G_W, G_Z0, 1, ds, dj + ny + t0 + nx, t1));
+    if (prec) {
+        SE2P_HIP_CHECK(hipMemcpyAsync(zf.data(), dz, sizeof(float2) * Gn, hipMemcpyDeviceToHost, s));
+        SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        for (size_t i = 0; i < Gn; ++i) {
+            z[2 * i] = zf[i].x;
+            z[2 * i + 1] = zf[i].y;
+        }
+    } else {
+        SE2P_TRY(download(ctx, z.data(), dz, 2 * Gn));
+    }
     const double sc = inverse ? 1.0 / ((double)Mx * (double)My) : 1.0;
     for (size_t i = 0; i < Gn; ++i) {
         re[i] = z[2 * i] * sc;

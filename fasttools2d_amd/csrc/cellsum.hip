@@ -127,6 +127,44 @@ __device__ __forceinline__ void slp_pair2(double dx, double dy, double r2, doubl
     b2 = hE * g2 - c * dx * cg;
 }
 
+// The FP32 tier's far pairs (x = xi^2 r^2 >= 1): e^{-x} from the hardware
+// exp2, E1 from the 'f32' fit, one hardware reciprocal for both E1 and 1/r^2;
+// the term (in the same cancellation-free form as slp_pair) is added to FP32
+// partial sums that the caller folds into its FP64 sums after every chunk.
+__device__ __forceinline__ void slp_radial_f32(float r2, float xi2, float& E, float& c)
+{
+    const float x = xi2 * r2;
+    const float e = __expf(-x);
+    const float pr = horner_rev_f(E1F32::farp, x), qr = horner_rev_f(E1F32::farq, x);
+    const float eD = e * __builtin_amdgcn_rcpf(x * qr);
+    E = eD * pr;
+    c = eD * qr * xi2;
+}
+
+__device__ __forceinline__ void slp_pair_f32(float dx, float dy, float r2, float xi2, float f1,
+                                             float f2, float& a1, float& a2)
+{
+    float E, c;
+    slp_radial_f32(r2, xi2, E, c);
+    const float cr = dx * f2 - dy * f1;
+    a1 += 0.5f * E * f1 + c * dy * cr;
+    a2 += 0.5f * E * f2 - c * dx * cr;
+}
+
+__device__ __forceinline__ void slp_pair2_f32(float dx, float dy, float r2, float xi2, float f1,
+                                              float f2, float g1, float g2, float& a1, float& a2,
+                                              float& b1, float& b2)
+{
+    float E, c;
+    slp_radial_f32(r2, xi2, E, c);
+    const float hE = 0.5f * E;
+    const float cf = dx * f2 - dy * f1, cg = dx * g2 - dy * g1;
+    a1 += hE * f1 + c * dy * cf;
+    a2 += hE * f2 - c * dx * cf;
+    b1 = hE * g1 + c * dy * cg;
+    b2 = hE * g2 - c * dx * cg;
+}
+
 // 4 pi times the DLP pair term for r = target - source:
 //   e^{-xi^2 r^2} [ -4 (1 + xi^2 r^2)/r^4 r (r.f)(r.n) + 2 xi^2 (f (r.n) + n (r.f) + r (f.n)) ]
 __device__ __forceinline__ void dlp_pair(double r1, double r2, double rr, double xi2,
@@ -140,6 +178,23 @@ __device__ __forceinline__ void dlp_pair(double r1, double r2, double rr, double
     const double c = 2.0 * xi2;
     a1 += e * (b * r1 + c * (s.f1 * rn + s.n1 * rf + s.fn * r1));
     a2 += e * (b * r2 + c * (s.f2 * rn + s.n2 * rf + s.fn * r2));
+}
+
+// dlp_pair in FP32 (the FP32 tier): every pair of the stresslet is a far pair
+// in this sense (no E1), so the whole term is FP32, added to FP32 partial sums.
+__device__ __forceinline__ void dlp_pair_f32(float r1, float r2, float rr, float xi2,
+                                             const DlpRec& s, float& a1, float& a2)
+{
+    const float e = __expf(-xi2 * rr);
+    const float ir = __builtin_amdgcn_rcpf(rr);
+    const float f1 = (float)s.f1, f2 = (float)s.f2, n1 = (float)s.n1, n2 = (float)s.n2;
+    const float rf = r1 * f1 + r2 * f2;
+    const float rn = r1 * n1 + r2 * n2;
+    const float b = -4.0f * (1.0f + xi2 * rr) * (ir * ir) * rf * rn;
+    const float c = 2.0f * xi2;
+    const float fn = (float)s.fn;
+    a1 += e * (b * r1 + c * (f1 * rn + n1 * rf + fn * r1));
+    a2 += e * (b * r2 + c * (f2 * rn + n2 * rf + fn * r2));
 }
 
 // 2 pi times the stresslet pressure pair term for r = target - source
@@ -618,6 +673,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     return (z ? pos_w1 : pos_w0) - bit * nch;
                 };
                 unsigned long long lo = fm[0], hi = fm[1];
+                float c1 = 0.f, c2 = 0.f;  // FP32 tier: this chunk's far pairs
                 if constexpr (SYM) {
                     // the next hit's record is read before this hit's LDS atomics
                     // are issued, so waiting for it does not wait for them
@@ -630,11 +686,20 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         const double2 qn = bxy[nidx], gn = bfd[nidx];
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
-                        double b1, b2;
-                        slp_pair2<TIER, false>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1, a2,
-                                               b1, b2);
-                        atomicAdd(&racc[idx].x, b1);
-                        atomicAdd(&racc[idx].y, b2);
+                        if constexpr (TIER == kF32) {
+                            float b1, b2;
+                            slp_pair2_f32((float)dx, (float)dy, (float)r2, (float)xi2,
+                                          (float)g.x, (float)g.y, (float)fi1, (float)fi2, c1, c2,
+                                          b1, b2);
+                            atomicAdd(&racc[idx].x, (double)b1);
+                            atomicAdd(&racc[idx].y, (double)b2);
+                        } else {
+                            double b1, b2;
+                            slp_pair2<TIER, false>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1, a2,
+                                                   b1, b2);
+                            atomicAdd(&racc[idx].x, b1);
+                            atomicAdd(&racc[idx].y, b2);
+                        }
                         idx = nidx;
                         q = qn;
                         g = gn;
@@ -649,9 +714,17 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
                         const double2 g = bfd[idx];
-                        slp_pair<TIER, false>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
+                        if constexpr (TIER == kF32)
+                            slp_pair_f32((float)dx, (float)dy, (float)r2, (float)xi2, (float)g.x,
+                                         (float)g.y, c1, c2);
+                        else
+                            slp_pair<TIER, false>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
                     } else if (KIND == SE2P_DLP) {
-                        dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
+                        if constexpr (TIER == kF32)
+                            dlp_pair_f32((float)dx, (float)dy, (float)r2, (float)xi2,
+                                         reinterpret_cast<const DlpRec&>(s), c1, c2);
+                        else
+                            dlp_pair(dx, dy, r2, xi2, reinterpret_cast<const DlpRec&>(s), a1, a2);
                     } else if (KIND == SE2P_SLP_PRESSURE) {
                         const double4& q = reinterpret_cast<const double4&>(s);
                         a1 += (dx * q.z + dy * q.w) * exp(-xi2 * r2) * rcp64(r2);
@@ -660,6 +733,11 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     } else {
                         derived_pair<KIND>(dx, dy, r2, xi2, reinterpret_cast<const double*>(&s), acc);
                     }
+                }
+                if constexpr (TIER == kF32) {
+                    // the chunk's FP32 partial sums join the FP64 sums exactly
+                    a1 += (double)c1;
+                    a2 += (double)c2;
                 }
                 if (KIND == SE2P_SLP) {
                     lo = nm[0];
@@ -678,12 +756,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             }
                         } else if (SYM) {
                             double b1, b2;
-                            slp_pair2<TIER, true>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1,
-                                                  a2, b1, b2);
+                            slp_pair2<fp64_tier(TIER), true>(dx, dy, r2, xi2, g.x, g.y, fi1,
+                                                             fi2, a1, a2, b1, b2);
                             atomicAdd(&racc[idx].x, b1);
                             atomicAdd(&racc[idx].y, b2);
                         } else {
-                            slp_pair<TIER, true>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
+                            slp_pair<fp64_tier(TIER), true>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
                         }
                     }
                 }
@@ -819,6 +897,9 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
         if (tier == kE1Full)
             hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Full, false, true>), gr, bl, 0, s, sys,
                                jobs, u);
+        else if (tier == kF32)
+            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kF32, false, true>), gr, bl, 0, s, sys,
+                               jobs, u);
         else
             hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Fast, false, true>), gr, bl, 0, s, sys,
                                jobs, u);
@@ -836,10 +917,17 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
     case SE2P_SLP:
         if (tier == kE1Full)
             SE2P_LAUNCH_REAL(SE2P_SLP, kE1Full);
+        else if (tier == kF32)
+            SE2P_LAUNCH_REAL(SE2P_SLP, kF32);
         else
             SE2P_LAUNCH_REAL(SE2P_SLP, kE1Fast);
         break;
-    case SE2P_DLP: SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full); break;
+    case SE2P_DLP:
+        if (tier == kF32)
+            SE2P_LAUNCH_REAL(SE2P_DLP, kF32);
+        else
+            SE2P_LAUNCH_REAL(SE2P_DLP, kE1Full);
+        break;
     case SE2P_SLP_PRESSURE: SE2P_LAUNCH_REAL(SE2P_SLP_PRESSURE, kE1Full); break;
     case SE2P_DLP_PRESSURE: SE2P_LAUNCH_REAL(SE2P_DLP_PRESSURE, kE1Full); break;
     case SE2P_SLP_GRADIENT: SE2P_LAUNCH_REAL(SE2P_SLP_GRADIENT, kE1Full); break;

@@ -19,6 +19,11 @@ namespace se2p {
 
 constexpr int kE1Full = 0;
 constexpr int kE1Fast = 1;
+// The FP32 tier of the real-space sums: far pairs (x >= 1) in FP32 with the
+// 'f32' fit below; near pairs, self terms and the sums' accumulation in FP64
+// (near E1 by the fast tier).
+constexpr int kF32 = 2;
+__host__ __device__ constexpr int fp64_tier(int tier) { return tier == kF32 ? kE1Fast : tier; }
 
 template <int N>
 __device__ __forceinline__ double horner(const double (&c)[N], double x)
@@ -55,6 +60,22 @@ struct E1Coeffs<kE1Fast> {
     static constexpr double farp[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARP;
     static constexpr double farq[SE2P_E1_FAST_FAR_DEG + 1] = SE2P_E1_FAST_FARQ;
 };
+
+// FP32 far fit E1(x) = e^{-x} P~(x)/(x Q~(x)) with the reversed polynomials
+// (relative error 2e-8 before rounding).
+struct E1F32 {
+    static constexpr float farp[SE2P_E1_F32_FAR_DEG + 1] = SE2P_E1_F32_FARP;
+    static constexpr float farq[SE2P_E1_F32_FAR_DEG + 1] = SE2P_E1_F32_FARQ;
+};
+
+template <int N>
+__device__ __forceinline__ float horner_rev_f(const float (&c)[N], float x)
+{
+    float r = c[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) r = fmaf(r, x, c[i]);
+    return r;
+}
 
 // e^{-x} for 0 <= x < 700: Cody-Waite reduction r = n ln 2 - x with n =
 // rint(x log2 e), |r| <= ln(2)/2, e^r by the degree-10 Chebyshev-interpolated

@@ -23,53 +23,85 @@ namespace se2p {
 
 namespace {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+// Complex values of scalar type S: double2 (FP64 grids) or float2 (FP32 grids).
+template <typename S>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    using T = double2;
+};
+template <>
+struct Cplx<float> {
+    using T = float2;
+};
+template <typename S>
+using cplx = typename Cplx<S>::T;
+
+template <typename V>
+__device__ __forceinline__ V mkc(decltype(V::x) x, decltype(V::x) y)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    V r;
+    r.x = x;
+    r.y = y;
+    return r;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b)
+{
+    return mkc<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return mkc<V>(a.x + b.x, a.y + b.y); }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return mkc<V>(a.x - b.x, a.y - b.y); }
 // v * (sgn i)
-__device__ __forceinline__ double2 rot(double2 v, double sgn) { return make_double2(-sgn * v.y, sgn * v.x); }
+template <typename V>
+__device__ __forceinline__ V rot(V v, decltype(V::x) sgn) { return mkc<V>(-sgn * v.y, sgn * v.x); }
 
 // In-register DFTs of length R with kernel exp(sgn 2 pi i r k / R), sgn = -1 forward.
-__device__ __forceinline__ void dft2(double2* v)
+template <typename V>
+__device__ __forceinline__ void dft2(V* v)
 {
-    const double2 a = v[0], b = v[1];
+    const V a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
 }
 
-__device__ __forceinline__ void dft4(double2& a, double2& b, double2& c, double2& d, double sgn)
+template <typename V>
+__device__ __forceinline__ void dft4(V& a, V& b, V& c, V& d, decltype(V::x) sgn)
 {
-    const double2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = rot(csub(b, d), sgn);
+    const V t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = rot(csub(b, d), sgn);
     a = cadd(t0, t2);
     c = csub(t0, t2);
     b = cadd(t1, t3);
     d = csub(t1, t3);
 }
 
-__device__ __forceinline__ void dft3(double2* v, double sgn)
+template <typename V>
+__device__ __forceinline__ void dft3(V* v, decltype(V::x) sgn)
 {
-    const double h = 0.86602540378443864676;  // sin(2 pi/3)
-    const double2 a = v[0], s = cadd(v[1], v[2]), d = rot(csub(v[1], v[2]), sgn * h);
-    const double2 m = make_double2(a.x - 0.5 * s.x, a.y - 0.5 * s.y);
+    using S = decltype(V::x);
+    const S h = (S)0.86602540378443864676;  // sin(2 pi/3)
+    const V a = v[0], s = cadd(v[1], v[2]), d = rot(csub(v[1], v[2]), sgn * h);
+    const V m = mkc<V>(a.x - (S)0.5 * s.x, a.y - (S)0.5 * s.y);
     v[0] = cadd(a, s);
     v[1] = cadd(m, d);
     v[2] = csub(m, d);
 }
 
-__device__ __forceinline__ void dft5(double2* v, double sgn)
+template <typename V>
+__device__ __forceinline__ void dft5(V* v, decltype(V::x) sgn)
 {
-    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos(2pi/5), cos(4pi/5)
-    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin(2pi/5), sin(4pi/5)
-    const double2 a = v[0];
-    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const double2 p1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-    const double2 p2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
-    const double2 q1 = rot(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sgn);
-    const double2 q2 = rot(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sgn);
+    using S = decltype(V::x);
+    const S c1 = (S)0.30901699437494742410, c2 = (S)-0.80901699437494742410;  // cos(2pi/5), cos(4pi/5)
+    const S s1 = (S)0.95105651629515357212, s2 = (S)0.58778525229247312917;   // sin(2pi/5), sin(4pi/5)
+    const V a = v[0];
+    const V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const V t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const V p1 = mkc<V>(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    const V p2 = mkc<V>(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    const V q1 = rot(mkc<V>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sgn);
+    const V q2 = rot(mkc<V>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sgn);
     v[0] = cadd(a, cadd(t1, t2));
     v[1] = cadd(p1, q1);
     v[4] = csub(p1, q1);
@@ -77,17 +109,19 @@ __device__ __forceinline__ void dft5(double2* v, double sgn)
     v[3] = csub(p2, q2);
 }
 
-__device__ __forceinline__ void dft8(double2* v, double sgn)
+template <typename V>
+__device__ __forceinline__ void dft8(V* v, decltype(V::x) sgn)
 {
-    const double r = 0.70710678118654752440;
-    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    using S = decltype(V::x);
+    const S r = (S)0.70710678118654752440;
+    V e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    V o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4(e0, e1, e2, e3, sgn);
     dft4(o0, o1, o2, o3, sgn);
     // W8^k = exp(sgn i pi k/4)
-    const double2 w1 = make_double2(r * (o1.x - sgn * o1.y), r * (o1.y + sgn * o1.x));
-    const double2 w2 = rot(o2, sgn);
-    const double2 w3 = make_double2(-r * (o3.x + sgn * o3.y), r * (sgn * o3.x - o3.y));
+    const V w1 = mkc<V>(r * (o1.x - sgn * o1.y), r * (o1.y + sgn * o1.x));
+    const V w2 = rot(o2, sgn);
+    const V w3 = mkc<V>(-r * (o3.x + sgn * o3.y), r * (sgn * o3.x - o3.y));
     v[0] = cadd(e0, o0);
     v[4] = csub(e0, o0);
     v[1] = cadd(e1, w1);
@@ -119,37 +153,38 @@ __device__ __forceinline__ void divmod(int j, int d, float rd, int& q, int& r)
     }
 }
 
-template <int R>
-__device__ __forceinline__ void butterfly(double2* v, double sgn);
+template <int R, typename V>
+__device__ __forceinline__ void butterfly(V* v, decltype(V::x) sgn);
 
 // DFT of length R = R1 R2 in registers: with r = r1 + R1 r2 and k = k2 + R2 k1,
 //   X[k] = sum_r1 W_R1^(r1 k1) W_R^(r1 k2) sum_r2 W_R2^(r2 k2) x[r1 + R1 r2],
 // i.e. R1 DFTs of length R2, the twiddles W_R^(r1 k2) (compile-time constants),
 // then R2 DFTs of length R1.
-template <int R1, int R2>
-__device__ __forceinline__ void dft_split(double2* v, double sgn)
+template <int R1, int R2, typename V>
+__device__ __forceinline__ void dft_split(V* v, decltype(V::x) sgn)
 {
+    using S = decltype(V::x);
     constexpr int R = R1 * R2;
-    double2 y[R1][R2];
+    V y[R1][R2];
 #pragma unroll
     for (int r1 = 0; r1 < R1; ++r1) {
 #pragma unroll
         for (int r2 = 0; r2 < R2; ++r2) y[r1][r2] = v[r1 + R1 * r2];
-        butterfly<R2>(y[r1], sgn);
+        butterfly<R2, V>(y[r1], sgn);
     }
 #pragma unroll
     for (int r1 = 1; r1 < R1; ++r1)
 #pragma unroll
         for (int k2 = 1; k2 < R2; ++k2) {
             const int m = (r1 * k2) % R;
-            y[r1][k2] = cmul(y[r1][k2], make_double2(FftRoots<R>::c[m], sgn * FftRoots<R>::s[m]));
+            y[r1][k2] = cmul(y[r1][k2], mkc<V>((S)FftRoots<R>::c[m], sgn * (S)FftRoots<R>::s[m]));
         }
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) {
-        double2 z[R1];
+        V z[R1];
 #pragma unroll
         for (int r1 = 0; r1 < R1; ++r1) z[r1] = y[r1][k2];
-        butterfly<R1>(z, sgn);
+        butterfly<R1, V>(z, sgn);
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) v[k2 + R2 * k1] = z[k1];
     }
@@ -157,22 +192,22 @@ __device__ __forceinline__ void dft_split(double2* v, double sgn)
 
 // In-register DFT of radix R in {2, 3, 4, 5, 8} directly, {6, 9, 10, 12, 15,
 // 16, 20, 25} split into two of those.
-template <int R>
-__device__ __forceinline__ void butterfly(double2* v, double sgn)
+template <int R, typename V>
+__device__ __forceinline__ void butterfly(V* v, decltype(V::x) sgn)
 {
     if constexpr (R == 2) dft2(v);
     else if constexpr (R == 3) dft3(v, sgn);
     else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3], sgn);
     else if constexpr (R == 5) dft5(v, sgn);
     else if constexpr (R == 8) dft8(v, sgn);
-    else if constexpr (R == 6) dft_split<2, 3>(v, sgn);
-    else if constexpr (R == 9) dft_split<3, 3>(v, sgn);
-    else if constexpr (R == 10) dft_split<2, 5>(v, sgn);
-    else if constexpr (R == 12) dft_split<4, 3>(v, sgn);
-    else if constexpr (R == 15) dft_split<3, 5>(v, sgn);
-    else if constexpr (R == 16) dft_split<4, 4>(v, sgn);
-    else if constexpr (R == 20) dft_split<4, 5>(v, sgn);
-    else dft_split<5, 5>(v, sgn);
+    else if constexpr (R == 6) dft_split<2, 3, V>(v, sgn);
+    else if constexpr (R == 9) dft_split<3, 3, V>(v, sgn);
+    else if constexpr (R == 10) dft_split<2, 5, V>(v, sgn);
+    else if constexpr (R == 12) dft_split<4, 3, V>(v, sgn);
+    else if constexpr (R == 15) dft_split<3, 5, V>(v, sgn);
+    else if constexpr (R == 16) dft_split<4, 4, V>(v, sgn);
+    else if constexpr (R == 20) dft_split<4, 5, V>(v, sgn);
+    else dft_split<5, 5, V>(v, sgn);
     static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 6 || R == 9 ||
                       R == 10 || R == 12 || R == 15 || R == 16 || R == 20 || R == 25,
                   "unsupported in-register radix");
@@ -184,22 +219,36 @@ __device__ __forceinline__ void butterfly(double2* v, double sgn)
 // must travel in the type for LDS to be accessed with ds_* and HBM with
 // global_* instructions.
 typedef double dvec2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(3))) dvec2 lds_d2;
-typedef __attribute__((address_space(1))) dvec2 glb_d2;
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+template <typename S>
+struct Mem;
+template <>
+struct Mem<double> {
+    typedef __attribute__((address_space(3))) dvec2 lds;
+    typedef __attribute__((address_space(1))) dvec2 glb;
+};
+template <>
+struct Mem<float> {
+    typedef __attribute__((address_space(3))) fvec2 lds;
+    typedef __attribute__((address_space(1))) fvec2 glb;
+};
+// the twiddle tables hold FP64 values for both grid types
+typedef Mem<double>::glb glb_d2;
 
-template <typename Ptr>
-__device__ __forceinline__ double2 ld2(Ptr p, int i)
+template <typename S, typename Ptr>
+__device__ __forceinline__ cplx<S> ld2(Ptr p, int i)
 {
-    const dvec2 v = p[i];
-    return make_double2(v.x, v.y);
+    const auto v = p[i];
+    return mkc<cplx<S>>((S)v.x, (S)v.y);
 }
 
-__device__ __forceinline__ double2 conj_if(double2 a, int c) { return make_double2(a.x, c ? -a.y : a.y); }
+template <typename V>
+__device__ __forceinline__ V conj_if(V a, int c) { return mkc<V>(a.x, c ? -a.y : a.y); }
 
-template <typename Ptr>
-__device__ __forceinline__ void st2(Ptr p, int i, double2 a)
+template <typename Ptr, typename V>
+__device__ __forceinline__ void st2(Ptr p, int i, V a)
 {
-    dvec2 v;
+    auto v = p[i];
     v.x = a.x;
     v.y = a.y;
     p[i] = v;
@@ -210,19 +259,21 @@ __device__ __forceinline__ void st2(Ptr p, int i, double2 a)
 // reads its inputs, a barrier, every thread writes its outputs.  FIRST reads the
 // lines from HBM (gdata) and LAST writes them back there; the other stages read
 // and write the LDS copy.  Twiddles: one table read per butterfly, the other
-// R-2 as powers of it.
-template <int R, bool FIRST, bool LAST, int T, int PPT>
-__device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
-                                                    glb_d2* __restrict__ gdata, int n, int nb,
-                                                    int Ns, const glb_d2* __restrict__ tw,
-                                                    int inverse)
+// R-2 as powers of it.  S is the grid's scalar type; the twiddle table is FP64
+// for both and rounded on load for FP32 lines.
+template <int R, bool FIRST, bool LAST, int T, int PPT, typename S>
+__device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __restrict__ lds,
+                                                    typename Mem<S>::glb* __restrict__ gdata,
+                                                    int n, int nb, int Ns,
+                                                    const glb_d2* __restrict__ tw, int inverse)
 {
+    using V = cplx<S>;
     constexpr int BPT = (PPT + R - 1) / R;  // butterflies per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
     const float rNs = 1.0f / (float)Ns, rm = 1.0f / (float)m;
     const int total = nb * m;
-    double2 v[BPT][R];
+    V v[BPT][R];
     int ob[BPT];
 #pragma unroll
     for (int t = 0; t < BPT; ++t) {
@@ -235,21 +286,22 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
             divmod(j, Ns, rNs, jq, jm);
             if constexpr (FIRST) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[t][r] = conj_if(ld2(gdata, b * n + j + r * m), inverse);
+                for (int r = 0; r < R; ++r)
+                    v[t][r] = conj_if(ld2<S>(gdata, b * n + j + r * m), inverse);
             } else {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[t][r] = ld2(lds, b * n + j + r * m);
+                for (int r = 0; r < R; ++r) v[t][r] = ld2<S>(lds, b * n + j + r * m);
             }
             if (Ns > 1) {
-                const double2 w1 = ld2(tw, jm * base);
-                double2 w = w1;
+                const V w1 = ld2<S>(tw, jm * base);
+                V w = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     v[t][r] = cmul(v[t][r], w);
                     if (r + 1 < R) w = cmul(w, w1);
                 }
             }
-            butterfly<R>(v[t], -1.0);
+            butterfly<R, V>(v[t], (S)-1.0);
             ob[t] = b * n + jq * Ns * R + jm;
         }
     }
@@ -270,18 +322,19 @@ __device__ __attribute__((noinline)) void stage_reg(lds_d2* __restrict__ lds,
 }
 
 // Any radix: one output per (thread, slot), O(R) work per output.
-template <bool FIRST, bool LAST, int T, int PPT>
-__device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds,
-                                                        glb_d2* __restrict__ gdata, int n, int nb,
-                                                        int Ns, int R,
+template <bool FIRST, bool LAST, int T, int PPT, typename S>
+__device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __restrict__ lds,
+                                                        typename Mem<S>::glb* __restrict__ gdata,
+                                                        int n, int nb, int Ns, int R,
                                                         const glb_d2* __restrict__ tw, int inverse)
 {
+    using V = cplx<S>;
     constexpr int OPT = PPT;  // outputs per thread at most
     const int m = n / R;
     const int base = n / (Ns * R);
     const float rn = 1.0f / (float)n, rm = 1.0f / (float)m, rNs = 1.0f / (float)Ns;
     const int total = nb * n;
-    double2 res[OPT];
+    V res[OPT];
     int ob[OPT];
 #pragma unroll
     for (int t = 0; t < OPT; ++t) {
@@ -294,22 +347,22 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
             divmod(j, Ns, rNs, jq, jm);
             const int step = (jm + k * Ns) * base;  // < n
             const int i0 = b * n + j;
-            double re = 0, im = 0;
+            S re = 0, im = 0;
             int e = 0;
             for (int r = 0; r < R; ++r) {
-                double2 x;
+                V x;
                 if constexpr (FIRST)
-                    x = conj_if(ld2(gdata, i0 + r * m), inverse);
+                    x = conj_if(ld2<S>(gdata, i0 + r * m), inverse);
                 else
-                    x = ld2(lds, i0 + r * m);
-                const double2 wv = ld2(tw, e);
-                const double wi = wv.y;
+                    x = ld2<S>(lds, i0 + r * m);
+                const V wv = ld2<S>(tw, e);
+                const S wi = wv.y;
                 re += x.x * wv.x - x.y * wi;
                 im += x.x * wi + x.y * wv.x;
                 e += step;
                 if (e >= n) e -= n;
             }
-            res[t] = make_double2(re, im);
+            res[t] = mkc<V>(re, im);
             ob[t] = b * n + jq * Ns * R + jm + k * Ns;
         }
     }
@@ -330,12 +383,15 @@ __device__ __attribute__((noinline)) void stage_generic(lds_d2* __restrict__ lds
 // (axis 0: My-lines, the column-major grid) or x (axis 1: Mx-lines, the
 // transposed grid); every line is contiguous.  The lines live in one LDS
 // buffer of B*n points between the first and the last stage.  T threads with
-// B*n <= PPT * T.
-template <int T, int PPT, bool COMP>
+// B*n <= PPT * T.  S = float: the grid holds float2 values (half the bytes of
+// the FP64 grid in the same buffer).
+template <int T, int PPT, bool COMP, typename S>
 __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys,
                                                  const Job* __restrict__ jobs, int axis, int sel,
                                                  int inverse)
 {
+    using LP = typename Mem<S>::lds;
+    using GP = typename Mem<S>::glb;
     extern __shared__ double2 lds[];
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
@@ -344,8 +400,8 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
     const int B = axis ? d.Bx : d.By;
     const FftAxis& ax = axis ? d.ax : d.ay;
     const glb_d2* __restrict__ tw = (const glb_d2*)(axis ? d.twx : d.twy);
-    glb_d2* __restrict__ data = (glb_d2*)(d.grid[sel] + (size_t)jb.y * B * n);
-    lds_d2* __restrict__ buf = (lds_d2*)lds;
+    GP* __restrict__ data = (GP*)d.grid[sel] + (size_t)jb.y * B * n;
+    LP* __restrict__ buf = (LP*)lds;
     const int nb = min(B, nlines - jb.y * B);
     // The table holds exp(-2 pi i k/n) and every stage runs the forward
     // transform; the inverse is conj(FFT(conj(x))): the first stage conjugates
@@ -355,11 +411,11 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
     for (int st = 0; st < nst; ++st) {
         const int R = ax.radix[st];
         const bool first = st == 0, last = st == nst - 1;
-#define SE2P_STAGE(RR)                                                          \
-    if (first && last) stage_reg<RR, true, true, T, PPT>(buf, data, n, nb, Ns, tw, inverse);  \
-    else if (first) stage_reg<RR, true, false, T, PPT>(buf, data, n, nb, Ns, tw, inverse);   \
-    else if (last) stage_reg<RR, false, true, T, PPT>(buf, data, n, nb, Ns, tw, inverse);    \
-    else stage_reg<RR, false, false, T, PPT>(buf, data, n, nb, Ns, tw, inverse)
+#define SE2P_STAGE(RR)                                                                        \
+    if (first && last) stage_reg<RR, true, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse); \
+    else if (first) stage_reg<RR, true, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse);  \
+    else if (last) stage_reg<RR, false, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse);   \
+    else stage_reg<RR, false, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
@@ -377,13 +433,13 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
 #undef SE2P_COMPOSITE
             default:
                 if (first && last)
-                    stage_generic<true, true, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<true, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
                 else if (first)
-                    stage_generic<true, false, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<true, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
                 else if (last)
-                    stage_generic<false, true, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<false, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
                 else
-                    stage_generic<false, false, T, PPT>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<false, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
                 break;
         }
 #undef SE2P_STAGE
@@ -393,22 +449,24 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
 
 // out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows
 // = Mx, cols = My (column-major grid to transposed), dir 1 the reverse.
+template <typename S>
 __global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ sys,
                                                    const Job* __restrict__ jobs, int in_sel,
                                                    int out_sel, int dir)
 {
-    __shared__ double2 tile[32][33];
+    using GP = typename Mem<S>::glb;
+    __shared__ cplx<S> tile[32][33];
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int rows = dir ? d.My : d.Mx, cols = dir ? d.Mx : d.My;
-    const glb_d2* __restrict__ in = (const glb_d2*)d.grid[in_sel];
-    glb_d2* __restrict__ out = (glb_d2*)d.grid[out_sel];
+    const GP* __restrict__ in = (const GP*)d.grid[in_sel];
+    GP* __restrict__ out = (GP*)d.grid[out_sel];
     const int ntc = (cols + 31) / 32;
     const int r0 = (jb.y / ntc) * 32, c0 = (jb.y % ntc) * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int k = ty; k < 32; k += 8) {
         const int r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[k][tx] = ld2(in, r * cols + c);
+        if (r < rows && c < cols) tile[k][tx] = ld2<S>(in, r * cols + c);
     }
     __syncthreads();
     for (int k = ty; k < 32; k += 8) {
@@ -421,7 +479,10 @@ __global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ s
 
 int fft_lines_per_block(int n) { return n >= 256 ? 1 : (256 + n - 1) / n; }
 
-size_t fft_lds_bytes(int n) { return sizeof(double2) * (size_t)fft_lines_per_block(n) * n; }
+size_t fft_lds_bytes(int n, int prec)
+{
+    return (prec ? sizeof(float2) : sizeof(double2)) * (size_t)fft_lines_per_block(n) * n;
+}
 
 int fft_axis_init(FftAxis* a, int n)
 {
@@ -490,9 +551,12 @@ int fft_prepare()
 {
     static bool attr_set = false;
     if (!attr_set) {
-        const void* kernels[] = {(const void*)k_fft_lines<256, 8, false>,
-                                 (const void*)k_fft_lines<256, 10, true>,
-                                 (const void*)k_fft_lines<512, 8, false>};
+        const void* kernels[] = {(const void*)k_fft_lines<256, 8, false, double>,
+                                 (const void*)k_fft_lines<256, 10, true, double>,
+                                 (const void*)k_fft_lines<512, 8, false, double>,
+                                 (const void*)k_fft_lines<256, 8, false, float>,
+                                 (const void*)k_fft_lines<256, 10, true, float>,
+                                 (const void*)k_fft_lines<512, 8, false, float>};
         for (const void* k : kernels)
             SE2P_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                kFftMaxLen * (int)sizeof(double2)));
@@ -503,30 +567,45 @@ int fft_prepare()
 
 int fft_line_class(int n) { return n <= 2048 ? 0 : n <= 2560 ? 1 : 2; }
 
-int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
-                     const SysDesc* sys, const Job* jobs, int njobs)
+template <typename S>
+static void fft_lines_launch(hipStream_t s, int axis, int sel, int inverse, int cls,
+                             size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs)
+{
+    if (cls == 0)
+        hipLaunchKernelGGL((k_fft_lines<256, 8, false, S>), dim3(njobs), dim3(256), max_lds, s, sys,
+                           jobs, axis, sel, inverse);
+    else if (cls == 1)
+        hipLaunchKernelGGL((k_fft_lines<256, 10, true, S>), dim3(njobs), dim3(256), max_lds, s, sys,
+                           jobs, axis, sel, inverse);
+    else
+        hipLaunchKernelGGL((k_fft_lines<512, 8, false, S>), dim3(njobs), dim3(512), max_lds, s, sys,
+                           jobs, axis, sel, inverse);
+}
+
+int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls,
+                     size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    if (cls == 0)
-        hipLaunchKernelGGL((k_fft_lines<256, 8, false>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
-                           axis, sel, inverse);
-    else if (cls == 1)
-        hipLaunchKernelGGL((k_fft_lines<256, 10, true>), dim3(njobs), dim3(256), max_lds, s, sys, jobs,
-                           axis, sel, inverse);
+    if (prec)
+        fft_lines_launch<float>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
     else
-        hipLaunchKernelGGL((k_fft_lines<512, 8, false>), dim3(njobs), dim3(512), max_lds, s, sys, jobs,
-                           axis, sel, inverse);
+        fft_lines_launch<double>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
 int transpose_tiles(int rows, int cols) { return ((rows + 31) / 32) * ((cols + 31) / 32); }
 
-int launch_transpose(hipStream_t s, int in, int out, int dir, const SysDesc* sys,
+int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
                      const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_transpose, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out, dir);
+    if (prec)
+        hipLaunchKernelGGL(k_transpose<float>, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out,
+                           dir);
+    else
+        hipLaunchKernelGGL(k_transpose<double>, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out,
+                           dir);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

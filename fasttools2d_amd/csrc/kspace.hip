@@ -20,6 +20,8 @@
 #include "se2p_internal.h"
 #include "kfilter.h"
 
+#include <type_traits>
+
 namespace se2p {
 
 namespace {
@@ -75,10 +77,46 @@ __device__ __forceinline__ void fill_row(double* row, int o, double e, double e3
     }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 rows: the weights exp(t (px - h (i - P/2))^2) of support nodes i, with
+// th2 = t h^2 and q = px/h, each evaluated directly (the FP32 recurrence would
+// accumulate a rounding per node).
 template <int P>
+__device__ __forceinline__ void fill_row_f32(float* row, int o, float q, float th2)
+{
+#pragma unroll
+    for (int x = 0; x < kSB; x += 2) *reinterpret_cast<float2*>(row + x) = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const int x = o + i;
+        const float z = q - (float)(i - P / 2);
+        row[x >= 0 && x < kSB ? x : kSB] = __expf(th2 * z * z);
+    }
+}
+
+// (hi, lo) += v exactly as a pair (Knuth's two-sum), so a block's FP32 sum
+// carries no rounding from one chunk to the next.
+__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
+{
+#pragma clang fp contract(off)
+    const float s = hi + v;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (v - bb);
+    hi = s;
+    lo += err;
+}
+
+// S = double: the FGG recurrence of the reference and v_mfma_f64_16x16x4f64,
+// accumulated straight into the block's two 16 x 16 products.  S = float (the
+// FP32 tier): weights evaluated directly, v_mfma_f32_16x16x4f32 into a
+// chunk-local product, folded into (hi, lo) pairs by two-sum after every chunk;
+// the block is stored as float2 (hi + lo rounded once).
+template <int P, typename S>
 __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs, int c)
 {
+    constexpr bool F32 = std::is_same<S, float>::value;
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int* __restrict__ tstart = d.sp_start;
@@ -94,8 +132,8 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
         e1[i] = __hiloint2double(hi, lo);
     }
-    __shared__ __align__(16) double WX[kSC][kRow];
-    __shared__ __align__(16) double WY[kSC][kRow];
+    __shared__ __align__(16) S WX[kSC][kRow];
+    __shared__ __align__(16) S WY[kSC][kRow];
     __shared__ double4 SF[kSC];
     __shared__ double2 DD[kSC];
     __shared__ int OX[kSC], OY[kSC];
@@ -103,13 +141,16 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const int lane = threadIdx.x;
     // rows past a chunk's count are read (and multiplied by a zero x weight) in
     // its last matrix step: keep every y row and density finite from the start
-    for (int i = lane; i < kSC * kRow; i += 64) (&WY[0][0])[i] = 0.0;
+    for (int i = lane; i < kSC * kRow; i += 64) (&WY[0][0])[i] = (S)0;
     if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
     const int X0 = (jb.y % ntx) * kSB, Y0 = (jb.y / ntx) * kSB;
     const int XE = min(X0 + kSB, Mx), YE = min(Y0 + kSB, My);
     const int q = lane & (kSC - 1);
     const int row = lane >> 4, col = lane & 15;  // MFMA operand: point 4 s + row, node col
     f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    float h0[4] = {0.f, 0.f, 0.f, 0.f}, l0[4] = {0.f, 0.f, 0.f, 0.f};
+    float h1[4] = {0.f, 0.f, 0.f, 0.f}, l1[4] = {0.f, 0.f, 0.f, 0.f};
+    const float th2 = (float)(d.t * d.h * d.h);
     // image kx: start node m.x - kx Mx; the window reaches images while X0 - P + kx Mx < Mx
     for (int kx = 0; X0 - P + kx * Mx < Mx; ++kx) {
         const int lox = max(0, X0 - P + kx * Mx), hix = min(Mx - 1, XE - 1 + kx * Mx);
@@ -149,55 +190,94 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
                         if (q < n) {  // lanes 0-31: x rows, 32-63: y rows, one code path
                             const bool xr = lane < kSC;
                             const double4 f4 = SF[q];
-                            fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q], xr ? f4.x : f4.z,
-                                        xr ? f4.y : f4.w, e1);
+                            if constexpr (F32)
+                                fill_row_f32<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
+                                                (float)(xr ? f4.x : f4.z), th2);
+                            else
+                                fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
+                                            xr ? f4.x : f4.z, xr ? f4.y : f4.w, e1);
                         }
                         __syncthreads();
                         // all operands of the chunk first, then the matrix steps back to back
-                        double a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
+                        S a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
 #pragma unroll
                         for (int st = 0; st < kSC / 4; ++st) {
                             const int kk = 4 * st + row;
-                            const double wx = WX[kk][col];
-                            a[st] = kk < n ? wx : 0.0;
-                            const double wy = WY[kk][col];
+                            const S wx = WX[kk][col];
+                            a[st] = kk < n ? wx : (S)0;
+                            const S wy = WY[kk][col];
                             const double2 dd = DD[kk];
-                            b0[st] = dd.x * wy;
-                            b1[st] = dd.y * wy;
+                            b0[st] = (S)dd.x * wy;
+                            b1[st] = (S)dd.y * wy;
                         }
+                        if constexpr (F32) {
+                            f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                        for (int st = 0; st < kSC / 4; ++st)
-                            if (4 * st < n) {
-                                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st], acc0, 0,
-                                                                            0, 0);
-                                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st], acc1, 0,
-                                                                            0, 0);
+                            for (int st = 0; st < kSC / 4; ++st)
+                                if (4 * st < n) {
+                                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b0[st], c0,
+                                                                              0, 0, 0);
+                                    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b1[st], c1,
+                                                                              0, 0, 0);
+                                }
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                two_sum_add(h0[r], l0[r], c0[r]);
+                                two_sum_add(h1[r], l1[r], c1[r]);
                             }
+                        } else {
+#pragma unroll
+                            for (int st = 0; st < kSC / 4; ++st)
+                                if (4 * st < n) {
+                                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st],
+                                                                                acc0, 0, 0, 0);
+                                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st],
+                                                                                acc1, 0, 0, 0);
+                                }
+                        }
                     }
                 }
         }
     }
-    // the product's element (x, y) sits in lane y + 16 (x % 4), register x / 4
-    double2* __restrict__ out = d.grid[c ? G_Z1 : G_Z0];
     const int iy = Y0 + col;
+    if constexpr (F32) {
+        // f32 product: element (x, y) in lane y + 16 (x / 4), register x % 4
+        float2* __restrict__ out = reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int ix = X0 + 4 * r + row;
-        if (ix < Mx && iy < My) out[(size_t)ix * My + iy] = make_double2(acc0[r], acc1[r]);
+        for (int r = 0; r < 4; ++r) {
+            const int ix = X0 + 4 * row + r;
+            if (ix < Mx && iy < My)
+                out[(size_t)ix * My + iy] = make_float2(h0[r] + l0[r], h1[r] + l1[r]);
+        }
+    } else {
+        // f64 product: element (x, y) in lane y + 16 (x % 4), register x / 4
+        double2* __restrict__ out = d.grid[c ? G_Z1 : G_Z0];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ix = X0 + 4 * r + row;
+            if (ix < Mx && iy < My) out[(size_t)ix * My + iy] = make_double2(acc0[r], acc1[r]);
+        }
     }
 }
 
+// Per binned point: its start node and either the FGG factors (ex, e3x, ey,
+// e3y) of the FP64 recurrence or, for the FP32 tier (direct), its offsets in
+// grid spacings (px/h, 0, py/h, 0).
 __global__ void k_spread_records(const int* __restrict__ perm, const int2* __restrict__ node,
                                  const double2* __restrict__ off, int n, double t, double w,
-                                 double h, int2* __restrict__ snode, double4* __restrict__ fac)
+                                 double h, int direct, int2* __restrict__ snode,
+                                 double4* __restrict__ fac)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int p = perm[k];
     const double2 o = off[p];
     snode[k] = node[p];
-    fac[k] = make_double4(exp(t * (o.x * o.x + 2 * w * o.x)), exp(-2 * t * h * o.x),
-                          exp(t * (o.y * o.y + 2 * w * o.y)), exp(-2 * t * h * o.y));
+    if (direct)
+        fac[k] = make_double4(o.x / h, 0.0, o.y / h, 0.0);
+    else
+        fac[k] = make_double4(exp(t * (o.x * o.x + 2 * w * o.x)), exp(-2 * t * h * o.x),
+                              exp(t * (o.y * o.y + 2 * w * o.y)), exp(-2 * t * h * o.y));
 }
 
 __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__ sys,
@@ -227,9 +307,11 @@ __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__
 // with rows My - iy, and in the rows that are their own partners (0, My/2) the
 // modes ix <= Mx - ix take both.  Each output is the one filter_mode gives for
 // its mode alone.
-template <int KIND>
+// S = float: the grids hold float2 values; each mode is filtered in FP64.
+template <int KIND, typename S>
 __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict__ jobs)
 {
+    using V = typename std::conditional<std::is_same<S, float>::value, float2, double2>::type;
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int Mx = d.Mx, My = d.My;
@@ -240,20 +322,29 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
     if (jy == iy && jx < ix) return;  // taken with its partner
     const long gk = gid, gm = (long)jy * Mx + jx;
-    const double2* __restrict__ W = d.grid[G_W];
-    const double2* __restrict__ D = d.grid[G_D];
+    const V* __restrict__ W = reinterpret_cast<const V*>(d.grid[G_W]);
+    const V* __restrict__ D = reinterpret_cast<const V*>(d.grid[G_D]);
+    V* __restrict__ Z0 = reinterpret_cast<V*>(d.grid[G_Z0]);
+    V* __restrict__ Z1 = reinterpret_cast<V*>(d.grid[G_Z1]);
+    auto ld = [](const V* a, long i) { return make_double2(a[i].x, a[i].y); };
+    auto st = [](V* a, long i, double2 v) {
+        V r;
+        r.x = v.x;
+        r.y = v.y;
+        a[i] = r;
+    };
     const double2 z = make_double2(0.0, 0.0);
     const bool two_in = dlp_family(KIND);
-    const double2 wk = W[gk], wm = W[gm];
-    const double2 dk = two_in ? D[gk] : z, dm = two_in ? D[gm] : z;
+    const double2 wk = ld(W, gk), wm = ld(W, gm);
+    const double2 dk = two_in ? ld(D, gk) : z, dm = two_in ? ld(D, gm) : z;
     double2 o0, o1;
     kf::filter_mode<KIND>(d, ix, iy, wk, wm, dk, dm, o0, o1);
-    d.grid[G_Z0][gk] = o0;
-    if (out_grids(KIND) == 2) d.grid[G_Z1][gk] = o1;
+    st(Z0, gk, o0);
+    if (out_grids(KIND) == 2) st(Z1, gk, o1);
     if (gm == gk) return;
     kf::filter_mode<KIND>(d, jx, jy, wm, wk, dm, dk, o0, o1);
-    d.grid[G_Z0][gm] = o0;
-    if (out_grids(KIND) == 2) d.grid[G_Z1][gm] = o1;
+    st(Z0, gm, o0);
+    if (out_grids(KIND) == 2) st(Z1, gm, o1);
 }
 
 // Gathering.  One 256-lane workgroup per tile of gTX x gTY start nodes: the
@@ -268,12 +359,17 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
 // does not change its sum, whose order is fixed: row sums over j with the y
 // weights held in registers (P is a template parameter, so the row unrolls and
 // its LDS reads are issued together), then the rows over i.
-template <int P>
+// S = float (the FP32 tier): the region is staged as float2, the weights are
+// evaluated directly in FP32 (as in spreading), each row is summed in FP32 and
+// the rows are added in FP64.
+template <int P, typename S>
 __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs,
                                                double* __restrict__ u_base, int g,
                                                int accumulate, int use_zm)
 {
+    constexpr bool F32 = std::is_same<S, float>::value;
+    using V = typename std::conditional<F32, float2, double2>::type;
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int* __restrict__ tstart = d.gt_start;
@@ -287,7 +383,9 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const double* zm = use_zm ? d.zm : nullptr;
     const bool sym = d.sym && g == 0;
     double* u = u_base + d.u_off;
-    extern __shared__ double2 reg[];
+    extern __shared__ double2 reg_raw[];
+    V* const reg = reinterpret_cast<V*>(reg_raw);
+    const V* __restrict__ Wg = reinterpret_cast<const V*>(W);
     __shared__ double e1s[P + 1];
     __shared__ int wcnt[4][16];
     __shared__ int order[256];
@@ -302,7 +400,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     for (int lx = wv; lx < R; lx += 4) {
         int gx = X0 + lx;
         while (gx >= Mx) gx -= Mx;
-        const double2* __restrict__ col = W + (size_t)gx * My;
+        const V* __restrict__ col = Wg + (size_t)gx * My;
         for (int ly = lane; ly < R; ly += 64) {
             int gy = Y0 + ly;
             while (gy >= My) gy -= My;
@@ -343,32 +441,56 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
             const int2 m = node[p];
             const double2 o = off[p];
             const int ox = m.x - X0, oy = m.y - Y0;
-            double wy[P + 1];
-            {
-                double ey = exp(t * (o.y * o.y + 2 * w * o.y));
-                const double e3y = exp(-2 * t * h * o.y);
-#pragma unroll
-                for (int j = 0; j <= P; ++j) {
-                    wy[j] = ey * e1s[j];
-                    ey *= e3y;
-                }
-            }
-            double ex = exp(t * (o.x * o.x + 2 * w * o.x));
-            const double e3x = exp(-2 * t * h * o.x);
             double s0 = 0, s1 = 0;
-            const double2* row = reg + ox * R + oy;
-            for (int i = 0; i <= P; ++i, row += R) {
-                double r0 = 0, r1 = 0;
+            const V* row = reg + ox * R + oy;
+            if constexpr (F32) {
+                const float th2 = (float)(t * h * h);
+                const float qx = (float)(o.x / h), qy = (float)(o.y / h);
+                float wy[P + 1];
 #pragma unroll
                 for (int j = 0; j <= P; ++j) {
-                    const double2 v = row[j];
-                    r0 = fma(wy[j], v.x, r0);
-                    r1 = fma(wy[j], v.y, r1);
+                    const float z = qy - (float)(j - P / 2);
+                    wy[j] = __expf(th2 * z * z);
                 }
-                const double gx = ex * e1s[i];
-                s0 = fma(gx, r0, s0);
-                s1 = fma(gx, r1, s1);
-                ex *= e3x;
+                for (int i = 0; i <= P; ++i, row += R) {
+                    float r0 = 0.f, r1 = 0.f;
+#pragma unroll
+                    for (int j = 0; j <= P; ++j) {
+                        const V v = row[j];
+                        r0 = fmaf(wy[j], v.x, r0);
+                        r1 = fmaf(wy[j], v.y, r1);
+                    }
+                    const float z = qx - (float)(i - P / 2);
+                    const double gx = (double)__expf(th2 * z * z);
+                    s0 = fma(gx, (double)r0, s0);
+                    s1 = fma(gx, (double)r1, s1);
+                }
+            } else {
+                double wy[P + 1];
+                {
+                    double ey = exp(t * (o.y * o.y + 2 * w * o.y));
+                    const double e3y = exp(-2 * t * h * o.y);
+#pragma unroll
+                    for (int j = 0; j <= P; ++j) {
+                        wy[j] = ey * e1s[j];
+                        ey *= e3y;
+                    }
+                }
+                double ex = exp(t * (o.x * o.x + 2 * w * o.x));
+                const double e3x = exp(-2 * t * h * o.x);
+                for (int i = 0; i <= P; ++i, row += R) {
+                    double r0 = 0, r1 = 0;
+#pragma unroll
+                    for (int j = 0; j <= P; ++j) {
+                        const V v = row[j];
+                        r0 = fma(wy[j], v.x, r0);
+                        r1 = fma(wy[j], v.y, r1);
+                    }
+                    const double gx = ex * e1s[i];
+                    s0 = fma(gx, r0, s0);
+                    s1 = fma(gx, r1, s1);
+                    ex *= e3x;
+                }
             }
             double v0 = s0 * scale + z0, v1 = s1 * scale + z1;
             if (d.nc == 1) {  // one real field: the real part of W
@@ -529,12 +651,13 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
     return SE2P_OK;
 }
 
-int launch_spread(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs, int njobs)
+template <typename S>
+static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs,
+                         int njobs)
 {
-    if (njobs <= 0) return SE2P_OK;
     switch (P) {
 #define SE2P_SPREAD_P(PP)                                                                         \
-    case PP: hipLaunchKernelGGL(k_spread<PP>, dim3(njobs), dim3(64), 0, s, sys, jobs, c); break;
+    case PP: hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c); break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
         SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
         SE2P_SPREAD_P(22) SE2P_SPREAD_P(24) SE2P_SPREAD_P(26) SE2P_SPREAD_P(28) SE2P_SPREAD_P(30)
@@ -542,17 +665,26 @@ int launch_spread(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jo
 #undef SE2P_SPREAD_P
     default: return set_error(SE2P_ERR_UNSUPPORTED, "spread: P = %d", P);
     }
+    return SE2P_OK;
+}
+
+int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
+                  int njobs)
+{
+    if (njobs <= 0) return SE2P_OK;
+    SE2P_TRY(prec ? spread_launch<float>(s, P, c, sys, jobs, njobs)
+                  : spread_launch<double>(s, P, c, sys, jobs, njobs));
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_spread_records(hipStream_t s, const KGeom& g, const int* perm, const int2* node,
-                          const double2* off, int n, int2* snode, double4* fac)
+int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* perm,
+                          const int2* node, const double2* off, int n, int2* snode, double4* fac)
 {
     if (n <= 0) return SE2P_OK;
     const double t = -2 * g.xi * g.xi / g.eta;
     hipLaunchKernelGGL(k_spread_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
-                       n, t, g.w, g.h, snode, fac);
+                       n, t, g.w, g.h, prec, snode, fac);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
@@ -566,11 +698,11 @@ int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int n
     return SE2P_OK;
 }
 
-int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs)
+template <typename S>
+static void filter_launch(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs)
 {
-    if (njobs <= 0) return SE2P_OK;
 #define SE2P_LAUNCH_FILTER(K) \
-    hipLaunchKernelGGL(k_filter<K>, dim3(njobs), dim3(256), 0, s, sys, jobs)
+    hipLaunchKernelGGL((k_filter<K, S>), dim3(njobs), dim3(256), 0, s, sys, jobs)
     switch (kind) {
     case SE2P_SLP: SE2P_LAUNCH_FILTER(SE2P_SLP); break;
     case SE2P_DLP: SE2P_LAUNCH_FILTER(SE2P_DLP); break;
@@ -586,19 +718,28 @@ int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, 
     default: SE2P_LAUNCH_FILTER(SE2P_DLP_STRESS); break;
     }
 #undef SE2P_LAUNCH_FILTER
+}
+
+int launch_filter(hipStream_t s, int prec, int kind, const SysDesc* sys, const Job* jobs,
+                  int njobs)
+{
+    if (njobs <= 0) return SE2P_OK;
+    if (prec)
+        filter_launch<float>(s, kind, sys, jobs, njobs);
+    else
+        filter_launch<double>(s, kind, sys, jobs, njobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }
 
-int launch_gather(hipStream_t s, int maxR, int P, int g, const SysDesc* sys, const Job* jobs,
-                  int njobs, double* u, int accumulate, int use_zm)
+template <typename S>
+static int gather_launch(hipStream_t s, size_t lds, int P, int g, const SysDesc* sys,
+                         const Job* jobs, int njobs, double* u, int accumulate, int use_zm)
 {
-    if (njobs <= 0) return SE2P_OK;
-    const size_t lds = sizeof(double2) * (size_t)maxR * maxR;
     switch (P) {
 #define SE2P_GATHER_P(PP)                                                                         \
     case PP:                                                                                      \
-        hipLaunchKernelGGL(k_gather<PP>, dim3(njobs), dim3(256), lds, s, sys, jobs, u, g,        \
+        hipLaunchKernelGGL((k_gather<PP, S>), dim3(njobs), dim3(256), lds, s, sys, jobs, u, g,   \
                            accumulate, use_zm);                                                   \
         break;
         SE2P_GATHER_P(2) SE2P_GATHER_P(4) SE2P_GATHER_P(6) SE2P_GATHER_P(8) SE2P_GATHER_P(10)
@@ -609,6 +750,16 @@ int launch_gather(hipStream_t s, int maxR, int P, int g, const SysDesc* sys, con
     default: return set_error(SE2P_ERR_UNSUPPORTED, "gather: P = %d", P);
     }
     static_assert(kMaxP == 40, "gather instantiations cover every even P up to kMaxP");
+    return SE2P_OK;
+}
+
+int launch_gather(hipStream_t s, int prec, int maxR, int P, int g, const SysDesc* sys,
+                  const Job* jobs, int njobs, double* u, int accumulate, int use_zm)
+{
+    if (njobs <= 0) return SE2P_OK;
+    const size_t lds = (prec ? sizeof(float2) : sizeof(double2)) * (size_t)maxR * maxR;
+    SE2P_TRY(prec ? gather_launch<float>(s, lds, P, g, sys, jobs, njobs, u, accumulate, use_zm)
+                  : gather_launch<double>(s, lds, P, g, sys, jobs, njobs, u, accumulate, use_zm));
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -45,7 +45,9 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
     G->prm = prm;
     G->has_real = want_real;
     G->has_k = want_k;
-    G->e1tier = prm.e1_tier == 1 ? kE1Fast : kE1Full;
+    // the FP32 tier covers the velocity kinds; the derived kinds stay FP64
+    G->prec = prm.precision == SE2P_FP32C && (kind == SE2P_SLP || kind == SE2P_DLP) ? 1 : 0;
+    G->e1tier = G->prec ? kF32 : prm.e1_tier == 1 ? kE1Fast : kE1Full;
     void* p;
     if (dlp_family(kind) && Ns > 0) {
         // normals are geometry: keep a copy in input order for the spreading passes
@@ -194,8 +196,8 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)skey, (int*)scount, G->sp_node, G->sp_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, ntiles, G->sp_perm,
                              G->sp_start, WS_SORT_TMP));
-        SE2P_TRY(launch_spread_records(s, g, G->sp_perm, G->sp_node, G->sp_off, Ns, G->sp_snode,
-                                       G->sp_fac));
+        SE2P_TRY(launch_spread_records(s, g, G->prec, G->sp_perm, G->sp_node, G->sp_off, Ns,
+                                       G->sp_snode, G->sp_fac));
         SE2P_TRY(launch_fill_int(s, (int*)tcount, gtiles, 0));
         SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.gTX, g.gTY, g.gntx,
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
@@ -213,6 +215,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
     d->Ns = G.Ns;
     d->Nt = G.Nt;
     d->e1tier = G.e1tier;
+    d->prec = G.prec;
     d->f_off = f_off;
     d->u_off = u_off;
     d->Lx = G.Lx;
@@ -309,10 +312,10 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         B->any_dlp |= dlp_family(G.kind);
         B->any_g1 |= G.has_k && out_grids(G.kind) == 2;
         if (G.has_k) {
-            size_t& ly = B->lds_y[fft_line_class(G.kg.My)];
-            size_t& lx = B->lds_x[fft_line_class(G.kg.Mx)];
-            ly = std::max(ly, fft_lds_bytes(G.kg.My));
-            lx = std::max(lx, fft_lds_bytes(G.kg.Mx));
+            size_t& ly = B->lds_y[G.prec][fft_line_class(G.kg.My)];
+            size_t& lx = B->lds_x[G.prec][fft_line_class(G.kg.Mx)];
+            ly = std::max(ly, fft_lds_bytes(G.kg.My, G.prec));
+            lx = std::max(lx, fft_lds_bytes(G.kg.Mx, G.prec));
             B->maxR = std::max(B->maxR, G.kg.gR);
         }
     }
@@ -330,11 +333,12 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             if (geoms[i]->has_real && rec_layout(d.kind) == layout) blocks(i, (d.Ns + 255) / 256);
         });
     for (int kind = 0; kind < kNumKinds; ++kind) {
-        add(B->filter[kind], [&](int i, const SysDesc& d) {
-            if (geoms[i]->has_k && d.kind == kind)
-                blocks(i, ((long)d.Mx * (d.My / 2 + 1) + 255) / 256);
-        });
-        for (int tier = 0; tier < 2; ++tier)
+        for (int prec = 0; prec < 2; ++prec)
+            add(B->filter[prec][kind], [&](int i, const SysDesc& d) {
+                if (geoms[i]->has_k && d.kind == kind && d.prec == prec)
+                    blocks(i, ((long)d.Mx * (d.My / 2 + 1) + 255) / 256);
+            });
+        for (int tier = 0; tier < 3; ++tier)
             for (int mode = 0; mode < kRealModes; ++mode)
                 add(B->real[kind][tier][mode], [&](int i, const SysDesc& d) {
                     const int m = d.sym ? RM_SYM : d.bs == 1 ? RM_ONE_CELL : RM_GENERAL;
@@ -353,46 +357,52 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     add(B->sdens, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_k) blocks(i, (d.Ns + 255) / 256);
     });
-    for (int c = 0; c < 2; ++c) {
-        auto want = [&](int i, const SysDesc& d) {
-            return geoms[i]->has_k && (c == 0 || dlp_family(d.kind));
-        };
-        for (int q = 0; q < kNumP; ++q)
-            add(B->spread[c][q], [&](int i, const SysDesc& d) {
-                if (want(i, d) && d.P == 2 * (q + 1)) blocks(i, (long)d.ntx * d.nty);
-            });
-        for (int cls = 0; cls < kFftClasses; ++cls) {
-            add(B->ffty[c][cls], [&](int i, const SysDesc& d) {
-                if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
-            });
-            add(B->fftx[c][cls], [&](int i, const SysDesc& d) {
-                if (want(i, d) && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
-            });
-        }
-        add(B->tr_fwd[c], [&](int i, const SysDesc& d) {
-            if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
-        });
-    }
-    for (int g = 0; g < 2; ++g) {
-        auto want = [&](int i, const SysDesc& d) {
-            return geoms[i]->has_k && (g == 0 || out_grids(d.kind) == 2);
-        };
-        for (int cls = 0; cls < kFftClasses; ++cls) {
-            add(B->ifftx[g][cls], [&](int i, const SysDesc& d) {
-                if (want(i, d) && fft_line_class(d.Mx) == cls) blocks(i, (d.My + d.Bx - 1) / d.Bx);
-            });
-            add(B->iffty[g][cls], [&](int i, const SysDesc& d) {
-                if (want(i, d) && fft_line_class(d.My) == cls) blocks(i, (d.Mx + d.By - 1) / d.By);
+    for (int prec = 0; prec < 2; ++prec) {
+        for (int c = 0; c < 2; ++c) {
+            auto want = [&](int i, const SysDesc& d) {
+                return geoms[i]->has_k && d.prec == prec && (c == 0 || dlp_family(d.kind));
+            };
+            for (int q = 0; q < kNumP; ++q)
+                add(B->spread[prec][c][q], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && d.P == 2 * (q + 1)) blocks(i, (long)d.ntx * d.nty);
+                });
+            for (int cls = 0; cls < kFftClasses; ++cls) {
+                add(B->ffty[prec][c][cls], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && fft_line_class(d.My) == cls)
+                        blocks(i, (d.Mx + d.By - 1) / d.By);
+                });
+                add(B->fftx[prec][c][cls], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && fft_line_class(d.Mx) == cls)
+                        blocks(i, (d.My + d.Bx - 1) / d.Bx);
+                });
+            }
+            add(B->tr_fwd[prec][c], [&](int i, const SysDesc& d) {
+                if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
             });
         }
-        add(B->tr_inv[g], [&](int i, const SysDesc& d) {
-            if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
-        });
-        for (int q = 0; q < kNumP; ++q)
-            add(B->gather[g][q], [&](int i, const SysDesc& d) {
-                if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1))
-                    blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
+        for (int g = 0; g < 2; ++g) {
+            auto want = [&](int i, const SysDesc& d) {
+                return geoms[i]->has_k && d.prec == prec && (g == 0 || out_grids(d.kind) == 2);
+            };
+            for (int cls = 0; cls < kFftClasses; ++cls) {
+                add(B->ifftx[prec][g][cls], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && fft_line_class(d.Mx) == cls)
+                        blocks(i, (d.My + d.Bx - 1) / d.Bx);
+                });
+                add(B->iffty[prec][g][cls], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && fft_line_class(d.My) == cls)
+                        blocks(i, (d.Mx + d.By - 1) / d.By);
+                });
+            }
+            add(B->tr_inv[prec][g], [&](int i, const SysDesc& d) {
+                if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
             });
+            for (int q = 0; q < kNumP; ++q)
+                add(B->gather[prec][g][q], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1))
+                        blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
+                });
+        }
     }
     if (B->has_k) SE2P_TRY(fft_prepare());
     void *psys, *pjobs;
@@ -426,7 +436,7 @@ int batch_apply_real(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doubl
     }
     StageTimer st(ctx, "real");
     for (int kind = 0; kind < kNumKinds; ++kind)
-        for (int tier = 0; tier < 2; ++tier)
+        for (int tier = 0; tier < 3; ++tier)
             for (int mode = 0; mode < kRealModes; ++mode) {
                 const JobRange& r = B.real[kind][tier][mode];
                 SE2P_TRY(launch_real(s, kind, tier, mode, B.sys, J + r.first, r.count, d_u));
@@ -441,18 +451,22 @@ int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const do
     // the densities of every pair are packed with the first
     if (c == 0)
         SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
-    for (int q = 0; q < kNumP; ++q)
-        SE2P_TRY(launch_spread(s, 2 * (q + 1), c, B.sys, B.jobs + B.spread[c][q].first,
-                               B.spread[c][q].count));
+    for (int prec = 0; prec < 2; ++prec)
+        for (int q = 0; q < kNumP; ++q)
+            SE2P_TRY(launch_spread(s, prec, 2 * (q + 1), c, B.sys,
+                                   B.jobs + B.spread[prec][c][q].first,
+                                   B.spread[prec][c][q].count));
     return SE2P_OK;
 }
 
 int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
 {
     StageTimer st(ctx, "gather");
-    for (int q = 0; q < kNumP; ++q)
-        SE2P_TRY(launch_gather(ctx->stream, B.maxR, 2 * (q + 1), 0, B.sys,
-                               B.jobs + B.gather[0][q].first, B.gather[0][q].count, d_u, 0, 0));
+    for (int prec = 0; prec < 2; ++prec)
+        for (int q = 0; q < kNumP; ++q)
+            SE2P_TRY(launch_gather(ctx->stream, prec, B.maxR, 2 * (q + 1), 0, B.sys,
+                                   B.jobs + B.gather[prec][0][q].first,
+                                   B.gather[prec][0][q].count, d_u, 0, 0));
     return SE2P_OK;
 }
 
@@ -465,33 +479,40 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
     //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
+    // Systems of the FP32 tier run the same passes on float2 grids.
+    auto fft = [&](int prec, int axis, int sel, int inv, const JobRange* r, const size_t* lds) {
+        for (int cls = 0; cls < kFftClasses; ++cls)
+            SE2P_TRY(launch_fft_lines(s, prec, axis, sel, inv, cls, lds[cls], B.sys, J + r[cls].first,
+                                      r[cls].count));
+        return SE2P_OK;
+    };
     for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
         SE2P_TRY(batch_spread_grid(ctx, s, B, d_f, c));
         StageTimer st(ctx, s, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
-        for (int cls = 0; cls < kFftClasses; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 0, Zc, 0, cls, B.lds_y[cls], B.sys,
-                                      J + B.ffty[c][cls].first, B.ffty[c][cls].count));
-        SE2P_TRY(launch_transpose(s, Zc, Tc, 0, B.sys, J + B.tr_fwd[c].first, B.tr_fwd[c].count));
-        for (int cls = 0; cls < kFftClasses; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 1, Tc, 0, cls, B.lds_x[cls], B.sys,
-                                      J + B.fftx[c][cls].first, B.fftx[c][cls].count));
+        for (int prec = 0; prec < 2; ++prec) {
+            SE2P_TRY(fft(prec, 0, Zc, 0, B.ffty[prec][c], B.lds_y[prec]));
+            SE2P_TRY(launch_transpose(s, prec, Zc, Tc, 0, B.sys, J + B.tr_fwd[prec][c].first,
+                                      B.tr_fwd[prec][c].count));
+            SE2P_TRY(fft(prec, 1, Tc, 0, B.fftx[prec][c], B.lds_x[prec]));
+        }
     }
     {
         StageTimer st(ctx, s, "filter");
-        for (int kind = 0; kind < kNumKinds; ++kind)
-            SE2P_TRY(launch_filter(s, kind, B.sys, J + B.filter[kind].first, B.filter[kind].count));
+        for (int prec = 0; prec < 2; ++prec)
+            for (int kind = 0; kind < kNumKinds; ++kind)
+                SE2P_TRY(launch_filter(s, prec, kind, B.sys, J + B.filter[prec][kind].first,
+                                       B.filter[prec][kind].count));
     }
     for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g) {
         StageTimer st(ctx, s, "fft");
         const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
-        for (int cls = 0; cls < kFftClasses; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 1, Zg, 1, cls, B.lds_x[cls], B.sys,
-                                      J + B.ifftx[g][cls].first, B.ifftx[g][cls].count));
-        SE2P_TRY(launch_transpose(s, Zg, Tg, 1, B.sys, J + B.tr_inv[g].first, B.tr_inv[g].count));
-        for (int cls = 0; cls < kFftClasses; ++cls)
-            SE2P_TRY(launch_fft_lines(s, 0, Tg, 1, cls, B.lds_y[cls], B.sys,
-                                      J + B.iffty[g][cls].first, B.iffty[g][cls].count));
+        for (int prec = 0; prec < 2; ++prec) {
+            SE2P_TRY(fft(prec, 1, Zg, 1, B.ifftx[prec][g], B.lds_x[prec]));
+            SE2P_TRY(launch_transpose(s, prec, Zg, Tg, 1, B.sys, J + B.tr_inv[prec][g].first,
+                                      B.tr_inv[prec][g].count));
+            SE2P_TRY(fft(prec, 0, Tg, 1, B.iffty[prec][g], B.lds_y[prec]));
+        }
     }
     return SE2P_OK;
 }
@@ -499,11 +520,13 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
 int batch_k_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u, bool accumulate, bool use_zm)
 {
     StageTimer st(ctx, "gather");
-    for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g)
-        for (int q = 0; q < kNumP; ++q)
-            SE2P_TRY(launch_gather(ctx->stream, B.maxR, 2 * (q + 1), g, B.sys,
-                                   B.jobs + B.gather[g][q].first, B.gather[g][q].count, d_u,
-                                   accumulate ? 1 : 0, use_zm && g == 0 ? 1 : 0));
+    for (int prec = 0; prec < 2; ++prec)
+        for (int g = 0; g < (B.any_g1 ? 2 : 1); ++g)
+            for (int q = 0; q < kNumP; ++q)
+                SE2P_TRY(launch_gather(ctx->stream, prec, B.maxR, 2 * (q + 1), g, B.sys,
+                                       B.jobs + B.gather[prec][g][q].first,
+                                       B.gather[prec][g][q].count, d_u, accumulate ? 1 : 0,
+                                       use_zm && g == 0 ? 1 : 0));
     return SE2P_OK;
 }
 

@@ -130,6 +130,8 @@ struct se2p_ctx {
     std::vector<void*> user_allocs;
     // device address of this GPU's scratch array (allocation-free evaluation)
     char* scratch = nullptr;
+    // precision of the one-shot entry points (mex equivalents and drivers)
+    int precision = SE2P_FP64;
 };
 
 namespace se2p {
@@ -224,6 +226,7 @@ enum GridSel { G_Z0 = 0, G_Z1 = 1, G_W = 2, G_D = 3 };
 
 struct SysDesc {
     int kind, Ns, Nt, e1tier, nc;  // nc: values per target
+    int prec;  // 1: FP32 tier (float2 grids, FP32 far pairs), 0: FP64
     long f_off, u_off;  // offsets (doubles) of this system in the apply's f and u
     double Lx, Ly;
     // real space
@@ -287,9 +290,10 @@ int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
 // or Z1 (c = 1); job b = grid block (b % ntx, b / ntx) of kSB x kSB nodes; all
 // jobs' systems of support P.
-int launch_spread(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs, int njobs);
+int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
+                  int njobs);
 // Spreading records of a geometry: start node and FGG factors of the k-th binned point.
-int launch_spread_records(hipStream_t s, const KGeom& g, const int* perm, const int2* node,
+int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* perm, const int2* node,
                           const double2* off, int n, int2* snode, double4* fac);
 // Densities of the binned points of every spreading pair (job (s, b): points
 // b*256 .. b*256+255 of system s).
@@ -298,11 +302,12 @@ int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int n
 // Filter (job b: pair modes b*256 .. of rows 0 .. My/2, each with its partner):
 // W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
 // Z0 (and Z1 for kinds with 4 values per target).
-int launch_filter(hipStream_t s, int kind, const SysDesc* sys, const Job* jobs, int njobs);
+int launch_filter(hipStream_t s, int prec, int kind, const SysDesc* sys, const Job* jobs,
+                  int njobs);
 // Gather of output grid g (W for g = 0, D for g = 1) into values 2g, 2g + 1 of
 // each target (+= when accumulate; + the zero mode when use_zm); maxR: the
 // largest gathering region side of the jobs' systems, all of support P.
-int launch_gather(hipStream_t s, int maxR, int P, int g, const SysDesc* sys, const Job* jobs,
+int launch_gather(hipStream_t s, int prec, int maxR, int P, int g, const SysDesc* sys, const Job* jobs,
                   int njobs, double* u, int accumulate, int use_zm);
 // Zero modes from the sorted records (DLP: sum_s (f.n)_s x_s / (Lx Ly); SLP
 // pressure: -sum_s f_s.x_s / (2 Lx Ly); DLP pressure: -sum_s (f.n)_s / (2 Lx Ly);
@@ -324,12 +329,12 @@ int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* j
 // class `cls` (fft_line_class); max_lds from fft_lds_bytes.
 constexpr int kFftClasses = 3;
 int fft_line_class(int n);
-int launch_fft_lines(hipStream_t s, int axis, int sel, int inverse, int cls, size_t max_lds,
+int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs);
-size_t fft_lds_bytes(int n);
+size_t fft_lds_bytes(int n, int prec);
 // Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
 // and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
-int launch_transpose(hipStream_t s, int in, int out, int dir, const SysDesc* sys,
+int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
                      const Job* jobs, int njobs);
 int transpose_tiles(int rows, int cols);
 
@@ -342,6 +347,7 @@ struct Geom {
     se2p_params prm{};
     bool has_real = false, has_k = false;
     int e1tier = 0;
+    int prec = 0;  // precision in use: SE2P_FP32C only for the velocity kinds
     double* nrm = nullptr;  // DLP normals, input order
     // real space
     int nbox = 0;
@@ -384,14 +390,15 @@ struct BatchExec {
     JobRange fill[2];     // by record layout
     JobRange zm_part, zm_sys;
     JobRange sdens;       // spreading densities: blocks of 256 binned points
-    JobRange real[kNumKinds][2][kRealModes];  // [kind][tier][RealMode]
-    JobRange spread[2][kNumP];  // [c][P / 2 - 1]
-    JobRange ffty[2][kFftClasses], fftx[2][kFftClasses], tr_fwd[2];  // [c][line class]
-    JobRange filter[kNumKinds];  // by kind
+    JobRange real[kNumKinds][3][kRealModes];  // [kind][tier: full, fast, FP32][RealMode]
+    // k-space passes, per grid precision (0 FP64, 1 FP32: float2 grids)
+    JobRange spread[2][2][kNumP];  // [prec][c][P / 2 - 1]
+    JobRange ffty[2][2][kFftClasses], fftx[2][2][kFftClasses], tr_fwd[2][2];  // [prec][c][class]
+    JobRange filter[2][kNumKinds];  // [prec][kind]
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
-    JobRange ifftx[2][kFftClasses], iffty[2][kFftClasses], tr_inv[2];  // [g][line class]
-    JobRange gather[2][kNumP];  // [g][P / 2 - 1]
-    size_t lds_y[kFftClasses] = {}, lds_x[kFftClasses] = {};  // [line class]
+    JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses], tr_inv[2][2];  // [prec][g][class]
+    JobRange gather[2][2][kNumP];  // [prec][g][P / 2 - 1]
+    size_t lds_y[2][kFftClasses] = {}, lds_x[2][kFftClasses] = {};  // [prec][line class]
     int maxR = 32;   // largest gathering region side
 };
 // Builds descriptors (f and u offsets: systems back to back, 2 Ns and 2 Nt

@@ -76,11 +76,11 @@ def eta_w(xi, P, L, M):
     return (2 * xi * w / m) ** 2, w
 
 
-def make_params(xi, nside_x, nside_y, Mx, My, P, Lx, e1_tier=0, symmetric=0):
+def make_params(xi, nside_x, nside_y, Mx, My, P, Lx, e1_tier=0, symmetric=0, precision=0):
     eta, w = eta_w(xi, P, Lx, Mx)
     return {"xi": xi, "rc": Lx / nside_x, "w": w, "eta": eta, "nside_x": nside_x,
             "nside_y": nside_y, "Mx": Mx, "My": My, "P": P, "kinf": Mx // 2, "converged": 1,
-            "e1_tier": e1_tier, "symmetric": symmetric}
+            "e1_tier": e1_tier, "symmetric": symmetric, "precision": precision}
 
 
 def rel_err(u, ref):

@@ -48,8 +48,16 @@ extern "C" {
 #define SE2P_DLP_STRESS 11       /* stresslet stress tensor (4 per target) */
 #define SE2P_NUM_KINDS 12
 
-/* Arithmetic of a plan (only FP64 in this build). */
-#define SE2P_FP64 0
+/* Arithmetic of an evaluation (se2p_params.precision). */
+#define SE2P_FP64 0 /* every stage in FP64 */
+/* FP32 tier, for tolerances down to 1e-6 (Stokeslet and stresslet velocity; the
+ * derived kinds run FP64 whatever is asked): spreading on the FP32 matrix
+ * cores with two-sum accumulation per chunk, float2 grids through the FFT and
+ * filter (each mode filtered in FP64), gathering in FP32 rows added in FP64,
+ * real-space far pairs (xi^2 r^2 >= 1) in FP32 added per chunk to FP64 sums;
+ * near pairs (the E1 series near r = 0), self terms, zero modes and every
+ * output sum stay FP64. */
+#define SE2P_FP32C 1
 
 typedef struct se2p_ctx se2p_ctx;
 typedef struct se2p_plan se2p_plan;
@@ -65,6 +73,11 @@ const char* se2p_last_error(void);
 int se2p_version(void);
 /* Number of visible GPUs (0 when none). */
 int se2p_device_count(int* count);
+/* Precision (SE2P_FP64, the default, or SE2P_FP32C) of this context's one-shot
+ * entry points: the mex equivalents and the drivers.  Plans, batches and
+ * se2p_eval_device take theirs from se2p_params.precision.  No counterpart in
+ * the reference, which computes in FP64 only. */
+int se2p_ctx_set_precision(se2p_ctx* ctx, int precision);
 
 /* ------------------------------------------------------------------------ */
 /* The four mex kernels, same argument order, sizes made explicit.          */
@@ -118,6 +131,7 @@ typedef struct {
      * their last bits, vary from run to run; 0 (the reference's order) sums each
      * target's pairs in a fixed order.  Ignored for other kinds and inputs. */
     int symmetric;
+    int precision; /* SE2P_FP64 or SE2P_FP32C */
 } se2p_params;
 
 /* ------------------------------------------------------------------------ */
