@@ -283,8 +283,9 @@ def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
     out = gen(nsys, seed=seed, nmin=nmin, nmax=nmax)
     for y in out:
         n = len(y["psrc"])
-        if recipe == "rule":
-            y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1)
+        if recipe == "rule":  # FP32 or FP64 per system by its tolerance
+            y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1,
+                                   precision=None)
         else:
             y["prm"] = capped_params(n, y["Lx"], y["Ly"], y["tol"])
             y["prm"]["e1_tier"] = 1
@@ -354,7 +355,9 @@ def batch_bench(args):
         "value": len(systems) * args.steps / dt, "unit": "system-matvecs/s",
         "points_per_s": ntot * args.steps / dt, "ms_per_batch": dt / args.steps * 1e3,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
-        "dtype": "f64", "data": "synthetic: uniform points, N(0,1) densities, seed 11",
+        "dtype": "f64+f32" if any(y["prm"]["precision"] for y in systems) else "f64",
+        "data": "synthetic: uniform points, N(0,1) densities, seed 11",
+        "fp32_systems": int(sum(y["prm"]["precision"] for y in systems)),
         "config": {"workload": f"{len(systems)} independent periodic SLP systems, aspect 1-4, "
                                "N log-uniform 500-200000, tol log-uniform 1e-10..1e-4",
                    "points": ntot, "recipe": args.recipe},
@@ -365,6 +368,90 @@ def batch_bench(args):
         a.free()
     b.close()
     print(json.dumps(out), flush=True)
+
+
+def clustered_bench(args):
+    """Configuration 3: N = 4e6 sources = targets from 64 Gaussian clusters
+    (sigma 0.01, wrapped) in the unit box, a Stokeslet sum with density f and a
+    stresslet sum with density g and unit normals n, each autotuned (grid at
+    most 2048 per side) at --tol, FP32 tier for tol >= 1e-6 and FP64 below.
+    One step = both matvecs (plans, stream launches) on resident inputs; the
+    errors are measured against the CPU oracle (tol 1e-13, all sources) on 1e4
+    targets."""
+    import fasttools2d_amd as ft
+    from fasttools2d_amd.plan import DeviceArray, Plan
+    from fasttools2d_amd.tune import autotune, rel_err
+    from fasttools2d_amd.workloads import clusters, normal_density, unit_normals
+    N, tol = args.cluster_n, args.tol
+    p = clusters(N, 1.0, 1.0, seed=3)
+    f, g, n = normal_density(N, seed=4), normal_density(N, seed=5), unit_normals(N, seed=6)
+    ctx = ft.Context(0)
+    plans, parts = [], {}
+    for kind, dens, nrm in (("slp", f, None), ("dlp", g, n)):
+        prm, rep = autotune(ctx, kind, p, p, dens, 1.0, 1.0, tol, nrm=nrm, reps=3,
+                            final_rounds=3, max_grid=2048, nside_scales=(1.0, 1.4), log=log)
+        pl = Plan(ctx, kind, p, p, 1.0, 1.0, prm, nrm=nrm)
+        df = DeviceArray.from_host(ctx, dens)
+        du = DeviceArray(ctx, (N, 2))
+        plans.append((pl, df, du))
+        parts[kind] = {"params": {k: prm[k] for k in ("xi", "nside_x", "Mx", "P", "precision")},
+                       "tune_ms": rep["ms"], "err_vs_gpu_reference": rep["err"],
+                       "symmetric_in_use": pl.params_in_use()["symmetric"]}
+        log(f"bench-clustered: {kind} {parts[kind]}")
+
+    def step():
+        for pl, df, du in plans:
+            pl.apply_device(df.ptr, du.ptr)
+
+    ms = time_steps(ctx, step, args.steps, args.warmup) * 1e3
+    stages = stage_profile(ctx, step, max(2, args.steps // 4), serial=True)
+    out = {"metric": "clustered periodic Stokes SLP+DLP evals/s, N=4M, 64 Gaussian clusters",
+           "value": 1e3 / ms, "unit": "evals/s (one eval = SLP + DLP matvec)", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32" if tol >= 1e-6 else "f64",
+           "data": "synthetic: 64 Gaussian clusters (sigma 0.01), N(0,1) densities, seed 3-6",
+           "config": {"workload": f"SLP+DLP, N={N} sources=targets, unit box, tol {tol}"},
+           "parts": parts, "stage_ms": stages}
+    if not args.no_cpu:
+        sub = np.sort(np.random.default_rng(7).choice(N, size=10_000, replace=False))
+        from oracle import oracle as O
+        errs = {}
+        tot_u, tot_r = 0.0, 0.0
+        for (kind, dens, nrm), (pl, df, du) in zip((("slp", f, None), ("dlp", g, n)), plans):
+            pl.apply_device(df.ptr, du.ptr)
+            u = du.download()[sub]
+            ref = clustered_oracle(O, kind, p, sub, dens, nrm)
+            errs[kind] = rel_err(u, ref)
+            tot_u, tot_r = tot_u + u, tot_r + ref
+        errs["combined"] = rel_err(tot_u, tot_r)
+        out["rel_err"] = errs["combined"]
+        out["oracle_errors"] = errs
+        log(f"bench-clustered: errors vs the CPU oracle {errs}")
+    for pl, df, du in plans:
+        pl.close()
+        df.free()
+        du.free()
+    print(json.dumps(out), flush=True)
+
+
+def clustered_oracle(O, kind, p, sub, dens, nrm, tol=1e-13):
+    """The checker for the clustered workload: the CPU oracle's spectral Ewald sum
+    at tol (all sources) at the targets p[sub], grid raised to a 2-3-5-smooth size."""
+    from fasttools2d_amd.tune import smooth_up
+    n = len(p)
+    prm = O.params(kind, 2 * n, O.density_q(dens), 1.0, 1.0, 24, O.default_Nb(2 * n), tol)
+    M, P = smooth_up(prm["Mx"], 8192), 24
+    w = P / M / 2
+    eta = (2 * prm["xi"] * w / (0.95 * math.sqrt(math.pi * P))) ** 2
+    pt = p[sub]
+    xi, ns = prm["xi"], prm["nside_x"]
+    if kind == "slp":
+        return O.slp_real(p, pt, dens, xi, ns, ns, 1.0, 1.0) + \
+            O.slp_kspace(p, pt, xi, eta, dens, M, M, 1.0, 1.0, w, P)
+    return O.dlp_real(p, pt, dens, nrm, xi, ns, ns, 1.0, 1.0) + \
+        O.dlp_kspace(p, pt, xi, eta, dens, nrm, M, M, 1.0, 1.0, w, P) + \
+        O.dlp_zero_mode(p, dens, nrm, 1.0, 1.0)
 
 
 def free_bench(args):
@@ -430,6 +517,9 @@ def main():
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
                     help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
+    ap.add_argument("--clustered", action="store_true",
+                    help="run the clustered SLP+DLP workload (N=4M, 64 clusters) at --tol instead")
+    ap.add_argument("--cluster-n", type=int, default=4_000_000)
     ap.add_argument("--free", action="store_true",
                     help="run the free-space (FMM replacement) benchmark instead")
     ap.add_argument("--free-n", type=int, default=100_000)
@@ -442,6 +532,9 @@ def main():
         return
     if args.free:
         free_bench(args)
+        return
+    if args.clustered:
+        clustered_bench(args)
         return
 
     import fasttools2d_amd as ft

@@ -83,6 +83,15 @@ def make_params(xi, nside_x, nside_y, Mx, My, P, Lx, e1_tier=0, symmetric=0, pre
             "e1_tier": e1_tier, "symmetric": symmetric, "precision": precision}
 
 
+FP32_MIN_TOL = 1e-6
+
+
+def precision_for(tol):
+    """The arithmetic a tolerance takes: the FP32 tier (SE2P_FP32C = 1) down to
+    1e-6, FP64 (0) below; the FP32 tier's own rounding is near 1e-7."""
+    return 1 if tol >= FP32_MIN_TOL else 0
+
+
 def rel_err(u, ref):
     return float(np.linalg.norm(u - ref) / np.linalg.norm(ref))
 
@@ -179,7 +188,7 @@ def fft_alternatives(m, max_n=4096, slack=1.12, low=0.9):
 
 def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
              nside_scales=(0.75, 1.0, 1.4, 1.9, 2.6, 3.4, 4.0, 4.7), reps=8, max_grid=4096,
-             heldout_seed=12345, final_rounds=5):
+             heldout_seed=12345, final_rounds=5, precision=None):
     """Returns (params, report).  ps, pt, f, nrm are (N, 2) host arrays.
 
     max_grid caps the grid per side.  After the sweep the fastest few accepted
@@ -190,13 +199,16 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
     is the out-of-sample error a new density gets.  A candidate whose held-out
     error exceeds tol is passed over for the next one.  report records the
     sweep, both errors, whether pair-symmetric sums were in use, and the re-timed
-    medians."""
+    medians.  precision: 0 FP64, 1 the FP32 tier, None by the tolerance
+    (precision_for); the calibration runs in FP64 (it isolates truncation
+    errors), the timed configurations in the chosen precision."""
     log = log or (lambda s: None)
     A, B = _aspect(Lx, Ly)
     Ns, Nt = len(ps), len(pt)
     sym = 1 if _lib.kind_code(kind) == _lib.SE2P_SLP and pt is ps else 0
     T = Tuner(ctx, kind, ps, pt, f, Lx, Ly, nrm=nrm, log=log)
-    report = {"tol": tol, "max_grid": max_grid}
+    prec = precision_for(tol) if precision is None else int(precision)
+    report = {"tol": tol, "max_grid": max_grid, "precision": prec}
     from .periodic_ewald import density_q, ewald_params
 
     def reference_params(dens):
@@ -214,14 +226,14 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         # exact-enough k space for calibrating the real space, and vice versa
         c_exact_r, c_exact_k, P_exact = 6.0, 2 * 6.0, 24
 
-        def prm_for(a, c_r, c_k, P, tier=0, M1=None):
+        def prm_for(a, c_r, c_k, P, tier=0, M1=None, pr=0):
             nsx, nsy = a * A, a * B
             rc = Lx / nsx
             xi = c_r / rc
             M0 = smooth_up(c_k * xi * Lx / math.pi / A, 4096)
             if M1 is not None:
                 M0 = M1
-            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym)
+            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym, pr)
 
         def fits(prm, cap=None):
             cap = max_grid if cap is None else cap
@@ -264,7 +276,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
         # found by bisection; a support that passes at the smallest c_k ends the
         # search (larger supports cost more at the same grid).
         best, kcands, kerr = None, [], {}
-        cks = (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0)
+        cks = (4.0, 4.5, 5.0, 5.5, 6.0, 6.5, 7.0, 8.0, 9.0, 10.0)
 
         def k_error(P, ck):
             if (P, ck) not in kerr:
@@ -309,7 +321,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
                 return
             tried.add((a, P, c_k, M1, cr))
             try:
-                prm = prm_for(a, cr, c_k, P, tier, M1)
+                prm = prm_for(a, cr, c_k, P, tier, M1, prec)
             except ValueError:
                 return
             if not fits(prm):
@@ -420,7 +432,7 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
 
 
 def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, margin=10.0,
-                symmetric=0):
+                symmetric=0, precision=0):
     """Parameters without measuring: the constants the tuner calibrates, as
     functions of the tolerance, fitted to its N = 1e6, tol = 1e-8 result
     (xi rc = 3.9, k_inf/xi = 8.0, P = 16, about 22 points per cell), with
@@ -436,7 +448,8 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
     for systems too many to tune one by one (batches); its accuracy is checked
     against accurate solutions where it is used.  symmetric is passed through
     (pair-symmetric real space for Stokeslet systems whose targets are their
-    sources)."""
+    sources); precision 0 (FP64), 1 (FP32 tier) or None (by the tolerance,
+    precision_for)."""
     A, B = _aspect(Lx, Ly)
     L = math.log(margin / tol)
     c_r = math.sqrt(max(L - 3.2, 1.0))
@@ -457,7 +470,9 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
         if a == 1:
             raise ValueError(f"no grid within {max_grid} per side for tol {tol}")
         a -= 1
-    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric)
+    prec = precision_for(tol) if precision is None else int(precision)
+    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric,
+                       prec)
 
 
 def capped_params(n, Lx, Ly, tol, P=24):

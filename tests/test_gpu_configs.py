@@ -113,6 +113,24 @@ def test_c3_clustered_fp64_autotuned(gpu_ctx, oracle, c3_points, kind):
     _C3[kind] = (u[sub], ref)
 
 
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+def test_c3_clustered_fp32_autotuned(gpu_ctx, oracle, c3_points, kind):
+    """Configuration 3's FP32 run: tolerance 1e-6, the FP32 tier."""
+    p, f, g, n = c3_points
+    tol = 1e-6
+    dens, nrm = (f, None) if kind == "slp" else (g, n)
+    prm, rep = autotune(gpu_ctx, kind, p, p, dens, 1.0, 1.0, tol, nrm=nrm, reps=2,
+                        final_rounds=2, max_grid=2048, nside_scales=(1.0, 1.4),
+                        log=lambda m: print(m, flush=True))
+    assert prm["precision"] == 1 and rep["err"] <= 0.8 * tol
+    u, used = _apply(gpu_ctx, kind, p, p, 1.0, 1.0, prm, dens, nrm)
+    assert used["precision"] == 1
+    sub = np.sort(np.random.default_rng(7).choice(len(p), 10_000, replace=False))
+    err = rel(u[sub], oracle_se(oracle, kind, p, p[sub], dens, nrm, 1.0, 1.0))
+    print(f"C3 FP32 {kind}: {rep['ms']:.2f} ms, rel-L2 vs oracle on 1e4 targets {err:.3e}")
+    assert err <= tol
+
+
 def test_c3_combined(gpu_ctx):
     if set(_C3) != {"slp", "dlp"}:
         pytest.skip("needs both C3 parts from this session")
@@ -126,10 +144,15 @@ def test_c4_serving_batch_512_systems(gpu_ctx, oracle):
     systems = serving_systems(512)
     for y in systems:
         n = len(y["psrc"])
-        y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1)
+        # FP32 or FP64 per system by its tolerance
+        y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1,
+                               precision=None)
     b = Batch(gpu_ctx, systems)
     try:
         assert sum(len(y["psrc"]) for y in systems) > 1e7
+        precs = [b.params_in_use(i)["precision"] for i in range(len(systems))]
+        assert precs == [int(y["tol"] >= 1e-6) for y in systems]
+        assert 0 < sum(precs) < len(systems)
         rng = np.random.default_rng(77)
         keep = {}
         for k in range(20):
@@ -144,6 +167,9 @@ def test_c4_serving_batch_512_systems(gpu_ctx, oracle):
         idx = sorted(set(np.linspace(0, len(systems) - 1, 12).astype(int)) |
                      {int(np.argmin([len(y["psrc"]) for y in systems])),
                       int(np.argmax([len(y["psrc"]) for y in systems]))})
+        # and at least two systems of each precision
+        idx = sorted(set(idx) | set([i for i in range(len(systems)) if precs[i]][:2]) |
+                     set([i for i in range(len(systems)) if not precs[i]][:2]))
         worst = 0.0
         for i in idx:
             y = systems[i]
