@@ -13,6 +13,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -175,36 +176,51 @@ def oracle_check(p, f, sub, Lx=1.0, Ly=1.0, tol=1e-13):
                "targets": int(len(sub)), "s": time.perf_counter() - t0}
 
 
-def cpu_baseline(p, f, prm, frac=0.05):
-    """Oracle (C/OpenMP port of the reference stages) on the host cores, on a
-    bounded sample: all sources, a random fraction of the targets; spreading and
-    FFTs are done in full, target-proportional stages are scaled up."""
+def cpu_info():
+    """Host CPU model and counts (lscpu when present)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core", "CPU(s)", "Flags"):
+                info[k.strip()] = v.strip() if k.strip() != "Flags" else \
+                    " ".join(x for x in v.split() if x in ("avx2", "fma", "avx512f"))
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(p, f, prm, reps=5):
+    """refcpu (the C/OpenMP restatement of the reference stages built with -O3,
+    AVX2/FMA and the rational-fit E1: oracle/Makefile's libse2p_refcpu.so) on
+    the host cores: the plain FP64 evaluation at the reference recipe, all
+    sources and all targets, one warm-up and the median of `reps` timed runs.
+    Its error against the GPU's plain result is reported beside the time."""
     from oracle import oracle as O
+    R = O.refcpu()
     n = len(p)
-    rng = np.random.default_rng(5)
-    sub = np.sort(rng.choice(n, size=max(1, int(n * frac)), replace=False))
-    pt = p[sub]
-    t0 = time.perf_counter()
-    O.slp_real(p, pt, f, prm["xi"], prm["nside_x"], prm["nside_y"], 1.0, 1.0)
-    t_real = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    H1, H2 = O.spread(p, f, 1.0, 1.0, prm["xi"], prm["w"], prm["eta"], prm["P"], prm["Mx"],
-                      prm["My"])
-    Z1, Z2 = O.fft2(H1), O.fft2(H2)
-    G1, G2 = O.slp_filter(Z1, Z2, 1.0, 1.0, prm["xi"], prm["eta"])
-    g1, g2 = O.fft2(G1, True).real, O.fft2(G2, True).real
-    t_grid = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.gather(g1, pt, 1.0, 1.0, prm["xi"], prm["w"], prm["eta"], prm["P"])
-    O.gather(g2, pt, 1.0, 1.0, prm["xi"], prm["w"], prm["eta"], prm["P"])
-    t_gather = time.perf_counter() - t0
-    t_eval = (t_real + t_gather) * (n / len(sub)) + t_grid
+
+    def run():
+        u = O.slp_real(p, p, f, prm["xi"], prm["nside_x"], prm["nside_y"], 1.0, 1.0, L=R)
+        u += O.slp_kspace(p, p, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], 1.0, 1.0,
+                          prm["w"], prm["P"], L=R)
+        return u
+
+    u = run()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    t_eval = float(np.median(ts))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / t_eval, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"N={n} sources, {len(sub)} targets ({frac:.0%}); real space and gather "
-                      f"scaled x{n / len(sub):.0f}, spread+FFT+filter in full; "
-                      f"est. {t_eval:.1f} s per eval",
-            "stage_s": {"real_sample": t_real, "grid": t_grid, "gather_sample": t_gather}}
+            "sample": f"full evaluation, N={n} sources = targets, reference recipe (nside "
+                      f"{prm['nside_x']}, M {prm['Mx']}, P {prm['P']}); 1 warm-up, median of "
+                      f"{reps}: {t_eval:.2f} s per eval",
+            "runs_s": ts, "cpu": cpu_info()}, u
 
 
 STAGES = ("bin", "bin_grid", "pack", "real", "spread", "fft", "filter", "gather")
@@ -454,6 +470,27 @@ def clustered_oracle(O, kind, p, sub, dens, nrm, tol=1e-13):
         O.dlp_zero_mode(p, dens, nrm, 1.0, 1.0)
 
 
+def selector_bench(args):
+    """Configuration 5: the learned parameter selector on held-out random
+    configurations (uniform and clustered, N log-uniform in 1e3..1e6, tolerance
+    log-uniform in 1e-10..1e-4, Stokeslet and stresslet), against the
+    analytic recipe it falls back to and the rule of tune.rule_params.  value =
+    geometric-mean speedup of the selector's matvecs over the analytic recipe's."""
+    import fasttools2d_amd as ft
+    from fasttools2d_amd import selector as S
+    ctx = ft.Context(0)
+    rows = S.evaluate(ctx, S.load(), S.random_configs(args.nsys, 1001, nmin=1e3, nmax=1e6),
+                      log=log)
+    summ = S.summarize(rows)
+    out = {"metric": "learned selector: matvec speedup over the analytic recipe (geometric mean)",
+           "value": summ["speedup_vs_analytic_gmean"], "unit": "x", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64+f32", "data": "synthetic held-out "
+           "configurations (seed 1001), unseen in training", "config": {
+               "workload": f"{args.nsys} random periodic systems, N 1e3-1e6, tol 1e-10..1e-4"},
+           "summary": summ}
+    print(json.dumps(out), flush=True)
+
+
 def free_bench(args):
     """Free-space Stokeslet sums (the replacement of the FMM mex stokesSLPfmm):
     N sources = targets uniform in [-1, 1]^2, f = 10 U(0, 1) (FMM/tests/
@@ -517,6 +554,8 @@ def main():
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
                     help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
+    ap.add_argument("--selector", action="store_true",
+                    help="evaluate the learned parameter selector on --nsys held-out systems")
     ap.add_argument("--clustered", action="store_true",
                     help="run the clustered SLP+DLP workload (N=4M, 64 clusters) at --tol instead")
     ap.add_argument("--cluster-n", type=int, default=4_000_000)
@@ -535,6 +574,9 @@ def main():
         return
     if args.clustered:
         clustered_bench(args)
+        return
+    if args.selector:
+        selector_bench(args)
         return
 
     import fasttools2d_amd as ft
@@ -685,8 +727,11 @@ def main():
                             "two densities"
         out["oracle_errors"] = oracle
         out["plain"]["rel_err"] = oracle["plain"]
-        log("bench: CPU baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(p, f, prm)
+        log("bench: CPU baseline (refcpu) ...")
+        out["cpu_baseline"], u_cpu = cpu_baseline(p, f, prm)
+        out["cpu_baseline"]["rel_err_vs_gpu_plain"] = rel_err(u_cpu, u_plain)
+        out["gpu_vs_cpu"] = {"tuned": out["value"] / out["cpu_baseline"]["value"],
+                             "plain": out["plain"]["evals_per_s"] / out["cpu_baseline"]["value"]}
         log(f"bench: cpu {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

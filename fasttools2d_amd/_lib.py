@@ -81,7 +81,9 @@ EXPORTS = (
     "se2p_dlp_pressure_real", "se2p_dlp_pressure_kspace", "se2p_dlp_pressure_ewald",
 ) + tuple(f"se2p_{layer}_{q}_{part}" for layer in ("slp", "dlp")
           for q in ("gradient", "pressure_grad", "vorticity", "stress")
-          for part in ("real", "kspace")) + ("se2p_free_slp", "se2p_free_dlp")
+          for part in ("real", "kspace")) + ("se2p_free_slp", "se2p_free_dlp",
+                                                  "se2p_mlp_forward", "se2p_mlp_loss_grad",
+                                                  "se2p_mlp_adam")
 
 
 class Se2pError(RuntimeError):
@@ -198,6 +200,10 @@ def lib():
                for q in ("pressure_grad", "vorticity")},
             "se2p_free_slp": [P, P, P, P, P, I, P, P, I, I, I, P, P],
             "se2p_free_dlp": [P, P, P, P, P, P, P, I, P, P],
+            "se2p_mlp_forward": [P, P, P, I, P],
+            "se2p_mlp_loss_grad": [P, P, P, P, I, P, P],
+            "se2p_mlp_adam": [P, P, P, P, P, I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -1254,7 +1254,8 @@ static size_t noalloc_bytes(int kind, int Ns, int Nt, const se2p_params& prm)
     b += r(512) + r(4 * ns) + r(4 * ntile) + r(4 * ns) + r(4 * (ntile + 1)) + r(8 * ns) +
          r(16 * ns) + r(8 * ns) + r(32 * ns) + r((dlp ? 32 : 16) * ns) + r(4 * nt) +
          r(4 * gtile) + r(4 * nt) + r(4 * (gtile + 1)) + r(8 * nt) + r(16 * nt) +
-         r(16 * G) * (two ? 4 : 2) + r(16.0 * prm.Mx) + r(16.0 * prm.My);
+         r(16 * G) * (two ? 4 : 2) + r(16.0 * prm.Mx) + r(16.0 * prm.My) +
+         r(8 * nt) + r(16 * nt) + r(4 * nt);  // binned gathering records
     // four stable sorts (source cells, target cells, spreading and gathering
     // tiles): scratch of n and a list of nkeys + 1 each
     for (double n : {ns, nt, ns, nt}) b += r(4 * n);
@@ -1464,6 +1465,34 @@ int se2p_batch_destroy(se2p_batch* batch)
     ws_release(&batch->bws);
     delete batch;
     return SE2P_OK;
+}
+
+int se2p_mlp_forward(se2p_ctx* ctx, const float* d_w, const float* d_x, int n, float* d_y)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (n < 0 || (n > 0 && (!d_w || !d_x || !d_y))) return set_error(SE2P_ERR_ARG, "bad arrays");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return mlp_forward(ctx, d_w, d_x, n, d_y);
+}
+
+int se2p_mlp_loss_grad(se2p_ctx* ctx, const float* d_w, const float* d_x, const float* d_t,
+                       int n, float* d_grad, float* d_loss)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (n <= 0 || !d_w || !d_x || !d_t || !d_grad)
+        return set_error(SE2P_ERR_ARG, "bad arrays or no samples");
+    if (ctx->capturing) return set_error(SE2P_ERR_ARG, "not under graph capture");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return mlp_grad(ctx, &ctx->ws, d_w, d_x, d_t, n, d_grad, d_loss);
+}
+
+int se2p_mlp_adam(se2p_ctx* ctx, float* d_w, const float* d_g, float* d_m, float* d_v, int step,
+                  float lr, float beta1, float beta2, float eps)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (!d_w || !d_g || !d_m || !d_v || step < 1) return set_error(SE2P_ERR_ARG, "bad arguments");
+    SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+    return mlp_adam(ctx, d_w, d_g, d_m, d_v, step, lr, beta1, beta2, eps);
 }
 
 int se2p_ctx_set_precision(se2p_ctx* ctx, int precision)

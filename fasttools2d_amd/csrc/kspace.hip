@@ -280,6 +280,19 @@ __global__ void k_spread_records(const int* __restrict__ perm, const int2* __res
                               exp(t * (o.y * o.y + 2 * w * o.y)), exp(-2 * t * h * o.y));
 }
 
+__global__ void k_gather_records(const int* __restrict__ perm, const int2* __restrict__ node,
+                                 const double2* __restrict__ off, const int* __restrict__ spos,
+                                 int n, int2* __restrict__ bnode, double2* __restrict__ boff,
+                                 int* __restrict__ sidx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = perm[k];
+    bnode[k] = node[p];
+    boff[k] = off[p];
+    if (spos) sidx[k] = spos[p];
+}
+
 __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__ sys,
                                                      const Job* __restrict__ jobs,
                                                      const double* __restrict__ f_base)
@@ -374,8 +387,8 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const SysDesc& d = sys[jb.x];
     const int* __restrict__ tstart = d.gt_start;
     const int* __restrict__ perm = d.gt_perm;
-    const int2* __restrict__ node = d.gt_node;
-    const double2* __restrict__ off = d.gt_off;
+    const int2* __restrict__ node = d.gt_node;    // binned order
+    const double2* __restrict__ off = d.gt_off;   // binned order
     const double2* __restrict__ W = d.grid[g ? G_D : G_W];
     const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
@@ -412,7 +425,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
         const int cnt = min(256, k1 - kb);
         int key = -1;
         if (tid < cnt) {
-            const int2 m = node[perm[kb + tid]];
+            const int2 m = node[kb + tid];
             key = ((m.x - X0) * R + (m.y - Y0)) & 15;
         }
         int rank = 0;
@@ -437,9 +450,10 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
         }
         __syncthreads();
         if (tid < cnt) {
-            const int p = perm[kb + order[tid]];
-            const int2 m = node[p];
-            const double2 o = off[p];
+            const int kk = kb + order[tid];  // binned index
+            const int p = perm[kk];
+            const int2 m = node[kk];
+            const double2 o = off[kk];
             const int ox = m.x - X0, oy = m.y - Y0;
             double s0 = 0, s1 = 0;
             const V* row = reg + ox * R + oy;
@@ -499,7 +513,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
                 double* up = u + (size_t)d.nc * p + 2 * g;
                 if (accumulate && sym) {
                     // the pair-symmetric real-space sums, in sorted order
-                    const double2 a = d.sacc[d.spos[p]];
+                    const double2 a = d.sacc[d.gt_sidx[kk]];
                     v0 = a.x / (4 * kPi) + v0;
                     v1 = a.y / (4 * kPi) + v1;
                 } else if (accumulate) {
@@ -685,6 +699,16 @@ int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* pe
     const double t = -2 * g.xi * g.xi / g.eta;
     hipLaunchKernelGGL(k_spread_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
                        n, t, g.w, g.h, prec, snode, fac);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
+                          const int* spos, int n, int2* bnode, double2* boff, int* sidx)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_gather_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
+                       spos, n, bnode, boff, sidx);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

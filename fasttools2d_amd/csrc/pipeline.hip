@@ -203,6 +203,18 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, gtiles, G->gt_perm,
                              G->gt_start, WS_SORT_TMP));
+        // the gather reads its targets' records in binned order (contiguous)
+        SE2P_TRY(ws_get(ctx, ws, WS_GAT_BNODE, sizeof(int2) * (size_t)Nt, &p));
+        G->gt_bnode = (int2*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_GAT_BOFF, sizeof(double2) * (size_t)Nt, &p));
+        G->gt_boff = (double2*)p;
+        if (G->sym) {
+            SE2P_TRY(ws_get(ctx, ws, WS_GAT_SIDX, sizeof(int) * (size_t)Nt, &p));
+            G->gt_sidx = (int*)p;
+        }
+        SE2P_TRY(launch_gather_records(s, G->gt_perm, G->gt_node, G->gt_off,
+                                       G->sym ? G->spos : nullptr, Nt, G->gt_bnode, G->gt_boff,
+                                       G->gt_sidx));
     }
     return SE2P_OK;
 }
@@ -282,8 +294,9 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->sp_snode = G.sp_snode;
         d->sp_fac = G.sp_fac;
         d->sp_dens = G.sp_dens;
-        d->gt_node = G.gt_node;
-        d->gt_off = G.gt_off;
+        d->gt_node = G.gt_bnode;
+        d->gt_off = G.gt_boff;
+        d->gt_sidx = G.sym ? G.gt_sidx : nullptr;
         d->e1tab = G.e1tab;
         d->grid[G_Z0] = G.Z0;
         d->grid[G_Z1] = G.Z1;

@@ -96,7 +96,7 @@ enum Slot {
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
     WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
-    WS_NUM_SLOTS
+    WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_NUM_SLOTS
 };
 
 // Grow-only device buffers addressed by slot (one set per context, one per plan).
@@ -255,7 +255,8 @@ struct SysDesc {
     const int2 *sp_snode, *gt_node;  // spreading: start nodes in binned order
     const double4* sp_fac;  // spreading: FGG factors (ex, e3x, ey, e3y) in binned order
     double2* sp_dens;       // spreading: (d0, d1) of pair c at c Ns + k, binned order
-    const double2* gt_off;
+    const double2* gt_off;  // gathering: offsets in binned order (gt_node likewise)
+    const int* gt_sidx;     // pair-symmetric sums: sorted position of each binned target
     const double *e1tab, *nrm;
     double2* grid[4];  // GridSel
     const double2 *twx, *twy;
@@ -295,6 +296,10 @@ int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, con
 // Spreading records of a geometry: start node and FGG factors of the k-th binned point.
 int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* perm, const int2* node,
                           const double2* off, int n, int2* snode, double4* fac);
+// Gathering records in binned order: bnode[k] = node[perm[k]], boff[k] =
+// off[perm[k]] and, when spos is given, sidx[k] = spos[perm[k]].
+int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
+                          const int* spos, int n, int2* bnode, double2* boff, int* sidx);
 // Densities of the binned points of every spreading pair (job (s, b): points
 // b*256 .. b*256+255 of system s).
 int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
@@ -339,6 +344,15 @@ int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const Sy
 int transpose_tiles(int rows, int cols);
 
 // ---------------------------------------------------------------------------
+// Parameter selector network (mlp.hip): packed FP32 parameters, device arrays
+// ---------------------------------------------------------------------------
+int mlp_forward(se2p_ctx* ctx, const float* w, const float* x, int n, float* y);
+int mlp_grad(se2p_ctx* ctx, Workspace* ws, const float* w, const float* x, const float* t, int n,
+             float* grad, float* loss);
+int mlp_adam(se2p_ctx* ctx, float* w, const float* g, float* m, float* v, int step, float lr,
+             float b1, float b2, float eps);
+
+// ---------------------------------------------------------------------------
 // Pipeline (pipeline.hip): geometry set-up once, density-dependent work per apply
 // ---------------------------------------------------------------------------
 struct Geom {
@@ -372,8 +386,11 @@ struct Geom {
     double4* sp_fac = nullptr;
     double2* sp_dens = nullptr;
     int *gt_perm = nullptr, *gt_start = nullptr;
-    int2* gt_node = nullptr;
+    int2* gt_node = nullptr;    // start nodes and offsets in input order
     double2* gt_off = nullptr;
+    int2* gt_bnode = nullptr;   // the same in gathering-binned order
+    double2* gt_boff = nullptr;
+    int* gt_sidx = nullptr;     // pair-symmetric sums: sorted position, binned order
     double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr, *D = nullptr;
 };
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,

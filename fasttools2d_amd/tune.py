@@ -209,11 +209,10 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
     T = Tuner(ctx, kind, ps, pt, f, Lx, Ly, nrm=nrm, log=log)
     prec = precision_for(tol) if precision is None else int(precision)
     report = {"tol": tol, "max_grid": max_grid, "precision": prec}
-    from .periodic_ewald import density_q, ewald_params
+    from .periodic_ewald import density_q
 
     def reference_params(dens):
-        return ewald_params(kind, Ns, Nt, density_q(dens[:, 0], dens[:, 1]), Lx, Ly, P=24,
-                            tol=1e-13)
+        return recipe_params(kind, Ns, Nt, density_q(dens[:, 0], dens[:, 1]), Lx, Ly, 1e-13)
 
     try:
         # 1. reference solution
@@ -473,6 +472,23 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
     prec = precision_for(tol) if precision is None else int(precision)
     return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric,
                        prec)
+
+
+def recipe_params(kind, Ns, Nt, Q, Lx, Ly, tol, P=24, max_grid=4096):
+    """The reference recipe (periodic_ewald.ewald_params) with its points-per-cell
+    option raised until the grid fits max_grid per side (larger cells, smaller
+    xi, coarser grid): wide boxes at tight tolerances otherwise ask for grids
+    beyond the FFT's longest line."""
+    from .periodic_ewald import ewald_params
+    prm = ewald_params(kind, Ns, Nt, Q, Lx, Ly, P=P, tol=tol)
+    nb = (Ns + Nt) / (prm["nside_x"] * prm["nside_y"])
+    while max(prm["Mx"], prm["My"]) > max_grid:
+        nb *= 1.25
+        prm = ewald_params(kind, Ns, Nt, Q, Lx, Ly, P=P, tol=tol, Nb=nb)
+        if prm["nside_x"] < 1 or prm["nside_y"] < 1:
+            raise ValueError(f"no grid within {max_grid} per side for N={Ns}+{Nt}, "
+                             f"box {Lx}x{Ly}, tol {tol}")
+    return prm
 
 
 def capped_params(n, Lx, Ly, tol, P=24):

@@ -422,6 +422,30 @@ int se2p_batch_params(const se2p_batch* batch, int sys, se2p_params* used);
 int se2p_batch_destroy(se2p_batch* batch);
 
 /* ------------------------------------------------------------------------ */
+/* Parameter selector network.  No counterpart in the reference, whose       */
+/* drivers pick (xi, M, P, rc) from analytic error estimates                 */
+/* (StokesSLP_ewald_2p.m:84-104,147-202).  An MLP of SE2P_MLP_IN inputs, two */
+/* hidden layers of SE2P_MLP_HIDDEN GELU units (gelu(z) = z Phi(z)) and      */
+/* SE2P_MLP_OUT outputs, in FP32, its SE2P_MLP_PARAMS parameters packed as   */
+/* W1[64][8], b1[64], W2[64][64], b2[64], W3[4][64], b3[4].  Device arrays,  */
+/* stream-ordered on the context's stream.                                   */
+/* ------------------------------------------------------------------------ */
+#define SE2P_MLP_IN 8
+#define SE2P_MLP_HIDDEN 64
+#define SE2P_MLP_OUT 4
+#define SE2P_MLP_PARAMS 4996
+/* y (n x 4) = the network applied to x (n x 8). */
+int se2p_mlp_forward(se2p_ctx* ctx, const float* d_w, const float* d_x, int n, float* d_y);
+/* d_loss (1 float) = mean over the n x 4 outputs of (y - t)^2 and d_grad
+ * (SE2P_MLP_PARAMS floats) its gradient, summed in a fixed order. */
+int se2p_mlp_loss_grad(se2p_ctx* ctx, const float* d_w, const float* d_x, const float* d_t,
+                       int n, float* d_grad, float* d_loss);
+/* One Adam step (bias-corrected, step >= 1) on the packed parameters with
+ * gradient d_g and moment arrays d_m, d_v (updated in place). */
+int se2p_mlp_adam(se2p_ctx* ctx, float* d_w, const float* d_g, float* d_m, float* d_v, int step,
+                  float lr, float beta1, float beta2, float eps);
+
+/* ------------------------------------------------------------------------ */
 /* Stage-level entry points (host buffers, synchronous): the helpers of      */
 /* PeriodicEwald/src/mex/common/ewald_tools.h, each replaced by the GPU path  */
 /* the mex kernels above use internally.                                     */

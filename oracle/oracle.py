@@ -30,63 +30,83 @@ def build(force=False):
     return _LIB_PATH
 
 
+_REF_PATH = os.path.join(_HERE, "libse2p_refcpu.so")
+_ref = None
+
+
+def refcpu():
+    """The CPU baseline build of the same stages (Makefile target
+    libse2p_refcpu.so: -O3, AVX2/FMA, rational-fit E1); bench.py times it."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(_REF_PATH):
+            build(force=True)
+        _ref = _load(_REF_PATH)
+    return _ref
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        sig = {
-            "orc_assign": [_P, _I, _D, _D, _I, _I, _P, _P, _P],
-            "orc_find_closest_node": [_D, _D, _D, _D, _D, _I, _P, _P, _P, _P],
-            "orc_slp_real": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
-            "orc_dlp_real": [_P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
-            "orc_spread": [_P, _P, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P],
-            "orc_fft2": [_P, _P, _I, _I, _I],
-            "orc_slp_filter": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D],
-            "orc_dlp_filter": [_P] * 8 + [_I, _I, _D, _D, _D, _D],
-            "orc_gather": [_P, _I, _I, _P, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P],
-            "orc_slp_kspace": [_P, _P, _D, _D, _P, _I, _I, _I, _I, _D, _D, _D, _I, _P],
-            "orc_dlp_kspace": [_P, _P, _D, _D, _P, _P, _I, _I, _I, _I, _D, _D, _D, _I, _P],
-            "orc_slp_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
-            "orc_slp_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
-            "orc_dlp_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
-            "orc_dlp_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
-            "orc_dlp_zero_mode": [_P, _P, _P, _I, _D, _D, _P],
-            "orc_params": [_I, _I, _D, _D, _D, _I, _D, _D, _P, _P],
-            "orc_rat": [_D, _P, _P],
-            "orc_slp_pressure_real": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
-            "orc_slp_pressure_filter": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D],
-            "orc_slp_pressure_kspace": [_P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I, _P],
-            "orc_slp_pressure_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
-            "orc_slp_pressure_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
-            "orc_dlp_pressure_real": [_P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
-            "orc_dlp_pressure_filter": [_P] * 8 + [_I, _I, _D, _D, _D, _D],
-            "orc_dlp_pressure_kspace": [_P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
-                                        _P],
-            "orc_dlp_pressure_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
-            "orc_dlp_pressure_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
-            "orc_derived_real": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
-            "orc_derived_kspace": [_I, _P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
-                                   _P],
-            "orc_derived_real_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
-            "orc_derived_kspace_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
-            "orc_free_slp": [_P, _P, _I, _P, _I, _I, _P],
-            "orc_free_dlp": [_P, _P, _P, _I, _P],
-        }
-        for name, args in sig.items():
-            fn = getattr(L, name)
-            fn.argtypes = args
-            fn.restype = None
-        L.orc_params.restype = _I
-        L.orc_e1.argtypes = [_D]
-        L.orc_e1.restype = _D
-        L.orc_slp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
-        L.orc_slp_pressure_zero_mode.restype = _D
-        L.orc_dlp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
-        L.orc_dlp_pressure_zero_mode.restype = _D
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def _load(path):
+    """ctypes handle of an oracle build with its argument types."""
+    L = ctypes.CDLL(path)
+    sig = {
+        "orc_assign": [_P, _I, _D, _D, _I, _I, _P, _P, _P],
+        "orc_find_closest_node": [_D, _D, _D, _D, _D, _I, _P, _P, _P, _P],
+        "orc_slp_real": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+        "orc_dlp_real": [_P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+        "orc_spread": [_P, _P, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P],
+        "orc_fft2": [_P, _P, _I, _I, _I],
+        "orc_slp_filter": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D],
+        "orc_dlp_filter": [_P] * 8 + [_I, _I, _D, _D, _D, _D],
+        "orc_gather": [_P, _I, _I, _P, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P],
+        "orc_slp_kspace": [_P, _P, _D, _D, _P, _I, _I, _I, _I, _D, _D, _D, _I, _P],
+        "orc_dlp_kspace": [_P, _P, _D, _D, _P, _P, _I, _I, _I, _I, _D, _D, _D, _I, _P],
+        "orc_slp_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
+        "orc_slp_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+        "orc_dlp_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
+        "orc_dlp_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+        "orc_dlp_zero_mode": [_P, _P, _P, _I, _D, _D, _P],
+        "orc_params": [_I, _I, _D, _D, _D, _I, _D, _D, _P, _P],
+        "orc_rat": [_D, _P, _P],
+        "orc_slp_pressure_real": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+        "orc_slp_pressure_filter": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D],
+        "orc_slp_pressure_kspace": [_P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I, _P],
+        "orc_slp_pressure_real_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
+        "orc_slp_pressure_kspace_ds": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+        "orc_dlp_pressure_real": [_P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+        "orc_dlp_pressure_filter": [_P] * 8 + [_I, _I, _D, _D, _D, _D],
+        "orc_dlp_pressure_kspace": [_P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
+                                    _P],
+        "orc_dlp_pressure_real_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
+        "orc_dlp_pressure_kspace_ds": [_P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+        "orc_derived_real": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _I, _D, _D, _P],
+        "orc_derived_kspace": [_I, _P, _P, _P, _P, _D, _D, _I, _I, _I, _I, _D, _D, _D, _I,
+                               _P],
+        "orc_derived_real_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _P],
+        "orc_derived_kspace_ds": [_I, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+        "orc_free_slp": [_P, _P, _I, _P, _I, _I, _P],
+        "orc_free_dlp": [_P, _P, _P, _I, _P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = None
+    L.orc_params.restype = _I
+    L.orc_e1.argtypes = [_D]
+    L.orc_e1.restype = _D
+    L.orc_slp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
+    L.orc_slp_pressure_zero_mode.restype = _D
+    L.orc_dlp_pressure_zero_mode.argtypes = [_P, _P, _I, _D, _D]
+    L.orc_dlp_pressure_zero_mode.restype = _D
+    return L
 
 
 def _c(a, dtype=np.float64):
@@ -120,10 +140,10 @@ def e1(x):
     return lib().orc_e1(float(x))
 
 
-def slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly):
+def slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, L=None):
     psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
     u = np.zeros((ptar.shape[0], 2))
-    lib().orc_slp_real(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0], ptar.shape[0], xi,
+    (L or lib()).orc_slp_real(_ptr(psrc), _ptr(ptar), _ptr(f), psrc.shape[0], ptar.shape[0], xi,
                        nside_x, nside_y, Lx, Ly, _ptr(u))
     return u
 
@@ -178,10 +198,10 @@ def gather(H, ptar, Lx, Ly, xi, w, eta, P):
     return out[:, 0]
 
 
-def slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P):
+def slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P, L=None):
     psrc, ptar, f = _c(psrc), _c(ptar), _c(f)
     u = np.zeros((ptar.shape[0], 2))
-    lib().orc_slp_kspace(_ptr(psrc), _ptr(ptar), xi, eta, _ptr(f), psrc.shape[0],
+    (L or lib()).orc_slp_kspace(_ptr(psrc), _ptr(ptar), xi, eta, _ptr(f), psrc.shape[0],
                          ptar.shape[0], Mx, My, Lx, Ly, w, P, _ptr(u))
     return u
 

@@ -8,6 +8,13 @@
  * not depend on the OpenMP thread count: every parallel loop writes disjoint
  * outputs and every floating-point reduction runs in a fixed order.
  *
+ * The same source built with -DSE2P_REFCPU (-O3, AVX2/FMA, contraction allowed)
+ * is the benchmark's CPU baseline, "refcpu": its real-space E1 then comes from
+ * rational fits in double precision (accuracy 1e-18 before rounding, the
+ * coefficient header e1_coeffs.h) as the reference's mex does with its own fits
+ * (mex_stokes_slp_real.cpp:19-48,166-205), instead of the checker's long-double
+ * series and continued fraction.
+ *
  * Reference paths below are relative to the reference repository root.
  */
 #include "se2p_oracle.h"
@@ -123,6 +130,30 @@ double orc_e1(double xd)
     return (double)(hcf * expl(-x));
 }
 
+#ifdef SE2P_REFCPU
+#include "e1_coeffs.h"
+/* E1 from rational fits: x < 1: -gamma - ln x + x A(x); x >= 1: e^{-x} t P(t)/Q(t),
+ * t = 1/x (e = e^{-x} given by the caller). */
+static inline double refcpu_e1(double x, double e)
+{
+    static const double near[] = SE2P_E1_FULL_NEAR;
+    static const double fp[] = SE2P_E1_FULL_FARP;
+    static const double fq[] = SE2P_E1_FULL_FARQ;
+    if (x < 1.0) {
+        double a = near[SE2P_E1_FULL_NEAR_DEG];
+        for (int i = SE2P_E1_FULL_NEAR_DEG - 1; i >= 0; --i) a = a * x + near[i];
+        return -(double)ORC_EULER - log(x) + x * a;
+    }
+    const double t = 1.0 / x;
+    double p = fp[SE2P_E1_FULL_FAR_DEG], q = fq[SE2P_E1_FULL_FAR_DEG];
+    for (int i = SE2P_E1_FULL_FAR_DEG - 1; i >= 0; --i) {
+        p = p * t + fp[i];
+        q = q * t + fq[i];
+    }
+    return e * t * p / q;
+}
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* Real-space pair kernels                                                   */
 /* ------------------------------------------------------------------------ */
@@ -135,7 +166,11 @@ static inline void slp_pair(double dx, double dy, double r2, double xi2, double 
 {
     double x = xi2 * r2;
     double e = exp(-x);
+#ifdef SE2P_REFCPU
+    double E = refcpu_e1(x, e);
+#else
     double E = orc_e1(x);
+#endif
     double c = e / r2;
     double cr = dx * f2 - dy * f1;
     *a1 += 0.5 * E * f1 + c * dy * cr;

@@ -360,3 +360,17 @@ def test_density_q_sums_columns_in_order(oracle):
     a = oracle.params("slp", 2000, q + 1.0, 1.0, 1.0, 24, oracle.default_Nb(2000), 1e-10)
     b = oracle.params("slp", 2000, (q1 + q2) + 1, 1.0, 1.0, 24, oracle.default_Nb(2000), 1e-10)
     assert a["xi"] != b["xi"]
+
+
+def test_refcpu_build_matches_the_checker(oracle):
+    """The CPU baseline build (rational-fit E1, -O3 with contraction) computes
+    the same sums as the checker to rounding."""
+    p = uniform_points(3000, 1.0, 1.0, seed=77)
+    f = normal_density(3000, seed=78)
+    R = oracle.refcpu()
+    a = oracle.slp_real(p, p, f, 40.0, 7, 7, 1.0, 1.0, L=R)
+    b = oracle.slp_real(p, p, f, 40.0, 7, 7, 1.0, 1.0)
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b)
+    a = oracle.slp_kspace(p, p, 40.0, 0.5, f, 96, 96, 1.0, 1.0, 24 / 96 / 2, 24, L=R)
+    b = oracle.slp_kspace(p, p, 40.0, 0.5, f, 96, 96, 1.0, 1.0, 24 / 96 / 2, 24)
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b)
