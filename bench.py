@@ -288,7 +288,7 @@ def profile_only(ctx, dev, ps, p, f, args):
                       "stage_ms": stages}))
 
 
-def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
+def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule", ctx=None):
     """The batched serving workload (fasttools2d_amd.workloads.serving_systems)
     with parameters per system at its tolerance: recipe "rule" is
     tune.rule_params (the tuner's calibrated constants as functions of the
@@ -302,6 +302,12 @@ def serving_systems(nsys, seed=11, nmin=500, nmax=200_000, recipe="rule"):
         if recipe == "rule":  # FP32 or FP64 per system by its tolerance
             y["prm"] = rule_params("slp", n, n, y["Lx"], y["Ly"], y["tol"], symmetric=1,
                                    precision=None)
+        elif recipe == "selector":  # the learned selector, checked per system
+            from fasttools2d_amd import selector as S
+            net = getattr(serving_systems, "_net", None) or S.load()
+            serving_systems._net = net
+            y["prm"], y["select"] = S.select(ctx, net, "slp", y["psrc"], y["Lx"], y["Ly"],
+                                             y["tol"])
         else:
             y["prm"] = capped_params(n, y["Lx"], y["Ly"], y["tol"])
             y["prm"]["e1_tier"] = 1
@@ -318,10 +324,12 @@ def batch_bench(args):
     from fasttools2d_amd.batch import Batch
     from fasttools2d_amd.plan import DeviceArray, Plan
     from fasttools2d_amd.tune import capped_params, rel_err
-    systems = serving_systems(args.nsys, recipe=args.recipe)
-    ntot = sum(len(y["psrc"]) for y in systems)
-    log(f"bench-batch: {len(systems)} systems, {ntot} points")
     ctx = ft.Context(0)
+    t0 = time.perf_counter()
+    systems = serving_systems(args.nsys, recipe=args.recipe, ctx=ctx)
+    t_params = time.perf_counter() - t0
+    ntot = sum(len(y["psrc"]) for y in systems)
+    log(f"bench-batch: {len(systems)} systems, {ntot} points, parameters in {t_params:.1f} s")
     t0 = time.perf_counter()
     b = Batch(ctx, systems)
     ctx.sync()
@@ -377,7 +385,8 @@ def batch_bench(args):
         "config": {"workload": f"{len(systems)} independent periodic SLP systems, aspect 1-4, "
                                "N log-uniform 500-200000, tol log-uniform 1e-10..1e-4",
                    "points": ntot, "recipe": args.recipe},
-        "create_s": t_create, "worst_err_over_tol": worst,
+        "create_s": t_create, "params_s": t_params, "worst_err_over_tol": worst,
+        "selector_fallbacks": sum(1 for y in systems if y.get("select", {}).get("fallback")),
         "graph_replay_ms_per_batch": dt_graph / args.steps * 1e3,
     }
     for a in dens:
@@ -548,7 +557,7 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="run the batched serving workload instead (own JSON line)")
     ap.add_argument("--nsys", type=int, default=512)
-    ap.add_argument("--recipe", choices=["rule", "ref"], default="rule",
+    ap.add_argument("--recipe", choices=["rule", "ref", "selector"], default="rule",
                     help="parameters of the batched systems")
     ap.add_argument("--profile", choices=["plain", "tuned", "copy"],
                     help="only run that path for --steps evaluations (for rocprofv3)")
