@@ -276,8 +276,13 @@ def profile_only(ctx, dev, ps, p, f, args):
     q = json.loads(args.params)
     xi = q["c_r"] * q["nside"]
     prm = make_params(xi, q["nside"], q["nside"], q["M"], q["M"], q["P"], 1.0, q.get("tier", 1),
-                      q.get("symmetric", 0))
-    plan = Plan(ctx, "slp", p, p, 1.0, 1.0, prm)
+                      q.get("symmetric", 0), 0, q.get("far32", 0.0))
+    kind = q.get("kind", "slp")
+    nrm = None
+    if kind == "dlp":
+        from fasttools2d_amd.workloads import unit_normals
+        nrm = unit_normals(len(p), seed=9)
+    plan = Plan(ctx, kind, p, p, 1.0, 1.0, prm, nrm=nrm)
     df = DeviceArray.from_host(ctx, f)
     du = DeviceArray(ctx, (len(p), 2))
     # every stage on one stream, warm-up included, so the profiler's kernel
@@ -460,6 +465,67 @@ def clustered_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def dlp_bench(args):
+    """The stresslet (DLP) at the headline size: N = --n sources = targets
+    uniform in the unit box, N(0,1) density, random unit normals, autotuned at
+    --tol (FP64), plan reuse; stream launches or graph replay, whichever is
+    faster.  Stage times, the dominant kernel's roofline and the error against
+    the CPU oracle (tol 1e-13, all sources) on 1e4 targets."""
+    import fasttools2d_amd as ft
+    from fasttools2d_amd.plan import DeviceArray, Plan
+    from fasttools2d_amd.tune import autotune, rel_err
+    from fasttools2d_amd.workloads import unit_normals
+    N, tol = args.n, args.tol
+    p, f = workload(N)
+    nrm = unit_normals(N, seed=9)
+    ctx = ft.Context(0)
+    prm, rep = autotune(ctx, "dlp", p, p, f, 1.0, 1.0, tol, nrm=nrm, log=log)
+    log(f"bench-dlp: tuned parameters {prm}")
+    pl = Plan(ctx, "dlp", p, p, 1.0, 1.0, prm, nrm=nrm)
+    df = DeviceArray.from_host(ctx, f)
+    du = DeviceArray(ctx, (N, 2))
+    apply = lambda: pl.apply_device(df.ptr, du.ptr)  # noqa: E731
+    replay = lambda: pl.apply_device(df.ptr, du.ptr, graph=True)  # noqa: E731
+    time_steps(ctx, replay, 1, 2)
+    ms = {"streams": time_steps(ctx, apply, args.steps, args.warmup) * 1e3,
+          "graph": time_steps(ctx, replay, args.steps, args.warmup) * 1e3}
+    mode = min(ms, key=ms.get)
+    stages = stage_profile(ctx, apply, args.steps, serial=True)
+    cand, evald = real_space_work(p, prm)
+    wm = work_model(prm, N, cand, evald, kind="dlp")
+    dom = max(("real", "spread", "gather", "fft"), key=lambda k: stages.get(k, 0))
+    achieved = wm[dom] / (stages[dom] * 1e-3) / 1e12
+    out = {"metric": "periodic Stokes DLP evals/s, N=1M, rel err<=1e-8", "value": 1e3 / ms[mode],
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms[mode], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: uniform points in the unit box, f ~ N(0,1), random unit "
+                   "normals, seed 1/9",
+           "config": {"workload": f"DLP spectral Ewald, N={N} sources=targets, unit box, "
+                                  f"tol {tol}",
+                      "params": {k: prm[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier")}},
+           "launch": mode, "stream_launch_ms": ms["streams"], "graph_replay_ms": ms["graph"],
+           "stage_ms": stages, "work_model_flops": wm,
+           "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved,
+                        "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
+                        "per_launch_flops": wm[dom], "avg_launch_ms": stages[dom]},
+           "tune_report": rep}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        sub = np.sort(np.random.default_rng(7).choice(N, size=min(N, 10_000), replace=False))
+        apply()
+        u = du.download()[sub]
+        ref = clustered_oracle(O, "dlp", p, sub, f, nrm)
+        out["rel_err"] = rel_err(u, ref)
+        out["rel_err_vs"] = "CPU oracle (tol 1e-13, all sources, 1e4-target subset)"
+        log(f"bench-dlp: error vs the CPU oracle {out['rel_err']:.3e}")
+    pl.close()
+    df.free()
+    du.free()
+    print(json.dumps(out), flush=True)
+
+
 def clustered_oracle(O, kind, p, sub, dens, nrm, tol=1e-13):
     """The checker for the clustered workload: the CPU oracle's spectral Ewald sum
     at tol (all sources) at the targets p[sub], grid raised to a 2-3-5-smooth size."""
@@ -562,7 +628,8 @@ def main():
     ap.add_argument("--profile", choices=["plain", "tuned", "copy"],
                     help="only run that path for --steps evaluations (for rocprofv3)")
     ap.add_argument("--params", default=None,
-                    help="JSON of tuned parameters (nside, M, P, c_r, tier) for --profile tuned")
+                    help="JSON of tuned parameters (nside, M, P, c_r, tier; optional symmetric, "
+                         "far32, kind slp|dlp) for --profile tuned")
     ap.add_argument("--selector", action="store_true",
                     help="evaluate the learned parameter selector on --nsys held-out systems")
     ap.add_argument("--clustered", action="store_true",
@@ -571,6 +638,8 @@ def main():
     ap.add_argument("--free", action="store_true",
                     help="run the free-space (FMM replacement) benchmark instead")
     ap.add_argument("--free-n", type=int, default=100_000)
+    ap.add_argument("--dlp", action="store_true",
+                    help="run the stresslet (DLP) at --n and --tol instead (own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
@@ -586,6 +655,9 @@ def main():
         return
     if args.selector:
         selector_bench(args)
+        return
+    if args.dlp:
+        dlp_bench(args)
         return
 
     import fasttools2d_amd as ft
@@ -699,7 +771,7 @@ def main():
                    "path": "tuned: autotuned split, plan reuse, pair-symmetric real space, "
                            f"real-space and k-space passes on two streams ({mode}), FP64",
                    "params": {k: used[k] for k in ("xi", "nside_x", "Mx", "P", "e1_tier",
-                                                   "symmetric")}},
+                                                   "symmetric", "far32")}},
         "rel_err": err_tuned,
         "rel_err_vs": "GPU plan at tol 1e-13 (all targets)",
         "rel_err_gpu_reference": err_tuned,

@@ -102,7 +102,8 @@ class Params(ctypes.Structure):
                 ("eta", ctypes.c_double), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
                 ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int),
-                ("symmetric", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("symmetric", ctypes.c_int), ("precision", ctypes.c_int),
+                ("far32", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -218,6 +218,7 @@ int check_params(const se2p_params* prm, double Lx, double Ly)
         return set_error(SE2P_ERR_ARG, "symmetric must be 0 or 1");
     if (prm->precision != SE2P_FP64 && prm->precision != SE2P_FP32C)
         return set_error(SE2P_ERR_ARG, "precision must be SE2P_FP64 or SE2P_FP32C");
+    if (!(prm->far32 >= 0)) return set_error(SE2P_ERR_ARG, "far32 must be >= 0");
     return SE2P_OK;
 }
 

@@ -448,6 +448,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const double ixi2 = 1.0 / xi2;
     const int nround = (total + CAP - 1) / CAP;
     const double trim_w = d.trim_w;
+    const double far32_r2 = d.far32_r2;
     const bool trim = !UNI && trim_w > 0 && d.unsorted && *d.unsorted == 0;
 
     for (int tb = 0; tb < ntar; tb += kBlkThreads) {
@@ -620,12 +621,13 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 // kw[w], lands on bit kw[w] - 1 - k), then combined with the range masks;
                 // kw[w] is rounded up to a multiple of 8 and the candidates past the
                 // chunk's end are masked off as not valid
-                unsigned long long fm[kWords], nm[kWords];
+                unsigned long long fm[kWords], nm[kWords], g32[kWords];
                 int kw[kWords];
 #pragma unroll
                 for (int w = 0; w < kWords; ++w) {
                     kw[w] = (min(64, max(0, kc - 64 * w)) + 7) & ~7;
                     unsigned long long cm = 0, zm = 0;  // r^2 < cut^2; SLP r^2 < 1/xi^2, DLP r^2 != 0
+                    unsigned long long gm = 0;  // fast tier: r^2 >= far32_r2 (FP32 far pairs)
                     for (int k0 = 0; k0 < kw[w]; k0 += 8) {
                         double2 sp[8];
 #pragma unroll
@@ -638,6 +640,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             const double dx = pt.x - sp[j].x, dy = pt.y - sp[j].y;
                             const double r2 = dx * dx + dy * dy;
                             cm = (cm << 1) | (r2 < cut2 ? 1ull : 0ull);
+                            if (KIND == SE2P_SLP && TIER == kE1Fast)
+                                gm = (gm << 1) | (r2 >= far32_r2 ? 1ull : 0ull);
                             if (KIND == SE2P_SLP)
                                 zm = (zm << 1) | (r2 < ixi2 ? 1ull : 0ull);
                             else if (KIND == SE2P_DLP_VORTICITY)  // own-cell pairs r^2 < 1e-13 skipped
@@ -659,6 +663,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         fm[w] = ((cm & ~own) | (own & zm)) & valid;
                         nm[w] = 0;
                     }
+                    g32[w] = fm[w] & gm;
                 }
                 // candidate position of the lowest set bit of (lo, hi), which is cleared
                 // candidate position of bit 0 of each word
@@ -672,8 +677,11 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     hi = z ? m2 : hi;
                     return (z ? pos_w1 : pos_w0) - bit * nch;
                 };
-                unsigned long long lo = fm[0], hi = fm[1];
-                float c1 = 0.f, c2 = 0.f;  // FP32 tier: this chunk's far pairs
+                // Far pairs of the chunk: FP64 terms into (a1, a2), FP32 terms (the FP32
+                // tier; in the fast tier the pairs with r^2 >= far32_r2) into (c1, c2).
+                float c1 = 0.f, c2 = 0.f;
+                auto far_loop = [&](unsigned long long lo, unsigned long long hi, auto f32tag) {
+                constexpr bool F32 = decltype(f32tag)::value;
                 if constexpr (SYM) {
                     // the next hit's record is read before this hit's LDS atomics
                     // are issued, so waiting for it does not wait for them
@@ -686,7 +694,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         const double2 qn = bxy[nidx], gn = bfd[nidx];
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
-                        if constexpr (TIER == kF32) {
+                        if constexpr (F32) {
                             float b1, b2;
                             slp_pair2_f32((float)dx, (float)dy, (float)r2, (float)xi2,
                                           (float)g.x, (float)g.y, (float)fi1, (float)fi2, c1, c2,
@@ -695,8 +703,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                             atomicAdd(&racc[idx].y, (double)b2);
                         } else {
                             double b1, b2;
-                            slp_pair2<TIER, false>(dx, dy, r2, xi2, g.x, g.y, fi1, fi2, a1, a2,
-                                                   b1, b2);
+                            slp_pair2<fp64_tier(TIER), false>(dx, dy, r2, xi2, g.x, g.y, fi1,
+                                                              fi2, a1, a2, b1, b2);
                             atomicAdd(&racc[idx].x, b1);
                             atomicAdd(&racc[idx].y, b2);
                         }
@@ -714,13 +722,13 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     const double r2 = dx * dx + dy * dy;
                     if (KIND == SE2P_SLP) {
                         const double2 g = bfd[idx];
-                        if constexpr (TIER == kF32)
+                        if constexpr (F32)
                             slp_pair_f32((float)dx, (float)dy, (float)r2, (float)xi2, (float)g.x,
                                          (float)g.y, c1, c2);
                         else
-                            slp_pair<TIER, false>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
+                            slp_pair<fp64_tier(TIER), false>(dx, dy, r2, xi2, g.x, g.y, a1, a2);
                     } else if (KIND == SE2P_DLP) {
-                        if constexpr (TIER == kF32)
+                        if constexpr (F32)
                             dlp_pair_f32((float)dx, (float)dy, (float)r2, (float)xi2,
                                          reinterpret_cast<const DlpRec&>(s), c1, c2);
                         else
@@ -734,14 +742,20 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         derived_pair<KIND>(dx, dy, r2, xi2, reinterpret_cast<const double*>(&s), acc);
                     }
                 }
-                if constexpr (TIER == kF32) {
+                };
+                if constexpr (KIND == SE2P_SLP && TIER == kE1Fast) {
+                    far_loop(fm[0] & ~g32[0], fm[1] & ~g32[1], std::false_type{});
+                    far_loop(g32[0], g32[1], std::true_type{});
+                } else {
+                    far_loop(fm[0], fm[1], std::integral_constant<bool, TIER == kF32>{});
+                }
+                if constexpr (TIER == kF32 || (TIER == kE1Fast && KIND == SE2P_SLP)) {
                     // the chunk's FP32 partial sums join the FP64 sums exactly
                     a1 += (double)c1;
                     a2 += (double)c2;
                 }
                 if (KIND == SE2P_SLP) {
-                    lo = nm[0];
-                    hi = nm[1];
+                    unsigned long long lo = nm[0], hi = nm[1];
                     while (lo | hi) {
                         const int idx = staged(next(lo, hi));
                         const double2 q = rec_xy(idx), g = bfd[idx];

@@ -246,6 +246,8 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         // lies within max(rc, cell height) in y; the margin covers the 2^-20
         // resolution of the in-cell y order
         d->trim_w = std::max(std::sqrt(d->cut2), G.Ly / nsy) * (1.0 + 1e-5);
+        d->far32_r2 = G.e1tier == kE1Fast && G.prm.far32 > 0 ? G.prm.far32 / (xi * xi)
+                                                              : HUGE_VAL;
         d->unsorted = G.unsorted;
         d->sym = G.sym ? 1 : 0;
         d->ykey = G.ykey;

@@ -235,6 +235,9 @@ struct SysDesc {
     // half-width in y of the candidates a target keeps from each stencil column
     // (sources y-ordered in their cells); unusable while *unsorted != 0
     double trim_w;
+    // fast E1 tier: pairs with r^2 >= far32_r2 (xi^2 r^2 >= se2p_params.far32)
+    // are evaluated in FP32; +inf when off
+    double far32_r2;
     const int* unsorted;
     // pair-symmetric sums (sources are the targets, same sorted order): ykey
     // holds the in-cell y key of each sorted source, sacc the sums in sorted order

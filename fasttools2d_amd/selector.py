@@ -40,7 +40,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 from .plan import DeviceArray, Plan
-from .tune import _aspect, make_params, precision_for, rel_err, smooth_up
+from .tune import _aspect, far32_for, make_params, precision_for, rel_err, smooth_up
 from .workloads import clusters, density_moments, normal_density, uniform_points, unit_normals
 
 N_IN, N_HID, N_OUT = 8, 64, 4
@@ -83,8 +83,9 @@ def params_from_outputs(y, kind, n, tol, Lx, Ly, max_grid=4096, symmetric=0):
     except ValueError:  # no smooth size between want and the cap: the largest below
         M0 = max(m for m in range(P + 2, cap + 1, 2) if _smooth(m))
     tier = 1 if tol >= 1e-10 else 0
-    return make_params(xi, a * A, a * B, A * M0, B * M0, P, Lx, tier, symmetric,
-                       precision_for(tol))
+    prec = precision_for(tol)
+    return make_params(xi, a * A, a * B, A * M0, B * M0, P, Lx, tier, symmetric, prec,
+                       far32_for(tol) if tier == 1 and prec == 0 else 0.0)
 
 
 # ---------------------------------------------------------------- network on the GPU
@@ -282,16 +283,21 @@ def safer(prm, Lx, Ly, tol):
     cap = 4096 // max(A, B)
     M0 = min(cap, smooth_up(min(cap, 1.1 * prm["Mx"] / A), cap))
     return make_params(1.1 * prm["xi"], prm["nside_x"], prm["nside_y"], A * M0, B * M0, P, Lx,
-                       prm["e1_tier"], prm["symmetric"], prm["precision"])
+                       prm["e1_tier"], prm["symmetric"], prm["precision"], prm.get("far32", 0.0))
 
 
 def select(ctx, net, kind, points, Lx, Ly, tol, nrm=None, check_seed=99, symmetric=1,
-           margin=0.35):
+           margin=0.5, draws=4):
     """(params, report) for a system whose targets are its sources.  The
-    network's choice is checked a posteriori: its result for a random density
-    must lie within margin * tol of the analytic recipe's (whose own error is
-    far below tol); the margin covers how much another density's error can
-    exceed this one's.  If it fails, a safer variant of the same split
+    network's choice is checked a posteriori against the analytic recipe (whose
+    own error is far below tol) for a random density g.  A relative error
+    ||E f|| / ||S f|| (E the parameters' error operator, S the sum) varies
+    with the density mostly through ||S f||, which the few lowest Fourier modes
+    of f dominate; ||E f|| / ||f|| is nearly the same for every random f.  So
+    the check bounds the error of any random density by
+    (||E g|| / ||g||) / min_f (||S f|| / ||f||), the minimum over g and `draws`
+    more N(0,1) densities applied with the same plan, and accepts it if that is
+    at most margin * tol.  If it fails, a safer variant of the same split
     (safer()) is checked the same way, and if that fails too the analytic
     recipe is used."""
     n = len(points)
@@ -299,20 +305,31 @@ def select(ctx, net, kind, points, Lx, Ly, tol, nrm=None, check_seed=99, symmetr
     sym = symmetric if k == _lib.SE2P_SLP else 0
     y = net.predict(ctx, features(kind, n, tol, Lx, Ly, points))[0]
     prm = params_from_outputs(y, kind, n, tol, Lx, Ly, symmetric=sym)
-    g = np.random.default_rng(check_seed).standard_normal((n, 2))
+    rng = np.random.default_rng(check_seed)
+    dens = [rng.standard_normal((n, 2)) for _ in range(1 + draws)]
+    g = dens[0]
     ref = analytic_params(kind, n, tol, Lx, Ly, g)
     ref["symmetric"] = sym
 
-    def run(q):
+    def run(q, ds):
         pl = Plan(ctx, kind, points, points, Lx, Ly, q, nrm=nrm)
         try:
-            return pl.apply(g)
+            return [pl.apply(d) for d in ds]
         finally:
             pl.close()
 
-    u_ref = run(ref)
-    e = rel_err(run(prm), u_ref)
-    rep = {"check_err": e, "predicted": prm, "fallback": False, "safer": False}
+    u_ref = run(ref, [g])[0]
+
+    def bound(q):
+        us = run(q, dens)
+        e_abs = np.linalg.norm(us[0] - u_ref) / np.linalg.norm(g)
+        s_min = min([np.linalg.norm(u_ref) / np.linalg.norm(g)] +
+                    [np.linalg.norm(u) / np.linalg.norm(d) for u, d in zip(us[1:], dens[1:])])
+        return float(e_abs / max(s_min, 1e-300)), rel_err(us[0], u_ref)
+
+    e, e_g = bound(prm)
+    rep = {"check_err": e, "check_err_g": e_g, "predicted": prm, "fallback": False,
+           "safer": False}
     if e <= margin * tol:
         return prm, rep
     try:
@@ -320,7 +337,7 @@ def select(ctx, net, kind, points, Lx, Ly, tol, nrm=None, check_seed=99, symmetr
     except ValueError:
         prm2 = None
     if prm2 is not None:
-        e2 = rel_err(run(prm2), u_ref)
+        e2, _ = bound(prm2)
         rep["check_err_safer"] = e2
         if e2 <= margin * tol:
             rep["safer"] = True

@@ -76,11 +76,25 @@ def eta_w(xi, P, L, M):
     return (2 * xi * w / m) ** 2, w
 
 
-def make_params(xi, nside_x, nside_y, Mx, My, P, Lx, e1_tier=0, symmetric=0, precision=0):
+def far32_for(tol):
+    """se2p_params.far32 for a tolerance: FP32 far pairs from xi^2 r^2 =
+    ln(1e-6 / tol) on, where a pair's term is below 1e6 tol of the largest.
+    FP32 evaluation (the exponential's argument rounded to FP32 included) is
+    good to about 1e-6 relative per term, and the output moves by about
+    5e-10 e^{-far32} relative at N = 1e6 (measured), 5e-4 tol.  Off below 1e-10
+    (the full E1 tier) and in the FP32 tier (all far pairs FP32)."""
+    if tol < 1e-10 or tol >= FP32_MIN_TOL:
+        return 0.0
+    return math.log(1e-6 / tol)
+
+
+def make_params(xi, nside_x, nside_y, Mx, My, P, Lx, e1_tier=0, symmetric=0, precision=0,
+                far32=0.0):
     eta, w = eta_w(xi, P, Lx, Mx)
     return {"xi": xi, "rc": Lx / nside_x, "w": w, "eta": eta, "nside_x": nside_x,
             "nside_y": nside_y, "Mx": Mx, "My": My, "P": P, "kinf": Mx // 2, "converged": 1,
-            "e1_tier": e1_tier, "symmetric": symmetric, "precision": precision}
+            "e1_tier": e1_tier, "symmetric": symmetric, "precision": precision,
+            "far32": far32}
 
 
 FP32_MIN_TOL = 1e-6
@@ -232,7 +246,8 @@ def autotune(ctx, kind, ps, pt, f, Lx, Ly, tol, nrm=None, u_ref=None, log=None,
             M0 = smooth_up(c_k * xi * Lx / math.pi / A, 4096)
             if M1 is not None:
                 M0 = M1
-            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym, pr)
+            return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, sym, pr,
+                               far32_for(tol) if tier == 1 and pr == 0 else 0.0)
 
         def fits(prm, cap=None):
             cap = max_grid if cap is None else cap
@@ -470,8 +485,9 @@ def rule_params(kind, Nsrc, Ntar, Lx, Ly, tol, per_cell=22.0, max_grid=4096, mar
             raise ValueError(f"no grid within {max_grid} per side for tol {tol}")
         a -= 1
     prec = precision_for(tol) if precision is None else int(precision)
-    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, 1 if tol >= 1e-10 else 0, symmetric,
-                       prec)
+    tier = 1 if tol >= 1e-10 else 0
+    return make_params(xi, nsx, nsy, A * M0, B * M0, P, Lx, tier, symmetric, prec,
+                       far32_for(tol) if tier == 1 and prec == 0 else 0.0)
 
 
 def recipe_params(kind, Ns, Nt, Q, Lx, Ly, tol, P=24, max_grid=4096):

@@ -132,6 +132,11 @@ typedef struct {
      * target's pairs in a fixed order.  Ignored for other kinds and inputs. */
     int symmetric;
     int precision; /* SE2P_FP64 or SE2P_FP32C */
+    /* Fast E1 tier of the Stokeslet (e1_tier 1, FP64): real-space pairs with
+     * xi^2 r^2 >= far32 are evaluated in FP32 (their terms are below e^{-far32}
+     * of the largest, so FP32 rounding stays that far below the tolerance) and
+     * added per chunk to the FP64 sums; 0 evaluates every pair in FP64. */
+    double far32;
 } se2p_params;
 
 /* ------------------------------------------------------------------------ */

@@ -39,7 +39,7 @@ def test_selector_falls_back_when_the_check_fails(gpu_ctx):
     c = S.random_configs(1, 77, nmin=2e4, nmax=3e4, tmin=1e-9, tmax=1e-9)[0]
     prm, rep = S.select(gpu_ctx, net, c["kind"], c["points"], c["Lx"], c["Ly"], c["tol"],
                         nrm=c["nrm"])
-    ok = rep["check_err"] <= 0.35 * c["tol"]
+    ok = rep["check_err"] <= 0.5 * c["tol"]
     assert ok or rep["safer"] or rep["fallback"]
     if rep["fallback"]:
         assert prm["nside_x"] == S.analytic_params(c["kind"], c["N"], c["tol"], c["Lx"], c["Ly"],
