@@ -133,11 +133,14 @@ def time_steps(ctx, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def measured_traffic(kernel, prm):
-    """HBM bytes per launch of `kernel` at parameters `prm`, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (each
-    counter scaled by what it reads for a device copy of known size), or None
-    when no record matches these parameters."""
+def measured_traffic(kernel, prm, kind="slp"):
+    """HBM bytes per launch of stage `kernel` at parameters `prm`, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json
+    (each counter scaled by what it reads for a device copy of known size), or
+    None when no record matches.  A real-space record matches the split (nside),
+    E1 tier, pair-symmetric flag and whether FP32 far pairs are on -- what sets
+    its staged sources and atomics -- within 0.3 of its xi rc; a spreading or
+    gathering record matches the grid and the support."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     try:
         with open(path) as fh:
@@ -146,10 +149,16 @@ def measured_traffic(kernel, prm):
         return None
     for r in recs:
         q = r["params"]
-        if (r["stage"] == kernel and q["nside"] == prm["nside_x"] and q["M"] == prm["Mx"]
-                and q["P"] == prm["P"] and q["tier"] == prm["e1_tier"]
-                and q.get("symmetric", 0) == prm.get("symmetric", 0)
-                and abs(q["c_r"] * q["nside"] - prm["xi"]) < 1e-6 * prm["xi"]):
+        if r["stage"] != kernel or r.get("kind", "slp") != kind:
+            continue
+        if kernel == "real":
+            ok = (q["nside"] == prm["nside_x"] and q["tier"] == prm["e1_tier"]
+                  and q.get("symmetric", 0) == prm.get("symmetric", 0)
+                  and (q.get("far32", 0) > 0) == (prm.get("far32", 0) > 0)
+                  and abs(q["c_r"] - prm["xi"] * prm["rc"]) < 0.3)
+        else:
+            ok = q["M"] == prm["Mx"] and q["P"] == prm["P"]
+        if ok:
             return r["hbm_bytes"]
     return None
 
@@ -508,7 +517,8 @@ def dlp_bench(args):
            "stage_ms": stages, "work_model_flops": wm,
            "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved,
                         "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
+                        "frac": achieved / MI355X_FP64_TFLOPS,
+                        "traffic": measured_traffic(dom, prm, kind="dlp"),
                         "per_launch_flops": wm[dom], "avg_launch_ms": stages[dom]},
            "tune_report": rep}
     if not args.no_cpu:

@@ -136,7 +136,8 @@ __device__ __forceinline__ void dft8(V* v, decltype(V::x) sgn)
 // inputs are held in registers across the stage's barrier, so PPT sets the
 // registers and with them how many lines a CU holds at once):
 //   class 0: n <= 2048, T = 256, PPT = 8;  class 1: n <= 2560, T = 256, PPT = 10;
-//   class 2: n <= kFftMaxLen, T = 512, PPT = 8.
+//   class 2: n <= 4096, T = 512, PPT = 8;  class 3: n <= kFftMaxLen (10240),
+//   T = 1024, PPT = 10, one line per CU in up to 160 KiB of LDS.
 
 // q = j / d, r = j % d for 0 <= j < 2^24, from a float reciprocal and one
 // correction step (no integer division in the butterfly loops).
@@ -475,6 +476,10 @@ __global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ s
     }
 }
 
+// Lines above kFftMaxLenSmall need more than 64 KiB of LDS per workgroup; set
+// when the device accepts that for the class-3 kernels.
+bool g_long_lines = false;
+
 }  // namespace
 
 int fft_lines_per_block(int n) { return n >= 256 ? 1 : (256 + n - 1) / n; }
@@ -488,6 +493,13 @@ int fft_axis_init(FftAxis* a, int n)
 {
     if (n < 1 || n > kFftMaxLen)
         return set_error(SE2P_ERR_UNSUPPORTED, "FFT length %d outside [1, %d]", n, kFftMaxLen);
+    if (n > kFftMaxLenSmall) {
+        SE2P_TRY(fft_prepare());
+        if (!g_long_lines)
+            return set_error(SE2P_ERR_UNSUPPORTED,
+                             "FFT length %d: lines above %d need 160 KiB of LDS per workgroup, "
+                             "which this device refused", n, kFftMaxLenSmall);
+    }
     a->n = n;
     a->nst = 0;
     // The 2-3-5-smooth part of n in as few in-register stages as possible (each
@@ -559,13 +571,21 @@ int fft_prepare()
                                  (const void*)k_fft_lines<512, 8, false, float>};
         for (const void* k : kernels)
             SE2P_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               kFftMaxLen * (int)sizeof(double2)));
+                                               kFftMaxLenSmall * (int)sizeof(double2)));
+        const hipError_t e1 = hipFuncSetAttribute(
+            (const void*)k_fft_lines<1024, 10, false, double>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, kFftMaxLen * (int)sizeof(double2));
+        const hipError_t e2 = hipFuncSetAttribute(
+            (const void*)k_fft_lines<1024, 10, false, float>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, kFftMaxLen * (int)sizeof(float2));
+        g_long_lines = e1 == hipSuccess && e2 == hipSuccess;
+        if (!g_long_lines) (void)hipGetLastError();  // lines above 4096 stay unsupported
         attr_set = true;
     }
     return SE2P_OK;
 }
 
-int fft_line_class(int n) { return n <= 2048 ? 0 : n <= 2560 ? 1 : 2; }
+int fft_line_class(int n) { return n <= 2048 ? 0 : n <= 2560 ? 1 : n <= 4096 ? 2 : 3; }
 
 template <typename S>
 static void fft_lines_launch(hipStream_t s, int axis, int sel, int inverse, int cls,
@@ -577,9 +597,12 @@ static void fft_lines_launch(hipStream_t s, int axis, int sel, int inverse, int 
     else if (cls == 1)
         hipLaunchKernelGGL((k_fft_lines<256, 10, true, S>), dim3(njobs), dim3(256), max_lds, s, sys,
                            jobs, axis, sel, inverse);
-    else
+    else if (cls == 2)
         hipLaunchKernelGGL((k_fft_lines<512, 8, false, S>), dim3(njobs), dim3(512), max_lds, s, sys,
                            jobs, axis, sel, inverse);
+    else
+        hipLaunchKernelGGL((k_fft_lines<1024, 10, false, S>), dim3(njobs), dim3(1024), max_lds, s,
+                           sys, jobs, axis, sel, inverse);
 }
 
 int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls,

@@ -203,7 +203,10 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab);
 // ---------------------------------------------------------------------------
 // FFT axes (fft.hip)
 // ---------------------------------------------------------------------------
-constexpr int kFftMaxLen = 4096;
+// Lines up to 4096 points use 64 KiB of LDS per workgroup; longer lines (up to
+// kFftMaxLen, 160 KiB for FP64) need the device's full LDS per workgroup.
+constexpr int kFftMaxLen = 10240;
+constexpr int kFftMaxLenSmall = 4096;
 constexpr int kFftMaxStages = 24;
 struct FftAxis {
     int n;
@@ -335,7 +338,7 @@ int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* j
 // Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
 // x (axis 1: Mx-lines of the transposed grid); every job's line length must be of
 // class `cls` (fft_line_class); max_lds from fft_lds_bytes.
-constexpr int kFftClasses = 3;
+constexpr int kFftClasses = 4;
 int fft_line_class(int n);
 int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs);

@@ -494,7 +494,8 @@ def recipe_params(kind, Ns, Nt, Q, Lx, Ly, tol, P=24, max_grid=4096):
     """The reference recipe (periodic_ewald.ewald_params) with its points-per-cell
     option raised until the grid fits max_grid per side (larger cells, smaller
     xi, coarser grid): wide boxes at tight tolerances otherwise ask for grids
-    beyond the FFT's longest line."""
+    beyond the FFT's longest line (10240 points), and grids above 4096 per side
+    run their lines one per CU."""
     from .periodic_ewald import ewald_params
     prm = ewald_params(kind, Ns, Nt, Q, Lx, Ly, P=P, tol=tol)
     nb = (Ns + Nt) / (prm["nside_x"] * prm["nside_y"])
@@ -509,8 +510,9 @@ def recipe_params(kind, Ns, Nt, Q, Lx, Ly, tol, P=24, max_grid=4096):
 
 def capped_params(n, Lx, Ly, tol, P=24):
     """Reference recipe for an SLP system with sources = targets, its points-per-
-    cell option raised until the grid fits the library's 4096-point FFT lines
-    (larger cells, smaller xi, coarser grid)."""
+    cell option raised until the grid fits 4096 per side (larger cells, smaller
+    xi, coarser grid): lines up to 4096 points run several to a CU, longer ones
+    (up to 10240) one per CU, too slow for a serving batch."""
     from .periodic_ewald import ewald_params
     prm = ewald_params("slp", n, n, 2.0 * n + 1, Lx, Ly, tol=tol, P=P)
     nb = n / (prm["nside_x"] * prm["nside_y"])

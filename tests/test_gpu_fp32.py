@@ -53,7 +53,7 @@ def test_fp32_spread_matches_oracle(ctx32, oracle, N, Mx, My, P):
 
 
 @pytest.mark.parametrize("My,Mx", [(64, 64), (366, 732), (2160, 24), (24, 2560), (3000, 6),
-                                   (902, 8)])
+                                   (902, 8), (10240, 2), (4, 6000)])
 def test_fp32_fft2_matches_numpy(ctx32, My, Mx):
     rng = np.random.default_rng(My * 7 + Mx)
     Z = rng.standard_normal((My, Mx)) + 1j * rng.standard_normal((My, Mx))

@@ -13,7 +13,9 @@ import numpy as np
 import pytest
 
 import fasttools2d_amd as ft
+from fasttools2d_amd import _lib
 from fasttools2d_amd import periodic_ewald as pe
+from fasttools2d_amd.workloads import unit_normals
 from tests.inputs import circles, normal_density, uniform_points
 
 pytestmark = pytest.mark.gpu
@@ -141,7 +143,11 @@ def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
                                    (2160, 24), (24, 2560), (2250, 6), (2304, 10), (2400, 12),
                                    (2106, 4),
                                    # longer lines (512 threads, radices up to 8)
-                                   (3000, 6), (8, 4096), (3072, 2), (2592, 8)])
+                                   (3000, 6), (8, 4096), (3072, 2), (2592, 8),
+                                   # lines above 4096 points (1024 threads, one line per
+                                   # CU in up to 160 KiB of LDS; 10240 is the longest)
+                                   (4098, 2), (6, 6000), (10240, 2), (2, 10240), (8192, 4),
+                                   (4, 10000), (5120, 6), (7, 4374)])
 def test_fft2_matches_numpy(gpu_ctx, My, Mx):
     rng = np.random.default_rng(My + Mx)
     Z = rng.standard_normal((My, Mx)) + 1j * rng.standard_normal((My, Mx))
@@ -214,6 +220,34 @@ def test_slp_kspace_matches_oracle(gpu_ctx, oracle, Ns, Nt, Mx, My, Lx, Ly, xi, 
     got = pe.mex_stokes_slp_kspace(ps.T, pt.T, xi, eta, f.T, Mx, My, Lx, Ly, w, P)
     ref = oracle.slp_kspace(ps, pt, xi, eta, f, Mx, My, Lx, Ly, w, P)
     assert _rel(got.T, ref) < RTOL
+
+
+@pytest.mark.parametrize("Lx,Ly", [(4.0, 1.0), (1.0, 4.0)])
+def test_kspace_long_lines_match_oracle(gpu_ctx, oracle, Lx, Ly):
+    """A 4 x 1 box with a 5120 x 1280 grid (and its transpose): the long side's
+    lines run the longest-line class through spreading, both transforms, the
+    filter and gathering, for the Stokeslet and the stresslet."""
+    Mx, My = (5120, 1280) if Lx > Ly else (1280, 5120)
+    xi, P, Ns, Nt = 300.0, 16, 3000, 1500
+    eta, w = _eta_w(xi, P, Lx, Mx)
+    ps = uniform_points(Ns, Lx, Ly, seed=41)
+    pt = uniform_points(Nt, Lx, Ly, seed=42)
+    f = normal_density(Ns, seed=43)
+    got = pe.mex_stokes_slp_kspace(ps.T, pt.T, xi, eta, f.T, Mx, My, Lx, Ly, w, P)
+    ref = oracle.slp_kspace(ps, pt, xi, eta, f, Mx, My, Lx, Ly, w, P)
+    assert _rel(got.T, ref) < RTOL
+    n = unit_normals(Ns, seed=44)
+    got = pe.mex_stokes_dlp_kspace(ps.T, pt.T, xi, eta, f.T, n.T, Mx, My, Lx, Ly, w, P)
+    ref = oracle.dlp_kspace(ps, pt, xi, eta, f, n, Mx, My, Lx, Ly, w, P)
+    assert _rel(got.T, ref) < RTOL
+
+
+def test_grid_beyond_longest_line_is_refused(gpu_ctx):
+    ps = uniform_points(10, 1.0, 1.0, seed=1).T
+    f = normal_density(10, seed=2).T
+    with pytest.raises(ft.Se2pError) as e:
+        pe.mex_stokes_slp_kspace(ps, ps, 10.0, 0.5, f, 10242, 10242, 1.0, 1.0, 0.1, 24)
+    assert e.value.code == _lib.SE2P_ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("Ns,Nt,Mx,My,Lx,Ly,xi,P", KS_CASES)
