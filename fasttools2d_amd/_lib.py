@@ -32,6 +32,9 @@ SE2P_DLP_VORTICITY = 10
 SE2P_DLP_STRESS = 11
 SE2P_FP64 = 0
 SE2P_FP32C = 1
+SE2P_FREE_AUTO = 0
+SE2P_FREE_DIRECT = 1
+SE2P_FREE_EWALD = 2
 
 _KINDS = {"slp": SE2P_SLP, "dlp": SE2P_DLP, "slp_pressure": SE2P_SLP_PRESSURE,
           "dlp_pressure": SE2P_DLP_PRESSURE, "slp_gradient": SE2P_SLP_GRADIENT,
@@ -82,6 +85,7 @@ EXPORTS = (
 ) + tuple(f"se2p_{layer}_{q}_{part}" for layer in ("slp", "dlp")
           for q in ("gradient", "pressure_grad", "vorticity", "stress")
           for part in ("real", "kspace")) + ("se2p_free_slp", "se2p_free_dlp",
+                                                  "se2p_free_params", "se2p_ctx_set_free_method",
                                                   "se2p_mlp_forward", "se2p_mlp_loss_grad",
                                                   "se2p_mlp_adam")
 
@@ -103,7 +107,7 @@ class Params(ctypes.Structure):
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
                 ("kinf", ctypes.c_int), ("converged", ctypes.c_int), ("e1_tier", ctypes.c_int),
                 ("symmetric", ctypes.c_int), ("precision", ctypes.c_int),
-                ("far32", ctypes.c_double)]
+                ("far32", ctypes.c_double), ("free_R", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -201,6 +205,8 @@ def lib():
                for q in ("pressure_grad", "vorticity")},
             "se2p_free_slp": [P, P, P, P, P, I, P, P, I, I, I, P, P],
             "se2p_free_dlp": [P, P, P, P, P, P, P, I, P, P],
+            "se2p_free_params": [I, I, D, D, D, P, P, P],
+            "se2p_ctx_set_free_method": [P, I],
             "se2p_mlp_forward": [P, P, P, I, P],
             "se2p_mlp_loss_grad": [P, P, P, P, I, P, P],
             "se2p_mlp_adam": [P, P, P, P, P, I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -259,6 +265,11 @@ class Context:
 
     def sync(self):
         check(lib().se2p_ctx_sync(self.handle))
+
+    def set_free_method(self, method):
+        """SE2P_FREE_AUTO (0), SE2P_FREE_DIRECT (1) or SE2P_FREE_EWALD (2) for
+        this context's free-space Stokeslet sums (stokesSLPfmm)."""
+        check(lib().se2p_ctx_set_free_method(self.handle, int(method)))
 
     def set_precision(self, precision):
         """SE2P_FP64 (0) or SE2P_FP32C (1) for this context's one-shot calls (the

@@ -219,6 +219,8 @@ int check_params(const se2p_params* prm, double Lx, double Ly)
     if (prm->precision != SE2P_FP64 && prm->precision != SE2P_FP32C)
         return set_error(SE2P_ERR_ARG, "precision must be SE2P_FP64 or SE2P_FP32C");
     if (!(prm->far32 >= 0)) return set_error(SE2P_ERR_ARG, "far32 must be >= 0");
+    if (!(prm->free_R >= 0) || !std::isfinite(prm->free_R))
+        return set_error(SE2P_ERR_ARG, "free_R must be finite and >= 0");
     return SE2P_OK;
 }
 
@@ -1063,6 +1065,156 @@ int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar
 // Free-space layer potentials (the FMM mex functions, computed directly)
 // ---------------------------------------------------------------------------
 
+// Smallest even 2-3-5-smooth integer >= n.
+static int even_smooth_up(int n)
+{
+    for (int m = std::max(2, n + (n & 1));; m += 2) {
+        int r = m;
+        for (int p : {2, 3, 5})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+int se2p_free_params(int Nsrc, int Ntar, double wx, double wy, double tol, se2p_params* prm,
+                     double* Lx, double* Ly)
+{
+    if (!prm || !Lx || !Ly) return set_error(SE2P_ERR_ARG, "null output");
+    if (Nsrc < 0 || Ntar < 0 || Nsrc + Ntar < 1) return set_error(SE2P_ERR_ARG, "no points");
+    if (!(wx >= 0) || !(wy >= 0) || !std::isfinite(wx) || !std::isfinite(wy))
+        return set_error(SE2P_ERR_ARG, "box sides must be finite and >= 0");
+    if (!(tol >= 1e-13) || !(tol < 1)) return set_error(SE2P_ERR_ARG, "tol must be in [1e-13, 1)");
+    // a degenerate extent (points on a line, or one point) gets a box of its own
+    double W = std::max(wx, wy);
+    if (!(W > 0)) W = 1;
+    wx = std::max(wx, 1e-3 * W);
+    wy = std::max(wy, 1e-3 * W);
+    // the periodic tuner's tolerance rules (tune.rule_params) with a margin of
+    // 100 rather than 10: their real-space truncation exp(-(xi rc)^2) alone is
+    // e^{3.2}/margin of the tolerance, which points on curves (few neighbours
+    // within rc, sums that do not grow with the density's mean) reach
+    const double L = std::log(100 / tol);
+    const double cr = std::sqrt(std::max(L - 3.2, 1.0));
+    const int P = std::min(24, std::max(8, 2 * (int)std::ceil(0.4 * L + 0.6)));
+    const double npts = std::max(Nsrc, Ntar);
+    // cells of about 22 points of the point box, at most the box
+    double rc = std::min(std::sqrt(22.0 * wx * wy / npts), std::max(wx, wy));
+    double xi, h, delta, R;
+    int Mx, My;
+    for (;;) {
+        xi = cr / rc;
+        // the screened truncated kernel differs from the free-space one by the
+        // jump of the truncated kernel at R times the screening function's
+        // tail, ~ (xi delta)^2 exp(-(xi delta)^2) at distance delta from it
+        const double Dg = std::hypot(wx, wy);
+        delta = std::sqrt(L + 2 * std::log(std::max(xi * (Dg + 1.0 / xi), 1.0))) / xi;
+        R = Dg + delta;
+        // k_inf = ck xi = pi / h.  The truncated kernel's transform keeps an
+        // oscillating tail, R J1(K R)/(2 K^3) times K^2, of relative size
+        // ~ (R K)^{1/2} against the periodic 1/K^2, so the Gaussian must fall
+        // further at k_inf than the periodic rule's exp(-(L - 2.4)): by
+        // (R k_inf)^{1/2}.
+        double ck = 2 * std::sqrt(std::max(L - 2.4, 1.0));
+        for (int it = 0; it < 3; ++it)
+            ck = 2 * std::sqrt(std::max(L - 2.4 + 0.5 * std::log(std::max(R * ck * xi, 1.0)), 1.0));
+        h = kPi / (ck * xi);
+        Mx = even_smooth_up((int)std::ceil((wx + R + delta) / h));
+        My = even_smooth_up((int)std::ceil((wy + R + delta) / h));
+        if (std::max(Mx, My) <= kFftMaxLen) break;
+        rc *= 1.25;  // coarser grid, more pairs in real space
+    }
+    *Lx = Mx * h;
+    *Ly = My * h;
+    const int nsx = std::max(1, (int)std::floor(*Lx / rc)), nsy = std::max(1, (int)std::floor(*Ly / rc));
+    *prm = se2p_params{};
+    prm->P = P;
+    prm->nside_x = nsx;
+    prm->nside_y = nsy;
+    prm->rc = *Lx / nsx;
+    prm->xi = xi;
+    prm->kinf = Mx / 2;
+    prm->Mx = Mx;
+    prm->My = My;
+    prm->w = P * h / 2;
+    prm->eta = std::pow(2 * xi * prm->w / (0.95 * std::sqrt(kPi * P)), 2);
+    prm->converged = 1;
+    prm->e1_tier = tol >= 1e-10 ? 1 : 0;
+    prm->symmetric = 0;
+    prm->precision = SE2P_FP64;
+    prm->far32 = prm->e1_tier == 1 && tol < 1e-6 ? std::log(1e-6 / tol) : 0;
+    prm->free_R = R;
+    return SE2P_OK;
+}
+
+// FMM accuracy flag -> relative tolerance (cfmm2dpart.f:351-361), kept within
+// what the spectral Ewald sum reaches.
+static double free_tol(int iprec)
+{
+    static const double eps[] = {5e-3, 5e-3, 5e-3, 5e-4, 5e-7, 5e-10, 5e-13};
+    return eps[std::min(std::max(iprec, -2), 4) + 2];
+}
+
+// The free-space Stokeslet sum by spectral Ewald: the points centred in the box
+// of se2p_free_params, one evaluation (pair-symmetric real space when the
+// targets are the sources).
+static int free_slp_ewald(se2p_ctx* ctx, const double* f1, const double* f2, const double* xs,
+                          const double* ys, int Ns, const double* xt, const double* yt, int Nt,
+                          bool same, double tol, double* u1, double* u2)
+{
+    double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
+    auto extend = [&](const double* x, const double* y, int n) {
+        for (int i = 0; i < n; ++i) {
+            x0 = std::min(x0, x[i]);
+            x1 = std::max(x1, x[i]);
+            y0 = std::min(y0, y[i]);
+            y1 = std::max(y1, y[i]);
+        }
+    };
+    extend(xs, ys, Ns);
+    if (!same) extend(xt, yt, Nt);
+    if (!std::isfinite(x1 - x0) || !std::isfinite(y1 - y0))
+        return set_error(SE2P_ERR_ARG, "non-finite point coordinates");
+    se2p_params prm;
+    double Lx, Ly;
+    SE2P_TRY(se2p_free_params(Ns, Nt, x1 - x0, y1 - y0, tol, &prm, &Lx, &Ly));
+    prm.symmetric = same ? 1 : 0;
+    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+    std::vector<double> ps(2 * (size_t)Ns), f(2 * (size_t)Ns), pt(same ? 0 : 2 * (size_t)Nt);
+    for (int i = 0; i < Ns; ++i) {
+        ps[2 * i] = xs[i] - cx;
+        ps[2 * i + 1] = ys[i] - cy;
+        f[2 * i] = f1[i];
+        f[2 * i + 1] = f2[i];
+    }
+    for (int i = 0; i < (same ? 0 : Nt); ++i) {
+        pt[2 * i] = xt[i] - cx;
+        pt[2 * i + 1] = yt[i] - cy;
+    }
+    double *dps, *dpt, *df;
+    SE2P_TRY(upload(ctx, WS_PTS_A, ps.data(), ps.size(), &dps));
+    if (same)
+        dpt = dps;  // one pointer: the pair-symmetric sums apply
+    else
+        SE2P_TRY(upload(ctx, WS_PTS_B, pt.data(), pt.size(), &dpt));
+    SE2P_TRY(upload(ctx, WS_DENS_A, f.data(), f.size(), &df));
+    void* out = nullptr;
+    SE2P_TRY(ws_get(ctx, &ctx->ws, WS_OUT_A, sizeof(double) * 2 * (size_t)Nt, &out));
+    Geom G;
+    SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dps, dpt, nullptr, Ns, Nt, Lx, Ly, prm, true,
+                        true));
+    BatchExec B;
+    const Geom* gp = &G;
+    SE2P_TRY(batch_build(ctx, &ctx->ws, &B, &gp, 1));
+    SE2P_TRY(batch_apply(ctx, B, df, (double*)out));
+    std::vector<double> h(2 * (size_t)Nt);
+    SE2P_TRY(download(ctx, h.data(), (double*)out, h.size()));
+    for (int i = 0; i < Nt; ++i) {
+        u1[i] = h[2 * i];
+        u2[i] = h[2 * i + 1];
+    }
+    return SE2P_OK;
+}
+
 int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
                   const double* ysrc, int Nsrc, const double* xtar, const double* ytar, int Ntar,
                   int iequal, int iprec, double* u1, double* u2)
@@ -1081,6 +1233,14 @@ int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const doubl
     const double* xt = iequal ? xsrc : xtar;
     const double* yt = iequal ? ysrc : ytar;
     if (Ntar > 0 && (!xt || !yt)) return set_error(SE2P_ERR_ARG, "null target array");
+    const bool ewald = iprec <= 4 && Nsrc > 0 && Ntar > 0 &&
+                       (ctx->free_method == SE2P_FREE_EWALD ||
+                        (ctx->free_method == SE2P_FREE_AUTO && (double)Nsrc * Ntar >= 1e9));
+    if (ewald) {
+        SE2P_HIP_CHECK(hipSetDevice(ctx->device));
+        return free_slp_ewald(ctx, f1, f2, xsrc, ysrc, Nsrc, xt, yt, Ntar, iequal == 1,
+                              free_tol(iprec), u1, u2);
+    }
     return free_eval(ctx, false, f1, f2, xsrc, ysrc, nullptr, nullptr, Nsrc, xt, yt, Ntar,
                      iequal == 1, u1, u2);
 }
@@ -1502,6 +1662,15 @@ int se2p_ctx_set_precision(se2p_ctx* ctx, int precision)
     if (precision != SE2P_FP64 && precision != SE2P_FP32C)
         return set_error(SE2P_ERR_ARG, "precision must be SE2P_FP64 or SE2P_FP32C");
     ctx->precision = precision;
+    return SE2P_OK;
+}
+
+int se2p_ctx_set_free_method(se2p_ctx* ctx, int method)
+{
+    if (!ctx) return set_error(SE2P_ERR_ARG, "null context");
+    if (method != SE2P_FREE_AUTO && method != SE2P_FREE_DIRECT && method != SE2P_FREE_EWALD)
+        return set_error(SE2P_ERR_ARG, "free-space method must be SE2P_FREE_AUTO, _DIRECT or _EWALD");
+    ctx->free_method = method;
     return SE2P_OK;
 }
 

@@ -3,7 +3,8 @@
 // Follows mex_stokes_slp_real.cpp:58-357 and mex_stokes_dlp_real.cpp:15-211
 // (PeriodicEwald/src/mex/...): cells of size Lx/nside_x x Ly/nside_y; a target
 // takes every source of its own cell with no cutoff (SLP: coincident points
-// r^2 < 1e-15 take the self term -(1 + gamma/2 + ln xi) f; DLP: coincident
+// r^2 < 1e-15 take the self term -(1 + gamma/2 + ln xi) f -- free-space sums:
+// exactly coincident points only, SysDesc::self_r2; DLP: coincident
 // points are skipped) and every source of the 8 neighbour cells (periodic
 // images) with r^2 < Lx Ly/(nside_x nside_y).  Output scaled by 1/(4 pi) and
 // written in input order.  The Stokeslet pressure
@@ -449,6 +450,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int nround = (total + CAP - 1) / CAP;
     const double trim_w = d.trim_w;
     const double far32_r2 = d.far32_r2;
+    const double self_r2 = d.self_r2;
     const bool trim = !UNI && trim_w > 0 && d.unsorted && *d.unsorted == 0;
 
     for (int tb = 0; tb < ntar; tb += kBlkThreads) {
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         const double2 q = rec_xy(idx), g = bfd[idx];
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
-                        if (idx >= o0 && idx < o1 && fabs(r2) < 1e-15) {
+                        if (idx >= o0 && idx < o1 && fabs(r2) < self_r2) {
                             a1 += self * g.x;
                             a2 += self * g.y;
                             if (SYM) {

@@ -31,14 +31,67 @@ __device__ __forceinline__ void unpack(double2 zk, double2 zm, cplx& a, cplx& b)
 
 // SLP filter at (k1, k2) applied to q = (q1, q2) (mex_stokes_slp_kspace.cpp:130-139):
 //   (K^2 q - k (k.q)) (1/K^4 + 1/(4 xi^2 K^2)) exp(-(1 - eta) K^2/(4 xi^2))
+// The scalar factor e is passed in: the periodic one above, or the free-space
+// one of free_factor.
+__device__ __forceinline__ void slp_apply_e(double k1, double k2, double e, cplx q1, cplx q2,
+                                            cplx& g1, cplx& g2)
+{
+    const double K2 = k1 * k1 + k2 * k2;
+    const cplx kd = cadd(cscale(q1, k1), cscale(q2, k2));
+    g1 = cscale(csub(cscale(q1, K2), cscale(kd, k1)), e);
+    g2 = cscale(csub(cscale(q2, K2), cscale(kd, k2)), e);
+}
+
 __device__ __forceinline__ void slp_apply(double k1, double k2, double a, double eta, cplx q1,
                                           cplx q2, cplx& g1, cplx& g2)
 {
     const double K2 = k1 * k1 + k2 * k2;
     const double e = (1.0 / (K2 * K2) + a / K2) * exp(-(1 - eta) * a * K2);
-    const cplx kd = cadd(cscale(q1, k1), cscale(q2, k2));
-    g1 = cscale(csub(cscale(q1, K2), cscale(kd, k1)), e);
-    g2 = cscale(csub(cscale(q2, K2), cscale(kd, k2)), e);
+    slp_apply_e(k1, k2, e, q1, q2, g1, g2);
+}
+
+// Fourier transform of the biharmonic kernel B(r) = r^2 (ln r - 1)/(8 pi)
+// (Delta^2 B = delta, so its transform is 1/K^4) less the quadratic that
+// matches its value and slope at R, truncated to r < R:
+//   B~(r) = B(r) - B(R) - beta (r^2 - R^2),  beta = B'(R)/(2 R) = (2 ln R - 1)/(16 pi),
+//   B~_R(K) = (1 - J0(K R))/K^4 - R J1(K R)/(2 K^3)
+// (the transform of B's truncation has R^3 J1/K and R^2 J2/K^2 terms that the
+// quadratic's cancels).  B~ and its slope vanish at R, so the tail of B~_R is
+// ~R^{1/2} K^{-7/2}, (R K)^{1/2} times the periodic 1/K^4 instead of the
+// (R K)^{5/2} a plain truncation leaves, and rounding errors are not
+// amplified at large K.  (d_i d_j - delta_ij Lap) of the quadratic is the
+// constant -2 beta delta_ij, which the sum's constant term takes back.  For
+// x = K R < 2 the two terms cancel, and the power series is used instead:
+//   B~_R(K) = R^4/16 sum_{m>=2} (-1)^m (m - 1) (x/2)^{2m-4}/(m!)^2
+// (16 terms: the last is below 1e-30 of the first at x = 2).
+__device__ __forceinline__ double bhat_trunc(double K, double R)
+{
+    const double x = K * R;
+    if (x < 2.0) {
+        const double t = 0.25 * x * x;
+        double s = 0.0, c = 1.0, fac = 2.0;  // fac = m!
+#pragma unroll
+        for (int m = 2; m < 18; ++m) {
+            s += ((m & 1) ? -1.0 : 1.0) * (m - 1) * c / (fac * fac);
+            c *= t;
+            fac *= m + 1;
+        }
+        const double R2 = R * R;
+        return R2 * R2 / 16.0 * s;
+    }
+    const double iK = 1.0 / K, iK2 = iK * iK;
+    return (1.0 - j0(x)) * iK2 * iK2 - 0.5 * R * j1(x) * iK2 * iK;
+}
+
+// Free-space Stokeslet factor: the periodic 1/K^4 (1 + K^2/(4 xi^2)) becomes
+// B~_R(K) (1 + K^2/(4 xi^2)), so that the k-space kernel is the truncated
+// biharmonic's, screened: it equals the free-space k-space kernel at distances
+// below R - delta and vanishes beyond R + delta, and sampled on the periodic
+// grid of a box wider than the points' extent plus R + delta it gives the
+// free-space sum without images.
+__device__ __forceinline__ double free_factor(double K2, double a, double eta, double R)
+{
+    return bhat_trunc(sqrt(K2), R) * (1.0 + a * K2) * exp(-(1 - eta) * a * K2);
 }
 
 // DLP bracket at (k1, k2) (mex_stokes_dlp_kspace.cpp:164-191): with Qs = Q12 + Q21,
@@ -193,8 +246,16 @@ __device__ __forceinline__ void filter_mode(const SysDesc& d, int ix, int iy, do
         cplx q1, q2;
         unpack(wk, wm, q1, q2);
         cplx p1, p2, r1, r2;
-        slp_apply(k1, k2, a, eta, q1, q2, p1, p2);
-        slp_apply(m1, m2, a, eta, q1, q2, r1, r2);
+        if (d.free_R > 0) {
+            // the partner index has the same |K| (its wavenumbers are -k, or k's
+            // mirror on the Nyquist lines)
+            const double e = free_factor(k1 * k1 + k2 * k2, a, eta, d.free_R);
+            slp_apply_e(k1, k2, e, q1, q2, p1, p2);
+            slp_apply_e(m1, m2, e, q1, q2, r1, r2);
+        } else {
+            slp_apply(k1, k2, a, eta, q1, q2, p1, p2);
+            slp_apply(m1, m2, a, eta, q1, q2, r1, r2);
+        }
         g1 = cscale(cadd(p1, r1), 0.5);
         g2 = cscale(cadd(p2, r2), 0.5);
     } else if (KIND == SE2P_SLP_PRESSURE) {

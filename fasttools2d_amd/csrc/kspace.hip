@@ -555,6 +555,10 @@ __global__ __launch_bounds__(256) void k_zero_mode_part(const SysDesc* __restric
             const double* r = rec + 8 * (size_t)i;
             ax += r[6] * r[0];
             ay += r[6] * r[1];
+        } else if (d.kind == SE2P_SLP && d.free_R > 0) {  // free-space Stokeslet: f
+            const double* r = rec + 4 * (size_t)i;
+            ax += r[2];
+            ay += r[3];
         } else if (d.kind == SE2P_DLP_PRESSURE) {  // f.n
             ax += rec[8 * (size_t)i + 6];
         } else if (d.kind == SE2P_SLP_VORTICITY) {  // {x, y, f1, f2}: f1 y - f2 x
@@ -594,6 +598,15 @@ __global__ void k_zero_mode_final(const SysDesc* __restrict__ sys, const Job* __
     if (d.kind == SE2P_DLP) {  // StokesDLP_ewald_2p.m:149-151
         d.zm[0] = sx * inv_area;
         d.zm[1] = sy * inv_area;
+    } else if (d.kind == SE2P_SLP) {
+        // free-space Stokeslet: G = (dd - I lap) B + I/(8 pi) with B the
+        // biharmonic kernel r^2 (ln r - 1)/(8 pi), and (dd - I lap) B =
+        // (dd - I lap) B~ - 2 beta I for the B~ of the filter (B less a
+        // quadratic, beta = (2 ln R - 1)/(16 pi)); the k-space sum carries
+        // (dd - I lap) B~, this adds the constant, (1 - ln R)/(4 pi) sum(f)
+        const double c = (1.0 - log(d.free_R)) / (4 * kPi);
+        d.zm[0] = sx * c;
+        d.zm[1] = sy * c;
     } else {  // StokesSLP_pressure_ewald_2p.m:127-128, StokesDLP_pressure_ewald_2p.m:156-157,
               // StokesSLP_vorticity_ewald_2p.m:145
         d.zm[0] = -sx * inv_area / 2;

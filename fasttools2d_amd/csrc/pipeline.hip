@@ -36,6 +36,9 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                const se2p_params& prm, bool want_real, bool want_k)
 {
     hipStream_t s = ctx->stream;
+    if (prm.free_R > 0 && (kind != SE2P_SLP || !want_real || !want_k))
+        return set_error(SE2P_ERR_UNSUPPORTED,
+                         "free-space sums (free_R > 0) are Stokeslet velocity sums with both parts");
     *G = Geom{};
     G->kind = kind;
     G->Ns = Ns;
@@ -128,7 +131,7 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
-        if (has_zero_mode(kind)) {
+        if (has_zero_mode(kind, prm.free_R)) {
             SE2P_TRY(ws_get(ctx, ws, WS_REDUCE, sizeof(double) * 512, &p));
             G->zm_part = (double*)p;
             SE2P_TRY(ws_get(ctx, ws, WS_ZM, sizeof(double) * 2, &p));
@@ -248,6 +251,8 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->trim_w = std::max(std::sqrt(d->cut2), G.Ly / nsy) * (1.0 + 1e-5);
         d->far32_r2 = G.e1tier == kE1Fast && G.prm.far32 > 0 ? G.prm.far32 / (xi * xi)
                                                               : HUGE_VAL;
+        d->free_R = G.prm.free_R;
+        d->self_r2 = G.prm.free_R > 0 ? 4.9406564584124654e-324 : 1e-15;
         d->unsorted = G.unsorted;
         d->sym = G.sym ? 1 : 0;
         d->ykey = G.ykey;
@@ -364,10 +369,10 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     }
 
     add(B->zm_part, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, d.zm_nblk);
+        if (geoms[i]->has_real && has_zero_mode(d.kind, d.free_R)) blocks(i, d.zm_nblk);
     });
     add(B->zm_sys, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_real && has_zero_mode(d.kind)) blocks(i, 1);
+        if (geoms[i]->has_real && has_zero_mode(d.kind, d.free_R)) blocks(i, 1);
     });
     add(B->sdens, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_k) blocks(i, (d.Ns + 255) / 256);

@@ -45,6 +45,12 @@ __host__ __device__ constexpr bool has_zero_mode(int kind)
     return kind == SE2P_DLP || kind == SE2P_SLP_PRESSURE || kind == SE2P_DLP_PRESSURE ||
            kind == SE2P_SLP_VORTICITY;
 }
+// Systems with a zero-mode term: the kinds above, and free-space Stokeslet sums
+// (the constant sum(f)/(8 pi)).
+__host__ __device__ constexpr bool has_zero_mode(int kind, double free_R)
+{
+    return has_zero_mode(kind) || (kind == SE2P_SLP && free_R > 0);
+}
 // Sorted source records: stresslet {x, y, f1, f2, n1, n2, f.n, 0}, else {x, y, f1, f2}.
 __host__ __device__ constexpr int rec_layout(int kind) { return dlp_family(kind) ? 1 : 0; }
 // Complex grids of the k-space output (two real values per grid).
@@ -132,6 +138,8 @@ struct se2p_ctx {
     char* scratch = nullptr;
     // precision of the one-shot entry points (mex equivalents and drivers)
     int precision = SE2P_FP64;
+    // free-space Stokeslet method (SE2P_FREE_*)
+    int free_method = SE2P_FREE_AUTO;
 };
 
 namespace se2p {
@@ -241,6 +249,13 @@ struct SysDesc {
     // fast E1 tier: pairs with r^2 >= far32_r2 (xi^2 r^2 >= se2p_params.far32)
     // are evaluated in FP32; +inf when off
     double far32_r2;
+    // free-space Stokeslet sum: truncation radius of the biharmonic kernel in
+    // the k-space filter (0: periodic); the zero-mode slot then holds sum(f)/(8 pi)
+    double free_R;
+    // own-cell Stokeslet pairs with r^2 below this take the self term instead:
+    // 1e-15 (the reference's periodic sums), or the least positive double for
+    // free-space sums (which leave out exactly coincident pairs only)
+    double self_r2;
     const int* unsorted;
     // pair-symmetric sums (sources are the targets, same sorted order): ykey
     // holds the in-cell y key of each sorted source, sacc the sums in sorted order

@@ -78,6 +78,15 @@ int se2p_device_count(int* count);
  * se2p_eval_device take theirs from se2p_params.precision.  No counterpart in
  * the reference, which computes in FP64 only. */
 int se2p_ctx_set_precision(se2p_ctx* ctx, int precision);
+/* Method of this context's free-space Stokeslet sums (se2p_free_slp):
+ * SE2P_FREE_AUTO (the default) takes the spectral Ewald sum when Nsrc x Ntar >=
+ * 1e9 and iprec <= 4, direct summation otherwise; SE2P_FREE_DIRECT and
+ * SE2P_FREE_EWALD force one.  No counterpart in the reference, whose FMM has no
+ * such choice. */
+#define SE2P_FREE_AUTO 0
+#define SE2P_FREE_DIRECT 1
+#define SE2P_FREE_EWALD 2
+int se2p_ctx_set_free_method(se2p_ctx* ctx, int method);
 
 /* ------------------------------------------------------------------------ */
 /* The four mex kernels, same argument order, sizes made explicit.          */
@@ -137,6 +146,16 @@ typedef struct {
      * of the largest, so FP32 rounding stays that far below the tolerance) and
      * added per chunk to the FP64 sums; 0 evaluates every pair in FP64. */
     double far32;
+    /* Stokeslet only: > 0 evaluates the free-space (non-periodic) sum instead of
+     * the periodic one.  The k-space filter then uses the Fourier transform of
+     * the biharmonic kernel, less the quadratic matching its value and slope at
+     * free_R, truncated at radius free_R, and (1 - ln free_R) sum(f)/(4 pi) is
+     * added to every target.  Every source and target must lie in a sub-box of
+     * sides wx, wy with free_R >= sqrt(wx^2 + wy^2) + delta, Lx >= wx + free_R +
+     * delta and Ly >= wy + free_R + delta, delta several 1/xi (se2p_free_params
+     * sets all of it); both the real-space and the k-space parts are needed.
+     * 0: the periodic sum. */
+    double free_R;
 } se2p_params;
 
 /* ------------------------------------------------------------------------ */
@@ -279,16 +298,21 @@ int se2p_dlp_stress_kspace(se2p_ctx* ctx, const double* psrc, const double* ptar
 
 /* ------------------------------------------------------------------------ */
 /* Free-space (non-periodic) layer potentials.  The reference evaluates them */
-/* with a Fortran FMM; these compute the same sums directly on the GPU in    */
-/* FP64 (exact to rounding, O(Nsrc Ntar) work).                              */
+/* with a Fortran FMM.  Here: direct sums on the GPU in FP64 (exact to       */
+/* rounding, O(Nsrc Ntar) work), and for the Stokeslet a free-space spectral */
+/* Ewald sum (O(N log N), accurate to the FMM's tolerance for iprec).        */
 /* ------------------------------------------------------------------------ */
 /* [u1, u2] = stokesSLPfmm(f1, f2, xsrc, ysrc, xtar, ytar, iequal, iprec)
  * (FMM/src/StokesSLP/stokesSLPfmm.F:10-149, stokesSLP.f): u = sum_s (-1/2
  * log(rho^2) f + (r.f) r/rho^2)/(4 pi), r = x - x_s (the direct sum of
  * FMM/tests/stokes_SLP_FMM_test.m:29-51).  iequal = 1: the targets are the
  * sources (Ntar == Nsrc; xtar, ytar may be NULL) and each source's own term is
- * left out.  iprec (-2..5) is the FMM's accuracy level, accepted and not
- * needed.  Coincident source-target pairs are skipped. */
+ * left out.  iprec (-2..5) is the FMM's accuracy level (cfmm2dpart.f:351-361:
+ * relative tolerance 0.5 10^-k, k = 0, 1, 2, 3, 6, 9, 12, 15).  Coincident
+ * source-target pairs are skipped.  The context's free-space method
+ * (se2p_ctx_set_free_method) picks direct summation or the free-space spectral
+ * Ewald sum at the tolerance of iprec (at most 5e-3, at least 5e-13; iprec 5
+ * always sums directly). */
 int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
                   const double* ysrc, int Nsrc, const double* xtar, const double* ytar, int Ntar,
                   int iequal, int iprec, double* u1, double* u2);
@@ -299,6 +323,18 @@ int se2p_free_slp(se2p_ctx* ctx, const double* f1, const double* f2, const doubl
 int se2p_free_dlp(se2p_ctx* ctx, const double* f1, const double* f2, const double* xsrc,
                   const double* ysrc, const double* n1, const double* n2, int N, double* u1,
                   double* u2);
+/* Parameters of a free-space spectral Ewald Stokeslet sum (se2p_params.free_R)
+ * for Nsrc sources and Ntar targets inside a wx x wy box at relative tolerance
+ * tol (>= 1e-13): xi, cells, grid and support from the tolerance rules of the
+ * periodic tuner with L = ln(100/tol) (xi rc = sqrt(L - 3.2), about 22 points
+ * per cell of the point box, grids at most 10240 per side) and k_inf/xi =
+ * 2 sqrt(L - 2.4 + 0.5 ln(free_R k_inf)) for the truncated kernel's transform,
+ * delta = sqrt(L + 2 ln(xi D))/xi with D = sqrt(wx^2 + wy^2), free_R = D + delta and the
+ * periodic box *Lx x *Ly = (wx + free_R + delta, wy + free_R + delta) rounded
+ * up to whole grid steps.  The points go in centred: subtract the centre of
+ * their wx x wy box.  No counterpart in the reference, which uses an FMM.  */
+int se2p_free_params(int Nsrc, int Ntar, double wx, double wy, double tol, se2p_params* prm,
+                     double* Lx, double* Ly);
 
 void se2p_options_default(se2p_options* opt);
 
