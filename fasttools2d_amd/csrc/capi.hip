@@ -378,6 +378,7 @@ int params_ref(int kind, int Ns, int Nt, double Q, double Lx, double Ly, const s
     const double kinf = newton(kind, 1, std::round(5 * xi), Q, Lx, Lx, rc, xi, o->tol, &ok2);
     const int kx = (int)std::round(Lx * kinf / (2 * kPi));
     const int M0 = 2 * kx < 10000 ? 2 * kx : 10000;
+    *out = se2p_params{};  // every field this recipe does not set takes its default (0)
     out->P = P;
     out->nside_x = nsx;
     out->nside_y = nsy;

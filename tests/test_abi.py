@@ -76,3 +76,17 @@ def test_parameter_recipe_matches_oracle(oracle, kind, N, Lx, Ly, tol, P, Nb):
         assert abs(got[k] - ref[k]) <= 1e-14 * abs(ref[k]), k
     assert got["converged"] == int(ref["converged"])
     assert got["e1_tier"] == 0
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_parameter_recipe_clears_the_other_fields(kind):
+    """se2p_params_ref writes every field: values a caller's struct held before
+    (here a free-space radius and an FP32 far-pair threshold) do not survive."""
+    from fasttools2d_amd import _lib
+    L = _lib.lib()
+    prm = _lib.Params()
+    prm.free_R, prm.far32, prm.symmetric, prm.precision = 3.0, 4.6, 1, 1
+    rc = L.se2p_params_ref(kind, 1000, 1000, 2001.0, 2.0, 1.0, None, ctypes.byref(prm))
+    assert rc == _lib.SE2P_OK
+    assert (prm.free_R, prm.far32, prm.symmetric, prm.precision) == (0.0, 0.0, 0, 0)
+    assert prm.Mx > 0 and prm.xi > 0
