@@ -381,11 +381,13 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const SysDesc& d = sys[jb.x];
     const int nsx = d.nsx, nsy = d.nsy, bs = d.bs, nbx = d.nbx;
     const double Lx = d.Lx, Ly = d.Ly, xi2 = d.xi2, cut2 = d.cut2, self = d.self;
-    const int* __restrict__ tstart = d.t_start;
-    const int* __restrict__ tperm = d.t_perm;
-    const double2* __restrict__ tsorted = d.t_sorted;
-    const int* __restrict__ sstart = d.s_start;
-    const Rec* __restrict__ ssorted = reinterpret_cast<const Rec*>(d.s_rec);
+    const glob<const int>* __restrict__ tstart = gp(d.t_start);
+    const glob<const int>* __restrict__ tperm = gp(d.t_perm);
+    const glob<const double2>* __restrict__ tsorted = gp(d.t_sorted);
+    const glob<const int>* __restrict__ sstart = gp(d.s_start);
+    const glob<const Rec>* __restrict__ ssorted = gp(reinterpret_cast<const Rec*>(d.s_rec));
+    const glob<const int>* __restrict__ ykey = gp(d.ykey);
+    glob<double>* __restrict__ sacc = gp(reinterpret_cast<double*>(d.sacc));
     double* __restrict__ u = u_base + d.u_off;
     const int tid = threadIdx.x;
     const int bx0 = (jb.y % nbx) * bs, by0 = (jb.y / nbx) * bs;
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 const double4 q = ssorted[ti];
                 fi1 = q.z;
                 fi2 = q.w;
-                qyi = d.ykey[ti];
+                qyi = ykey[ti];
             }
         }
         constexpr int NC = out_comps(KIND);
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     buf[k] = rec;
                 }
                 if constexpr (SYM) {
-                    rkey[k] = d.ykey[sk];
+                    rkey[k] = ykey[sk];
                     racc[k] = make_double2(0.0, 0.0);
                 }
             }
@@ -790,9 +792,10 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     if (v.x != 0.0 || v.y != 0.0) {
                         const int g = r + k * nround;
                         const int e = entry_of(g);
-                        double* a = &d.sacc[e_src[e] + (g - e_off[e])].x;
-                        unsafeAtomicAdd(a, v.x);
-                        unsafeAtomicAdd(a + 1, v.y);
+                        glob<double>* a = sacc + 2 * (e_src[e] + (g - e_off[e]));
+                        __hip_atomic_fetch_add(a, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(a + 1, v.y, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
             }
@@ -802,9 +805,11 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             constexpr double dv = out_div(KIND);
             if (SYM) {
                 // the pair of the target with itself: the self term
-                double* a = &d.sacc[ti].x;
-                unsafeAtomicAdd(a, a1 + self * fi1);
-                unsafeAtomicAdd(a + 1, a2 + self * fi2);
+                glob<double>* a = sacc + 2 * ti;
+                __hip_atomic_fetch_add(a, a1 + self * fi1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(a + 1, a2 + self * fi2, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             } else if (KIND >= SE2P_SLP_GRADIENT) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) u[NC * o + c] = acc[c] / dv;

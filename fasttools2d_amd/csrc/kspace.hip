@@ -26,9 +26,10 @@ namespace se2p {
 
 namespace {
 
-__device__ __forceinline__ void point_density(int mode, int p, const double* __restrict__ f,
-                                              const double* __restrict__ nrm, double& d0,
-                                              double& d1)
+__device__ __forceinline__ void point_density(int mode, int p,
+                                              const glob<const double>* __restrict__ f,
+                                              const glob<const double>* __restrict__ nrm,
+                                              double& d0, double& d1)
 {
     const double f1 = f[2 * p], f2 = f[2 * p + 1];
     if (mode == 0) {  // SLP: (f1, f2)
@@ -119,15 +120,15 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     constexpr bool F32 = std::is_same<S, float>::value;
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
-    const int* __restrict__ tstart = d.sp_start;
-    const int2* __restrict__ snode = d.sp_snode;
-    const double4* __restrict__ fac = d.sp_fac;
-    const double2* __restrict__ dens = d.sp_dens + (size_t)c * d.Ns;
+    const glob<const int>* __restrict__ tstart = gp(d.sp_start);
+    const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
+    const glob<const double4>* __restrict__ fac = gp(d.sp_fac);
+    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.Ns;
     const int Mx = d.Mx, My = d.My, ntx = d.ntx;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
-        const double v = d.e1tab[i];
+        const double v = gp(d.e1tab)[i];
         const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
         const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
         e1[i] = __hiloint2double(hi, lo);
@@ -151,98 +152,154 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     float h0[4] = {0.f, 0.f, 0.f, 0.f}, l0[4] = {0.f, 0.f, 0.f, 0.f};
     float h1[4] = {0.f, 0.f, 0.f, 0.f}, l1[4] = {0.f, 0.f, 0.f, 0.f};
     const float th2 = (float)(d.t * d.h * d.h);
-    // image kx: start node m.x - kx Mx; the window reaches images while X0 - P + kx Mx < Mx
-    for (int kx = 0; X0 - P + kx * Mx < Mx; ++kx) {
-        const int lox = max(0, X0 - P + kx * Mx), hix = min(Mx - 1, XE - 1 + kx * Mx);
-        for (int ky = 0; Y0 - P + ky * My < My; ++ky) {
-            const int loy = max(0, Y0 - P + ky * My), hiy = min(My - 1, YE - 1 + ky * My);
-            for (int ty = loy / kSB; ty <= hiy / kSB; ++ty)
-                for (int tx = lox / kSB; tx <= hix / kSB; ++tx) {
-                    const int tile = ty * ntx + tx;
-                    const int k0 = tstart[tile], k1 = tstart[tile + 1];
-                    for (int kb = k0; kb < k1; kb += kSC) {
-                        // one round of independent, contiguous loads per chunk
-                        const int k = kb + lane;
-                        const bool cand = lane < kSC && k < k1;
-                        int2 m = make_int2(-1, -1);
-                        double4 fk = make_double4(0.0, 0.0, 0.0, 0.0);
-                        double2 dk = make_double2(0.0, 0.0);
-                        if (cand) {
-                            m = snode[k];
-                            fk = fac[k];
-                            dk = dens[k];
-                        }
-                        const bool keep = cand && m.x >= lox && m.x <= hix && m.y >= loy &&
-                                          m.y <= hiy;
-                        const unsigned long long bal = __ballot(keep);
-                        if (!bal) continue;
-                        const int n = __popcll(bal);
-                        __syncthreads();  // the previous chunk's rows have been read
-                        if (keep) {
-                            const int pos = __builtin_amdgcn_mbcnt_hi(
-                                (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                            OX[pos] = m.x - kx * Mx - X0;
-                            OY[pos] = m.y - ky * My - Y0;
-                            SF[pos] = fk;
-                            DD[pos] = dk;
-                        }
-                        __syncthreads();
-                        if (q < n) {  // lanes 0-31: x rows, 32-63: y rows, one code path
-                            const bool xr = lane < kSC;
-                            const double4 f4 = SF[q];
-                            if constexpr (F32)
-                                fill_row_f32<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
-                                                (float)(xr ? f4.x : f4.z), th2);
-                            else
-                                fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
-                                            xr ? f4.x : f4.z, xr ? f4.y : f4.w, e1);
-                        }
-                        __syncthreads();
-                        // all operands of the chunk first, then the matrix steps back to back
-                        S a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
-#pragma unroll
-                        for (int st = 0; st < kSC / 4; ++st) {
-                            const int kk = 4 * st + row;
-                            const S wx = WX[kk][col];
-                            a[st] = kk < n ? wx : (S)0;
-                            const S wy = WY[kk][col];
-                            const double2 dd = DD[kk];
-                            b0[st] = (S)dd.x * wy;
-                            b1[st] = (S)dd.y * wy;
-                        }
-                        if constexpr (F32) {
-                            f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                            for (int st = 0; st < kSC / 4; ++st)
-                                if (4 * st < n) {
-                                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b0[st], c0,
-                                                                              0, 0, 0);
-                                    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b1[st], c1,
-                                                                              0, 0, 0);
-                                }
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                two_sum_add(h0[r], l0[r], c0[r]);
-                                two_sum_add(h1[r], l1[r], c1[r]);
-                            }
-                        } else {
-#pragma unroll
-                            for (int st = 0; st < kSC / 4; ++st)
-                                if (4 * st < n) {
-                                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st],
-                                                                                acc0, 0, 0, 0);
-                                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st],
-                                                                                acc1, 0, 0, 0);
-                                }
-                        }
+    // The chunks the block visits, in the order of the loops image kx, image ky,
+    // tile ty, tile tx, binned order (image kx: start node m.x - kx Mx; the window
+    // reaches images while X0 - P + kx Mx < Mx), generated one ahead: the next
+    // chunk's records are loaded while the current one is staged and multiplied.
+    // Every quantity of the sequence is the same for the whole wave.
+    struct Chunk {
+        int kb, ke, sx, sy, lox, hix, loy, hiy;
+    };
+    int gkx = 0, gky = 0, gtx = 0, gty = 0, gkb = 0, gke = 0;
+    int glox = 0, ghix = -1, gloy = 0, ghiy = -1;
+    auto image_x = [&]() {
+        glox = max(0, X0 - P + gkx * Mx);
+        ghix = min(Mx - 1, XE - 1 + gkx * Mx);
+    };
+    auto image_y = [&]() {
+        gloy = max(0, Y0 - P + gky * My);
+        ghiy = min(My - 1, YE - 1 + gky * My);
+    };
+    image_x();
+    image_y();
+    gtx = glox / kSB;
+    gty = gloy / kSB;
+    {
+        const int tile = gty * ntx + gtx;
+        gkb = tstart[tile];
+        gke = tstart[tile + 1];
+    }
+    // the chunk at the generator's position, then the position advanced past it
+    // (false when the sequence has ended)
+    auto next_chunk = [&](Chunk& ch) -> bool {
+        while (gkb >= gke) {  // tile exhausted: the next tile, image row or image
+            if (++gtx > ghix / kSB) {
+                if (++gty > ghiy / kSB) {
+                    ++gky;
+                    if (!(Y0 - P + gky * My < My)) {
+                        gky = 0;
+                        ++gkx;
+                        if (!(X0 - P + gkx * Mx < Mx)) return false;
+                        image_x();
                     }
+                    image_y();
+                    gty = gloy / kSB;
                 }
+                gtx = glox / kSB;
+            }
+            const int tile = gty * ntx + gtx;
+            gkb = tstart[tile];
+            gke = tstart[tile + 1];
         }
+        ch = Chunk{gkb, gke, gkx * Mx, gky * My, glox, ghix, gloy, ghiy};
+        gkb += kSC;
+        return true;
+    };
+    // one round of independent, contiguous loads per chunk
+    auto load = [&](const Chunk& ch, int2& m, double4& fk, double2& dk) {
+        const int k = ch.kb + lane;
+        m = make_int2(-1, -1);
+        fk = make_double4(0.0, 0.0, 0.0, 0.0);
+        dk = make_double2(0.0, 0.0);
+        if (lane < kSC && k < ch.ke) {
+            m = snode[k];
+            fk = fac[k];
+            dk = dens[k];
+        }
+    };
+    Chunk cur;
+    bool have = next_chunk(cur);
+    int2 m;
+    double4 fk;
+    double2 dk;
+    if (have) load(cur, m, fk, dk);
+    while (have) {
+        Chunk nxt;
+        const bool have_n = next_chunk(nxt);
+        int2 m_n;
+        double4 fk_n;
+        double2 dk_n;
+        if (have_n) load(nxt, m_n, fk_n, dk_n);
+        // lanes past the chunk's end hold start node (-1, -1), outside every window
+        const bool keep = m.x >= cur.lox && m.x <= cur.hix && m.y >= cur.loy && m.y <= cur.hiy;
+        const unsigned long long bal = __ballot(keep);
+        if (bal) {
+            const int n = __popcll(bal);
+            __syncthreads();  // the previous chunk's rows have been read
+            if (keep) {
+                const int pos = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                OX[pos] = m.x - cur.sx - X0;
+                OY[pos] = m.y - cur.sy - Y0;
+                SF[pos] = fk;
+                DD[pos] = dk;
+            }
+            __syncthreads();
+            if (q < n) {  // lanes 0-31: x rows, 32-63: y rows, one code path
+                const bool xr = lane < kSC;
+                const double4 f4 = SF[q];
+                if constexpr (F32)
+                    fill_row_f32<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
+                                    (float)(xr ? f4.x : f4.z), th2);
+                else
+                    fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q], xr ? f4.x : f4.z,
+                                xr ? f4.y : f4.w, e1);
+            }
+            __syncthreads();
+            // all operands of the chunk first, then the matrix steps back to back
+            S a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
+#pragma unroll
+            for (int st = 0; st < kSC / 4; ++st) {
+                const int kk = 4 * st + row;
+                const S wx = WX[kk][col];
+                a[st] = kk < n ? wx : (S)0;
+                const S wy = WY[kk][col];
+                const double2 dd = DD[kk];
+                b0[st] = (S)dd.x * wy;
+                b1[st] = (S)dd.y * wy;
+            }
+            if constexpr (F32) {
+                f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int st = 0; st < kSC / 4; ++st)
+                    if (4 * st < n) {
+                        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b0[st], c0, 0, 0, 0);
+                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b1[st], c1, 0, 0, 0);
+                    }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    two_sum_add(h0[r], l0[r], c0[r]);
+                    two_sum_add(h1[r], l1[r], c1[r]);
+                }
+            } else {
+#pragma unroll
+                for (int st = 0; st < kSC / 4; ++st)
+                    if (4 * st < n) {
+                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st], acc1, 0, 0, 0);
+                    }
+            }
+        }
+        cur = nxt;
+        m = m_n;
+        fk = fk_n;
+        dk = dk_n;
+        have = have_n;
     }
     const int iy = Y0 + col;
     if constexpr (F32) {
         // f32 product: element (x, y) in lane y + 16 (x / 4), register x % 4
-        float2* __restrict__ out = reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]);
+        glob<float2>* __restrict__ out = gp(reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]));
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int ix = X0 + 4 * row + r;
@@ -251,7 +308,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         }
     } else {
         // f64 product: element (x, y) in lane y + 16 (x % 4), register x / 4
-        double2* __restrict__ out = d.grid[c ? G_Z1 : G_Z0];
+        glob<double2>* __restrict__ out = gp(d.grid[c ? G_Z1 : G_Z0]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int ix = X0 + 4 * r + row;
@@ -301,18 +358,20 @@ __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__
     const SysDesc& d = sys[jb.x];
     const int k = jb.y * 256 + threadIdx.x;
     if (k >= d.Ns) return;
-    const int p = d.sp_perm[k];
-    const double* __restrict__ f = f_base + d.f_off;
+    const int p = gp(d.sp_perm)[k];
+    const glob<const double>* __restrict__ f = gp(f_base + d.f_off);
+    const glob<const double>* __restrict__ nrm = gp(d.nrm);
+    glob<double2>* __restrict__ out = gp(d.sp_dens);
     double d0, d1;
     if (!dlp_family(d.kind)) {
-        point_density(0, p, f, d.nrm, d0, d1);
-        d.sp_dens[k] = make_double2(d0, d1);
+        point_density(0, p, f, nrm, d0, d1);
+        out[k] = make_double2(d0, d1);
         return;
     }
-    point_density(1, p, f, d.nrm, d0, d1);
-    d.sp_dens[k] = make_double2(d0, d1);
-    point_density(2, p, f, d.nrm, d0, d1);
-    d.sp_dens[(size_t)d.Ns + k] = make_double2(d0, d1);
+    point_density(1, p, f, nrm, d0, d1);
+    out[k] = make_double2(d0, d1);
+    point_density(2, p, f, nrm, d0, d1);
+    out[(size_t)d.Ns + k] = make_double2(d0, d1);
 }
 
 // Each mode and its Hermitian partner (-k) are filtered together from the same
@@ -335,12 +394,15 @@ __global__ void k_filter(const SysDesc* __restrict__ sys, const Job* __restrict_
     const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
     if (jy == iy && jx < ix) return;  // taken with its partner
     const long gk = gid, gm = (long)jy * Mx + jx;
-    const V* __restrict__ W = reinterpret_cast<const V*>(d.grid[G_W]);
-    const V* __restrict__ D = reinterpret_cast<const V*>(d.grid[G_D]);
-    V* __restrict__ Z0 = reinterpret_cast<V*>(d.grid[G_Z0]);
-    V* __restrict__ Z1 = reinterpret_cast<V*>(d.grid[G_Z1]);
-    auto ld = [](const V* a, long i) { return make_double2(a[i].x, a[i].y); };
-    auto st = [](V* a, long i, double2 v) {
+    const glob<const V>* __restrict__ W = gp(reinterpret_cast<const V*>(d.grid[G_W]));
+    const glob<const V>* __restrict__ D = gp(reinterpret_cast<const V*>(d.grid[G_D]));
+    glob<V>* __restrict__ Z0 = gp(reinterpret_cast<V*>(d.grid[G_Z0]));
+    glob<V>* __restrict__ Z1 = gp(reinterpret_cast<V*>(d.grid[G_Z1]));
+    auto ld = [](const glob<const V>* a, long i) {
+        const V v = a[i];
+        return make_double2(v.x, v.y);
+    };
+    auto st = [](glob<V>* a, long i, double2 v) {
         V r;
         r.x = v.x;
         r.y = v.y;
@@ -385,20 +447,22 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     using V = typename std::conditional<F32, float2, double2>::type;
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
-    const int* __restrict__ tstart = d.gt_start;
-    const int* __restrict__ perm = d.gt_perm;
-    const int2* __restrict__ node = d.gt_node;    // binned order
-    const double2* __restrict__ off = d.gt_off;   // binned order
+    const glob<const int>* __restrict__ tstart = gp(d.gt_start);
+    const glob<const int>* __restrict__ perm = gp(d.gt_perm);
+    const glob<const int2>* __restrict__ node = gp(d.gt_node);    // binned order
+    const glob<const double2>* __restrict__ off = gp(d.gt_off);   // binned order
     const double2* __restrict__ W = d.grid[g ? G_D : G_W];
     const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
-    const double* __restrict__ e1tab = d.e1tab;
-    const double* zm = use_zm ? d.zm : nullptr;
+    const glob<const double>* __restrict__ e1tab = gp(d.e1tab);
+    const glob<const double>* zm = use_zm ? gp(d.zm) : nullptr;
     const bool sym = d.sym && g == 0;
-    double* u = u_base + d.u_off;
+    glob<double>* u = gp(u_base + d.u_off);
+    const glob<const double2>* __restrict__ sacc = gp(d.sacc);
+    const glob<const int>* __restrict__ sidx = gp(d.gt_sidx);
     extern __shared__ double2 reg_raw[];
     V* const reg = reinterpret_cast<V*>(reg_raw);
-    const V* __restrict__ Wg = reinterpret_cast<const V*>(W);
+    const glob<const V>* __restrict__ Wg = gp(reinterpret_cast<const V*>(W));
     __shared__ double e1s[P + 1];
     __shared__ int wcnt[4][16];
     __shared__ int order[256];
@@ -413,7 +477,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     for (int lx = wv; lx < R; lx += 4) {
         int gx = X0 + lx;
         while (gx >= Mx) gx -= Mx;
-        const V* __restrict__ col = Wg + (size_t)gx * My;
+        const glob<const V>* __restrict__ col = Wg + (size_t)gx * My;
         for (int ly = lane; ly < R; ly += 64) {
             int gy = Y0 + ly;
             while (gy >= My) gy -= My;
@@ -510,10 +574,10 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
             if (d.nc == 1) {  // one real field: the real part of W
                 u[p] = accumulate ? u[p] + v0 : v0;
             } else {
-                double* up = u + (size_t)d.nc * p + 2 * g;
+                glob<double>* up = u + (size_t)d.nc * p + 2 * g;
                 if (accumulate && sym) {
                     // the pair-symmetric real-space sums, in sorted order
-                    const double2 a = d.sacc[d.gt_sidx[kk]];
+                    const double2 a = sacc[sidx[kk]];
                     v0 = a.x / (4 * kPi) + v0;
                     v1 = a.y / (4 * kPi) + v1;
                 } else if (accumulate) {
@@ -624,18 +688,17 @@ __global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restri
     const SysDesc& d = sys[jb.x];
     const int i = jb.y * 256 + threadIdx.x;
     if (i >= d.Ns) return;
-    const double* __restrict__ f = f_base + d.f_off;
-    const int o = d.s_perm[i];
+    const glob<const double>* __restrict__ f = gp(f_base + d.f_off);
+    const int o = gp(d.s_perm)[i];
     const double f1 = f[2 * o], f2 = f[2 * o + 1];
     if (LAYOUT == 0) {
-        double4* rec = reinterpret_cast<double4*>(d.s_rec);
-        rec[i].z = f1;
-        rec[i].w = f2;
+        glob<double2>* rec = gp(reinterpret_cast<double2*>(d.s_rec));
+        rec[2 * i + 1] = make_double2(f1, f2);
         // pair-symmetric sums (sources = targets, one sorted order): clear the
         // accumulator the real-space pass adds into
-        if (d.sym) d.sacc[i] = make_double2(0.0, 0.0);
+        if (d.sym) gp(d.sacc)[i] = make_double2(0.0, 0.0);
     } else {
-        double* r = d.s_rec + 8 * (size_t)i;
+        glob<double>* r = gp(d.s_rec + 8 * (size_t)i);
         r[2] = f1;
         r[3] = f2;
         r[6] = f1 * r[4] + f2 * r[5];

@@ -56,6 +56,27 @@ __host__ __device__ constexpr int rec_layout(int kind) { return dlp_family(kind)
 // Complex grids of the k-space output (two real values per grid).
 __host__ __device__ constexpr int out_grids(int kind) { return out_comps(kind) == 4 ? 2 : 1; }
 
+// Global memory seen as such.  Pointers read from a descriptor are generic, so
+// loads, stores and atomics through them compile to flat_* instructions, which
+// also count against the LDS wait counter: every wait for an LDS result then
+// waits for the global loads in flight too.  gp(p) is the same address as a
+// global-address-space pointer, whose accesses compile to global_* instructions.
+// (The host pass of a HIP compilation parses kernel bodies without generating
+// them and cannot assign HIP vector types across address spaces, so there the
+// alias is the plain type.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using glob = __attribute__((address_space(1))) T;
+#else
+template <class T>
+using glob = T;
+#endif
+template <class T>
+__device__ __forceinline__ glob<T>* gp(T* p)
+{
+    return (glob<T>*)p;
+}
+
 // ---------------------------------------------------------------------------
 // Status helpers
 // ---------------------------------------------------------------------------
