@@ -65,7 +65,7 @@ constexpr double out_div(int kind)
                                         : 4 * kPi;
 }
 
-constexpr int kBlkThreads = 256;
+constexpr int kBlkThreads = kRealLanes;
 constexpr int kMaxBlockSide = 4;
 constexpr int kMaxEntries = (kMaxBlockSide + 2) * (kMaxBlockSide + 2);
 constexpr int kWords = 2, kChunk = 64 * kWords;  // candidates per test chunk
@@ -390,7 +390,10 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     glob<double>* __restrict__ sacc = gp(reinterpret_cast<double*>(d.sacc));
     double* __restrict__ u = u_base + d.u_off;
     const int tid = threadIdx.x;
-    const int bx0 = (jb.y % nbx) * bs, by0 = (jb.y / nbx) * bs;
+    // job: block blk, and the block's targets [part, part + 1) x kBlkThreads
+    const int nblk = nbx * ((nsy + bs - 1) / bs);
+    const int blk = jb.y % nblk, part = jb.y / nblk;
+    const int bx0 = (blk % nbx) * bs, by0 = (blk / nbx) * bs;
     const int bw = min(bs, nsx - bx0), bh = min(bs, nsy - by0);
     // window column height; entry (ux, uy) is (ux+1) H + (uy+1), or (ux+1) H + uy
     // for SYM, whose window starts at row 0
@@ -455,7 +458,9 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
     const double self_r2 = d.self_r2;
     const bool trim = !UNI && trim_w > 0 && d.unsorted && *d.unsorted == 0;
 
-    for (int tb = 0; tb < ntar; tb += kBlkThreads) {
+    // the job list holds one job per 256 targets of the block (at least one)
+    const int nparts = max(1, (ntar + kBlkThreads - 1) / kBlkThreads);
+    for (int tb = part * kBlkThreads; tb < ntar; tb += nparts * kBlkThreads) {
         const int i = tb + tid;
         const bool act = i < ntar;
         int ti = 0, cx = 0, cy = 0, kin = 0;

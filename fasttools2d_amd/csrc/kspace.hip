@@ -466,11 +466,14 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     __shared__ double e1s[P + 1];
     __shared__ int wcnt[4][16];
     __shared__ int order[256];
-    const int tile = jb.y;
+    // job: tile, and its targets [part, part + 1) x kGatherPart
+    const int ntiles = ntx * d.gnty;
+    const int tile = jb.y % ntiles, part = jb.y / ntiles;
     const int tx = tile % ntx, ty = tile / ntx;
     const int X0 = tx * TX, Y0 = ty * TY;
-    const int k0 = tstart[tile], k1 = tstart[tile + 1];
-    if (k0 == k1) return;
+    const int k0 = tstart[tile] + part * kGatherPart;
+    const int k1 = min(tstart[tile + 1], k0 + kGatherPart);
+    if (k0 >= k1) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int i = tid; i <= P; i += blockDim.x) e1s[i] = e1tab[i];
     // region rows (periodic wrap; a grid may be narrower than the region)

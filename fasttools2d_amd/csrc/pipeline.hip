@@ -16,6 +16,7 @@
 #include "e1_eval.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -128,6 +129,12 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
             SE2P_HIP_CHECK(hipStreamSynchronize(s));
             G->sym = uns == 0;
         }
+        // the targets per cell, for the job lists: a block of cells holding more
+        // targets than a workgroup has lanes is split into one job per 256 targets
+        G->t_start_h.resize((size_t)nbox + 1);
+        SE2P_HIP_CHECK(hipMemcpyAsync(G->t_start_h.data(), G->t_start,
+                                      sizeof(int) * ((size_t)nbox + 1), hipMemcpyDeviceToHost, s));
+        SE2P_HIP_CHECK(hipStreamSynchronize(s));
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));
@@ -206,6 +213,12 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, gtiles, G->gt_perm,
                              G->gt_start, WS_SORT_TMP));
+        // the targets per gathering tile, for the job lists (heavy tiles are split)
+        G->gt_start_h.resize((size_t)gtiles + 1);
+        SE2P_HIP_CHECK(hipMemcpyAsync(G->gt_start_h.data(), G->gt_start,
+                                      sizeof(int) * ((size_t)gtiles + 1), hipMemcpyDeviceToHost,
+                                      s));
+        SE2P_HIP_CHECK(hipStreamSynchronize(s));
         // the gather reads its targets' records in binned order (contiguous)
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_BNODE, sizeof(int2) * (size_t)Nt, &p));
         G->gt_bnode = (int2*)p;
@@ -282,6 +295,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->gTX = g.gTX;
         d->gTY = g.gTY;
         d->gntx = g.gntx;
+        d->gnty = g.gnty;
         d->gR = g.gR;
         d->Bx = fft_lines_per_block(g.Mx);
         d->By = fft_lines_per_block(g.My);
@@ -348,6 +362,38 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     auto blocks = [&](int i, long n) {
         for (long b = 0; b < n; ++b) J.push_back(make_int2(i, (int)b));
     };
+    // Real space: one job per block of bs x bs cells and 256 of its targets (job y
+    // = block + part * nblk), so a dense block runs on as many workgroups as it
+    // needs 256-lane passes instead of making one workgroup loop over them.
+    bool too_many = false;
+    auto real_jobs = [&](int i, const SysDesc& d, const Geom& G) {
+        const long nby = (d.nsy + d.bs - 1) / d.bs, nblk = (long)d.nbx * nby;
+        for (long b = 0; b < nblk; ++b) {
+            const int bx0 = (int)(b % d.nbx) * d.bs, by0 = (int)(b / d.nbx) * d.bs;
+            const int bw = std::min(d.bs, d.nsx - bx0), bh = std::min(d.bs, d.nsy - by0);
+            long ntar = 0;
+            for (int cy = 0; cy < bh; ++cy) {
+                const int c0 = (by0 + cy) * d.nsx + bx0;
+                ntar += G.t_start_h[c0 + bw] - G.t_start_h[c0];
+            }
+            const long parts = std::max(1L, (ntar + kRealLanes - 1) / kRealLanes);
+            if (b + (parts - 1) * nblk > INT_MAX) too_many = true;
+            for (long q = 0; q < parts && !too_many; ++q)
+                J.push_back(make_int2(i, (int)(b + q * nblk)));
+        }
+    };
+    // Gathering: one job per tile and kGatherPart of its targets (job y = tile +
+    // part * ntiles).
+    auto gather_jobs = [&](int i, const Geom& G) {
+        const long nt = (long)G.kg.gntx * G.kg.gnty;
+        for (long t = 0; t < nt; ++t) {
+            const long cnt = G.gt_start_h[t + 1] - G.gt_start_h[t];
+            const long parts = std::max(1L, (cnt + kGatherPart - 1) / kGatherPart);
+            if (t + (parts - 1) * nt > INT_MAX) too_many = true;
+            for (long q = 0; q < parts && !too_many; ++q)
+                J.push_back(make_int2(i, (int)(t + q * nt)));
+        }
+    };
     for (int layout = 0; layout < 2; ++layout)
         add(B->fill[layout], [&](int i, const SysDesc& d) {
             if (geoms[i]->has_real && rec_layout(d.kind) == layout) blocks(i, (d.Ns + 255) / 256);
@@ -364,7 +410,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                     const int m = d.sym ? RM_SYM : d.bs == 1 ? RM_ONE_CELL : RM_GENERAL;
                     if (geoms[i]->has_real && d.kind == kind && d.e1tier == tier && m == mode &&
                         d.Nt > 0)
-                        blocks(i, (long)d.nbx * ((d.nsy + d.bs - 1) / d.bs));
+                        real_jobs(i, d, *geoms[i]);
                 });
     }
 
@@ -419,11 +465,11 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             });
             for (int q = 0; q < kNumP; ++q)
                 add(B->gather[prec][g][q], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1))
-                        blocks(i, (long)d.gntx * geoms[i]->kg.gnty);
+                    if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1)) gather_jobs(i, *geoms[i]);
                 });
         }
     }
+    if (too_many) return set_error(SE2P_ERR_UNSUPPORTED, "too many blocks for the job lists");
     if (B->has_k) SE2P_TRY(fft_prepare());
     void *psys, *pjobs;
     SE2P_TRY(ws_get(ctx, ws, WS_SYS, sizeof(SysDesc) * (size_t)std::max(1, nsys), &psys));

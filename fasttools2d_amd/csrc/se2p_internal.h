@@ -291,7 +291,7 @@ struct SysDesc {
     int zm_nblk;
     // k space
     int Mx, My, P, TX, TY, ntx, nty, Bx, By;
-    int gTX, gTY, gntx, gR;
+    int gTX, gTY, gntx, gnty, gR;
     double t, w, h, xi, eta, gscale;
     const int *sp_start, *sp_perm, *gt_start, *gt_perm;
     const int2 *sp_snode, *gt_node;  // spreading: start nodes in binned order
@@ -320,9 +320,15 @@ int launch_e1(hipStream_t s, const double* x, int n, int tier, double* out);
 
 // Real-space blocks of bs x bs cells for a mean cell occupancy.
 int real_block_side(int nsx, int nsy, int mean_targets);
+// Lanes of a real-space workgroup: a job takes a block's targets [256 part,
+// 256 part + 256).
+constexpr int kRealLanes = 256;
+// Targets of a gathering tile per job (4 passes of 256 lanes over one staged region).
+constexpr int kGatherPart = 1024;
 // Real-space launch modes: general blocks, one-cell blocks, pair-symmetric blocks.
 enum RealMode { RM_GENERAL = 0, RM_ONE_CELL = 1, RM_SYM = 2, kRealModes = 3 };
-// k_real_stencil<kind, tier, mode> over the jobs (block indices).  RM_GENERAL
+// k_real_stencil<kind, tier, mode> over the jobs (block + part * blocks, part
+// the block's 256-target slice).  RM_GENERAL
 // and RM_ONE_CELL set u; RM_SYM adds into sacc (zeroed by the density fill) with
 // atomics, and the gather adds sacc to the k-space part (only systems with a
 // k-space part take pair-symmetric sums).
@@ -416,6 +422,7 @@ struct Geom {
     int* ykey = nullptr;      // in-cell y keys of the sorted sources (sym only)
     double2* sacc = nullptr;  // pair-symmetric sums in sorted order (sym only)
     int* spos = nullptr;      // sorted position of each point (sym only)
+    std::vector<int> t_start_h;  // host copy of t_start: the real-space jobs split heavy blocks
     // k space
     KGeom kg{};
     FftAxis ax{}, ay{};
@@ -434,6 +441,7 @@ struct Geom {
     double2* gt_boff = nullptr;
     int* gt_sidx = nullptr;     // pair-symmetric sums: sorted position, binned order
     double2 *Z0 = nullptr, *Z1 = nullptr, *W = nullptr, *D = nullptr;
+    std::vector<int> gt_start_h;  // host copy of gt_start: the gathering jobs split heavy tiles
 };
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
                const double* d_ptar, const double* d_n, int Ns, int Nt, double Lx, double Ly,
