@@ -14,7 +14,7 @@ namespace se2p {
 
 namespace {
 
-constexpr int kSortCap = 4096;  // largest bucket sorted in LDS by one workgroup
+constexpr int kSortCap = kSortCellCap;  // largest bucket sorted in LDS by one workgroup
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
@@ -216,13 +216,13 @@ __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
                                                       const double* __restrict__ p, double Lx,
                                                       double Ly, int nsx, int nsy, int yorder,
                                                       int* __restrict__ unsorted,
-                                                      int* __restrict__ ykey)
+                                                      int* __restrict__ ykey, int big_sorted)
 {
     __shared__ unsigned key[kSortCap];
     __shared__ int old[kSortCap];
     const int b = blockIdx.x;
     const int lo = start[b], size = start[b + 1] - lo;
-    if (size > kSortCap && yorder && threadIdx.x == 0) *unsorted = 1;
+    if (size > kSortCap && yorder && !big_sorted && threadIdx.x == 0) *unsorted = 1;
     // a single point is in order, but its y key is still wanted
     if (size > kSortCap || size == 0 || (size == 1 && !(yorder && ykey))) return;
     const int bx = b % nsx, by = b / nsx;
@@ -272,6 +272,84 @@ __global__ __launch_bounds__(256) void k_morton_cells(int* __restrict__ perm,
     for (int i = threadIdx.x; i < size; i += blockDim.x) {
         perm[lo + i] = old[key[i] & 0xFFFu];
         if (yorder && ykey) ykey[lo + i] = (int)(key[i] >> 12);
+    }
+}
+
+// y order inside the cells larger than kSortCap (one workgroup per listed cell):
+// the same order as k_morton_cells with yorder -- by the 20-bit in-cell y key,
+// ties in index order -- by a stable LSD radix sort on 4-bit digits through
+// global scratch (keys and indices, two buffers each, at the cell's offset).
+__global__ __launch_bounds__(256) void k_ysort_big(int* __restrict__ perm,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ cells,
+                                                   const double* __restrict__ p, double Ly,
+                                                   int nsx, int nsy, int* __restrict__ unsorted,
+                                                   int* __restrict__ ykey, int* __restrict__ kA,
+                                                   int* __restrict__ vA, int* __restrict__ kB,
+                                                   int* __restrict__ vB)
+{
+    __shared__ int hist[16], base[16], wcnt[4][16];
+    const int b = cells[blockIdx.x];
+    const int lo = start[b], size = start[b + 1] - lo;
+    const int by = b / nsx;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i < size; i += 256) {
+        const int o = perm[lo + i];
+        const double fy = (p[2 * o + 1] / Ly + 0.5) * nsy - by;
+        if (!(fy >= 0.0 && fy < 1.0)) *unsorted = 1;
+        kA[lo + i] = min(1048575, max(0, (int)(fy * 1048576.0)));
+        vA[lo + i] = o;
+    }
+    int *ks = kA, *vs = vA, *kd = kB, *vd = vB;
+    for (int shift = 0; shift < 20; shift += 4) {
+        __syncthreads();
+        if (tid < 16) hist[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < size; i += 256) atomicAdd(&hist[(ks[lo + i] >> shift) & 15], 1);
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int d = 0; d < 16; ++d) {
+                base[d] = acc;
+                acc += hist[d];
+            }
+        }
+        __syncthreads();
+        // chunks of 256 in order; inside a chunk, ranks by wave ballots per digit
+        for (int c0 = 0; c0 < size; c0 += 256) {
+            const int i = c0 + tid;
+            const bool valid = i < size;
+            const int kv = valid ? ks[lo + i] : 0, vv = valid ? vs[lo + i] : 0;
+            const int dg = valid ? (kv >> shift) & 15 : 16;
+            int rank = 0;
+            for (int d = 0; d < 16; ++d) {
+                const unsigned long long m = __ballot(dg == d);
+                if (dg == d)
+                    rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                if (lane == 0) wcnt[wv][d] = __popcll(m);
+            }
+            __syncthreads();
+            if (valid) {
+                int pos = base[dg] + rank;
+                for (int w = 0; w < wv; ++w) pos += wcnt[w][dg];
+                kd[lo + pos] = kv;
+                vd[lo + pos] = vv;
+            }
+            __syncthreads();
+            if (tid < 16) base[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        }
+        int* t = ks;
+        ks = kd;
+        kd = t;
+        t = vs;
+        vs = vd;
+        vd = t;
+    }
+    __syncthreads();
+    for (int i = tid; i < size; i += 256) {
+        perm[lo + i] = vs[lo + i];
+        if (ykey) ykey[lo + i] = ks[lo + i];
     }
 }
 
@@ -348,10 +426,22 @@ int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n,
 
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
                         double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted,
-                        int* ykey)
+                        int* ykey, int big_sorted)
 {
     hipLaunchKernelGGL(k_morton_cells, dim3(nbox), dim3(256), 0, s, perm, start, p, Lx, Ly, nsx,
-                       nsy, yorder, unsorted, ykey);
+                       nsy, yorder, unsorted, ykey, big_sorted);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_ysort_big(hipStream_t s, int* perm, const int* start, const int* cells, int ncells,
+                     const double* p, double Ly, int nsx, int nsy, int* unsorted, int* ykey,
+                     int* tmp, int n)
+{
+    if (ncells <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_ysort_big, dim3(ncells), dim3(256), 0, s, perm, start, cells, p, Ly, nsx,
+                       nsy, unsorted, ykey, tmp, tmp + n, tmp + 2 * (size_t)n,
+                       tmp + 3 * (size_t)n);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

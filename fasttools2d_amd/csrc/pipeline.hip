@@ -106,6 +106,16 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(launch_cell_keys(s, d_psrc, Ns, Lx, Ly, nsx, nsy, (int*)skey, (int*)scount));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, nbox, G->s_perm, G->s_start,
                              WS_SORT_TMP));
+        // the sources per cell on the host: the cells too large for the in-LDS y
+        // order are ordered by a second kernel, and (sources = targets) the job
+        // lists split heavy blocks
+        std::vector<int> sst((size_t)nbox + 1);
+        SE2P_HIP_CHECK(hipMemcpyAsync(sst.data(), G->s_start, sizeof(int) * ((size_t)nbox + 1),
+                                      hipMemcpyDeviceToHost, s));
+        SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<int> big;
+        for (int c = 0; c < nbox; ++c)
+            if (sst[c + 1] - sst[c] > kSortCellCap) big.push_back(c);
         if (!sym) {
             SE2P_TRY(launch_fill_int(s, (int*)tcount, nbox, 0));
             SE2P_TRY(launch_cell_keys(s, d_ptar, Nt, Lx, Ly, nsx, nsy, (int*)tkey, (int*)tcount));
@@ -116,7 +126,18 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         // y), targets along a Morton curve (neighbouring lanes, similar stencils)
         SE2P_TRY(launch_fill_int(s, G->unsorted, 1, 0));
         SE2P_TRY(launch_morton_cells(s, G->s_perm, G->s_start, nbox, d_psrc, Lx, Ly, nsx, nsy, 1,
-                                     G->unsorted, G->ykey));
+                                     G->unsorted, G->ykey, big.empty() ? 0 : 1));
+        if (!big.empty()) {
+            void *pc, *pt;
+            SE2P_TRY(ws_get(ctx, ws, WS_BIGCELLS, sizeof(int) * big.size(), &pc));
+            SE2P_TRY(ws_get(ctx, ws, WS_YSORT, sizeof(int) * 4 * (size_t)Ns, &pt));
+            SE2P_HIP_CHECK(hipMemcpyAsync(pc, big.data(), sizeof(int) * big.size(),
+                                          hipMemcpyHostToDevice, s));
+            SE2P_TRY(launch_ysort_big(s, G->s_perm, G->s_start, (const int*)pc, (int)big.size(),
+                                      d_psrc, Ly, nsx, nsy, G->unsorted, G->ykey, (int*)pt, Ns));
+            // `big` is read by the copy above before it goes out of scope
+            SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        }
         if (!sym)
             SE2P_TRY(launch_morton_cells(s, G->t_perm, G->t_start, nbox, d_ptar, Lx, Ly, nsx,
                                          nsy, 0, G->unsorted));
@@ -131,10 +152,15 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         }
         // the targets per cell, for the job lists: a block of cells holding more
         // targets than a workgroup has lanes is split into one job per 256 targets
-        G->t_start_h.resize((size_t)nbox + 1);
-        SE2P_HIP_CHECK(hipMemcpyAsync(G->t_start_h.data(), G->t_start,
-                                      sizeof(int) * ((size_t)nbox + 1), hipMemcpyDeviceToHost, s));
-        SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        if (sym) {
+            G->t_start_h = std::move(sst);
+        } else {
+            G->t_start_h.resize((size_t)nbox + 1);
+            SE2P_HIP_CHECK(hipMemcpyAsync(G->t_start_h.data(), G->t_start,
+                                          sizeof(int) * ((size_t)nbox + 1),
+                                          hipMemcpyDeviceToHost, s));
+            SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        }
         // positions (and normals) of the sorted sources; densities are filled per apply
         SE2P_TRY(launch_pack_sources(s, kind, G->s_perm, Ns, d_psrc, d_psrc, G->nrm, G->s_rec));
         SE2P_TRY(launch_pack_targets(s, G->t_perm, Nt, d_ptar, G->t_sorted));

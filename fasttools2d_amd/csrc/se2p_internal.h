@@ -123,7 +123,7 @@ enum Slot {
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
     WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
-    WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_NUM_SLOTS
+    WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_YSORT, WS_BIGCELLS, WS_NUM_SLOTS
 };
 
 // Grow-only device buffers addressed by slot (one set per context, one per plan).
@@ -209,9 +209,18 @@ int launch_fill_int(hipStream_t s, int* p, int n, int v);
 // Morton order (yorder 0) or y order (yorder 1; sets *unsorted where a cell cannot be
 // ordered) inside each cell of a cell-sorted permutation.  With yorder and a
 // non-null ykey, ykey[k] is the 20-bit in-cell y key of the k-th sorted point.
+// Cells of more than kSortCellCap points keep their order here; with yorder
+// they set *unsorted unless big_sorted says launch_ysort_big orders them.
+constexpr int kSortCellCap = 4096;
 int launch_morton_cells(hipStream_t s, int* perm, const int* start, int nbox, const double* p,
                         double Lx, double Ly, int nsx, int nsy, int yorder, int* unsorted,
-                        int* ykey = nullptr);
+                        int* ykey = nullptr, int big_sorted = 0);
+// The y order of launch_morton_cells for the ncells cells listed in `cells`
+// (device array), each larger than kSortCellCap; tmp holds 4 n ints (n >= the
+// permutation's length).
+int launch_ysort_big(hipStream_t s, int* perm, const int* start, const int* cells, int ncells,
+                     const double* p, double Ly, int nsx, int nsy, int* unsorted, int* ykey,
+                     int* tmp, int n);
 // FindClosestNode for n points, unwrapped start nodes.
 int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, double Ly,
                          double h, int P, int* mx, int* my, double* px, double* py);

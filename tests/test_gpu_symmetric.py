@@ -210,3 +210,50 @@ def test_fast_e1_tier_within_its_bound(gpu_ctx, oracle, sym):
     r += oracle.slp_kspace(p, p, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], 1.0, 1.0,
                            prm["w"], prm["P"])
     assert _rel(fast, r) < 1e-12
+
+
+def _cluster_cell(n_bg, n_cl, center, sigma, seed):
+    rng = np.random.default_rng(seed)
+    bg = uniform_points(n_bg, 1.0, 1.0, seed=seed + 1)
+    cl = np.clip(rng.normal(center, sigma, (n_cl, 2)), -0.499, 0.499)
+    return np.concatenate([bg, cl])
+
+
+def test_cells_beyond_the_in_lds_sort(gpu_ctx, oracle):
+    """A cell of more than 4096 points (a tight cluster) is put in y order by the
+    global radix sort, so the pair-symmetric sums still run, and the fixed-order
+    sums trim their stencil columns by y: both match the oracle."""
+    p = _cluster_cell(3000, 6000, [0.2, -0.1], 0.004, 51)
+    f = normal_density(len(p), seed=53)
+    prm = _prm(len(p), 1.0, 1.0, 12, 12)
+    cells = np.floor((p + 0.5) * 12).astype(int)
+    assert np.max(np.bincount(cells[:, 1] * 12 + cells[:, 0])) > 4096
+    plan = Plan(gpu_ctx, "slp", p, p, 1.0, 1.0, dict(prm, symmetric=1))
+    try:
+        assert plan.params_in_use()["symmetric"] == 1
+    finally:
+        plan.close()
+    _check(gpu_ctx, p, f, 1.0, 1.0, prm, oracle)
+    r = oracle.slp_real(p, p, f, prm["xi"], 12, 12, 1.0, 1.0)
+    r += oracle.slp_kspace(p, p, prm["xi"], prm["eta"], f, prm["Mx"], prm["My"], 1.0, 1.0,
+                           prm["w"], prm["P"])
+    assert _rel(_apply(gpu_ctx, p, f, 1.0, 1.0, prm, False)[0], r) < REL
+
+
+def test_big_cell_with_a_point_outside_falls_back(gpu_ctx):
+    """A point outside the box in a cell of more than 4096 points: that cell has
+    no y order, so the plan sums each target's pairs in the fixed order.  Its
+    targets keep the sources' order, so its waves, and with them the chunking of
+    long candidate lists, differ from a plan asked for the fixed order: the two
+    agree to rounding, not bit for bit."""
+    p = _cluster_cell(3000, 6000, [0.0, 0.46], 0.004, 61)
+    p[-1] = [0.0, 0.52]
+    f = normal_density(len(p), seed=63)
+    prm = _prm(len(p), 1.0, 1.0, 12, 12)
+    plan = Plan(gpu_ctx, "slp", p, p, 1.0, 1.0, dict(prm, symmetric=1))
+    try:
+        assert plan.params_in_use()["symmetric"] == 0
+    finally:
+        plan.close()
+    u = _apply(gpu_ctx, p, f, 1.0, 1.0, prm, True)
+    assert _rel(u[0], _apply(gpu_ctx, p, f, 1.0, 1.0, prm, False)[0]) < REL
