@@ -75,13 +75,15 @@ def work_model(prm, N, cand, evald, kind="slp"):
     """Flops per matvec of the fixed work model: 6 cand + 60 E + spread + gather + FFT.
     cand and evald count ordered (target, source) pairs; pair-symmetric sums
     (prm["symmetric"]) test and evaluate each unordered pair once, and an
-    evaluation there also adds the source's term (8 flops more)."""
+    evaluation there also adds the source's term (8 flops more for the
+    Stokeslet, 27 for the stresslet, whose second term shares only the
+    exponential and the radial factors)."""
     P1 = (prm["P"] + 1) ** 2
     G = prm["Mx"] * prm["My"]
     cs = 2 if kind == "slp" else 4
     nfft = 4 if kind == "slp" else 6
-    if kind == "slp" and prm.get("symmetric", 0):
-        real = (6 * cand + 68 * evald) / 2
+    if prm.get("symmetric", 0):
+        real = (6 * cand + (68 if kind == "slp" else 72) * evald) / 2
     else:
         real = 6 * cand + (60 if kind == "slp" else 45) * evald
     spread = 2 * cs * N * P1

@@ -198,6 +198,47 @@ __device__ __forceinline__ void dlp_pair_f32(float r1, float r2, float rr, float
     a2 += e * (b * r2 + c * (f2 * rn + n2 * rf + fn * r2));
 }
 
+// dlp_pair for a pair evaluated once for both points: (a1, a2) += the term of
+// the source (f, n) at the target; (b1, b2) = the term of the target's (g, m)
+// at the source, which is the same expression negated (the kernel is odd in r).
+__device__ __forceinline__ void dlp_pair2(double r1, double r2, double rr, double xi2, double f1,
+                                          double f2, double n1, double n2, double g1, double g2,
+                                          double m1, double m2, double& a1, double& a2,
+                                          double& b1, double& b2)
+{
+    const double e = exp(-xi2 * rr);
+    const double ir = rcp64(rr);
+    const double k4 = -4.0 * (1.0 + xi2 * rr) * (ir * ir);
+    const double c = 2.0 * xi2;
+    const double rf = r1 * f1 + r2 * f2, rn = r1 * n1 + r2 * n2, fn = f1 * n1 + f2 * n2;
+    const double bs = k4 * rf * rn;
+    a1 += e * (bs * r1 + c * (f1 * rn + n1 * rf + fn * r1));
+    a2 += e * (bs * r2 + c * (f2 * rn + n2 * rf + fn * r2));
+    const double rg = r1 * g1 + r2 * g2, rm = r1 * m1 + r2 * m2, gm = g1 * m1 + g2 * m2;
+    const double bt = k4 * rg * rm;
+    b1 = -e * (bt * r1 + c * (g1 * rm + m1 * rg + gm * r1));
+    b2 = -e * (bt * r2 + c * (g2 * rm + m2 * rg + gm * r2));
+}
+
+__device__ __forceinline__ void dlp_pair2_f32(float r1, float r2, float rr, float xi2, float f1,
+                                              float f2, float n1, float n2, float g1, float g2,
+                                              float m1, float m2, float& a1, float& a2,
+                                              float& b1, float& b2)
+{
+    const float e = __expf(-xi2 * rr);
+    const float ir = __builtin_amdgcn_rcpf(rr);
+    const float k4 = -4.0f * (1.0f + xi2 * rr) * (ir * ir);
+    const float c = 2.0f * xi2;
+    const float rf = r1 * f1 + r2 * f2, rn = r1 * n1 + r2 * n2, fn = f1 * n1 + f2 * n2;
+    const float bs = k4 * rf * rn;
+    a1 += e * (bs * r1 + c * (f1 * rn + n1 * rf + fn * r1));
+    a2 += e * (bs * r2 + c * (f2 * rn + n2 * rf + fn * r2));
+    const float rg = r1 * g1 + r2 * g2, rm = r1 * m1 + r2 * m2, gm = g1 * m1 + g2 * m2;
+    const float bt = k4 * rg * rm;
+    b1 = -e * (bt * r1 + c * (g1 * rm + m1 * rg + gm * r1));
+    b2 = -e * (bt * r2 + c * (g2 * rm + m2 * rg + gm * r2));
+}
+
 // 2 pi times the stresslet pressure pair term for r = target - source
 // (mex_stokes_dlp_pressure_real.cpp:113-116):
 //   -e^{-xi^2 r^2} ((f.n - 2 xi^2 (r.n)(r.f))/r^2 - 2 (r.n)(r.f)/r^4)
@@ -340,7 +381,7 @@ __device__ __forceinline__ int round_index(int g, int r, int nround)
 // UNI: blocks of one cell (bs = 1), where every lane has the same stencil and
 // the candidate bookkeeping is wave-uniform (scalar registers, LDS broadcast).
 //
-// SYM (Stokeslet, sources = targets in one y order): each pair is evaluated
+// SYM (velocity sums, sources = targets in one y order): each pair is evaluated
 // once, by the point the other lies "after": the source's image in the stencil
 // row above, or in the same row with a larger in-cell y key (ties: the image in
 // the column to the right; in the own cell: later in the sorted order).  This
@@ -350,21 +391,30 @@ __device__ __forceinline__ int round_index(int g, int r, int nround)
 // trim_w.  The window holds rows 0 .. bh only; the term for the source is
 // accumulated in LDS (racc) and added after each round, the target's own sum
 // after the last round, with atomics into sacc, which follows the sorted order
-// (so a cell's atomics share cache lines); the gather adds it to u.
+// (so a cell's atomics share cache lines); the gather adds it to u.  Both
+// kernels are even (Stokeslet) or odd (stresslet) in r, so a pair's term for
+// the source is the target's expression with the target's density.
+// For the stresslet the staged sources carry their normals, in a smaller window.
 template <int KIND, int TIER, bool UNI, bool SYM = false>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_real_stencil(
     const SysDesc* __restrict__ sys, const Job* __restrict__ jobs, double* __restrict__ u_base)
 {
-    static_assert(!SYM || (KIND == SE2P_SLP && !UNI), "pair-symmetric sums: Stokeslet blocks");
+    static_assert(!SYM || ((KIND == SE2P_SLP || KIND == SE2P_DLP) && !UNI),
+                  "pair-symmetric sums: velocity blocks");
     using Rec = typename SrcOf<KIND>::T;
     // with the reflected sums and keys, 4 workgroups per CU still fit in LDS
-    constexpr int CAP = SYM ? 768 : SrcOf<KIND>::cap;
-    __shared__ Rec buf[CAP];
-    // Stokeslet records are staged as two arrays, positions and densities, so a
+    constexpr int CAP = SYM ? (KIND == SE2P_SLP ? 768 : 576) : SrcOf<KIND>::cap;
+    // Stokeslet records, and the stresslet's in pair-symmetric sums, are staged
+    // as arrays of 16-byte entries -- positions, densities (and normals) -- so a
     // candidate test reads 16-byte entries at a 16-byte stride
-    constexpr bool SOA = KIND == SE2P_SLP;
-    double2* const bxy = reinterpret_cast<double2*>(buf);
+    constexpr bool SOA = KIND == SE2P_SLP || SYM;
+    constexpr int NW = KIND == SE2P_SLP ? 2 : 3;  // staged 16-byte entries per source
+    constexpr int BUF = SOA ? NW * CAP : CAP * (int)(sizeof(Rec) / 16);
+    __shared__ __align__(64) double2 raw[BUF];
+    Rec* const buf = reinterpret_cast<Rec*>(raw);
+    double2* const bxy = raw;
     double2* const bfd = bxy + CAP;
+    double2* const bnr = bxy + 2 * CAP;  // stresslet normals (pair-symmetric sums)
     auto rec_xy = [&](int i) -> double2 {
         if constexpr (SOA)
             return bxy[i];
@@ -465,7 +515,8 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
         const bool act = i < ntar;
         int ti = 0, cx = 0, cy = 0, kin = 0;
         double2 pt = make_double2(0.0, 0.0);
-        double fi1 = 0, fi2 = 0;  // SYM: the target's density and y key
+        double fi1 = 0, fi2 = 0;  // SYM: the target's density (and normal) and y key
+        double ni1 = 0, ni2 = 0;
         int qyi = 0;
         if (act) {
             int c = 0;
@@ -475,12 +526,18 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             cx = c / bh;
             cy = c % bh;
             pt = tsorted[ti];
-            if constexpr (SYM) {
+            if constexpr (SYM && KIND == SE2P_SLP) {
                 const double4 q = ssorted[ti];
                 fi1 = q.z;
                 fi2 = q.w;
-                qyi = ykey[ti];
+            } else if constexpr (SYM) {
+                const DlpRec q = ssorted[ti];
+                fi1 = q.f1;
+                fi2 = q.f2;
+                ni1 = q.n1;
+                ni2 = q.n2;
             }
+            if constexpr (SYM) qyi = ykey[ti];
         }
         constexpr int NC = out_comps(KIND);
         double a1 = 0, a2 = 0;                // velocity and pressure kinds
@@ -495,9 +552,14 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                 Rec rec = ssorted[sk];
                 rec.x += e_ox[e];
                 rec.y += e_oy[e];
-                if constexpr (SOA) {
+                if constexpr (SOA && KIND == SE2P_SLP) {
                     bxy[k] = make_double2(rec.x, rec.y);
                     bfd[k] = make_double2(rec.z, rec.w);
+                } else if constexpr (SOA) {
+                    const DlpRec& q = reinterpret_cast<const DlpRec&>(rec);
+                    bxy[k] = make_double2(q.x, q.y);
+                    bfd[k] = make_double2(q.f1, q.f2);
+                    bnr[k] = make_double2(q.n1, q.n2);
                 } else {
                     buf[k] = rec;
                 }
@@ -697,13 +759,29 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                     bool more = (lo | hi) != 0;
                     int idx = more ? staged(next(lo, hi)) : 0;
                     double2 q = bxy[idx], g = bfd[idx];
+                    double2 m = KIND == SE2P_DLP ? bnr[idx] : make_double2(0.0, 0.0);
                     while (more) {
                         more = (lo | hi) != 0;
                         const int nidx = more ? staged(next(lo, hi)) : idx;
                         const double2 qn = bxy[nidx], gn = bfd[nidx];
+                        const double2 mn = KIND == SE2P_DLP ? bnr[nidx] : m;
                         const double dx = pt.x - q.x, dy = pt.y - q.y;
                         const double r2 = dx * dx + dy * dy;
-                        if constexpr (F32) {
+                        if constexpr (KIND == SE2P_DLP && F32) {
+                            float b1, b2;
+                            dlp_pair2_f32((float)dx, (float)dy, (float)r2, (float)xi2,
+                                          (float)g.x, (float)g.y, (float)m.x, (float)m.y,
+                                          (float)fi1, (float)fi2, (float)ni1, (float)ni2, c1, c2,
+                                          b1, b2);
+                            atomicAdd(&racc[idx].x, (double)b1);
+                            atomicAdd(&racc[idx].y, (double)b2);
+                        } else if constexpr (KIND == SE2P_DLP) {
+                            double b1, b2;
+                            dlp_pair2(dx, dy, r2, xi2, g.x, g.y, m.x, m.y, fi1, fi2, ni1, ni2, a1,
+                                      a2, b1, b2);
+                            atomicAdd(&racc[idx].x, b1);
+                            atomicAdd(&racc[idx].y, b2);
+                        } else if constexpr (F32) {
                             float b1, b2;
                             slp_pair2_f32((float)dx, (float)dy, (float)r2, (float)xi2,
                                           (float)g.x, (float)g.y, (float)fi1, (float)fi2, c1, c2,
@@ -720,6 +798,7 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
                         idx = nidx;
                         q = qn;
                         g = gn;
+                        m = mn;
                     }
                     lo = hi = 0;
                 }
@@ -809,12 +888,12 @@ __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(4))
             const int o = tperm[ti];
             constexpr double dv = out_div(KIND);
             if (SYM) {
-                // the pair of the target with itself: the self term
+                // the Stokeslet's pair of the target with itself: the self term
+                const double s1 = KIND == SE2P_SLP ? a1 + self * fi1 : a1;
+                const double s2 = KIND == SE2P_SLP ? a2 + self * fi2 : a2;
                 glob<double>* a = sacc + 2 * ti;
-                __hip_atomic_fetch_add(a, a1 + self * fi1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(a + 1, a2 + self * fi2, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(a, s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(a + 1, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (KIND >= SE2P_SLP_GRADIENT) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) u[NC * o + c] = acc[c] / dv;
@@ -918,17 +997,27 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
     dim3 gr(njobs), bl(kBlkThreads);
     const bool uni = mode == RM_ONE_CELL;
     if (mode == RM_SYM) {
-        if (kind != SE2P_SLP)
-            return set_error(SE2P_ERR_UNSUPPORTED, "pair-symmetric sums are Stokeslet only");
-        if (tier == kE1Full)
-            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Full, false, true>), gr, bl, 0, s, sys,
-                               jobs, u);
-        else if (tier == kF32)
-            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kF32, false, true>), gr, bl, 0, s, sys,
-                               jobs, u);
-        else
-            hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Fast, false, true>), gr, bl, 0, s, sys,
-                               jobs, u);
+        if (kind == SE2P_SLP) {
+            if (tier == kE1Full)
+                hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Full, false, true>), gr, bl, 0, s,
+                                   sys, jobs, u);
+            else if (tier == kF32)
+                hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kF32, false, true>), gr, bl, 0, s,
+                                   sys, jobs, u);
+            else
+                hipLaunchKernelGGL((k_real_stencil<SE2P_SLP, kE1Fast, false, true>), gr, bl, 0, s,
+                                   sys, jobs, u);
+        } else if (kind == SE2P_DLP) {
+            // the stresslet has no E1: its FP64 tiers are the same sums
+            if (tier == kF32)
+                hipLaunchKernelGGL((k_real_stencil<SE2P_DLP, kF32, false, true>), gr, bl, 0, s,
+                                   sys, jobs, u);
+            else
+                hipLaunchKernelGGL((k_real_stencil<SE2P_DLP, kE1Full, false, true>), gr, bl, 0, s,
+                                   sys, jobs, u);
+        } else {
+            return set_error(SE2P_ERR_UNSUPPORTED, "pair-symmetric sums are velocity sums only");
+        }
         SE2P_HIP_CHECK(hipGetLastError());
         return SE2P_OK;
     }

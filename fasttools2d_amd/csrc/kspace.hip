@@ -705,6 +705,7 @@ __global__ void k_fill_dens(const SysDesc* __restrict__ sys, const Job* __restri
         r[2] = f1;
         r[3] = f2;
         r[6] = f1 * r[4] + f2 * r[5];
+        if (d.sym) gp(d.sacc)[i] = make_double2(0.0, 0.0);
     }
 }
 

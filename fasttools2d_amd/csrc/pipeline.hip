@@ -75,10 +75,10 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->s_perm = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SRC_START, sizeof(int) * (size_t)(nbox + 1), &p));
         G->s_start = (int*)p;
-        // pair-symmetric Stokeslet sums need the targets to be the sources (they
+        // pair-symmetric velocity sums need the targets to be the sources (they
         // then share the sources' order) and a k-space part (its gather adds them)
-        const bool sym = prm.symmetric == 1 && kind == SE2P_SLP && d_psrc == d_ptar && Ns == Nt &&
-                         want_k;
+        const bool sym = prm.symmetric == 1 && (kind == SE2P_SLP || kind == SE2P_DLP) &&
+                         d_psrc == d_ptar && Ns == Nt && want_k;
         if (sym) {
             G->t_perm = G->s_perm;
             G->t_start = G->s_start;

@@ -302,7 +302,7 @@ def select(ctx, net, kind, points, Lx, Ly, tol, nrm=None, check_seed=99, symmetr
     recipe is used."""
     n = len(points)
     k = _lib.kind_code(kind)
-    sym = symmetric if k == _lib.SE2P_SLP else 0
+    sym = symmetric if k in (_lib.SE2P_SLP, _lib.SE2P_DLP) else 0
     y = net.predict(ctx, features(kind, n, tol, Lx, Ly, points))[0]
     prm = params_from_outputs(y, kind, n, tol, Lx, Ly, symmetric=sym)
     rng = np.random.default_rng(check_seed)
@@ -381,7 +381,7 @@ def evaluate(ctx, net, configs, log=None):
     rows = []
     for i, c in enumerate(configs):
         kind, p, n, tol, Lx, Ly = c["kind"], c["points"], c["N"], c["tol"], c["Lx"], c["Ly"]
-        sym = 1 if kind == "slp" else 0
+        sym = 1 if kind in ("slp", "dlp") else 0
         f = c["f"]
         acc = analytic_params(kind, n, 1e-13, Lx, Ly, f)
         acc["precision"] = 0

@@ -134,11 +134,13 @@ typedef struct {
     int nside_x, nside_y, Mx, My, P, kinf;
     int converged; /* 0 when a Newton solve hit its iteration cap */
     int e1_tier;   /* real-space E1: 0 full FP64 (reference), 1 rel. error < 1e-12 */
-    /* Stokeslet real space with the sources as the targets (plans and batch
-     * systems whose d_psrc and d_ptar are the same pointer): 1 evaluates each
-     * pair once and adds it to both points with atomics, so the sums' order, and
-     * their last bits, vary from run to run; 0 (the reference's order) sums each
-     * target's pairs in a fixed order.  Ignored for other kinds and inputs. */
+    /* Real space of the velocity sums (Stokeslet and stresslet) with the
+     * sources as the targets (plans and batch systems whose d_psrc and d_ptar
+     * are the same pointer, with a k-space part): 1 evaluates each pair once and
+     * adds it to both points with atomics, so the sums' order, and their last
+     * bits, vary from run to run; 0 (the reference's order) sums each target's
+     * pairs in a fixed order.  Ignored for other kinds and inputs, and where
+     * a point lies outside the box (se2p_plan_params reports what is in use). */
     int symmetric;
     int precision; /* SE2P_FP64 or SE2P_FP32C */
     /* Fast E1 tier of the Stokeslet (e1_tier 1, FP64): real-space pairs with

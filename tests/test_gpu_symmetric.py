@@ -1,4 +1,5 @@
-"""Pair-symmetric Stokeslet real space (sources = targets, se2p_params.symmetric).
+"""Pair-symmetric real space of the velocity sums (sources = targets,
+se2p_params.symmetric): the Stokeslet, and the stresslet at the end.
 
 Each pair is evaluated once and added to both points with atomics, so results
 are compared to tolerance, not bit for bit: against the fixed-order sums of the
@@ -160,21 +161,79 @@ def test_batch_mixes_symmetric_and_fixed_order_systems(gpu_ctx):
             np.testing.assert_array_equal(u, ref)
 
 
-def test_stresslet_ignores_the_flag(gpu_ctx):
-    N = 3000
-    p = uniform_points(N, 1.0, 1.0, seed=81)
-    th = np.random.default_rng(82).random(N) * 2 * np.pi
-    n = np.stack([np.cos(th), np.sin(th)], 1)
-    f = normal_density(N, seed=83)
-    prm = ft.ewald_params("dlp", N, N, 2 * N + 1.0, 1.0, 1.0, tol=1e-10)
-    out = []
-    for sym in (0, 1):
-        plan = Plan(gpu_ctx, "dlp", p, p, 1.0, 1.0, dict(prm, symmetric=sym), nrm=n)
-        try:
-            out.append(plan.apply(f))
-        finally:
-            plan.close()
-    np.testing.assert_array_equal(out[0], out[1])
+def _dlp_apply(ctx, p, f, n, Lx, Ly, prm, sym, graph=False, reps=1):
+    plan = Plan(ctx, "dlp", p, p, Lx, Ly, dict(prm, symmetric=1 if sym else 0), nrm=n)
+    try:
+        used = plan.params_in_use()["symmetric"]
+        return [plan.apply(f, graph=graph) for _ in range(reps)], used
+    finally:
+        plan.close()
+
+
+def _normals(N, seed):
+    th = np.random.default_rng(seed).random(N) * 2 * np.pi
+    return np.stack([np.cos(th), np.sin(th)], 1)
+
+
+@pytest.mark.parametrize("case", ["uniform", "duplicates", "narrow", "cluster"])
+def test_stresslet_pairs_once_for_both_points(gpu_ctx, oracle, case):
+    """The stresslet's pair-symmetric sums (its kernel is odd in r, so a pair's
+    term for the source is the target's expression negated, with the target's
+    density and normal) against its fixed-order sums and the oracle: uniform
+    points, exact duplicates (coincident pairs are skipped both ways), a box
+    one cell wide, and a dense cluster (windows in several rounds, blocks of
+    more than 256 targets)."""
+    Lx, Ly, nsx, nsy = 1.0, 1.0, 18, 18
+    if case == "uniform":
+        p = uniform_points(6000, Lx, Ly, seed=81)
+    elif case == "duplicates":
+        q = uniform_points(2500, Lx, Ly, seed=82)
+        p = np.concatenate([q, q[:500]])
+    elif case == "narrow":
+        Lx, Ly, nsx, nsy = 3.0, 1.0, 3, 1
+        p = uniform_points(2000, Lx, Ly, seed=83)
+    else:
+        rng = np.random.default_rng(84)
+        p = np.concatenate([uniform_points(3000, Lx, Ly, seed=85),
+                            np.clip(rng.normal(-0.2, 0.01, (2500, 2)), -0.49, 0.49)])
+    N = len(p)
+    n = _normals(N, 86)
+    f = normal_density(N, seed=87)
+    prm = dict(ft.ewald_params("dlp", N, N, 2 * N + 1.0, Lx, Ly, tol=1e-10))
+    prm["nside_x"], prm["nside_y"], prm["rc"] = nsx, nsy, Lx / nsx
+    (ref,), used0 = _dlp_apply(gpu_ctx, p, f, n, Lx, Ly, prm, False)
+    got, used1 = _dlp_apply(gpu_ctx, p, f, n, Lx, Ly, prm, True, reps=2)
+    assert (used0, used1) == (0, 1)
+    for u in got:
+        assert _rel(u, ref) < REL
+        assert np.any(u != ref)  # the pairs were summed in the other order
+    r = oracle.dlp_real(p, p, f, n, prm["xi"], nsx, nsy, Lx, Ly)
+    r += oracle.dlp_kspace(p, p, prm["xi"], prm["eta"], f, n, prm["Mx"], prm["My"], Lx, Ly,
+                           prm["w"], prm["P"])
+    r += oracle.dlp_zero_mode(p, f, n, Lx, Ly)
+    assert _rel(got[0], r) < REL
+
+
+def test_stresslet_fp32_tier_and_graph_replays(gpu_ctx, oracle):
+    """The stresslet's pair-symmetric sums in the FP32 tier within 1e-6 of the
+    oracle, and FP64 graph replays equal to their eager applies to rounding."""
+    N = 8000
+    p = uniform_points(N, 1.0, 1.0, seed=88)
+    n = _normals(N, 89)
+    f = normal_density(N, seed=90)
+    prm = dict(ft.ewald_params("dlp", N, N, 2 * N + 1.0, 1.0, 1.0, tol=1e-10))
+    prm["nside_x"] = prm["nside_y"] = 21
+    prm["rc"] = 1.0 / 21
+    r = oracle.dlp_real(p, p, f, n, prm["xi"], 21, 21, 1.0, 1.0)
+    r += oracle.dlp_kspace(p, p, prm["xi"], prm["eta"], f, n, prm["Mx"], prm["My"], 1.0, 1.0,
+                           prm["w"], prm["P"])
+    r += oracle.dlp_zero_mode(p, f, n, 1.0, 1.0)
+    (u32,), used = _dlp_apply(gpu_ctx, p, f, n, 1.0, 1.0, dict(prm, precision=1), True)
+    assert used == 1 and _rel(u32, r) < 1e-6
+    (eager,), _ = _dlp_apply(gpu_ctx, p, f, n, 1.0, 1.0, prm, True)
+    replays, _ = _dlp_apply(gpu_ctx, p, f, n, 1.0, 1.0, prm, True, graph=True, reps=3)
+    for u in replays:
+        assert _rel(u, eager) < REL and _rel(u, r) < REL
 
 
 def test_full_size_tuned_split_matches_fixed_order(gpu_ctx):
