@@ -15,6 +15,7 @@
 // grid (Z[ix*My + iy]), and x lines after a tiled transpose.
 #include "se2p_internal.h"
 #include "fft_roots.h"
+#include "kfilter.h"
 
 #include <cmath>
 #include <vector>
@@ -266,7 +267,8 @@ template <int R, bool FIRST, bool LAST, int T, int PPT, typename S>
 __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __restrict__ lds,
                                                     typename Mem<S>::glb* __restrict__ gdata,
                                                     int n, int nb, int Ns,
-                                                    const glb_d2* __restrict__ tw, int inverse)
+                                                    const glb_d2* __restrict__ tw, int inverse,
+                                                    int gstride)
 {
     using V = cplx<S>;
     constexpr int BPT = (PPT + R - 1) / R;  // butterflies per thread at most
@@ -288,7 +290,7 @@ __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __rest
             if constexpr (FIRST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    v[t][r] = conj_if(ld2<S>(gdata, b * n + j + r * m), inverse);
+                    v[t][r] = conj_if(ld2<S>(gdata, b * gstride + j + r * m), inverse);
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[t][r] = ld2<S>(lds, b * n + j + r * m);
@@ -303,7 +305,7 @@ __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __rest
                 }
             }
             butterfly<R, V>(v[t], (S)-1.0);
-            ob[t] = b * n + jq * Ns * R + jm;
+            ob[t] = b * (LAST ? gstride : n) + jq * Ns * R + jm;
         }
     }
     if (FIRST == LAST) __syncthreads();  // inputs and outputs share memory
@@ -327,7 +329,8 @@ template <bool FIRST, bool LAST, int T, int PPT, typename S>
 __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __restrict__ lds,
                                                         typename Mem<S>::glb* __restrict__ gdata,
                                                         int n, int nb, int Ns, int R,
-                                                        const glb_d2* __restrict__ tw, int inverse)
+                                                        const glb_d2* __restrict__ tw, int inverse,
+                                                        int gstride)
 {
     using V = cplx<S>;
     constexpr int OPT = PPT;  // outputs per thread at most
@@ -347,7 +350,7 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
             divmod(qq, m, rm, k, j);
             divmod(j, Ns, rNs, jq, jm);
             const int step = (jm + k * Ns) * base;  // < n
-            const int i0 = b * n + j;
+            const int i0 = b * (FIRST ? gstride : n) + j;
             S re = 0, im = 0;
             int e = 0;
             for (int r = 0; r < R; ++r) {
@@ -364,7 +367,7 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
                 if (e >= n) e -= n;
             }
             res[t] = mkc<V>(re, im);
-            ob[t] = b * n + jq * Ns * R + jm + k * Ns;
+            ob[t] = b * (LAST ? gstride : n) + jq * Ns * R + jm + k * Ns;
         }
     }
     if (FIRST == LAST) __syncthreads();
@@ -413,10 +416,10 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
         const int R = ax.radix[st];
         const bool first = st == 0, last = st == nst - 1;
 #define SE2P_STAGE(RR)                                                                        \
-    if (first && last) stage_reg<RR, true, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse); \
-    else if (first) stage_reg<RR, true, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse);  \
-    else if (last) stage_reg<RR, false, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse);   \
-    else stage_reg<RR, false, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse)
+    if (first && last) stage_reg<RR, true, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n); \
+    else if (first) stage_reg<RR, true, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n);  \
+    else if (last) stage_reg<RR, false, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n);   \
+    else stage_reg<RR, false, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
@@ -434,18 +437,130 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
 #undef SE2P_COMPOSITE
             default:
                 if (first && last)
-                    stage_generic<true, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<true, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
                 else if (first)
-                    stage_generic<true, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<true, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
                 else if (last)
-                    stage_generic<false, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<false, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
                 else
-                    stage_generic<false, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse);
+                    stage_generic<false, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
                 break;
         }
 #undef SE2P_STAGE
         Ns *= R;
     }
+}
+
+// The stages of one transform over nb lines in `buf`.  GIN: the first stage
+// reads the lines from HBM (g, line b at b gstride), else from `buf`; GOUT: the
+// last stage writes them to HBM (g), else to `buf`.
+template <int T, int PPT, bool GIN, bool GOUT, typename S>
+__device__ __forceinline__ void line_stages(typename Mem<S>::lds* __restrict__ buf,
+                                            typename Mem<S>::glb* __restrict__ g, int gstride,
+                                            int n, int nb, const FftAxis& ax,
+                                            const glb_d2* __restrict__ tw, int inverse)
+{
+    const int nst = ax.nst;
+    int Ns = 1;
+    for (int st = 0; st < nst; ++st) {
+        const int R = ax.radix[st];
+        const bool first = GIN && st == 0, last = GOUT && st == nst - 1;
+#define SE2P_STAGE(RR)                                                                          \
+    if (first && last)                                                                          \
+        stage_reg<RR, true, true, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);          \
+    else if (first)                                                                             \
+        stage_reg<RR, true, false, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);         \
+    else if (last)                                                                              \
+        stage_reg<RR, false, true, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);         \
+    else                                                                                        \
+        stage_reg<RR, false, false, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride)
+        switch (R) {
+            case 2: SE2P_STAGE(2); break;
+            case 3: SE2P_STAGE(3); break;
+            case 4: SE2P_STAGE(4); break;
+            case 5: SE2P_STAGE(5); break;
+            case 8: SE2P_STAGE(8); break;
+            default:
+                if (first && last)
+                    stage_generic<true, true, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse, gstride);
+                else if (first)
+                    stage_generic<true, false, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
+                                                          gstride);
+                else if (last)
+                    stage_generic<false, true, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
+                                                          gstride);
+                else
+                    stage_generic<false, false, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
+                                                           gstride);
+                break;
+        }
+#undef SE2P_STAGE
+        Ns *= R;
+    }
+}
+
+// The filter of rows p and q = -p (mod My) held in buf (nb = 1: row p is its own
+// partner), in place, each output stored conjugated (the inverse transform runs
+// the forward stages on conj(U)).  Not inlined: its constants then live in its
+// own register budget instead of the kernel's.  The periodic Stokeslet filter
+// only: the free-space one's Bessel functions would not fit beside the stages.
+template <int T, typename S>
+__device__ __attribute__((noinline)) void filter_rows(typename Mem<S>::lds* __restrict__ buf,
+                                                      const SysDesc& d, int n, int nb, int p,
+                                                      int q)
+{
+    using V = cplx<S>;
+    for (int ix = threadIdx.x; ix < n; ix += T) {
+        const int jx = ix ? n - ix : 0;
+        if (nb == 1 && jx < ix) continue;  // taken with its partner
+        const int im = (nb - 1) * n + jx;  // buf index of mode (-ix, -p)
+        const V a = ld2<S>(buf, ix), b = ld2<S>(buf, im);
+        const double2 wk = make_double2(a.x, a.y), wm = make_double2(b.x, b.y);
+        const double2 o = kf::slp_filter_periodic(d, ix, p, wk, wm);
+        st2(buf, ix, mkc<V>((S)o.x, -(S)o.y));
+        if (im != ix) {
+            const double2 r = kf::slp_filter_periodic(d, jx, q, wm, wk);
+            st2(buf, im, mkc<V>((S)r.x, -(S)r.y));
+        }
+    }
+}
+
+// The Stokeslet's x-direction passes with the filter between them, for x lines
+// of 512 to 2048 points: job (s, p) loads the x lines p and My - p of the
+// transposed forward spectrum W (the rows of modes k and -k), transforms them
+// in LDS, filters every mode with its partner (unpack, Stokes filter, k = 0
+// removal, 1/(Mx My), repack: the same filter_mode as k_filter), transforms
+// them back and writes the rows of Z0.  Rows 0 and My/2 are their own partners.
+// It runs the same stages and filter arithmetic as the separate passes on values
+// that stay in LDS instead of HBM; the results agree with theirs to rounding
+// (within 4e-16 relative on FP64 grids, where instruction selection differs in
+// the last bits; FP32 grids bit for bit), and the grid makes one HBM round trip
+// instead of three.  512 threads, 8 points each: two
+// lines of up to 2048 points.
+template <typename S>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_xfilter(
+    const SysDesc* __restrict__ sys,
+                                                     const Job* __restrict__ jobs)
+{
+    constexpr int T = 512, PPT = 8;
+    using LP = typename Mem<S>::lds;
+    using GP = typename Mem<S>::glb;
+    extern __shared__ double2 lds[];
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const int n = d.Mx, My = d.My;
+    const int p = jb.y, q = p ? My - p : 0;
+    const int nb = q == p ? 1 : 2;
+    const int gs = (q - p) * n;  // line 1 starts gs elements after line 0 (unused for nb = 1)
+    const glb_d2* __restrict__ tw = (const glb_d2*)d.twx;
+    LP* __restrict__ buf = (LP*)lds;
+    line_stages<T, PPT, true, false, S>(buf, (GP*)d.grid[G_W] + (size_t)p * n, gs, n, nb, d.ax,
+                                        tw, 0);
+    // the stages end with a barrier: every forward value is in buf
+    filter_rows<T, S>(buf, d, n, nb, p, q);
+    __syncthreads();
+    line_stages<T, PPT, false, true, S>(buf, (GP*)d.grid[G_Z0] + (size_t)p * n, gs, n, nb, d.ax,
+                                        tw, 1);
 }
 
 // out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows
@@ -613,6 +728,23 @@ int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, in
         fft_lines_launch<float>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
     else
         fft_lines_launch<double>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+bool fft_xfilter_fits(int kind, int Mx, double free_R)
+{
+    return kind == SE2P_SLP && !(free_R > 0) && Mx >= 512 && Mx <= 2048;
+}
+
+int launch_fft_xfilter(hipStream_t s, int prec, size_t max_lds, const SysDesc* sys,
+                       const Job* jobs, int njobs)
+{
+    if (njobs <= 0) return SE2P_OK;
+    if (prec)
+        hipLaunchKernelGGL(k_fft_xfilter<float>, dim3(njobs), dim3(512), max_lds, s, sys, jobs);
+    else
+        hipLaunchKernelGGL(k_fft_xfilter<double>, dim3(njobs), dim3(512), max_lds, s, sys, jobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -304,5 +304,27 @@ __device__ __forceinline__ void filter_mode(const SysDesc& d, int ix, int iy, do
     o0 = make_double2((g1.re - g2.im) * scale, (g1.im + g2.re) * scale);
 }
 
+// filter_mode<SE2P_SLP> of a periodic system (free_R = 0), the same arithmetic
+// in the same order without the free-space branch, for the fused x-line
+// transform, whose registers it has to share with the transform stages.
+__device__ __forceinline__ double2 slp_filter_periodic(const SysDesc& d, int ix, int iy,
+                                                       double2 wk, double2 wm)
+{
+    const int Mx = d.Mx, My = d.My;
+    if (ix == 0 && iy == 0) return make_double2(0.0, 0.0);
+    const int jx = ix ? Mx - ix : 0, jy = iy ? My - iy : 0;
+    const double k1 = wavenum(ix, Mx, d.Lx), k2 = wavenum(iy, My, d.Ly);
+    const double m1 = wavenum(jx, Mx, d.Lx), m2 = wavenum(jy, My, d.Ly);
+    const double a = 0.25 / (d.xi * d.xi), eta = d.eta;
+    const double scale = 1.0 / ((double)Mx * (double)My);
+    cplx q1, q2;
+    unpack(wk, wm, q1, q2);
+    cplx p1, p2, r1, r2;
+    slp_apply(k1, k2, a, eta, q1, q2, p1, p2);
+    slp_apply(m1, m2, a, eta, q1, q2, r1, r2);
+    const cplx g1 = cscale(cadd(p1, r1), 0.5), g2 = cscale(cadd(p2, r2), 0.5);
+    return make_double2((g1.re - g2.im) * scale, (g1.im + g2.re) * scale);
+}
+
 }  // namespace kf
 }  // namespace se2p

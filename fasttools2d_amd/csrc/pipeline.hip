@@ -427,7 +427,8 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     for (int kind = 0; kind < kNumKinds; ++kind) {
         for (int prec = 0; prec < 2; ++prec)
             add(B->filter[prec][kind], [&](int i, const SysDesc& d) {
-                if (geoms[i]->has_k && d.kind == kind && d.prec == prec)
+                if (geoms[i]->has_k && d.kind == kind && d.prec == prec &&
+                    !fft_xfilter_fits(d.kind, d.Mx, d.free_R))
                     blocks(i, ((long)d.Mx * (d.My / 2 + 1) + 255) / 256);
             });
         for (int tier = 0; tier < 3; ++tier)
@@ -464,7 +465,8 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                         blocks(i, (d.Mx + d.By - 1) / d.By);
                 });
                 add(B->fftx[prec][c][cls], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && fft_line_class(d.Mx) == cls)
+                    if (want(i, d) && fft_line_class(d.Mx) == cls &&
+                        !(c == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
                         blocks(i, (d.My + d.Bx - 1) / d.Bx);
                 });
             }
@@ -478,7 +480,8 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             };
             for (int cls = 0; cls < kFftClasses; ++cls) {
                 add(B->ifftx[prec][g][cls], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && fft_line_class(d.Mx) == cls)
+                    if (want(i, d) && fft_line_class(d.Mx) == cls &&
+                        !(g == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
                         blocks(i, (d.My + d.Bx - 1) / d.Bx);
                 });
                 add(B->iffty[prec][g][cls], [&](int i, const SysDesc& d) {
@@ -489,6 +492,14 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             add(B->tr_inv[prec][g], [&](int i, const SysDesc& d) {
                 if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
             });
+            if (g == 0)
+                add(B->xfilt[prec], [&](int i, const SysDesc& d) {
+                    if (want(i, d) && fft_xfilter_fits(d.kind, d.Mx, d.free_R)) {
+                        blocks(i, d.My / 2 + 1);
+                        B->lds_xf[prec] = std::max(B->lds_xf[prec],
+                                                   (prec ? 8 : 16) * 2 * (size_t)d.Mx);
+                    }
+                });
             for (int q = 0; q < kNumP; ++q)
                 add(B->gather[prec][g][q], [&](int i, const SysDesc& d) {
                     if (want(i, d) && d.Nt > 0 && d.P == 2 * (q + 1)) gather_jobs(i, *geoms[i]);
@@ -569,6 +580,7 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
     // (Z[ix*My + iy]), then a transpose to Z[iy*Mx + ix] for the x lines, the
     // filter in that layout, and back.
     //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
+    //        (x lines of 512-2048 points: xFFT, filter and xIFFT as one launch)
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     // Systems of the FP32 tier run the same passes on float2 grids.
@@ -588,6 +600,12 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
                                       B.tr_fwd[prec][c].count));
             SE2P_TRY(fft(prec, 1, Tc, 0, B.fftx[prec][c], B.lds_x[prec]));
         }
+    }
+    {
+        StageTimer st(ctx, s, "fft");
+        for (int prec = 0; prec < 2; ++prec)
+            SE2P_TRY(launch_fft_xfilter(s, prec, B.lds_xf[prec], B.sys, J + B.xfilt[prec].first,
+                                        B.xfilt[prec].count));
     }
     {
         StageTimer st(ctx, s, "filter");

@@ -394,6 +394,12 @@ int fft_line_class(int n);
 int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls, size_t max_lds,
                      const SysDesc* sys, const Job* jobs, int njobs);
 size_t fft_lds_bytes(int n, int prec);
+// The Stokeslet's forward x passes, filter and inverse x passes as one launch
+// (W rows p and My - p -> Z0 rows p and My - p; job (s, p), p = 0 .. My/2), for
+// periodic systems where fft_xfilter_fits(kind, Mx, free_R); LDS 2 Mx complex values.
+bool fft_xfilter_fits(int kind, int Mx, double free_R);
+int launch_fft_xfilter(hipStream_t s, int prec, size_t max_lds, const SysDesc* sys,
+                       const Job* jobs, int njobs);
 // Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
 // and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
 int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
@@ -474,6 +480,8 @@ struct BatchExec {
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
     JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses], tr_inv[2][2];  // [prec][g][class]
     JobRange gather[2][2][kNumP];  // [prec][g][P / 2 - 1]
+    JobRange xfilt[2];              // [prec]: fused x passes and filter (Stokeslet)
+    size_t lds_xf[2] = {};
     size_t lds_y[2][kFftClasses] = {}, lds_x[2][kFftClasses] = {};  // [prec][line class]
     int maxR = 32;   // largest gathering region side
 };
