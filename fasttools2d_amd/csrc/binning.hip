@@ -77,6 +77,43 @@ __global__ void k_tile_keys(const double* __restrict__ p, int n, double Lx, doub
     atomicAdd(&count[k], 1);
 }
 
+__global__ void k_strip_count(const double* __restrict__ p, int n, double Lx, double Ly,
+                              double h, int P, int Mx, int My, int ntx, int* __restrict__ cnt,
+                              int2* __restrict__ node, double2* __restrict__ off)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int mx, my;
+    double px, py;
+    closest_node(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+    const int wx = ((mx % Mx) + Mx) % Mx;
+    const int wy = ((my % My) + My) % My;
+    node[i] = make_int2(wx, wy);
+    off[i] = make_double2(px, py);
+    int c = 0;
+    for_each_strip(wx, Mx, P, ntx, [&](int, int) { ++c; });
+    cnt[i] = c;
+}
+
+__global__ void k_strip_keys(const int2* __restrict__ node, const int* __restrict__ offs, int n,
+                             int P, int Mx, int My, int ntx, int* __restrict__ key,
+                             int* __restrict__ count, int2* __restrict__ rnode,
+                             int* __restrict__ owner)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 m = node[i];
+    int o = offs[i];
+    for_each_strip(m.x, Mx, P, ntx, [&](int s, int ox) {
+        const int k = s * My + m.y;
+        key[o] = k;
+        rnode[o] = make_int2(ox, m.y);
+        owner[o] = i;
+        ++o;
+        atomicAdd(&count[k], 1);
+    });
+}
+
 __global__ void k_closest_nodes(const double* __restrict__ p, int n, double Lx, double Ly,
                                 double h, int P, int* __restrict__ mx, int* __restrict__ my,
                                 double* __restrict__ px, double* __restrict__ py)
@@ -117,6 +154,57 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ count, in
         run += count[i];
     }
     if (t == 1023) start[n] = part[1023];
+}
+
+// Longer scans in three passes: per-tile sums (kScanTile entries per 256-lane
+// workgroup, 16 contiguous per lane), a k_scan of the tile sums, then each tile
+// rescanned from its offset.
+constexpr int kScanTile = 4096;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* part, int& total)
+{
+    const int t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int u = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += u;
+        __syncthreads();
+    }
+    total = part[255];
+    const int r = part[t] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const int* __restrict__ count, int n,
+                                                    int* __restrict__ tsum)
+{
+    __shared__ int part[256];
+    const int b = blockIdx.x * kScanTile + threadIdx.x * 16;
+    int s = 0;
+    for (int i = b; i < min(n, b + 16); ++i) s += count[i];
+    int total;
+    block_exclusive_scan(s, part, total);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int* __restrict__ count, int n,
+                                                    const int* __restrict__ toff,
+                                                    int* __restrict__ start)
+{
+    __shared__ int part[256];
+    const int b = blockIdx.x * kScanTile + threadIdx.x * 16;
+    int s = 0;
+    for (int i = b; i < min(n, b + 16); ++i) s += count[i];
+    int total;
+    int run = toff[blockIdx.x] + block_exclusive_scan(s, part, total);
+    for (int i = b; i < min(n, b + 16); ++i) {
+        start[i] = run;
+        run += count[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = toff[gridDim.x];
 }
 
 __global__ void k_copy_int(const int* __restrict__ a, int* __restrict__ b, int n)
@@ -396,13 +484,54 @@ int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly
     return SE2P_OK;
 }
 
+int launch_strip_count(hipStream_t s, const double* p, int n, double Lx, double Ly, double h,
+                       int P, int Mx, int My, int ntx, int* cnt, int2* node, double2* off)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_strip_count, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, Lx, Ly, h, P,
+                       Mx, My, ntx, cnt, node, off);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int launch_strip_keys(hipStream_t s, const int2* node, const int* offs, int n, int P, int Mx,
+                      int My, int ntx, int* key, int* count, int2* rnode, int* owner)
+{
+    if (n <= 0) return SE2P_OK;
+    hipLaunchKernelGGL(k_strip_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, node, offs, n, P,
+                       Mx, My, ntx, key, count, rnode, owner);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
+int scan_ints(se2p_ctx* ctx, Workspace* ws, const int* count, int n, int* start)
+{
+    hipStream_t s = ctx->stream;
+    if (n <= 8 * kScanTile) {
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, n, start);
+        SE2P_HIP_CHECK(hipGetLastError());
+        return SE2P_OK;
+    }
+    const int nt = (n + kScanTile - 1) / kScanTile;
+    void* tmp = nullptr;
+    SE2P_TRY(ws_get(ctx, ws, WS_SCAN, sizeof(int) * (2 * (size_t)nt + 1), &tmp));
+    int* tsum = (int*)tmp;
+    int* toff = tsum + nt;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, count, n, tsum);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, tsum, nt, toff);
+    SE2P_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, count, n, toff, start);
+    SE2P_HIP_CHECK(hipGetLastError());
+    return SE2P_OK;
+}
+
 int sort_by_key(se2p_ctx* ctx, Workspace* ws, const int* key, int* count, int n, int nkeys,
                 int* perm, int* start, Slot tmp_slot)
 {
     hipStream_t s = ctx->stream;
     // count[] was filled by the key kernel; start = exclusive scan.
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, nkeys, start);
-    SE2P_HIP_CHECK(hipGetLastError());
+    SE2P_TRY(scan_ints(ctx, ws, count, nkeys, start));
     if (n <= 0) return SE2P_OK;
     // count[] is reused as the scatter cursor.
     hipLaunchKernelGGL(k_copy_int, dim3(grid_for(nkeys, 256)), dim3(256), 0, s, start, count,

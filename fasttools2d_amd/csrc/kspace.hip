@@ -8,11 +8,14 @@
 // fast-Gaussian-gridding recurrence wx[i] = exp(t(px^2 + 2 w px)) e3x^i e1[i],
 // e3x = exp(-2 t h px), e1[i] = exp(t h^2 (i - P/2)^2), t = -2 xi^2/eta.
 //
-// Spreading.  Points are binned by the tile (kSB x kSB start nodes) their
-// start node falls in, in increasing index order.  Each kSB x kSB block of the
-// grid sums the points whose support reaches it as the matrix product
-// H_c = A^T diag(d_c) B, A[k][x] = wx_k(x), B[k][y] = wy_k(y) (zero outside
-// the support), on the FP64 matrix cores (k_spread below).
+// Spreading.  For supports of up to 16 nodes (P <= 14), each strip of kSB grid
+// columns sums the points whose support reaches it as matrix products H_c =
+// A^T diag(d_c) B, A[k][x] = wx_k(x) over the strip's columns, B[k][y] =
+// wy_k(y) over a ring of rows that follows the start rows down the strip, on
+// the FP64 matrix cores (k_spread below); a point has one record per strip
+// (and periodic image) it reaches, binned by (strip, start row) in increasing
+// point order.  Wider supports are summed per 16 x 16 block of grid nodes
+// (k_spread_block), the points binned by tiles of 16 x 16 start nodes.
 //
 // Two real grids travel as one complex grid Z = H_0 + i H_1, so the SLP needs
 // one forward and one inverse complex FFT; the DLP spreads (f1 n1, f2 n1) and
@@ -20,6 +23,7 @@
 #include "se2p_internal.h"
 #include "kfilter.h"
 
+#include <climits>
 #include <type_traits>
 
 namespace se2p {
@@ -42,8 +46,314 @@ __device__ __forceinline__ void point_density(int mode, int p,
     }
 }
 
-// Spreading.  One 64-lane wave per kSB x kSB block of grid nodes, which it
-// owns.  It visits every point whose support reaches the block -- the points
+// Spreading, supports of up to 16 nodes.  One 64-lane wave per strip of kSB
+// grid columns [X0, X0 + kSB) and segment of rows [Y0, Y1).  Its matrix-core
+// accumulators (the 16 x 16 output of v_mfma_f64_16x16x4f64, one per component)
+// hold the strip's columns over a ring of R = 16 rows: grid row v lives in
+// output column v mod R.  (The code is written for R = 16 NT, NT ring tiles,
+// but instantiated for NT = 1 only.)  A record (a
+// point seen by the strip, with start column o relative to X0 and start row vy)
+// adds d wx(x) wy(v) over the rows v = vy .. vy + P; its y weights go to the
+// ring slots of those rows (zero elsewhere), so every multiply-add in y falls
+// on the support, and its x weights over the strip's columns (zero off the
+// support).  Each v_mfma step adds four records, A = their x rows, B = d_c
+// times their y rows rotated to the records' start slots.
+//
+// The strip's records are binned by start row, so the segment's start rows Y0 -
+// P .. Y1 - 1 are one or (across the periodic wrap) a few contiguous runs.  The
+// ring is the window of rows [F, F + R), F the first row not yet complete: a
+// record with start row vy < F + R - P fits, so the records are multiplied in in
+// order while they do; at the first that does not, rows F .. F + R - P - 1 are
+// complete -- those in [Y0, Y1) are stored, all are cleared to become rows F + R
+// .. -- and F advances by R - P.  Rows before Y0 only build up the ring (they
+// belong to the segment above).  Records are staged in chunks of kSC: lanes 0-31
+// fill their x rows in LDS and lanes 32-63 their y rows (the reference's
+// recurrence, e1 table in scalar registers, one code path), and a ballot over
+// the chunk's start rows finds how many fit before the next batch of rows must
+// be stored.  Records are visited in a fixed order, so every node sum is
+// bit-reproducible.
+constexpr int kSC = 32;  // records staged per chunk
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// A weight row in LDS, R + 2 entries: the weight of support node i (0 <= i <= P)
+// goes to entry o + i -- x rows: o = start column - the strip's first column,
+// nodes outside [0, kSB) to the padding slot kSB -- or (o + i) mod R -- y rows:
+// o = the start row's ring slot, so the row is already rotated to the ring.
+// The first R entries are cleared first, so the stores need no branch.
+template <int R>
+__device__ __forceinline__ int row_slot(bool ring, int x)
+{
+    return ring ? (x >= R ? x - R : x) : (x >= 0 && x < kSB ? x : kSB);
+}
+
+// FP64: e e3^i e1[i], one factor e3 per node as in the reference's recurrence.
+template <int P, int R>
+__device__ __forceinline__ void fill_row(double* row, bool ring, int o, double e, double e3,
+                                         const double (&e1)[P + 1])
+{
+#pragma unroll
+    for (int x = 0; x < R; x += 2) *reinterpret_cast<double2*>(row + x) = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        row[row_slot<R>(ring, o + i)] = e * e1[i];
+        e *= e3;
+    }
+}
+
+// FP32: exp(t (px - h (i - P/2))^2) with th2 = t h^2 and q = px/h, each weight
+// evaluated directly (the FP32 recurrence would accumulate a rounding per node).
+template <int P, int R>
+__device__ __forceinline__ void fill_row_f32(float* row, bool ring, int o, float q, float th2)
+{
+#pragma unroll
+    for (int x = 0; x < R; x += 2) *reinterpret_cast<float2*>(row + x) = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const float z = q - (float)(i - P / 2);
+        row[row_slot<R>(ring, o + i)] = __expf(th2 * z * z);
+    }
+}
+
+// (hi, lo) += v exactly as a pair (Knuth's two-sum), so a strip's FP32 sums
+// carry no rounding from one run of matrix steps to the next.
+__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
+{
+#pragma clang fp contract(off)
+    const float s = hi + v;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (v - bb);
+    hi = s;
+    lo += err;
+}
+
+// S = double: the FGG recurrence of the reference and v_mfma_f64_16x16x4f64,
+// accumulated straight into the ring.  S = float (the FP32 tier): weights
+// evaluated directly, v_mfma_f32_16x16x4f32 into a product local to each run of
+// steps, folded into (hi, lo) pairs by two-sum; rows are stored as float2 (hi +
+// lo rounded once).
+template <int P, typename S>
+__global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
+                                               const Job* __restrict__ jobs, int c)
+{
+    constexpr bool F32 = std::is_same<S, float>::value;
+    static_assert(spread_ring(P), "wider supports use k_spread_block");
+    constexpr int NT = (P + 16) / 16;  // ring tiles: ceil((P + 1) / 16)
+    constexpr int R = 16 * NT;         // ring rows
+    constexpr int BR = R - P;          // rows completed per batch
+    constexpr int kLen = R + 2;        // LDS row: consecutive rows start in different banks
+    const Job jb = jobs[blockIdx.x];
+    const SysDesc& d = sys[jb.x];
+    const glob<const int>* __restrict__ tstart = gp(d.sp_start);
+    const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
+    // FGG factors per record as two halves: (ex, e3x), (ey, e3y)
+    const glob<const double2>* __restrict__ fac = gp(reinterpret_cast<const double2*>(d.sp_fac));
+    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.snrec;
+    const int Mx = d.Mx, My = d.My, ntx = d.ntx;
+    double e1[P + 1];  // the same for every lane: held in scalar registers
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const double v = gp(d.e1tab)[i];
+        const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+        const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+        e1[i] = __hiloint2double(hi, lo);
+    }
+    __shared__ __align__(16) S W[2][kSC][kLen];  // x rows, y rows (ring order)
+    __shared__ double2 DD[kSC];
+
+    const int lane = threadIdx.x;
+    // records past a chunk's count are read (and multiplied by a zero x weight):
+    // keep every y row and density finite from the start
+    for (int i = lane; i < kSC * kLen; i += 64) (&W[1][0][0])[i] = (S)0;
+    if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
+    const int strip = jb.y % ntx;
+    const int X0 = strip * kSB;
+    const int Y0 = (jb.y / ntx) * d.sseg, Y1 = min(Y0 + d.sseg, My);
+    const bool xr = lane < kSC;  // staging: lanes 0-31 fill x rows, 32-63 y rows
+    const int q = lane & (kSC - 1);
+    const int row = lane >> 4, col = lane & 15;  // MFMA operand: record 4 s + row, node col
+    f64x4 acc0[NT], acc1[NT];
+    float h0[NT][4], l0[NT][4], h1[NT][4], l1[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        acc0[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+        acc1[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h0[t][r] = l0[t][r] = h1[t][r] = l1[t][r] = 0.f;
+    }
+    const float th2 = (float)(d.t * d.h * d.h);
+    glob<double2>* __restrict__ out64 = gp(d.grid[c ? G_Z1 : G_Z0]);
+    glob<float2>* __restrict__ out32 = gp(reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]));
+
+    // this lane's output column of register r: f64 product element (x, y) in lane
+    // y + 16 (x % 4), register x / 4; f32 product: lane y + 16 (x / 4), register x % 4
+    glob<double2>* col64[4];
+    glob<float2>* col32[4];
+    bool colok[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ix = F32 ? X0 + 4 * row + r : X0 + 4 * r + row;
+        colok[r] = ix < Mx;
+        col64[r] = out64 + (size_t)min(ix, Mx - 1) * My;
+        col32[r] = out32 + (size_t)min(ix, Mx - 1) * My;
+    }
+    int F = Y0 - P;  // first row of the ring window
+    // rows F .. F + BR - 1 are complete: store those of the segment, clear all
+    auto flush = [&]() {
+        int fm = F % R;
+        fm = fm < 0 ? fm + R : fm;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            int off = 16 * t + col - fm;
+            off = off < 0 ? off + R : off;
+            const int v = F + off;
+            const bool done = off < BR, keep = done && v >= Y0 && v < Y1;
+            if constexpr (F32) {
+                const float z = done ? 0.f : 1.f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (keep && colok[r])
+                        col32[r][v] = make_float2(h0[t][r] + l0[t][r], h1[t][r] + l1[t][r]);
+                    h0[t][r] *= z;
+                    l0[t][r] *= z;
+                    h1[t][r] *= z;
+                    l1[t][r] *= z;
+                }
+            } else {
+                const double z = done ? 0.0 : 1.0;  // finite sums: x 1 keeps them exactly
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (keep && colok[r]) col64[r][v] = make_double2(acc0[t][r], acc1[t][r]);
+                acc0[t] *= z;
+                acc1[t] *= z;
+            }
+        }
+        F += BR;
+    };
+    // the matrix steps of the staged records [lo, hi), two at a time (operands
+    // of both read before either multiplies)
+    auto steps = [&](int lo, int hi) {
+        f32x4 c0[NT], c1[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) c0[t] = c1[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int st = lo >> 2; 4 * st < hi; st += 2) {
+            const bool two = 4 * (st + 1) < hi;
+            S a[2], wy[2][NT];
+            double2 dd[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int kk = (4 * (st + u) + row) & (kSC - 1);
+                const S wx = W[0][kk][col];
+                a[u] = kk >= lo && kk < hi && (u == 0 || two) ? wx : (S)0;
+                dd[u] = DD[kk];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) wy[u][t] = W[1][kk][16 * t + col];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (u == 1 && !two) break;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if constexpr (F32) {
+                        c0[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], (S)dd[u].x * wy[u][t],
+                                                                     c0[t], 0, 0, 0);
+                        c1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], (S)dd[u].y * wy[u][t],
+                                                                     c1[t], 0, 0, 0);
+                    } else {
+                        acc0[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], dd[u].x * wy[u][t],
+                                                                       acc0[t], 0, 0, 0);
+                        acc1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], dd[u].y * wy[u][t],
+                                                                       acc1[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if constexpr (F32) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    two_sum_add(h0[t][r], l0[t][r], c0[t][r]);
+                    two_sum_add(h1[t][r], l1[t][r], c1[t][r]);
+                }
+        }
+    };
+    // The segment's start rows Y0 - P .. Y1 - 1 as runs within one period, each
+    // a contiguous range of records; run() moves to the next non-empty one.
+    int a = Y0 - P, kb = 0, ke = 0, shift = 0;
+    auto run = [&]() -> bool {
+        while (a < Y1) {
+            int r0 = a % My;
+            r0 = r0 < 0 ? r0 + My : r0;
+            const int b = min(Y1, a + (My - r0));
+            shift = a - r0;  // start row of a record = its grid row + shift
+            kb = tstart[strip * My + r0];
+            ke = tstart[strip * My + r0 + (b - a)];
+            a = b;
+            if (ke > kb) return true;
+        }
+        return false;
+    };
+    // chunks of kSC records, the next one's records loaded while the current
+    // one is staged and multiplied
+    int2 m = make_int2(0, 0), m_n = make_int2(0, 0);
+    double2 f = make_double2(0.0, 0.0), f_n = f, dd = f, dd_n = f;
+    auto load = [&](int p0, int end, int2& mm, double2& ff, double2& dv) {
+        const int k = p0 + q;
+        if (k < end) {
+            mm = snode[k];
+            ff = fac[2 * (size_t)k + (xr ? 0 : 1)];
+            if (!xr) dv = dens[k];
+        }
+    };
+    bool have = run();
+    int p0 = kb, end = ke, sh = shift;
+    if (have) load(p0, end, m, f, dd);
+    while (have) {
+        int p0n = p0 + kSC, endn = end, shn = sh;
+        bool haven = true;
+        if (p0n >= end) {
+            haven = run();
+            p0n = kb;
+            endn = ke;
+            shn = shift;
+        }
+        if (haven) load(p0n, endn, m_n, f_n, dd_n);
+        const int n = min(kSC, end - p0);
+        const int vy = m.y + sh;  // start row of this lane's record
+        __syncthreads();          // the previous chunk's rows have been read
+        if (q < n) {
+            int sl = vy % R;
+            sl = sl < 0 ? sl + R : sl;
+            if constexpr (F32)
+                fill_row_f32<P, R>(W[xr ? 0 : 1][q], !xr, xr ? m.x : sl, (float)f.x, th2);
+            else
+                fill_row<P, R>(W[xr ? 0 : 1][q], !xr, xr ? m.x : sl, f.x, f.y, e1);
+            if (!xr) DD[q] = dd;
+        }
+        __syncthreads();
+        // records are in start-row order: those that fit the ring are a prefix
+        for (int pos = 0;;) {
+            const int cnt = __popcll(__ballot(lane < n && lane >= pos && vy < F + BR));
+            if (cnt) steps(pos, pos + cnt);
+            pos += cnt;
+            if (pos >= n) break;
+            flush();
+        }
+        have = haven;
+        p0 = p0n;
+        end = endn;
+        sh = shn;
+        m = m_n;
+        f = f_n;
+        dd = dd_n;
+    }
+    while (F < Y1) flush();
+}
+
+// Spreading for supports of more than 16 nodes (P >= 16), where the ring would
+// need two or three tiles: one 64-lane wave per kSB x kSB block of grid nodes,
+// which it owns.  It visits every point whose support reaches the block -- the points
 // of the spreading tiles (kSB x kSB start nodes, binned at build) that cover
 // the block's window of start nodes [X0 - P, X0 + kSB) x [Y0 - P, Y0 + kSB),
 // one periodic image at a time (several on grids narrower than the window) --
@@ -55,9 +365,7 @@ __device__ __forceinline__ void point_density(int mode, int p,
 // first 32 lanes fill the x rows, the other 32 the y rows).  The block is
 // written to the grid once, zeros included.  Points are visited in a fixed
 // order (image, tile, binned order), so every node sum is bit-reproducible.
-constexpr int kSC = 32;       // points staged per chunk
-constexpr int kRow = kSB + 2;  // padded LDS row: consecutive lanes' rows start in different banks
-typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kBRow = kSB + 2;  // padded LDS row: consecutive lanes' rows start in different banks
 
 // row[x], x in [0, kSB): the weight of support node i = x - o (0 <= i <= P) of a
 // point whose start node lies at block offset o, e e3^i e1[i] with one factor
@@ -65,7 +373,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // nodes outside the block go to the row's padding slot kSB, so the stores
 // need no branch.
 template <int P>
-__device__ __forceinline__ void fill_row(double* row, int o, double e, double e3,
+__device__ __forceinline__ void block_row(double* row, int o, double e, double e3,
                                          const double (&e1)[P + 1])
 {
 #pragma unroll
@@ -78,13 +386,11 @@ __device__ __forceinline__ void fill_row(double* row, int o, double e, double e3
     }
 }
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
 // FP32 rows: the weights exp(t (px - h (i - P/2))^2) of support nodes i, with
 // th2 = t h^2 and q = px/h, each evaluated directly (the FP32 recurrence would
 // accumulate a rounding per node).
 template <int P>
-__device__ __forceinline__ void fill_row_f32(float* row, int o, float q, float th2)
+__device__ __forceinline__ void block_row_f32(float* row, int o, float q, float th2)
 {
 #pragma unroll
     for (int x = 0; x < kSB; x += 2) *reinterpret_cast<float2*>(row + x) = make_float2(0.f, 0.f);
@@ -96,25 +402,13 @@ __device__ __forceinline__ void fill_row_f32(float* row, int o, float q, float t
     }
 }
 
-// (hi, lo) += v exactly as a pair (Knuth's two-sum), so a block's FP32 sum
-// carries no rounding from one chunk to the next.
-__device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
-{
-#pragma clang fp contract(off)
-    const float s = hi + v;
-    const float bb = s - hi;
-    const float err = (hi - (s - bb)) + (v - bb);
-    hi = s;
-    lo += err;
-}
-
 // S = double: the FGG recurrence of the reference and v_mfma_f64_16x16x4f64,
 // accumulated straight into the block's two 16 x 16 products.  S = float (the
 // FP32 tier): weights evaluated directly, v_mfma_f32_16x16x4f32 into a
 // chunk-local product, folded into (hi, lo) pairs by two-sum after every chunk;
 // the block is stored as float2 (hi + lo rounded once).
 template <int P, typename S>
-__global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__ sys,
                                                const Job* __restrict__ jobs, int c)
 {
     constexpr bool F32 = std::is_same<S, float>::value;
@@ -123,7 +417,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const glob<const int>* __restrict__ tstart = gp(d.sp_start);
     const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
     const glob<const double4>* __restrict__ fac = gp(d.sp_fac);
-    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.Ns;
+    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.snrec;
     const int Mx = d.Mx, My = d.My, ntx = d.ntx;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
@@ -133,8 +427,8 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
         e1[i] = __hiloint2double(hi, lo);
     }
-    __shared__ __align__(16) S WX[kSC][kRow];
-    __shared__ __align__(16) S WY[kSC][kRow];
+    __shared__ __align__(16) S WX[kSC][kBRow];
+    __shared__ __align__(16) S WY[kSC][kBRow];
     __shared__ double4 SF[kSC];
     __shared__ double2 DD[kSC];
     __shared__ int OX[kSC], OY[kSC];
@@ -142,7 +436,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const int lane = threadIdx.x;
     // rows past a chunk's count are read (and multiplied by a zero x weight) in
     // its last matrix step: keep every y row and density finite from the start
-    for (int i = lane; i < kSC * kRow; i += 64) (&WY[0][0])[i] = (S)0;
+    for (int i = lane; i < kSC * kBRow; i += 64) (&WY[0][0])[i] = (S)0;
     if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
     const int X0 = (jb.y % ntx) * kSB, Y0 = (jb.y / ntx) * kSB;
     const int XE = min(X0 + kSB, Mx), YE = min(Y0 + kSB, My);
@@ -249,10 +543,10 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
                 const bool xr = lane < kSC;
                 const double4 f4 = SF[q];
                 if constexpr (F32)
-                    fill_row_f32<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
+                    block_row_f32<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q],
                                     (float)(xr ? f4.x : f4.z), th2);
                 else
-                    fill_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q], xr ? f4.x : f4.z,
+                    block_row<P>(xr ? WX[q] : WY[q], xr ? OX[q] : OY[q], xr ? f4.x : f4.z,
                                 xr ? f4.y : f4.w, e1);
             }
             __syncthreads();
@@ -317,19 +611,22 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     }
 }
 
-// Per binned point: its start node and either the FGG factors (ex, e3x, ey,
-// e3y) of the FP64 recurrence or, for the FP32 tier (direct), its offsets in
-// grid spacings (px/h, 0, py/h, 0).
-__global__ void k_spread_records(const int* __restrict__ perm, const int2* __restrict__ node,
+// Per binned record (a point of a tile for the block kernel, owner null): its
+// strip- or grid-relative start node, its point and either the point's FGG factors (ex, e3x, ey, e3y) of the FP64 recurrence or, for the
+// FP32 tier (direct), its offsets in grid spacings (px/h, 0, py/h, 0).
+__global__ void k_spread_records(const int* __restrict__ rperm, const int2* __restrict__ rnode,
+                                 const int* __restrict__ owner,
                                  const double2* __restrict__ off, int n, double t, double w,
                                  double h, int direct, int2* __restrict__ snode,
-                                 double4* __restrict__ fac)
+                                 double4* __restrict__ fac, int* __restrict__ pidx)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const int p = perm[k];
+    const int r = rperm[k];
+    const int p = owner ? owner[r] : r;
     const double2 o = off[p];
-    snode[k] = node[p];
+    snode[k] = rnode[r];
+    pidx[k] = p;
     if (direct)
         fac[k] = make_double4(o.x / h, 0.0, o.y / h, 0.0);
     else
@@ -357,7 +654,7 @@ __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int k = jb.y * 256 + threadIdx.x;
-    if (k >= d.Ns) return;
+    if (k >= d.snrec) return;
     const int p = gp(d.sp_perm)[k];
     const glob<const double>* __restrict__ f = gp(f_base + d.f_off);
     const glob<const double>* __restrict__ nrm = gp(d.nrm);
@@ -371,7 +668,7 @@ __global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__
     point_density(1, p, f, nrm, d0, d1);
     out[k] = make_double2(d0, d1);
     point_density(2, p, f, nrm, d0, d1);
-    out[(size_t)d.Ns + k] = make_double2(d0, d1);
+    out[(size_t)d.snrec + k] = make_double2(d0, d1);
 }
 
 // Each mode and its Hermitian partner (-k) are filtered together from the same
@@ -725,10 +1022,26 @@ int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi,
     g->xi = xi;
     g->eta = eta;
     g->w = w;
-    // spreading tiles and blocks: kSB x kSB start nodes / grid nodes
-    g->TX = g->TY = kSB;
     g->ntx = (Mx + kSB - 1) / kSB;
-    g->nty = (My + kSB - 1) / kSB;
+    g->ring = spread_ring(P);
+    if (g->ring) {
+        // one wave per strip of kSB grid columns and segment of sseg rows (records
+        // binned by strip and start row); segments as long as a launch of 4096
+        // waves allows, at least 32 rows (each starts P rows early to build up
+        // its ring)
+        g->TX = kSB;
+        g->TY = 1;
+        g->sseg = 128;
+        while (g->sseg > 32 && (long)g->ntx * ((My + g->sseg - 1) / g->sseg) < 4096)
+            g->sseg /= 2;
+        g->nty = (My + g->sseg - 1) / g->sseg;
+    } else {
+        // one wave per block of kSB x kSB grid nodes, points binned by tiles of
+        // kSB x kSB start nodes
+        g->TX = g->TY = kSB;
+        g->sseg = 0;
+        g->nty = (My + kSB - 1) / kSB;
+    }
     // gathering tiles: 32 x 32 start nodes (region up to 56 x 56 for P <= 24) for
     // 256 lanes per tile
     g->gTX = g->gTY = P <= 24 ? 32 : 16;
@@ -753,7 +1066,13 @@ static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const 
 #define SE2P_SPREAD_P(PP)                                                                         \
     case PP: hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c); break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
-        SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
+        SE2P_SPREAD_P(12) SE2P_SPREAD_P(14)
+#undef SE2P_SPREAD_P
+#define SE2P_SPREAD_P(PP)                                                                         \
+    case PP:                                                                                      \
+        hipLaunchKernelGGL((k_spread_block<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c);   \
+        break;
+        SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
         SE2P_SPREAD_P(22) SE2P_SPREAD_P(24) SE2P_SPREAD_P(26) SE2P_SPREAD_P(28) SE2P_SPREAD_P(30)
         SE2P_SPREAD_P(32) SE2P_SPREAD_P(34) SE2P_SPREAD_P(36) SE2P_SPREAD_P(38) SE2P_SPREAD_P(40)
 #undef SE2P_SPREAD_P
@@ -772,13 +1091,14 @@ int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, con
     return SE2P_OK;
 }
 
-int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* perm,
-                          const int2* node, const double2* off, int n, int2* snode, double4* fac)
+int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* rperm,
+                          const int2* rnode, const int* owner, const double2* off, int n,
+                          int2* snode, double4* fac, int* pidx)
 {
     if (n <= 0) return SE2P_OK;
     const double t = -2 * g.xi * g.xi / g.eta;
-    hipLaunchKernelGGL(k_spread_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
-                       n, t, g.w, g.h, prec, snode, fac);
+    hipLaunchKernelGGL(k_spread_records, dim3((n + 255) / 256), dim3(256), 0, s, rperm, rnode,
+                       owner, off, n, t, g.w, g.h, prec, snode, fac, pidx);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

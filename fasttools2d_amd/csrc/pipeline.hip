@@ -179,28 +179,27 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         SE2P_TRY(fft_twiddles(ctx, ws, Mx, WS_TWIDDLE_X, &G->twx));
         SE2P_TRY(fft_twiddles(ctx, ws, My, WS_TWIDDLE_Y, &G->twy));
         const KGeom& g = G->kg;
-        const int ntiles = g.ntx * g.nty, gtiles = g.gntx * g.gnty;
+        // spreading bins: (strip, start row), strip-major (ring), or tiles of
+        // kSB x kSB start nodes (blocks)
+        const long nbins = g.ring ? (long)g.ntx * My : (long)g.ntx * g.nty;
+        const int gtiles = g.gntx * g.gnty;
+        if (nbins >= INT_MAX) return set_error(SE2P_ERR_UNSUPPORTED, "grid too large");
         const size_t Gn = (size_t)Mx * My;
-        void *skey, *scount, *tkey, *tcount;
+        void *skey, *scount, *tkey, *tcount, *scnt = nullptr, *soffs = nullptr, *rnode = nullptr,
+             *owner = nullptr, *rperm;
         SE2P_TRY(ws_get(ctx, ws, WS_E1TAB, sizeof(double) * 64, &p));
         G->e1tab = (double*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_KEY, sizeof(int) * (size_t)Ns, &skey));
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_COUNT, sizeof(int) * (size_t)ntiles, &scount));
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_PERM, sizeof(int) * (size_t)Ns, &p));
-        G->sp_perm = (int*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_START, sizeof(int) * (size_t)(ntiles + 1), &p));
-        G->sp_start = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_NODE, sizeof(int2) * (size_t)Ns, &p));
         G->sp_node = (int2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_OFF, sizeof(double2) * (size_t)Ns, &p));
         G->sp_off = (double2*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_SNODE, sizeof(int2) * (size_t)Ns, &p));
-        G->sp_snode = (int2*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_FAC, sizeof(double4) * (size_t)Ns, &p));
-        G->sp_fac = (double4*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
-                        sizeof(double2) * (dlp_family(kind) ? 2 : 1) * (size_t)Ns, &p));
-        G->sp_dens = (double2*)p;
+        if (g.ring) {
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_CNT, sizeof(int) * (size_t)std::max(Ns, 1), &scnt));
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_OFFS, sizeof(int) * ((size_t)Ns + 1), &soffs));
+        }
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_COUNT, sizeof(int) * (size_t)nbins, &scount));
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_START, sizeof(int) * (size_t)(nbins + 1), &p));
+        G->sp_start = (int*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_KEY, sizeof(int) * (size_t)Nt, &tkey));
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_COUNT, sizeof(int) * (size_t)gtiles, &tcount));
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_PERM, sizeof(int) * (size_t)Nt, &p));
@@ -227,13 +226,51 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         }
         StageTimer st(ctx, "bin_grid");
         SE2P_TRY(launch_e1_table(s, g, G->e1tab));
-        SE2P_TRY(launch_fill_int(s, (int*)scount, ntiles, 0));
-        SE2P_TRY(launch_tile_keys(s, d_psrc, Ns, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
-                                  (int*)skey, (int*)scount, G->sp_node, G->sp_off));
-        SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, Ns, ntiles, G->sp_perm,
+        // spreading records (ring: count per point, offsets, then keys; blocks:
+        // one per point)
+        int nrec = Ns;
+        if (g.ring) {
+            SE2P_TRY(launch_strip_count(s, d_psrc, Ns, Lx, Ly, g.h, P, Mx, My, g.ntx, (int*)scnt,
+                                        G->sp_node, G->sp_off));
+            SE2P_TRY(scan_ints(ctx, ws, (const int*)scnt, Ns, (int*)soffs));
+            SE2P_HIP_CHECK(hipMemcpyAsync(&nrec, (int*)soffs + Ns, sizeof(int),
+                                          hipMemcpyDeviceToHost, s));
+            SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        G->sp_nrec = nrec;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_KEY, sizeof(int) * (size_t)std::max(nrec, 1), &skey));
+        if (g.ring) {
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_RNODE, sizeof(int2) * (size_t)std::max(nrec, 1),
+                            &rnode));
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_OWNER, sizeof(int) * (size_t)std::max(nrec, 1),
+                            &owner));
+        }
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_RPERM, sizeof(int) * (size_t)std::max(nrec, 1), &rperm));
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_PERM, sizeof(int) * (size_t)std::max(nrec, 1), &p));
+        G->sp_perm = (int*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_SNODE, sizeof(int2) * (size_t)std::max(nrec, 1), &p));
+        G->sp_snode = (int2*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_FAC, sizeof(double4) * (size_t)std::max(nrec, 1), &p));
+        G->sp_fac = (double4*)p;
+        SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
+                        sizeof(double2) * (dlp_family(kind) ? 2 : 1) * (size_t)std::max(nrec, 1),
+                        &p));
+        G->sp_dens = (double2*)p;
+        SE2P_TRY(launch_fill_int(s, (int*)scount, (int)nbins, 0));
+        if (g.ring) {
+            SE2P_TRY(launch_strip_keys(s, G->sp_node, (const int*)soffs, Ns, P, Mx, My, g.ntx,
+                                       (int*)skey, (int*)scount, (int2*)rnode, (int*)owner));
+        } else {
+            SE2P_TRY(launch_tile_keys(s, d_psrc, Ns, Lx, Ly, g.h, P, Mx, My, g.TX, g.TY, g.ntx,
+                                      (int*)skey, (int*)scount, G->sp_node, G->sp_off));
+            rnode = G->sp_node;  // one record per point: its grid start node
+            owner = nullptr;
+        }
+        SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, nrec, (int)nbins, (int*)rperm,
                              G->sp_start, WS_SORT_TMP));
-        SE2P_TRY(launch_spread_records(s, g, G->prec, G->sp_perm, G->sp_node, G->sp_off, Ns,
-                                       G->sp_snode, G->sp_fac));
+        SE2P_TRY(launch_spread_records(s, g, G->prec, (const int*)rperm, (const int2*)rnode,
+                                       (const int*)owner, G->sp_off, nrec, G->sp_snode,
+                                       G->sp_fac, G->sp_perm));
         SE2P_TRY(launch_fill_int(s, (int*)tcount, gtiles, 0));
         SE2P_TRY(launch_tile_keys(s, d_ptar, Nt, Lx, Ly, g.h, P, Mx, My, g.gTX, g.gTY, g.gntx,
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
@@ -318,6 +355,8 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->TY = g.TY;
         d->ntx = g.ntx;
         d->nty = g.nty;
+        d->sseg = g.sseg;
+        d->snrec = G.sp_nrec;
         d->gTX = g.gTX;
         d->gTY = g.gTY;
         d->gntx = g.gntx;
@@ -448,7 +487,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         if (geoms[i]->has_real && has_zero_mode(d.kind, d.free_R)) blocks(i, 1);
     });
     add(B->sdens, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k) blocks(i, (d.Ns + 255) / 256);
+        if (geoms[i]->has_k) blocks(i, (d.snrec + 255) / 256);
     });
     for (int prec = 0; prec < 2; ++prec) {
         for (int c = 0; c < 2; ++c) {

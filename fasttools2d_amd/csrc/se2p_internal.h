@@ -19,9 +19,27 @@ constexpr int kNumKinds = SE2P_NUM_KINDS;
 // Gaussian support: P even in [2, kMaxP] (P + 1 grid nodes per direction)
 constexpr int kMaxP = 40;
 constexpr int kNumP = kMaxP / 2;
-// spreading: tiles of kSB x kSB start nodes, each grid block of kSB x kSB nodes
-// summed by one wave (the 16 x 16 output of an FP64 matrix-core step)
+// spreading: strips of kSB grid columns (the 16 x 16 output of an FP64
+// matrix-core step is 16 columns by 16 ring rows) for supports of up to 16
+// nodes, blocks of kSB x kSB grid nodes for wider ones
 constexpr int kSB = 16;
+__host__ __device__ constexpr bool spread_ring(int P) { return P + 1 <= kSB; }
+
+// Spreading records.  A point with start column sx reaches strip s (grid
+// columns [kSB s, kSB s + kSB)) through periodic image kx (virtual start column
+// vx = sx - kx Mx) when its support [vx, vx + P] meets the strip; it has one
+// record per such (image, strip), visited image by image (kx = 0, 1, ... while
+// vx + P >= 0), strips in increasing order: f(strip, vx - kSB strip).
+template <class F>
+__host__ __device__ inline void for_each_strip(int sx, int Mx, int P, int ntx, F f)
+{
+    for (int kx = 0; sx - kx * Mx + P >= 0; ++kx) {
+        const int vx = sx - kx * Mx;
+        const int lo = vx > kSB - 1 ? (vx - (kSB - 1) + kSB - 1) / kSB : 0;
+        const int hi0 = (vx + P) / kSB, hi = hi0 < ntx - 1 ? hi0 : ntx - 1;
+        for (int s = lo; s <= hi; ++s) f(s, vx - kSB * s);
+    }
+}
 // Stresslet kinds: normals, two spreading passes (f n1, f n2).
 __host__ __device__ constexpr bool dlp_family(int kind)
 {
@@ -123,7 +141,8 @@ enum Slot {
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
     WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
-    WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_YSORT, WS_BIGCELLS, WS_NUM_SLOTS
+    WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_YSORT, WS_BIGCELLS, WS_SPR_CNT, WS_SPR_OFFS, WS_SPR_RNODE,
+    WS_SPR_OWNER, WS_SPR_RPERM, WS_SCAN, WS_NUM_SLOTS
 };
 
 // Grow-only device buffers addressed by slot (one set per context, one per plan).
@@ -196,8 +215,18 @@ struct StageTimer {
 // floor(nside*(x/L + 0.5)) with clamping.
 int launch_cell_keys(hipStream_t s, const double* p, int n, double Lx, double Ly, int nsx,
                      int nsy, int* key, int* count);
-// Grid-tile keys of the spreading / gathering tiles; also stores each point's
-// wrapped start node (mx, my) and in-cell offset (px, py).
+// Spreading records, first pass: each point's start node (mx, my), offset (px,
+// py) and number of records (for_each_strip).
+int launch_strip_count(hipStream_t s, const double* p, int n, double Lx, double Ly, double h,
+                       int P, int Mx, int My, int ntx, int* cnt, int2* node, double2* off);
+// Second pass: point i's records at offs[i] .. offs[i + 1] - 1 get the key
+// strip My + my (count[] incremented), rnode = (vx - kSB strip, my) and owner i.
+int launch_strip_keys(hipStream_t s, const int2* node, const int* offs, int n, int P, int Mx,
+                      int My, int ntx, int* key, int* count, int2* rnode, int* owner);
+// Exclusive scan of count[0..n) into start[0..n] (on ctx's stream).
+int scan_ints(se2p_ctx* ctx, Workspace* ws, const int* count, int n, int* start);
+// Grid-tile keys of the gathering tiles (TX x TY start nodes, ntx per row, row-major);
+// also stores each point's wrapped start node (mx, my) and in-cell offset (px, py).
 int launch_tile_keys(hipStream_t s, const double* p, int n, double Lx, double Ly, double h,
                      int P, int Mx, int My, int TX, int TY, int ntx, int* key, int* count,
                      int2* node, double2* off);
@@ -231,7 +260,11 @@ int launch_closest_nodes(hipStream_t s, const double* p, int n, double Lx, doubl
 struct KGeom {
     int Mx, My, P;
     double Lx, Ly, h, xi, eta, w;
-    int TX, TY, ntx, nty;  // spreading tiles (TX x TY start nodes) = grid blocks
+    // spreading: ring (strips of kSB columns x nty segments of sseg rows, records
+    // binned by strip and start row) or blocks (ntx x nty blocks of kSB x kSB
+    // nodes, points binned by TX x TY tiles of start nodes)
+    int TX, TY, ntx, nty;
+    int ring, sseg;
     int gTX, gTY, gntx, gnty, gR;  // gathering tiles, region gR x gR
 };
 int kgeom_init(KGeom* g, int Mx, int My, int P, double Lx, double Ly, double xi, double eta,
@@ -299,13 +332,15 @@ struct SysDesc {
     double *zm, *zm_part;
     int zm_nblk;
     // k space
-    int Mx, My, P, TX, TY, ntx, nty, Bx, By;
+    int Mx, My, P, TX, TY, ntx, nty, sseg, snrec, Bx, By;
     int gTX, gTY, gntx, gnty, gR;
     double t, w, h, xi, eta, gscale;
     const int *sp_start, *sp_perm, *gt_start, *gt_perm;
-    const int2 *sp_snode, *gt_node;  // spreading: start nodes in binned order
-    const double4* sp_fac;  // spreading: FGG factors (ex, e3x, ey, e3y) in binned order
-    double2* sp_dens;       // spreading: (d0, d1) of pair c at c Ns + k, binned order
+    // spreading records in binned order (snrec of them): sp_perm = their points,
+    // sp_snode = (start column - strip's first column, start row)
+    const int2 *sp_snode, *gt_node;
+    const double4* sp_fac;  // spreading: FGG factors (ex, e3x, ey, e3y) per record
+    double2* sp_dens;       // spreading: (d0, d1) of pair c at c snrec + k, per record
     const double2* gt_off;  // gathering: offsets in binned order (gt_node likewise)
     const int* gt_sidx;     // pair-symmetric sums: sorted position of each binned target
     const double *e1tab, *nrm;
@@ -346,18 +381,23 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
 // pos[perm[k]] = k for k < n.
 int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
-// or Z1 (c = 1); job b = grid block (b % ntx, b / ntx) of kSB x kSB nodes; all
+// or Z1 (c = 1); job b = strip b % ntx of kSB grid columns and segment b / ntx of
+// sseg rows (spread_ring(P)) or block (b % ntx, b / ntx) of kSB x kSB nodes; all
 // jobs' systems of support P.
 int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
                   int njobs);
-// Spreading records of a geometry: start node and FGG factors of the k-th binned point.
-int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* perm, const int2* node,
-                          const double2* off, int n, int2* snode, double4* fac);
+// Spreading records of a geometry in binned order: record k = rperm[k] of the
+// key pass gets snode[k] = rnode[rperm[k]], its point pidx[k] = owner[rperm[k]]
+// (rperm[k] itself when owner is null: one record per point) and that point's
+// FGG factors.
+int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* rperm,
+                          const int2* rnode, const int* owner, const double2* off, int n,
+                          int2* snode, double4* fac, int* pidx);
 // Gathering records in binned order: bnode[k] = node[perm[k]], boff[k] =
 // off[perm[k]] and, when spos is given, sidx[k] = spos[perm[k]].
 int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
                           const int* spos, int n, int2* bnode, double2* boff, int* sidx);
-// Densities of the binned points of every spreading pair (job (s, b): points
+// Densities of the spreading records of every pair (job (s, b): records
 // b*256 .. b*256+255 of system s).
 int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
                        const double* f);
@@ -444,6 +484,7 @@ struct Geom {
     double2 *twx = nullptr, *twy = nullptr;
     double* e1tab = nullptr;
     int *sp_perm = nullptr, *sp_start = nullptr;
+    int sp_nrec = 0;  // spreading records
     int2* sp_node = nullptr;
     double2* sp_off = nullptr;
     int2* sp_snode = nullptr;

@@ -38,7 +38,7 @@ def _gauss(xi, P, L, M):
 
 
 @pytest.mark.parametrize("N,Mx,My,P", [(3000, 64, 64, 8), (5000, 96, 48, 14), (2000, 50, 34, 16),
-                                       (1500, 40, 40, 24)])
+                                       (1500, 40, 40, 24), (800, 6, 4, 4), (6000, 96, 200, 12)])
 def test_fp32_spread_matches_oracle(ctx32, oracle, N, Mx, My, P):
     Lx = 1.0
     Ly = Lx * My / Mx

@@ -121,6 +121,40 @@ def test_spread_block_edges_match_oracle(gpu_ctx, oracle, N, Mx, My, P):
     assert _rel(H1.T, R1) < RTOL and _rel(H2.T, R2) < RTOL
 
 
+@pytest.mark.parametrize("N,Mx,My,P,dist", [
+    (300, 6, 4, 2, "uniform"),       # grid narrower than a strip: many images each way
+    (500, 16, 6, 14, "uniform"),     # one strip; rows wrap inside the first segment
+    (900, 50, 34, 6, "uniform"),     # partial last strip
+    (3000, 96, 200, 14, "uniform"),  # several segments, the last one partial
+    (4000, 130, 66, 10, "uniform"),
+    (2500, 40, 300, 12, "uniform"),
+    (20000, 128, 128, 14, "clusters"),  # dense start rows: chunks spanning many rows
+    (20000, 64, 256, 4, "clusters"),
+])
+def test_spread_ring_edges_match_oracle(gpu_ctx, oracle, N, Mx, My, P, dist):
+    """The strip spreading (supports of up to 16 nodes): grids narrower than a
+    strip, periodic images in both directions, partial strips and segments,
+    and clustered points whose records fill many chunks per ring batch."""
+    from fasttools2d_amd.workloads import clusters
+    Lx = 1.0
+    Ly = Lx * My / Mx
+    xi = 4.0 * Mx / 64 + 2.0
+    h = Lx / Mx
+    w = P * h / 2
+    eta = (2 * xi * w / (0.95 * math.sqrt(math.pi * P))) ** 2
+    if dist == "clusters":
+        p = clusters(N, Lx, Ly, seed=N + My, ncl=6, sigma=0.02)
+    else:
+        p = uniform_points(N, Lx, Ly, seed=N + Mx)
+    f = normal_density(N, seed=N + My)
+    H1, H2 = pe.Spread(p.T, f.T, Lx, Ly, xi, w, eta, P, Mx, My)
+    R1, R2 = oracle.spread(p, f, Lx, Ly, xi, w, eta, P, Mx, My)
+    assert _rel(H1.T, R1) < RTOL and _rel(H2.T, R2) < RTOL
+    # a fixed visiting order: the same sums bit for bit on a second call
+    G1, G2 = pe.Spread(p.T, f.T, Lx, Ly, xi, w, eta, P, Mx, My)
+    assert np.array_equal(G1, H1) and np.array_equal(G2, H2)
+
+
 @pytest.mark.parametrize("N,Mx,My,P,Lx,Ly", [(700, 64, 64, 24, 1.0, 1.0), (900, 92, 46, 16, 2.0, 1.0),
                                               (300, 26, 26, 24, 1.0, 1.0)])
 def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
