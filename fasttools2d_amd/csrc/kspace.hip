@@ -134,7 +134,8 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
 // lo rounded once).
 template <int P, typename S>
 __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
-                                               const Job* __restrict__ jobs, int c)
+                                               const Job* __restrict__ jobs, int c,
+                                               const double* __restrict__ f_base)
 {
     constexpr bool F32 = std::is_same<S, float>::value;
     static_assert(spread_ring(P), "wider supports use k_spread_block");
@@ -148,7 +149,12 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
     // FGG factors per record as two halves: (ex, e3x), (ey, e3y)
     const glob<const double2>* __restrict__ fac = gp(reinterpret_cast<const double2*>(d.sp_fac));
-    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.snrec;
+    // densities straight from the input: record k's point, its (f1, f2) or,
+    // for the stresslet, (f1 n_c, f2 n_c) as the density pass forms them
+    const glob<const int>* __restrict__ pidx = gp(d.sp_perm);
+    const glob<const double>* __restrict__ fsrc = gp(f_base + d.f_off);
+    const glob<const double>* __restrict__ nrm = gp(d.nrm);
+    const int mode = dlp_family(d.kind) ? c + 1 : 0;
     const int Mx = d.Mx, My = d.My, ntx = d.ntx;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
@@ -303,7 +309,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         if (k < end) {
             mm = snode[k];
             ff = fac[2 * (size_t)k + (xr ? 0 : 1)];
-            if (!xr) dv = dens[k];
+            if (!xr) point_density(mode, pidx[k], fsrc, nrm, dv.x, dv.y);
         }
     };
     bool have = run();
@@ -1060,11 +1066,13 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
 
 template <typename S>
 static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs,
-                         int njobs)
+                         int njobs, const double* f)
 {
     switch (P) {
 #define SE2P_SPREAD_P(PP)                                                                         \
-    case PP: hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c); break;
+    case PP:                                                                                      \
+        hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);      \
+        break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
         SE2P_SPREAD_P(12) SE2P_SPREAD_P(14)
 #undef SE2P_SPREAD_P
@@ -1082,11 +1090,11 @@ static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const 
 }
 
 int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
-                  int njobs)
+                  int njobs, const double* f)
 {
     if (njobs <= 0) return SE2P_OK;
-    SE2P_TRY(prec ? spread_launch<float>(s, P, c, sys, jobs, njobs)
-                  : spread_launch<double>(s, P, c, sys, jobs, njobs));
+    SE2P_TRY(prec ? spread_launch<float>(s, P, c, sys, jobs, njobs, f)
+                  : spread_launch<double>(s, P, c, sys, jobs, njobs, f));
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -252,10 +252,14 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->sp_snode = (int2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_FAC, sizeof(double4) * (size_t)std::max(nrec, 1), &p));
         G->sp_fac = (double4*)p;
-        SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
-                        sizeof(double2) * (dlp_family(kind) ? 2 : 1) * (size_t)std::max(nrec, 1),
-                        &p));
-        G->sp_dens = (double2*)p;
+        G->sp_dens = nullptr;  // the strips read the apply's densities directly
+        if (!g.ring) {
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
+                            sizeof(double2) * (dlp_family(kind) ? 2 : 1) *
+                                (size_t)std::max(nrec, 1),
+                            &p));
+            G->sp_dens = (double2*)p;
+        }
         SE2P_TRY(launch_fill_int(s, (int*)scount, (int)nbins, 0));
         if (g.ring) {
             SE2P_TRY(launch_strip_keys(s, G->sp_node, (const int*)soffs, Ns, P, Mx, My, g.ntx,
@@ -487,7 +491,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
         if (geoms[i]->has_real && has_zero_mode(d.kind, d.free_R)) blocks(i, 1);
     });
     add(B->sdens, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k) blocks(i, (d.snrec + 255) / 256);
+        if (geoms[i]->has_k && !spread_ring(d.P)) blocks(i, (d.snrec + 255) / 256);
     });
     for (int prec = 0; prec < 2; ++prec) {
         for (int c = 0; c < 2; ++c) {
@@ -590,14 +594,14 @@ int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const do
                       int c)
 {
     StageTimer st(ctx, s, "spread");
-    // the densities of every pair are packed with the first
+    // the densities of every pair are packed with the first (block kernel only)
     if (c == 0)
         SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
     for (int prec = 0; prec < 2; ++prec)
         for (int q = 0; q < kNumP; ++q)
             SE2P_TRY(launch_spread(s, prec, 2 * (q + 1), c, B.sys,
                                    B.jobs + B.spread[prec][c][q].first,
-                                   B.spread[prec][c][q].count));
+                                   B.spread[prec][c][q].count, d_f));
     return SE2P_OK;
 }
 

@@ -383,9 +383,10 @@ int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
 // or Z1 (c = 1); job b = strip b % ntx of kSB grid columns and segment b / ntx of
 // sseg rows (spread_ring(P)) or block (b % ntx, b / ntx) of kSB x kSB nodes; all
-// jobs' systems of support P.
+// jobs' systems of support P.  The strips take their densities from f (the
+// apply's input); the blocks from the density pass.
 int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
-                  int njobs);
+                  int njobs, const double* f);
 // Spreading records of a geometry in binned order: record k = rperm[k] of the
 // key pass gets snode[k] = rnode[rperm[k]], its point pidx[k] = owner[rperm[k]]
 // (rperm[k] itself when owner is null: one record per point) and that point's
@@ -397,8 +398,8 @@ int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* rp
 // off[perm[k]] and, when spos is given, sidx[k] = spos[perm[k]].
 int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
                           const int* spos, int n, int2* bnode, double2* boff, int* sidx);
-// Densities of the spreading records of every pair (job (s, b): records
-// b*256 .. b*256+255 of system s).
+// Densities of the spreading records of every pair for the block kernel (job
+// (s, b): records b*256 .. b*256+255 of system s).
 int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
                        const double* f);
 // Filter (job b: pair modes b*256 .. of rows 0 .. My/2, each with its partner):
