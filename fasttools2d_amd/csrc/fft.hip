@@ -530,7 +530,8 @@ __device__ __attribute__((noinline)) void filter_rows(typename Mem<S>::lds* __re
 // transposed forward spectrum W (the rows of modes k and -k), transforms them
 // in LDS, filters every mode with its partner (unpack, Stokes filter, k = 0
 // removal, 1/(Mx My), repack: the same filter_mode as k_filter), transforms
-// them back and writes the rows of Z0.  Rows 0 and My/2 are their own partners.
+// them back and writes the rows of Z0.  Rows 0 and My/2 are their own partners
+// and share one job.
 // It runs the same stages and filter arithmetic as the separate passes on values
 // that stay in LDS instead of HBM; the results agree with theirs to rounding
 // (within 4e-16 relative on FP64 grids, where instruction selection differs in
@@ -549,15 +550,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const Job jb = jobs[blockIdx.x];
     const SysDesc& d = sys[jb.x];
     const int n = d.Mx, My = d.My;
-    const int p = jb.y, q = p ? My - p : 0;
-    const int nb = q == p ? 1 : 2;
-    const int gs = (q - p) * n;  // line 1 starts gs elements after line 0 (unused for nb = 1)
+    // job p < My/2: rows p and My - p; job 0 takes rows 0 and My/2, each its own
+    // partner (My/2 jobs, a whole number of rounds of two per CU)
+    const int p = jb.y, q = p ? My - p : My / 2;
+    const int nb = 2;
+    const int gs = (q - p) * n;  // line 1 starts gs elements after line 0
     const glb_d2* __restrict__ tw = (const glb_d2*)d.twx;
     LP* __restrict__ buf = (LP*)lds;
     line_stages<T, PPT, true, false, S>(buf, (GP*)d.grid[G_W] + (size_t)p * n, gs, n, nb, d.ax,
                                         tw, 0);
     // the stages end with a barrier: every forward value is in buf
-    filter_rows<T, S>(buf, d, n, nb, p, q);
+    if (p) {
+        filter_rows<T, S>(buf, d, n, 2, p, q);
+    } else {
+        filter_rows<T, S>(buf, d, n, 1, 0, 0);
+        filter_rows<T, S>(buf + n, d, n, 1, q, q);
+    }
     __syncthreads();
     line_stages<T, PPT, false, true, S>(buf, (GP*)d.grid[G_Z0] + (size_t)p * n, gs, n, nb, d.ax,
                                         tw, 1);

@@ -538,7 +538,7 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             if (g == 0)
                 add(B->xfilt[prec], [&](int i, const SysDesc& d) {
                     if (want(i, d) && fft_xfilter_fits(d.kind, d.Mx, d.free_R)) {
-                        blocks(i, d.My / 2 + 1);
+                        blocks(i, d.My / 2);  // row pairs; rows 0 and My/2 share job 0
                         B->lds_xf[prec] = std::max(B->lds_xf[prec],
                                                    (prec ? 8 : 16) * 2 * (size_t)d.Mx);
                     }
