@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const glob<const double>* __restrict__ fsrc = gp(f_base + d.f_off);
     const glob<const double>* __restrict__ nrm = gp(d.nrm);
     const int mode = dlp_family(d.kind) ? c + 1 : 0;
-    const int Mx = d.Mx, My = d.My, ntx = d.ntx;
+    const int Mx = d.Mx, My = d.My;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
@@ -172,9 +172,10 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     // keep every y row and density finite from the start
     for (int i = lane; i < kSC * kLen; i += 64) (&W[1][0][0])[i] = (S)0;
     if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
-    const int strip = jb.y % ntx;
+    const int4 sg = d.sp_segs[jb.y];
+    const int strip = sg.x;
     const int X0 = strip * kSB;
-    const int Y0 = (jb.y / ntx) * d.sseg, Y1 = min(Y0 + d.sseg, My);
+    const int Y0 = sg.y, Y1 = sg.z;
     const bool xr = lane < kSC;  // staging: lanes 0-31 fill x rows, 32-63 y rows
     const int q = lane & (kSC - 1);
     const int row = lane >> 4, col = lane & 15;  // MFMA operand: record 4 s + row, node col

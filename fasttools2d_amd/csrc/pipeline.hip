@@ -30,6 +30,44 @@ __global__ void k_copy_d(const double* __restrict__ a, double* __restrict__ b, l
     if (i < n) b[i] = a[i];
 }
 
+// The strip spreading jobs: each strip's rows cut into segments of at most S
+// rows, halved while a segment's records (start rows [y0 - P, y1), the P rows
+// before it build up its ring) exceed 1.5 times those of an S-row segment at the
+// mean density, so a strip through a dense cluster runs as several shorter jobs
+// instead of one long one; never below P rows (or 2), which would repeat each
+// dense row's records in more than two segments' build-up.  st[s My + r] = first
+// record of strip s, start row r (st[ntx My] = nrec).
+std::vector<int4> strip_segments(const std::vector<int>& st, int ntx, int My, int P, int S,
+                                 long nrec)
+{
+    std::vector<int4> out;
+    const double mean = (double)nrec / ((double)ntx * My);
+    const double cap = std::max(1.5 * mean * (S + P), 128.0);  // at least four chunks
+    for (int s = 0; s < ntx; ++s) {
+        const int* c = st.data() + (size_t)s * My;
+        // records of strip s with start rows in [a, b), rows taken periodically
+        auto rows = [&](int a, int b) {
+            long n = 0;
+            while (a < b) {
+                int r0 = a % My;
+                r0 = r0 < 0 ? r0 + My : r0;
+                const int take = std::min(b - a, My - r0);
+                n += c[r0 + take] - c[r0];
+                a += take;
+            }
+            return n;
+        };
+        for (int y0 = 0; y0 < My;) {
+            int len = std::min(S, My - y0);
+            const int lmin = std::max(2, P);
+            while (len / 2 >= lmin && (double)rows(y0 - P, y0 + len) > cap) len = (len + 1) / 2;
+            out.push_back(make_int4(s, y0, y0 + len, 0));
+            y0 += len;
+        }
+    }
+    return out;
+}
+
 }  // namespace
 
 int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_psrc,
@@ -272,6 +310,14 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         }
         SE2P_TRY(sort_by_key(ctx, ws, (int*)skey, (int*)scount, nrec, (int)nbins, (int*)rperm,
                              G->sp_start, WS_SORT_TMP));
+        // the records per strip and start row, for the strip segments
+        std::vector<int> sst_h;
+        if (g.ring) {
+            sst_h.resize((size_t)nbins + 1);
+            SE2P_HIP_CHECK(hipMemcpyAsync(sst_h.data(), G->sp_start,
+                                          sizeof(int) * ((size_t)nbins + 1),
+                                          hipMemcpyDeviceToHost, s));
+        }
         SE2P_TRY(launch_spread_records(s, g, G->prec, (const int*)rperm, (const int2*)rnode,
                                        (const int*)owner, G->sp_off, nrec, G->sp_snode,
                                        G->sp_fac, G->sp_perm));
@@ -286,6 +332,17 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                       sizeof(int) * ((size_t)gtiles + 1), hipMemcpyDeviceToHost,
                                       s));
         SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        if (g.ring) {
+            const std::vector<int4> segs = strip_segments(sst_h, g.ntx, My, P, g.sseg, nrec);
+            if (segs.size() >= (size_t)INT_MAX)
+                return set_error(SE2P_ERR_UNSUPPORTED, "too many spreading segments");
+            SE2P_TRY(ws_get(ctx, ws, WS_SPR_SEGS, sizeof(int4) * segs.size(), &p));
+            G->sp_segs = (int4*)p;
+            G->sp_nseg = (int)segs.size();
+            SE2P_HIP_CHECK(hipMemcpyAsync(G->sp_segs, segs.data(), sizeof(int4) * segs.size(),
+                                          hipMemcpyHostToDevice, s));
+            SE2P_HIP_CHECK(hipStreamSynchronize(s));
+        }
         // the gather reads its targets' records in binned order (contiguous)
         SE2P_TRY(ws_get(ctx, ws, WS_GAT_BNODE, sizeof(int2) * (size_t)Nt, &p));
         G->gt_bnode = (int2*)p;
@@ -361,6 +418,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->nty = g.nty;
         d->sseg = g.sseg;
         d->snrec = G.sp_nrec;
+        d->sp_segs = G.sp_segs;
         d->gTX = g.gTX;
         d->gTY = g.gTY;
         d->gntx = g.gntx;
@@ -500,7 +558,9 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             };
             for (int q = 0; q < kNumP; ++q)
                 add(B->spread[prec][c][q], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && d.P == 2 * (q + 1)) blocks(i, (long)d.ntx * d.nty);
+                    if (want(i, d) && d.P == 2 * (q + 1))
+                        blocks(i, spread_ring(d.P) ? (long)geoms[i]->sp_nseg
+                                                   : (long)d.ntx * d.nty);
                 });
             for (int cls = 0; cls < kFftClasses; ++cls) {
                 add(B->ffty[prec][c][cls], [&](int i, const SysDesc& d) {

@@ -142,7 +142,7 @@ enum Slot {
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
     WS_DENS_B, WS_OUT_A, WS_NRM, WS_ZM, WS_PLAIN_U, WS_GRID_D, WS_SYS, WS_JOBS, WS_FREE_PART,
     WS_MLP_PART, WS_GAT_BNODE, WS_GAT_BOFF, WS_GAT_SIDX, WS_YSORT, WS_BIGCELLS, WS_SPR_CNT, WS_SPR_OFFS, WS_SPR_RNODE,
-    WS_SPR_OWNER, WS_SPR_RPERM, WS_SCAN, WS_NUM_SLOTS
+    WS_SPR_OWNER, WS_SPR_RPERM, WS_SCAN, WS_SPR_SEGS, WS_NUM_SLOTS
 };
 
 // Grow-only device buffers addressed by slot (one set per context, one per plan).
@@ -336,6 +336,7 @@ struct SysDesc {
     int gTX, gTY, gntx, gnty, gR;
     double t, w, h, xi, eta, gscale;
     const int *sp_start, *sp_perm, *gt_start, *gt_perm;
+    const int4* sp_segs;  // strip spreading: job j's {strip, first row, end row, 0}
     // spreading records in binned order (snrec of them): sp_perm = their points,
     // sp_snode = (start column - strip's first column, start row)
     const int2 *sp_snode, *gt_node;
@@ -381,9 +382,9 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
 // pos[perm[k]] = k for k < n.
 int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
-// or Z1 (c = 1); job b = strip b % ntx of kSB grid columns and segment b / ntx of
-// sseg rows (spread_ring(P)) or block (b % ntx, b / ntx) of kSB x kSB nodes; all
-// jobs' systems of support P.  The strips take their densities from f (the
+// or Z1 (c = 1); job b = segment sp_segs[b] of a strip of kSB grid columns
+// (spread_ring(P)) or block (b % ntx, b / ntx) of kSB x kSB nodes; all jobs'
+// systems of support P.  The strips take their densities from f (the
 // apply's input); the blocks from the density pass.
 int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
                   int njobs, const double* f);
@@ -486,6 +487,9 @@ struct Geom {
     double* e1tab = nullptr;
     int *sp_perm = nullptr, *sp_start = nullptr;
     int sp_nrec = 0;  // spreading records
+    // strip spreading: one job per segment {strip, first row, end row, 0}
+    int4* sp_segs = nullptr;
+    int sp_nseg = 0;
     int2* sp_node = nullptr;
     double2* sp_off = nullptr;
     int2* sp_snode = nullptr;
