@@ -1400,6 +1400,20 @@ static size_t noalloc_bytes(int kind, int Ns, int Nt, const se2p_params& prm)
     const double nbox = (double)prm.nside_x * prm.nside_y;
     const int P = prm.P;
     const double ntile = std::ceil(prm.Mx / 16.0) * std::ceil(prm.My / 16.0);
+    // spreading: strips (records per point and strip, bins per strip and start
+    // row, a segment table) or blocks (one record per point, 16 x 16 tiles)
+    const bool ring = spread_ring(P);
+    const int ntx = (prm.Mx + kSB - 1) / kSB;
+    int dmax = 1;  // records per point at most
+    if (ring)
+        for (int sx = 0; sx < prm.Mx; ++sx) {
+            int c = 0;
+            for_each_strip(sx, prm.Mx, P, ntx, [&](int, int) { ++c; });
+            dmax = std::max(dmax, c);
+        }
+    const double nrec = ns * dmax;
+    const double nbins = ring ? (double)ntx * prm.My : ntile;
+    const double nseg = ring ? (double)ntx * (std::ceil((double)prm.My / std::max(2, P)) + 1) : 0;
     const int gT = P <= 24 ? 32 : 16;
     const double gtile = std::ceil((double)prm.Mx / gT) * std::ceil((double)prm.My / gT);
     const double G = (double)prm.Mx * prm.My;
@@ -1411,21 +1425,29 @@ static size_t noalloc_bytes(int kind, int Ns, int Nt, const se2p_params& prm)
     b += r(4 * ns) + r(4 * nbox) + r(4 * nt) + r(4 * nbox) + r(4 * ns) + r(4 * (nbox + 1)) +
          r(4 * nt) + r(4 * (nbox + 1)) + r(4 * ns) + r(16 * ns) + r(4 * ns) +
          r((dlp ? 64 : 32) * ns) + r(16 * nt) + r(4) + r(8 * 512) + r(16);
-    // k space: table, keys, counts, permutations, starts, nodes, offsets,
-    // records, densities, grids, twiddles
-    b += r(512) + r(4 * ns) + r(4 * ntile) + r(4 * ns) + r(4 * (ntile + 1)) + r(8 * ns) +
-         r(16 * ns) + r(8 * ns) + r(32 * ns) + r((dlp ? 32 : 16) * ns) + r(4 * nt) +
-         r(4 * gtile) + r(4 * nt) + r(4 * (gtile + 1)) + r(8 * nt) + r(16 * nt) +
-         r(16 * G) * (two ? 4 : 2) + r(16.0 * prm.Mx) + r(16.0 * prm.My) +
-         r(8 * nt) + r(16 * nt) + r(4 * nt);  // binned gathering records
-    // four stable sorts (source cells, target cells, spreading and gathering
-    // tiles): scratch of n and a list of nkeys + 1 each
-    for (double n : {ns, nt, ns, nt}) b += r(4 * n);
-    for (double k : {nbox, nbox, ntile, gtile}) b += r(4 * (k + 1));
+    // k space: table, points' nodes and offsets, record counts and offsets
+    // (strips), bin counts and starts, gathering keys, counts, permutations,
+    // starts, nodes and offsets, grids, twiddles
+    b += r(512) + r(8 * ns) + r(16 * ns) + (ring ? r(4 * ns) + r(4 * (ns + 1)) : 0) +
+         r(4 * nbins) + r(4 * (nbins + 1)) + r(4 * nt) + r(4 * gtile) + r(4 * nt) +
+         r(4 * (gtile + 1)) + r(8 * nt) + r(16 * nt) + r(16 * G) * (two ? 4 : 2) +
+         r(16.0 * prm.Mx) + r(16.0 * prm.My);
+    // spreading records: keys, strip nodes and owners (strips), sort order, points,
+    // start nodes, FGG factors, densities (blocks), segment table (strips)
+    b += r(4 * nrec) + (ring ? r(8 * nrec) + r(4 * nrec) : 0) + r(4 * nrec) + r(4 * nrec) +
+         r(8 * nrec) + r(32 * nrec) + (ring ? 0 : r((dlp ? 32 : 16) * nrec)) +
+         (ring ? r(16 * nseg) : 0);
+    b += r(8 * nt) + r(16 * nt) + r(4 * nt);  // binned gathering records
+    // four stable sorts (source cells, target cells, spreading records and
+    // gathering tiles): scratch of n and a list of nkeys + 1 each; the scans of
+    // long counts (each may take a fresh region) in three passes
+    for (double n : {ns, nt, nrec, nt}) b += r(4 * n);
+    for (double k : {nbox, nbox, nbins, gtile}) b += r(4 * (k + 1));
+    for (double n : {ns, nbox, nbox, nbins, gtile}) b += r(4 * (2 * std::ceil(n / 4096) + 1));
     // descriptor and job list (every job range of one system, generously)
     b += r(sizeof(SysDesc)) +
-         r(sizeof(Job) * (8 * (ns + nt) / 256 + 8 * nbox + 8 * ntile + 4 * gtile + 4 * G / 256 +
-                          4.0 * (prm.Mx + prm.My) + 64));
+         r(sizeof(Job) * (8 * (ns + nt) / 256 + 8 * nbox + 8 * ntile + 2 * nseg + 4 * gtile +
+                          4 * nrec / 256 + 4 * G / 256 + 4.0 * (prm.Mx + prm.My) + 64));
     return b;
 }
 

@@ -4,7 +4,9 @@ whose workspace is carved from the library's fixed per-GPU scratch array.
 Its results must equal se2p_eval_device's bit for bit (same kernels, same
 inputs, fixed-order sums), on two calls in a row, the second smaller than the
 first (the scratch keeps the first call's contents), for a Stokeslet and a
-stresslet system; sizes beyond the scratch are rejected with an error."""
+stresslet system, with the recipe's support (block spreading) and a narrower
+one (strip spreading, whose records take more scratch); sizes beyond the
+scratch are rejected with an error."""
 import ctypes
 
 import numpy as np
@@ -41,7 +43,7 @@ def _eval(ctx, name, kind, ps, pt, f, nrm, Lx, Ly, prm):
                 a.free()
 
 
-def _case(kind, n, seed):
+def _case(kind, n, seed, P=None):
     if kind == "slp":
         Lx, Ly = 1.0, 1.0
         ps = uniform_points(n, Lx, Ly, seed)
@@ -55,14 +57,16 @@ def _case(kind, n, seed):
     prm = ft.ewald_params(kind, len(ps), len(pt), ft.density_q(f[:, 0], f[:, 1]), Lx, Ly,
                           tol=1e-10)
     prm["symmetric"] = 0
+    if P is not None:  # a narrower support: strip spreading instead of blocks
+        prm["P"] = P
     assert npts > 0
     return ps, pt, f, nrm, Lx, Ly, prm
 
 
-@pytest.mark.parametrize("kind", ["slp", "dlp"])
-def test_noalloc_matches_eval_device_twice(gpu_ctx, kind):
+@pytest.mark.parametrize("kind,P", [("slp", None), ("dlp", None), ("slp", 12), ("dlp", 10)])
+def test_noalloc_matches_eval_device_twice(gpu_ctx, kind, P):
     for n, seed in ((120_000, 1), (9_000, 5)):  # the second call smaller
-        ps, pt, f, nrm, Lx, Ly, prm = _case(kind, n, seed)
+        ps, pt, f, nrm, Lx, Ly, prm = _case(kind, n, seed, P)
         a = _eval(gpu_ctx, "se2p_eval_device_noalloc", kind, ps, pt, f, nrm, Lx, Ly, prm)
         b = _eval(gpu_ctx, "se2p_eval_device", kind, ps, pt, f, nrm, Lx, Ly, prm)
         assert np.all(np.isfinite(a))
