@@ -1072,16 +1072,14 @@ static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const 
     switch (P) {
 #define SE2P_SPREAD_P(PP)                                                                         \
     case PP:                                                                                      \
-        hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);      \
+        if constexpr (spread_ring(PP))                                                            \
+            hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);  \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_spread_block<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs,   \
+                               c);                                                                \
         break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
-        SE2P_SPREAD_P(12) SE2P_SPREAD_P(14)
-#undef SE2P_SPREAD_P
-#define SE2P_SPREAD_P(PP)                                                                         \
-    case PP:                                                                                      \
-        hipLaunchKernelGGL((k_spread_block<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c);   \
-        break;
-        SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
+        SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
         SE2P_SPREAD_P(22) SE2P_SPREAD_P(24) SE2P_SPREAD_P(26) SE2P_SPREAD_P(28) SE2P_SPREAD_P(30)
         SE2P_SPREAD_P(32) SE2P_SPREAD_P(34) SE2P_SPREAD_P(36) SE2P_SPREAD_P(38) SE2P_SPREAD_P(40)
 #undef SE2P_SPREAD_P
