@@ -132,9 +132,9 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float v)
 // evaluated directly, v_mfma_f32_16x16x4f32 into a product local to each run of
 // steps, folded into (hi, lo) pairs by two-sum; rows are stored as float2 (hi +
 // lo rounded once).
-template <int P, typename S>
+template <int P, typename S, int NC>
 __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
-                                               const Job* __restrict__ jobs, int c,
+                                               const Job* __restrict__ jobs,
                                                const double* __restrict__ f_base)
 {
     constexpr bool F32 = std::is_same<S, float>::value;
@@ -150,11 +150,12 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     // FGG factors per record as two halves: (ex, e3x), (ey, e3y)
     const glob<const double2>* __restrict__ fac = gp(reinterpret_cast<const double2*>(d.sp_fac));
     // densities straight from the input: record k's point, its (f1, f2) or,
-    // for the stresslet, (f1 n_c, f2 n_c) as the density pass forms them
+    // for the stresslet's NC = 2 pairs, (f1 n_c, f2 n_c) as the density pass
+    // forms them; pair c goes to grid Z0 (c = 0) or Z1 (c = 1)
     const glob<const int>* __restrict__ pidx = gp(d.sp_perm);
     const glob<const double>* __restrict__ fsrc = gp(f_base + d.f_off);
     const glob<const double>* __restrict__ nrm = gp(d.nrm);
-    const int mode = dlp_family(d.kind) ? c + 1 : 0;
+    const bool dlp = dlp_family(d.kind);
     const int Mx = d.Mx, My = d.My;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
@@ -165,13 +166,14 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         e1[i] = __hiloint2double(hi, lo);
     }
     __shared__ __align__(16) S W[2][kSC][kLen];  // x rows, y rows (ring order)
-    __shared__ double2 DD[kSC];
+    __shared__ double2 DD[NC][kSC];
 
     const int lane = threadIdx.x;
     // records past a chunk's count are read (and multiplied by a zero x weight):
     // keep every y row and density finite from the start
     for (int i = lane; i < kSC * kLen; i += 64) (&W[1][0][0])[i] = (S)0;
-    if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
+    if (lane < kSC)
+        for (int cc = 0; cc < NC; ++cc) DD[cc][lane] = make_double2(0.0, 0.0);
     const int4 sg = d.sp_segs[jb.y];
     const int strip = sg.x;
     const int X0 = strip * kSB;
@@ -179,30 +181,29 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     const bool xr = lane < kSC;  // staging: lanes 0-31 fill x rows, 32-63 y rows
     const int q = lane & (kSC - 1);
     const int row = lane >> 4, col = lane & 15;  // MFMA operand: record 4 s + row, node col
-    f64x4 acc0[NT], acc1[NT];
-    float h0[NT][4], l0[NT][4], h1[NT][4], l1[NT][4];
+    f64x4 acc[NC][2][NT];
+    float hs[NC][2][NT][4], ls[NC][2][NT][4];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        acc0[t] = f64x4{0.0, 0.0, 0.0, 0.0};
-        acc1[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) h0[t][r] = l0[t][r] = h1[t][r] = l1[t][r] = 0.f;
-    }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                acc[cc][j][t] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hs[cc][j][t][r] = ls[cc][j][t][r] = 0.f;
+            }
     const float th2 = (float)(d.t * d.h * d.h);
-    glob<double2>* __restrict__ out64 = gp(d.grid[c ? G_Z1 : G_Z0]);
-    glob<float2>* __restrict__ out32 = gp(reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]));
 
     // this lane's output column of register r: f64 product element (x, y) in lane
     // y + 16 (x % 4), register x / 4; f32 product: lane y + 16 (x / 4), register x % 4
-    glob<double2>* col64[4];
-    glob<float2>* col32[4];
+    size_t colo[4];  // offset of the column's first node
     bool colok[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ix = F32 ? X0 + 4 * row + r : X0 + 4 * r + row;
         colok[r] = ix < Mx;
-        col64[r] = out64 + (size_t)min(ix, Mx - 1) * My;
-        col32[r] = out32 + (size_t)min(ix, Mx - 1) * My;
+        colo[r] = (size_t)min(ix, Mx - 1) * My;
     }
     int F = Y0 - P;  // first row of the ring window
     // rows F .. F + BR - 1 are complete: store those of the segment, clear all
@@ -215,24 +216,33 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
             off = off < 0 ? off + R : off;
             const int v = F + off;
             const bool done = off < BR, keep = done && v >= Y0 && v < Y1;
-            if constexpr (F32) {
-                const float z = done ? 0.f : 1.f;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (keep && colok[r])
-                        col32[r][v] = make_float2(h0[t][r] + l0[t][r], h1[t][r] + l1[t][r]);
-                    h0[t][r] *= z;
-                    l0[t][r] *= z;
-                    h1[t][r] *= z;
-                    l1[t][r] *= z;
+            for (int cc = 0; cc < NC; ++cc) {
+                if constexpr (F32) {
+                    glob<float2>* __restrict__ out =
+                        gp(reinterpret_cast<float2*>(d.grid[cc ? G_Z1 : G_Z0]));
+                    const float z = done ? 0.f : 1.f;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if (keep && colok[r])
+                            out[colo[r] + v] = make_float2(hs[cc][0][t][r] + ls[cc][0][t][r],
+                                                           hs[cc][1][t][r] + ls[cc][1][t][r]);
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) {
+                            hs[cc][j][t][r] *= z;
+                            ls[cc][j][t][r] *= z;
+                        }
+                    }
+                } else {
+                    glob<double2>* __restrict__ out = gp(d.grid[cc ? G_Z1 : G_Z0]);
+                    const double z = done ? 0.0 : 1.0;  // finite sums: x 1 keeps them exactly
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (keep && colok[r])
+                            out[colo[r] + v] = make_double2(acc[cc][0][t][r], acc[cc][1][t][r]);
+                    acc[cc][0][t] *= z;
+                    acc[cc][1][t] *= z;
                 }
-            } else {
-                const double z = done ? 0.0 : 1.0;  // finite sums: x 1 keeps them exactly
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (keep && colok[r]) col64[r][v] = make_double2(acc0[t][r], acc1[t][r]);
-                acc0[t] *= z;
-                acc1[t] *= z;
             }
         }
         F += BR;
@@ -240,19 +250,22 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     // the matrix steps of the staged records [lo, hi), two at a time (operands
     // of both read before either multiplies)
     auto steps = [&](int lo, int hi) {
-        f32x4 c0[NT], c1[NT];
+        f32x4 cz[NC][2][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) c0[t] = c1[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cz[cc][0][t] = cz[cc][1][t] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int st = lo >> 2; 4 * st < hi; st += 2) {
             const bool two = 4 * (st + 1) < hi;
             S a[2], wy[2][NT];
-            double2 dd[2];
+            double2 dd[2][NC];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int kk = (4 * (st + u) + row) & (kSC - 1);
                 const S wx = W[0][kk][col];
                 a[u] = kk >= lo && kk < hi && (u == 0 || two) ? wx : (S)0;
-                dd[u] = DD[kk];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) dd[u][cc] = DD[cc][kk];
 #pragma unroll
                 for (int t = 0; t < NT; ++t) wy[u][t] = W[1][kk][16 * t + col];
             }
@@ -260,29 +273,33 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
             for (int u = 0; u < 2; ++u) {
                 if (u == 1 && !two) break;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    if constexpr (F32) {
-                        c0[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], (S)dd[u].x * wy[u][t],
-                                                                     c0[t], 0, 0, 0);
-                        c1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], (S)dd[u].y * wy[u][t],
-                                                                     c1[t], 0, 0, 0);
-                    } else {
-                        acc0[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], dd[u].x * wy[u][t],
-                                                                       acc0[t], 0, 0, 0);
-                        acc1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], dd[u].y * wy[u][t],
-                                                                       acc1[t], 0, 0, 0);
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) {
+                        if constexpr (F32) {
+                            cz[cc][0][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[u], (S)dd[u][cc].x * wy[u][t], cz[cc][0][t], 0, 0, 0);
+                            cz[cc][1][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[u], (S)dd[u][cc].y * wy[u][t], cz[cc][1][t], 0, 0, 0);
+                        } else {
+                            acc[cc][0][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[u], dd[u][cc].x * wy[u][t], acc[cc][0][t], 0, 0, 0);
+                            acc[cc][1][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[u], dd[u][cc].y * wy[u][t], acc[cc][1][t], 0, 0, 0);
+                        }
                     }
-                }
             }
         }
         if constexpr (F32) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
+            for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    two_sum_add(h0[t][r], l0[t][r], c0[t][r]);
-                    two_sum_add(h1[t][r], l1[t][r], c1[t][r]);
-                }
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            two_sum_add(hs[cc][j][t][r], ls[cc][j][t][r], cz[cc][j][t][r]);
         }
     };
     // The segment's start rows Y0 - P .. Y1 - 1 as runs within one period, each
@@ -304,13 +321,20 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
     // chunks of kSC records, the next one's records loaded while the current
     // one is staged and multiplied
     int2 m = make_int2(0, 0), m_n = make_int2(0, 0);
-    double2 f = make_double2(0.0, 0.0), f_n = f, dd = f, dd_n = f;
-    auto load = [&](int p0, int end, int2& mm, double2& ff, double2& dv) {
+    double2 f = make_double2(0.0, 0.0), f_n = f, dd[NC], dd_n[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) dd[cc] = dd_n[cc] = f;
+    auto load = [&](int p0, int end, int2& mm, double2& ff, double2 (&dv)[NC]) {
         const int k = p0 + q;
         if (k < end) {
             mm = snode[k];
             ff = fac[2 * (size_t)k + (xr ? 0 : 1)];
-            if (!xr) point_density(mode, pidx[k], fsrc, nrm, dv.x, dv.y);
+            if (!xr) {
+                const int pt = pidx[k];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc)
+                    point_density(dlp ? cc + 1 : 0, pt, fsrc, nrm, dv[cc].x, dv[cc].y);
+            }
         }
     };
     bool have = run();
@@ -336,7 +360,9 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
                 fill_row_f32<P, R>(W[xr ? 0 : 1][q], !xr, xr ? m.x : sl, (float)f.x, th2);
             else
                 fill_row<P, R>(W[xr ? 0 : 1][q], !xr, xr ? m.x : sl, f.x, f.y, e1);
-            if (!xr) DD[q] = dd;
+            if (!xr)
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) DD[cc][q] = dd[cc];
         }
         __syncthreads();
         // records are in start-row order: those that fit the ring are a prefix
@@ -353,7 +379,8 @@ __global__ __launch_bounds__(64) void k_spread(const SysDesc* __restrict__ sys,
         sh = shn;
         m = m_n;
         f = f_n;
-        dd = dd_n;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) dd[cc] = dd_n[cc];
     }
     while (F < Y1) flush();
 }
@@ -414,9 +441,9 @@ __device__ __forceinline__ void block_row_f32(float* row, int o, float q, float 
 // FP32 tier): weights evaluated directly, v_mfma_f32_16x16x4f32 into a
 // chunk-local product, folded into (hi, lo) pairs by two-sum after every chunk;
 // the block is stored as float2 (hi + lo rounded once).
-template <int P, typename S>
+template <int P, typename S, int NC>
 __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__ sys,
-                                               const Job* __restrict__ jobs, int c)
+                                                     const Job* __restrict__ jobs)
 {
     constexpr bool F32 = std::is_same<S, float>::value;
     const Job jb = jobs[blockIdx.x];
@@ -424,7 +451,8 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
     const glob<const int>* __restrict__ tstart = gp(d.sp_start);
     const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
     const glob<const double4>* __restrict__ fac = gp(d.sp_fac);
-    const glob<const double2>* __restrict__ dens = gp(d.sp_dens) + (size_t)c * d.snrec;
+    // NC density pairs (the stresslet's two), pair c at c snrec, into grid Z0, Z1
+    const glob<const double2>* __restrict__ dens = gp(d.sp_dens);
     const int Mx = d.Mx, My = d.My, ntx = d.ntx;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
@@ -437,21 +465,29 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
     __shared__ __align__(16) S WX[kSC][kBRow];
     __shared__ __align__(16) S WY[kSC][kBRow];
     __shared__ double4 SF[kSC];
-    __shared__ double2 DD[kSC];
+    __shared__ double2 DD[NC][kSC];
     __shared__ int OX[kSC], OY[kSC];
 
     const int lane = threadIdx.x;
     // rows past a chunk's count are read (and multiplied by a zero x weight) in
     // its last matrix step: keep every y row and density finite from the start
     for (int i = lane; i < kSC * kBRow; i += 64) (&WY[0][0])[i] = (S)0;
-    if (lane < kSC) DD[lane] = make_double2(0.0, 0.0);
+    if (lane < kSC)
+        for (int cc = 0; cc < NC; ++cc) DD[cc][lane] = make_double2(0.0, 0.0);
     const int X0 = (jb.y % ntx) * kSB, Y0 = (jb.y / ntx) * kSB;
     const int XE = min(X0 + kSB, Mx), YE = min(Y0 + kSB, My);
     const int q = lane & (kSC - 1);
     const int row = lane >> 4, col = lane & 15;  // MFMA operand: point 4 s + row, node col
-    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    float h0[4] = {0.f, 0.f, 0.f, 0.f}, l0[4] = {0.f, 0.f, 0.f, 0.f};
-    float h1[4] = {0.f, 0.f, 0.f, 0.f}, l1[4] = {0.f, 0.f, 0.f, 0.f};
+    f64x4 acc[NC][2];
+    float hs[NC][2][4], ls[NC][2][4];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[cc][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hs[cc][j][r] = ls[cc][j][r] = 0.f;
+        }
     const float th2 = (float)(d.t * d.h * d.h);
     // The chunks the block visits, in the order of the loops image kx, image ky,
     // tile ty, tile tx, binned order (image kx: start node m.x - kx Mx; the window
@@ -507,29 +543,31 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
         return true;
     };
     // one round of independent, contiguous loads per chunk
-    auto load = [&](const Chunk& ch, int2& m, double4& fk, double2& dk) {
+    auto load = [&](const Chunk& ch, int2& m, double4& fk, double2 (&dk)[NC]) {
         const int k = ch.kb + lane;
         m = make_int2(-1, -1);
         fk = make_double4(0.0, 0.0, 0.0, 0.0);
-        dk = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) dk[cc] = make_double2(0.0, 0.0);
         if (lane < kSC && k < ch.ke) {
             m = snode[k];
             fk = fac[k];
-            dk = dens[k];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) dk[cc] = dens[(size_t)cc * d.snrec + k];
         }
     };
     Chunk cur;
     bool have = next_chunk(cur);
     int2 m;
     double4 fk;
-    double2 dk;
+    double2 dk[NC];
     if (have) load(cur, m, fk, dk);
     while (have) {
         Chunk nxt;
         const bool have_n = next_chunk(nxt);
         int2 m_n;
         double4 fk_n;
-        double2 dk_n;
+        double2 dk_n[NC];
         if (have_n) load(nxt, m_n, fk_n, dk_n);
         // lanes past the chunk's end hold start node (-1, -1), outside every window
         const bool keep = m.x >= cur.lox && m.x <= cur.hix && m.y >= cur.loy && m.y <= cur.hiy;
@@ -543,7 +581,8 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
                 OX[pos] = m.x - cur.sx - X0;
                 OY[pos] = m.y - cur.sy - Y0;
                 SF[pos] = fk;
-                DD[pos] = dk;
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) DD[cc][pos] = dk[cc];
             }
             __syncthreads();
             if (q < n) {  // lanes 0-31: x rows, 32-63: y rows, one code path
@@ -558,62 +597,83 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
             }
             __syncthreads();
             // all operands of the chunk first, then the matrix steps back to back
-            S a[kSC / 4], b0[kSC / 4], b1[kSC / 4];
+            S a[kSC / 4], wyv[kSC / 4];
+            double2 dd[NC][kSC / 4];
 #pragma unroll
             for (int st = 0; st < kSC / 4; ++st) {
                 const int kk = 4 * st + row;
                 const S wx = WX[kk][col];
                 a[st] = kk < n ? wx : (S)0;
-                const S wy = WY[kk][col];
-                const double2 dd = DD[kk];
-                b0[st] = (S)dd.x * wy;
-                b1[st] = (S)dd.y * wy;
+                wyv[st] = WY[kk][col];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) dd[cc][st] = DD[cc][kk];
             }
             if constexpr (F32) {
-                f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+                f32x4 cz[NC][2];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) cz[cc][0] = cz[cc][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int st = 0; st < kSC / 4; ++st)
                     if (4 * st < n) {
-                        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b0[st], c0, 0, 0, 0);
-                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b1[st], c1, 0, 0, 0);
+#pragma unroll
+                        for (int cc = 0; cc < NC; ++cc) {
+                            cz[cc][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[st], (S)dd[cc][st].x * wyv[st], cz[cc][0], 0, 0, 0);
+                            cz[cc][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[st], (S)dd[cc][st].y * wyv[st], cz[cc][1], 0, 0, 0);
+                        }
                     }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    two_sum_add(h0[r], l0[r], c0[r]);
-                    two_sum_add(h1[r], l1[r], c1[r]);
-                }
+                for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        two_sum_add(hs[cc][0][r], ls[cc][0][r], cz[cc][0][r]);
+                        two_sum_add(hs[cc][1][r], ls[cc][1][r], cz[cc][1][r]);
+                    }
             } else {
 #pragma unroll
                 for (int st = 0; st < kSC / 4; ++st)
                     if (4 * st < n) {
-                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b0[st], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b1[st], acc1, 0, 0, 0);
+#pragma unroll
+                        for (int cc = 0; cc < NC; ++cc) {
+                            acc[cc][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[st], dd[cc][st].x * wyv[st], acc[cc][0], 0, 0, 0);
+                            acc[cc][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[st], dd[cc][st].y * wyv[st], acc[cc][1], 0, 0, 0);
+                        }
                     }
             }
         }
         cur = nxt;
         m = m_n;
         fk = fk_n;
-        dk = dk_n;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) dk[cc] = dk_n[cc];
         have = have_n;
     }
     const int iy = Y0 + col;
-    if constexpr (F32) {
-        // f32 product: element (x, y) in lane y + 16 (x / 4), register x % 4
-        glob<float2>* __restrict__ out = gp(reinterpret_cast<float2*>(d.grid[c ? G_Z1 : G_Z0]));
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ix = X0 + 4 * row + r;
-            if (ix < Mx && iy < My)
-                out[(size_t)ix * My + iy] = make_float2(h0[r] + l0[r], h1[r] + l1[r]);
-        }
-    } else {
-        // f64 product: element (x, y) in lane y + 16 (x % 4), register x / 4
-        glob<double2>* __restrict__ out = gp(d.grid[c ? G_Z1 : G_Z0]);
+    for (int cc = 0; cc < NC; ++cc) {
+        if constexpr (F32) {
+            // f32 product: element (x, y) in lane y + 16 (x / 4), register x % 4
+            glob<float2>* __restrict__ out =
+                gp(reinterpret_cast<float2*>(d.grid[cc ? G_Z1 : G_Z0]));
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ix = X0 + 4 * r + row;
-            if (ix < Mx && iy < My) out[(size_t)ix * My + iy] = make_double2(acc0[r], acc1[r]);
+            for (int r = 0; r < 4; ++r) {
+                const int ix = X0 + 4 * row + r;
+                if (ix < Mx && iy < My)
+                    out[(size_t)ix * My + iy] = make_float2(hs[cc][0][r] + ls[cc][0][r],
+                                                            hs[cc][1][r] + ls[cc][1][r]);
+            }
+        } else {
+            // f64 product: element (x, y) in lane y + 16 (x % 4), register x / 4
+            glob<double2>* __restrict__ out = gp(d.grid[cc ? G_Z1 : G_Z0]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ix = X0 + 4 * r + row;
+                if (ix < Mx && iy < My)
+                    out[(size_t)ix * My + iy] = make_double2(acc[cc][0][r], acc[cc][1][r]);
+            }
         }
     }
 }
@@ -1065,18 +1125,18 @@ int launch_e1_table(hipStream_t s, const KGeom& g, double* tab)
     return SE2P_OK;
 }
 
-template <typename S>
-static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const Job* jobs,
-                         int njobs, const double* f)
+template <typename S, int NC>
+static int spread_launch(hipStream_t s, int P, const SysDesc* sys, const Job* jobs, int njobs,
+                         const double* f)
 {
     switch (P) {
 #define SE2P_SPREAD_P(PP)                                                                         \
     case PP:                                                                                      \
         if constexpr (spread_ring(PP))                                                            \
-            hipLaunchKernelGGL((k_spread<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs, c, f);  \
+            hipLaunchKernelGGL((k_spread<PP, S, NC>), dim3(njobs), dim3(64), 0, s, sys, jobs, f); \
         else                                                                                      \
-            hipLaunchKernelGGL((k_spread_block<PP, S>), dim3(njobs), dim3(64), 0, s, sys, jobs,   \
-                               c);                                                                \
+            hipLaunchKernelGGL((k_spread_block<PP, S, NC>), dim3(njobs), dim3(64), 0, s, sys,     \
+                               jobs);                                                             \
         break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
         SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
@@ -1088,12 +1148,15 @@ static int spread_launch(hipStream_t s, int P, int c, const SysDesc* sys, const 
     return SE2P_OK;
 }
 
-int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
-                  int njobs, const double* f)
+int launch_spread(hipStream_t s, int prec, int P, int npairs, const SysDesc* sys,
+                  const Job* jobs, int njobs, const double* f)
 {
     if (njobs <= 0) return SE2P_OK;
-    SE2P_TRY(prec ? spread_launch<float>(s, P, c, sys, jobs, njobs, f)
-                  : spread_launch<double>(s, P, c, sys, jobs, njobs, f));
+    const int rc = npairs == 2 ? (prec ? spread_launch<float, 2>(s, P, sys, jobs, njobs, f)
+                                       : spread_launch<double, 2>(s, P, sys, jobs, njobs, f))
+                               : (prec ? spread_launch<float, 1>(s, P, sys, jobs, njobs, f)
+                                       : spread_launch<double, 1>(s, P, sys, jobs, njobs, f));
+    SE2P_TRY(rc);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

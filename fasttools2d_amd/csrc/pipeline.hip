@@ -556,9 +556,12 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
             auto want = [&](int i, const SysDesc& d) {
                 return geoms[i]->has_k && d.prec == prec && (c == 0 || dlp_family(d.kind));
             };
+            // spreading: list c = 0 the one-pair (Stokeslet) systems, c = 1 the
+            // two-pair (stresslet) systems, both pairs in one pass
             for (int q = 0; q < kNumP; ++q)
                 add(B->spread[prec][c][q], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && d.P == 2 * (q + 1))
+                    if (geoms[i]->has_k && d.prec == prec && dlp_family(d.kind) == (c == 1) &&
+                        d.P == 2 * (q + 1))
                         blocks(i, spread_ring(d.P) ? (long)geoms[i]->sp_nseg
                                                    : (long)d.ntx * d.nty);
                 });
@@ -654,14 +657,16 @@ int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const do
                       int c)
 {
     StageTimer st(ctx, s, "spread");
-    // the densities of every pair are packed with the first (block kernel only)
-    if (c == 0)
-        SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
+    // every pair is spread with the first: the densities packed (block kernel
+    // only), then one pass per system over its points for one or two pairs
+    if (c != 0) return SE2P_OK;
+    SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
     for (int prec = 0; prec < 2; ++prec)
-        for (int q = 0; q < kNumP; ++q)
-            SE2P_TRY(launch_spread(s, prec, 2 * (q + 1), c, B.sys,
-                                   B.jobs + B.spread[prec][c][q].first,
-                                   B.spread[prec][c][q].count, d_f));
+        for (int pairs = 1; pairs <= 2; ++pairs)
+            for (int q = 0; q < kNumP; ++q)
+                SE2P_TRY(launch_spread(s, prec, 2 * (q + 1), pairs, B.sys,
+                                       B.jobs + B.spread[prec][pairs - 1][q].first,
+                                       B.spread[prec][pairs - 1][q].count, d_f));
     return SE2P_OK;
 }
 

@@ -381,13 +381,14 @@ int launch_real(hipStream_t s, int kind, int tier, int mode, const SysDesc* sys,
                 int njobs, double* u);
 // pos[perm[k]] = k for k < n.
 int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
-// Spreading of component pair c (SLP: c = 0; DLP: c = 0, 1) into grid Z0 (c = 0)
-// or Z1 (c = 1); job b = segment sp_segs[b] of a strip of kSB grid columns
-// (spread_ring(P)) or block (b % ntx, b / ntx) of kSB x kSB nodes; all jobs'
-// systems of support P.  The strips take their densities from f (the
-// apply's input); the blocks from the density pass.
-int launch_spread(hipStream_t s, int prec, int P, int c, const SysDesc* sys, const Job* jobs,
-                  int njobs, const double* f);
+// Spreading of npairs density pairs (Stokeslet kinds: 1, (f1, f2); stresslet
+// kinds: 2, (f1 n1, f2 n1) and (f1 n2, f2 n2)), pair c into grid Z0 (c = 0) or Z1
+// (c = 1), both pairs from one pass over the points; job b = segment sp_segs[b]
+// of a strip of kSB grid columns (spread_ring(P)) or block (b % ntx, b / ntx) of
+// kSB x kSB nodes; all jobs' systems of support P and npairs.  The strips take
+// their densities from f (the apply's input); the blocks from the density pass.
+int launch_spread(hipStream_t s, int prec, int P, int npairs, const SysDesc* sys,
+                  const Job* jobs, int njobs, const double* f);
 // Spreading records of a geometry in binned order: record k = rperm[k] of the
 // key pass gets snode[k] = rnode[rperm[k]], its point pidx[k] = owner[rperm[k]]
 // (rperm[k] itself when owner is null: one record per point) and that point's
@@ -520,7 +521,7 @@ struct BatchExec {
     JobRange sdens;       // spreading densities: blocks of 256 binned points
     JobRange real[kNumKinds][3][kRealModes];  // [kind][tier: full, fast, FP32][RealMode]
     // k-space passes, per grid precision (0 FP64, 1 FP32: float2 grids)
-    JobRange spread[2][2][kNumP];  // [prec][c][P / 2 - 1]
+    JobRange spread[2][2][kNumP];  // [prec][pairs - 1][P / 2 - 1]
     JobRange ffty[2][2][kFftClasses], fftx[2][2][kFftClasses], tr_fwd[2][2];  // [prec][c][class]
     JobRange filter[2][kNumKinds];  // [prec][kind]
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
@@ -549,7 +550,8 @@ int batch_apply_parts(se2p_ctx* ctx, const BatchExec& B, const double* d_f, doub
                       double* d_uk, bool accumulate, bool use_zm);
 // u = real + k (+ DLP zero mode)
 int batch_apply(se2p_ctx* ctx, const BatchExec& B, const double* d_f, double* d_u);
-// Spreading of component pair c into Z0 (c = 0) or Z1 (c = 1) only.
+// Spreading into Z0 (and Z1 for the stresslet's second pair): the whole work at
+// c = 0, nothing at c = 1 (both pairs come from one pass).
 int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f,
                       int c);
 // Gather from W only.
