@@ -1433,10 +1433,9 @@ static size_t noalloc_bytes(int kind, int Ns, int Nt, const se2p_params& prm)
          r(4 * (gtile + 1)) + r(8 * nt) + r(16 * nt) + r(16 * G) * (two ? 4 : 2) +
          r(16.0 * prm.Mx) + r(16.0 * prm.My);
     // spreading records: keys, strip nodes and owners (strips), sort order, points,
-    // start nodes, FGG factors, densities (blocks), segment table (strips)
+    // start nodes, FGG factors, segment table (strips)
     b += r(4 * nrec) + (ring ? r(8 * nrec) + r(4 * nrec) : 0) + r(4 * nrec) + r(4 * nrec) +
-         r(8 * nrec) + r(32 * nrec) + (ring ? 0 : r((dlp ? 32 : 16) * nrec)) +
-         (ring ? r(16 * nseg) : 0);
+         r(8 * nrec) + r(32 * nrec) + (ring ? r(16 * nseg) : 0);
     b += r(8 * nt) + r(16 * nt) + r(4 * nt);  // binned gathering records
     // four stable sorts (source cells, target cells, spreading records and
     // gathering tiles): scratch of n and a list of nkeys + 1 each; the scans of

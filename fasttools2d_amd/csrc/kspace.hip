@@ -443,7 +443,8 @@ __device__ __forceinline__ void block_row_f32(float* row, int o, float q, float 
 // the block is stored as float2 (hi + lo rounded once).
 template <int P, typename S, int NC>
 __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__ sys,
-                                                     const Job* __restrict__ jobs)
+                                                     const Job* __restrict__ jobs,
+                                                     const double* __restrict__ f_base)
 {
     constexpr bool F32 = std::is_same<S, float>::value;
     const Job jb = jobs[blockIdx.x];
@@ -451,8 +452,12 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
     const glob<const int>* __restrict__ tstart = gp(d.sp_start);
     const glob<const int2>* __restrict__ snode = gp(d.sp_snode);
     const glob<const double4>* __restrict__ fac = gp(d.sp_fac);
-    // NC density pairs (the stresslet's two), pair c at c snrec, into grid Z0, Z1
-    const glob<const double2>* __restrict__ dens = gp(d.sp_dens);
+    // NC density pairs (the stresslet's two) straight from the input: record k's
+    // point, its (f1, f2) or (f1 n_c, f2 n_c); pair c into grid Z0 (c = 0), Z1
+    const glob<const int>* __restrict__ pidx = gp(d.sp_perm);
+    const glob<const double>* __restrict__ fsrc = gp(f_base + d.f_off);
+    const glob<const double>* __restrict__ nrm = gp(d.nrm);
+    const bool dlp = dlp_family(d.kind);
     const int Mx = d.Mx, My = d.My, ntx = d.ntx;
     double e1[P + 1];  // the same for every lane: held in scalar registers
 #pragma unroll
@@ -552,8 +557,10 @@ __global__ __launch_bounds__(64) void k_spread_block(const SysDesc* __restrict__
         if (lane < kSC && k < ch.ke) {
             m = snode[k];
             fk = fac[k];
+            const int pt = pidx[k];
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc) dk[cc] = dens[(size_t)cc * d.snrec + k];
+            for (int cc = 0; cc < NC; ++cc)
+                point_density(dlp ? cc + 1 : 0, pt, fsrc, nrm, dk[cc].x, dk[cc].y);
         }
     };
     Chunk cur;
@@ -712,30 +719,6 @@ __global__ void k_gather_records(const int* __restrict__ perm, const int2* __res
     bnode[k] = node[p];
     boff[k] = off[p];
     if (spos) sidx[k] = spos[p];
-}
-
-__global__ __launch_bounds__(256) void k_spread_dens(const SysDesc* __restrict__ sys,
-                                                     const Job* __restrict__ jobs,
-                                                     const double* __restrict__ f_base)
-{
-    const Job jb = jobs[blockIdx.x];
-    const SysDesc& d = sys[jb.x];
-    const int k = jb.y * 256 + threadIdx.x;
-    if (k >= d.snrec) return;
-    const int p = gp(d.sp_perm)[k];
-    const glob<const double>* __restrict__ f = gp(f_base + d.f_off);
-    const glob<const double>* __restrict__ nrm = gp(d.nrm);
-    glob<double2>* __restrict__ out = gp(d.sp_dens);
-    double d0, d1;
-    if (!dlp_family(d.kind)) {
-        point_density(0, p, f, nrm, d0, d1);
-        out[k] = make_double2(d0, d1);
-        return;
-    }
-    point_density(1, p, f, nrm, d0, d1);
-    out[k] = make_double2(d0, d1);
-    point_density(2, p, f, nrm, d0, d1);
-    out[(size_t)d.snrec + k] = make_double2(d0, d1);
 }
 
 // Each mode and its Hermitian partner (-k) are filtered together from the same
@@ -1136,7 +1119,7 @@ static int spread_launch(hipStream_t s, int P, const SysDesc* sys, const Job* jo
             hipLaunchKernelGGL((k_spread<PP, S, NC>), dim3(njobs), dim3(64), 0, s, sys, jobs, f); \
         else                                                                                      \
             hipLaunchKernelGGL((k_spread_block<PP, S, NC>), dim3(njobs), dim3(64), 0, s, sys,     \
-                               jobs);                                                             \
+                               jobs, f);                                                          \
         break;
         SE2P_SPREAD_P(2) SE2P_SPREAD_P(4) SE2P_SPREAD_P(6) SE2P_SPREAD_P(8) SE2P_SPREAD_P(10)
         SE2P_SPREAD_P(12) SE2P_SPREAD_P(14) SE2P_SPREAD_P(16) SE2P_SPREAD_P(18) SE2P_SPREAD_P(20)
@@ -1179,15 +1162,6 @@ int launch_gather_records(hipStream_t s, const int* perm, const int2* node, cons
     if (n <= 0) return SE2P_OK;
     hipLaunchKernelGGL(k_gather_records, dim3((n + 255) / 256), dim3(256), 0, s, perm, node, off,
                        spos, n, bnode, boff, sidx);
-    SE2P_HIP_CHECK(hipGetLastError());
-    return SE2P_OK;
-}
-
-int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
-                       const double* f)
-{
-    if (njobs <= 0) return SE2P_OK;
-    hipLaunchKernelGGL(k_spread_dens, dim3(njobs), dim3(256), 0, s, sys, jobs, f);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

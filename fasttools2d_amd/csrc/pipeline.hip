@@ -290,14 +290,6 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
         G->sp_snode = (int2*)p;
         SE2P_TRY(ws_get(ctx, ws, WS_SPR_FAC, sizeof(double4) * (size_t)std::max(nrec, 1), &p));
         G->sp_fac = (double4*)p;
-        G->sp_dens = nullptr;  // the strips read the apply's densities directly
-        if (!g.ring) {
-            SE2P_TRY(ws_get(ctx, ws, WS_SPR_DENS,
-                            sizeof(double2) * (dlp_family(kind) ? 2 : 1) *
-                                (size_t)std::max(nrec, 1),
-                            &p));
-            G->sp_dens = (double2*)p;
-        }
         SE2P_TRY(launch_fill_int(s, (int*)scount, (int)nbins, 0));
         if (g.ring) {
             SE2P_TRY(launch_strip_keys(s, G->sp_node, (const int*)soffs, Ns, P, Mx, My, g.ntx,
@@ -441,7 +433,6 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->gt_perm = G.gt_perm;
         d->sp_snode = G.sp_snode;
         d->sp_fac = G.sp_fac;
-        d->sp_dens = G.sp_dens;
         d->gt_node = G.gt_bnode;
         d->gt_off = G.gt_boff;
         d->gt_sidx = G.sym ? G.gt_sidx : nullptr;
@@ -547,9 +538,6 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     });
     add(B->zm_sys, [&](int i, const SysDesc& d) {
         if (geoms[i]->has_real && has_zero_mode(d.kind, d.free_R)) blocks(i, 1);
-    });
-    add(B->sdens, [&](int i, const SysDesc& d) {
-        if (geoms[i]->has_k && !spread_ring(d.P)) blocks(i, (d.snrec + 255) / 256);
     });
     for (int prec = 0; prec < 2; ++prec) {
         for (int c = 0; c < 2; ++c) {
@@ -657,10 +645,9 @@ int batch_spread_grid(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const do
                       int c)
 {
     StageTimer st(ctx, s, "spread");
-    // every pair is spread with the first: the densities packed (block kernel
-    // only), then one pass per system over its points for one or two pairs
+    // every pair is spread with the first: one pass per system over its points
+    // for one or two pairs, the densities read from the input
     if (c != 0) return SE2P_OK;
-    SE2P_TRY(launch_spread_dens(s, B.sys, B.jobs + B.sdens.first, B.sdens.count, d_f));
     for (int prec = 0; prec < 2; ++prec)
         for (int pairs = 1; pairs <= 2; ++pairs)
             for (int q = 0; q < kNumP; ++q)

@@ -136,7 +136,7 @@ enum Slot {
     WS_TAR_KEY, WS_TAR_PERM, WS_TAR_START, WS_TAR_COUNT,
     WS_SRC_SORTED, WS_TAR_SORTED,
     WS_SPR_KEY, WS_SPR_PERM, WS_SPR_START, WS_SPR_COUNT, WS_SPR_NODE, WS_SPR_OFF,
-    WS_SPR_SNODE, WS_SPR_FAC, WS_SPR_DENS, WS_UNSORTED, WS_SRC_YKEY, WS_SYM_ACC, WS_SYM_POS,
+    WS_SPR_SNODE, WS_SPR_FAC, WS_UNSORTED, WS_SRC_YKEY, WS_SYM_ACC, WS_SYM_POS,
     WS_GAT_KEY, WS_GAT_PERM, WS_GAT_START, WS_GAT_COUNT, WS_GAT_NODE, WS_GAT_OFF,
     WS_GRID_A, WS_GRID_B, WS_GRID_C, WS_TWIDDLE_X, WS_TWIDDLE_Y,
     WS_E1TAB, WS_BIGLIST, WS_SORT_TMP, WS_REDUCE, WS_PTS_A, WS_PTS_B, WS_DENS_A,
@@ -341,7 +341,6 @@ struct SysDesc {
     // sp_snode = (start column - strip's first column, start row)
     const int2 *sp_snode, *gt_node;
     const double4* sp_fac;  // spreading: FGG factors (ex, e3x, ey, e3y) per record
-    double2* sp_dens;       // spreading: (d0, d1) of pair c at c snrec + k, per record
     const double2* gt_off;  // gathering: offsets in binned order (gt_node likewise)
     const int* gt_sidx;     // pair-symmetric sums: sorted position of each binned target
     const double *e1tab, *nrm;
@@ -385,8 +384,8 @@ int launch_inverse_perm(hipStream_t s, const int* perm, int n, int* pos);
 // kinds: 2, (f1 n1, f2 n1) and (f1 n2, f2 n2)), pair c into grid Z0 (c = 0) or Z1
 // (c = 1), both pairs from one pass over the points; job b = segment sp_segs[b]
 // of a strip of kSB grid columns (spread_ring(P)) or block (b % ntx, b / ntx) of
-// kSB x kSB nodes; all jobs' systems of support P and npairs.  The strips take
-// their densities from f (the apply's input); the blocks from the density pass.
+// kSB x kSB nodes; all jobs' systems of support P and npairs, the densities read
+// from f (the apply's input) through each record's point.
 int launch_spread(hipStream_t s, int prec, int P, int npairs, const SysDesc* sys,
                   const Job* jobs, int njobs, const double* f);
 // Spreading records of a geometry in binned order: record k = rperm[k] of the
@@ -400,10 +399,6 @@ int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* rp
 // off[perm[k]] and, when spos is given, sidx[k] = spos[perm[k]].
 int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
                           const int* spos, int n, int2* bnode, double2* boff, int* sidx);
-// Densities of the spreading records of every pair for the block kernel (job
-// (s, b): records b*256 .. b*256+255 of system s).
-int launch_spread_dens(hipStream_t s, const SysDesc* sys, const Job* jobs, int njobs,
-                       const double* f);
 // Filter (job b: pair modes b*256 .. of rows 0 .. My/2, each with its partner):
 // W (and D for stresslet kinds) in transposed layout Z[iy*Mx + ix] ->
 // Z0 (and Z1 for kinds with 4 values per target).
@@ -495,7 +490,6 @@ struct Geom {
     double2* sp_off = nullptr;
     int2* sp_snode = nullptr;
     double4* sp_fac = nullptr;
-    double2* sp_dens = nullptr;
     int *gt_perm = nullptr, *gt_start = nullptr;
     int2* gt_node = nullptr;    // start nodes and offsets in input order
     double2* gt_off = nullptr;
@@ -518,7 +512,6 @@ struct BatchExec {
     Job* jobs = nullptr;
     JobRange fill[2];     // by record layout
     JobRange zm_part, zm_sys;
-    JobRange sdens;       // spreading densities: blocks of 256 binned points
     JobRange real[kNumKinds][3][kRealModes];  // [kind][tier: full, fast, FP32][RealMode]
     // k-space passes, per grid precision (0 FP64, 1 FP32: float2 grids)
     JobRange spread[2][2][kNumP];  // [prec][pairs - 1][P / 2 - 1]
