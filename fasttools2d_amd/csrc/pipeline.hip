@@ -318,6 +318,8 @@ int geom_build(se2p_ctx* ctx, Workspace* ws, Geom* G, int kind, const double* d_
                                   (int*)tkey, (int*)tcount, G->gt_node, G->gt_off));
         SE2P_TRY(sort_by_key(ctx, ws, (int*)tkey, (int*)tcount, Nt, gtiles, G->gt_perm,
                              G->gt_start, WS_SORT_TMP));
+        SE2P_TRY(launch_gather_order(s, G->gt_perm, G->gt_start, G->gt_node, gtiles, g.gntx,
+                                     g.gTX, g.gTY, g.gR));
         // the targets per gathering tile, for the job lists (heavy tiles are split)
         G->gt_start_h.resize((size_t)gtiles + 1);
         SE2P_HIP_CHECK(hipMemcpyAsync(G->gt_start_h.data(), G->gt_start,

@@ -395,6 +395,12 @@ int launch_spread(hipStream_t s, int prec, int P, int npairs, const SysDesc* sys
 int launch_spread_records(hipStream_t s, const KGeom& g, int prec, const int* rperm,
                           const int2* rnode, const int* owner, const double2* off, int n,
                           int2* snode, double4* fac, int* pidx);
+// The gathering lanes' order: within every chunk of 256 targets of each of the
+// ntiles tiles (TX x TY start nodes, ntx per row, region side R), perm is
+// reordered in place so the targets are interleaved by their region offset
+// modulo 16 (LDS bank groups).
+int launch_gather_order(hipStream_t s, int* perm, const int* start, const int2* node, int ntiles,
+                        int ntx, int TX, int TY, int R);
 // Gathering records in binned order: bnode[k] = node[perm[k]], boff[k] =
 // off[perm[k]] and, when spos is given, sidx[k] = spos[perm[k]].
 int launch_gather_records(hipStream_t s, const int* perm, const int2* node, const double2* off,
