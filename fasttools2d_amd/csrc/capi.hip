@@ -1913,14 +1913,13 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_GRID_C, sizeof(double2) * Gn, &tmp));
     d.grid[G_Z0] = (double2*)dz;
     d.grid[G_W] = (double2*)tmp;
-    // y lines of the column-major grid, transpose, x lines, transpose back
+    // x lines read as the columns of the column-major Z0 into the row-major W,
+    // then y lines read as the columns of W back into Z0 (the pipeline runs
+    // both line kernels on contiguous lines too, and the second read mode)
     const int ny = (Mx + d.By - 1) / d.By, nx = (My + d.Bx - 1) / d.Bx;
-    const int t0 = transpose_tiles(Mx, My), t1 = transpose_tiles(My, Mx);
     std::vector<Job> J;
-    for (int b = 0; b < ny; ++b) J.push_back(make_int2(0, b));
-    for (int b = 0; b < t0; ++b) J.push_back(make_int2(0, b));
     for (int b = 0; b < nx; ++b) J.push_back(make_int2(0, b));
-    for (int b = 0; b < t1; ++b) J.push_back(make_int2(0, b));
+    for (int b = 0; b < ny; ++b) J.push_back(make_int2(0, b));
     void *psys, *pjobs;
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_SYS, sizeof(SysDesc), &psys));
     SE2P_TRY(ws_get(ctx, &ctx->ws, WS_JOBS, sizeof(Job) * J.size(), &pjobs));
@@ -1930,12 +1929,10 @@ int se2p_fft2(se2p_ctx* ctx, double* re, double* im, int My, int Mx, int inverse
     const SysDesc* ds = (const SysDesc*)psys;
     const Job* dj = (const Job*)pjobs;
     const int inv = inverse ? 1 : 0;
-    SE2P_TRY(launch_fft_lines(s, prec, 0, G_Z0, inv, fft_line_class(My), fft_lds_bytes(My, prec),
-                              ds, dj, ny));
-    SE2P_TRY(launch_transpose(s, prec, G_Z0, G_W, 0, ds, dj + ny, t0));
-    SE2P_TRY(launch_fft_lines(s, prec, 1, G_W, inv, fft_line_class(Mx), fft_lds_bytes(Mx, prec),
-                              ds, dj + ny + t0, nx));
-    SE2P_TRY(launch_transpose(s, prec, G_W, G_Z0, 1, ds, dj + ny + t0 + nx, t1));
+    SE2P_TRY(launch_fft_lines(s, prec, 1, G_Z0, G_W, 1, inv, fft_line_class(Mx),
+                              fft_lds_bytes(Mx, prec), ds, dj, nx));
+    SE2P_TRY(launch_fft_lines(s, prec, 0, G_W, G_Z0, 1, inv, fft_line_class(My),
+                              fft_lds_bytes(My, prec), ds, dj + nx, ny));
     if (prec) {
         SE2P_HIP_CHECK(hipMemcpyAsync(zf.data(), dz, sizeof(float2) * Gn, hipMemcpyDeviceToHost, s));
         SE2P_HIP_CHECK(hipStreamSynchronize(s));

@@ -11,8 +11,11 @@
 // by a direct O(R) sum per output).  Stage s with radix R and Ns = product of the earlier
 // radices maps output (j, k), j < n/R, k < R, to
 //   out[(j / Ns) Ns R + j % Ns + k Ns] = sum_r in[j + r n/R] W_n^{r ((j % Ns) + k Ns) n/(Ns R)}.
-// Every pass works on contiguous lines: y lines of the My x Mx column-major
-// grid (Z[ix*My + iy]), and x lines after a tiled transpose.
+// A pass writes its lines contiguously and reads them contiguously or as the
+// columns of a row-major grid.  Forward: y lines of the column-major grid
+// (Z[ix*My + iy]), a tiled transpose to Z[iy*Mx + ix], x lines.  Inverse: x
+// lines, then y lines read as the columns of that grid, which transposes back
+// in the first stage's loads.
 #include "se2p_internal.h"
 #include "fft_roots.h"
 #include "kfilter.h"
@@ -256,19 +259,27 @@ __device__ __forceinline__ void st2(Ptr p, int i, V a)
     p[i] = v;
 }
 
+// Where the lines of a transform sit in HBM: point k of line b is at
+// b * li + k * ei of the input and b * lo + k * eo of the output.  Contiguous
+// lines have (n, 1); lines that are the columns of a row-major grid with
+// `nlines` columns have (1, nlines).
+struct GStride {
+    int li, ei, lo, eo;
+};
+
 // One Stockham stage of radix R over nb lines of length n.  Butterflies are
 // staged through registers, so the lines stay in one LDS buffer: every thread
 // reads its inputs, a barrier, every thread writes its outputs.  FIRST reads the
-// lines from HBM (gdata) and LAST writes them back there; the other stages read
-// and write the LDS copy.  Twiddles: one table read per butterfly, the other
-// R-2 as powers of it.  S is the grid's scalar type; the twiddle table is FP64
-// for both and rounded on load for FP32 lines.
+// lines from HBM (gin) and LAST writes them there (gout, which may be gin); the
+// other stages read and write the LDS copy.  Twiddles: one table read per
+// butterfly, the other R-2 as powers of it.  S is the grid's scalar type; the
+// twiddle table is FP64 for both and rounded on load for FP32 lines.
 template <int R, bool FIRST, bool LAST, int T, int PPT, typename S>
 __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __restrict__ lds,
-                                                    typename Mem<S>::glb* __restrict__ gdata,
-                                                    int n, int nb, int Ns,
-                                                    const glb_d2* __restrict__ tw, int inverse,
-                                                    int gstride)
+                                                    typename Mem<S>::glb* gin,
+                                                    typename Mem<S>::glb* gout, int n, int nb,
+                                                    int Ns, const glb_d2* __restrict__ tw,
+                                                    int inverse, GStride gs)
 {
     using V = cplx<S>;
     constexpr int BPT = (PPT + R - 1) / R;  // butterflies per thread at most
@@ -290,7 +301,7 @@ __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __rest
             if constexpr (FIRST) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    v[t][r] = conj_if(ld2<S>(gdata, b * gstride + j + r * m), inverse);
+                    v[t][r] = conj_if(ld2<S>(gin, b * gs.li + (j + r * m) * gs.ei), inverse);
             } else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[t][r] = ld2<S>(lds, b * n + j + r * m);
@@ -305,17 +316,18 @@ __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __rest
                 }
             }
             butterfly<R, V>(v[t], (S)-1.0);
-            ob[t] = b * (LAST ? gstride : n) + jq * Ns * R + jm;
+            ob[t] = LAST ? b * gs.lo + (jq * Ns * R + jm) * gs.eo : b * n + jq * Ns * R + jm;
         }
     }
-    if (FIRST == LAST) __syncthreads();  // inputs and outputs share memory
+    if (FIRST == LAST) __syncthreads();  // inputs and outputs may share memory
+    const int os = LAST ? Ns * gs.eo : Ns;
 #pragma unroll
     for (int t = 0; t < BPT; ++t) {
         if (ob[t] >= 0) {
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 if constexpr (LAST)
-                    st2(gdata, ob[t] + k * Ns, conj_if(v[t][k], inverse));
+                    st2(gout, ob[t] + k * os, conj_if(v[t][k], inverse));
                 else
                     st2(lds, ob[t] + k * Ns, v[t][k]);
             }
@@ -327,10 +339,11 @@ __device__ __attribute__((noinline)) void stage_reg(typename Mem<S>::lds* __rest
 // Any radix: one output per (thread, slot), O(R) work per output.
 template <bool FIRST, bool LAST, int T, int PPT, typename S>
 __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __restrict__ lds,
-                                                        typename Mem<S>::glb* __restrict__ gdata,
-                                                        int n, int nb, int Ns, int R,
+                                                        typename Mem<S>::glb* gin,
+                                                        typename Mem<S>::glb* gout, int n,
+                                                        int nb, int Ns, int R,
                                                         const glb_d2* __restrict__ tw, int inverse,
-                                                        int gstride)
+                                                        GStride gs)
 {
     using V = cplx<S>;
     constexpr int OPT = PPT;  // outputs per thread at most
@@ -350,15 +363,16 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
             divmod(qq, m, rm, k, j);
             divmod(j, Ns, rNs, jq, jm);
             const int step = (jm + k * Ns) * base;  // < n
-            const int i0 = b * (FIRST ? gstride : n) + j;
+            const int i0 = FIRST ? b * gs.li + j * gs.ei : b * n + j;
+            const int is = FIRST ? m * gs.ei : m;
             S re = 0, im = 0;
             int e = 0;
             for (int r = 0; r < R; ++r) {
                 V x;
                 if constexpr (FIRST)
-                    x = conj_if(ld2<S>(gdata, i0 + r * m), inverse);
+                    x = conj_if(ld2<S>(gin, i0 + r * is), inverse);
                 else
-                    x = ld2<S>(lds, i0 + r * m);
+                    x = ld2<S>(lds, i0 + r * is);
                 const V wv = ld2<S>(tw, e);
                 const S wi = wv.y;
                 re += x.x * wv.x - x.y * wi;
@@ -367,7 +381,8 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
                 if (e >= n) e -= n;
             }
             res[t] = mkc<V>(re, im);
-            ob[t] = b * (LAST ? gstride : n) + jq * Ns * R + jm + k * Ns;
+            ob[t] = LAST ? b * gs.lo + (jq * Ns * R + jm + k * Ns) * gs.eo
+                         : b * n + jq * Ns * R + jm + k * Ns;
         }
     }
     if (FIRST == LAST) __syncthreads();
@@ -375,7 +390,7 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
     for (int t = 0; t < OPT; ++t) {
         if (ob[t] >= 0) {
             if constexpr (LAST)
-                st2(gdata, ob[t], conj_if(res[t], inverse));
+                st2(gout, ob[t], conj_if(res[t], inverse));
             else
                 st2(lds, ob[t], res[t]);
         }
@@ -383,16 +398,19 @@ __device__ __attribute__((noinline)) void stage_generic(typename Mem<S>::lds* __
     if (!LAST) __syncthreads();
 }
 
-// Job (s, b): lines b*B .. b*B + B - 1 of grid `sel` of system s, along y
-// (axis 0: My-lines, the column-major grid) or x (axis 1: Mx-lines, the
-// transposed grid); every line is contiguous.  The lines live in one LDS
-// buffer of B*n points between the first and the last stage.  T threads with
-// B*n <= PPT * T.  S = float: the grid holds float2 values (half the bytes of
-// the FP64 grid in the same buffer).
+// Job (s, b): lines b*B .. b*B + B - 1 of system s, along y (axis 0: My-lines)
+// or x (axis 1: Mx-lines), read from grid `sel` and written contiguously (line l
+// at l*n) to grid `osel` (which may be `sel`).  The input holds the lines
+// contiguously too (tin 0) or as the columns of a row-major grid (tin 1: point k
+// of line l at k*nlines + l, so the pass also transposes; a group of lines that
+// shares cache lines runs on one XCD, and its 16-byte reads hit in that L2).
+// The lines live in one LDS buffer of B*n points between the first and the last
+// stage.  T threads with B*n <= PPT * T.  S = float: the grid holds float2
+// values (half the bytes of the FP64 grid in the same buffer).
 template <int T, int PPT, bool COMP, typename S>
 __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys,
                                                  const Job* __restrict__ jobs, int axis, int sel,
-                                                 int inverse)
+                                                 int osel, int tin, int inverse)
 {
     using LP = typename Mem<S>::lds;
     using GP = typename Mem<S>::glb;
@@ -404,9 +422,12 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
     const int B = axis ? d.Bx : d.By;
     const FftAxis& ax = axis ? d.ax : d.ay;
     const glb_d2* __restrict__ tw = (const glb_d2*)(axis ? d.twx : d.twy);
-    GP* __restrict__ data = (GP*)d.grid[sel] + (size_t)jb.y * B * n;
+    const int l0 = jb.y * B;
+    GP* gin = (GP*)d.grid[sel] + (tin ? (size_t)l0 : (size_t)l0 * n);
+    GP* gout = (GP*)d.grid[osel] + (size_t)l0 * n;
+    const GStride gs{tin ? 1 : n, tin ? nlines : 1, n, 1};
     LP* __restrict__ buf = (LP*)lds;
-    const int nb = min(B, nlines - jb.y * B);
+    const int nb = min(B, nlines - l0);
     // The table holds exp(-2 pi i k/n) and every stage runs the forward
     // transform; the inverse is conj(FFT(conj(x))): the first stage conjugates
     // what it loads and the last what it stores.
@@ -415,11 +436,11 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
     for (int st = 0; st < nst; ++st) {
         const int R = ax.radix[st];
         const bool first = st == 0, last = st == nst - 1;
-#define SE2P_STAGE(RR)                                                                        \
-    if (first && last) stage_reg<RR, true, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n); \
-    else if (first) stage_reg<RR, true, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n);  \
-    else if (last) stage_reg<RR, false, true, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n);   \
-    else stage_reg<RR, false, false, T, PPT, S>(buf, data, n, nb, Ns, tw, inverse, n)
+#define SE2P_STAGE(RR)                                                                           \
+    if (first && last) stage_reg<RR, true, true, T, PPT, S>(buf, gin, gout, n, nb, Ns, tw, inverse, gs); \
+    else if (first) stage_reg<RR, true, false, T, PPT, S>(buf, gin, gout, n, nb, Ns, tw, inverse, gs);  \
+    else if (last) stage_reg<RR, false, true, T, PPT, S>(buf, gin, gout, n, nb, Ns, tw, inverse, gs);   \
+    else stage_reg<RR, false, false, T, PPT, S>(buf, gin, gout, n, nb, Ns, tw, inverse, gs)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
@@ -437,13 +458,17 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
 #undef SE2P_COMPOSITE
             default:
                 if (first && last)
-                    stage_generic<true, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
+                    stage_generic<true, true, T, PPT, S>(buf, gin, gout, n, nb, Ns, R, tw, inverse,
+                                                         gs);
                 else if (first)
-                    stage_generic<true, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
+                    stage_generic<true, false, T, PPT, S>(buf, gin, gout, n, nb, Ns, R, tw,
+                                                          inverse, gs);
                 else if (last)
-                    stage_generic<false, true, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
+                    stage_generic<false, true, T, PPT, S>(buf, gin, gout, n, nb, Ns, R, tw,
+                                                          inverse, gs);
                 else
-                    stage_generic<false, false, T, PPT, S>(buf, data, n, nb, Ns, R, tw, inverse, n);
+                    stage_generic<false, false, T, PPT, S>(buf, gin, gout, n, nb, Ns, R, tw,
+                                                           inverse, gs);
                 break;
         }
 #undef SE2P_STAGE
@@ -460,6 +485,7 @@ __device__ __forceinline__ void line_stages(typename Mem<S>::lds* __restrict__ b
                                             int n, int nb, const FftAxis& ax,
                                             const glb_d2* __restrict__ tw, int inverse)
 {
+    const GStride gs{gstride, 1, gstride, 1};
     const int nst = ax.nst;
     int Ns = 1;
     for (int st = 0; st < nst; ++st) {
@@ -467,13 +493,13 @@ __device__ __forceinline__ void line_stages(typename Mem<S>::lds* __restrict__ b
         const bool first = GIN && st == 0, last = GOUT && st == nst - 1;
 #define SE2P_STAGE(RR)                                                                          \
     if (first && last)                                                                          \
-        stage_reg<RR, true, true, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);          \
+        stage_reg<RR, true, true, T, PPT, S>(buf, g, g, n, nb, Ns, tw, inverse, gs);          \
     else if (first)                                                                             \
-        stage_reg<RR, true, false, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);         \
+        stage_reg<RR, true, false, T, PPT, S>(buf, g, g, n, nb, Ns, tw, inverse, gs);         \
     else if (last)                                                                              \
-        stage_reg<RR, false, true, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride);         \
+        stage_reg<RR, false, true, T, PPT, S>(buf, g, g, n, nb, Ns, tw, inverse, gs);         \
     else                                                                                        \
-        stage_reg<RR, false, false, T, PPT, S>(buf, g, n, nb, Ns, tw, inverse, gstride)
+        stage_reg<RR, false, false, T, PPT, S>(buf, g, g, n, nb, Ns, tw, inverse, gs)
         switch (R) {
             case 2: SE2P_STAGE(2); break;
             case 3: SE2P_STAGE(3); break;
@@ -482,16 +508,16 @@ __device__ __forceinline__ void line_stages(typename Mem<S>::lds* __restrict__ b
             case 8: SE2P_STAGE(8); break;
             default:
                 if (first && last)
-                    stage_generic<true, true, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse, gstride);
+                    stage_generic<true, true, T, PPT, S>(buf, g, g, n, nb, Ns, R, tw, inverse, gs);
                 else if (first)
-                    stage_generic<true, false, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
-                                                          gstride);
+                    stage_generic<true, false, T, PPT, S>(buf, g, g, n, nb, Ns, R, tw, inverse,
+                                                          gs);
                 else if (last)
-                    stage_generic<false, true, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
-                                                          gstride);
+                    stage_generic<false, true, T, PPT, S>(buf, g, g, n, nb, Ns, R, tw, inverse,
+                                                          gs);
                 else
-                    stage_generic<false, false, T, PPT, S>(buf, g, n, nb, Ns, R, tw, inverse,
-                                                           gstride);
+                    stage_generic<false, false, T, PPT, S>(buf, g, g, n, nb, Ns, R, tw, inverse,
+                                                           gs);
                 break;
         }
 #undef SE2P_STAGE
@@ -711,31 +737,33 @@ int fft_prepare()
 int fft_line_class(int n) { return n <= 2048 ? 0 : n <= 2560 ? 1 : n <= 4096 ? 2 : 3; }
 
 template <typename S>
-static void fft_lines_launch(hipStream_t s, int axis, int sel, int inverse, int cls,
-                             size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs)
+static void fft_lines_launch(hipStream_t s, int axis, int sel, int osel, int tin, int inverse,
+                             int cls, size_t max_lds, const SysDesc* sys, const Job* jobs,
+                             int njobs)
 {
     if (cls == 0)
         hipLaunchKernelGGL((k_fft_lines<256, 8, false, S>), dim3(njobs), dim3(256), max_lds, s, sys,
-                           jobs, axis, sel, inverse);
+                           jobs, axis, sel, osel, tin, inverse);
     else if (cls == 1)
         hipLaunchKernelGGL((k_fft_lines<256, 10, true, S>), dim3(njobs), dim3(256), max_lds, s, sys,
-                           jobs, axis, sel, inverse);
+                           jobs, axis, sel, osel, tin, inverse);
     else if (cls == 2)
         hipLaunchKernelGGL((k_fft_lines<512, 8, false, S>), dim3(njobs), dim3(512), max_lds, s, sys,
-                           jobs, axis, sel, inverse);
+                           jobs, axis, sel, osel, tin, inverse);
     else
         hipLaunchKernelGGL((k_fft_lines<1024, 10, false, S>), dim3(njobs), dim3(1024), max_lds, s,
-                           sys, jobs, axis, sel, inverse);
+                           sys, jobs, axis, sel, osel, tin, inverse);
 }
 
-int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls,
-                     size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs)
+int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int osel, int tin, int inverse,
+                     int cls, size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs)
 {
     if (njobs <= 0) return SE2P_OK;
     if (prec)
-        fft_lines_launch<float>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
+        fft_lines_launch<float>(s, axis, sel, osel, tin, inverse, cls, max_lds, sys, jobs, njobs);
     else
-        fft_lines_launch<double>(s, axis, sel, inverse, cls, max_lds, sys, jobs, njobs);
+        fft_lines_launch<double>(s, axis, sel, osel, tin, inverse, cls, max_lds, sys, jobs,
+                                 njobs);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -482,6 +482,15 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
     auto blocks = [&](int i, long n) {
         for (long b = 0; b < n; ++b) J.push_back(make_int2(i, (int)b));
     };
+    // Jobs 0 .. n-1 in an order where the workgroups that run on one XCD (every
+    // 8th of a launch) take consecutive jobs: job k of group g is g*(n/8) + k (the
+    // last n mod 8 jobs keep their place).  FFT lines read as the columns of a
+    // row-major grid then share each cache line within one L2.
+    auto xcd_blocks = [&](int i, long n) {
+        const long q = n / 8;
+        for (long b = 0; b < n; ++b)
+            J.push_back(make_int2(i, (int)(b < 8 * q ? (b % 8) * q + b / 8 : b)));
+    };
     // Real space: one job per block of bs x bs cells and 256 of its targets (job y
     // = block + part * nblk), so a dense block runs on as many workgroups as it
     // needs 256-lane passes instead of making one workgroup loop over them.
@@ -582,12 +591,9 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 });
                 add(B->iffty[prec][g][cls], [&](int i, const SysDesc& d) {
                     if (want(i, d) && fft_line_class(d.My) == cls)
-                        blocks(i, (d.Mx + d.By - 1) / d.By);
+                        xcd_blocks(i, (d.Mx + d.By - 1) / d.By);
                 });
             }
-            add(B->tr_inv[prec][g], [&](int i, const SysDesc& d) {
-                if (want(i, d)) blocks(i, transpose_tiles(d.My, d.Mx));
-            });
             if (g == 0)
                 add(B->xfilt[prec], [&](int i, const SysDesc& d) {
                     if (want(i, d) && fft_xfilter_fits(d.kind, d.Mx, d.free_R)) {
@@ -673,18 +679,21 @@ int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
 int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f)
 {
     const Job* J = B.jobs;
-    // Every FFT pass runs on contiguous lines: y lines in the spread layout
-    // (Z[ix*My + iy]), then a transpose to Z[iy*Mx + ix] for the x lines, the
-    // filter in that layout, and back.
-    //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
+    // Forward: y lines in place in the spread layout (Z[ix*My + iy]), a tiled
+    // transpose to W[iy*Mx + ix], x lines and the filter in that layout.
+    // Inverse: x lines in place, then y lines read as the columns of that grid
+    // and written in the layout the gather reads (the transpose back rides on
+    // the first stage's loads).
+    //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
     //        (x lines of 512-2048 points: xFFT, filter and xIFFT as one launch)
-    //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -T-> W -yIFFT-> gather(W)
+    //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     // Systems of the FP32 tier run the same passes on float2 grids.
-    auto fft = [&](int prec, int axis, int sel, int inv, const JobRange* r, const size_t* lds) {
+    auto fft = [&](int prec, int axis, int sel, int osel, int tin, int inv, const JobRange* r,
+                   const size_t* lds) {
         for (int cls = 0; cls < kFftClasses; ++cls)
-            SE2P_TRY(launch_fft_lines(s, prec, axis, sel, inv, cls, lds[cls], B.sys, J + r[cls].first,
-                                      r[cls].count));
+            SE2P_TRY(launch_fft_lines(s, prec, axis, sel, osel, tin, inv, cls, lds[cls], B.sys,
+                                      J + r[cls].first, r[cls].count));
         return SE2P_OK;
     };
     for (int c = 0; c < (B.any_dlp ? 2 : 1); ++c) {
@@ -692,10 +701,10 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
         StageTimer st(ctx, s, "fft");
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
         for (int prec = 0; prec < 2; ++prec) {
-            SE2P_TRY(fft(prec, 0, Zc, 0, B.ffty[prec][c], B.lds_y[prec]));
+            SE2P_TRY(fft(prec, 0, Zc, Zc, 0, 0, B.ffty[prec][c], B.lds_y[prec]));
             SE2P_TRY(launch_transpose(s, prec, Zc, Tc, 0, B.sys, J + B.tr_fwd[prec][c].first,
                                       B.tr_fwd[prec][c].count));
-            SE2P_TRY(fft(prec, 1, Tc, 0, B.fftx[prec][c], B.lds_x[prec]));
+            SE2P_TRY(fft(prec, 1, Tc, Tc, 0, 0, B.fftx[prec][c], B.lds_x[prec]));
         }
     }
     {
@@ -715,10 +724,8 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
         StageTimer st(ctx, s, "fft");
         const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
         for (int prec = 0; prec < 2; ++prec) {
-            SE2P_TRY(fft(prec, 1, Zg, 1, B.ifftx[prec][g], B.lds_x[prec]));
-            SE2P_TRY(launch_transpose(s, prec, Zg, Tg, 1, B.sys, J + B.tr_inv[prec][g].first,
-                                      B.tr_inv[prec][g].count));
-            SE2P_TRY(fft(prec, 0, Tg, 1, B.iffty[prec][g], B.lds_y[prec]));
+            SE2P_TRY(fft(prec, 1, Zg, Zg, 0, 1, B.ifftx[prec][g], B.lds_x[prec]));
+            SE2P_TRY(fft(prec, 0, Zg, Tg, 2, 1, B.iffty[prec][g], B.lds_y[prec]));
         }
     }
     return SE2P_OK;

@@ -430,25 +430,28 @@ int launch_free(hipStream_t s, bool dlp, const double* src, int Ns, const double
 // Density fill of the sorted source records (layout: rec_layout).
 int launch_fill_dens(hipStream_t s, int layout, const SysDesc* sys, const Job* jobs, int njobs,
                      const double* f);
-// Line FFTs of grid `sel` along y (axis 0: My-lines of the column-major grid) or
-// x (axis 1: Mx-lines of the transposed grid); every job's line length must be of
-// class `cls` (fft_line_class); max_lds from fft_lds_bytes.
+// Line FFTs along y (axis 0: My-lines, Mx of them) or x (axis 1: Mx-lines, My
+// of them), from grid `sel` to grid `osel` (may be `sel`), which holds them
+// contiguously (line l at l*n).  tin 0: so does the input; 1: the input holds
+// them as the columns of a row-major grid (point k of line l at k*nlines + l).
+// Every job's line length must be of class `cls` (fft_line_class); max_lds from
+// fft_lds_bytes.
 constexpr int kFftClasses = 4;
 int fft_line_class(int n);
-int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int inverse, int cls, size_t max_lds,
-                     const SysDesc* sys, const Job* jobs, int njobs);
+int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int osel, int tin, int inverse,
+                     int cls, size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs);
 size_t fft_lds_bytes(int n, int prec);
+// Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
+// and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
+int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
+                     const Job* jobs, int njobs);
+int transpose_tiles(int rows, int cols);
 // The Stokeslet's forward x passes, filter and inverse x passes as one launch
 // (W rows p and My - p -> Z0 rows p and My - p; job (s, p), p = 0 .. My/2), for
 // periodic systems where fft_xfilter_fits(kind, Mx, free_R); LDS 2 Mx complex values.
 bool fft_xfilter_fits(int kind, int Mx, double free_R);
 int launch_fft_xfilter(hipStream_t s, int prec, size_t max_lds, const SysDesc* sys,
                        const Job* jobs, int njobs);
-// Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
-// and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
-int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
-                     const Job* jobs, int njobs);
-int transpose_tiles(int rows, int cols);
 
 // ---------------------------------------------------------------------------
 // Parameter selector network (mlp.hip): packed FP32 parameters, device arrays
@@ -524,7 +527,7 @@ struct BatchExec {
     JobRange ffty[2][2][kFftClasses], fftx[2][2][kFftClasses], tr_fwd[2][2];  // [prec][c][class]
     JobRange filter[2][kNumKinds];  // [prec][kind]
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
-    JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses], tr_inv[2][2];  // [prec][g][class]
+    JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses];  // [prec][g][class]
     JobRange gather[2][2][kNumP];  // [prec][g][P / 2 - 1]
     JobRange xfilt[2];              // [prec]: fused x passes and filter (Stokeslet)
     size_t lds_xf[2] = {};
