@@ -1841,23 +1841,25 @@ int se2p_gather(se2p_ctx* ctx, const double* H, const double* ptar, int Ntar, do
     if (Ntar == 0) return SE2P_OK;
     double* dpt;
     SE2P_TRY(upload(ctx, WS_PTS_B, ptar, 2 * (size_t)Ntar, &dpt));
-    // k-space geometry of the targets only (no sources); the grid goes into W
+    // k-space geometry of the targets only (no sources); the grid goes where the
+    // gather reads it
     Geom G;
     se2p_params kp = k_only(xi, eta, Mx, My, w, P);
     kp.precision = ctx->precision;
     SE2P_TRY(geom_build(ctx, &ctx->ws, &G, SE2P_SLP, dpt, dpt, nullptr, 0, Ntar, Lx, Ly, kp, false,
                         true));
     const size_t Gn = (size_t)Mx * My;
+    double2* const gdst = fft_xfilter_fits(SE2P_SLP, Mx, kp.free_R) ? G.Z0 : G.W;
     std::vector<double> wz(2 * Gn, 0.0);
     std::vector<float2> wf;
     if (G.prec) {  // FP32 tier: the grid travels as float2
         wf.assign(Gn, make_float2(0.f, 0.f));
         for (size_t i = 0; i < Gn; ++i) wf[i].x = (float)H[i];
-        SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wf.data(), sizeof(float2) * Gn, hipMemcpyHostToDevice,
-                                      ctx->stream));
+        SE2P_HIP_CHECK(hipMemcpyAsync(gdst, wf.data(), sizeof(float2) * Gn,
+                                      hipMemcpyHostToDevice, ctx->stream));
     } else {
         for (size_t i = 0; i < Gn; ++i) wz[2 * i] = H[i];
-        SE2P_HIP_CHECK(hipMemcpyAsync(G.W, wz.data(), sizeof(double) * 2 * Gn,
+        SE2P_HIP_CHECK(hipMemcpyAsync(gdst, wz.data(), sizeof(double) * 2 * Gn,
                                       hipMemcpyHostToDevice, ctx->stream));
     }
     BatchExec B;

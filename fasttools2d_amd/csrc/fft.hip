@@ -477,15 +477,14 @@ __global__ __launch_bounds__(T) void k_fft_lines(const SysDesc* __restrict__ sys
 }
 
 // The stages of one transform over nb lines in `buf`.  GIN: the first stage
-// reads the lines from HBM (g, line b at b gstride), else from `buf`; GOUT: the
-// last stage writes them to HBM (g), else to `buf`.
+// reads the lines from HBM (g, laid out as gs.li, gs.ei say), else from `buf`;
+// GOUT: the last stage writes them to HBM (g, as gs.lo, gs.eo say), else to `buf`.
 template <int T, int PPT, bool GIN, bool GOUT, typename S>
 __device__ __forceinline__ void line_stages(typename Mem<S>::lds* __restrict__ buf,
-                                            typename Mem<S>::glb* __restrict__ g, int gstride,
-                                            int n, int nb, const FftAxis& ax,
-                                            const glb_d2* __restrict__ tw, int inverse)
+                                            typename Mem<S>::glb* g, GStride gs, int n, int nb,
+                                            const FftAxis& ax, const glb_d2* __restrict__ tw,
+                                            int inverse)
 {
-    const GStride gs{gstride, 1, gstride, 1};
     const int nst = ax.nst;
     int Ns = 1;
     for (int st = 0; st < nst; ++st) {
@@ -553,10 +552,11 @@ __device__ __attribute__((noinline)) void filter_rows(typename Mem<S>::lds* __re
 
 // The Stokeslet's x-direction passes with the filter between them, for x lines
 // of 512 to 2048 points: job (s, p) loads the x lines p and My - p of the
-// transposed forward spectrum W (the rows of modes k and -k), transforms them
-// in LDS, filters every mode with its partner (unpack, Stokes filter, k = 0
-// removal, 1/(Mx My), repack: the same filter_mode as k_filter), transforms
-// them back and writes the rows of Z0.  Rows 0 and My/2 are their own partners
+// y-transformed spectrum in the column-major Z0 (the rows of modes k and -k,
+// read across the columns, which also transposes), transforms them in LDS,
+// filters every mode with its partner (unpack, Stokes filter, k = 0 removal,
+// 1/(Mx My), repack: the same filter_mode as k_filter), transforms them back and
+// writes the rows of the row-major W.  Rows 0 and My/2 are their own partners
 // and share one job.
 // It runs the same stages and filter arithmetic as the separate passes on values
 // that stay in LDS instead of HBM; the results agree with theirs to rounding
@@ -580,11 +580,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     // partner (My/2 jobs, a whole number of rounds of two per CU)
     const int p = jb.y, q = p ? My - p : My / 2;
     const int nb = 2;
-    const int gs = (q - p) * n;  // line 1 starts gs elements after line 0
     const glb_d2* __restrict__ tw = (const glb_d2*)d.twx;
     LP* __restrict__ buf = (LP*)lds;
-    line_stages<T, PPT, true, false, S>(buf, (GP*)d.grid[G_W] + (size_t)p * n, gs, n, nb, d.ax,
-                                        tw, 0);
+    // rows p and q of the y-transformed column-major grid Z0 (point ix of row p
+    // at ix*My + p); the jobs of one XCD take consecutive p, so their 16-byte
+    // loads share cache lines in its L2
+    line_stages<T, PPT, true, false, S>(buf, (GP*)d.grid[G_Z0] + p, GStride{q - p, My, 0, 0}, n,
+                                        nb, d.ax, tw, 0);
     // the stages end with a barrier: every forward value is in buf
     if (p) {
         filter_rows<T, S>(buf, d, n, 2, p, q);
@@ -593,8 +595,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         filter_rows<T, S>(buf + n, d, n, 1, q, q);
     }
     __syncthreads();
-    line_stages<T, PPT, false, true, S>(buf, (GP*)d.grid[G_Z0] + (size_t)p * n, gs, n, nb, d.ax,
-                                        tw, 1);
+    // rows p and q of the row-major W
+    line_stages<T, PPT, false, true, S>(buf, (GP*)d.grid[G_W] + (size_t)p * n,
+                                        GStride{0, 0, (q - p) * n, 1}, n, nb, d.ax, tw, 1);
 }
 
 // out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows

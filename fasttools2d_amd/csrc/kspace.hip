@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void k_gather(const SysDesc* __restrict__ sys,
     const glob<const int>* __restrict__ perm = gp(d.gt_perm);
     const glob<const int2>* __restrict__ node = gp(d.gt_node);    // binned order
     const glob<const double2>* __restrict__ off = gp(d.gt_off);   // binned order
-    const double2* __restrict__ W = d.grid[g ? G_D : G_W];
+    const double2* __restrict__ W = d.grid[g ? G_D : d.out0];
     const int Mx = d.Mx, My = d.My, TX = d.gTX, TY = d.gTY, ntx = d.gntx, R = d.gR;
     const double t = d.t, w = d.w, h = d.h, scale = d.gscale;
     const glob<const double>* __restrict__ e1tab = gp(d.e1tab);

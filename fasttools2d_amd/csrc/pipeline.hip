@@ -443,6 +443,7 @@ void sysdesc_from_geom(const Geom& G, long f_off, long u_off, SysDesc* d)
         d->grid[G_Z1] = G.Z1;
         d->grid[G_W] = G.W;
         d->grid[G_D] = G.D;
+        d->out0 = fft_xfilter_fits(G.kind, G.kg.Mx, d->free_R) ? G_Z0 : G_W;
         d->twx = G.twx;
         d->twy = G.twy;
         d->ax = G.ax;
@@ -576,7 +577,8 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 });
             }
             add(B->tr_fwd[prec][c], [&](int i, const SysDesc& d) {
-                if (want(i, d)) blocks(i, transpose_tiles(d.Mx, d.My));
+                if (want(i, d) && !(c == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
+                    blocks(i, transpose_tiles(d.Mx, d.My));
             });
         }
         for (int g = 0; g < 2; ++g) {
@@ -590,14 +592,21 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                         blocks(i, (d.My + d.Bx - 1) / d.Bx);
                 });
                 add(B->iffty[prec][g][cls], [&](int i, const SysDesc& d) {
-                    if (want(i, d) && fft_line_class(d.My) == cls)
+                    if (want(i, d) && fft_line_class(d.My) == cls &&
+                        !(g == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
                         xcd_blocks(i, (d.Mx + d.By - 1) / d.By);
                 });
+                if (g == 0)
+                    add(B->iffty_xf[prec][cls], [&](int i, const SysDesc& d) {
+                        if (want(i, d) && fft_line_class(d.My) == cls &&
+                            fft_xfilter_fits(d.kind, d.Mx, d.free_R))
+                            xcd_blocks(i, (d.Mx + d.By - 1) / d.By);
+                    });
             }
             if (g == 0)
                 add(B->xfilt[prec], [&](int i, const SysDesc& d) {
                     if (want(i, d) && fft_xfilter_fits(d.kind, d.Mx, d.free_R)) {
-                        blocks(i, d.My / 2);  // row pairs; rows 0 and My/2 share job 0
+                        xcd_blocks(i, d.My / 2);  // row pairs; rows 0 and My/2 share job 0
                         B->lds_xf[prec] = std::max(B->lds_xf[prec],
                                                    (prec ? 8 : 16) * 2 * (size_t)d.Mx);
                     }
@@ -685,8 +694,10 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
     // and written in the layout the gather reads (the transpose back rides on
     // the first stage's loads).
     //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
-    //        (x lines of 512-2048 points: xFFT, filter and xIFFT as one launch)
     //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
+    // The Stokeslet with x lines of 512-2048 points runs xFFT, filter and xIFFT
+    // as one launch that reads its rows across the columns of Z0 (no transpose):
+    //        Z0 -yFFT-> -(T)xFFT-filter-xIFFT-> W -yIFFT(T)-> Z0 -> gather(Z0)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     // Systems of the FP32 tier run the same passes on float2 grids.
     auto fft = [&](int prec, int axis, int sel, int osel, int tin, int inv, const JobRange* r,
@@ -725,7 +736,8 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
         const int Zg = g ? G_Z1 : G_Z0, Tg = g ? G_D : G_W;
         for (int prec = 0; prec < 2; ++prec) {
             SE2P_TRY(fft(prec, 1, Zg, Zg, 0, 1, B.ifftx[prec][g], B.lds_x[prec]));
-            SE2P_TRY(fft(prec, 0, Zg, Tg, 2, 1, B.iffty[prec][g], B.lds_y[prec]));
+            SE2P_TRY(fft(prec, 0, Zg, Tg, 1, 1, B.iffty[prec][g], B.lds_y[prec]));
+            if (g == 0) SE2P_TRY(fft(prec, 0, G_W, G_Z0, 1, 1, B.iffty_xf[prec], B.lds_y[prec]));
         }
     }
     return SE2P_OK;

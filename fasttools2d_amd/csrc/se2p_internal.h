@@ -345,6 +345,8 @@ struct SysDesc {
     const int* gt_sidx;     // pair-symmetric sums: sorted position of each binned target
     const double *e1tab, *nrm;
     double2* grid[4];  // GridSel
+    int out0;  // the grid holding the first output for the gather: G_W, or G_Z0 for
+               // systems on the fused x pass (fft_xfilter_fits)
     const double2 *twx, *twy;
     FftAxis ax, ay;
 };
@@ -447,7 +449,8 @@ int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const Sy
                      const Job* jobs, int njobs);
 int transpose_tiles(int rows, int cols);
 // The Stokeslet's forward x passes, filter and inverse x passes as one launch
-// (W rows p and My - p -> Z0 rows p and My - p; job (s, p), p = 0 .. My/2), for
+// (column-major Z0 rows p and My - p -> row-major W rows p and My - p; job (s, p),
+// p = 0 .. My/2 - 1, job 0 taking rows 0 and My/2), for
 // periodic systems where fft_xfilter_fits(kind, Mx, free_R); LDS 2 Mx complex values.
 bool fft_xfilter_fits(int kind, int Mx, double free_R);
 int launch_fft_xfilter(hipStream_t s, int prec, size_t max_lds, const SysDesc* sys,
@@ -528,6 +531,7 @@ struct BatchExec {
     JobRange filter[2][kNumKinds];  // [prec][kind]
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
     JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses];  // [prec][g][class]
+    JobRange iffty_xf[2][kFftClasses];  // [prec][class]: systems on the fused x pass
     JobRange gather[2][2][kNumP];  // [prec][g][P / 2 - 1]
     JobRange xfilt[2];              // [prec]: fused x passes and filter (Stokeslet)
     size_t lds_xf[2] = {};
