@@ -11,11 +11,12 @@
 // by a direct O(R) sum per output).  Stage s with radix R and Ns = product of the earlier
 // radices maps output (j, k), j < n/R, k < R, to
 //   out[(j / Ns) Ns R + j % Ns + k Ns] = sum_r in[j + r n/R] W_n^{r ((j % Ns) + k Ns) n/(Ns R)}.
-// A pass writes its lines contiguously and reads them contiguously or as the
-// columns of a row-major grid.  Forward: y lines of the column-major grid
-// (Z[ix*My + iy]), a tiled transpose to Z[iy*Mx + ix], x lines.  Inverse: x
-// lines, then y lines read as the columns of that grid, which transposes back
-// in the first stage's loads.
+// A pass writes its lines contiguously and reads them contiguously or across
+// the columns of the other layout, which transposes the grid in the first
+// stage's loads.  Forward: y lines of the column-major grid (Z[ix*My + iy]) in
+// place, then x lines read across its columns into the row-major Z[iy*Mx + ix].
+// Inverse: x lines in place, then y lines read across the row-major grid's
+// columns into the column-major one.
 #include "se2p_internal.h"
 #include "fft_roots.h"
 #include "kfilter.h"
@@ -600,34 +601,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                                         GStride{0, 0, (q - p) * n, 1}, n, nb, d.ax, tw, 1);
 }
 
-// out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS; dir 0: rows
-// = Mx, cols = My (column-major grid to transposed), dir 1 the reverse.
-template <typename S>
-__global__ __launch_bounds__(256) void k_transpose(const SysDesc* __restrict__ sys,
-                                                   const Job* __restrict__ jobs, int in_sel,
-                                                   int out_sel, int dir)
-{
-    using GP = typename Mem<S>::glb;
-    __shared__ cplx<S> tile[32][33];
-    const Job jb = jobs[blockIdx.x];
-    const SysDesc& d = sys[jb.x];
-    const int rows = dir ? d.My : d.Mx, cols = dir ? d.Mx : d.My;
-    const GP* __restrict__ in = (const GP*)d.grid[in_sel];
-    GP* __restrict__ out = (GP*)d.grid[out_sel];
-    const int ntc = (cols + 31) / 32;
-    const int r0 = (jb.y / ntc) * 32, c0 = (jb.y % ntc) * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int k = ty; k < 32; k += 8) {
-        const int r = r0 + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[k][tx] = ld2<S>(in, r * cols + c);
-    }
-    __syncthreads();
-    for (int k = ty; k < 32; k += 8) {
-        const int c = c0 + k, r = r0 + tx;
-        if (r < rows && c < cols) st2(out, c * rows + r, tile[tx][k]);
-    }
-}
-
 // Lines above kFftMaxLenSmall need more than 64 KiB of LDS per workgroup; set
 // when the device accepts that for the class-3 kernels.
 bool g_long_lines = false;
@@ -784,22 +757,6 @@ int launch_fft_xfilter(hipStream_t s, int prec, size_t max_lds, const SysDesc* s
         hipLaunchKernelGGL(k_fft_xfilter<float>, dim3(njobs), dim3(512), max_lds, s, sys, jobs);
     else
         hipLaunchKernelGGL(k_fft_xfilter<double>, dim3(njobs), dim3(512), max_lds, s, sys, jobs);
-    SE2P_HIP_CHECK(hipGetLastError());
-    return SE2P_OK;
-}
-
-int transpose_tiles(int rows, int cols) { return ((rows + 31) / 32) * ((cols + 31) / 32); }
-
-int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
-                     const Job* jobs, int njobs)
-{
-    if (njobs <= 0) return SE2P_OK;
-    if (prec)
-        hipLaunchKernelGGL(k_transpose<float>, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out,
-                           dir);
-    else
-        hipLaunchKernelGGL(k_transpose<double>, dim3(njobs), dim3(256), 0, s, sys, jobs, in, out,
-                           dir);
     SE2P_HIP_CHECK(hipGetLastError());
     return SE2P_OK;
 }

@@ -573,13 +573,9 @@ int batch_build(se2p_ctx* ctx, Workspace* ws, BatchExec* B, const Geom* const* g
                 add(B->fftx[prec][c][cls], [&](int i, const SysDesc& d) {
                     if (want(i, d) && fft_line_class(d.Mx) == cls &&
                         !(c == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
-                        blocks(i, (d.My + d.Bx - 1) / d.Bx);
+                        xcd_blocks(i, (d.My + d.Bx - 1) / d.Bx);
                 });
             }
-            add(B->tr_fwd[prec][c], [&](int i, const SysDesc& d) {
-                if (want(i, d) && !(c == 0 && fft_xfilter_fits(d.kind, d.Mx, d.free_R)))
-                    blocks(i, transpose_tiles(d.Mx, d.My));
-            });
         }
         for (int g = 0; g < 2; ++g) {
             auto want = [&](int i, const SysDesc& d) {
@@ -688,16 +684,17 @@ int batch_gather(se2p_ctx* ctx, const BatchExec& B, double* d_u)
 int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double* d_f)
 {
     const Job* J = B.jobs;
-    // Forward: y lines in place in the spread layout (Z[ix*My + iy]), a tiled
-    // transpose to W[iy*Mx + ix], x lines and the filter in that layout.
-    // Inverse: x lines in place, then y lines read as the columns of that grid
-    // and written in the layout the gather reads (the transpose back rides on
-    // the first stage's loads).
-    //   SLP: Z0 -yFFT-> -T-> W -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
-    //   DLP: Z0, Z1 -yFFT-> -T-> W, D -xFFT-> -filter-> Z0 -xIFFT-> -yIFFT(T)-> W -> gather(W)
+    // Forward: y lines in place in the spread layout (Z[ix*My + iy]), then x
+    // lines read across its columns and written as the rows of W[iy*Mx + ix],
+    // where the filter runs.  Inverse: x lines in place, then y lines read across
+    // the columns of that grid and written in the layout the gather reads.  The
+    // transposes ride on the first stages' loads, 16 bytes at a stride of My or
+    // Mx, with the lines of one XCD's workgroups sharing cache lines in its L2.
+    //   SLP: Z0 -yFFT-> -(T)xFFT-> W -filter-> Z0 -xIFFT-> -(T)yIFFT-> W -> gather(W)
+    //   DLP: Z0, Z1 -yFFT-> -(T)xFFT-> W, D -filter-> Z0 -xIFFT-> -(T)yIFFT-> W -> gather(W)
     // The Stokeslet with x lines of 512-2048 points runs xFFT, filter and xIFFT
-    // as one launch that reads its rows across the columns of Z0 (no transpose):
-    //        Z0 -yFFT-> -(T)xFFT-filter-xIFFT-> W -yIFFT(T)-> Z0 -> gather(Z0)
+    // as one launch:
+    //        Z0 -yFFT-> -(T)xFFT-filter-xIFFT-> W -(T)yIFFT-> Z0 -> gather(Z0)
     // Kinds with 4 values per target also filter into Z1 and return it through D.
     // Systems of the FP32 tier run the same passes on float2 grids.
     auto fft = [&](int prec, int axis, int sel, int osel, int tin, int inv, const JobRange* r,
@@ -713,9 +710,7 @@ int batch_k_grids(se2p_ctx* ctx, hipStream_t s, const BatchExec& B, const double
         const int Zc = c ? G_Z1 : G_Z0, Tc = c ? G_D : G_W;
         for (int prec = 0; prec < 2; ++prec) {
             SE2P_TRY(fft(prec, 0, Zc, Zc, 0, 0, B.ffty[prec][c], B.lds_y[prec]));
-            SE2P_TRY(launch_transpose(s, prec, Zc, Tc, 0, B.sys, J + B.tr_fwd[prec][c].first,
-                                      B.tr_fwd[prec][c].count));
-            SE2P_TRY(fft(prec, 1, Tc, Tc, 0, 0, B.fftx[prec][c], B.lds_x[prec]));
+            SE2P_TRY(fft(prec, 1, Zc, Tc, 1, 0, B.fftx[prec][c], B.lds_x[prec]));
         }
     }
     {

@@ -443,11 +443,6 @@ int fft_line_class(int n);
 int launch_fft_lines(hipStream_t s, int prec, int axis, int sel, int osel, int tin, int inverse,
                      int cls, size_t max_lds, const SysDesc* sys, const Job* jobs, int njobs);
 size_t fft_lds_bytes(int n, int prec);
-// Tiled transpose of grid `in` into `out`: dir 0 reads Mx x My (Z[ix*My + iy])
-// and writes Z[iy*Mx + ix]; dir 1 the reverse.  Job b = tile index.
-int launch_transpose(hipStream_t s, int prec, int in, int out, int dir, const SysDesc* sys,
-                     const Job* jobs, int njobs);
-int transpose_tiles(int rows, int cols);
 // The Stokeslet's forward x passes, filter and inverse x passes as one launch
 // (column-major Z0 rows p and My - p -> row-major W rows p and My - p; job (s, p),
 // p = 0 .. My/2 - 1, job 0 taking rows 0 and My/2), for
@@ -527,7 +522,7 @@ struct BatchExec {
     JobRange real[kNumKinds][3][kRealModes];  // [kind][tier: full, fast, FP32][RealMode]
     // k-space passes, per grid precision (0 FP64, 1 FP32: float2 grids)
     JobRange spread[2][2][kNumP];  // [prec][pairs - 1][P / 2 - 1]
-    JobRange ffty[2][2][kFftClasses], fftx[2][2][kFftClasses], tr_fwd[2][2];  // [prec][c][class]
+    JobRange ffty[2][2][kFftClasses], fftx[2][2][kFftClasses];  // [prec][c][class]
     JobRange filter[2][kNumKinds];  // [prec][kind]
     // inverse passes of output grid g (g = 1 for kinds with 4 values per target)
     JobRange ifftx[2][2][kFftClasses], iffty[2][2][kFftClasses];  // [prec][g][class]
