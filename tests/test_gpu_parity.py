@@ -237,6 +237,9 @@ KS_CASES = [
     (2000, 1500, 120, 60, 2.0, 1.0, 20.0, 16),
     (300, 200, 30, 30, 1.0, 1.0, 6.0, 24),
     (5000, 4000, 256, 256, 1.0, 1.0, 40.0, 12),
+    # x lines of 512-2048 points: the fused x passes and filter, reading their rows
+    # across the columns (My/2 = 130 row pairs: two jobs past the XCD groups of 16)
+    (3000, 2000, 520, 260, 2.0, 1.0, 60.0, 12),
 ]
 
 
