@@ -156,7 +156,9 @@ def test_spread_ring_edges_match_oracle(gpu_ctx, oracle, N, Mx, My, P, dist):
 
 
 @pytest.mark.parametrize("N,Mx,My,P,Lx,Ly", [(700, 64, 64, 24, 1.0, 1.0), (900, 92, 46, 16, 2.0, 1.0),
-                                              (300, 26, 26, 24, 1.0, 1.0)])
+                                              (300, 26, 26, 24, 1.0, 1.0),
+                                              # a grid of the fused x passes (read from Z0)
+                                              (800, 520, 260, 12, 2.0, 1.0)])
 def test_gather_matches_oracle(gpu_ctx, oracle, N, Mx, My, P, Lx, Ly):
     xi = 10.0
     h = Lx / Mx
